@@ -1,0 +1,215 @@
+"""Readers and writers for DGM's on-disk input and restart formats.
+
+These mirror the reference's file handling so that a DGM run directory
+(``root.inp``, ``root.msh``, ``root.con``, ``root.ic``) can drive this engine
+and its restart files can be compared with ``dgm_diff`` semantics:
+
+* ``root.inp``  ``key = value  # comment`` parameter table
+  (src/Domain.cpp:236-277, src/Table.hpp)
+* ``root.msh``  ``**MESH DATA**`` / ``ne nsd`` / per element ``ELEMENT id TYPE``
+  followed by one line of vertex coordinates per space dimension
+  (src/Field.cpp:2600-2700, src/Mesh.cpp)
+* ``root.con``  ``**CONNECTIVITY DATA**`` / ``ne`` / rows ``E eid sid leid lsid``
+  (src/Field.cpp:3478-3556)
+* ``root.ic``   initial-condition code and parameters (src/Euler.cpp:880-1190)
+* restart files: old 10-line text header or a JSON header, then per field
+  ``glen, nModes[ne], disp[ne], modal FP64 coefficients``
+  (src/Domain.cpp:1473-1639, src/vField.cpp:362-445, 1082-1201)
+"""
+from __future__ import annotations
+
+import json
+import re
+import struct
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+_NVERT = {"QUAD": 4, "NODALQUAD": 4, "HEX": 8, "NODALHEX": 8}
+
+
+def read_inp(path: str) -> Dict[str, object]:
+    """Parse a ``root.inp`` parameter file into a dict of numbers/strings."""
+    params: Dict[str, object] = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if "=" not in line:
+                continue
+            k, v = (s.strip() for s in line.split("=", 1))
+            try:
+                params[k] = int(v)
+            except ValueError:
+                try:
+                    params[k] = float(v)
+                except ValueError:
+                    params[k] = v
+    return params
+
+
+@dataclass
+class Mesh:
+    """Element vertices ``vert[e, dim, v]`` and ``.con`` rows ``(eid, sid, leid, lsid)``."""
+    nsd: int
+    vert: np.ndarray
+    con: np.ndarray
+    porder: Optional[np.ndarray] = None   # per-element order from root.ord
+
+    @property
+    def ne(self) -> int:
+        return int(self.vert.shape[0])
+
+
+def read_msh(path: str) -> np.ndarray:
+    with open(path) as f:
+        f.readline()
+        hdr = f.readline().split()
+        ne, nsd = int(hdr[0]), int(hdr[1])
+        verts = []
+        for _ in range(ne):
+            toks = f.readline().split()
+            etype = toks[2].upper()
+            nv = _NVERT.get(etype)
+            if nv is None:
+                raise ValueError(f"{path}: unsupported element type {etype}")
+            rows = [[float(x) for x in f.readline().split()] for _ in range(nsd)]
+            verts.append(rows)
+    return np.asarray(verts, dtype=np.float64)
+
+
+def read_con(path: str) -> np.ndarray:
+    rows: List[List[int]] = []
+    with open(path) as f:
+        f.readline()
+        f.readline()
+        for line in f:
+            toks = line.split()
+            if len(toks) >= 5 and toks[0] == "E":
+                rows.append([int(t) for t in toks[1:5]])
+    return np.asarray(rows, dtype=np.int32).reshape(-1, 4)
+
+
+def read_mesh(root: str) -> Mesh:
+    vert = read_msh(root + ".msh")
+    con = read_con(root + ".con")
+    return Mesh(nsd=vert.shape[1], vert=vert, con=con)
+
+
+@dataclass
+class InitialCondition:
+    code: str
+    values: List[float] = field(default_factory=list)
+
+
+def read_ic(path: str) -> InitialCondition:
+    """Read the first IC block of ``root.ic`` (the reference reads only that one)."""
+    with open(path) as f:
+        f.readline()
+        code = f.readline().split()[0]
+        lc = code.lower()
+        if lc == "vortex":
+            nv = int(f.readline().split()[0])
+            vals: List[float] = [float(nv)]
+            for _ in range(nv):
+                vals += [float(x) for x in f.readline().split()[:5]]
+            return InitialCondition(code, vals)
+        if lc == "acousticpulse":
+            return InitialCondition(code, [float(x) for x in f.readline().split()[:3]])
+        if lc == "given":
+            vals = []
+            for line in f:
+                m = re.match(r"\s*[A-Za-z]+\s*:=\s*([-+0-9.eE]+)\s*;", line)
+                if m:
+                    vals.append(float(m.group(1)))
+            return InitialCondition(code, vals)
+    raise ValueError(f"{path}: unsupported IC code {code}")
+
+
+# ---------------------------------------------------------------------------
+# Restart files
+# ---------------------------------------------------------------------------
+_TYPES = {"int32": "i", "int64": "q", "uint32": "I", "uint64": "Q", "float64": "d", "float32": "f"}
+
+
+@dataclass
+class Restart:
+    header: Dict[str, object]
+    nmodes: np.ndarray           # per-element modal counts
+    data: np.ndarray             # [field][e][nmodes] (uniform order) as float64
+
+
+def read_restart(path: str, ne: int) -> Restart:
+    """Decode a DGM restart file (``.rst``/``.ref``) whose fields share ``ne`` elements."""
+    raw = open(path, "rb").read()
+    if raw[:1] == b"{":
+        depth, pos = 0, 0
+        for pos, ch in enumerate(raw):
+            if ch == ord("{"):
+                depth += 1
+            elif ch == ord("}"):
+                depth -= 1
+                if depth == 0:
+                    break
+        end = pos + 1
+        while end < len(raw) and raw[end:end + 1] in (b"\n", b"\r"):
+            end += 1
+        header = json.loads(raw[:pos + 1].decode())
+        types = header.get("Types", {})
+        gfmt = _TYPES[types.get("GlobalSize", "int32")]
+        lfmt = _TYPES[types.get("LocalSize", "int32")]
+        sfmt = _TYPES[types.get("Scalar", "float64")]
+        nf = int(header["Number of fields"])
+        off = end
+    else:
+        lines = raw.split(b"\n", 10)
+        hdr = [ln.decode(errors="replace") for ln in lines[:10]]
+        off = sum(len(ln) + 1 for ln in lines[:10])
+        nel, nsd, lmax = (int(x) for x in hdr[3].split()[:3])
+        header = {"Nel": nel, "Nsd": nsd, "Lmax": lmax, "Step": int(hdr[4].split()[0]),
+                  "Time": float(hdr[5].split()[0]), "Time step": float(hdr[6].split()[0]),
+                  "Viscosity": float(hdr[7].split()[0]),
+                  "Format": hdr[9].split()[0]}
+        names = hdr[8].split()
+        nf = len(names) - 2 if names[-2:] == ["Field", "names"] else len(names)
+        header["Field names"] = names[:nf]
+        gfmt, lfmt, sfmt = "i", "i", "d"
+    gsz, lsz, ssz = (struct.calcsize(c) for c in (gfmt, lfmt, sfmt))
+    fields = []
+    nmodes = None
+    for _ in range(nf):
+        off += gsz                                             # global length
+        nm = np.frombuffer(raw, dtype="<" + lfmt, count=ne, offset=off).astype(np.int64)
+        off += ne * lsz + ne * gsz                             # lengths, displacements
+        tot = int(nm.sum())
+        vals = np.frombuffer(raw, dtype="<" + sfmt, count=tot, offset=off).astype(np.float64)
+        off += tot * ssz
+        if nmodes is None:
+            nmodes = nm
+        if not np.all(nm == nm[0]):
+            raise ValueError(f"{path}: variable-order restart files are not supported")
+        fields.append(vals.reshape(ne, int(nm[0])))
+    if off != len(raw):
+        raise ValueError(f"{path}: {len(raw) - off} trailing bytes after {nf} fields")
+    return Restart(header=header, nmodes=nmodes, data=np.stack(fields))
+
+
+def write_restart(path: str, data: np.ndarray, header: Dict[str, object]) -> None:
+    """Write a JSON-header restart file (int32 sizes, float64 modes) readable by DGM.
+
+    ``data`` is ``[field][e][nmodes]``.
+    """
+    nf, ne, nm = data.shape
+    hdr = dict(header)
+    hdr.setdefault("Basis type", "Normalized")
+    hdr["Nel"] = ne
+    hdr["Number of fields"] = nf
+    hdr["Types"] = {"ByteOrder": "LittleEndian", "GlobalSize": "int32",
+                    "LocalSize": "int32", "Scalar": "float64"}
+    with open(path, "wb") as f:
+        f.write((json.dumps(hdr, indent=3, sort_keys=True) + "\n").encode())
+        for v in range(nf):
+            f.write(struct.pack("<i", ne * nm))
+            f.write(np.full(ne, nm, dtype="<i4").tobytes())
+            f.write((np.arange(ne, dtype="<i4") * nm).tobytes())
+            f.write(np.ascontiguousarray(data[v], dtype="<f8").tobytes())

@@ -1,0 +1,113 @@
+"""Initial conditions evaluated at the nodal (GLL) coordinates.
+
+Each function returns conservative variables ``U[var, ...]`` with the shape of
+the coordinate arrays.  Reference counterparts:
+
+* ``oseen_vortex``   ``Vortex`` IC code: superposed Oseen vortices in
+  conservation variables (src/Euler.cpp:920-937, src/EulerIC.hpp:203-237)
+* ``acoustic_pulse`` ``AcousticPulse`` IC code, 3D (src/EulerIC.hpp:238-300)
+* ``given_constant`` ``Given`` IC code with constant right-hand sides
+* ``taylor_green``   the 3D Taylor-Green vortex of runs/TaylorGreen/tg.ic with
+  the background pressure 1/(gamma Ma^2)
+* ``isentropic_vortex`` the 2D vortex of the C1 workload: freestream plus a
+  Gaussian isentropic vortex of given strength (Shu's form)
+* ``fourier_modes`` smooth seeded perturbation about a uniform state
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def oseen_vortex(x, y, vortices, gamma=1.4):
+    """``vortices``: iterable of (x0, y0, radius, amp, Ma); added onto zero state."""
+    U = np.zeros((4,) + np.shape(x))
+    for x0, y0, rad, amp, Ma in vortices:
+        r2 = ((x - x0) * (x - x0) + (y - y0) * (y - y0)) / rad / rad
+        rho = np.power(1.0 - 0.5 * (gamma - 1.0) * Ma * Ma * np.exp(1.0 - r2), 1.0 / (gamma - 1.0))
+        u = -amp * (y - y0) / rad * np.exp(0.5 * (1.0 - r2))
+        v = amp * (x - x0) / rad * np.exp(0.5 * (1.0 - r2))
+        t = np.power(rho, gamma - 1.0)
+        U[0] += rho
+        U[1] += rho * u
+        U[2] += rho * v
+        U[3] += amp * amp * rho * t / (gamma * (gamma - 1) * Ma * Ma) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def acoustic_pulse(x, amp, x0, sigma, gamma=1.4):
+    gm1 = gamma - 1.0
+    Ma, r, u, v, w = 1.0, 1.0, 0.0, 0.0, 0.0
+    e = 1.0 / (gamma * gm1 * Ma * Ma)
+    c = np.sqrt(gamma * gm1 * e)
+    c3 = np.exp(-0.5 * np.power((x - x0) / sigma, 2.0))
+    rp = (0.5 * c3) / (c * c)
+    up = c3 * 0.5 / (r * c)
+    pp = 0.5 * c3
+    ep = (pp / gm1 - rp * e) / r
+    r = r + amp * rp
+    u = u + amp * up
+    e = e + amp * ep
+    U = np.zeros((5,) + np.shape(x))
+    U[0] = r
+    U[1] = r * u
+    U[2] = r * v
+    U[3] = r * w
+    U[4] = r * e + 0.5 * r * (u * u + v * v + w * w)
+    return U
+
+
+def given_constant(shape, values):
+    U = np.zeros((len(values),) + tuple(shape))
+    for i, v in enumerate(values):
+        U[i] = v
+    return U
+
+
+def taylor_green(x, y, z, gamma=1.4, Ma=0.1):
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    p = 1.0 / (gamma * Ma * Ma) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    rho = np.ones_like(x)
+    U = np.zeros((5,) + np.shape(x))
+    U[0] = rho
+    U[1] = rho * u
+    U[2] = rho * v
+    U[4] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def isentropic_vortex(x, y, x0, y0, strength=5.0, rho0=1.0, u0=0.5, v0=0.0,
+                      p0=1.0 / 1.4, gamma=1.4, period=None):
+    """Freestream plus a Gaussian isentropic vortex; periodic images if ``period``."""
+    dx = x - x0
+    dy = y - y0
+    if period is not None:
+        dx = dx - period[0] * np.round(dx / period[0])
+        dy = dy - period[1] * np.round(dy / period[1])
+    r2 = dx * dx + dy * dy
+    f = strength / (2 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    u = u0 - dy * f
+    v = v0 + dx * f
+    T0 = p0 / rho0
+    T = T0 - (gamma - 1.0) * strength ** 2 / (8 * gamma * np.pi ** 2) * np.exp(1.0 - r2)
+    rho = rho0 * np.power(T / T0, 1.0 / (gamma - 1.0))
+    p = rho * T
+    U = np.zeros((4,) + np.shape(x))
+    U[0] = rho
+    U[1] = rho * u
+    U[2] = rho * v
+    U[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def fourier_modes(coords, lengths, nmodes, amp, rng):
+    """Sum of ``nmodes`` random smooth Fourier modes, amplitude ``amp`` (seeded)."""
+    out = np.zeros(np.shape(coords[0]))
+    for _ in range(nmodes):
+        phase = rng.uniform(0, 2 * np.pi)
+        arg = phase
+        for c, L in zip(coords, lengths):
+            k = rng.integers(1, 4)
+            arg = arg + 2 * np.pi * k * c / L
+        out += np.sin(arg)
+    return amp * out / max(nmodes, 1)

@@ -1,0 +1,51 @@
+"""Drive a DGM regression case (root.inp/.msh/.con/.ic) through the oracle."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from dgm_amd import formats, ic
+from oracle_lib import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def initial_state(icd, x, nsd, gamma=1.4):
+    code = icd.code.lower()
+    if code == "vortex":
+        n = int(icd.values[0])
+        vs = [icd.values[1 + 5 * i:6 + 5 * i] for i in range(n)]
+        return ic.oseen_vortex(x[0], x[1], vs, gamma)
+    if code == "acousticpulse":
+        return ic.acoustic_pulse(x[0], *icd.values, gamma=gamma)
+    if code == "given":
+        return ic.given_constant(x[0].shape, icd.values)
+    raise ValueError(code)
+
+
+def load_case(root):
+    base = os.path.join(GOLDEN, root)
+    prm = formats.read_inp(base + ".inp")
+    mesh = formats.read_mesh(base)
+    icd = formats.read_ic(base + ".ic")
+    return prm, mesh, icd
+
+
+def run_oracle_case(root, nsteps=None):
+    """Run the case as Domain::solve does (restart writes every Ntout steps
+    project the state, src/Domain.cpp:898-947) and return (oracle, modal state)."""
+    prm, mesh, icd = load_case(root)
+    o = Oracle(mesh, int(prm["p"]), int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    x = o.coords()
+    o.set_state(initial_state(icd, x, mesh.nsd))
+    o.project_state()
+    Nt = int(prm["Nt"]) if nsteps is None else nsteps
+    Ntout, dt, it = int(prm["Ntout"]), float(prm["dt"]), int(prm["inttype"])
+    done = 0
+    while done < Nt:
+        n = min(Ntout, Nt - done)
+        o.advance(it, dt, n)
+        done += n
+        o.project_state()
+    return o, o.modal()

@@ -1,0 +1,142 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    lib = ctypes.CDLL(LIB)
+    lib.orc_create.restype = ctypes.c_void_p
+    lib.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_int]
+    for name in ("orc_destroy", "orc_project_state"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+        getattr(lib, name).restype = None
+    for name in ("orc_qtot", "orc_nmodes", "orc_status"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+        getattr(lib, name).restype = ctypes.c_int
+    for name in ("orc_coords", "orc_set_state", "orc_get_state", "orc_modal", "orc_set_modal",
+                 "orc_norms"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, _dp]
+        getattr(lib, name).restype = None
+    lib.orc_geometry.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.orc_basis.argtypes = [ctypes.c_void_p] + [_dp] * 6
+    lib.orc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.orc_rhs.restype = ctypes.c_int
+    lib.orc_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.orc_advance.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """CPU restatement of one DGM Euler/NS domain (nodal GLL storage)."""
+
+    def __init__(self, mesh, p, eqn, gamma=1.4, vis=0.0, Pr=0.72, flux=0):
+        self.lib = _load()
+        self.nsd = mesh.nsd
+        self.ne = mesh.ne
+        self.p = p
+        self.nv = self.nsd + 2
+        self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
+        self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
+        self.h = self.lib.orc_create(self.nsd, self.ne, p, _ptr(self._vert), len(self._con),
+                                     _ptr(self._con, _ip), eqn, gamma, vis, Pr, flux)
+        self.qtot = self.lib.orc_qtot(self.h)
+        self.nm = self.lib.orc_nmodes(self.h)
+
+    def __del__(self):
+        try:
+            self.lib.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def coords(self):
+        x = np.zeros((self.nsd, self.ne, self.qtot))
+        self.lib.orc_coords(self.h, _ptr(x))
+        return x
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nv, self.ne, self.qtot)
+        self.lib.orc_set_state(self.h, _ptr(U))
+
+    def state(self):
+        U = np.zeros((self.nv, self.ne, self.qtot))
+        self.lib.orc_get_state(self.h, _ptr(U))
+        return U
+
+    def project_state(self):
+        self.lib.orc_project_state(self.h)
+
+    def modal(self):
+        Uh = np.zeros((self.nv, self.ne, self.nm))
+        self.lib.orc_modal(self.h, _ptr(Uh))
+        return Uh
+
+    def set_modal(self, Uh):
+        Uh = np.ascontiguousarray(Uh, dtype=np.float64)
+        self.lib.orc_set_modal(self.h, _ptr(Uh))
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        F = np.zeros_like(U)
+        st = self.lib.orc_rhs(self.h, _ptr(U), _ptr(F))
+        if st:
+            raise FloatingPointError("oracle: non-realizable state")
+        return F
+
+    def advance(self, inttype, dt, nsteps):
+        st = self.lib.orc_advance(self.h, inttype, dt, nsteps)
+        if st:
+            raise FloatingPointError(f"oracle advance failed ({st})")
+
+    def norms(self):
+        out = np.zeros((self.nv, 3))
+        self.lib.orc_norms(self.h, _ptr(out))
+        return out
+
+    def geometry(self):
+        J = np.zeros(self.ne)
+        M = np.zeros((self.ne, 9))
+        self.lib.orc_geometry(self.h, _ptr(J), _ptr(M))
+        return J, M
+
+    def basis(self):
+        q, qf, L = self.p + 2, self.p + 1, self.p + 1
+        z, w, zg, wg = np.zeros(q), np.zeros(q), np.zeros(qf), np.zeros(qf)
+        D, B = np.zeros((q, q)), np.zeros((L, q))
+        self.lib.orc_basis(self.h, _ptr(z), _ptr(w), _ptr(zg), _ptr(wg), _ptr(D), _ptr(B))
+        return dict(z=z, w=w, zg=zg, wg=wg, D=D, B=B)
+
+
+def dgm_diff(oracle_a: Oracle, Uh_ref, Uh_test):
+    """``Domain::diff`` norms (src/Domain.cpp:1000-1071): returns (ref, diff) norm arrays."""
+    oracle_a.set_modal(Uh_ref)
+    ref = oracle_a.norms()
+    oracle_a.set_modal(Uh_ref - Uh_test)
+    dif = oracle_a.norms()
+    return ref, dif
