@@ -1,0 +1,701 @@
+// Host engine and C ABI (include/dgx.h): mesh tables, affine geometry, side
+// links and orientation flips, basis tables, device buffers, and the explicit
+// Runge-Kutta drivers that chain the residual kernels on one HIP stream.
+#include "../../include/dgx.h"
+#include "dgx_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return fail(DGX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+typedef long double ld;
+
+// ---------------------------------------------------------------------------
+// 1D quadrature and basis tables from Legendre recurrences in long double.
+// GLL nodes: -1, +1 and the roots of P'_{q-1}; Gauss nodes: roots of P_{qf}
+// (Karniadakis & Sherwin, Appendix B; src/Polylib.tpp restates the same).
+void leg(int n, ld z, ld &P, ld &dP) {
+  ld p0 = 1, p1 = z;
+  if (n == 0) { P = 1; dP = 0; return; }
+  for (int k = 1; k < n; ++k) {
+    ld p2 = ((2 * k + 1) * z * p1 - k * p0) / (k + 1);
+    p0 = p1; p1 = p2;
+  }
+  P = p1;
+  if (fabsl(z * z - 1) < 1e-30L) dP = (z > 0 ? 1 : ((n % 2) ? 1 : -1)) * (ld)n * (n + 1) / 2;
+  else dP = n * (z * p1 - p0) / (z * z - 1);
+}
+
+void gll_nodes(int q, std::vector<ld> &z, std::vector<ld> &w) {
+  const int N = q - 1;
+  z.assign(q, 0); w.assign(q, 0);
+  z[0] = -1; z[N] = 1;
+  for (int i = 1; i < N; ++i) {
+    ld x = -cosl(M_PIl * i / N);
+    for (int it = 0; it < 100; ++it) {
+      ld P, dP; leg(N, x, P, dP);
+      const ld d2 = (2 * x * dP - N * (N + 1) * P) / (1 - x * x);
+      const ld dx = dP / d2;
+      x -= dx;
+      if (fabsl(dx) < 1e-30L) break;
+    }
+    z[i] = x;
+  }
+  for (int i = 0; i < q; ++i) { ld P, dP; leg(N, z[i], P, dP); w[i] = 2.0L / (N * (N + 1) * P * P); }
+}
+
+void gauss_nodes(int n, std::vector<ld> &z, std::vector<ld> &w) {
+  z.assign(n, 0); w.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    ld x = -cosl(M_PIl * (4 * i + 3) / (4 * n + 2));
+    for (int it = 0; it < 100; ++it) {
+      ld P, dP; leg(n, x, P, dP);
+      const ld dx = P / dP;
+      x -= dx;
+      if (fabsl(dx) < 1e-30L) break;
+    }
+    ld P, dP; leg(n, x, P, dP);
+    z[i] = x; w[i] = 2.0L / ((1 - x * x) * dP * dP);
+  }
+}
+
+ld lagrange(const std::vector<ld> &nodes, int j, ld x) {
+  ld h = 1;
+  for (size_t m = 0; m < nodes.size(); ++m)
+    if ((int)m != j) h *= (x - nodes[m]) / (nodes[j] - nodes[m]);
+  return h;
+}
+
+ld normleg(int l, ld x) { ld P, dP; leg(l, x, P, dP); return P * sqrtl((2 * l + 1) / 2.0L); }
+
+struct HostBasis {
+  int p, L, Q, QF;
+  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig;
+};
+
+HostBasis make_basis(int p) {
+  HostBasis b;
+  b.p = p; b.L = p + 1; b.Q = p + 2; b.QF = p + 1;
+  const int L = b.L, Q = b.Q, QF = b.QF;
+  std::vector<ld> z, w, zg, wg;
+  gll_nodes(Q, z, w);
+  gauss_nodes(QF, zg, wg);
+  const int N = Q - 1;
+  std::vector<ld> PN(Q), Dl(Q * Q);
+  for (int i = 0; i < Q; ++i) { ld dP; leg(N, z[i], PN[i], dP); }
+  for (int i = 0; i < Q; ++i)
+    for (int j = 0; j < Q; ++j) {
+      ld v = 0;
+      if (i != j) v = PN[i] / (PN[j] * (z[i] - z[j]));
+      else if (i == 0) v = -(ld)N * (N + 1) / 4;
+      else if (i == N) v = (ld)N * (N + 1) / 4;
+      Dl[i * Q + j] = v;
+    }
+  b.z.resize(Q); b.w.resize(Q); b.D.resize(Q * Q);
+  for (int i = 0; i < Q; ++i) { b.z[i] = (double)z[i]; b.w[i] = (double)w[i]; }
+  for (int i = 0; i < Q * Q; ++i) b.D[i] = (double)Dl[i];
+  b.BT.resize(Q * L); b.BW.resize(L * Q); b.QD.resize(L * Q);
+  b.Bg.resize(QF * L); b.Bend.resize(2 * L); b.PW.resize(L * QF); b.Ig.resize(QF * Q);
+  for (int l = 0; l < L; ++l) {
+    for (int i = 0; i < Q; ++i) {
+      b.BT[i * L + l] = (double)normleg(l, z[i]);
+      b.BW[l * Q + i] = (double)(normleg(l, z[i]) * w[i]);
+    }
+    for (int k = 0; k < Q; ++k) {
+      ld s = 0;
+      for (int i = 0; i < Q; ++i) s += normleg(l, z[i]) * w[i] * Dl[i * Q + k];
+      b.QD[l * Q + k] = (double)s;
+    }
+    for (int a = 0; a < QF; ++a) {
+      b.Bg[a * L + l] = (double)normleg(l, zg[a]);
+      b.PW[l * QF + a] = (double)(wg[a] * normleg(l, zg[a]));
+    }
+    b.Bend[l] = (double)normleg(l, -1.0L);
+    b.Bend[L + l] = (double)normleg(l, 1.0L);
+  }
+  for (int a = 0; a < QF; ++a)
+    for (int i = 0; i < Q; ++i) b.Ig[a * Q + i] = (double)lagrange(z, i, zg[a]);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+struct Side {
+  bool master = false;
+  int le = -1, ls = -1;
+  bool fa = false, fb = false, tr = false;
+};
+
+}  // namespace
+
+struct dgx_domain {
+  int nsd = 0, ne = 0, p = 0, L = 0, Q = 0, QF = 0, nf = 0, nm = 0, nq = 0, ns = 0, qs = 0;
+  int eqn = 0, flux = 0, prec = 0;
+  bool visc = false;
+  double gamma = 1.4, vis = 0.0, Pr = 0.72;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  HostBasis hb;
+  std::vector<double> coords, metric, J;
+  // device
+  double *d_tab = nullptr, *d_metric = nullptr, *d_fnorm = nullptr, *d_fjr = nullptr;
+  int *d_nbr = nullptr, *d_nside = nullptr, *d_flag = nullptr;
+  uint8_t *d_permid = nullptr;
+  uint16_t *d_perm = nullptr;
+  double *U = nullptr, *Ut = nullptr, *R0 = nullptr, *R1 = nullptr, *T = nullptr;
+  double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr;
+  dgx::Tab tab;
+  dgx::Geo geo;
+  dgx::Phys phys;
+  long long stages = 0;
+
+  size_t state_len() const { return (size_t)nf * ne * nm; }
+};
+
+namespace {
+
+int alloc(double **p, size_t n) {
+  if (hipMalloc((void **)p, n * sizeof(double) + 64) != hipSuccess)
+    return fail(DGX_ERR_MEMORY, "hipMalloc of " + std::to_string(n * 8) + " bytes failed");
+  return DGX_OK;
+}
+
+int launch(dgx_domain *d, int op, const double *U, double *out) {
+  dgx::LaunchArgs a;
+  a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf; a.ns = d->visc;
+  a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
+  a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
+  hipError_t e = (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return DGX_OK;
+}
+
+int transform(dgx_domain *d, int op, const double *x, double *y) {
+  dgx::LaunchArgs a;
+  a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf;
+  a.X = x; a.Y = y; a.tab = d->tab; a.stream = d->stream;
+  hipError_t e = (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("transform: ") + hipGetErrorString(e));
+  return DGX_OK;
+}
+
+// Residual of state `U` into `R` (Domain::time_derivative).
+int residual(dgx_domain *d, const double *U, double *R) {
+  int rc;
+  if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
+  if (d->visc) {
+    if ((rc = launch(d, dgx::OP_GRAD, U, nullptr))) return rc;
+    if ((rc = launch(d, dgx::OP_VISC, U, nullptr))) return rc;
+  }
+  if ((rc = launch(d, dgx::OP_RHS, U, R))) return rc;
+  ++d->stages;
+  return DGX_OK;
+}
+
+int ensure_nodal(dgx_domain *d) {
+  if (d->nodal) return DGX_OK;
+  return alloc(&d->nodal, (size_t)d->nf * d->ne * d->nq);
+}
+
+int build(dgx_domain *d, const dgx_domain_desc *desc) {
+  const int nsd = desc->nsd, ne = desc->ne, p = desc->p;
+  d->nsd = nsd; d->ne = ne; d->p = p;
+  d->L = p + 1; d->Q = p + 2; d->QF = p + 1;
+  d->nf = nsd + 2;
+  d->nm = nsd == 2 ? d->L * d->L : d->L * d->L * d->L;
+  d->nq = nsd == 2 ? d->Q * d->Q : d->Q * d->Q * d->Q;
+  d->ns = nsd == 2 ? 4 : 6;
+  d->qs = nsd == 2 ? d->QF : d->QF * d->QF;
+  d->eqn = desc->eqntype; d->flux = desc->flux; d->prec = desc->precision;
+  d->visc = (desc->eqntype == DGX_NAVIER_STOKES || desc->eqntype == DGX_NAVIER_STOKES3D);
+  d->gamma = desc->gamma > 0 ? desc->gamma : 1.4;
+  d->vis = desc->vis;
+  d->Pr = desc->Pr > 0 ? desc->Pr : 0.72;
+  d->hb = make_basis(p);
+  const HostBasis &b = d->hb;
+  const int Q = d->Q, nvert = nsd == 2 ? 4 : 8;
+
+  // geometry (affine elements: constant Jacobian, src/Quad.cpp:1227-1292,
+  // src/Hex.cpp:1124-1286) and nodal coordinates (vertex-mode interpolation)
+  d->coords.assign((size_t)nsd * ne * d->nq, 0.0);
+  d->metric.assign((size_t)ne * 9, 0.0);
+  d->J.assign(ne, 0.0);
+  std::vector<double> fnorm((size_t)ne * d->ns * 3, 0.0), fjr((size_t)ne * d->ns, 0.0);
+  auto vmode = [](int which, double x) { return which ? 0.5 * (1.0 + x) : 0.5 * (1.0 - x); };
+  for (int e = 0; e < ne; ++e) {
+    const double *vx = desc->vert + (size_t)e * nsd * nvert, *vy = vx + nvert;
+    const double *vz = nsd == 3 ? vy + nvert : nullptr;
+    double *cx = &d->coords[((size_t)0 * ne + e) * d->nq];
+    double *cy = &d->coords[((size_t)1 * ne + e) * d->nq];
+    double *mt = &d->metric[(size_t)e * 9];
+    if (nsd == 2) {
+      static const int ai[4] = {0, 1, 1, 0}, bi[4] = {0, 0, 1, 1};
+      for (int n = 0; n < 4; ++n)
+        for (int i = 0; i < Q; ++i)
+          for (int j = 0; j < Q; ++j) {
+            const double phi = vmode(ai[n], b.z[i]) * vmode(bi[n], b.z[j]);
+            cx[i * Q + j] += vx[n] * phi;
+            cy[i * Q + j] += vy[n] * phi;
+          }
+      const double xr = (vx[1] - vx[0]) * 0.5, xs = (vx[3] - vx[0]) * 0.5;
+      const double yr = (vy[1] - vy[0]) * 0.5, ys = (vy[3] - vy[0]) * 0.5;
+      const double tol = 1e-10 * (std::fabs(xr) + std::fabs(xs) + std::fabs(yr) + std::fabs(ys));
+      if (std::fabs(xr - (vx[2] - vx[3]) * 0.5) > tol || std::fabs(xs - (vx[2] - vx[1]) * 0.5) > tol ||
+          std::fabs(yr - (vy[2] - vy[3]) * 0.5) > tol || std::fabs(ys - (vy[2] - vy[1]) * 0.5) > tol)
+        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " is not affine");
+      const double det = xr * ys - xs * yr;
+      if (!(det > 0)) return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has J <= 0");
+      d->J[e] = det;
+      mt[0] = ys / det; mt[1] = -xs / det; mt[3] = -yr / det; mt[4] = xr / det;
+      const double nrm[4][2] = {{-mt[3], -mt[4]}, {mt[0], mt[1]}, {mt[3], mt[4]}, {-mt[0], -mt[1]}};
+      const double elen[4] = {std::sqrt(xr * xr + yr * yr), std::sqrt(xs * xs + ys * ys),
+                              std::sqrt(xr * xr + yr * yr), std::sqrt(xs * xs + ys * ys)};
+      for (int s = 0; s < 4; ++s) {
+        const double len = std::sqrt(nrm[s][0] * nrm[s][0] + nrm[s][1] * nrm[s][1]);
+        fnorm[((size_t)e * 4 + s) * 3 + 0] = nrm[s][0] / len;
+        fnorm[((size_t)e * 4 + s) * 3 + 1] = nrm[s][1] / len;
+        fjr[(size_t)e * 4 + s] = elen[s] / det;
+      }
+    } else {
+      double *cz = &d->coords[((size_t)2 * ne + e) * d->nq];
+      static const int ai[8] = {0, 1, 1, 0, 0, 1, 1, 0}, bi[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                       ci[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+      for (int n = 0; n < 8; ++n)
+        for (int i = 0; i < Q; ++i)
+          for (int j = 0; j < Q; ++j)
+            for (int k = 0; k < Q; ++k) {
+              const double phi = vmode(ai[n], b.z[i]) * vmode(bi[n], b.z[j]) * vmode(ci[n], b.z[k]);
+              const size_t m = (size_t)(i * Q + j) * Q + k;
+              cx[m] += vx[n] * phi; cy[m] += vy[n] * phi; cz[m] += vz[n] * phi;
+            }
+      const double xr = (vx[1] - vx[0]) * 0.5, xs = (vx[3] - vx[0]) * 0.5, xt = (vx[4] - vx[0]) * 0.5;
+      const double yr = (vy[1] - vy[0]) * 0.5, ys = (vy[3] - vy[0]) * 0.5, yt = (vy[4] - vy[0]) * 0.5;
+      const double zr = (vz[1] - vz[0]) * 0.5, zs = (vz[3] - vz[0]) * 0.5, zt = (vz[4] - vz[0]) * 0.5;
+      // affine check: opposite edges equal (src/Hex.cpp:1139-1163)
+      const double scale = std::fabs(xr) + std::fabs(ys) + std::fabs(zt) + std::fabs(xs) + std::fabs(xt) +
+                           std::fabs(yr) + std::fabs(yt) + std::fabs(zr) + std::fabs(zs);
+      const double tol = 1e-10 * scale;
+      auto nq_ = [&](double a, double c) { return std::fabs(a - c) > tol; };
+      if (nq_(xr, (vx[6] - vx[7]) * 0.5) || nq_(xs, (vx[6] - vx[5]) * 0.5) || nq_(xt, (vx[6] - vx[2]) * 0.5) ||
+          nq_(yr, (vy[6] - vy[7]) * 0.5) || nq_(ys, (vy[6] - vy[5]) * 0.5) || nq_(yt, (vy[6] - vy[2]) * 0.5) ||
+          nq_(zr, (vz[6] - vz[7]) * 0.5) || nq_(zs, (vz[6] - vz[5]) * 0.5) || nq_(zt, (vz[6] - vz[2]) * 0.5))
+        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " is not affine");
+      const double det = xr * ys * zt + yr * zs * xt + zr * xs * yt - xt * ys * zr - zt * xs * yr - xr * zs * yt;
+      if (!(det > 0)) return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has J <= 0");
+      d->J[e] = det;
+      mt[0] = (ys * zt - yt * zs) / det;  mt[1] = -(xs * zt - xt * zs) / det; mt[2] = (xs * yt - xt * ys) / det;
+      mt[3] = -(yr * zt - yt * zr) / det; mt[4] = (xr * zt - xt * zr) / det;  mt[5] = -(xr * yt - xt * yr) / det;
+      mt[6] = (yr * zs - ys * zr) / det;  mt[7] = -(xr * zs - xs * zr) / det; mt[8] = (xr * ys - xs * yr) / det;
+      auto cross = [](const double a[3], const double c[3]) {
+        const double x = a[1] * c[2] - a[2] * c[1], y = a[2] * c[0] - a[0] * c[2], z = a[0] * c[1] - a[1] * c[0];
+        return std::sqrt(x * x + y * y + z * z);
+      };
+      const double Xr[3] = {xr, yr, zr}, Xs[3] = {xs, ys, zs}, Xt[3] = {xt, yt, zt};
+      // faces: 0 (t=-1), 1 (s=-1), 2 (r=+1), 3 (s=+1), 4 (r=-1), 5 (t=+1)
+      const int dirm[6] = {2, 1, 0, 1, 0, 2};
+      const double sgn[6] = {-1, -1, 1, 1, -1, 1};
+      const double fj[6] = {cross(Xr, Xs), cross(Xr, Xt), cross(Xs, Xt), cross(Xr, Xt), cross(Xs, Xt), cross(Xr, Xs)};
+      for (int s = 0; s < 6; ++s) {
+        const double *g = mt + 3 * dirm[s];
+        const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        for (int c = 0; c < 3; ++c) fnorm[((size_t)e * 6 + s) * 3 + c] = sgn[s] * g[c] / len;
+        fjr[(size_t)e * 6 + s] = fj[s] / det;
+      }
+    }
+  }
+
+  // side links: the first .con row naming a pair makes its side the master
+  // (src/Field.cpp:3478-3556); flips are set on masters (src/Field.cpp:416-668)
+  std::vector<Side> sides((size_t)ne * d->ns);
+  for (int r = 0; r < desc->ncon; ++r) {
+    const int e = desc->con[4 * r], s = desc->con[4 * r + 1], le = desc->con[4 * r + 2], ls = desc->con[4 * r + 3];
+    if (e < 0 || e >= ne || le < 0 || le >= ne || s < 0 || s >= d->ns || ls < 0 || ls >= d->ns)
+      return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + " out of range");
+    Side &me = sides[(size_t)e * d->ns + s], &nb = sides[(size_t)le * d->ns + ls];
+    if (nb.master) {
+      if (nb.le != e || nb.ls != s) return fail(DGX_ERR_ARGUMENT, "contradictory side links");
+      continue;
+    }
+    me.master = true; me.le = le; me.ls = ls;
+    nb.master = false; nb.le = e; nb.ls = s;
+  }
+  for (size_t i = 0; i < sides.size(); ++i)
+    if (sides[i].le < 0)
+      return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(i / d->ns) + " side " +
+                                        std::to_string(i % d->ns) +
+                                        " has no neighbour (boundary conditions are not supported yet)");
+  static const int con2d[4][4] = {{1, 1, 0, 0}, {1, 1, 0, 0}, {0, 0, 1, 1}, {0, 0, 1, 1}};
+  static const int svm[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5}, {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+  for (int e = 0; e < ne; ++e)
+    for (int s = 0; s < d->ns; ++s) {
+      Side &S = sides[(size_t)e * d->ns + s];
+      if (!S.master) continue;
+      if (nsd == 2) { S.fa = con2d[s][S.ls]; continue; }
+      auto vec = [&](int el, int v1, int v0, double o[3]) {
+        const double *bb = desc->vert + (size_t)el * 3 * 8;
+        double n = 0;
+        for (int c = 0; c < 3; ++c) { o[c] = bb[c * 8 + v1] - bb[c * 8 + v0]; n += o[c] * o[c]; }
+        n = std::sqrt(n);
+        for (int c = 0; c < 3; ++c) o[c] /= n;
+      };
+      auto dot = [](const double *a, const double *c) { return a[0] * c[0] + a[1] * c[1] + a[2] * c[2]; };
+      double a0[3], b0[3], a1[3], b1[3], a1p[3];
+      vec(e, svm[s][1], svm[s][0], a0);
+      vec(e, svm[s][3], svm[s][0], b0);
+      vec(S.le, svm[S.ls][1], svm[S.ls][0], a1);
+      vec(S.le, svm[S.ls][3], svm[S.ls][0], b1);
+      vec(S.le, svm[S.ls][2], svm[S.ls][3], a1p);
+      const double x1 = dot(a0, a1), x2 = dot(a0, a1p);
+      if (std::fabs(std::fabs(x1) - 1) < 1e-8 || std::fabs(std::fabs(x2) - 1) < 1e-8) {
+        S.tr = false; S.fa = x1 < 0; S.fb = dot(b0, b1) < 0;
+      } else {
+        S.tr = true; S.fa = dot(b0, a1) < 0; S.fb = dot(a0, b1) < 0;
+      }
+    }
+  // own face point -> neighbour face point permutations (src/BaseQuad.hpp:53-57,
+  // src/BaseHex.hpp:291-313): master point sideMap(g) pairs slave point g
+  const int QF = d->QF, qs = d->qs;
+  auto sidemap = [&](const Side &S, int q) {
+    if (nsd == 2) return S.fa ? QF - 1 - q : q;
+    const int il = q / QF, jl = q % QF;
+    int i, j;
+    if (S.tr) { j = il; i = jl; if (S.fa) j = QF - 1 - j; if (S.fb) i = QF - 1 - i; }
+    else { i = il; j = jl; if (S.fa) i = QF - 1 - i; if (S.fb) j = QF - 1 - j; }
+    return j + QF * i;
+  };
+  std::map<std::vector<uint16_t>, int> perms;
+  std::vector<uint16_t> permtab;
+  std::vector<uint8_t> permid((size_t)ne * d->ns);
+  std::vector<int> nbr((size_t)ne * d->ns), nside((size_t)ne * d->ns);
+  for (int e = 0; e < ne; ++e)
+    for (int s = 0; s < d->ns; ++s) {
+      const Side &S = sides[(size_t)e * d->ns + s];
+      std::vector<uint16_t> m(qs);
+      if (S.master) {
+        for (int g = 0; g < qs; ++g) m[sidemap(S, g)] = (uint16_t)g;
+      } else {
+        const Side &M = sides[(size_t)S.le * d->ns + S.ls];
+        for (int g = 0; g < qs; ++g) m[g] = (uint16_t)sidemap(M, g);
+      }
+      auto it = perms.find(m);
+      int id;
+      if (it == perms.end()) {
+        id = (int)perms.size();
+        perms[m] = id;
+        permtab.insert(permtab.end(), m.begin(), m.end());
+      } else id = it->second;
+      permid[(size_t)e * d->ns + s] = (uint8_t)id;
+      nbr[(size_t)e * d->ns + s] = S.le;
+      nside[(size_t)e * d->ns + s] = S.ls;
+    }
+
+  // device buffers
+  const std::vector<const std::vector<double> *> tabs = {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig};
+  size_t ntab = 0;
+  for (auto t : tabs) ntab += t->size();
+  int rc;
+  if ((rc = alloc(&d->d_tab, ntab))) return rc;
+  {
+    std::vector<double> packed;
+    for (auto t : tabs) packed.insert(packed.end(), t->begin(), t->end());
+    HIPCHK(hipMemcpy(d->d_tab, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
+    const double *ptr = d->d_tab;
+    const double **dst[7] = {&d->tab.BT, &d->tab.BW, &d->tab.QD, &d->tab.Bg, &d->tab.Bend, &d->tab.PW, &d->tab.Ig};
+    for (int i = 0; i < 7; ++i) { *dst[i] = ptr; ptr += tabs[i]->size(); }
+  }
+  if ((rc = alloc(&d->d_metric, d->metric.size()))) return rc;
+  if ((rc = alloc(&d->d_fnorm, fnorm.size()))) return rc;
+  if ((rc = alloc(&d->d_fjr, fjr.size()))) return rc;
+  HIPCHK(hipMemcpy(d->d_metric, d->metric.data(), d->metric.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_fnorm, fnorm.data(), fnorm.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_fjr, fjr.data(), fjr.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void **)&d->d_nbr, nbr.size() * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d->d_nside, nside.size() * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d->d_permid, permid.size()));
+  HIPCHK(hipMalloc((void **)&d->d_perm, permtab.size() * sizeof(uint16_t)));
+  HIPCHK(hipMalloc((void **)&d->d_flag, 64));
+  HIPCHK(hipMemcpy(d->d_nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_nside, nside.data(), nside.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_permid, permid.data(), permid.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_perm, permtab.data(), permtab.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(d->d_flag, 0, 64));
+  d->geo.metric = d->d_metric; d->geo.fnorm = d->d_fnorm; d->geo.fjr = d->d_fjr;
+  d->geo.nbr = d->d_nbr; d->geo.nside = d->d_nside; d->geo.permid = d->d_permid; d->geo.perm = d->d_perm;
+  d->phys.gamma = d->gamma; d->phys.vis = d->vis; d->phys.Pr = d->Pr; d->phys.flux = d->flux;
+
+  const size_t n = d->state_len();
+  if ((rc = alloc(&d->U, n)) || (rc = alloc(&d->Ut, n)) || (rc = alloc(&d->R0, n)) || (rc = alloc(&d->R1, n)))
+    return rc;
+  HIPCHK(hipMemset(d->U, 0, n * 8));
+  HIPCHK(hipMemset(d->Ut, 0, n * 8));
+  HIPCHK(hipMemset(d->R0, 0, n * 8));
+  HIPCHK(hipMemset(d->R1, 0, n * 8));
+  if ((rc = alloc(&d->T, (size_t)ne * d->ns * d->nf * qs))) return rc;
+  if (d->visc) {
+    const int ng = nsd == 2 ? 6 : 12;
+    if ((rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
+    if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * d->nm))) return rc;
+    if ((rc = alloc(&d->TV, (size_t)ne * d->ns * (d->nf - 1) * qs))) return rc;
+  }
+  return DGX_OK;
+}
+
+void release(dgx_domain *d) {
+  void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
+                  d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dgx_version(void) { return DGX_VERSION; }
+const char *dgx_last_error(void) { return g_err.c_str(); }
+
+int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out) {
+  if (!desc || !out) return fail(DGX_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (desc->nsd != 2 && desc->nsd != 3) return fail(DGX_ERR_ARGUMENT, "nsd must be 2 or 3");
+  if (desc->p < 1 || desc->p > 10) return fail(DGX_ERR_ARGUMENT, "p must be in [1, 10]");
+  if (desc->ne <= 0 || !desc->vert || !desc->con) return fail(DGX_ERR_ARGUMENT, "empty mesh");
+  const bool euler = desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES;
+  const bool euler3 = desc->eqntype == DGX_EULER3D || desc->eqntype == DGX_NAVIER_STOKES3D;
+  if ((desc->nsd == 2 && !euler) || (desc->nsd == 3 && !euler3))
+    return fail(DGX_ERR_ARGUMENT, "eqntype does not match nsd");
+  if (desc->flux == DGX_FLUX_ROE && desc->nsd == 3)
+    return fail(DGX_ERR_ARGUMENT, "the Roe flux is 2D only (Euler3d is Lax-Friedrichs only)");
+  if (desc->precision != DGX_FP64) return fail(DGX_ERR_ARGUMENT, "only DGX_FP64 is available");
+  if (desc->eqntype == DGX_NAVIER_STOKES3D && dgx::lds_bytes3(desc->p, 3) > 160 * 1024)
+    return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d supports p <= 7 (LDS)");
+  dgx_domain *d = new dgx_domain();
+  if (stream) d->stream = (hipStream_t)stream;
+  else {
+    if (hipStreamCreate(&d->stream) != hipSuccess) { delete d; return fail(DGX_ERR_DEVICE, "hipStreamCreate failed"); }
+    d->own_stream = true;
+  }
+  int rc = build(d, desc);
+  if (rc) { release(d); delete d; return rc; }
+  *out = d;
+  return DGX_OK;
+}
+
+int dgx_domain_destroy(dgx_handle h) {
+  if (!h) return DGX_OK;
+  (void)hipStreamSynchronize(h->stream);
+  release(h);
+  delete h;
+  return DGX_OK;
+}
+
+int dgx_sizes(dgx_handle h, int *qtot, int *nmodes, int *nfields, int *ne) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  if (qtot) *qtot = h->nq;
+  if (nmodes) *nmodes = h->nm;
+  if (nfields) *nfields = h->nf;
+  if (ne) *ne = h->ne;
+  return DGX_OK;
+}
+
+int dgx_coordinates(dgx_handle h, double *xyz) {
+  if (!h || !xyz) return fail(DGX_ERR_ARGUMENT, "null argument");
+  std::memcpy(xyz, h->coords.data(), h->coords.size() * sizeof(double));
+  return DGX_OK;
+}
+
+int dgx_state_device_ptr(dgx_handle h, double **U) {
+  if (!h || !U) return fail(DGX_ERR_ARGUMENT, "null argument");
+  *U = h->U;
+  return DGX_OK;
+}
+
+int dgx_state_upload(dgx_handle h, const double *U, int layout, int on_device) {
+  if (!h || !U) return fail(DGX_ERR_ARGUMENT, "null argument");
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (layout == DGX_MODAL) {
+    HIPCHK(hipMemcpyAsync(h->U, U, h->state_len() * 8, kind, h->stream));
+    return DGX_OK;
+  }
+  int rc = ensure_nodal(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h->nodal, U, (size_t)h->nf * h->ne * h->nq * 8, kind, h->stream));
+  return transform(h, dgx::OP_FORWARD, h->nodal, h->U);
+}
+
+static int download(dgx_handle h, const double *src, double *dst, int layout, int on_device) {
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (layout == DGX_MODAL) {
+    HIPCHK(hipMemcpyAsync(dst, src, h->state_len() * 8, kind, h->stream));
+  } else {
+    int rc = ensure_nodal(h);
+    if (rc) return rc;
+    if ((rc = transform(h, dgx::OP_BACKWARD, src, h->nodal))) return rc;
+    HIPCHK(hipMemcpyAsync(dst, h->nodal, (size_t)h->nf * h->ne * h->nq * 8, kind, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DGX_OK;
+}
+
+int dgx_state_download(dgx_handle h, double *U, int layout, int on_device) {
+  if (!h || !U) return fail(DGX_ERR_ARGUMENT, "null argument");
+  return download(h, h->U, U, layout, on_device);
+}
+
+int dgx_rhs(dgx_handle h) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  return residual(h, h->U, h->R0);
+}
+
+int dgx_residual_download(dgx_handle h, double *F, int layout, int on_device) {
+  if (!h || !F) return fail(DGX_ERR_ARGUMENT, "null argument");
+  return download(h, h->R0, F, layout, on_device);
+}
+
+int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  const size_t n = h->state_len();
+  hipStream_t s = h->stream;
+  int rc;
+#define RUN(x) do { if ((rc = (x))) return rc; } while (0)
+#define BL(x) do { hipError_t _e = (x); if (_e != hipSuccess) return fail(DGX_ERR_DEVICE, hipGetErrorString(_e)); } while (0)
+  for (int step = 0; step < nsteps; ++step) {
+    if (inttype == DGX_RK4) {
+      // RungeKutta::advance (src/TimeInt.cpp:267-302)
+      static const double a[3] = {0.5, 0.5, 1.0};
+      static const double b[4] = {0.16666666666666667, 0.33333333333333333, 0.33333333333333333,
+                                  0.16666666666666667};
+      RUN(residual(h, h->U, h->R0));
+      BL(dgx::blas_axpy(-a[0] * dt, h->R0, h->U, h->Ut, n, s));
+      BL(dgx::blas_scale(-b[0], h->R0, n, s));
+      RUN(residual(h, h->Ut, h->R1));
+      BL(dgx::blas_axpy(-a[1] * dt, h->R1, h->U, h->Ut, n, s));
+      BL(dgx::blas_axpy(-b[1], h->R1, h->R0, h->R0, n, s));
+      RUN(residual(h, h->Ut, h->R1));
+      BL(dgx::blas_axpy(-a[2] * dt, h->R1, h->U, h->Ut, n, s));
+      BL(dgx::blas_axpy(-b[2], h->R1, h->R0, h->R0, n, s));
+      RUN(residual(h, h->Ut, h->R1));
+      BL(dgx::blas_axpy(-b[3], h->R1, h->R0, h->R0, n, s));
+      BL(dgx::blas_axpy(dt, h->R0, h->U, h->U, n, s));
+    } else if (inttype == DGX_TVD) {
+      // TVD::advance (src/TimeInt.cpp:824-856)
+      RUN(residual(h, h->U, h->R0));
+      BL(dgx::blas_axpy(-dt, h->R0, h->U, h->Ut, n, s));
+      RUN(residual(h, h->Ut, h->R0));
+      BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
+      BL(dgx::blas_axpby(3.0 / 4.0, h->U, 1.0 / 4.0, h->R0, h->Ut, n, s));
+      RUN(residual(h, h->Ut, h->R0));
+      BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
+      BL(dgx::blas_axpby(2.0 / 3.0, h->R0, 1.0 / 3.0, h->U, h->U, n, s));
+    } else if (inttype == DGX_LSRK14) {
+      // LSRK14::advance (src/TimeInt.cpp:1989-2013), coefficients :1870-1917
+      static const double A[14] = {0.0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
+          -0.8552979934029281, -3.9564138245774565, -1.5780575380587385, -2.0837094552574054,
+          -0.7483334182761610, -0.7032861106563359, 0.0013917096117681, -0.0932075369637460,
+          -0.9514200470875948, -7.1151571693922548};
+      static const double B[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
+          0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
+          0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
+          0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
+      for (int st = 0; st < 14; ++st) {
+        RUN(residual(h, h->U, h->R0));
+        BL(dgx::blas_axpby(-dt, h->R0, A[st], h->Ut, h->Ut, n, s));
+        BL(dgx::blas_axpy(B[st], h->Ut, h->U, h->U, n, s));
+      }
+    } else {
+      return fail(DGX_ERR_ARGUMENT, "unsupported inttype " + std::to_string(inttype));
+    }
+  }
+#undef RUN
+#undef BL
+  return DGX_OK;
+}
+
+int dgx_synchronize(dgx_handle h) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  int flag = 0;
+  HIPCHK(hipMemcpy(&flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) {
+    HIPCHK(hipMemset(h->d_flag, 0, sizeof(int)));
+    return fail(DGX_ERR_NONREALIZABLE, "density or pressure below aepsM in a residual evaluation");
+  }
+  return DGX_OK;
+}
+
+long long dgx_stage_count(dgx_handle h) { return h ? h->stages : -1; }
+
+int dgx_norms(dgx_handle h, double *out) {
+  if (!h || !out) return fail(DGX_ERR_ARGUMENT, "null argument");
+  std::vector<double> Un((size_t)h->nf * h->ne * h->nq);
+  int rc = dgx_state_download(h, Un.data(), DGX_NODAL, 0);
+  if (rc) return rc;
+  const int Q = h->Q, nq = h->nq, nsd = h->nsd;
+  const std::vector<double> &w = h->hb.w, &D = h->hb.D;
+  std::vector<double> fr(nq), fs(nq), ft(nq);
+  for (int v = 0; v < h->nf; ++v) {
+    double Linf = 0, L2 = 0, H1 = 0, area = 0;
+    for (int e = 0; e < h->ne; ++e) {
+      const double *u = &Un[((size_t)v * h->ne + e) * nq];
+      const double *mt = &h->metric[(size_t)e * 9];
+      std::fill(fr.begin(), fr.end(), 0.0); std::fill(fs.begin(), fs.end(), 0.0); std::fill(ft.begin(), ft.end(), 0.0);
+      if (nsd == 2) {
+        for (int i = 0; i < Q; ++i)
+          for (int j = 0; j < Q; ++j)
+            for (int k = 0; k < Q; ++k) {
+              fr[i * Q + j] += D[i * Q + k] * u[k * Q + j];
+              fs[i * Q + j] += D[j * Q + k] * u[i * Q + k];
+            }
+      } else {
+        for (int i = 0; i < Q; ++i)
+          for (int j = 0; j < Q; ++j)
+            for (int k = 0; k < Q; ++k)
+              for (int m = 0; m < Q; ++m) {
+                fr[(i * Q + j) * Q + k] += D[i * Q + m] * u[(m * Q + j) * Q + k];
+                fs[(i * Q + j) * Q + k] += D[j * Q + m] * u[(i * Q + m) * Q + k];
+                ft[(i * Q + j) * Q + k] += D[k * Q + m] * u[(i * Q + j) * Q + m];
+              }
+      }
+      for (int n = 0; n < nq; ++n) {
+        double wj = h->J[e];
+        if (nsd == 2) wj *= w[n / Q] * w[n % Q];
+        else wj *= w[n / (Q * Q)] * w[(n / Q) % Q] * w[n % Q];
+        const double gx = mt[0] * fr[n] + mt[3] * fs[n] + mt[6] * ft[n];
+        const double gy = mt[1] * fr[n] + mt[4] * fs[n] + mt[7] * ft[n];
+        const double gz = mt[2] * fr[n] + mt[5] * fs[n] + mt[8] * ft[n];
+        Linf = std::max(Linf, std::fabs(u[n]));
+        L2 += wj * u[n] * u[n];
+        H1 += wj * (u[n] * u[n] + gx * gx + gy * gy + gz * gz);
+        area += wj;
+      }
+    }
+    out[3 * v] = Linf;
+    out[3 * v + 1] = std::sqrt(std::fabs(L2 / area));
+    out[3 * v + 2] = std::sqrt(std::fabs(H1 / area));
+  }
+  return DGX_OK;
+}
+
+}  // extern "C"

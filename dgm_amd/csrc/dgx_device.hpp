@@ -1,0 +1,150 @@
+// Device-side building blocks of the DG residual: LDS tensor contractions
+// (sum factorization), pointwise Euler/Navier-Stokes fluxes and the
+// Lax-Friedrichs / Roe numerical fluxes.  Everything here is FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dgx {
+
+// Basis tables of one polynomial order (device memory, row-major).
+//   BT  [Q][L]  B_l(xi_i)                 modal -> GLL nodes
+//   BW  [L][Q]  B_l(xi_i) w_i             GLL inner product
+//   QD  [L][Q]  sum_i B_l(xi_i) w_i D_ik  inner product of a collocation derivative
+//   Bg  [QF][L] B_l(eta_a)                modal -> Gauss face points
+//   Bend[2][L]  B_l(-1), B_l(+1)
+//   PW  [L][QF] wg_a B_l(eta_a)           Gauss face inner product
+//   Ig  [QF][Q] GLL -> Gauss interpolation
+struct Tab {
+  const double *BT, *BW, *QD, *Bg, *Bend, *PW, *Ig;
+};
+
+// Per-element affine geometry and face links (device memory).
+struct Geo {
+  const double *metric;          // [ne][9]: rx ry rz sx sy sz tx ty tz
+  const double *fnorm;           // [ne][nsides][3] outward unit normal
+  const double *fjr;             // [ne][nsides] J_side / J_vol
+  const int *nbr;                // [ne][nsides] neighbour element
+  const int *nside;              // [ne][nsides] neighbour side
+  const uint8_t *permid;         // [ne][nsides] row of `perm`
+  const uint16_t *perm;          // [nperm][QF^(d-1)] own point -> neighbour point
+};
+
+struct Phys {
+  double gamma, vis, Pr;
+  int flux;                      // 0 Lax-Friedrichs, 1 Roe (2D)
+};
+
+// Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last
+// index fastest) with the NO x NI matrix M; output has NO in place of NI.
+// One thread owns one pencil: it reads its NI inputs once and writes NO
+// outputs (register blocking).  ACC adds into `out`.
+template <int N0, int N1, int N2, int DIMC, int NO, int NT, bool ACC = false>
+__device__ __forceinline__ void contract(const double *__restrict__ M, const double *in,
+                                         double *out, int tid) {
+  constexpr int NI = DIMC == 0 ? N0 : (DIMC == 1 ? N1 : N2);
+  constexpr int NP = (N0 * N1 * N2) / NI;
+  constexpr int O1 = DIMC == 1 ? NO : N1;
+  constexpr int O2 = DIMC == 2 ? NO : N2;
+  constexpr int SI = DIMC == 0 ? N1 * N2 : (DIMC == 1 ? N2 : 1);
+  constexpr int SO = DIMC == 0 ? O1 * O2 : (DIMC == 1 ? O2 : 1);
+  for (int pc = tid; pc < NP; pc += NT) {
+    int i0, i1, i2;
+    if (DIMC == 0) { i0 = 0; i1 = pc / N2; i2 = pc - i1 * N2; }
+    else if (DIMC == 1) { i0 = pc / N2; i1 = 0; i2 = pc - i0 * N2; }
+    else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
+    const int bi = (i0 * N1 + i1) * N2 + i2;
+    const int bo = (i0 * O1 + i1) * O2 + i2;
+    double x[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
+      if (ACC) out[bo + o * SO] += s;
+      else out[bo + o * SO] = s;
+    }
+  }
+}
+
+// Copy n doubles global -> LDS (or LDS -> global) with the workgroup.
+template <int NT>
+__device__ __forceinline__ void copy_block(const double *__restrict__ src, double *dst, int n,
+                                           int tid) {
+  for (int i = tid; i < n; i += NT) dst[i] = src[i];
+}
+
+// aepsM = eps^0.66 (src/Constants.hpp:43): realizability threshold.
+constexpr double kAepsM = 4.6012e-11;
+
+// Physical normal flux F(U).n of compressible Euler (3D), pressure and sound speed.
+__device__ __forceinline__ void euler_normal_flux3(const double U[5], double nx, double ny,
+                                                   double nz, double gm1, double F[5],
+                                                   double &lam, bool &bad) {
+  const double ir = 1.0 / U[0];
+  const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  const double p = gm1 * (U[4] - 0.5 * (U[1] * u + U[2] * v + U[3] * w));
+  const double un = u * nx + v * ny + w * nz;
+  F[0] = U[0] * un;
+  F[1] = U[1] * un + p * nx;
+  F[2] = U[2] * un + p * ny;
+  F[3] = U[3] * un + p * nz;
+  F[4] = un * (U[4] + p);
+  const double c = sqrt((gm1 + 1.0) * p * ir);
+  lam = fabs(un) + c;
+  bad = !(U[0] > kAepsM) || !(p > kAepsM);
+}
+
+__device__ __forceinline__ void euler_normal_flux2(const double U[4], double nx, double ny,
+                                                   double gm1, double F[4], double &lam,
+                                                   bool &bad) {
+  const double ir = 1.0 / U[0];
+  const double u = U[1] * ir, v = U[2] * ir;
+  const double p = gm1 * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+  const double un = u * nx + v * ny;
+  F[0] = U[0] * un;
+  F[1] = U[1] * un + p * nx;
+  F[2] = U[2] * un + p * ny;
+  F[3] = un * (U[3] + p);
+  const double c = sqrt((gm1 + 1.0) * p * ir);
+  lam = fabs(un) + c;
+  bad = !(U[0] > kAepsM) || !(p > kAepsM);
+}
+
+// 2D Roe flux without entropy fix in the face frame (src/Euler.cpp:682-741),
+// returns the full numerical flux in Cartesian components.
+__device__ __forceinline__ void roe_flux2(const double UL[4], const double UR[4], double nx,
+                                          double ny, double gamma, const double FL[4],
+                                          double Fh[4]) {
+  const double gm1 = gamma - 1.0;
+  const double rl = UL[0], rul = UL[1] * nx + UL[2] * ny, rvl = UL[2] * nx - UL[1] * ny, rEl = UL[3];
+  const double rr = UR[0], rur = UR[1] * nx + UR[2] * ny, rvr = UR[2] * nx - UR[1] * ny, rEr = UR[3];
+  const double ul = rul / rl, vl = rvl / rl, ur = rur / rr, vr = rvr / rr;
+  const double pl = gm1 * (rEl - 0.5 * (rul * ul + rvl * vl));
+  const double pr = gm1 * (rEr - 0.5 * (rur * ur + rvr * vr));
+  const double Hl = (rEl + pl) / rl, Hr = (rEr + pr) / rr;
+  const double R = sqrt(rr / rl), iRp1 = 1.0 / (1.0 + R);
+  const double u = (ul + R * ur) * iRp1, v = (vl + R * vr) * iRp1, H = (Hl + R * Hr) * iRp1;
+  const double a = sqrt(gm1 * (H - 0.5 * (u * u + v * v))), ainv = 1.0 / a;
+  const double c1inv = a * a / gm1;
+  double al[4], lm[4], G[4];
+  al[2] = (rvr - rvl) - v * (rr - rl);
+  al[1] = gm1 * ainv * ainv * ((rr - rl) * (H - u * u) + u * (rur - rul) -
+                               ((rEr - rEl) - ((rvr - rvl) - v * (rr - rl)) * v));
+  al[0] = 0.5 * ainv * ((rr - rl) * (u + a) - (rur - rul) - a * al[1]);
+  al[3] = (rr - rl) - (al[0] + al[1]);
+  lm[0] = fmin(u - a, 0.0); lm[1] = fmin(u, 0.0); lm[2] = lm[1]; lm[3] = fmin(u + a, 0.0);
+  G[0] = lm[0] * al[0] + lm[1] * al[1] + lm[3] * al[3];
+  G[1] = lm[0] * (u - a) * al[0] + lm[1] * u * al[1] + lm[3] * (u + a) * al[3];
+  G[2] = lm[0] * v * al[0] + lm[1] * v * al[1] + lm[2] * al[2] + lm[3] * v * al[3];
+  G[3] = lm[0] * (H - u * a) * al[0] + lm[1] * (H - c1inv) * al[1] + lm[2] * v * al[2] +
+         lm[3] * (H + u * a) * al[3];
+  Fh[0] = FL[0] + G[0];
+  Fh[1] = FL[1] + G[1] * nx - G[2] * ny;
+  Fh[2] = FL[2] + G[1] * ny + G[2] * nx;
+  Fh[3] = FL[3] + G[3];
+}
+
+}  // namespace dgx

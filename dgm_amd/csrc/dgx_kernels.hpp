@@ -1,0 +1,42 @@
+// Launch interface between the host engine (dgx_api.cpp) and the templated
+// kernels (dgx_kernels2d.hip, dgx_kernels3d.hip).
+#pragma once
+#include "dgx_device.hpp"
+
+namespace dgx {
+
+enum Op {
+  OP_TRACES = 0,    // modal U -> face traces T
+  OP_RHS = 1,       // U, T [, V, TV] -> R = -dU/dt
+  OP_GRAD = 2,      // U, T -> lifted gradients G          (Navier-Stokes)
+  OP_VISC = 3,      // U, G -> viscous volume V, traces TV  (Navier-Stokes)
+  OP_FORWARD = 4,   // nodal X -> modal Y
+  OP_BACKWARD = 5   // modal X -> nodal Y
+};
+
+struct LaunchArgs {
+  int op = 0, p = 0, ne = 0, nfield = 0;
+  bool ns = false;
+  const double *U = nullptr;
+  double *T = nullptr, *R = nullptr, *G = nullptr, *V = nullptr, *TV = nullptr;
+  const double *X = nullptr;
+  double *Y = nullptr;
+  int *flag = nullptr;
+  Tab tab;
+  Geo geo;
+  Phys phys;
+  hipStream_t stream = nullptr;
+};
+
+hipError_t launch_2d(const LaunchArgs &a);
+hipError_t launch_3d(const LaunchArgs &a);
+int lds_bytes2(int p, int which);
+int lds_bytes3(int p, int which);
+
+// Flat-array stage updates (dgx_blas.hip)
+hipError_t blas_axpy(double a, const double *x, const double *y, double *z, size_t n, hipStream_t s);
+hipError_t blas_axpby(double a, const double *x, double b, const double *y, double *z, size_t n,
+                      hipStream_t s);
+hipError_t blas_scale(double a, double *x, size_t n, hipStream_t s);
+
+}  // namespace dgx

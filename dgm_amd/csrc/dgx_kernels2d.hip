@@ -1,0 +1,382 @@
+// Quad (2D) kernels of the DG residual; same structure as the hex kernels:
+// one workgroup per element, LDS tensors, register-blocked pencil contractions.
+//   face traces      Quad::fill_sides                 src/Quad.cpp:845-869
+//   volume term      Euler::convection                src/Euler.cpp:296-390
+//   numerical flux   Euler::convective_flux (LF, Roe) src/Euler.cpp:399-757
+//   lift + project   Quad::lift, Element::project     src/Quad.cpp:1971-2038, src/Element.hpp:427-437
+//   BR1 viscous      Navier_Stokes::diffusion         src/Navier_Stokes.cpp:113-258
+#include "dgx_kernels.hpp"
+
+namespace dgx {
+
+template <int P> struct H2 {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
+  static constexpr int L2 = L * L, Q2 = Q * Q;
+  static constexpr int NT = 64;
+};
+
+template <int P>
+__device__ __forceinline__ void backward2(const Tab &tb, const double *m, double *s1, double *out,
+                                          int tid) {
+  constexpr int L = H2<P>::L, Q = H2<P>::Q, NT = H2<P>::NT;
+  contract<1, L, L, 1, Q, NT>(tb.BT, m, s1, tid);
+  __syncthreads();
+  contract<1, Q, L, 2, Q, NT>(tb.BT, s1, out, tid);
+  __syncthreads();
+}
+
+// Edge contributions at modal (l,m): edges 0,2 normal to b, edges 1,3 normal to a.
+template <int P>
+__device__ __forceinline__ double face_add2(const double *__restrict__ Bend, const double *H, int l,
+                                            int m, const double c[4]) {
+  constexpr int L = H2<P>::L;
+  const double *B0 = Bend, *B1 = Bend + L;
+  double s = 0.0;
+  s = fma(c[0] * B0[m], H[0 * L + l], s);
+  s = fma(c[2] * B1[m], H[2 * L + l], s);
+  s = fma(c[1] * B1[l], H[1 * L + m], s);
+  s = fma(c[3] * B0[l], H[3 * L + m], s);
+  return s;
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, int nf) {
+  using S = H2<P>;
+  constexpr int L = S::L, QF = S::QF, L2 = S::L2, NT = S::NT;
+  __shared__ double A[L2], Kb[2 * L], Ka[2 * L], O[2 * QF];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  double *Te = T + (size_t)e * 4 * nf * QF;
+  for (int v = 0; v < nf; ++v) {
+    copy_block<NT>(U + ((size_t)v * ne + e) * L2, A, L2, tid);
+    __syncthreads();
+    contract<1, L, L, 2, 2, NT>(tb.Bend, A, Kb, tid);   // [L][2]
+    contract<1, L, L, 1, 2, NT>(tb.Bend, A, Ka, tid);   // [2][L]
+    __syncthreads();
+    contract<1, L, 2, 1, QF, NT>(tb.Bg, Kb, O, tid);    // [QF][2]: edges 0 / 2
+    __syncthreads();
+    for (int i = tid; i < 2 * QF; i += NT) {
+      const int a = i >> 1, end = i & 1;
+      Te[((end ? 2 : 0) * nf + v) * QF + a] = O[i];
+    }
+    __syncthreads();
+    contract<1, 2, L, 2, QF, NT>(tb.Bg, Ka, O, tid);    // [2][QF]: edges 3 / 1
+    __syncthreads();
+    for (int i = tid; i < 2 * QF; i += NT) {
+      const int end = i / QF, b = i - end * QF;
+      Te[((end ? 1 : 3) * nf + v) * QF + b] = O[i];
+    }
+    __syncthreads();
+  }
+}
+
+template <int P, bool NS>
+__global__ void __launch_bounds__(64)
+k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double *__restrict__ V,
+       const double *__restrict__ TV, double *__restrict__ R, Tab tb, Geo geo, Phys ph, int ne,
+       int *__restrict__ flag) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double Un[4 * Q2], gF[16 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const double gm1 = ph.gamma - 1.0;
+  for (int v = 0; v < 4; ++v) {
+    copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
+    __syncthreads();
+    backward2<P>(tb, Rv, X1, Un + v * Q2, tid);
+  }
+  bool bad = false;
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, a = i - s * QF;
+    const int es = e * 4 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int an = geo.perm[geo.permid[es] * QF + a];
+    const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1];
+    const double jr = geo.fjr[es];
+    double UL[4], UR[4], FL[4], FR[4], Fh[4], lL, lR;
+    const double *tl = T + ((size_t)e * 4 + s) * 4 * QF;
+    const double *tr = T + ((size_t)nb * 4 + ns) * 4 * QF;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { UL[v] = tl[v * QF + a]; UR[v] = tr[v * QF + an]; }
+    bool bl, br;
+    euler_normal_flux2(UL, nx, ny, gm1, FL, lL, bl);
+    euler_normal_flux2(UR, nx, ny, gm1, FR, lR, br);
+    bad |= bl | br;
+    if (ph.flux == 1) {
+      roe_flux2(UL, UR, nx, ny, ph.gamma, FL, Fh);
+    } else {
+      const double lmax = fmax(lL, lR);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double g = jr * (Fh[v] - FL[v]);
+      if (NS && v > 0) {
+        const double *vl = TV + ((size_t)e * 4 + s) * 3 * QF;
+        const double *vr = TV + ((size_t)nb * 4 + ns) * 3 * QF;
+        g += 0.5 * jr * (vl[(v - 1) * QF + a] + vr[(v - 1) * QF + an]);
+      }
+      gF[(v * 4 + s) * QF + a] = g;
+    }
+  }
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  const double *mt = geo.metric + 9 * (size_t)e;
+  const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int v = 0; v < 4; ++v) {
+    for (int k = tid; k < Q2; k += NT) {
+      const double r = Un[k], ru = Un[Q2 + k], rv = Un[2 * Q2 + k], rE = Un[3 * Q2 + k];
+      const double ir = 1.0 / r, u = ru * ir, vv = rv * ir;
+      const double p = gm1 * (rE - 0.5 * (ru * u + rv * vv));
+      double fx, fy;
+      if (v == 0) { fx = ru; fy = rv; }
+      else if (v == 1) { fx = ru * u + p; fy = ru * vv; }
+      else if (v == 2) { fx = rv * u; fy = rv * vv + p; }
+      else { const double h = rE + p; fx = u * h; fy = vv * h; }
+      Fr[k] = rx * fx + ry * fy;
+      Fs[k] = sx * fx + sy * fy;
+    }
+    __syncthreads();
+    contract<1, Q, Q, 2, L, NT>(tb.BW, Fr, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(tb.QD, Fs, X2, tid);
+    contract<1, 4, QF, 2, L, NT>(tb.PW, gF + v * 4 * QF, H, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT>(tb.QD, X1, Rv, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT, true>(tb.BW, X2, Rv, tid);
+    __syncthreads();
+    for (int k = tid; k < L2; k += NT) {
+      const int l = k / L, m = k - l * L;
+      double r = Rv[k] + face_add2<P>(tb.Bend, H, l, m, cw);
+      if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
+      R[((size_t)v * ne + e) * L2 + k] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// lifted, projected gradients of (u, v, e): G[e][2*prim + dir][L2]
+template <int P>
+__global__ void __launch_bounds__(64)
+k_grad2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ G, Tab tb,
+        Geo geo, int ne) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double Un[4 * Q2], Jq[12 * QF], Fb[Q2], Xa[Q * L], Xc[Q * L], D2[2 * L2], H[4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int v = 0; v < 4; ++v) {
+    copy_block<NT>(U + ((size_t)v * ne + e) * L2, D2, L2, tid);
+    __syncthreads();
+    backward2<P>(tb, D2, Xa, Un + v * Q2, tid);
+  }
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, a = i - s * QF;
+    const int es = e * 4 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int an = geo.perm[geo.permid[es] * QF + a];
+    const double jr = geo.fjr[es];
+    const double *tl = T + ((size_t)e * 4 + s) * 4 * QF;
+    const double *tr = T + ((size_t)nb * 4 + ns) * 4 * QF;
+    double pl[3], pr[3];
+    {
+      const double r = tl[a], u = tl[QF + a] / r, v = tl[2 * QF + a] / r;
+      double en = tl[3 * QF + a] / r;
+      double wk = u * u; wk = v * v + wk;
+      pl[0] = u; pl[1] = v; pl[2] = -0.5 * wk + en;
+    }
+    {
+      const double r = tr[an], u = tr[QF + an] / r, v = tr[2 * QF + an] / r;
+      double en = tr[3 * QF + an] / r;
+      double wk = u * u; wk = v * v + wk;
+      pr[0] = u; pr[1] = v; pr[2] = -0.5 * wk + en;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Jq[(q * 4 + s) * QF + a] = jr * (0.5 * (pl[q] + pr[q]) - pl[q]);
+  }
+  __syncthreads();
+  const double *mt = geo.metric + 9 * (size_t)e;
+  for (int q = 0; q < 3; ++q) {
+    for (int k = tid; k < Q2; k += NT) {
+      const double r = Un[k], u = Un[Q2 + k] / r, v = Un[2 * Q2 + k] / r;
+      double val;
+      if (q == 0) val = u;
+      else if (q == 1) val = v;
+      else {
+        double en = Un[3 * Q2 + k] / r;
+        double wk = u * u; wk = v * v + wk;
+        val = -0.5 * wk + en;
+      }
+      Fb[k] = val;
+    }
+    __syncthreads();
+    contract<1, Q, Q, 2, L, NT>(tb.BW, Fb, Xa, tid);
+    contract<1, Q, Q, 2, L, NT>(tb.QD, Fb, Xc, tid);
+    contract<1, 4, QF, 2, L, NT>(tb.PW, Jq + q * 4 * QF, H, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT>(tb.QD, Xa, D2, tid);
+    contract<1, Q, L, 1, L, NT>(tb.BW, Xc, D2 + L2, tid);
+    __syncthreads();
+    for (int k = tid; k < L2; k += NT) {
+      const int l = k / L, m = k - l * L;
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        double c[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) c[s] = geo.fnorm[3 * (e * 4 + s) + d];
+        G[((size_t)e * 6 + 2 * q + d) * L2 + k] =
+            mt[d] * D2[k] + mt[3 + d] * D2[L2 + k] + face_add2<P>(tb.Bend, H, l, m, c);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_visc2(const double *__restrict__ U, const double *__restrict__ G, double *__restrict__ V,
+        double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double Un[4 * Q2], Gn[6 * Q2], M0[L2], S1[Q * L], Lay[12 * Q], Wf[12 * QF], X1[Q * L],
+      X2[Q * L], Vo[L2];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int v = 0; v < 10; ++v) {
+    const double *src = v < 4 ? U + ((size_t)v * ne + e) * L2 : G + ((size_t)e * 6 + (v - 4)) * L2;
+    double *dst = v < 4 ? Un + v * Q2 : Gn + (v - 4) * Q2;
+    copy_block<NT>(src, M0, L2, tid);
+    __syncthreads();
+    backward2<P>(tb, M0, S1, dst, tid);
+  }
+  const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
+  auto vflux = [&](int k, double out[3][2]) {
+    const double r = Un[k], u = Un[Q2 + k] / r, v = Un[2 * Q2 + k] / r;
+    const double ux = Gn[k], uy = Gn[Q2 + k], vx = Gn[2 * Q2 + k], vy = Gn[3 * Q2 + k];
+    const double ex = Gn[4 * Q2 + k], ey = Gn[5 * Q2 + k];
+    const double t12 = mu * (uy + vx);
+    const double t11 = mu * ((2.0 / 3.0) * (2.0 * ux - vy));
+    const double t22 = mu * ((2.0 / 3.0) * (2.0 * vy - ux));
+    out[0][0] = t11; out[0][1] = t12;
+    out[1][0] = t12; out[1][1] = t22;
+    out[2][0] = t11 * u + t12 * v + kap * ex;
+    out[2][1] = t12 * u + t22 * v + kap * ey;
+  };
+  for (int i = tid; i < 4 * Q; i += NT) {
+    const int s = i / Q, j = i - s * Q;
+    int k;
+    switch (s) {
+      case 0: k = j * Q; break;
+      case 1: k = (Q - 1) * Q + j; break;
+      case 2: k = j * Q + Q - 1; break;
+      default: k = j; break;
+    }
+    const double *nn = geo.fnorm + 3 * (e * 4 + s);
+    double f[3][2];
+    vflux(k, f);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Lay[(s * 3 + q) * Q + j] = f[q][0] * nn[0] + f[q][1] * nn[1];
+  }
+  __syncthreads();
+  contract<1, 12, Q, 2, QF, NT>(tb.Ig, Lay, Wf, tid);
+  __syncthreads();
+  copy_block<NT>(Wf, TV + (size_t)e * 12 * QF, 12 * QF, tid);
+  const double *mt = geo.metric + 9 * (size_t)e;
+  for (int k = tid; k < Q2; k += NT) {
+    double f[3][2];
+    vflux(k, f);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Gn[(2 * q) * Q2 + k] = mt[0] * f[q][0] + mt[1] * f[q][1];
+      Gn[(2 * q + 1) * Q2 + k] = mt[3] * f[q][0] + mt[4] * f[q][1];
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < 3; ++q) {
+    contract<1, Q, Q, 2, L, NT>(tb.BW, Gn + 2 * q * Q2, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(tb.QD, Gn + (2 * q + 1) * Q2, X2, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT>(tb.QD, X1, Vo, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT, true>(tb.BW, X2, Vo, tid);
+    __syncthreads();
+    copy_block<NT>(Vo, V + ((size_t)e * 3 + q) * L2, L2, tid);
+    __syncthreads();
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_forward2(const double *__restrict__ Fn, double *__restrict__ Fm, Tab tb, int ne) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double A[Q2], X[Q * L], O[L2];
+  const int e = blockIdx.x % ne, v = blockIdx.x / ne, tid = threadIdx.x;
+  copy_block<NT>(Fn + ((size_t)v * ne + e) * Q2, A, Q2, tid);
+  __syncthreads();
+  contract<1, Q, Q, 2, L, NT>(tb.BW, A, X, tid);
+  __syncthreads();
+  contract<1, Q, L, 1, L, NT>(tb.BW, X, O, tid);
+  __syncthreads();
+  copy_block<NT>(O, Fm + ((size_t)v * ne + e) * L2, L2, tid);
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_backward2(const double *__restrict__ Fm, double *__restrict__ Fn, Tab tb, int ne) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double A[L2], X[Q * L], O[Q2];
+  const int e = blockIdx.x % ne, v = blockIdx.x / ne, tid = threadIdx.x;
+  copy_block<NT>(Fm + ((size_t)v * ne + e) * L2, A, L2, tid);
+  __syncthreads();
+  backward2<P>(tb, A, X, O, tid);
+  copy_block<NT>(O, Fn + ((size_t)v * ne + e) * Q2, Q2, tid);
+}
+
+template <int P>
+static hipError_t launch2(const LaunchArgs &a) {
+  const int ne = a.ne;
+  switch (a.op) {
+    case OP_TRACES:
+      hipLaunchKernelGGL(k_traces2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, 4);
+      break;
+    case OP_RHS:
+      if (a.ns)
+        hipLaunchKernelGGL((k_rhs2<P, true>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.V, a.TV,
+                           a.R, a.tab, a.geo, a.phys, ne, a.flag);
+      else
+        hipLaunchKernelGGL((k_rhs2<P, false>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.V, a.TV,
+                           a.R, a.tab, a.geo, a.phys, ne, a.flag);
+      break;
+    case OP_GRAD:
+      hipLaunchKernelGGL(k_grad2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.G, a.tab, a.geo, ne);
+      break;
+    case OP_VISC:
+      hipLaunchKernelGGL(k_visc2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.G, a.V, a.TV, a.tab,
+                         a.geo, a.phys, ne);
+      break;
+    case OP_FORWARD:
+      hipLaunchKernelGGL(k_forward2<P>, dim3(ne * a.nfield), dim3(64), 0, a.stream, a.X, a.Y, a.tab, ne);
+      break;
+    case OP_BACKWARD:
+      hipLaunchKernelGGL(k_backward2<P>, dim3(ne * a.nfield), dim3(64), 0, a.stream, a.X, a.Y, a.tab, ne);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int lds_bytes2(int p, int which) { return (p >= 1 && p <= 10) ? 0 : -1; }
+
+hipError_t launch_2d(const LaunchArgs &a) {
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch2<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9) DGX_CASE(10)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace dgx

@@ -1,0 +1,157 @@
+"""ctypes binding of libdgx.so (include/dgx.h): one DG domain on one GPU.
+
+The library is the product path; this module fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgx.so")
+
+DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D = 1, 2, 8, 9
+DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
+DGX_FP64, DGX_MIXED = 0, 1
+DGX_TVD, DGX_RK4, DGX_LSRK14 = 3, 4, 14
+DGX_NODAL, DGX_MODAL = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class DomainDesc(ctypes.Structure):
+    _fields_ = [("nsd", ctypes.c_int), ("ne", ctypes.c_int), ("p", ctypes.c_int),
+                ("eqntype", ctypes.c_int), ("flux", ctypes.c_int), ("precision", ctypes.c_int),
+                ("gamma", ctypes.c_double), ("vis", ctypes.c_double), ("Pr", ctypes.c_double),
+                ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip)]
+
+
+class DgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dgx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libdgx.so; raise if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    h = ctypes.c_void_p
+    sig = {
+        "dgx_domain_create": ([ctypes.POINTER(DomainDesc), ctypes.c_void_p, ctypes.POINTER(h)], ctypes.c_int),
+        "dgx_domain_destroy": ([h], ctypes.c_int),
+        "dgx_sizes": ([h, _ip, _ip, _ip, _ip], ctypes.c_int),
+        "dgx_coordinates": ([h, _dp], ctypes.c_int),
+        "dgx_state_upload": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "dgx_state_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "dgx_state_device_ptr": ([h, ctypes.POINTER(_dp)], ctypes.c_int),
+        "dgx_rhs": ([h], ctypes.c_int),
+        "dgx_residual_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "dgx_advance": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "dgx_synchronize": ([h], ctypes.c_int),
+        "dgx_norms": ([h, _dp], ctypes.c_int),
+        "dgx_stage_count": ([h], ctypes.c_longlong),
+        "dgx_last_error": ([], ctypes.c_char_p),
+        "dgx_version": ([], ctypes.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc != 0:
+        raise DgxError(rc, load().dgx_last_error().decode())
+
+
+class GPUDomain:
+    """A DG domain on the GPU: modal FP64 state, residual and RK stepping."""
+
+    def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
+                 precision=DGX_FP64, stream=None):
+        lib = load()
+        self.lib = lib
+        self.nsd = int(mesh.nsd)
+        self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
+        self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
+        desc = DomainDesc(self.nsd, int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
+                          float(gamma), float(vis), float(Pr), _ptr(self._vert), len(self._con),
+                          self._con.ctypes.data_as(_ip))
+        self.h = ctypes.c_void_p()
+        _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
+                                     ctypes.byref(self.h)))
+        q, m, f, ne = (ctypes.c_int() for _ in range(4))
+        _check(lib.dgx_sizes(self.h, ctypes.byref(q), ctypes.byref(m), ctypes.byref(f), ctypes.byref(ne)))
+        self.qtot, self.nm, self.nf, self.ne = q.value, m.value, f.value, ne.value
+        self.p = int(p)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.dgx_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def coords(self):
+        x = np.zeros((self.nsd, self.ne, self.qtot))
+        _check(self.lib.dgx_coordinates(self.h, _ptr(x)))
+        return x
+
+    def upload(self, U, layout=DGX_NODAL):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        _check(self.lib.dgx_state_upload(self.h, _ptr(U), layout, 0))
+
+    def download(self, layout=DGX_NODAL):
+        n = self.qtot if layout == DGX_NODAL else self.nm
+        U = np.zeros((self.nf, self.ne, n))
+        _check(self.lib.dgx_state_download(self.h, _ptr(U), layout, 0))
+        return U
+
+    def rhs(self, layout=DGX_NODAL):
+        _check(self.lib.dgx_rhs(self.h))
+        n = self.qtot if layout == DGX_NODAL else self.nm
+        F = np.zeros((self.nf, self.ne, n))
+        _check(self.lib.dgx_residual_download(self.h, _ptr(F), layout, 0))
+        self.synchronize()
+        return F
+
+    def advance(self, inttype, dt, nsteps):
+        _check(self.lib.dgx_advance(self.h, int(inttype), float(dt), int(nsteps)))
+
+    def synchronize(self):
+        _check(self.lib.dgx_synchronize(self.h))
+
+    def norms(self):
+        out = np.zeros((self.nf, 3))
+        _check(self.lib.dgx_norms(self.h, _ptr(out)))
+        return out
+
+    def device_state_ptr(self):
+        p = _dp()
+        _check(self.lib.dgx_state_device_ptr(self.h, ctypes.byref(p)))
+        return ctypes.cast(p, ctypes.c_void_p).value
+
+    @property
+    def stage_count(self):
+        return int(self.lib.dgx_stage_count(self.h))

@@ -1,0 +1,62 @@
+"""Periodic structured meshes in DGM's .msh/.con conventions.
+
+Element numbering is lexicographic (x fastest).  Vertex order follows the
+reference's quad (counter-clockwise from the lower-left corner) and hex
+(bottom face 0-3, top face 4-7) conventions (src/BaseQuad.cpp:20-27,
+src/Field.cpp:316-330); side numbering is that of Quad (0 bottom, 1 right,
+2 top, 3 left) and Hex (0 back z-, 1 y-, 2 x+, 3 y+, 4 x-, 5 front z+).
+Connectivity rows list every side of every element in element order, which
+makes the lower-numbered element of a pair the master (src/Field.cpp:3478-3556),
+as the reference's periodic generator does (src/Mesh.cpp:538-541, 879-884).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .formats import Mesh
+
+
+def periodic_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0) -> Mesh:
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    vert = np.zeros((nx * ny, 2, 4))
+    con = []
+    for j in range(ny):
+        for i in range(nx):
+            e = i + nx * j
+            vert[e, 0] = [xs[i], xs[i + 1], xs[i + 1], xs[i]]
+            vert[e, 1] = [ys[j], ys[j], ys[j + 1], ys[j + 1]]
+            nb = [(i + nx * ((j - 1) % ny), 2), ((i + 1) % nx + nx * j, 3),
+                  (i + nx * ((j + 1) % ny), 0), ((i - 1) % nx + nx * j, 1)]
+            for s, (le, ls) in enumerate(nb):
+                con.append((e, s, le, ls))
+    return Mesh(nsd=2, vert=vert, con=np.asarray(con, dtype=np.int32))
+
+
+def periodic_hexes(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> Mesh:
+    xs = np.linspace(lo[0], hi[0], nx + 1)
+    ys = np.linspace(lo[1], hi[1], ny + 1)
+    zs = np.linspace(lo[2], hi[2], nz + 1)
+    ne = nx * ny * nz
+    vert = np.zeros((ne, 3, 8))
+    con = np.zeros((ne * 6, 4), dtype=np.int32)
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ii, jj, kk = (a.transpose(2, 1, 0).ravel() for a in (ii, jj, kk))   # x fastest
+    e = ii + nx * (jj + ny * kk)
+    vx = np.stack([xs[ii], xs[ii + 1], xs[ii + 1], xs[ii]] * 2, axis=1)
+    vy = np.stack([ys[jj], ys[jj], ys[jj + 1], ys[jj + 1]] * 2, axis=1)
+    vz = np.stack([zs[kk]] * 4 + [zs[kk + 1]] * 4, axis=1)
+    vert[e, 0], vert[e, 1], vert[e, 2] = vx, vy, vz
+
+    def eid(i, j, k):
+        return (i % nx) + nx * ((j % ny) + ny * (k % nz))
+
+    nbrs = [(eid(ii, jj, kk - 1), 5), (eid(ii, jj - 1, kk), 3), (eid(ii + 1, jj, kk), 4),
+            (eid(ii, jj + 1, kk), 1), (eid(ii - 1, jj, kk), 2), (eid(ii, jj, kk + 1), 0)]
+    for s, (le, ls) in enumerate(nbrs):
+        rows = e * 6 + s
+        con[rows, 0] = e
+        con[rows, 1] = s
+        con[rows, 2] = le
+        con[rows, 3] = ls
+    return Mesh(nsd=3, vert=vert, con=con)

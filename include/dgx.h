@@ -1,0 +1,142 @@
+/* dgx.h -- C ABI of the MI355X (gfx950) explicit-DG residual and RK engine.
+ *
+ * The engine replaces the hot path of DGM's compressible Euler and
+ * Navier-Stokes domains: the residual Domain::time_derivative and the
+ * explicit stepper TimeInt::advance.  Every entry point below names the
+ * reference interface it stands in for (paths relative to the reference
+ * root).  Plain pointers and sizes only; no C++ or torch types cross this
+ * boundary.  All functions return DGX_OK (0) or a negative error code;
+ * dgx_last_error() gives the message of the calling thread's last failure.
+ *
+ * Conventions
+ *  - A "domain" owns the mesh tables, the FP64 conserved state U, the RK
+ *    registers and all scratch on one GPU, and one HIP stream.  Calls on
+ *    different domains are independent and may run concurrently; calls on
+ *    one domain must not overlap.
+ *  - The conserved state is kept in the reference's modal coefficients
+ *    (normalized Legendre, src/Basis.cpp:74-102), layout [field][elem][mode]
+ *    with modes ordered as in Quad::inner_product / Hex::inner_product
+ *    (src/Quad.cpp:632-662, src/Hex.cpp:703-747).  Nodal values at the
+ *    q = p+2 Gauss-Lobatto-Legendre points use layout [field][elem][node],
+ *    nodes ordered i*q+j (quad) or (i*q+j)*q+k (hex) (src/Quad.cpp:767-789,
+ *    src/Hex.cpp:817-866).
+ *  - Residual sign follows the reference: F = -dU/dt (src/Domain.hpp:237-243).
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGX_VERSION "0.1.0"
+
+enum dgx_status {
+  DGX_OK = 0,
+  DGX_ERR_ARGUMENT = -1,      /* bad argument or unsupported configuration */
+  DGX_ERR_DEVICE = -2,        /* HIP runtime failure */
+  DGX_ERR_NONREALIZABLE = -3, /* density or pressure fell below aepsM
+                                 (src/Euler.cpp:479-488) */
+  DGX_ERR_MEMORY = -4         /* allocation failed */
+};
+
+/* Equation types: the reference's eqntype numbers (src/Reo.hpp:33-61). */
+enum dgx_eqntype {
+  DGX_EULER = 1,            /* Reo::Euler (2D)            */
+  DGX_NAVIER_STOKES = 2,    /* Reo::Navier_Stokes (2D)    */
+  DGX_EULER3D = 8,          /* Reo::Euler3d               */
+  DGX_NAVIER_STOKES3D = 9   /* Reo::Navier_Stokes3d       */
+};
+
+/* Numerical flux (src/Euler.cpp:102-113). */
+enum dgx_flux { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE = 1 };
+
+/* Arithmetic of the residual. */
+enum dgx_precision { DGX_FP64 = 0, DGX_MIXED = 1 };
+
+/* Time integrators: the reference's inttype numbers
+ * (src/Domain.cpp:564-628): 3 TVD-RK3, 4 RK4, 14 LSRK14. */
+enum dgx_inttype { DGX_TVD = 3, DGX_RK4 = 4, DGX_LSRK14 = 14 };
+
+/* Node layout selector for state transfers. */
+enum dgx_layout { DGX_NODAL = 0, DGX_MODAL = 1 };
+
+typedef struct dgx_domain *dgx_handle;
+
+/* Mesh and physics description.  `vert` holds, per element, nsd rows of
+ * vertex coordinates exactly as in a root.msh file ([elem][dim][vertex],
+ * 4 vertices for quads, 8 for hexes).  `con` holds `ncon` rows
+ * (eid, sid, leid, lsid) in root.con order; the first row naming a side pair
+ * makes that side the master (src/Field.cpp:3478-3556). */
+typedef struct {
+  int nsd;          /* 2 or 3 */
+  int ne;           /* number of elements */
+  int p;            /* polynomial order (L = p+1 modes, q = p+2 GLL points) */
+  int eqntype;      /* enum dgx_eqntype */
+  int flux;         /* enum dgx_flux */
+  int precision;    /* enum dgx_precision */
+  double gamma;     /* ratio of specific heats (default 1.4) */
+  double vis;       /* dynamic viscosity mu (Navier-Stokes) */
+  double Pr;        /* Prandtl number (default 0.72) */
+  const double *vert;
+  int ncon;
+  const int *con;
+} dgx_domain_desc;
+
+/* Build a domain: geometry, side links and flips, basis tables, device
+ * buffers.  `stream` is a hipStream_t or NULL (the library creates one).
+ * Replaces the Field/Element construction of Reo::Euler::Euler /
+ * Euler3d::Euler3d (src/Euler.cpp:47-211, src/Euler3d.cpp:47-150). */
+int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out);
+int dgx_domain_destroy(dgx_handle h);
+
+/* Sizes: nodes and modes per element, fields, elements. */
+int dgx_sizes(dgx_handle h, int *qtot, int *nmodes, int *nfields, int *ne);
+
+/* Nodal coordinates [dim][elem][node] to host memory (Element::C,
+ * src/Quad.cpp:993-1013, src/Hex.cpp:991-1039). */
+int dgx_coordinates(dgx_handle h, double *xyz);
+
+/* State transfers.  Nodal uploads are L2-projected into the modal space, as
+ * the reference's first restart write does (src/vField.cpp:1102-1181).
+ * `on_device` = 1 if the pointer is device memory. */
+int dgx_state_upload(dgx_handle h, const double *U, int layout, int on_device);
+int dgx_state_download(dgx_handle h, double *U, int layout, int on_device);
+
+/* Device pointer of the modal state ([field][elem][mode], FP64). */
+int dgx_state_device_ptr(dgx_handle h, double **U);
+
+/* Residual F = -dU/dt of the current state into the domain's residual
+ * buffer (Domain::time_derivative, src/Domain.hpp:237-243;
+ * Euler::time_derivative src/Euler.cpp:282-287; Euler3d src/Euler3d.cpp:455;
+ * Navier_Stokes(3d)::time_derivative src/Navier_Stokes.cpp:89-94,
+ * src/Navier_Stokes3d.cpp:135-161). */
+int dgx_rhs(dgx_handle h);
+int dgx_residual_download(dgx_handle h, double *F, int layout, int on_device);
+
+/* Advance nsteps steps of size dt with integrator `inttype`
+ * (TimeInt::advance: RungeKutta src/TimeInt.cpp:267-302, TVD :824-856,
+ * LSRK14 :1989-2013).  Asynchronous on the domain's stream. */
+int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps);
+
+/* Wait for the domain's stream; returns DGX_ERR_NONREALIZABLE if any
+ * residual met a non-realizable state since the last check. */
+int dgx_synchronize(dgx_handle h);
+
+/* Per-field Linf, L2, H1 norms of the state, out[3*field + {0,1,2}]
+ * (Field::norm, src/Field.cpp:4675-4697; Quad::norm_L2/H1 src/Quad.cpp:2556-2632). */
+int dgx_norms(dgx_handle h, double *out);
+
+/* Number of RHS evaluations (stages) performed so far. */
+long long dgx_stage_count(dgx_handle h);
+
+const char *dgx_last_error(void);
+const char *dgx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGX_H */

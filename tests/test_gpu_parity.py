@@ -1,0 +1,119 @@
+"""Parity of the HIP residual and steppers (libdgx.so) with the CPU oracle.
+
+Bar: relative L2 per conserved field <= 1e-12 for one residual evaluation on
+the same projected state (FP64 path), and <= 1e-11 after several RK steps.
+"""
+import numpy as np
+import pytest
+
+from dgm_amd import dgx, ic, mesh as meshgen
+from fixture_runs import initial_state, load_case
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_RHS = 1e-12
+TOL_STEP = 1e-11
+
+
+def rel_l2(a, b):
+    """Per-field relative L2 of nodal arrays [field][elem][node]."""
+    num = np.sqrt(((a - b) ** 2).reshape(a.shape[0], -1).sum(1))
+    den = np.sqrt((b ** 2).reshape(b.shape[0], -1).sum(1))
+    return num / np.maximum(den, 1e-300)
+
+
+def projected_state(orc, U):
+    orc.set_state(U)
+    orc.project_state()
+    return orc.state()
+
+
+def perturbed(U, x, amp=0.05):
+    """Add a smooth O(amp) perturbation to every field (all residual fields nonzero)."""
+    out = U.copy()
+    for v in range(U.shape[0]):
+        ph = 0.3 * (v + 1)
+        w = np.sin(2.1 * x[0] + ph) * np.cos(1.7 * x[1] - ph)
+        if len(x) == 3:
+            w = w * np.cos(1.3 * x[2] + 0.5 * ph)
+        out[v] = out[v] + amp * (1.0 + np.abs(U[v]).max()) * w
+    return out
+
+
+def check_rhs(m, p, eqn, U_fn, vis=0.0, flux=0, tol=TOL_RHS):
+    orc = Oracle(m, p, eqn, vis=vis, flux=flux)
+    x = orc.coords()
+    Up = projected_state(orc, U_fn(x))
+    F_ref = orc.rhs(Up)
+    gpu = dgx.GPUDomain(m, p, eqn, vis=vis, flux=flux)
+    gx = gpu.coords()
+    assert np.allclose(gx, x, rtol=0, atol=1e-13)
+    gpu.upload(Up)
+    F = gpu.rhs()
+    err = rel_l2(F, F_ref)
+    print(f"eqn={eqn} p={p} ne={m.ne} rel L2 per field: {err}")
+    assert np.all(err <= tol), err
+    gpu.close()
+
+
+def test_rhs_vhex_euler3d_p10():
+    prm, m, icd = load_case("vhex")
+    check_rhs(m, 10, 8, lambda x: perturbed(initial_state(icd, x, 3), x))
+
+
+@pytest.mark.parametrize("p", [2, 4, 7])
+def test_rhs_taylor_green_euler3d(p):
+    m = meshgen.periodic_hexes(3, 3, 3, (0, 0, 0), (2 * np.pi,) * 3)
+    check_rhs(m, p, 8, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x))
+
+
+@pytest.mark.parametrize("p", [3, 7])
+def test_rhs_taylor_green_ns3d(p):
+    m = meshgen.periodic_hexes(3, 3, 2, (0, 0, 0), (2 * np.pi,) * 3)
+    check_rhs(m, p, 9, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), vis=1.0 / 100.0)
+
+
+def test_rhs_vquad_ns2d_p10():
+    prm, m, icd = load_case("vquad")
+    check_rhs(m, 10, 2, lambda x: perturbed(initial_state(icd, x, 2), x), vis=0.02)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_rhs_isentropic_vortex_euler2d(flux):
+    m = meshgen.periodic_quads(6, 5, 0, 10, 0, 10)
+    check_rhs(m, 4, 1, lambda x: ic.isentropic_vortex(x[0], x[1], 4.0, 6.0, period=(10, 10)),
+              flux=flux)
+
+
+def test_advance_vquad_tvd():
+    prm, m, icd = load_case("vquad")
+    orc = Oracle(m, 10, 2, vis=0.02)
+    x = orc.coords()
+    Up = projected_state(orc, initial_state(icd, x, 2))
+    gpu = dgx.GPUDomain(m, 10, 2, vis=0.02)
+    gpu.upload(Up)
+    orc.set_state(Up)
+    orc.advance(3, 1e-3, 20)
+    gpu.advance(dgx.DGX_TVD, 1e-3, 20)
+    gpu.synchronize()
+    err = rel_l2(gpu.download(), orc.state())
+    print("vquad TVD 20 steps", err)
+    assert np.all(err <= TOL_STEP), err
+
+
+@pytest.mark.parametrize("inttype", [4, 14])
+def test_advance_tgv_ns3d(inttype):
+    m = meshgen.periodic_hexes(2, 2, 2, (0, 0, 0), (2 * np.pi,) * 3)
+    orc = Oracle(m, 4, 9, vis=1.0 / 1600)
+    x = orc.coords()
+    Up = projected_state(orc, perturbed(ic.taylor_green(x[0], x[1], x[2]), x))
+    gpu = dgx.GPUDomain(m, 4, 9, vis=1.0 / 1600)
+    gpu.upload(Up)
+    orc.set_state(Up)
+    orc.advance(inttype, 2e-3, 3)
+    gpu.advance(inttype, 2e-3, 3)
+    gpu.synchronize()
+    err = rel_l2(gpu.download(), orc.state())
+    print(f"TGV NS3d inttype={inttype} 3 steps", err)
+    assert np.all(err <= TOL_STEP), err
