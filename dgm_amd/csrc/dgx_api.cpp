@@ -169,6 +169,10 @@ struct dgx_domain {
   dgx::Geo geo;
   dgx::Phys phys;
   long long stages = 0;
+  // per-kernel timing (dgx_profile): event pairs and the kernel kind of each
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_kind;
 
   size_t state_len() const { return (size_t)nf * ne * nm; }
 };
@@ -181,12 +185,27 @@ int alloc(double **p, size_t n) {
   return DGX_OK;
 }
 
+// Record a start event before and a stop event after `fn` when profiling.
+template <typename Fn>
+hipError_t timed(dgx_domain *d, int kind, Fn fn) {
+  if (!d->profiling) return fn();
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return hipErrorOutOfMemory;
+  (void)hipEventRecord(a, d->stream);
+  hipError_t e = fn();
+  (void)hipEventRecord(b, d->stream);
+  d->ev.push_back(a);
+  d->ev.push_back(b);
+  d->ev_kind.push_back(kind);
+  return e;
+}
+
 int launch(dgx_domain *d, int op, const double *U, double *out) {
   dgx::LaunchArgs a;
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf; a.ns = d->visc;
   a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
   a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
-  hipError_t e = (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
+  hipError_t e = timed(d, op, [&] { return (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   return DGX_OK;
 }
@@ -580,7 +599,7 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
   hipStream_t s = h->stream;
   int rc;
 #define RUN(x) do { if ((rc = (x))) return rc; } while (0)
-#define BL(x) do { hipError_t _e = (x); if (_e != hipSuccess) return fail(DGX_ERR_DEVICE, hipGetErrorString(_e)); } while (0)
+#define BL(x) do { hipError_t _e = timed(h, 6, [&] { return (x); }); if (_e != hipSuccess) return fail(DGX_ERR_DEVICE, hipGetErrorString(_e)); } while (0)
   for (int step = 0; step < nsteps; ++step) {
     if (inttype == DGX_RK4) {
       // RungeKutta::advance (src/TimeInt.cpp:267-302)
@@ -646,6 +665,28 @@ int dgx_synchronize(dgx_handle h) {
 }
 
 long long dgx_stage_count(dgx_handle h) { return h ? h->stages : -1; }
+
+int dgx_profile(dgx_handle h, int inttype, double dt, int nsteps, double *ms, int *count, int nkinds) {
+  if (!h || !ms || nkinds < 7) return fail(DGX_ERR_ARGUMENT, "dgx_profile needs 7 slots");
+  for (int k = 0; k < nkinds; ++k) { ms[k] = 0.0; if (count) count[k] = 0; }
+  h->profiling = true;
+  h->ev.clear();
+  h->ev_kind.clear();
+  int rc = dgx_advance(h, inttype, dt, nsteps);
+  h->profiling = false;
+  if (!rc) rc = (hipStreamSynchronize(h->stream) == hipSuccess) ? DGX_OK : fail(DGX_ERR_DEVICE, "sync");
+  for (size_t i = 0; i < h->ev_kind.size(); ++i) {
+    float t = 0.f;
+    if (!rc && hipEventElapsedTime(&t, h->ev[2 * i], h->ev[2 * i + 1]) == hipSuccess) {
+      ms[h->ev_kind[i]] += t;
+      if (count) count[h->ev_kind[i]] += 1;
+    }
+  }
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  h->ev.clear();
+  h->ev_kind.clear();
+  return rc;
+}
 
 int dgx_norms(dgx_handle h, double *out) {
   if (!h || !out) return fail(DGX_ERR_ARGUMENT, "null argument");

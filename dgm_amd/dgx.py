@@ -61,6 +61,8 @@ def load():
         "dgx_synchronize": ([h], ctypes.c_int),
         "dgx_norms": ([h, _dp], ctypes.c_int),
         "dgx_stage_count": ([h], ctypes.c_longlong),
+        "dgx_profile": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip, ctypes.c_int],
+                        ctypes.c_int),
         "dgx_last_error": ([], ctypes.c_char_p),
         "dgx_version": ([], ctypes.c_char_p),
     }

@@ -132,6 +132,15 @@ int dgx_norms(dgx_handle h, double *out);
 /* Number of RHS evaluations (stages) performed so far. */
 long long dgx_stage_count(dgx_handle h);
 
+/* Run dgx_advance with HIP events around every kernel on the domain's
+ * stream and return the summed milliseconds (`ms`) and launch counts per
+ * kernel kind: 0 face traces, 1 residual, 2 lifted gradients, 3 viscous
+ * fluxes, 4 nodal->modal, 5 modal->nodal, 6 stage updates.  `nkinds` >= 7.
+ * Plays the role of the reference's FunctionTimer instrumentation
+ * (src/Utilities.hpp:185-212) for the device path. */
+int dgx_profile(dgx_handle h, int inttype, double dt, int nsteps, double *ms, int *count,
+                int nkinds);
+
 const char *dgx_last_error(void);
 const char *dgx_version(void);
 
