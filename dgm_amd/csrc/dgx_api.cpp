@@ -140,6 +140,30 @@ HostBasis make_basis(int p) {
   return b;
 }
 
+// Basis tables of all orders, packed as the kernels' constant memory expects
+// (dgx_device.hpp: tab_offset), uploaded once per process.
+int upload_all_tables() {
+  static int status = 1;
+  if (status == 0) return DGX_OK;
+  std::vector<double> packed(dgx::kTabTotal, 0.0);
+  for (int p = 1; p <= dgx::kMaxOrder; ++p) {
+    HostBasis b = make_basis(p);
+    size_t off = dgx::tab_offset(p);
+    for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig}) {
+      std::copy(t->begin(), t->end(), packed.begin() + off);
+      off += t->size();
+    }
+    if (off != (size_t)dgx::tab_offset(p) + dgx::tab_size(p)) return fail(DGX_ERR_ARGUMENT, "table layout");
+  }
+  hipError_t e;
+  if ((e = dgx::upload_tables_2d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_3d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess)
+    return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
+  status = 0;
+  return DGX_OK;
+}
+
 // ---------------------------------------------------------------------------
 struct Side {
   bool master = false;
@@ -164,7 +188,8 @@ struct dgx_domain {
   uint8_t *d_permid = nullptr;
   uint16_t *d_perm = nullptr;
   double *U = nullptr, *Ut = nullptr, *R0 = nullptr, *R1 = nullptr, *T = nullptr;
-  double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr;
+  double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr, *T2 = nullptr;
+  bool fused = false;   // hex fused stage kernels available for this order
   dgx::Tab tab;
   dgx::Geo geo;
   dgx::Phys phys;
@@ -219,9 +244,38 @@ int transform(dgx_domain *d, int op, const double *x, double *y) {
   return DGX_OK;
 }
 
+dgx::StageLaunch stage_args(dgx_domain *d) {
+  dgx::StageLaunch s;
+  s.p = d->p; s.ne = d->ne; s.ns = d->visc; s.flag = d->d_flag;
+  s.tab = d->tab; s.geo = d->geo; s.phys = d->phys; s.stream = d->stream;
+  return s;
+}
+
+int launch_stage(dgx_domain *d, const dgx::StageLaunch &s) {
+  hipError_t e = timed(d, s.op == 0 ? dgx::OP_VISC : dgx::OP_RHS, [&] { return dgx::launch_stage3(s); });
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("stage launch: ") + hipGetErrorString(e));
+  return DGX_OK;
+}
+
+// BR1 viscous terms of state U from traces T into V, TV (fused hex path).
+int vterm(dgx_domain *d, const double *U, const double *T) {
+  dgx::StageLaunch s = stage_args(d);
+  s.op = 0; s.U = const_cast<double *>(U); s.T = T; s.Vw = d->V; s.TVw = d->TV;
+  return launch_stage(d, s);
+}
+
 // Residual of state `U` into `R` (Domain::time_derivative).
 int residual(dgx_domain *d, const double *U, double *R) {
   int rc;
+  if (d->fused) {
+    if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
+    if (d->visc && (rc = vterm(d, U, d->T))) return rc;
+    dgx::StageLaunch s = stage_args(d);
+    s.op = 1; s.U = const_cast<double *>(U); s.T = d->T; s.V = d->V; s.TV = d->TV; s.R = R;
+    if ((rc = launch_stage(d, s))) return rc;
+    ++d->stages;
+    return DGX_OK;
+  }
   if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
   if (d->visc) {
     if ((rc = launch(d, dgx::OP_GRAD, U, nullptr))) return rc;
@@ -430,6 +484,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     }
 
   // device buffers
+  int rc0 = upload_all_tables();
+  if (rc0) return rc0;
   const std::vector<const std::vector<double> *> tabs = {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig};
   size_t ntab = 0;
   for (auto t : tabs) ntab += t->size();
@@ -471,9 +527,11 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   HIPCHK(hipMemset(d->R0, 0, n * 8));
   HIPCHK(hipMemset(d->R1, 0, n * 8));
   if ((rc = alloc(&d->T, (size_t)ne * d->ns * d->nf * qs))) return rc;
+  d->fused = (nsd == 3 && dgx::stage3_supported(p));
+  if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
   if (d->visc) {
     const int ng = nsd == 2 ? 6 : 12;
-    if ((rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
+    if (!d->fused && (rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
     if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * d->nm))) return rc;
     if ((rc = alloc(&d->TV, (size_t)ne * d->ns * (d->nf - 1) * qs))) return rc;
   }
@@ -482,7 +540,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
-                  d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal};
+                  d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
+                  d->T2};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -638,6 +697,20 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
           0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
           0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
           0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
+      if (h->fused) {
+        // one fused kernel per stage: residual, register update, next traces
+        if (step == 0) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));
+        for (int st = 0; st < 14; ++st) {
+          if (h->visc) RUN(vterm(h, h->U, h->T));
+          dgx::StageLaunch sa = stage_args(h);
+          sa.op = 1; sa.update = true; sa.U = h->U; sa.Ut = h->Ut; sa.T = h->T; sa.Tn = h->T2;
+          sa.V = h->V; sa.TV = h->TV; sa.ca = A[st]; sa.cb = B[st]; sa.dt = dt;
+          RUN(launch_stage(h, sa));
+          std::swap(h->T, h->T2);
+          ++h->stages;
+        }
+        continue;
+      }
       for (int st = 0; st < 14; ++st) {
         RUN(residual(h, h->U, h->R0));
         BL(dgx::blas_axpby(-dt, h->R0, A[st], h->Ut, h->Ut, n, s));

@@ -19,6 +19,34 @@ struct Tab {
   const double *BT, *BW, *QD, *Bg, *Bend, *PW, *Ig;
 };
 
+// The same tables for every supported order p = 1..10 in constant memory, so
+// that uniform coefficient reads become scalar loads.  Layout per order:
+// BT, BW, QD, Bg, Bend, PW, Ig (sizes above), orders packed in ascending p.
+constexpr int kMaxOrder = 10;
+constexpr int tab_size(int p) {
+  return 3 * (p + 2) * (p + 1) + 2 * (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1) * (p + 2);
+}
+constexpr int tab_offset(int p) {
+  int s = 0;
+  for (int k = 1; k < p; ++k) s += tab_size(k);
+  return s;
+}
+constexpr int kTabTotal = tab_offset(kMaxOrder + 1);
+
+// Each translation unit holds its own copy, filled by its upload_tables_* ().
+static __constant__ double c_tab[kTabTotal];
+
+template <int P> struct CT {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, b = tab_offset(P);
+  __device__ static __forceinline__ const double *BT() { return c_tab + b; }
+  __device__ static __forceinline__ const double *BW() { return c_tab + b + Q * L; }
+  __device__ static __forceinline__ const double *QD() { return c_tab + b + 2 * Q * L; }
+  __device__ static __forceinline__ const double *Bg() { return c_tab + b + 3 * Q * L; }
+  __device__ static __forceinline__ const double *Bend() { return c_tab + b + 3 * Q * L + QF * L; }
+  __device__ static __forceinline__ const double *PW() { return c_tab + b + 3 * Q * L + QF * L + 2 * L; }
+  __device__ static __forceinline__ const double *Ig() { return c_tab + b + 3 * Q * L + 2 * QF * L + 2 * L; }
+};
+
 // Per-element affine geometry and face links (device memory).
 struct Geo {
   const double *metric;          // [ne][9]: rx ry rz sx sy sz tx ty tz
@@ -37,8 +65,10 @@ struct Phys {
 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last
 // index fastest) with the NO x NI matrix M; output has NO in place of NI.
-// One thread owns one pencil: it reads its NI inputs once and writes NO
-// outputs (register blocking).  ACC adds into `out`.
+// One thread owns one pencil at a time and computes OC outputs per pass from
+// its NI inputs (register blocking).  Passes over output chunks are outermost
+// so that only OC x NI coefficients are live in scalar registers at a time.
+// ACC adds into `out`.
 template <int N0, int N1, int N2, int DIMC, int NO, int NT, bool ACC = false>
 __device__ __forceinline__ void contract(const double *__restrict__ M, const double *in,
                                          double *out, int tid) {
@@ -48,25 +78,46 @@ __device__ __forceinline__ void contract(const double *__restrict__ M, const dou
   constexpr int O2 = DIMC == 2 ? NO : N2;
   constexpr int SI = DIMC == 0 ? N1 * N2 : (DIMC == 1 ? N2 : 1);
   constexpr int SO = DIMC == 0 ? O1 * O2 : (DIMC == 1 ? O2 : 1);
-  for (int pc = tid; pc < NP; pc += NT) {
-    int i0, i1, i2;
-    if (DIMC == 0) { i0 = 0; i1 = pc / N2; i2 = pc - i1 * N2; }
-    else if (DIMC == 1) { i0 = pc / N2; i1 = 0; i2 = pc - i0 * N2; }
-    else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
-    const int bi = (i0 * N1 + i1) * N2 + i2;
-    const int bo = (i0 * O1 + i1) * O2 + i2;
-    double x[NI];
+  constexpr int OC = NI * 3 <= 40 ? 3 : 2;    // coefficients live per pass <= ~40 doubles
 #pragma unroll
-    for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
+  for (int o0 = 0; o0 < NO; o0 += OC) {
+    for (int pc = tid; pc < NP; pc += NT) {
+      int i0, i1, i2;
+      if (DIMC == 0) { i0 = 0; i1 = pc / N2; i2 = pc - i1 * N2; }
+      else if (DIMC == 1) { i0 = pc / N2; i1 = 0; i2 = pc - i0 * N2; }
+      else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
+      const int bi = (i0 * N1 + i1) * N2 + i2;
+      const int bo = (i0 * O1 + i1) * O2 + i2;
+      double x[NI];
 #pragma unroll
-    for (int o = 0; o < NO; ++o) {
-      double s = 0.0;
+      for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
 #pragma unroll
-      for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
-      if (ACC) out[bo + o * SO] += s;
-      else out[bo + o * SO] = s;
+      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
+        if (ACC) out[bo + o * SO] += s;
+        else out[bo + o * SO] = s;
+      }
     }
   }
+}
+
+// Contribution of the six face blocks H[s][.][.] at modal index (l,m,n),
+// weighted per face by c[s].  Face normal dims: 4,2 -> a; 1,3 -> b; 0,5 -> c.
+template <int P>
+__device__ __forceinline__ double face_add3(const double *__restrict__ Bend, const double *H,
+                                            int l, int m, int n, const double c[6]) {
+  constexpr int L = P + 1, L2 = L * L;
+  const double *B0 = Bend, *B1 = Bend + L;
+  double s = 0.0;
+  s = fma(c[4] * B0[l], H[4 * L2 + m * L + n], s);
+  s = fma(c[2] * B1[l], H[2 * L2 + m * L + n], s);
+  s = fma(c[1] * B0[m], H[1 * L2 + l * L + n], s);
+  s = fma(c[3] * B1[m], H[3 * L2 + l * L + n], s);
+  s = fma(c[0] * B0[n], H[0 * L2 + l * L + m], s);
+  s = fma(c[5] * B1[n], H[5 * L2 + l * L + m], s);
+  return s;
 }
 
 // Copy n doubles global -> LDS (or LDS -> global) with the workgroup.

@@ -28,10 +28,32 @@ struct LaunchArgs {
   hipStream_t stream = nullptr;
 };
 
+// Fill each translation unit's constant-memory basis tables (kTabTotal doubles).
+hipError_t upload_tables_2d(const double *h);
+hipError_t upload_tables_3d(const double *h);
+hipError_t upload_tables_s3d(const double *h);
+
 hipError_t launch_2d(const LaunchArgs &a);
 hipError_t launch_3d(const LaunchArgs &a);
 int lds_bytes2(int p, int which);
 int lds_bytes3(int p, int which);
+
+// Fused hex stage kernels (dgx_stage3d.hip), p <= 9.
+struct StageLaunch {
+  int op = 1;                    // 0: BR1 viscous terms (k_vterm3), 1: stage (k_stage3)
+  int p = 0, ne = 0;
+  bool ns = false, update = false;
+  const double *T = nullptr, *V = nullptr, *TV = nullptr;
+  double *U = nullptr, *Ut = nullptr, *R = nullptr, *Tn = nullptr, *Vw = nullptr, *TVw = nullptr;
+  double ca = 0.0, cb = 0.0, dt = 0.0;
+  int *flag = nullptr;
+  Tab tab;
+  Geo geo;
+  Phys phys;
+  hipStream_t stream = nullptr;
+};
+bool stage3_supported(int p);
+hipError_t launch_stage3(const StageLaunch &s);
 
 // Flat-array stage updates (dgx_blas.hip)
 hipError_t blas_axpy(double a, const double *x, const double *y, double *z, size_t n, hipStream_t s);

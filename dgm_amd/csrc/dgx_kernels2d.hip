@@ -19,9 +19,9 @@ template <int P>
 __device__ __forceinline__ void backward2(const Tab &tb, const double *m, double *s1, double *out,
                                           int tid) {
   constexpr int L = H2<P>::L, Q = H2<P>::Q, NT = H2<P>::NT;
-  contract<1, L, L, 1, Q, NT>(tb.BT, m, s1, tid);
+  contract<1, L, L, 1, Q, NT>(CT<P>::BT(), m, s1, tid);
   __syncthreads();
-  contract<1, Q, L, 2, Q, NT>(tb.BT, s1, out, tid);
+  contract<1, Q, L, 2, Q, NT>(CT<P>::BT(), s1, out, tid);
   __syncthreads();
 }
 
@@ -50,17 +50,17 @@ k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
   for (int v = 0; v < nf; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L2, A, L2, tid);
     __syncthreads();
-    contract<1, L, L, 2, 2, NT>(tb.Bend, A, Kb, tid);   // [L][2]
-    contract<1, L, L, 1, 2, NT>(tb.Bend, A, Ka, tid);   // [2][L]
+    contract<1, L, L, 2, 2, NT>(CT<P>::Bend(), A, Kb, tid);   // [L][2]
+    contract<1, L, L, 1, 2, NT>(CT<P>::Bend(), A, Ka, tid);   // [2][L]
     __syncthreads();
-    contract<1, L, 2, 1, QF, NT>(tb.Bg, Kb, O, tid);    // [QF][2]: edges 0 / 2
+    contract<1, L, 2, 1, QF, NT>(CT<P>::Bg(), Kb, O, tid);    // [QF][2]: edges 0 / 2
     __syncthreads();
     for (int i = tid; i < 2 * QF; i += NT) {
       const int a = i >> 1, end = i & 1;
       Te[((end ? 2 : 0) * nf + v) * QF + a] = O[i];
     }
     __syncthreads();
-    contract<1, 2, L, 2, QF, NT>(tb.Bg, Ka, O, tid);    // [2][QF]: edges 3 / 1
+    contract<1, 2, L, 2, QF, NT>(CT<P>::Bg(), Ka, O, tid);    // [2][QF]: edges 3 / 1
     __syncthreads();
     for (int i = tid; i < 2 * QF; i += NT) {
       const int end = i / QF, b = i - end * QF;
@@ -139,17 +139,17 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
       Fs[k] = sx * fx + sy * fy;
     }
     __syncthreads();
-    contract<1, Q, Q, 2, L, NT>(tb.BW, Fr, X1, tid);
-    contract<1, Q, Q, 2, L, NT>(tb.QD, Fs, X2, tid);
-    contract<1, 4, QF, 2, L, NT>(tb.PW, gF + v * 4 * QF, H, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
+    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
     __syncthreads();
-    contract<1, Q, L, 1, L, NT>(tb.QD, X1, Rv, tid);
+    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
     __syncthreads();
-    contract<1, Q, L, 1, L, NT, true>(tb.BW, X2, Rv, tid);
+    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
     __syncthreads();
     for (int k = tid; k < L2; k += NT) {
       const int l = k / L, m = k - l * L;
-      double r = Rv[k] + face_add2<P>(tb.Bend, H, l, m, cw);
+      double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
       if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
       R[((size_t)v * ne + e) * L2 + k] = r;
     }
@@ -211,12 +211,12 @@ k_grad2(const double *__restrict__ U, const double *__restrict__ T, double *__re
       Fb[k] = val;
     }
     __syncthreads();
-    contract<1, Q, Q, 2, L, NT>(tb.BW, Fb, Xa, tid);
-    contract<1, Q, Q, 2, L, NT>(tb.QD, Fb, Xc, tid);
-    contract<1, 4, QF, 2, L, NT>(tb.PW, Jq + q * 4 * QF, H, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fb, Xa, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fb, Xc, tid);
+    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), Jq + q * 4 * QF, H, tid);
     __syncthreads();
-    contract<1, Q, L, 1, L, NT>(tb.QD, Xa, D2, tid);
-    contract<1, Q, L, 1, L, NT>(tb.BW, Xc, D2 + L2, tid);
+    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), Xa, D2, tid);
+    contract<1, Q, L, 1, L, NT>(CT<P>::BW(), Xc, D2 + L2, tid);
     __syncthreads();
     for (int k = tid; k < L2; k += NT) {
       const int l = k / L, m = k - l * L;
@@ -226,7 +226,7 @@ k_grad2(const double *__restrict__ U, const double *__restrict__ T, double *__re
 #pragma unroll
         for (int s = 0; s < 4; ++s) c[s] = geo.fnorm[3 * (e * 4 + s) + d];
         G[((size_t)e * 6 + 2 * q + d) * L2 + k] =
-            mt[d] * D2[k] + mt[3 + d] * D2[L2 + k] + face_add2<P>(tb.Bend, H, l, m, c);
+            mt[d] * D2[k] + mt[3 + d] * D2[L2 + k] + face_add2<P>(CT<P>::Bend(), H, l, m, c);
       }
     }
     __syncthreads();
@@ -278,7 +278,7 @@ k_visc2(const double *__restrict__ U, const double *__restrict__ G, double *__re
     for (int q = 0; q < 3; ++q) Lay[(s * 3 + q) * Q + j] = f[q][0] * nn[0] + f[q][1] * nn[1];
   }
   __syncthreads();
-  contract<1, 12, Q, 2, QF, NT>(tb.Ig, Lay, Wf, tid);
+  contract<1, 12, Q, 2, QF, NT>(CT<P>::Ig(), Lay, Wf, tid);
   __syncthreads();
   copy_block<NT>(Wf, TV + (size_t)e * 12 * QF, 12 * QF, tid);
   const double *mt = geo.metric + 9 * (size_t)e;
@@ -293,12 +293,12 @@ k_visc2(const double *__restrict__ U, const double *__restrict__ G, double *__re
   }
   __syncthreads();
   for (int q = 0; q < 3; ++q) {
-    contract<1, Q, Q, 2, L, NT>(tb.BW, Gn + 2 * q * Q2, X1, tid);
-    contract<1, Q, Q, 2, L, NT>(tb.QD, Gn + (2 * q + 1) * Q2, X2, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Gn + 2 * q * Q2, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Gn + (2 * q + 1) * Q2, X2, tid);
     __syncthreads();
-    contract<1, Q, L, 1, L, NT>(tb.QD, X1, Vo, tid);
+    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Vo, tid);
     __syncthreads();
-    contract<1, Q, L, 1, L, NT, true>(tb.BW, X2, Vo, tid);
+    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Vo, tid);
     __syncthreads();
     copy_block<NT>(Vo, V + ((size_t)e * 3 + q) * L2, L2, tid);
     __syncthreads();
@@ -314,9 +314,9 @@ k_forward2(const double *__restrict__ Fn, double *__restrict__ Fm, Tab tb, int n
   const int e = blockIdx.x % ne, v = blockIdx.x / ne, tid = threadIdx.x;
   copy_block<NT>(Fn + ((size_t)v * ne + e) * Q2, A, Q2, tid);
   __syncthreads();
-  contract<1, Q, Q, 2, L, NT>(tb.BW, A, X, tid);
+  contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), A, X, tid);
   __syncthreads();
-  contract<1, Q, L, 1, L, NT>(tb.BW, X, O, tid);
+  contract<1, Q, L, 1, L, NT>(CT<P>::BW(), X, O, tid);
   __syncthreads();
   copy_block<NT>(O, Fm + ((size_t)v * ne + e) * L2, L2, tid);
 }
@@ -377,6 +377,10 @@ hipError_t launch_2d(const LaunchArgs &a) {
 #undef DGX_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t upload_tables_2d(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
 }
 
 }  // namespace dgx

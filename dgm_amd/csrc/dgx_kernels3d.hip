@@ -30,11 +30,11 @@ __device__ __forceinline__ void backward3(const Tab &tb, const double *m, double
                                           double *out, int tid) {
   using S = H3<P>;
   constexpr int L = S::L, Q = S::Q, NT = S::NT;
-  contract<L, L, L, 0, Q, NT>(tb.BT, m, s1, tid);
+  contract<L, L, L, 0, Q, NT>(CT<P>::BT(), m, s1, tid);
   __syncthreads();
-  contract<Q, L, L, 1, Q, NT>(tb.BT, s1, s2, tid);
+  contract<Q, L, L, 1, Q, NT>(CT<P>::BT(), s1, s2, tid);
   __syncthreads();
-  contract<Q, Q, L, 2, Q, NT>(tb.BT, s2, out, tid);
+  contract<Q, Q, L, 2, Q, NT>(CT<P>::BT(), s2, out, tid);
   __syncthreads();
 }
 
@@ -47,23 +47,23 @@ __device__ __forceinline__ void divergence3(const Tab &tb, double *Fb, double *X
   constexpr int L = S::L, Q = S::Q, NT = S::NT;
   fill(0, Fb);
   __syncthreads();
-  contract<Q, Q, Q, 2, L, NT>(tb.BW, Fb, X, tid);
+  contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), Fb, X, tid);
   __syncthreads();
-  contract<Q, Q, L, 1, L, NT>(tb.BW, X, Y1, tid);
+  contract<Q, Q, L, 1, L, NT>(CT<P>::BW(), X, Y1, tid);
   fill(1, Fb);
   __syncthreads();
-  contract<Q, Q, Q, 2, L, NT>(tb.BW, Fb, X, tid);
+  contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), Fb, X, tid);
   __syncthreads();
-  contract<Q, Q, L, 1, L, NT>(tb.QD, X, Y2, tid);
+  contract<Q, Q, L, 1, L, NT>(CT<P>::QD(), X, Y2, tid);
   fill(2, Fb);
   __syncthreads();
-  contract<Q, Q, Q, 2, L, NT>(tb.QD, Fb, X, tid);
+  contract<Q, Q, Q, 2, L, NT>(CT<P>::QD(), Fb, X, tid);
   __syncthreads();
-  contract<Q, Q, L, 1, L, NT, true>(tb.BW, X, Y2, tid);
+  contract<Q, Q, L, 1, L, NT, true>(CT<P>::BW(), X, Y2, tid);
   __syncthreads();
-  contract<Q, L, L, 0, L, NT>(tb.QD, Y1, out, tid);
+  contract<Q, L, L, 0, L, NT>(CT<P>::QD(), Y1, out, tid);
   __syncthreads();
-  contract<Q, L, L, 0, L, NT, true>(tb.BW, Y2, out, tid);
+  contract<Q, L, L, 0, L, NT, true>(CT<P>::BW(), Y2, out, tid);
   __syncthreads();
 }
 
@@ -74,27 +74,10 @@ __device__ __forceinline__ void face_project3(const Tab &tb, const double *g, do
                                               int tid) {
   using S = H3<P>;
   constexpr int L = S::L, QF = S::QF, NT = S::NT;
-  contract<6, QF, QF, 1, L, NT>(tb.PW, g, W, tid);
+  contract<6, QF, QF, 1, L, NT>(CT<P>::PW(), g, W, tid);
   __syncthreads();
-  contract<6, L, QF, 2, L, NT>(tb.PW, W, H, tid);
+  contract<6, L, QF, 2, L, NT>(CT<P>::PW(), W, H, tid);
   __syncthreads();
-}
-
-// Contribution of the six face blocks H[s][.][.] at modal index (l,m,n),
-// weighted per face by c[s].  Face normal dims: 4,2 -> a; 1,3 -> b; 0,5 -> c.
-template <int P>
-__device__ __forceinline__ double face_add3(const double *__restrict__ Bend, const double *H,
-                                            int l, int m, int n, const double c[6]) {
-  constexpr int L = H3<P>::L, L2 = H3<P>::L2;
-  const double *B0 = Bend, *B1 = Bend + L;
-  double s = 0.0;
-  s = fma(c[4] * B0[l], H[4 * L2 + m * L + n], s);
-  s = fma(c[2] * B1[l], H[2 * L2 + m * L + n], s);
-  s = fma(c[1] * B0[m], H[1 * L2 + l * L + n], s);
-  s = fma(c[3] * B1[m], H[3 * L2 + l * L + n], s);
-  s = fma(c[0] * B0[n], H[0 * L2 + l * L + m], s);
-  s = fma(c[5] * B1[n], H[5 * L2 + l * L + m], s);
-  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -112,15 +95,15 @@ k_traces3(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
   for (int v = 0; v < nf; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L3, A, L3, tid);
     __syncthreads();
-    contract<L, L, L, 0, 2, NT>(tb.Bend, A, K0, tid);
-    contract<L, L, L, 1, 2, NT>(tb.Bend, A, K1, tid);
-    contract<L, L, L, 2, 2, NT>(tb.Bend, A, K2, tid);
+    contract<L, L, L, 0, 2, NT>(CT<P>::Bend(), A, K0, tid);
+    contract<L, L, L, 1, 2, NT>(CT<P>::Bend(), A, K1, tid);
+    contract<L, L, L, 2, 2, NT>(CT<P>::Bend(), A, K2, tid);
     __syncthreads();
     double *Te = T + (size_t)e * 6 * nf * QF2;
     // faces 4 (xi=-1) and 2 (xi=+1): [2][QF(b)][QF(c)]
-    contract<2, L, L, 1, QF, NT>(tb.Bg, K0, W, tid);
+    contract<2, L, L, 1, QF, NT>(CT<P>::Bg(), K0, W, tid);
     __syncthreads();
-    contract<2, QF, L, 2, QF, NT>(tb.Bg, W, O, tid);
+    contract<2, QF, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
     __syncthreads();
     for (int i = tid; i < 2 * QF2; i += NT) {
       const int end = i / QF2, a = i - end * QF2;
@@ -128,9 +111,9 @@ k_traces3(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
     }
     __syncthreads();
     // faces 1 (eta=-1) and 3 (eta=+1): [QF(a)][2][QF(c)]
-    contract<L, 2, L, 0, QF, NT>(tb.Bg, K1, W, tid);
+    contract<L, 2, L, 0, QF, NT>(CT<P>::Bg(), K1, W, tid);
     __syncthreads();
-    contract<QF, 2, L, 2, QF, NT>(tb.Bg, W, O, tid);
+    contract<QF, 2, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
     __syncthreads();
     for (int i = tid; i < 2 * QF2; i += NT) {
       const int ia = i / (2 * QF), r = i - ia * 2 * QF, end = r / QF, ic = r - end * QF;
@@ -138,9 +121,9 @@ k_traces3(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
     }
     __syncthreads();
     // faces 0 (zeta=-1) and 5 (zeta=+1): [QF(a)][QF(b)][2]
-    contract<L, L, 2, 0, QF, NT>(tb.Bg, K2, W, tid);
+    contract<L, L, 2, 0, QF, NT>(CT<P>::Bg(), K2, W, tid);
     __syncthreads();
-    contract<QF, L, 2, 1, QF, NT>(tb.Bg, W, O, tid);
+    contract<QF, L, 2, 1, QF, NT>(CT<P>::Bg(), W, O, tid);
     __syncthreads();
     for (int i = tid; i < 2 * QF2; i += NT) {
       const int ab = i >> 1, end = i & 1;
@@ -241,7 +224,7 @@ k_rhs3(const double *__restrict__ U, const double *__restrict__ T, const double 
     for (int s = 0; s < 6; ++s) cw[s] = 1.0;
     for (int k = tid; k < L3; k += NT) {
       const int l = k / L2, m = (k / L) % L, n = k % L;
-      double r = Rv[k] + face_add3<P>(tb.Bend, Y, l, m, n, cw);
+      double r = Rv[k] + face_add3<P>(CT<P>::Bend(), Y, l, m, n, cw);
       if (NS && v > 0) r -= V[((size_t)e * 4 + (v - 1)) * L3 + k];
       R[((size_t)v * ne + e) * L3 + k] = r;
     }
@@ -320,16 +303,16 @@ k_grad3(const double *__restrict__ U, const double *__restrict__ T, double *__re
       Fb[k] = val;
     }
     __syncthreads();
-    contract<Q, Q, Q, 2, L, NT>(tb.BW, Fb, Xa, tid);
-    contract<Q, Q, Q, 2, L, NT>(tb.QD, Fb, Xc, tid);
+    contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), Fb, Xa, tid);
+    contract<Q, Q, Q, 2, L, NT>(CT<P>::QD(), Fb, Xc, tid);
     __syncthreads();
-    contract<Q, Q, L, 1, L, NT>(tb.BW, Xa, Ya, tid);
-    contract<Q, Q, L, 1, L, NT>(tb.QD, Xa, Yb, tid);
-    contract<Q, Q, L, 1, L, NT>(tb.BW, Xc, Yc, tid);
+    contract<Q, Q, L, 1, L, NT>(CT<P>::BW(), Xa, Ya, tid);
+    contract<Q, Q, L, 1, L, NT>(CT<P>::QD(), Xa, Yb, tid);
+    contract<Q, Q, L, 1, L, NT>(CT<P>::BW(), Xc, Yc, tid);
     __syncthreads();
-    contract<Q, L, L, 0, L, NT>(tb.QD, Ya, D3, tid);
-    contract<Q, L, L, 0, L, NT>(tb.BW, Yb, D3 + L3, tid);
-    contract<Q, L, L, 0, L, NT>(tb.BW, Yc, D3 + 2 * L3, tid);
+    contract<Q, L, L, 0, L, NT>(CT<P>::QD(), Ya, D3, tid);
+    contract<Q, L, L, 0, L, NT>(CT<P>::BW(), Yb, D3 + L3, tid);
+    contract<Q, L, L, 0, L, NT>(CT<P>::BW(), Yc, D3 + 2 * L3, tid);
     face_project3<P>(tb, Jq + q * 6 * QF2, Xa, Ya, tid);   // H -> Ya ([6][L][L])
     for (int k = tid; k < L3; k += NT) {
       const int l = k / L2, m = (k / L) % L, n = k % L;
@@ -340,7 +323,7 @@ k_grad3(const double *__restrict__ U, const double *__restrict__ T, double *__re
 #pragma unroll
         for (int s = 0; s < 6; ++s) c[s] = geo.fnorm[3 * (e * 6 + s) + d];
         const double g = mt[d] * dr + mt[3 + d] * ds + mt[6 + d] * dt +
-                         face_add3<P>(tb.Bend, Ya, l, m, n, c);
+                         face_add3<P>(CT<P>::Bend(), Ya, l, m, n, c);
         G[((size_t)e * 12 + 3 * q + d) * L3 + k] = g;
       }
     }
@@ -373,11 +356,11 @@ k_visc3(const double *__restrict__ U, const double *__restrict__ G, double *__re
     double *dst = v < 5 ? Un + v * Q3 : Gn + (v - 5) * Q3;
     copy_block<NT>(src, m0, L3, tid);
     __syncthreads();
-    contract<L, L, L, 0, Q, NT>(tb.BT, m0, s1, tid);
+    contract<L, L, L, 0, Q, NT>(CT<P>::BT(), m0, s1, tid);
     __syncthreads();
-    contract<Q, L, L, 1, Q, NT>(tb.BT, s1, s2, tid);
+    contract<Q, L, L, 1, Q, NT>(CT<P>::BT(), s1, s2, tid);
     __syncthreads();
-    contract<Q, Q, L, 2, Q, NT>(tb.BT, s2, dst, tid);
+    contract<Q, Q, L, 2, Q, NT>(CT<P>::BT(), s2, dst, tid);
     __syncthreads();
   }
   const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
@@ -421,9 +404,9 @@ k_visc3(const double *__restrict__ U, const double *__restrict__ G, double *__re
       Lay[(s * 4 + q) * Q2 + ij] = f[q][0] * nn[0] + f[q][1] * nn[1] + f[q][2] * nn[2];
   }
   __syncthreads();
-  contract<24, Q, Q, 1, QF, NT>(tb.Ig, Lay, Wf, tid);
+  contract<24, Q, Q, 1, QF, NT>(CT<P>::Ig(), Lay, Wf, tid);
   __syncthreads();
-  contract<24, QF, Q, 2, QF, NT>(tb.Ig, Wf, Lay, tid);   // [24][QF][QF] == [6][4][QF2]
+  contract<24, QF, Q, 2, QF, NT>(CT<P>::Ig(), Wf, Lay, tid);   // [24][QF][QF] == [6][4][QF2]
   __syncthreads();
   copy_block<NT>(Lay, TV + (size_t)e * 24 * QF2, 24 * QF2, tid);
   __syncthreads();
@@ -442,21 +425,21 @@ k_visc3(const double *__restrict__ U, const double *__restrict__ G, double *__re
   double *X = Sc, *Y1 = Sc + Q2 * L, *Y2 = Y1 + Q * L2, *Vo = Y2 + Q * L2;
   for (int q = 0; q < 4; ++q) {
     const double *Fr = Gn + 3 * q * Q3, *Fs = Fr + Q3, *Ft = Fs + Q3;
-    contract<Q, Q, Q, 2, L, NT>(tb.BW, Fr, X, tid);
+    contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X, tid);
     __syncthreads();
-    contract<Q, Q, L, 1, L, NT>(tb.BW, X, Y1, tid);
+    contract<Q, Q, L, 1, L, NT>(CT<P>::BW(), X, Y1, tid);
     __syncthreads();
-    contract<Q, Q, Q, 2, L, NT>(tb.BW, Fs, X, tid);
+    contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), Fs, X, tid);
     __syncthreads();
-    contract<Q, Q, L, 1, L, NT>(tb.QD, X, Y2, tid);
+    contract<Q, Q, L, 1, L, NT>(CT<P>::QD(), X, Y2, tid);
     __syncthreads();
-    contract<Q, Q, Q, 2, L, NT>(tb.QD, Ft, X, tid);
+    contract<Q, Q, Q, 2, L, NT>(CT<P>::QD(), Ft, X, tid);
     __syncthreads();
-    contract<Q, Q, L, 1, L, NT, true>(tb.BW, X, Y2, tid);
+    contract<Q, Q, L, 1, L, NT, true>(CT<P>::BW(), X, Y2, tid);
     __syncthreads();
-    contract<Q, L, L, 0, L, NT>(tb.QD, Y1, Vo, tid);
+    contract<Q, L, L, 0, L, NT>(CT<P>::QD(), Y1, Vo, tid);
     __syncthreads();
-    contract<Q, L, L, 0, L, NT, true>(tb.BW, Y2, Vo, tid);
+    contract<Q, L, L, 0, L, NT, true>(CT<P>::BW(), Y2, Vo, tid);
     __syncthreads();
     copy_block<NT>(Vo, V + ((size_t)e * 4 + q) * L3, L3, tid);
     __syncthreads();
@@ -476,11 +459,11 @@ k_forward3(const double *__restrict__ Fn, double *__restrict__ Fm, Tab tb, int n
   const int e = blockIdx.x % ne, v = blockIdx.x / ne, tid = threadIdx.x;
   copy_block<NT>(Fn + ((size_t)v * ne + e) * Q3, A, Q3, tid);
   __syncthreads();
-  contract<Q, Q, Q, 2, L, NT>(tb.BW, A, X, tid);
+  contract<Q, Q, Q, 2, L, NT>(CT<P>::BW(), A, X, tid);
   __syncthreads();
-  contract<Q, Q, L, 1, L, NT>(tb.BW, X, Y, tid);
+  contract<Q, Q, L, 1, L, NT>(CT<P>::BW(), X, Y, tid);
   __syncthreads();
-  contract<Q, L, L, 0, L, NT>(tb.BW, Y, O, tid);
+  contract<Q, L, L, 0, L, NT>(CT<P>::BW(), Y, O, tid);
   __syncthreads();
   copy_block<NT>(O, Fm + ((size_t)v * ne + e) * L3, L3, tid);
 }
@@ -593,6 +576,10 @@ hipError_t launch_3d(const LaunchArgs &a) {
 #undef DGX_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t upload_tables_3d(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
 }
 
 }  // namespace dgx

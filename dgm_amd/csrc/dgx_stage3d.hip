@@ -1,0 +1,553 @@
+// Fused hex stage kernels (p <= 9).
+//
+// One workgroup of 256 threads per element.  Every sum-factorized
+// contraction runs on all conserved fields of the element at once (pencils of
+// 5 fields x q^2 fill the workgroup instead of q^2 = 81 pencils at p = 7),
+// nodal values live in registers (one thread per node), and the last
+// contraction of each chain accumulates modal results in registers, so a
+// workgroup needs only two LDS tensors of 5 q^3 doubles.
+//
+//   k_stage3   residual of Euler3d::convection + project (src/Euler3d.cpp:468-575,
+//              584-817; src/Hex.cpp:1786-1952) plus, for Navier-Stokes, the BR1
+//              viscous face flux and the viscous volume term from k_vterm3;
+//              optionally fused with the LSRK14 register update
+//              (src/TimeInt.cpp:1989-2013) and the face traces of the new state
+//              (Hex::fill_sides, src/Hex.cpp:933-947) for the next stage.
+//   k_vterm3   BR1 lifted gradients of (u, v, w, e) and the viscous stresses and
+//              heat flux they give (src/Navier_Stokes3d.cpp:182-391,
+//              src/Euler3d.cpp:1144-1199): viscous volume divergence (modal) and
+//              normal viscous flux at the Gauss points of every face.
+#include "dgx_kernels.hpp"
+
+namespace dgx {
+
+template <int P> struct S3 {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
+  static constexpr int L2 = L * L, L3 = L * L * L, Q2 = Q * Q, Q3 = Q * Q * Q, QF2 = QF * QF;
+  static constexpr int NT = 256;
+  static constexpr int NPT = (Q3 + NT - 1) / NT;        // nodes per thread
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  static constexpr int NA = cmax(5 * Q3, 24 * Q2);      // doubles per LDS tensor
+  static constexpr int NSL = (5 * L2 + NT - 1) / NT;    // modal pencil slots (5 fields)
+  static constexpr int NSG = (4 * L2 + NT - 1) / NT;    // modal pencil slots (4 fields)
+  static constexpr int lds_stage = (2 * NA + 30 * QF2) * 8;
+  static constexpr int lds_vterm = (2 * NA + 24 * QF2) * 8;
+};
+
+// Final contraction of a chain, along dim 1 of the LDS tensor [NF][Q][L2],
+// accumulated into registers: acc[k][l] += c * sum_i M[l][i] in[f][i][pp] for
+// the thread's pencils G = tid + k*NT = f*L2 + pp.
+template <int P, int NF, int NS>
+__device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, const double *in,
+                                              double (&acc)[NS][P + 1], int tid) {
+  using S = S3<P>;
+  constexpr int L = S::L, Q = S::Q, L2 = S::L2, NT = S::NT;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int G = tid + k * NT;
+    if (G < NF * L2) {
+      const int f = G / L2, pp = G - f * L2;
+      double x[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) x[i] = in[(f * Q + i) * L2 + pp];
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) s = fma(M[l * Q + i], x[i], s);
+        acc[k][l] += s;
+      }
+    }
+  }
+}
+
+// Sum-factorized projected divergence chain for one reference direction d of
+// NF nodal fields in A ([NF][Q3]); result accumulated into registers.
+// d = 0 (r): QD along a;  d = 1 (s): QD along b;  d = 2 (t): QD along c.
+template <int V> struct IC { static constexpr int value = V; };
+
+template <int P, int NF, int NS, int D>
+__device__ __forceinline__ void div_chain(double *A, double *B, double (&acc)[NS][P + 1], int tid) {
+  using S = S3<P>;
+  constexpr int L = S::L, Q = S::Q, Q2 = S::Q2, NT = S::NT;
+  const double *Mc = D == 2 ? CT<P>::QD() : CT<P>::BW();
+  const double *Mb = D == 1 ? CT<P>::QD() : CT<P>::BW();
+  const double *Ma = D == 0 ? CT<P>::QD() : CT<P>::BW();
+  contract<1, NF * Q2, Q, 2, L, NT>(Mc, A, B, tid);     // [NF Q2][L]
+  __syncthreads();
+  contract<NF * Q, Q, L, 1, L, NT>(Mb, B, A, tid);      // [NF Q][L][L]
+  __syncthreads();
+  chain_to_regs<P, NF, NS>(Ma, A, acc, tid);            // [NF][L][L2]
+  __syncthreads();
+}
+
+// modal [NF][L3] in A -> nodal [NF][Q3] in B
+template <int P, int NF>
+__device__ __forceinline__ void backward_batch(double *A, double *B, int tid) {
+  using S = S3<P>;
+  constexpr int L = S::L, Q = S::Q, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  contract<NF, L, L2, 1, Q, NT>(CT<P>::BT(), A, B, tid);
+  __syncthreads();
+  contract<NF * Q, L, L, 1, Q, NT>(CT<P>::BT(), B, A, tid);
+  __syncthreads();
+  contract<1, NF * Q2, L, 2, Q, NT>(CT<P>::BT(), A, B, tid);
+  __syncthreads();
+}
+
+struct StageArgs {
+  const double *T;       // face traces of the input state
+  const double *V;       // viscous volume term [ne][4][L3] (Navier-Stokes)
+  const double *TV;      // viscous normal flux [ne][6][4][QF2] (Navier-Stokes)
+  double *U;             // modal state (read; written when updating)
+  double *Ut;            // LSRK register (update mode)
+  double *R;             // residual output (residual mode)
+  double *Tn;            // traces of the updated state (update mode)
+  double ca, cb, dt;     // Ut = ca*Ut - dt*R ; U += cb*Ut
+  int ne;
+  int *flag;
+};
+
+template <int P, bool NS, bool UPDATE>
+__global__ void __launch_bounds__(256)
+k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
+  using S = S3<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
+                NT = S::NT, NPT = S::NPT, NSL = S::NSL;
+  extern __shared__ double sm[];
+  double *A = sm, *B = A + S::NA, *gF = B + S::NA;
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const double gm1 = ph.gamma - 1.0;
+
+  // modal state -> nodal registers
+  for (int i = tid; i < 5 * L3; i += NT) {
+    const int v = i / L3, m = i - v * L3;
+    A[i] = a.U[((size_t)v * ne + e) * L3 + m];
+  }
+  __syncthreads();
+  backward_batch<P, 5>(A, B, tid);
+  double un[NPT][5];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int n = tid + r * NT;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) un[r][v] = n < Q3 ? B[v * Q3 + n] : 1.0;
+  }
+
+  // face corrections g = (J_side/J_vol) (Fhat - F(U-).n) [- BR1 viscous part]
+  bool bad = false;
+  for (int i = tid; i < 6 * QF2; i += NT) {
+    const int s = i / QF2, q = i - s * QF2;
+    const int es = e * 6 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int qn = geo.perm[geo.permid[es] * QF2 + q];
+    const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1], nz = geo.fnorm[3 * es + 2];
+    const double jr = geo.fjr[es];
+    double UL[5], UR[5], FL[5], FR[5], lL, lR;
+    const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
+    const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + q]; UR[v] = tr[v * QF2 + qn]; }
+    bool bl, br;
+    euler_normal_flux3(UL, nx, ny, nz, gm1, FL, lL, bl);
+    euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
+    bad |= bl | br;
+    const double lmax = fmax(lL, lR);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+      double g = jr * (Fh - FL[v]);
+      if (NS && v > 0) {
+        const double *vl = a.TV + ((size_t)e * 6 + s) * 4 * QF2;
+        const double *vr = a.TV + ((size_t)nb * 6 + ns) * 4 * QF2;
+        g += 0.5 * jr * (vl[(v - 1) * QF2 + q] + vr[(v - 1) * QF2 + qn]);
+      }
+      gF[(v * 6 + s) * QF2 + q] = g;
+    }
+  }
+  if (bad) atomicOr(a.flag, 1);
+
+  // volume: projected divergence of the contravariant Euler fluxes
+  double acc[NSL][L];
+#pragma unroll
+  for (int k = 0; k < NSL; ++k)
+#pragma unroll
+    for (int l = 0; l < L; ++l) acc[k][l] = 0.0;
+  const double *mt = geo.metric + 9 * (size_t)e;
+  double pr[NPT], ux[NPT], uy[NPT], uz[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const double ir = 1.0 / un[r][0];
+    ux[r] = un[r][1] * ir; uy[r] = un[r][2] * ir; uz[r] = un[r][3] * ir;
+    pr[r] = gm1 * (un[r][4] - 0.5 * (un[r][1] * ux[r] + un[r][2] * uy[r] + un[r][3] * uz[r]));
+  }
+  auto vol_dir = [&](auto Dc) {
+    constexpr int d = decltype(Dc)::value;
+    const double m0 = mt[3 * d], m1 = mt[3 * d + 1], m2 = mt[3 * d + 2];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int n = tid + r * NT;
+      if (n < Q3) {
+        const double vn = m0 * ux[r] + m1 * uy[r] + m2 * uz[r];   // contravariant velocity
+        A[n] = un[r][0] * vn;
+        A[Q3 + n] = un[r][1] * vn + m0 * pr[r];
+        A[2 * Q3 + n] = un[r][2] * vn + m1 * pr[r];
+        A[3 * Q3 + n] = un[r][3] * vn + m2 * pr[r];
+        A[4 * Q3 + n] = (un[r][4] + pr[r]) * vn;
+      }
+    }
+    __syncthreads();
+    div_chain<P, 5, NSL, d>(A, B, acc, tid);
+  };
+  vol_dir(IC<0>());
+  vol_dir(IC<1>());
+  vol_dir(IC<2>());
+
+  // faces: H = (PW (x) PW) gF for 5 fields x 6 faces
+  contract<30, QF, QF, 1, L, NT>(CT<P>::PW(), gF, A, tid);
+  __syncthreads();
+  contract<30, L, QF, 2, L, NT>(CT<P>::PW(), A, B, tid);
+  __syncthreads();
+  const double one6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+  for (int k = 0; k < NSL; ++k) {
+    const int G = tid + k * NT;
+    if (G < 5 * L2) {
+      const int v = G / L2, pp = G - v * L2, m = pp / L, nn = pp - m * L;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        double r = acc[k][l] + face_add3<P>(CT<P>::Bend(), B + v * 6 * L2, l, m, nn, one6);
+        if (NS && v > 0) r -= a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp];
+        const size_t gi = ((size_t)v * ne + e) * L3 + l * L2 + pp;
+        if (!UPDATE) {
+          a.R[gi] = r;
+        } else {
+          const double ut = a.ca * a.Ut[gi] - a.dt * r;
+          const double u = fma(a.cb, ut, a.U[gi]);
+          a.Ut[gi] = ut;
+          a.U[gi] = u;
+          acc[k][l] = u;
+        }
+      }
+    }
+  }
+  if (!UPDATE) return;
+
+  // face traces of the updated state for the next stage
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NSL; ++k) {
+    const int G = tid + k * NT;
+    if (G < 5 * L2) {
+      const int v = G / L2, pp = G - v * L2;
+#pragma unroll
+      for (int l = 0; l < L; ++l) A[v * L3 + l * L2 + pp] = acc[k][l];
+    }
+  }
+  __syncthreads();
+  double *K0 = B, *K1 = B + 10 * L2, *K2 = B + 20 * L2, *W = A, *O = A + 10 * QF * L;
+  contract<5, L, L2, 1, 2, NT>(CT<P>::Bend(), A, K0, tid);        // [5][2][L][L]
+  contract<5 * L, L, L, 1, 2, NT>(CT<P>::Bend(), A, K1, tid);     // [5][L][2][L]
+  contract<1, 5 * L2, L, 2, 2, NT>(CT<P>::Bend(), A, K2, tid);    // [5][L][L][2]
+  __syncthreads();
+  double *Te = a.Tn + (size_t)e * 6 * 5 * QF2;
+  // faces 4 / 2: [5][2][QF][QF]
+  contract<10, L, L, 1, QF, NT>(CT<P>::Bg(), K0, W, tid);
+  __syncthreads();
+  contract<10, QF, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
+  __syncthreads();
+  for (int i = tid; i < 10 * QF2; i += NT) {
+    const int ve = i / QF2, q = i - ve * QF2, v = ve >> 1, end = ve & 1;
+    Te[((end ? 2 : 4) * 5 + v) * QF2 + q] = O[i];
+  }
+  __syncthreads();
+  // faces 1 / 3: [5][QF][2][QF]
+  contract<5, L, 2 * L, 1, QF, NT>(CT<P>::Bg(), K1, W, tid);
+  __syncthreads();
+  contract<1, 10 * QF, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
+  __syncthreads();
+  for (int i = tid; i < 10 * QF2; i += NT) {
+    const int v = i / (2 * QF2), r = i - v * 2 * QF2, ia = r / (2 * QF), r2 = r - ia * 2 * QF;
+    const int end = r2 / QF, ic = r2 - end * QF;
+    Te[((end ? 3 : 1) * 5 + v) * QF2 + ia * QF + ic] = O[i];
+  }
+  __syncthreads();
+  // faces 0 / 5: [5][QF][QF][2]
+  contract<5, L, 2 * L, 1, QF, NT>(CT<P>::Bg(), K2, W, tid);
+  __syncthreads();
+  contract<5 * QF, L, 2, 1, QF, NT>(CT<P>::Bg(), W, O, tid);
+  __syncthreads();
+  for (int i = tid; i < 10 * QF2; i += NT) {
+    const int v = i / (2 * QF2), r = i - v * 2 * QF2, ab = r >> 1, end = r & 1;
+    Te[((end ? 5 : 0) * 5 + v) * QF2 + ab] = O[i];
+  }
+}
+
+// BR1 viscous terms: V[e][eq][L3] = modal divergence of the viscous flux,
+// TV[e][s][eq][QF2] = viscous flux . n at the Gauss points of face s.
+template <int P>
+__global__ void __launch_bounds__(256)
+k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
+         double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
+  using S = S3<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3,
+                QF2 = S::QF2, NT = S::NT, NPT = S::NPT, NSG = S::NSG;
+  extern __shared__ double sm[];
+  double *A = sm, *B = A + S::NA, *Jq = B + S::NA;
+  const int e = blockIdx.x, tid = threadIdx.x;
+
+  for (int i = tid; i < 5 * L3; i += NT) {
+    const int v = i / L3, m = i - v * L3;
+    A[i] = U[((size_t)v * ne + e) * L3 + m];
+  }
+  // primitive jumps J_side/J_vol ({q} - q-) at the Gauss points (average_flux '+')
+  for (int i = tid; i < 6 * QF2; i += NT) {
+    const int s = i / QF2, q = i - s * QF2;
+    const int es = e * 6 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int qn = geo.perm[geo.permid[es] * QF2 + q];
+    const double jr = geo.fjr[es];
+    const double *tl = T + ((size_t)e * 6 + s) * 5 * QF2;
+    const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
+    double pl[4], pq[4];
+    {
+      const double r = tl[q], u = tl[QF2 + q] / r, v = tl[2 * QF2 + q] / r, w = tl[3 * QF2 + q] / r;
+      double en = tl[4 * QF2 + q] / r;
+      double wk = u * u; wk = v * v + wk; wk = w * w + wk;
+      pl[0] = u; pl[1] = v; pl[2] = w; pl[3] = -0.5 * wk + en;
+    }
+    {
+      const double r = tr[qn], u = tr[QF2 + qn] / r, v = tr[2 * QF2 + qn] / r, w = tr[3 * QF2 + qn] / r;
+      double en = tr[4 * QF2 + qn] / r;
+      double wk = u * u; wk = v * v + wk; wk = w * w + wk;
+      pq[0] = u; pq[1] = v; pq[2] = w; pq[3] = -0.5 * wk + en;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Jq[(c * 6 + s) * QF2 + q] = jr * (0.5 * (pl[c] + pq[c]) - pl[c]);
+  }
+  __syncthreads();
+  backward_batch<P, 5>(A, B, tid);
+  double pv[NPT][4];   // u, v, w, e at the thread's nodes
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int n = tid + r * NT;
+    const double rho = n < Q3 ? B[n] : 1.0;
+    const double u = n < Q3 ? B[Q3 + n] / rho : 0.0, v = n < Q3 ? B[2 * Q3 + n] / rho : 0.0;
+    const double w = n < Q3 ? B[3 * Q3 + n] / rho : 0.0;
+    const double en = n < Q3 ? B[4 * Q3 + n] / rho : 0.0;
+    double wk = u * u; wk = v * v + wk; wk = w * w + wk;
+    pv[r][0] = u; pv[r][1] = v; pv[r][2] = w; pv[r][3] = -0.5 * wk + en;
+  }
+  __syncthreads();
+
+  // lifted, projected gradients: G[k][x][l] for the thread's pencils (prim, m, n)
+  const double *mt = geo.metric + 9 * (size_t)e;
+  double G[NSG][3][L];
+#pragma unroll
+  for (int k = 0; k < NSG; ++k)
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int l = 0; l < L; ++l) G[k][x][l] = 0.0;
+  auto grad_dir = [&](auto Dc) {
+    constexpr int d = decltype(Dc)::value;
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int n = tid + r * NT;
+      if (n < Q3)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) A[c * Q3 + n] = pv[r][c];
+    }
+    __syncthreads();
+    double D[NSG][L];
+#pragma unroll
+    for (int k = 0; k < NSG; ++k)
+#pragma unroll
+      for (int l = 0; l < L; ++l) D[k][l] = 0.0;
+    div_chain<P, 4, NSG, d>(A, B, D, tid);
+#pragma unroll
+    for (int k = 0; k < NSG; ++k)
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int l = 0; l < L; ++l) G[k][x][l] = fma(mt[3 * d + x], D[k][l], G[k][x][l]);
+  };
+  grad_dir(IC<0>());
+  grad_dir(IC<1>());
+  grad_dir(IC<2>());
+  contract<24, QF, QF, 1, L, NT>(CT<P>::PW(), Jq, A, tid);
+  __syncthreads();
+  contract<24, L, QF, 2, L, NT>(CT<P>::PW(), A, B, tid);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NSG; ++k) {
+    const int Gi = tid + k * NT;
+    if (Gi < 4 * L2) {
+      const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double cf[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) cf[s] = geo.fnorm[3 * (e * 6 + s) + x];
+#pragma unroll
+        for (int l = 0; l < L; ++l) G[k][x][l] += face_add3<P>(CT<P>::Bend(), B + c * 6 * L2, l, m, nn, cf);
+      }
+    }
+  }
+  __syncthreads();
+
+  // nodal gradients: batch x holds d/dx_x of (u, v, w, e)
+  double gn[NPT][12];   // [3*prim + x]
+  auto grad_nodal = [&](auto Xc) {
+    constexpr int x = decltype(Xc)::value;
+#pragma unroll
+    for (int k = 0; k < NSG; ++k) {
+      const int Gi = tid + k * NT;
+      if (Gi < 4 * L2) {
+        const int c = Gi / L2, pp = Gi - c * L2;
+#pragma unroll
+        for (int l = 0; l < L; ++l) A[(c * L + l) * L2 + pp] = G[k][x][l];
+      }
+    }
+    __syncthreads();
+    backward_batch<P, 4>(A, B, tid);
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int n = tid + r * NT;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gn[r][3 * c + x] = n < Q3 ? B[c * Q3 + n] : 0.0;
+    }
+    __syncthreads();
+  };
+  grad_nodal(IC<0>());
+  grad_nodal(IC<1>());
+  grad_nodal(IC<2>());
+
+  // viscous stress tensor and heat flux at the nodes: sg[r][3*eq + x]
+  const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
+  double sg[NPT][12];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const double ux = gn[r][0], uy = gn[r][1], uz = gn[r][2];
+    const double vx = gn[r][3], vy = gn[r][4], vz = gn[r][5];
+    const double wx = gn[r][6], wy = gn[r][7], wz = gn[r][8];
+    const double t12 = mu * (uy + vx), t13 = mu * (uz + wx), t23 = mu * (vz + wy);
+    const double t11 = mu * ((2.0 / 3.0) * ((2.0 * ux - vy) - wz));
+    const double t22 = mu * ((2.0 / 3.0) * ((2.0 * vy - ux) - wz));
+    const double t33 = mu * ((2.0 / 3.0) * ((2.0 * wz - ux) - vy));
+    const double u = pv[r][0], v = pv[r][1], w = pv[r][2];
+    sg[r][0] = t11; sg[r][1] = t12; sg[r][2] = t13;
+    sg[r][3] = t12; sg[r][4] = t22; sg[r][5] = t23;
+    sg[r][6] = t13; sg[r][7] = t23; sg[r][8] = t33;
+    sg[r][9] = t11 * u + t12 * v + t13 * w + kap * gn[r][9];
+    sg[r][10] = t12 * u + t22 * v + t23 * w + kap * gn[r][10];
+    sg[r][11] = t13 * u + t23 * v + t33 * w + kap * gn[r][11];
+  }
+
+  // normal viscous flux on the GLL face layers -> Gauss points
+  double *Lay = A;   // [6][4][Q2]
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int n = tid + r * NT;
+    if (n < Q3) {
+      const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+      int fs[6], li[6], cnt = 0;
+      if (k == 0) { fs[cnt] = 0; li[cnt++] = i * Q + j; }
+      if (k == Q - 1) { fs[cnt] = 5; li[cnt++] = i * Q + j; }
+      if (j == 0) { fs[cnt] = 1; li[cnt++] = i * Q + k; }
+      if (j == Q - 1) { fs[cnt] = 3; li[cnt++] = i * Q + k; }
+      if (i == 0) { fs[cnt] = 4; li[cnt++] = j * Q + k; }
+      if (i == Q - 1) { fs[cnt] = 2; li[cnt++] = j * Q + k; }
+      for (int c = 0; c < cnt; ++c) {
+        const double *nn = geo.fnorm + 3 * (e * 6 + fs[c]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          Lay[(fs[c] * 4 + q) * Q2 + li[c]] = sg[r][3 * q] * nn[0] + sg[r][3 * q + 1] * nn[1] + sg[r][3 * q + 2] * nn[2];
+      }
+    }
+  }
+  __syncthreads();
+  contract<24, Q, Q, 1, QF, NT>(CT<P>::Ig(), Lay, B, tid);
+  __syncthreads();
+  contract<24, QF, Q, 2, QF, NT>(CT<P>::Ig(), B, A, tid);
+  __syncthreads();
+  for (int i = tid; i < 24 * QF2; i += NT) TV[(size_t)e * 24 * QF2 + i] = A[i];
+  __syncthreads();
+
+  // viscous volume divergence of the 4 equations
+  double acc[NSG][L];
+#pragma unroll
+  for (int k = 0; k < NSG; ++k)
+#pragma unroll
+    for (int l = 0; l < L; ++l) acc[k][l] = 0.0;
+  auto visc_dir = [&](auto Dc) {
+    constexpr int d = decltype(Dc)::value;
+    const double m0 = mt[3 * d], m1 = mt[3 * d + 1], m2 = mt[3 * d + 2];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int n = tid + r * NT;
+      if (n < Q3)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          A[q * Q3 + n] = m0 * sg[r][3 * q] + m1 * sg[r][3 * q + 1] + m2 * sg[r][3 * q + 2];
+    }
+    __syncthreads();
+    div_chain<P, 4, NSG, d>(A, B, acc, tid);
+  };
+  visc_dir(IC<0>());
+  visc_dir(IC<1>());
+  visc_dir(IC<2>());
+#pragma unroll
+  for (int k = 0; k < NSG; ++k) {
+    const int Gi = tid + k * NT;
+    if (Gi < 4 * L2) {
+      const int q = Gi / L2, pp = Gi - q * L2;
+#pragma unroll
+      for (int l = 0; l < L; ++l) V[((size_t)e * 4 + q) * L3 + l * L2 + pp] = acc[k][l];
+    }
+  }
+}
+
+template <int P>
+static hipError_t launch_stage3_p(const StageLaunch &s) {
+  using S = S3<P>;
+  StageArgs a;
+  a.T = s.T; a.V = s.V; a.TV = s.TV; a.U = s.U; a.Ut = s.Ut; a.R = s.R; a.Tn = s.Tn;
+  a.ca = s.ca; a.cb = s.cb; a.dt = s.dt; a.ne = s.ne; a.flag = s.flag;
+  if (s.op == 0) {
+    const int sh = S::lds_vterm;
+    (void)hipFuncSetAttribute((const void *)k_vterm3<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(256), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
+                       s.geo, s.phys, s.ne);
+    return hipGetLastError();
+  }
+  const int sh = S::lds_stage;
+#define DGX_STAGE(NSV, UPD)                                                                          \
+  do {                                                                                               \
+    (void)hipFuncSetAttribute((const void *)k_stage3<P, NSV, UPD>,                                   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, sh);                       \
+    hipLaunchKernelGGL((k_stage3<P, NSV, UPD>), dim3(s.ne), dim3(256), sh, s.stream, a, s.tab, s.geo, \
+                       s.phys);                                                                      \
+  } while (0)
+  if (s.ns) { if (s.update) DGX_STAGE(true, true); else DGX_STAGE(true, false); }
+  else { if (s.update) DGX_STAGE(false, true); else DGX_STAGE(false, false); }
+#undef DGX_STAGE
+  return hipGetLastError();
+}
+
+bool stage3_supported(int p) { return p >= 1 && p <= 9; }
+
+hipError_t launch_stage3(const StageLaunch &s) {
+  switch (s.p) {
+#define DGX_CASE(PP) case PP: return launch_stage3_p<PP>(s);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t upload_tables_s3d(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+}
+
+}  // namespace dgx
