@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with one device-to-device copy of the modal state (a plain copy "
+                         "kernel of known size for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
@@ -129,6 +132,13 @@ def main():
     achieved = algo_bytes / mean_s / 1e9
     norms = gpu.norms()
     finite = bool(np.all(np.isfinite(norms)))
+    if args.copy_calibration:
+        nbytes = 8 * dofs
+        rc = gpu.lib.dgx_residual_download(gpu.h, ctypes.cast(gpu.device_state_ptr(),
+                                           ctypes.POINTER(ctypes.c_double)), dgx.DGX_MODAL, 1)
+        if rc != 0:
+            raise RuntimeError(gpu.lib.dgx_last_error().decode())
+        print(f"[bench] calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,

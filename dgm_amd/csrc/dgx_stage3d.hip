@@ -5,7 +5,9 @@
 // 5 fields x q^2 fill the workgroup instead of q^2 = 81 pencils at p = 7),
 // nodal values live in registers (one thread per node), and the last
 // contraction of each chain accumulates modal results in registers, so a
-// workgroup needs only two LDS tensors of 5 q^3 doubles.
+// workgroup needs only two LDS tensors of about 5 q^3 doubles.  The last
+// dimension of every LDS tensor is padded to an odd pitch, which keeps the
+// 64-bit LDS accesses of a wavefront on distinct banks.
 //
 //   k_stage3   residual of Euler3d::convection + project (src/Euler3d.cpp:468-575,
 //              584-817; src/Hex.cpp:1786-1952) plus, for Navier-Stokes, the BR1
@@ -21,35 +23,99 @@
 
 namespace dgx {
 
+constexpr int odd(int n) { return n | 1; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
 template <int P> struct S3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L * L * L, Q2 = Q * Q, Q3 = Q * Q * Q, QF2 = QF * QF;
+  static constexpr int LP = odd(L), QP = odd(Q), FP = odd(QF);   // padded pitches
   static constexpr int NT = 256;
   static constexpr int NPT = (Q3 + NT - 1) / NT;        // nodes per thread
-  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  static constexpr int NA = cmax(5 * Q3, 24 * Q2);      // doubles per LDS tensor
+  // doubles per LDS tensor: the largest intermediate of any phase
+  static constexpr int NA = cmax(cmax(cmax(5 * Q2 * QP, 5 * Q2 * LP),                 // nodal, backward
+                                      cmax(30 * L * FP, 30 * L * LP)),                 // face projection
+                                 cmax(cmax(24 * Q * QP, 24 * QF * QP),                 // viscous layers
+                                      cmax(cmax(20 * L * LP + 15 * L2,                 // trace collapse
+                                                10 * QF * cmax(LP, 3 * L) + 15 * QF2), // trace faces
+                                           24 * QF * FP)));
   static constexpr int NSL = (5 * L2 + NT - 1) / NT;    // modal pencil slots (5 fields)
   static constexpr int NSG = (4 * L2 + NT - 1) / NT;    // modal pencil slots (4 fields)
-  static constexpr int lds_stage = (2 * NA + 30 * QF2) * 8;
-  static constexpr int lds_vterm = (2 * NA + 24 * QF2) * 8;
+  static constexpr int NG = 30 * QF * FP;               // face corrections
+  static constexpr int lds_stage = (2 * NA + NG) * 8;
+  static constexpr int lds_vterm = (2 * NA + 24 * QF * FP) * 8;
+  // padded position of nodal index n = (i*Q + j)*Q + k
+  static __device__ __forceinline__ int nidx(int n) { return (n / Q) * QP + (n - (n / Q) * Q); }
 };
 
-// Final contraction of a chain, along dim 1 of the LDS tensor [NF][Q][L2],
-// accumulated into registers: acc[k][l] += c * sum_i M[l][i] in[f][i][pp] for
-// the thread's pencils G = tid + k*NT = f*L2 + pp.
+template <int V> struct IC { static constexpr int value = V; };
+
+// Coefficient rows per pass (about 40 doubles live in scalar registers).
+template <int NI> constexpr int out_chunk() { return NI * 3 <= 40 ? 3 : 2; }
+
+// [A][N][B][C] (last-dim pitch PI) -> [A][NO][B][C] (pitch PO), contracting N.
+template <int A, int N, int B, int C, int NO, int PI, int PO, int NT, bool ACC = false>
+__device__ __forceinline__ void con_mid(const double *__restrict__ M, const double *in,
+                                        double *out, int tid) {
+  constexpr int NP = A * B * C, OC = out_chunk<N>();
+#pragma unroll
+  for (int o0 = 0; o0 < NO; o0 += OC) {
+    for (int pc = tid; pc < NP; pc += NT) {
+      const int c = pc % C, t = pc / C, b = t % B, a = t / B;
+      double x[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) x[i] = in[((a * N + i) * B + b) * PI + c];
+#pragma unroll
+      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) s = fma(M[o * N + i], x[i], s);
+        double *dst = out + ((a * NO + o) * B + b) * PO + c;
+        if (ACC) *dst += s; else *dst = s;
+      }
+    }
+  }
+}
+
+// [R][NI] (pitch PI) -> [R][NO] (pitch PO), contracting the last dimension.
+template <int R, int NI, int NO, int PI, int PO, int NT>
+__device__ __forceinline__ void con_last(const double *__restrict__ M, const double *in, double *out,
+                                         int tid) {
+  constexpr int OC = out_chunk<NI>();
+#pragma unroll
+  for (int o0 = 0; o0 < NO; o0 += OC) {
+    for (int r = tid; r < R; r += NT) {
+      double x[NI];
+#pragma unroll
+      for (int i = 0; i < NI; ++i) x[i] = in[r * PI + i];
+#pragma unroll
+      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
+        out[r * PO + o] = s;
+      }
+    }
+  }
+}
+
+// Final contraction of a chain over the first spatial dimension of
+// [NF][Q][L][L] (pitch LP), accumulated into registers:
+// acc[k][l] += sum_i M[l][i] in[f][i][m][n] for the thread's pencils
+// G = tid + k*NT = f*L2 + m*L + n.
 template <int P, int NF, int NS>
 __device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, const double *in,
                                               double (&acc)[NS][P + 1], int tid) {
   using S = S3<P>;
-  constexpr int L = S::L, Q = S::Q, L2 = S::L2, NT = S::NT;
+  constexpr int L = S::L, Q = S::Q, L2 = S::L2, LP = S::LP, NT = S::NT;
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     const int G = tid + k * NT;
     if (G < NF * L2) {
-      const int f = G / L2, pp = G - f * L2;
+      const int f = G / L2, pp = G - f * L2, m = pp / L, n = pp - m * L;
       double x[Q];
 #pragma unroll
-      for (int i = 0; i < Q; ++i) x[i] = in[(f * Q + i) * L2 + pp];
+      for (int i = 0; i < Q; ++i) x[i] = in[((f * Q + i) * L + m) * LP + n];
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         double s = 0.0;
@@ -61,37 +127,62 @@ __device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, cons
   }
 }
 
-// Sum-factorized projected divergence chain for one reference direction d of
-// NF nodal fields in A ([NF][Q3]); result accumulated into registers.
-// d = 0 (r): QD along a;  d = 1 (s): QD along b;  d = 2 (t): QD along c.
-template <int V> struct IC { static constexpr int value = V; };
-
+// Sum-factorized projected divergence chain for reference direction D of NF
+// nodal fields in A ([NF][Q][Q][Q], pitch QP); result accumulated into registers.
+// D = 0 (r): QD along a;  D = 1 (s): QD along b;  D = 2 (t): QD along c.
 template <int P, int NF, int NS, int D>
 __device__ __forceinline__ void div_chain(double *A, double *B, double (&acc)[NS][P + 1], int tid) {
   using S = S3<P>;
-  constexpr int L = S::L, Q = S::Q, Q2 = S::Q2, NT = S::NT;
+  constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
   const double *Mc = D == 2 ? CT<P>::QD() : CT<P>::BW();
   const double *Mb = D == 1 ? CT<P>::QD() : CT<P>::BW();
   const double *Ma = D == 0 ? CT<P>::QD() : CT<P>::BW();
-  contract<1, NF * Q2, Q, 2, L, NT>(Mc, A, B, tid);     // [NF Q2][L]
+  con_last<NF * Q * Q, Q, L, QP, LP, NT>(Mc, A, B, tid);        // [NF Q Q][L]
   __syncthreads();
-  contract<NF * Q, Q, L, 1, L, NT>(Mb, B, A, tid);      // [NF Q][L][L]
+  con_mid<NF * Q, Q, 1, L, L, LP, LP, NT>(Mb, B, A, tid);       // [NF Q][L][L]
   __syncthreads();
-  chain_to_regs<P, NF, NS>(Ma, A, acc, tid);            // [NF][L][L2]
+  chain_to_regs<P, NF, NS>(Ma, A, acc, tid);                    // [NF][L][L][L]
   __syncthreads();
 }
 
-// modal [NF][L3] in A -> nodal [NF][Q3] in B
+// modal [NF][L][L][L] (pitch LP) in A -> nodal [NF][Q][Q][Q] (pitch QP) in B
 template <int P, int NF>
 __device__ __forceinline__ void backward_batch(double *A, double *B, int tid) {
   using S = S3<P>;
-  constexpr int L = S::L, Q = S::Q, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
-  contract<NF, L, L2, 1, Q, NT>(CT<P>::BT(), A, B, tid);
+  constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
+  con_mid<NF, L, L, L, Q, LP, LP, NT>(CT<P>::BT(), A, B, tid);   // [NF][Q][L][L]
   __syncthreads();
-  contract<NF * Q, L, L, 1, Q, NT>(CT<P>::BT(), B, A, tid);
+  con_mid<NF * Q, L, 1, L, Q, LP, LP, NT>(CT<P>::BT(), B, A, tid);   // [NF Q][Q][L]
   __syncthreads();
-  contract<1, NF * Q2, L, 2, Q, NT>(CT<P>::BT(), A, B, tid);
+  con_last<NF * Q * Q, L, Q, LP, QP, NT>(CT<P>::BT(), A, B, tid);   // [NF Q Q][Q]
   __syncthreads();
+}
+
+// Face blocks g[NFB][QF][QF] (pitch FP) -> H[NFB][L][L] (pitch LP) in B, via W in A.
+template <int P, int NFB>
+__device__ __forceinline__ void face_project(const double *g, double *A, double *B, int tid) {
+  using S = S3<P>;
+  constexpr int L = S::L, QF = S::QF, LP = S::LP, FP = S::FP, NT = S::NT;
+  con_mid<NFB, QF, 1, QF, L, FP, FP, NT>(CT<P>::PW(), g, A, tid);     // [NFB][L][QF]
+  __syncthreads();
+  con_last<NFB * L, QF, L, FP, LP, NT>(CT<P>::PW(), A, B, tid);     // [NFB][L][L]
+  __syncthreads();
+}
+
+// Contribution at modal (l,m,n) of the six face blocks H[s] ([L][L], pitch LP)
+// weighted by c[s].  Face normal dims: 4,2 -> a; 1,3 -> b; 0,5 -> c.
+template <int P>
+__device__ __forceinline__ double face_sum(const double *H, int l, int m, int n, const double c[6]) {
+  constexpr int L = P + 1, LP = odd(L), HB = L * LP;
+  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
+  double s = 0.0;
+  s = fma(c[4] * B0[l], H[4 * HB + m * LP + n], s);
+  s = fma(c[2] * B1[l], H[2 * HB + m * LP + n], s);
+  s = fma(c[1] * B0[m], H[1 * HB + l * LP + n], s);
+  s = fma(c[3] * B1[m], H[3 * HB + l * LP + n], s);
+  s = fma(c[0] * B0[n], H[0 * HB + l * LP + m], s);
+  s = fma(c[5] * B1[n], H[5 * HB + l * LP + m], s);
+  return s;
 }
 
 struct StageArgs {
@@ -112,7 +203,7 @@ __global__ void __launch_bounds__(256)
 k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   using S = S3<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
-                NT = S::NT, NPT = S::NPT, NSL = S::NSL;
+                LP = S::LP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSL = S::NSL;
   extern __shared__ double sm[];
   double *A = sm, *B = A + S::NA, *gF = B + S::NA;
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
@@ -121,16 +212,18 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   // modal state -> nodal registers
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3, m = i - v * L3;
-    A[i] = a.U[((size_t)v * ne + e) * L3 + m];
+    A[(m / L + v * L2) * LP + (m % L)] = a.U[((size_t)v * ne + e) * L3 + m];
   }
   __syncthreads();
   backward_batch<P, 5>(A, B, tid);
+  constexpr int FS = Q * Q * S::QP;   // padded nodal field size
   double un[NPT][5];
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
     const int n = tid + r * NT;
+    const int pn = S::nidx(n < Q3 ? n : 0);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) un[r][v] = n < Q3 ? B[v * Q3 + n] : 1.0;
+    for (int v = 0; v < 5; ++v) un[r][v] = n < Q3 ? B[v * FS + pn] : 1.0;
   }
 
   // face corrections g = (J_side/J_vol) (Fhat - F(U-).n) [- BR1 viscous part]
@@ -152,6 +245,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
     euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
     bad |= bl | br;
     const double lmax = fmax(lL, lR);
+    const int qa = q / QF, qb = q - qa * QF;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
@@ -161,7 +255,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
         const double *vr = a.TV + ((size_t)nb * 6 + ns) * 4 * QF2;
         g += 0.5 * jr * (vl[(v - 1) * QF2 + q] + vr[(v - 1) * QF2 + qn]);
       }
-      gF[(v * 6 + s) * QF2 + q] = g;
+      gF[((v * 6 + s) * QF + qa) * FP + qb] = g;
     }
   }
   if (bad) atomicOr(a.flag, 1);
@@ -187,12 +281,13 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
       if (n < Q3) {
+        const int pn = S::nidx(n);
         const double vn = m0 * ux[r] + m1 * uy[r] + m2 * uz[r];   // contravariant velocity
-        A[n] = un[r][0] * vn;
-        A[Q3 + n] = un[r][1] * vn + m0 * pr[r];
-        A[2 * Q3 + n] = un[r][2] * vn + m1 * pr[r];
-        A[3 * Q3 + n] = un[r][3] * vn + m2 * pr[r];
-        A[4 * Q3 + n] = (un[r][4] + pr[r]) * vn;
+        A[pn] = un[r][0] * vn;
+        A[FS + pn] = un[r][1] * vn + m0 * pr[r];
+        A[2 * FS + pn] = un[r][2] * vn + m1 * pr[r];
+        A[3 * FS + pn] = un[r][3] * vn + m2 * pr[r];
+        A[4 * FS + pn] = (un[r][4] + pr[r]) * vn;
       }
     }
     __syncthreads();
@@ -202,11 +297,8 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   vol_dir(IC<1>());
   vol_dir(IC<2>());
 
-  // faces: H = (PW (x) PW) gF for 5 fields x 6 faces
-  contract<30, QF, QF, 1, L, NT>(CT<P>::PW(), gF, A, tid);
-  __syncthreads();
-  contract<30, L, QF, 2, L, NT>(CT<P>::PW(), A, B, tid);
-  __syncthreads();
+  // faces: H = (PW (x) PW) gF for 5 fields x 6 faces, in B
+  face_project<P, 30>(gF, A, B, tid);
   const double one6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
 #pragma unroll
   for (int k = 0; k < NSL; ++k) {
@@ -215,7 +307,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
       const int v = G / L2, pp = G - v * L2, m = pp / L, nn = pp - m * L;
 #pragma unroll
       for (int l = 0; l < L; ++l) {
-        double r = acc[k][l] + face_add3<P>(CT<P>::Bend(), B + v * 6 * L2, l, m, nn, one6);
+        double r = acc[k][l] + face_sum<P>(B + v * 6 * L * LP, l, m, nn, one6);
         if (NS && v > 0) r -= a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp];
         const size_t gi = ((size_t)v * ne + e) * L3 + l * L2 + pp;
         if (!UPDATE) {
@@ -238,47 +330,49 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   for (int k = 0; k < NSL; ++k) {
     const int G = tid + k * NT;
     if (G < 5 * L2) {
-      const int v = G / L2, pp = G - v * L2;
+      const int v = G / L2, pp = G - v * L2, m = pp / L, nn = pp - m * L;
 #pragma unroll
-      for (int l = 0; l < L; ++l) A[v * L3 + l * L2 + pp] = acc[k][l];
+      for (int l = 0; l < L; ++l) A[((v * L + l) * L + m) * LP + nn] = acc[k][l];
     }
   }
   __syncthreads();
-  double *K0 = B, *K1 = B + 10 * L2, *K2 = B + 20 * L2, *W = A, *O = A + 10 * QF * L;
-  contract<5, L, L2, 1, 2, NT>(CT<P>::Bend(), A, K0, tid);        // [5][2][L][L]
-  contract<5 * L, L, L, 1, 2, NT>(CT<P>::Bend(), A, K1, tid);     // [5][L][2][L]
-  contract<1, 5 * L2, L, 2, 2, NT>(CT<P>::Bend(), A, K2, tid);    // [5][L][L][2]
+  constexpr int KP = 3;   // pitch of the [..][2] collapse
+  double *K0 = B, *K1 = B + 10 * L * LP, *K2 = B + 20 * L * LP;
+  con_mid<5, L, L, L, 2, LP, LP, NT>(CT<P>::Bend(), A, K0, tid);       // [5][2][L][L]
+  con_mid<5 * L, L, 1, L, 2, LP, LP, NT>(CT<P>::Bend(), A, K1, tid);   // [5][L][2][L]
+  con_last<5 * L2, L, 2, LP, KP, NT>(CT<P>::Bend(), A, K2, tid);       // [5][L][L][2]
   __syncthreads();
+  double *W = A, *O = A + 10 * QF * cmax(LP, KP * L);
   double *Te = a.Tn + (size_t)e * 6 * 5 * QF2;
   // faces 4 / 2: [5][2][QF][QF]
-  contract<10, L, L, 1, QF, NT>(CT<P>::Bg(), K0, W, tid);
+  con_mid<10, L, 1, L, QF, LP, LP, NT>(CT<P>::Bg(), K0, W, tid);      // [10][QF][L]
   __syncthreads();
-  contract<10, QF, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
+  con_last<10 * QF, L, QF, LP, FP, NT>(CT<P>::Bg(), W, O, tid);       // [10][QF][QF]
   __syncthreads();
   for (int i = tid; i < 10 * QF2; i += NT) {
     const int ve = i / QF2, q = i - ve * QF2, v = ve >> 1, end = ve & 1;
-    Te[((end ? 2 : 4) * 5 + v) * QF2 + q] = O[i];
+    Te[((end ? 2 : 4) * 5 + v) * QF2 + q] = O[(ve * QF + q / QF) * FP + q % QF];
   }
   __syncthreads();
   // faces 1 / 3: [5][QF][2][QF]
-  contract<5, L, 2 * L, 1, QF, NT>(CT<P>::Bg(), K1, W, tid);
+  con_mid<5, L, 2, L, QF, LP, LP, NT>(CT<P>::Bg(), K1, W, tid);       // [5][QF][2][L]
   __syncthreads();
-  contract<1, 10 * QF, L, 2, QF, NT>(CT<P>::Bg(), W, O, tid);
+  con_last<10 * QF, L, QF, LP, FP, NT>(CT<P>::Bg(), W, O, tid);       // [5][QF][2][QF]
   __syncthreads();
   for (int i = tid; i < 10 * QF2; i += NT) {
-    const int v = i / (2 * QF2), r = i - v * 2 * QF2, ia = r / (2 * QF), r2 = r - ia * 2 * QF;
-    const int end = r2 / QF, ic = r2 - end * QF;
-    Te[((end ? 3 : 1) * 5 + v) * QF2 + ia * QF + ic] = O[i];
+    const int v = i / (2 * QF2), r = i - v * 2 * QF2, end = r / QF2, q = r - end * QF2;
+    const int ia = q / QF, ic = q - ia * QF;
+    Te[((end ? 3 : 1) * 5 + v) * QF2 + q] = O[((v * QF + ia) * 2 + end) * FP + ic];
   }
   __syncthreads();
   // faces 0 / 5: [5][QF][QF][2]
-  contract<5, L, 2 * L, 1, QF, NT>(CT<P>::Bg(), K2, W, tid);
+  con_mid<5, L, L, 2, QF, KP, KP, NT>(CT<P>::Bg(), K2, W, tid);       // [5][QF][L][2]
   __syncthreads();
-  contract<5 * QF, L, 2, 1, QF, NT>(CT<P>::Bg(), W, O, tid);
+  con_mid<5 * QF, L, 1, 2, QF, KP, KP, NT>(CT<P>::Bg(), W, O, tid);   // [5][QF][QF][2]
   __syncthreads();
   for (int i = tid; i < 10 * QF2; i += NT) {
-    const int v = i / (2 * QF2), r = i - v * 2 * QF2, ab = r >> 1, end = r & 1;
-    Te[((end ? 5 : 0) * 5 + v) * QF2 + ab] = O[i];
+    const int v = i / (2 * QF2), r = i - v * 2 * QF2, end = r / QF2, q = r - end * QF2;
+    Te[((end ? 5 : 0) * 5 + v) * QF2 + q] = O[(v * QF2 + q) * KP + end];
   }
 }
 
@@ -289,15 +383,16 @@ __global__ void __launch_bounds__(256)
 k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
          double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
   using S = S3<P>;
-  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3,
-                QF2 = S::QF2, NT = S::NT, NPT = S::NPT, NSG = S::NSG;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
+                LP = S::LP, QP = S::QP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSG = S::NSG;
   extern __shared__ double sm[];
   double *A = sm, *B = A + S::NA, *Jq = B + S::NA;
   const int e = blockIdx.x, tid = threadIdx.x;
+  constexpr int FS = Q * Q * QP;
 
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3, m = i - v * L3;
-    A[i] = U[((size_t)v * ne + e) * L3 + m];
+    A[(m / L + v * L2) * LP + (m % L)] = U[((size_t)v * ne + e) * L3 + m];
   }
   // primitive jumps J_side/J_vol ({q} - q-) at the Gauss points (average_flux '+')
   for (int i = tid; i < 6 * QF2; i += NT) {
@@ -321,8 +416,9 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
       double wk = u * u; wk = v * v + wk; wk = w * w + wk;
       pq[0] = u; pq[1] = v; pq[2] = w; pq[3] = -0.5 * wk + en;
     }
+    const int qa = q / QF, qb = q - qa * QF;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) Jq[(c * 6 + s) * QF2 + q] = jr * (0.5 * (pl[c] + pq[c]) - pl[c]);
+    for (int c = 0; c < 4; ++c) Jq[((c * 6 + s) * QF + qa) * FP + qb] = jr * (0.5 * (pl[c] + pq[c]) - pl[c]);
   }
   __syncthreads();
   backward_batch<P, 5>(A, B, tid);
@@ -330,10 +426,11 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
     const int n = tid + r * NT;
-    const double rho = n < Q3 ? B[n] : 1.0;
-    const double u = n < Q3 ? B[Q3 + n] / rho : 0.0, v = n < Q3 ? B[2 * Q3 + n] / rho : 0.0;
-    const double w = n < Q3 ? B[3 * Q3 + n] / rho : 0.0;
-    const double en = n < Q3 ? B[4 * Q3 + n] / rho : 0.0;
+    const int pn = S::nidx(n < Q3 ? n : 0);
+    const double rho = n < Q3 ? B[pn] : 1.0;
+    const double u = n < Q3 ? B[FS + pn] / rho : 0.0, v = n < Q3 ? B[2 * FS + pn] / rho : 0.0;
+    const double w = n < Q3 ? B[3 * FS + pn] / rho : 0.0;
+    const double en = n < Q3 ? B[4 * FS + pn] / rho : 0.0;
     double wk = u * u; wk = v * v + wk; wk = w * w + wk;
     pv[r][0] = u; pv[r][1] = v; pv[r][2] = w; pv[r][3] = -0.5 * wk + en;
   }
@@ -353,9 +450,11 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
-      if (n < Q3)
+      if (n < Q3) {
+        const int pn = S::nidx(n);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) A[c * Q3 + n] = pv[r][c];
+        for (int c = 0; c < 4; ++c) A[c * FS + pn] = pv[r][c];
+      }
     }
     __syncthreads();
     double D[NSG][L];
@@ -374,10 +473,7 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   grad_dir(IC<0>());
   grad_dir(IC<1>());
   grad_dir(IC<2>());
-  contract<24, QF, QF, 1, L, NT>(CT<P>::PW(), Jq, A, tid);
-  __syncthreads();
-  contract<24, L, QF, 2, L, NT>(CT<P>::PW(), A, B, tid);
-  __syncthreads();
+  face_project<P, 24>(Jq, A, B, tid);   // H in B: [4][6][L][L]
 #pragma unroll
   for (int k = 0; k < NSG; ++k) {
     const int Gi = tid + k * NT;
@@ -389,7 +485,7 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
         for (int s = 0; s < 6; ++s) cf[s] = geo.fnorm[3 * (e * 6 + s) + x];
 #pragma unroll
-        for (int l = 0; l < L; ++l) G[k][x][l] += face_add3<P>(CT<P>::Bend(), B + c * 6 * L2, l, m, nn, cf);
+        for (int l = 0; l < L; ++l) G[k][x][l] += face_sum<P>(B + c * 6 * L * LP, l, m, nn, cf);
       }
     }
   }
@@ -403,9 +499,9 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     for (int k = 0; k < NSG; ++k) {
       const int Gi = tid + k * NT;
       if (Gi < 4 * L2) {
-        const int c = Gi / L2, pp = Gi - c * L2;
+        const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
 #pragma unroll
-        for (int l = 0; l < L; ++l) A[(c * L + l) * L2 + pp] = G[k][x][l];
+        for (int l = 0; l < L; ++l) A[((c * L + l) * L + m) * LP + nn] = G[k][x][l];
       }
     }
     __syncthreads();
@@ -413,8 +509,9 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
+      const int pn = S::nidx(n < Q3 ? n : 0);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) gn[r][3 * c + x] = n < Q3 ? B[c * Q3 + n] : 0.0;
+      for (int c = 0; c < 4; ++c) gn[r][3 * c + x] = n < Q3 ? B[c * FS + pn] : 0.0;
     }
     __syncthreads();
   };
@@ -443,34 +540,38 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     sg[r][11] = t13 * u + t23 * v + t33 * w + kap * gn[r][11];
   }
 
-  // normal viscous flux on the GLL face layers -> Gauss points
-  double *Lay = A;   // [6][4][Q2]
+  // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss points
+  double *Lay = A;
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
     const int n = tid + r * NT;
     if (n < Q3) {
-      const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
-      int fs[6], li[6], cnt = 0;
-      if (k == 0) { fs[cnt] = 0; li[cnt++] = i * Q + j; }
-      if (k == Q - 1) { fs[cnt] = 5; li[cnt++] = i * Q + j; }
-      if (j == 0) { fs[cnt] = 1; li[cnt++] = i * Q + k; }
-      if (j == Q - 1) { fs[cnt] = 3; li[cnt++] = i * Q + k; }
-      if (i == 0) { fs[cnt] = 4; li[cnt++] = j * Q + k; }
-      if (i == Q - 1) { fs[cnt] = 2; li[cnt++] = j * Q + k; }
+      const int i = n / (Q * Q), j = (n / Q) % Q, k = n % Q;
+      int fs[6], l1[6], l2[6], cnt = 0;
+      if (k == 0) { fs[cnt] = 0; l1[cnt] = i; l2[cnt++] = j; }
+      if (k == Q - 1) { fs[cnt] = 5; l1[cnt] = i; l2[cnt++] = j; }
+      if (j == 0) { fs[cnt] = 1; l1[cnt] = i; l2[cnt++] = k; }
+      if (j == Q - 1) { fs[cnt] = 3; l1[cnt] = i; l2[cnt++] = k; }
+      if (i == 0) { fs[cnt] = 4; l1[cnt] = j; l2[cnt++] = k; }
+      if (i == Q - 1) { fs[cnt] = 2; l1[cnt] = j; l2[cnt++] = k; }
       for (int c = 0; c < cnt; ++c) {
         const double *nn = geo.fnorm + 3 * (e * 6 + fs[c]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          Lay[(fs[c] * 4 + q) * Q2 + li[c]] = sg[r][3 * q] * nn[0] + sg[r][3 * q + 1] * nn[1] + sg[r][3 * q + 2] * nn[2];
+          Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
+              sg[r][3 * q] * nn[0] + sg[r][3 * q + 1] * nn[1] + sg[r][3 * q + 2] * nn[2];
       }
     }
   }
   __syncthreads();
-  contract<24, Q, Q, 1, QF, NT>(CT<P>::Ig(), Lay, B, tid);
+  con_mid<24, Q, 1, Q, QF, QP, QP, NT>(CT<P>::Ig(), Lay, B, tid);   // [24][QF][Q]
   __syncthreads();
-  contract<24, QF, Q, 2, QF, NT>(CT<P>::Ig(), B, A, tid);
+  con_last<24 * QF, Q, QF, QP, FP, NT>(CT<P>::Ig(), B, A, tid);     // [24][QF][QF]
   __syncthreads();
-  for (int i = tid; i < 24 * QF2; i += NT) TV[(size_t)e * 24 * QF2 + i] = A[i];
+  for (int i = tid; i < 24 * QF2; i += NT) {
+    const int sq = i / QF2, q = i - sq * QF2;
+    TV[(size_t)e * 24 * QF2 + i] = A[(sq * QF + q / QF) * FP + q % QF];
+  }
   __syncthreads();
 
   // viscous volume divergence of the 4 equations
@@ -485,10 +586,12 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
-      if (n < Q3)
+      if (n < Q3) {
+        const int pn = S::nidx(n);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          A[q * Q3 + n] = m0 * sg[r][3 * q] + m1 * sg[r][3 * q + 1] + m2 * sg[r][3 * q + 2];
+          A[q * FS + pn] = m0 * sg[r][3 * q] + m1 * sg[r][3 * q + 1] + m2 * sg[r][3 * q + 2];
+      }
     }
     __syncthreads();
     div_chain<P, 4, NSG, d>(A, B, acc, tid);
