@@ -10,12 +10,21 @@ A step is one LSRK14 step of the whole domain with the state resident in HBM.
 value = modal DOF x stages / wall time of the timed steps.
 
 Also printed in the same JSON line:
-  roofline      dominant kernel's algorithmic HBM bytes per launch (32 B per
-                DOF-stage: read U, read Ut, write Ut, write U of a fused 2N-storage
-                stage) / its mean launch time from HIP events on the library's
-                stream, against MI355X's 8 TB/s
-  cpu_baseline  the CPU oracle (1 core) on a bounded sample of the same
-                workload (same p and physics on a 3^3 periodic box)
+  roofline      the dominant kernel (largest share of a step's time, from HIP
+                events on the library's stream) against the FP64 compute
+                roofline: its algorithmic flops per launch / its mean launch
+                time, against MI355X's 78.6 TFLOP/s FP64 peak.  Algorithmic
+                flops per DOF-stage at p = 7 (minimal sum-factorized operator):
+                787 in all, 256 of them in the inviscid residual (k_stage3), the
+                other 531 in the BR1 viscous terms (k_vterm3).  `traffic` is the
+                kernel's HBM bytes per launch measured with rocprofv3 FETCH_SIZE
+                and WRITE_SIZE (scaled by a plain copy of known size).
+  roofline_hbm  the HBM view of a whole stage: 32 B per DOF-stage (read U,
+                read Ut, write Ut, write U of a fused 2N-storage stage) / the
+                mean stage time, against 8 TB/s
+  cpu_baseline  the CPU oracle on the host cores (one worker per core, at
+                most 16) on a bounded sample of the same workload: same p and
+                physics on a 3^3 periodic box per worker, ~15 s
 """
 from __future__ import annotations
 
@@ -35,7 +44,14 @@ from dgm_amd import dgx, ic, mesh as meshgen  # noqa: E402
 
 METRIC = ("DG DOF-stage updates/s, mixed vs FP64 at ≤1e-6 rel err; p50/p99 solve latency")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_DOF_STAGE = 32.0
+# algorithmic flops per DOF-stage by kernel kind, per order (3D NS, BR1)
+FLOPS_PER_DOF_STAGE = {7: {"residual": 256.0, "visc": 531.0}}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
+# (profiles/tgv_ns3d_32cubed_p7_fp64.md)
+MEASURED_TRAFFIC = {(32, 7, "visc"): 2.50e9, (32, 7, "residual"): 5.93e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 
 
@@ -50,8 +66,9 @@ def tgv_dt(n, p, gamma=1.4, Ma=0.1, cfl=0.5):
     return cfl * h / ((p + 1) ** 2 * c)
 
 
-def cpu_baseline(p, mu, seconds=15.0):
-    """Time the oracle's LSRK14 on a 3^3-element box of the same order/physics."""
+def _oracle_worker(args):
+    """One CPU worker: the oracle's LSRK14 on its own 3^3-element box for `seconds`."""
+    p, mu, seconds = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
     n = 3
@@ -67,10 +84,28 @@ def cpu_baseline(p, mu, seconds=15.0):
     while time.perf_counter() - t0 < seconds:
         orc.advance(14, dt, 1)
         steps += 1
-    el = time.perf_counter() - t0
-    return {"value": dofs * 14 * steps / el, "unit": "DOF-stage/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle, NS3d TGV {n}^3 hexes p={p}, {steps} LSRK14 steps "
-                      f"({14 * steps} residuals) in {el:.1f} s on 1 thread"}
+    return dofs * 14 * steps, time.perf_counter() - t0, steps
+
+
+def cpu_baseline(p, mu, seconds=15.0, workers=None):
+    """The oracle on the host cores: one independent worker per core (the
+    reference's one-MPI-rank-per-core layout with single-threaded BLAS, here
+    without halo exchange, so an upper bound of its throughput), run in spawned
+    processes before this process touches the GPU."""
+    import multiprocessing as mp
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    workers = workers or max(1, min(16, ncpu))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_oracle_worker, [(p, mu, seconds)] * workers)
+    total = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    per_core = [r[0] / r[1] for r in res]
+    steps = sum(r[2] for r in res)
+    return {"value": total / wall, "unit": "DOF-stage/s", "cores": workers, "kind": "port",
+            "per_core_mean": float(np.mean(per_core)),
+            "sample": f"CPU oracle (C++ restatement, FP64), NS3d TGV on {workers} independent 3^3-hex "
+                      f"boxes at p={p}, one per core, {steps} LSRK14 steps in all over {wall:.1f} s "
+                      f"(first step excluded)"}
 
 
 def main():
@@ -82,6 +117,7 @@ def main():
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline workers (0: one per core, at most 16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -92,6 +128,7 @@ def main():
         sys.exit(2)
 
     n, p, mu = args.n, args.p, 1.0 / 1600.0
+    cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers)
     m = tgv_domain(n, p, mu)
     gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71)
     x = gpu.coords()
@@ -128,8 +165,23 @@ def main():
                        "ms_mean": (ms[k] / cnt[k]) if cnt[k] else 0.0} for k in range(7)}
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     mean_s = kern[dom]["ms_mean"] / 1e3
+    flops = FLOPS_PER_DOF_STAGE.get(p, {}).get(dom)
+    if flops is not None:
+        algo_flops = flops * dofs
+        achieved = algo_flops / mean_s / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                    "traffic": MEASURED_TRAFFIC.get((n, p, dom)), "kernel": dom,
+                    "arithmetic": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                    "algorithmic_flops_per_launch": algo_flops, "ms_mean": kern[dom]["ms_mean"]}
+    else:
+        roofline = None
+    stage_s = sum(kern[k]["ms_total"] for k in kern) / 1e3 / (14 * prof_steps)
     algo_bytes = BYTES_PER_DOF_STAGE * dofs
-    achieved = algo_bytes / mean_s / 1e9
+    hbm = algo_bytes / stage_s / 1e9
+    roofline_hbm = {"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm / HBM_PEAK_GBS, "algorithmic_bytes_per_stage": algo_bytes,
+                    "ms_per_stage": stage_s * 1e3}
     norms = gpu.norms()
     finite = bool(np.all(np.isfinite(norms)))
     if args.copy_calibration:
@@ -157,15 +209,13 @@ def main():
                                f"p={p}, Re=1600, Ma=0.1, Pr=0.71, LF + BR1, LSRK14, FP64",
                    "dofs": dofs, "stages_per_step": 14, "dt": dt, "precision": "fp64",
                    "state_finite": finite},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                     "algorithmic_bytes_per_launch": algo_bytes,
-                     "stage_rate_GBs": value * BYTES_PER_DOF_STAGE / 1e9},
+        "roofline": roofline,
+        "roofline_hbm": roofline_hbm,
         "kernels": kern,
     }
     gpu.close()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, mu, args.cpu_seconds)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

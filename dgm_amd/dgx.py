@@ -122,6 +122,9 @@ class GPUDomain:
 
     def upload(self, U, layout=DGX_NODAL):
         U = np.ascontiguousarray(U, dtype=np.float64)
+        n = self.qtot if layout == DGX_NODAL else self.nm
+        if U.shape != (self.nf, self.ne, n):
+            raise ValueError(f"state shape {U.shape} != {(self.nf, self.ne, n)}")
         _check(self.lib.dgx_state_upload(self.h, _ptr(U), layout, 0))
 
     def download(self, layout=DGX_NODAL):

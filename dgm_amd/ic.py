@@ -12,6 +12,8 @@ the coordinate arrays.  Reference counterparts:
 * ``isentropic_vortex`` the 2D vortex of the C1 workload: freestream plus a
   Gaussian isentropic vortex of given strength (Shu's form)
 * ``fourier_modes`` smooth seeded perturbation about a uniform state
+* ``from_code``      dispatch on a parsed ``root.ic`` block (Euler::set_ic,
+  src/Euler.cpp:880-1190)
 """
 from __future__ import annotations
 
@@ -111,3 +113,17 @@ def fourier_modes(coords, lengths, nmodes, amp, rng):
             arg = arg + 2 * np.pi * k * c / L
         out += np.sin(arg)
     return amp * out / max(nmodes, 1)
+
+
+def from_code(icd, x, gamma=1.4):
+    """Evaluate a parsed ``root.ic`` block (dgm_amd.formats.read_ic) at nodal coordinates ``x``."""
+    code = icd.code.lower()
+    if code == "vortex":
+        n = int(icd.values[0])
+        vs = [icd.values[1 + 5 * i:6 + 5 * i] for i in range(n)]
+        return oseen_vortex(x[0], x[1], vs, gamma)
+    if code == "acousticpulse":
+        return acoustic_pulse(x[0], *icd.values, gamma=gamma)
+    if code == "given":
+        return given_constant(np.shape(x[0]), icd.values)
+    raise ValueError(f"unsupported initial condition code {icd.code}")

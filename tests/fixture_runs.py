@@ -12,16 +12,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def initial_state(icd, x, nsd, gamma=1.4):
-    code = icd.code.lower()
-    if code == "vortex":
-        n = int(icd.values[0])
-        vs = [icd.values[1 + 5 * i:6 + 5 * i] for i in range(n)]
-        return ic.oseen_vortex(x[0], x[1], vs, gamma)
-    if code == "acousticpulse":
-        return ic.acoustic_pulse(x[0], *icd.values, gamma=gamma)
-    if code == "given":
-        return ic.given_constant(x[0].shape, icd.values)
-    raise ValueError(code)
+    return ic.from_code(icd, x, gamma)
 
 
 def load_case(root):

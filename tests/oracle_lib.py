@@ -81,7 +81,8 @@ class Oracle:
         return x
 
     def set_state(self, U):
-        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nv, self.ne, self.qtot)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self._check(U)
         self.lib.orc_set_state(self.h, _ptr(U))
 
     def state(self):
@@ -99,10 +100,17 @@ class Oracle:
 
     def set_modal(self, Uh):
         Uh = np.ascontiguousarray(Uh, dtype=np.float64)
+        if Uh.shape != (self.nv, self.ne, self.nm):
+            raise ValueError(f"modal shape {Uh.shape} != {(self.nv, self.ne, self.nm)}")
         self.lib.orc_set_modal(self.h, _ptr(Uh))
+
+    def _check(self, U):
+        if U.shape != (self.nv, self.ne, self.qtot):
+            raise ValueError(f"state shape {U.shape} != {(self.nv, self.ne, self.qtot)}")
 
     def rhs(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
+        self._check(U)
         F = np.zeros_like(U)
         st = self.lib.orc_rhs(self.h, _ptr(U), _ptr(F))
         if st:

@@ -1,0 +1,83 @@
+"""libdgx.so loads and exports the C ABI of include/dgx.h (no GPU needed).
+
+Only argument validation runs here: every call below is rejected before the
+library touches a device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from dgm_amd import dgx, mesh as meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dgx.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(dgx_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(dgx.LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "dgm_amd", "csrc"), "-j8"])
+    return dgx.load()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _declared()
+    assert "dgx_rhs" in names and "dgx_advance" in names and len(names) >= 15
+    raw = ctypes.CDLL(dgx.LIB_PATH)
+    missing = [n for n in names if not hasattr(raw, n)]
+    assert not missing, missing
+
+
+def test_version_string(lib):
+    import dgm_amd
+    v = lib.dgx_version().decode()
+    assert re.fullmatch(r"\d+\.\d+\.\d+", v)
+    assert v == dgm_amd.__version__
+
+
+def _desc(m, p, eqn, **kw):
+    vert = np.ascontiguousarray(m.vert, dtype=np.float64)
+    con = np.ascontiguousarray(m.con, dtype=np.int32)
+    d = dgx.DomainDesc(nsd=m.nsd, ne=m.ne, p=p, eqntype=eqn, flux=kw.get("flux", 0),
+                       precision=kw.get("precision", 0), gamma=1.4, vis=kw.get("vis", 0.0),
+                       Pr=0.72, vert=vert.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                       ncon=len(con), con=con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return d, (vert, con)
+
+
+@pytest.mark.parametrize("case,msg", [
+    (dict(p=0), b"p must be"),
+    (dict(p=11), b"p must be"),
+    (dict(eqn=1), b"eqntype"),
+    (dict(flux=1), b"Roe"),
+    (dict(p=8, eqn=9), b"p <= 7"),
+])
+def test_create_rejects_bad_arguments(lib, case, msg):
+    m = meshgen.periodic_hexes(2, 2, 2)
+    d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0))
+    h = ctypes.c_void_p()
+    rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
+    assert rc == -1
+    assert msg in lib.dgx_last_error()
+    assert not h.value
+
+
+def test_null_arguments_rejected(lib):
+    h = ctypes.c_void_p()
+    assert lib.dgx_domain_create(None, None, ctypes.byref(h)) == -1
+    assert lib.dgx_rhs(None) == -1
+    assert lib.dgx_advance(None, 14, 1e-3, 1) == -1
+    assert lib.dgx_domain_destroy(None) == 0
