@@ -120,6 +120,17 @@ __device__ __forceinline__ double face_add3(const double *__restrict__ Bend, con
   return s;
 }
 
+// Element of workgroup i.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2; this bijection of [0, ne) gives XCD x the
+// contiguous block of elements x*q + min(x, r) + [0, q + (x < r)) (ne = 8q + r),
+// so that face neighbours (which read each other's traces) mostly share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_element(int i, int ne) {
+  const int q = ne / kXcds, r = ne - q * kXcds;
+  const int x = i % kXcds, j = i / kXcds;
+  return x * q + (x < r ? x : r) + j;
+}
+
 // Copy n doubles global -> LDS (or LDS -> global) with the workgroup.
 template <int NT>
 __device__ __forceinline__ void copy_block(const double *__restrict__ src, double *dst, int n,

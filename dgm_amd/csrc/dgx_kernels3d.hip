@@ -91,7 +91,7 @@ k_traces3(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
   extern __shared__ double sm[];
   double *A = sm, *K0 = A + L3, *K1 = K0 + 2 * L2, *K2 = K1 + 2 * L2;
   double *W = K2 + 2 * L2, *O = W + 2 * QF * L;
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   for (int v = 0; v < nf; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L3, A, L3, tid);
     __syncthreads();
@@ -151,7 +151,7 @@ k_rhs3(const double *__restrict__ U, const double *__restrict__ T, const double 
   double *X = Fb + Q3;             // [SX]
   double *Y = X + S::SX;           // [SY]
   double *Y1 = Y, *Y2 = Y + Q * L2;
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const double gm1 = ph.gamma - 1.0;
 
   for (int v = 0; v < 5; ++v) {
@@ -252,7 +252,7 @@ k_grad3(const double *__restrict__ U, const double *__restrict__ T, double *__re
   double *Ya = Xc + S::SX;         // [Q L2] x 3
   double *Yb = Ya + Q * L2, *Yc = Yb + Q * L2;
   double *D3 = Yc + Q * L2;        // [3][L3]
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
 
   for (int v = 0; v < 5; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L3, Fb, L3, tid);
@@ -348,7 +348,7 @@ k_visc3(const double *__restrict__ U, const double *__restrict__ G, double *__re
   double *Un = sm;               // [5][Q3]
   double *Gn = Un + 5 * Q3;      // [12][Q3]
   double *Sc = Gn + 12 * Q3;     // scratch
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   // backward transforms (scratch: Sc[0:L3], Sc+L3 [Q L2], then [Q2 L])
   double *m0 = Sc, *s1 = Sc + L3, *s2 = s1 + Q * L2;
   for (int v = 0; v < 17; ++v) {

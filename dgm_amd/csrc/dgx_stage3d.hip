@@ -24,6 +24,7 @@
 namespace dgx {
 
 constexpr int odd(int n) { return n | 1; }
+template <int V> struct IC { static constexpr int value = V; };
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <int P> struct S3 {
@@ -47,8 +48,6 @@ template <int P> struct S3 {
   // padded position of nodal index n = (i*Q + j)*Q + k
   static __device__ __forceinline__ int nidx(int n) { return (n / Q) * QP + (n - (n / Q) * Q); }
 };
-
-template <int V> struct IC { static constexpr int value = V; };
 
 // Coefficient rows per pass (about 40 doubles live in scalar registers).
 template <int NI> constexpr int out_chunk() { return NI * 3 <= 40 ? 3 : 2; }
@@ -206,7 +205,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
                 LP = S::LP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSL = S::NSL;
   extern __shared__ double sm[];
   double *A = sm, *B = A + S::NA, *gF = B + S::NA;
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const double gm1 = ph.gamma - 1.0;
 
   // modal state -> nodal registers
@@ -387,7 +386,7 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
                 LP = S::LP, QP = S::QP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSG = S::NSG;
   extern __shared__ double sm[];
   double *A = sm, *B = A + S::NA, *Jq = B + S::NA;
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   constexpr int FS = Q * Q * QP;
 
   for (int i = tid; i < 5 * L3; i += NT) {

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdgx.so")
+LIB_PATH = os.environ.get("DGX_LIBRARY") or os.path.join(HERE, "libdgx.so")
 
 DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D = 1, 2, 8, 9
 DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
