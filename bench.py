@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=32, help="elements per direction")
     ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline workers (0: one per core, at most 16)")
@@ -130,7 +131,8 @@ def main():
     n, p, mu = args.n, args.p, 1.0 / 1600.0
     cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers)
     m = tgv_domain(n, p, mu)
-    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71)
+    prec = dgx.DGX_MIXED if args.precision == "mixed" else dgx.DGX_FP64
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=prec)
     x = gpu.coords()
     U = ic.taylor_green(x[0], x[1], x[2])
     del x

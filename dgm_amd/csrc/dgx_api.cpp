@@ -158,7 +158,8 @@ int upload_all_tables() {
   hipError_t e;
   if ((e = dgx::upload_tables_2d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_3d(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -177,6 +178,7 @@ struct dgx_domain {
   int nsd = 0, ne = 0, p = 0, L = 0, Q = 0, QF = 0, nf = 0, nm = 0, nq = 0, ns = 0, qs = 0;
   int eqn = 0, flux = 0, prec = 0;
   bool visc = false;
+  bool mixed = false;   // DGX_MIXED: FP32 MFMA contractions (dgx_mixed3d.hip)
   double gamma = 1.4, vis = 0.0, Pr = 0.72;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -189,6 +191,7 @@ struct dgx_domain {
   uint16_t *d_perm = nullptr;
   double *U = nullptr, *Ut = nullptr, *R0 = nullptr, *R1 = nullptr, *T = nullptr;
   double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr, *T2 = nullptr;
+  float *Vf = nullptr, *TVf = nullptr;   // viscous terms of the mixed path
   bool fused = false;   // hex fused stage kernels available for this order
   dgx::Tab tab;
   dgx::Geo geo;
@@ -252,7 +255,8 @@ dgx::StageLaunch stage_args(dgx_domain *d) {
 }
 
 int launch_stage(dgx_domain *d, const dgx::StageLaunch &s) {
-  hipError_t e = timed(d, s.op == 0 ? dgx::OP_VISC : dgx::OP_RHS, [&] { return dgx::launch_stage3(s); });
+  hipError_t e = timed(d, s.op == 0 ? dgx::OP_VISC : dgx::OP_RHS,
+                       [&] { return d->mixed ? dgx::launch_mixed3(s) : dgx::launch_stage3(s); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("stage launch: ") + hipGetErrorString(e));
   return DGX_OK;
 }
@@ -261,6 +265,7 @@ int launch_stage(dgx_domain *d, const dgx::StageLaunch &s) {
 int vterm(dgx_domain *d, const double *U, const double *T) {
   dgx::StageLaunch s = stage_args(d);
   s.op = 0; s.U = const_cast<double *>(U); s.T = T; s.Vw = d->V; s.TVw = d->TV;
+  s.Vfw = d->Vf; s.TVfw = d->TVf;
   return launch_stage(d, s);
 }
 
@@ -272,6 +277,7 @@ int residual(dgx_domain *d, const double *U, double *R) {
     if (d->visc && (rc = vterm(d, U, d->T))) return rc;
     dgx::StageLaunch s = stage_args(d);
     s.op = 1; s.U = const_cast<double *>(U); s.T = d->T; s.V = d->V; s.TV = d->TV; s.R = R;
+    s.Vf = d->Vf; s.TVf = d->TVf;
     if ((rc = launch_stage(d, s))) return rc;
     ++d->stages;
     return DGX_OK;
@@ -529,7 +535,15 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   if ((rc = alloc(&d->T, (size_t)ne * d->ns * d->nf * qs))) return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p));
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
-  if (d->visc) {
+  d->mixed = d->prec == DGX_MIXED;
+  if (d->visc && d->mixed) {
+    const size_t nv = (size_t)ne * (d->nf - 1) * d->nm, ntv = (size_t)ne * d->ns * (d->nf - 1) * qs;
+    if (hipMalloc((void **)&d->Vf, nv * sizeof(float) + 64) != hipSuccess ||
+        hipMalloc((void **)&d->TVf, ntv * sizeof(float) + 64) != hipSuccess)
+      return fail(DGX_ERR_MEMORY, "hipMalloc of the FP32 viscous terms failed");
+    HIPCHK(hipMemset(d->Vf, 0, nv * sizeof(float)));
+    HIPCHK(hipMemset(d->TVf, 0, ntv * sizeof(float)));
+  } else if (d->visc) {
     const int ng = nsd == 2 ? 6 : 12;
     if (!d->fused && (rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
     if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * d->nm))) return rc;
@@ -541,7 +555,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2};
+                  d->T2, d->Vf, d->TVf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -566,9 +580,13 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "eqntype does not match nsd");
   if (desc->flux == DGX_FLUX_ROE && desc->nsd == 3)
     return fail(DGX_ERR_ARGUMENT, "the Roe flux is 2D only (Euler3d is Lax-Friedrichs only)");
-  if (desc->precision != DGX_FP64) return fail(DGX_ERR_ARGUMENT, "only DGX_FP64 is available");
-  if (desc->eqntype == DGX_NAVIER_STOKES3D && dgx::lds_bytes3(desc->p, 3) > 160 * 1024)
-    return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d supports p <= 7 (LDS)");
+  if (desc->precision != DGX_FP64 && desc->precision != DGX_MIXED)
+    return fail(DGX_ERR_ARGUMENT, "precision must be DGX_FP64 or DGX_MIXED");
+  if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
+    return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
+  if (desc->eqntype == DGX_NAVIER_STOKES3D && !dgx::stage3_supported(desc->p) &&
+      dgx::lds_bytes3(desc->p, 3) > 160 * 1024)
+    return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d supports p <= 9 (LDS)");
   dgx_domain *d = new dgx_domain();
   if (stream) d->stream = (hipStream_t)stream;
   else {
@@ -704,7 +722,8 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
           if (h->visc) RUN(vterm(h, h->U, h->T));
           dgx::StageLaunch sa = stage_args(h);
           sa.op = 1; sa.update = true; sa.U = h->U; sa.Ut = h->Ut; sa.T = h->T; sa.Tn = h->T2;
-          sa.V = h->V; sa.TV = h->TV; sa.ca = A[st]; sa.cb = B[st]; sa.dt = dt;
+          sa.V = h->V; sa.TV = h->TV; sa.Vf = h->Vf; sa.TVf = h->TVf;
+          sa.ca = A[st]; sa.cb = B[st]; sa.dt = dt;
           RUN(launch_stage(h, sa));
           std::swap(h->T, h->T2);
           ++h->stages;

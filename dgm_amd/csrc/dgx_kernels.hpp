@@ -1,5 +1,5 @@
 // Launch interface between the host engine (dgx_api.cpp) and the templated
-// kernels (dgx_kernels2d.hip, dgx_kernels3d.hip).
+// kernels (dgx_kernels2d.hip, dgx_kernels3d.hip, dgx_stage3d.hip, dgx_mixed3d.hip).
 #pragma once
 #include "dgx_device.hpp"
 
@@ -32,6 +32,7 @@ struct LaunchArgs {
 hipError_t upload_tables_2d(const double *h);
 hipError_t upload_tables_3d(const double *h);
 hipError_t upload_tables_s3d(const double *h);
+hipError_t upload_tables_m3d(const double *h);
 
 hipError_t launch_2d(const LaunchArgs &a);
 hipError_t launch_3d(const LaunchArgs &a);
@@ -45,6 +46,8 @@ struct StageLaunch {
   bool ns = false, update = false;
   const double *T = nullptr, *V = nullptr, *TV = nullptr;
   double *U = nullptr, *Ut = nullptr, *R = nullptr, *Tn = nullptr, *Vw = nullptr, *TVw = nullptr;
+  const float *Vf = nullptr, *TVf = nullptr;   // mixed precision: FP32 viscous terms
+  float *Vfw = nullptr, *TVfw = nullptr;
   double ca = 0.0, cb = 0.0, dt = 0.0;
   int *flag = nullptr;
   Tab tab;
@@ -54,6 +57,10 @@ struct StageLaunch {
 };
 bool stage3_supported(int p);
 hipError_t launch_stage3(const StageLaunch &s);
+
+// Mixed-precision fused hex stage kernels (dgx_mixed3d.hip), p <= 8.
+bool mixed3_supported(int p);
+hipError_t launch_mixed3(const StageLaunch &s);
 
 // Flat-array stage updates (dgx_blas.hip)
 hipError_t blas_axpy(double a, const double *x, const double *y, double *z, size_t n, hipStream_t s);
