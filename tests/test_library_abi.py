@@ -63,16 +63,27 @@ def _desc(m, p, eqn, **kw):
     (dict(p=11), b"p must be"),
     (dict(eqn=1), b"eqntype"),
     (dict(flux=1), b"Roe"),
-    (dict(p=8, eqn=9), b"p <= 7"),
+    (dict(p=10, eqn=9), b"p <= 9"),
+    (dict(p=9, eqn=8, precision=1), b"DGX_MIXED"),
+    (dict(p=3, eqn=8, precision=7), b"precision"),
 ])
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
-    d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0))
+    d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0),
+                    precision=case.get("precision", 0))
     h = ctypes.c_void_p()
     rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
     assert rc == -1
     assert msg in lib.dgx_last_error()
     assert not h.value
+
+
+def test_mixed_rejected_in_2d(lib):
+    m = meshgen.periodic_quads(2, 2)
+    d, keep = _desc(m, 3, 1, precision=1)
+    h = ctypes.c_void_p()
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert b"DGX_MIXED" in lib.dgx_last_error()
 
 
 def test_null_arguments_rejected(lib):
