@@ -1,24 +1,30 @@
 """Benchmark: DG DOF-stage updates per second on one MI355X.
 
-Default workload: 3D compressible Navier-Stokes Taylor-Green vortex on the
-periodic box [0, 2pi]^3, 32^3 affine hexes at p = 7 (8^3 modes per element,
-16.8 M modal DOF per field, 5 fields), Re = 1600 (mu = 1/1600), Ma = 0.1,
-Pr = 0.71, Lax-Friedrichs flux with BR1 viscous terms (the reference's
-Navier_Stokes3d operator), LSRK14 stepping (14 stages per step), FP64.
+Workload: 3D compressible Navier-Stokes Taylor-Green vortex on the periodic
+box [0, 2pi]^3, 32^3 affine hexes at p = 7 (8^3 modes per element, 16.8 M
+modal DOF per field, 5 fields), Re = 1600 (mu = 1/1600), Ma = 0.1, Pr = 0.71,
+Lax-Friedrichs flux with BR1 viscous terms (the reference's Navier_Stokes3d
+operator), LSRK14 stepping (14 stages per step).
+
+Both paths run on the same initial state: the FP64 path (`value`) and the
+mixed path (FP32 MFMA contractions, FP64 state and registers; `mixed`), whose
+final state is compared with the FP64 one per conserved field (relative L2 of
+the modal coefficients, the Domain::diff L2).
 
 A step is one LSRK14 step of the whole domain with the state resident in HBM.
 value = modal DOF x stages / wall time of the timed steps.
 
 Also printed in the same JSON line:
   roofline      the dominant kernel (largest share of a step's time, from HIP
-                events on the library's stream) against the FP64 compute
-                roofline: its algorithmic flops per launch / its mean launch
-                time, against MI355X's 78.6 TFLOP/s FP64 peak.  Algorithmic
-                flops per DOF-stage at p = 7 (minimal sum-factorized operator):
-                787 in all, 256 of them in the inviscid residual (k_stage3), the
-                other 531 in the BR1 viscous terms (k_vterm3).  `traffic` is the
-                kernel's HBM bytes per launch measured with rocprofv3 FETCH_SIZE
-                and WRITE_SIZE (scaled by a plain copy of known size).
+                events on the library's stream) against the compute roofline:
+                its algorithmic flops per launch / its mean launch time, against
+                MI355X's 78.6 TFLOP/s FP64 peak (157.3 TFLOP/s FP32 matrix for the
+                mixed path).  Algorithmic flops per DOF-stage at p = 7 (minimal
+                sum-factorized operator): 787 in all, 256 of them in the inviscid
+                residual kernel, the other 531 in the BR1 viscous kernel.
+                `traffic` is the kernel's HBM bytes per launch measured with
+                rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of
+                known size).
   roofline_hbm  the HBM view of a whole stage: 32 B per DOF-stage (read U,
                 read Ut, write Ut, write U of a fused 2N-storage stage) / the
                 mean stage time, against 8 TB/s
@@ -45,13 +51,16 @@ from dgm_amd import dgx, ic, mesh as meshgen  # noqa: E402
 METRIC = ("DG DOF-stage updates/s, mixed vs FP64 at ≤1e-6 rel err; p50/p99 solve latency")
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+ARITHMETIC = {"fp64": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+              "mixed": "FP32 MFMA contractions with FP64 pointwise work; peak is the FP32 matrix rate"}
 BYTES_PER_DOF_STAGE = 32.0
 # algorithmic flops per DOF-stage by kernel kind, per order (3D NS, BR1)
 FLOPS_PER_DOF_STAGE = {7: {"residual": 256.0, "visc": 531.0}}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
 # (profiles/tgv_ns3d_32cubed_p7_fp64.md)
-MEASURED_TRAFFIC = {(32, 7, "visc"): 2.50e9, (32, 7, "residual"): 5.93e9}
+MEASURED_TRAFFIC = {("fp64", 32, 7, "visc"): 2.50e9, ("fp64", 32, 7, "residual"): 5.93e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 
 
@@ -108,6 +117,75 @@ def cpu_baseline(p, mu, seconds=15.0, workers=None):
                       f"(first step excluded)"}
 
 
+def run_path(m, n, p, mu, precision, args):
+    """Time one precision on the TGV workload; returns (result dict, final modal state)."""
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision)
+    x = gpu.coords()
+    U = ic.taylor_green(x[0], x[1], x[2])
+    del x
+    gpu.upload(U)
+    del U
+    gpu.synchronize()
+    dt = tgv_dt(n, p)
+    dofs = 5 * m.ne * (p + 1) ** 3
+    name = "mixed" if precision == dgx.DGX_MIXED else "fp64"
+    print(f"[bench] {name}: TGV NS3d {n}^3 p={p}: {dofs / 1e6:.1f} M DOF, dt={dt:.3e}", file=sys.stderr,
+          flush=True)
+    gpu.advance(dgx.DGX_LSRK14, dt, args.warmup)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.advance(dgx.DGX_LSRK14, dt, args.steps)
+    gpu.synchronize()
+    el = time.perf_counter() - t0
+    value = dofs * 14 * args.steps / el
+    print(f"[bench] {name}: timed {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel times from HIP events on the library's stream (one more step)
+    ms = np.zeros(7)
+    cnt = np.zeros(7, dtype=np.int32)
+    prof_steps = 1
+    rc = gpu.lib.dgx_profile(gpu.h, dgx.DGX_LSRK14, dt, prof_steps,
+                             ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 7)
+    if rc != 0:
+        raise RuntimeError(gpu.lib.dgx_last_error().decode())
+    gpu.synchronize()
+    kern = {KINDS[k]: {"ms_total": float(ms[k]), "launches": int(cnt[k]),
+                       "ms_mean": (ms[k] / cnt[k]) if cnt[k] else 0.0} for k in range(7)}
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    mean_s = kern[dom]["ms_mean"] / 1e3
+    flops = FLOPS_PER_DOF_STAGE.get(p, {}).get(dom)
+    roofline = None
+    if flops is not None:
+        algo_flops = flops * dofs
+        achieved = algo_flops / mean_s / 1e12
+        peak = FP64_PEAK_TFLOPS if precision == dgx.DGX_FP64 else FP32_MFMA_PEAK_TFLOPS
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": MEASURED_TRAFFIC.get((name, n, p, dom)),
+                    "kernel": dom, "arithmetic": ARITHMETIC[name],
+                    "algorithmic_flops_per_launch": algo_flops, "ms_mean": kern[dom]["ms_mean"]}
+    stage_s = sum(kern[k]["ms_total"] for k in kern) / 1e3 / (14 * prof_steps)
+    algo_bytes = BYTES_PER_DOF_STAGE * dofs
+    hbm = algo_bytes / stage_s / 1e9
+    roofline_hbm = {"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm / HBM_PEAK_GBS, "algorithmic_bytes_per_stage": algo_bytes,
+                    "ms_per_stage": stage_s * 1e3}
+    finite = bool(np.all(np.isfinite(gpu.norms())))
+    state = gpu.download(dgx.DGX_MODAL)
+    if args.copy_calibration:
+        nbytes = 8 * dofs
+        rc = gpu.lib.dgx_residual_download(gpu.h, ctypes.cast(gpu.device_state_ptr(),
+                                           ctypes.POINTER(ctypes.c_double)), dgx.DGX_MODAL, 1)
+        if rc != 0:
+            raise RuntimeError(gpu.lib.dgx_last_error().decode())
+        gpu.synchronize()
+        print(f"[bench] calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
+    gpu.close()
+    res = {"value": value, "ms_per_step": el / args.steps * 1e3, "dt": dt, "dofs": dofs,
+           "state_finite": finite, "roofline": roofline, "roofline_hbm": roofline_hbm, "kernels": kern}
+    return res, state
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +193,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=32, help="elements per direction")
     ap.add_argument("--p", type=int, default=7)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="paths to run; with both, the mixed path is also compared with the FP64 one")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline workers (0: one per core, at most 16)")
@@ -131,91 +210,58 @@ def main():
     n, p, mu = args.n, args.p, 1.0 / 1600.0
     cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers)
     m = tgv_domain(n, p, mu)
-    prec = dgx.DGX_MIXED if args.precision == "mixed" else dgx.DGX_FP64
-    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=prec)
-    x = gpu.coords()
-    U = ic.taylor_green(x[0], x[1], x[2])
-    del x
-    gpu.upload(U)
-    del U
-    gpu.synchronize()
-    dt = tgv_dt(n, p)
-    dofs = 5 * m.ne * (p + 1) ** 3
-    print(f"[bench] TGV NS3d {n}^3 p={p}: {dofs / 1e6:.1f} M DOF, dt={dt:.3e}", file=sys.stderr, flush=True)
+    runs, states = {}, {}
+    if args.precision in ("both", "fp64"):
+        runs["fp64"], states["fp64"] = run_path(m, n, p, mu, dgx.DGX_FP64, args)
+    if args.precision in ("both", "mixed"):
+        runs["mixed"], states["mixed"] = run_path(m, n, p, mu, dgx.DGX_MIXED, args)
 
-    gpu.advance(dgx.DGX_LSRK14, dt, args.warmup)
-    gpu.synchronize()
-    t0 = time.perf_counter()
-    gpu.advance(dgx.DGX_LSRK14, dt, args.steps)
-    gpu.synchronize()
-    el = time.perf_counter() - t0
-    stages = 14 * args.steps
-    value = dofs * stages / el
-    print(f"[bench] timed {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
-
-    # per-kernel times from HIP events on the library's stream
-    ms = np.zeros(7)
-    cnt = np.zeros(7, dtype=np.int32)
-    prof_steps = 1
-    rc = gpu.lib.dgx_profile(gpu.h, dgx.DGX_LSRK14, dt, prof_steps,
-                             ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                             cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 7)
-    if rc != 0:
-        raise RuntimeError(gpu.lib.dgx_last_error().decode())
-    gpu.synchronize()
-    kern = {KINDS[k]: {"ms_total": float(ms[k]), "launches": int(cnt[k]),
-                       "ms_mean": (ms[k] / cnt[k]) if cnt[k] else 0.0} for k in range(7)}
-    dom = max(kern, key=lambda k: kern[k]["ms_total"])
-    mean_s = kern[dom]["ms_mean"] / 1e3
-    flops = FLOPS_PER_DOF_STAGE.get(p, {}).get(dom)
-    if flops is not None:
-        algo_flops = flops * dofs
-        achieved = algo_flops / mean_s / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                    "traffic": MEASURED_TRAFFIC.get((n, p, dom)), "kernel": dom,
-                    "arithmetic": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-                    "algorithmic_flops_per_launch": algo_flops, "ms_mean": kern[dom]["ms_mean"]}
-    else:
-        roofline = None
-    stage_s = sum(kern[k]["ms_total"] for k in kern) / 1e3 / (14 * prof_steps)
-    algo_bytes = BYTES_PER_DOF_STAGE * dofs
-    hbm = algo_bytes / stage_s / 1e9
-    roofline_hbm = {"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": hbm / HBM_PEAK_GBS, "algorithmic_bytes_per_stage": algo_bytes,
-                    "ms_per_stage": stage_s * 1e3}
-    norms = gpu.norms()
-    finite = bool(np.all(np.isfinite(norms)))
-    if args.copy_calibration:
-        nbytes = 8 * dofs
-        rc = gpu.lib.dgx_residual_download(gpu.h, ctypes.cast(gpu.device_state_ptr(),
-                                           ctypes.POINTER(ctypes.c_double)), dgx.DGX_MODAL, 1)
-        if rc != 0:
-            raise RuntimeError(gpu.lib.dgx_last_error().decode())
-        print(f"[bench] calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
-
+    head = runs.get("fp64") or runs["mixed"]
+    head_name = "fp64" if "fp64" in runs else "mixed"
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "DOF-stage/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head_name == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"3D Navier-Stokes Taylor-Green vortex, periodic [0,2pi]^3, {n}^3 hexes, "
-                               f"p={p}, Re=1600, Ma=0.1, Pr=0.71, LF + BR1, LSRK14, FP64",
-                   "dofs": dofs, "stages_per_step": 14, "dt": dt, "precision": "fp64",
-                   "state_finite": finite},
-        "roofline": roofline,
-        "roofline_hbm": roofline_hbm,
-        "kernels": kern,
+                               f"p={p}, Re=1600, Ma=0.1, Pr=0.71, LF + BR1, LSRK14, "
+                               f"{'FP64' if head_name == 'fp64' else 'mixed'}",
+                   "dofs": head["dofs"], "stages_per_step": 14, "dt": head["dt"], "path": head_name,
+                   "state_finite": head["state_finite"]},
+        "roofline": head["roofline"],
+        "roofline_hbm": head["roofline_hbm"],
+        "kernels": head["kernels"],
     }
-    gpu.close()
+    if "mixed" in runs and "fp64" in runs:
+        mx = runs["mixed"]
+        a, b = states["fp64"], states["mixed"]
+        d = np.sqrt(np.sum((b - a) ** 2, axis=(1, 2)))
+        r = np.sqrt(np.sum(a ** 2, axis=(1, 2)))
+        mom = np.sqrt(np.sum(a[1:4] ** 2))
+        rel = (d / r).tolist()
+        w_mom = float(d[3] / mom)
+        ok = all(rel[v] <= 1e-6 for v in (0, 1, 2, 4)) and w_mom <= 1e-6
+        line["mixed"] = {
+            "value": mx["value"], "unit": "DOF-stage/s", "ms_per_step": mx["ms_per_step"],
+            "speedup_vs_fp64": mx["value"] / runs["fp64"]["value"],
+            "rel_err_l2_per_field": rel,
+            "rho_w_err_vs_momentum_norm": w_mom,
+            "compared_after_steps": args.warmup + args.steps + 1,
+            "criterion": "per-field relative L2 <= 1e-6 for rho, rho u, rho v, E; rho w (zero at t = 0) "
+                         "against the momentum norm",
+            "criterion_met": ok,
+            "roofline": mx["roofline"], "roofline_hbm": mx["roofline_hbm"], "kernels": mx["kernels"],
+        }
+    elif "mixed" in runs:
+        line["mixed"] = {"value": runs["mixed"]["value"]}
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
