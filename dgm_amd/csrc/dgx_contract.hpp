@@ -5,7 +5,14 @@
 #pragma once
 #include "dgx_device.hpp"
 
+#ifndef DGX_STAGE_THREADS
+#define DGX_STAGE_THREADS 256
+#endif
+
 namespace dgx {
+
+// Threads per workgroup (one element per workgroup) of the fused hex kernels.
+constexpr int kStageThreads = DGX_STAGE_THREADS;
 
 constexpr int odd(int n) { return n | 1; }
 template <int V> struct IC { static constexpr int value = V; };

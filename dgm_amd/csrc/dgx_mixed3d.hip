@@ -167,7 +167,7 @@ template <int P> struct M3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L * L * L, Q2 = Q * Q, Q3 = Q * Q * Q, QF2 = QF * QF;
   static constexpr int LP = odd(L), QP = odd(Q), FP = odd(QF);
-  static constexpr int NT = 256;
+  static constexpr int NT = kStageThreads;
   static constexpr int NPT = (Q3 + NT - 1) / NT;
   static constexpr int NOB = (L + 3) / 4;               // output blocks of L modes
   static constexpr int NA = cmax(cmax(cmax(5 * Q2 * QP, 5 * Q2 * LP), cmax(30 * L * FP, 30 * L * LP)),
@@ -356,7 +356,7 @@ struct MixedArgs {
 
 // Residual (or fused LSRK14 stage) of Euler3d / Navier_Stokes3d, mixed precision.
 template <int P, bool NS, bool UPDATE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStageThreads)
 k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   using S = M3<P>;
   using T = MT<P>;
@@ -600,7 +600,7 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
 // the viscous flux, TV[e][s][eq][QF2] (FP32) = viscous flux . n at the Gauss
 // points of face s.
 template <int P>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStageThreads)
 k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__restrict__ V,
           float *__restrict__ TV, Geo geo, Phys ph, int ne) {
   using S = M3<P>;
@@ -866,7 +866,7 @@ hipError_t launch_mixed3_p(const StageLaunch &s) {
   if (s.op == 0) {
     const int sh = S::lds_vterm;
     (void)hipFuncSetAttribute((const void *)k_vterm3m<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3m<P>, dim3(s.ne), dim3(256), sh, s.stream, s.U, s.T, s.Vfw, s.TVfw,
+    hipLaunchKernelGGL(k_vterm3m<P>, dim3(s.ne), dim3(kStageThreads), sh, s.stream, s.U, s.T, s.Vfw, s.TVfw,
                        s.geo, s.phys, s.ne);
     return hipGetLastError();
   }
@@ -878,7 +878,7 @@ hipError_t launch_mixed3_p(const StageLaunch &s) {
   do {                                                                                               \
     (void)hipFuncSetAttribute((const void *)k_stage3m<P, NSV, UPD>,                                  \
                               hipFuncAttributeMaxDynamicSharedMemorySize, sh);                       \
-    hipLaunchKernelGGL((k_stage3m<P, NSV, UPD>), dim3(s.ne), dim3(256), sh, s.stream, a, s.geo,      \
+    hipLaunchKernelGGL((k_stage3m<P, NSV, UPD>), dim3(s.ne), dim3(kStageThreads), sh, s.stream, a, s.geo,      \
                        s.phys);                                                                      \
   } while (0)
   if (s.ns) { if (s.update) DGX_MSTAGE(true, true); else DGX_MSTAGE(true, false); }

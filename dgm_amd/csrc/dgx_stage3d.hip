@@ -1,6 +1,7 @@
 // Fused hex stage kernels (p <= 9).
 //
-// One workgroup of 256 threads per element.  Every sum-factorized
+// One workgroup per element (512 threads for the stage kernel, 256 for the
+// viscous kernel, whose 198 VGPRs allow only 2 waves per SIMD).  Every sum-factorized
 // contraction runs on all conserved fields of the element at once (pencils of
 // 5 fields x q^2 fill the workgroup instead of q^2 = 81 pencils at p = 7),
 // nodal values live in registers (one thread per node), and the last
@@ -24,11 +25,11 @@
 
 namespace dgx {
 
-template <int P> struct S3 {
+template <int P, int NTH = kStageThreads> struct S3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L * L * L, Q2 = Q * Q, Q3 = Q * Q * Q, QF2 = QF * QF;
   static constexpr int LP = odd(L), QP = odd(Q), FP = odd(QF);   // padded pitches
-  static constexpr int NT = 256;
+  static constexpr int NT = NTH;
   static constexpr int NPT = (Q3 + NT - 1) / NT;        // nodes per thread
   // doubles per LDS tensor: the largest intermediate of any phase
   static constexpr int NA = cmax(cmax(cmax(5 * Q2 * QP, 5 * Q2 * LP),                 // nodal, backward
@@ -50,10 +51,10 @@ template <int P> struct S3 {
 // [NF][Q][L][L] (pitch LP), accumulated into registers:
 // acc[k][l] += sum_i M[l][i] in[f][i][m][n] for the thread's pencils
 // G = tid + k*NT = f*L2 + m*L + n.
-template <int P, int NF, int NS>
+template <int P, int NF, int NS, int NTH = kStageThreads>
 __device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, const double *in,
                                               double (&acc)[NS][P + 1], int tid) {
-  using S = S3<P>;
+  using S = S3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, L2 = S::L2, LP = S::LP, NT = S::NT;
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
@@ -77,9 +78,9 @@ __device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, cons
 // Sum-factorized projected divergence chain for reference direction D of NF
 // nodal fields in A ([NF][Q][Q][Q], pitch QP); result accumulated into registers.
 // D = 0 (r): QD along a;  D = 1 (s): QD along b;  D = 2 (t): QD along c.
-template <int P, int NF, int NS, int D>
+template <int P, int NF, int NS, int D, int NTH = kStageThreads>
 __device__ __forceinline__ void div_chain(double *A, double *B, double (&acc)[NS][P + 1], int tid) {
-  using S = S3<P>;
+  using S = S3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
   const double *Mc = D == 2 ? CT<P>::QD() : CT<P>::BW();
   const double *Mb = D == 1 ? CT<P>::QD() : CT<P>::BW();
@@ -88,14 +89,14 @@ __device__ __forceinline__ void div_chain(double *A, double *B, double (&acc)[NS
   __syncthreads();
   con_mid<NF * Q, Q, 1, L, L, LP, LP, NT>(Mb, B, A, tid);       // [NF Q][L][L]
   __syncthreads();
-  chain_to_regs<P, NF, NS>(Ma, A, acc, tid);                    // [NF][L][L][L]
+  chain_to_regs<P, NF, NS, NTH>(Ma, A, acc, tid);               // [NF][L][L][L]
   __syncthreads();
 }
 
 // modal [NF][L][L][L] (pitch LP) in A -> nodal [NF][Q][Q][Q] (pitch QP) in B
-template <int P, int NF>
+template <int P, int NF, int NTH = kStageThreads>
 __device__ __forceinline__ void backward_batch(double *A, double *B, int tid) {
-  using S = S3<P>;
+  using S = S3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
   con_mid<NF, L, L, L, Q, LP, LP, NT>(CT<P>::BT(), A, B, tid);   // [NF][Q][L][L]
   __syncthreads();
@@ -106,9 +107,9 @@ __device__ __forceinline__ void backward_batch(double *A, double *B, int tid) {
 }
 
 // Face blocks g[NFB][QF][QF] (pitch FP) -> H[NFB][L][L] (pitch LP) in B, via W in A.
-template <int P, int NFB>
+template <int P, int NFB, int NTH = kStageThreads>
 __device__ __forceinline__ void face_project(const double *g, double *A, double *B, int tid) {
-  using S = S3<P>;
+  using S = S3<P, NTH>;
   constexpr int L = S::L, QF = S::QF, LP = S::LP, FP = S::FP, NT = S::NT;
   con_mid<NFB, QF, 1, QF, L, FP, FP, NT>(CT<P>::PW(), g, A, tid);     // [NFB][L][QF]
   __syncthreads();
@@ -145,10 +146,15 @@ struct StageArgs {
   int *flag;
 };
 
+// The stage kernel (125 VGPRs, 76 KB LDS at p = 7) runs 8 waves per element:
+// every contraction phase is then a single pass, and a CU holds 4 waves per
+// SIMD instead of 2 within the same LDS.
+constexpr int kStage3Threads = 2 * kStageThreads;
+
 template <int P, bool NS, bool UPDATE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStage3Threads)
 k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
-  using S = S3<P>;
+  using S = S3<P, kStage3Threads>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
                 LP = S::LP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSL = S::NSL;
   extern __shared__ double sm[];
@@ -162,7 +168,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
     A[(m / L + v * L2) * LP + (m % L)] = a.U[((size_t)v * ne + e) * L3 + m];
   }
   __syncthreads();
-  backward_batch<P, 5>(A, B, tid);
+  backward_batch<P, 5, kStage3Threads>(A, B, tid);
   constexpr int FS = Q * Q * S::QP;   // padded nodal field size
   double un[NPT][5];
 #pragma unroll
@@ -238,14 +244,14 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
       }
     }
     __syncthreads();
-    div_chain<P, 5, NSL, d>(A, B, acc, tid);
+    div_chain<P, 5, NSL, d, kStage3Threads>(A, B, acc, tid);
   };
   vol_dir(IC<0>());
   vol_dir(IC<1>());
   vol_dir(IC<2>());
 
   // faces: H = (PW (x) PW) gF for 5 fields x 6 faces, in B
-  face_project<P, 30>(gF, A, B, tid);
+  face_project<P, 30, kStage3Threads>(gF, A, B, tid);
   const double one6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
 #pragma unroll
   for (int k = 0; k < NSL; ++k) {
@@ -326,7 +332,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
 // BR1 viscous terms: V[e][eq][L3] = modal divergence of the viscous flux,
 // TV[e][s][eq][QF2] = viscous flux . n at the Gauss points of face s.
 template <int P>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStageThreads)
 k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
          double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
   using S = S3<P>;
@@ -566,16 +572,16 @@ static hipError_t launch_stage3_p(const StageLaunch &s) {
   if (s.op == 0) {
     const int sh = S::lds_vterm;
     (void)hipFuncSetAttribute((const void *)k_vterm3<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(256), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
+    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(kStageThreads), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
                        s.geo, s.phys, s.ne);
     return hipGetLastError();
   }
-  const int sh = S::lds_stage;
+  const int sh = S3<P, kStage3Threads>::lds_stage;
 #define DGX_STAGE(NSV, UPD)                                                                          \
   do {                                                                                               \
     (void)hipFuncSetAttribute((const void *)k_stage3<P, NSV, UPD>,                                   \
                               hipFuncAttributeMaxDynamicSharedMemorySize, sh);                       \
-    hipLaunchKernelGGL((k_stage3<P, NSV, UPD>), dim3(s.ne), dim3(256), sh, s.stream, a, s.tab, s.geo, \
+    hipLaunchKernelGGL((k_stage3<P, NSV, UPD>), dim3(s.ne), dim3(kStage3Threads), sh, s.stream, a, s.tab, s.geo, \
                        s.phys);                                                                      \
   } while (0)
   if (s.ns) { if (s.update) DGX_STAGE(true, true); else DGX_STAGE(true, false); }
