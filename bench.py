@@ -28,6 +28,9 @@ Also printed in the same JSON line:
   roofline_hbm  the HBM view of a whole stage: 32 B per DOF-stage (read U,
                 read Ut, write Ut, write U of a fused 2N-storage stage) / the
                 mean stage time, against 8 TB/s
+  serving       p50/p99 latency and throughput of a serving-loop sample: Poisson
+                arrivals of 2D Euler forward solves of random size and order,
+                continuously batched (dgm_amd.serve)
   cpu_baseline  the CPU oracle on the host cores (one worker per core, at
                 most 16) on a bounded sample of the same workload: same p and
                 physics on a 3^3 periodic box per worker, ~15 s
@@ -117,6 +120,50 @@ def cpu_baseline(p, mu, seconds=15.0, workers=None):
                       f"(first step excluded)"}
 
 
+def serving_sample(args, seed=1234):
+    """A serving mix shaped like the ensemble workload: Poisson arrivals of 2D
+    Euler forward solves (freestream plus 8 seeded Fourier modes of amplitude
+    0.05), n x n periodic quads with n log-uniform in [8, nmax], p uniform in
+    {2..8}, 200 LSRK14 steps each, FP64, continuous batching on one GPU."""
+    from dgm_amd import serve
+    from dgm_amd.mesh import quad_nodes
+    rng = np.random.default_rng(seed)
+    jobs, dofs = [], []
+    base = [1.0, 0.5, 0.0, (1.0 / 1.4) / 0.4 + 0.125]
+    for _ in range(args.serve_requests):
+        nn = int(round(np.exp(rng.uniform(np.log(8), np.log(args.serve_nmax)))))
+        pp = int(rng.integers(2, 9))
+        m = meshgen.periodic_quads(nn, nn, 0.0, 10.0, 0.0, 10.0)
+        x = quad_nodes(m, pp)   # nodal GLL coordinates without a device domain
+        U = np.stack([b + ic.fourier_modes(x, (10.0, 10.0), 8, 0.05, rng) for b in base])
+        dt = 0.2 * (10.0 / nn) / ((pp + 1) ** 2 * (0.5 + np.sqrt(1.4 * (1 / 1.4))))
+        jobs.append(dict(mesh=m, p=pp, eqntype=dgx.DGX_EULER, U0=U, inttype=dgx.DGX_LSRK14, dt=dt,
+                         nsteps=args.serve_steps))
+        dofs.append(4 * nn * nn * (pp + 1) ** 2)
+    srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active,
+                       steps_per_slice=args.serve_slice)
+    arrivals = serve.poisson_arrivals(len(jobs), rate=args.serve_rate, rng=rng)
+    t0 = time.perf_counter()
+    ids = serve.serve_timeline(srv, jobs, arrivals)
+    wall = time.perf_counter() - t0
+    lat, wait = [], []
+    for rid in ids:
+        _, (ts, ta, td) = srv.result(rid)
+        lat.append(td - ts)
+        wait.append(ta - ts)
+    peak_bytes, peak_active = srv.peak()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    return {"requests": len(jobs), "p50_latency_ms": float(np.percentile(lat_ms, 50)),
+            "p99_latency_ms": float(np.percentile(lat_ms, 99)),
+            "mean_queue_wait_ms": float(np.mean(wait) * 1e3),
+            "throughput_dof_stage_per_s": float(sum(dofs) * 14 * args.serve_steps / wall),
+            "makespan_s": wall, "peak_active": peak_active, "peak_device_bytes": peak_bytes,
+            "sample": f"{len(jobs)} 2D Euler solves, n log-uniform in [8,{args.serve_nmax}], p in 2..8, "
+                      f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, FP64, "
+                      f"{args.serve_active} slots, slices of {args.serve_slice} steps replayed from hipGraphs"}
+
+
 def run_path(m, n, p, mu, precision, args):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
     gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision)
@@ -198,6 +245,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline workers (0: one per core, at most 16)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-nmax", type=int, default=64)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=32.0, help="arrivals per second")
+    ap.add_argument("--serve-active", type=int, default=32)
+    ap.add_argument("--serve-slice", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -262,6 +316,8 @@ def main():
         }
     elif "mixed" in runs:
         line["mixed"] = {"value": runs["mixed"]["value"]}
+    if not args.no_serve:
+        line["serving"] = serving_sample(args)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

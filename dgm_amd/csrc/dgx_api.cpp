@@ -15,13 +15,20 @@
 #include <vector>
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+namespace dgx {
+// Error message of the calling thread's last failure (dgx_last_error).
+int set_error(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+}  // namespace dgx
+
+namespace {
+
+int fail(int code, const std::string &msg) { return dgx::set_error(code, msg); }
 
 #define HIPCHK(expr)                                                                  \
   do {                                                                                \
@@ -568,6 +575,30 @@ extern "C" {
 const char *dgx_version(void) { return DGX_VERSION; }
 const char *dgx_last_error(void) { return g_err.c_str(); }
 
+int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
+  if (!desc || !bytes) return fail(DGX_ERR_ARGUMENT, "null argument");
+  if ((desc->nsd != 2 && desc->nsd != 3) || desc->p < 1 || desc->p > 10 || desc->ne <= 0)
+    return fail(DGX_ERR_ARGUMENT, "bad domain description");
+  const size_t nsd = desc->nsd, ne = desc->ne, L = desc->p + 1, Q = desc->p + 2, QF = desc->p + 1;
+  const size_t nf = nsd + 2, nm = nsd == 2 ? L * L : L * L * L, nq = nsd == 2 ? Q * Q : Q * Q * Q;
+  const size_t ns = nsd == 2 ? 4 : 6, qs = nsd == 2 ? QF : QF * QF;
+  const bool visc = desc->eqntype == DGX_NAVIER_STOKES || desc->eqntype == DGX_NAVIER_STOKES3D;
+  const bool fused = nsd == 3 && dgx::stage3_supported(desc->p);
+  const bool mixed = desc->precision == DGX_MIXED;
+  size_t b = 4 * nf * ne * nm * 8;                          // U, Ut, R0, R1
+  b += (fused ? 2 : 1) * ne * ns * nf * qs * 8;             // face traces
+  if (visc) {
+    const size_t w = mixed ? 4 : 8;
+    if (!fused && !mixed) b += ne * (nsd == 2 ? 6 : 12) * nm * 8;
+    b += ne * (nf - 1) * nm * w + ne * ns * (nf - 1) * qs * w;
+  }
+  b += nf * ne * nq * 8;                                    // nodal transfer buffer
+  b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
+  b += 8 * qs * 2 + (size_t)(1 << 20);                      // orientation tables, tables, rounding
+  *bytes = b;
+  return DGX_OK;
+}
+
 int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out) {
   if (!desc || !out) return fail(DGX_ERR_ARGUMENT, "null argument");
   *out = nullptr;
@@ -578,8 +609,8 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
   const bool euler3 = desc->eqntype == DGX_EULER3D || desc->eqntype == DGX_NAVIER_STOKES3D;
   if ((desc->nsd == 2 && !euler) || (desc->nsd == 3 && !euler3))
     return fail(DGX_ERR_ARGUMENT, "eqntype does not match nsd");
-  if (desc->flux == DGX_FLUX_ROE && desc->nsd == 3)
-    return fail(DGX_ERR_ARGUMENT, "the Roe flux is 2D only (Euler3d is Lax-Friedrichs only)");
+  if (desc->flux != DGX_FLUX_LAX_FRIEDRICHS && desc->flux != DGX_FLUX_ROE)
+    return fail(DGX_ERR_ARGUMENT, "flux must be DGX_FLUX_LAX_FRIEDRICHS or DGX_FLUX_ROE");
   if (desc->precision != DGX_FP64 && desc->precision != DGX_MIXED)
     return fail(DGX_ERR_ARGUMENT, "precision must be DGX_FP64 or DGX_MIXED");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))

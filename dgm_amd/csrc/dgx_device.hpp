@@ -60,7 +60,7 @@ struct Geo {
 
 struct Phys {
   double gamma, vis, Pr;
-  int flux;                      // 0 Lax-Friedrichs, 1 Roe (2D)
+  int flux;                      // 0 Lax-Friedrichs, 1 Roe
 };
 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last
@@ -173,6 +173,67 @@ __device__ __forceinline__ void euler_normal_flux2(const double U[4], double nx,
   const double c = sqrt((gm1 + 1.0) * p * ir);
   lam = fabs(un) + c;
   bad = !(U[0] > kAepsM) || !(p > kAepsM);
+}
+
+// 3D Roe flux without entropy fix: the reference's 2D Roe flux
+// (src/Euler.cpp:682-741) extended by the second shear wave, written without a
+// tangent basis (the two shear waves share the eigenvalue u.n, so only the
+// tangential jump vector enters).  FL = F(UL).n; Fh = FL + sum_k min(lambda_k,0)
+// alpha_k r_k.  The reference has no 3D Roe flux.
+__device__ __forceinline__ void roe_flux3(const double UL[5], const double UR[5], double nx, double ny,
+                                          double nz, double gamma, const double FL[5], double Fh[5]) {
+  const double gm1 = gamma - 1.0;
+  const double rl = UL[0], rr = UR[0];
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double ulx = UL[1] * irl, uly = UL[2] * irl, ulz = UL[3] * irl;
+  const double urx = UR[1] * irr, ury = UR[2] * irr, urz = UR[3] * irr;
+  const double pl = gm1 * (UL[4] - 0.5 * (UL[1] * ulx + UL[2] * uly + UL[3] * ulz));
+  const double pr = gm1 * (UR[4] - 0.5 * (UR[1] * urx + UR[2] * ury + UR[3] * urz));
+  const double Hl = (UL[4] + pl) * irl, Hr = (UR[4] + pr) * irr;
+  const double R = sqrt(rr / rl), iRp1 = 1.0 / (1.0 + R);
+  const double ux = (ulx + R * urx) * iRp1, uy = (uly + R * ury) * iRp1, uz = (ulz + R * urz) * iRp1;
+  const double H = (Hl + R * Hr) * iRp1;
+  const double un = ux * nx + uy * ny + uz * nz;
+  const double tx = ux - un * nx, ty = uy - un * ny, tz = uz - un * nz;   // tangential velocity
+  const double a2 = gm1 * (H - 0.5 * (ux * ux + uy * uy + uz * uz));
+  const double a = sqrt(a2), ainv = 1.0 / a, c1inv = a2 / gm1;
+  const double dr = rr - rl, dE = UR[4] - UL[4];
+  const double dmx = UR[1] - UL[1], dmy = UR[2] - UL[2], dmz = UR[3] - UL[3];
+  const double dmn = dmx * nx + dmy * ny + dmz * nz;
+  // shear strengths (tangential momentum jump minus tangential velocity x density jump)
+  const double sx = (dmx - dmn * nx) - tx * dr, sy = (dmy - dmn * ny) - ty * dr, sz = (dmz - dmn * nz) - tz * dr;
+  const double st = sx * tx + sy * ty + sz * tz;
+  const double a1 = gm1 * ainv * ainv * (dr * (H - un * un) + un * dmn - (dE - st));
+  const double a0 = 0.5 * ainv * (dr * (un + a) - dmn - a * a1);
+  const double a4 = dr - (a0 + a1);
+  const double l0 = fmin(un - a, 0.0), l1 = fmin(un, 0.0), l4 = fmin(un + a, 0.0);
+  const double g0 = l0 * a0 + l1 * a1 + l4 * a4;
+  const double gn = l0 * (un - a) * a0 + l1 * un * a1 + l4 * (un + a) * a4;
+  Fh[0] = FL[0] + g0;
+  Fh[1] = FL[1] + gn * nx + g0 * tx + l1 * sx;
+  Fh[2] = FL[2] + gn * ny + g0 * ty + l1 * sy;
+  Fh[3] = FL[3] + gn * nz + g0 * tz + l1 * sz;
+  Fh[4] = FL[4] + l0 * (H - un * a) * a0 + l1 * (H - c1inv) * a1 + l1 * st + l4 * (H + un * a) * a4;
+}
+
+// Numerical flux of Euler3d at one face point: FL = F(UL).n and Fh by
+// Lax-Friedrichs (flux 0, src/Euler3d.cpp:772-795) or Roe (flux 1).
+__device__ __forceinline__ void riemann3(const double UL[5], const double UR[5], double nx, double ny,
+                                         double nz, double gamma, int flux, double FL[5], double Fh[5],
+                                         bool &bad) {
+  const double gm1 = gamma - 1.0;
+  double FR[5], lL, lR;
+  bool bl, br;
+  euler_normal_flux3(UL, nx, ny, nz, gm1, FL, lL, bl);
+  euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
+  bad |= bl | br;
+  if (flux == 1) {
+    roe_flux3(UL, UR, nx, ny, nz, gamma, FL, Fh);
+  } else {
+    const double lmax = fmax(lL, lR);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+  }
 }
 
 // 2D Roe flux without entropy fix in the face frame (src/Euler.cpp:682-741),

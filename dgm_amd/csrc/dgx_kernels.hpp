@@ -1,6 +1,8 @@
 // Launch interface between the host engine (dgx_api.cpp) and the templated
 // kernels (dgx_kernels2d.hip, dgx_kernels3d.hip, dgx_stage3d.hip, dgx_mixed3d.hip).
 #pragma once
+#include <string>
+
 #include "dgx_device.hpp"
 
 namespace dgx {
@@ -61,6 +63,9 @@ hipError_t launch_stage3(const StageLaunch &s);
 // Mixed-precision fused hex stage kernels (dgx_mixed3d.hip), p <= 8.
 bool mixed3_supported(int p);
 hipError_t launch_mixed3(const StageLaunch &s);
+
+// Error message of the calling thread's last failure (dgx_api.cpp).
+int set_error(int code, const std::string &msg);
 
 // Flat-array stage updates (dgx_blas.hip)
 hipError_t blas_axpy(double a, const double *x, const double *y, double *z, size_t n, hipStream_t s);

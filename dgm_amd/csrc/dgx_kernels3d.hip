@@ -169,20 +169,16 @@ k_rhs3(const double *__restrict__ U, const double *__restrict__ T, const double 
     const int an = geo.perm[geo.permid[es] * QF2 + a];
     const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1], nz = geo.fnorm[3 * es + 2];
     const double jr = geo.fjr[es];
-    double UL[5], UR[5], FL[5], FR[5], lL, lR;
+    double UL[5], UR[5], FL[5];
     const double *tl = T + ((size_t)e * 6 + s) * 5 * QF2;
     const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
     for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + a]; UR[v] = tr[v * QF2 + an]; }
-    bool bl, br;
-    euler_normal_flux3(UL, nx, ny, nz, gm1, FL, lL, bl);
-    euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
-    bad |= bl | br;
-    const double lmax = fmax(lL, lR);
+    double Fh[5];
+    riemann3(UL, UR, nx, ny, nz, ph.gamma, ph.flux, FL, Fh, bad);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-      double g = jr * (Fh - FL[v]);
+      double g = jr * (Fh[v] - FL[v]);
       if (NS && v > 0) {
         const double *vl = TV + ((size_t)e * 6 + s) * 4 * QF2;
         const double *vr = TV + ((size_t)nb * 6 + ns) * 4 * QF2;

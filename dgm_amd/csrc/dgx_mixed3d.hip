@@ -163,11 +163,11 @@ __device__ __forceinline__ void mcon_last(int off, const float *in, float *out, 
   }
 }
 
-template <int P> struct M3 {
+template <int P, int NTH = kStageThreads> struct M3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L * L * L, Q2 = Q * Q, Q3 = Q * Q * Q, QF2 = QF * QF;
   static constexpr int LP = odd(L), QP = odd(Q), FP = odd(QF);
-  static constexpr int NT = kStageThreads;
+  static constexpr int NT = NTH;
   static constexpr int NPT = (Q3 + NT - 1) / NT;
   static constexpr int NOB = (L + 3) / 4;               // output blocks of L modes
   static constexpr int NA = cmax(cmax(cmax(5 * Q2 * QP, 5 * Q2 * LP), cmax(30 * L * FP, 30 * L * LP)),
@@ -201,10 +201,10 @@ template <int P> struct M3 {
 // [NF][Q][L][L] (pitch LP), accumulated into the MFMA result registers:
 // acc[k][ob][r] += sum_i M[4ob+r][i] in[f][i][m][n] for the pencils
 // G = tid + k*NT = f*L2 + m*L + n.
-template <int P, int NF, int NS>
+template <int P, int NF, int NS, int NTH = kStageThreads>
 __device__ __forceinline__ void mchain_to_regs(int off, const float *in, f4 (&acc)[NS][M3<P>::NOB],
                                                int tid) {
-  using S = M3<P>;
+  using S = M3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, L2 = S::L2, LP = S::LP, NT = S::NT, NOB = S::NOB;
   float cf[NOB][Q];
   load_coef<L, Q>(off, tid, cf);
@@ -226,9 +226,9 @@ __device__ __forceinline__ void mchain_to_regs(int off, const float *in, f4 (&ac
 
 // Projected divergence chain for reference direction D of NF nodal fields in
 // A ([NF][Q][Q][Q], pitch QP); result accumulated into registers.
-template <int P, int NF, int NS, int D>
+template <int P, int NF, int NS, int D, int NTH = kStageThreads>
 __device__ __forceinline__ void mdiv_chain(float *A, float *B, f4 (&acc)[NS][M3<P>::NOB], int tid) {
-  using S = M3<P>;
+  using S = M3<P, NTH>;
   using T = MT<P>;
   constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
   constexpr int oc = D == 2 ? T::oQD : T::oBW, ob = D == 1 ? T::oQD : T::oBW, oa = D == 0 ? T::oQD : T::oBW;
@@ -236,14 +236,14 @@ __device__ __forceinline__ void mdiv_chain(float *A, float *B, f4 (&acc)[NS][M3<
   __syncthreads();
   mcon_mid<NF * Q, Q, 1, L, L, LP, LP, NT>(ob, B, A, tid);       // [NF Q][L][L]
   __syncthreads();
-  mchain_to_regs<P, NF, NS>(oa, A, acc, tid);                    // [NF][L][L][L]
+  mchain_to_regs<P, NF, NS, NTH>(oa, A, acc, tid);               // [NF][L][L][L]
   __syncthreads();
 }
 
 // modal [NF][L][L][L] (pitch LP) in A -> nodal [NF][Q][Q][Q] (pitch QP) in B
-template <int P, int NF>
+template <int P, int NF, int NTH = kStageThreads>
 __device__ __forceinline__ void mbackward(float *A, float *B, int tid) {
-  using S = M3<P>;
+  using S = M3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT, o = MT<P>::oBT;
   mcon_mid<NF, L, L, L, Q, LP, LP, NT>(o, A, B, tid);            // [NF][Q][L][L]
   __syncthreads();
@@ -254,9 +254,9 @@ __device__ __forceinline__ void mbackward(float *A, float *B, int tid) {
 }
 
 // Face blocks g[NFB][QF][QF] (pitch FP) -> H[NFB][L][L] (pitch LP) in B, via A.
-template <int P, int NFB>
+template <int P, int NFB, int NTH = kStageThreads>
 __device__ __forceinline__ void mface_project(const float *g, float *A, float *B, int tid) {
-  using S = M3<P>;
+  using S = M3<P, NTH>;
   constexpr int L = S::L, QF = S::QF, LP = S::LP, FP = S::FP, NT = S::NT, o = MT<P>::oPW;
   mcon_mid<NFB, QF, 1, QF, L, FP, FP, NT>(o, g, A, tid);         // [NFB][L][QF]
   __syncthreads();
@@ -282,10 +282,10 @@ __device__ __forceinline__ float mface_sum(const float *H, int l, int m, int n, 
 
 // Load the modal state of element e: constant modes into cn[] (nodal value of
 // the constant mode, FP64), the other modes as FP32 into A ([5][L][L][L], pitch LP).
-template <int P>
+template <int P, int NTH = kStageThreads>
 __device__ __forceinline__ void load_split(const double *__restrict__ U, int e, int ne, float *A,
                                            double (&cn)[5], int tid) {
-  using S = M3<P>;
+  using S = M3<P, NTH>;
   constexpr int L = S::L, L2 = S::L2, L3 = S::L3, LP = S::LP, NT = S::NT;
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3, m = i - v * L3;
@@ -340,6 +340,12 @@ __device__ __forceinline__ void euler_flux_jvp(const double c[5], double n0, dou
 }
 
 }  // namespace
+
+#ifndef DGX_VTERM_M_THREADS
+#define DGX_VTERM_M_THREADS 512
+#endif
+// Threads per element of the mixed viscous kernel.
+constexpr int kVterm3mThreads = DGX_VTERM_M_THREADS;
 
 struct MixedArgs {
   const double *T;       // face traces of the input state (FP64)
@@ -411,21 +417,17 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
     const int qn = geo.perm[geo.permid[es] * QF2 + q];
     const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1], nz = geo.fnorm[3 * es + 2];
     const double jr = geo.fjr[es];
-    double UL[5], UR[5], FL[5], FR[5], lL, lR;
+    double UL[5], UR[5], FL[5];
     const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
     const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
     for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + q]; UR[v] = tr[v * QF2 + qn]; }
-    bool bl, br;
-    euler_normal_flux3(UL, nx, ny, nz, gm1, FL, lL, bl);
-    euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
-    bad |= bl | br;
-    const double lmax = fmax(lL, lR);
+    double Fh[5];
+    riemann3(UL, UR, nx, ny, nz, ph.gamma, ph.flux, FL, Fh, bad);
     const int qa = q / QF, qb = q - qa * QF;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-      double g = jr * (Fh - FL[v]);
+      double g = jr * (Fh[v] - FL[v]);
       if (NS && v > 0) {
         const float *vl = a.TV + ((size_t)e * 6 + s) * 4 * QF2;
         const float *vr = a.TV + ((size_t)nb * 6 + ns) * 4 * QF2;
@@ -596,14 +598,40 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   if constexpr (4 * S::TB < 5) batch(IC<4 * S::TB>());
 }
 
+// Face-lift contribution of the two faces normal to reference direction D (see
+// face_pair in dgx_stage3d.hip), FP32.
+template <int P, int D>
+__device__ __forceinline__ float mface_pair(const float *H, int l, int m, int n, float clo, float chi) {
+  constexpr int L = P + 1, LP = odd(L), HB = L * LP;
+  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
+  if (D == 0) return fmaf(clo * (float)B0[l], H[4 * HB + m * LP + n], chi * (float)B1[l] * H[2 * HB + m * LP + n]);
+  if (D == 1) return fmaf(clo * (float)B0[m], H[1 * HB + l * LP + n], chi * (float)B1[m] * H[3 * HB + l * LP + n]);
+  return fmaf(clo * (float)B0[n], H[0 * HB + l * LP + m], chi * (float)B1[n] * H[5 * HB + l * LP + m]);
+}
+
+// Viscous flux of equation q contracted with (n0, n1, n2) from the stress
+// tensor t (t11 t12 t13 t22 t23 t33), heat flux h and velocity, FP32.
+__device__ __forceinline__ float mvisc_flux(const float t[6], const float h[3], float u, float v, float w,
+                                            int q, float n0, float n1, float n2) {
+  const float r0 = t[0] * n0 + t[1] * n1 + t[2] * n2;
+  const float r1 = t[1] * n0 + t[3] * n1 + t[4] * n2;
+  const float r2 = t[2] * n0 + t[4] * n1 + t[5] * n2;
+  if (q == 0) return r0;
+  if (q == 1) return r1;
+  if (q == 2) return r2;
+  return (r0 * u + r1 * v + r2 * w) + (h[0] * n0 + h[1] * n1 + h[2] * n2);
+}
+
 // BR1 viscous terms, mixed precision: V[e][eq][L3] (FP32) = modal divergence of
 // the viscous flux, TV[e][s][eq][QF2] (FP32) = viscous flux . n at the Gauss
-// points of face s.
+// points of face s.  The lifted gradient is assembled one reference direction
+// at a time (affine faces: normals -+ grad xi_d / |grad xi_d|) straight into
+// the stress tensor and heat flux at the thread's nodes.
 template <int P>
-__global__ void __launch_bounds__(kStageThreads)
+__global__ void __launch_bounds__(kVterm3mThreads)
 k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__restrict__ V,
           float *__restrict__ TV, Geo geo, Phys ph, int ne) {
-  using S = M3<P>;
+  using S = M3<P, kVterm3mThreads>;
   using T = MT<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
                 LP = S::LP, QP = S::QP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSG = S::NSG, NOB = S::NOB,
@@ -613,7 +641,7 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
   const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
 
   double cn[5];
-  load_split<P>(U, e, ne, A, cn, tid);
+  load_split<P, NT>(U, e, ne, A, cn, tid);
   // primitive jumps J_side/J_vol ({q} - q-) at the Gauss points (average_flux '+'), FP64
   for (int i = tid; i < 6 * QF2; i += NT) {
     const int s = i / QF2, q = i - s * QF2;
@@ -642,7 +670,7 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
       Jq[((c * 6 + s) * QF + qa) * FP + qb] = (float)(jr * (0.5 * (pl[c] + pq[c]) - pl[c]));
   }
   __syncthreads();
-  mbackward<P, 5>(A, B, tid);
+  mbackward<P, 5, NT>(A, B, tid);
   // primitives: deviations from the constant state (FP32) for the gradient
   // chains, full velocities (FP32) for the viscous work
   const double icr = 1.0 / cn[0];
@@ -665,16 +693,20 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
     uw[r][0] = (float)u; uw[r][1] = (float)v; uw[r][2] = (float)w;
   }
   __syncthreads();
+  // face lift of the jumps: (PW (x) PW) Jq -> H[4][6][L][L], kept in the Jq region
+  mcon_mid<24, QF, 1, QF, L, FP, FP, NT>(T::oPW, Jq, A, tid);     // [24][L][QF]
+  __syncthreads();
+  mcon_last<24 * L, QF, L, FP, LP, NT>(T::oPW, A, Jq, tid);     // [24][L][L]
+  __syncthreads();
+  const float *H = Jq;
 
-  // lifted, projected gradients: G[k][x][ob] for the thread's pencils (prim, m, n)
   const double *mt = geo.metric + 9 * (size_t)e;
-  f4 G[NSG][3][NOB];
+  const float mu = (float)ph.vis, kap = (float)(ph.vis * ph.gamma / ph.Pr);
+  float tq[NPT][9];   // t11 t12 t13 t22 t23 t33, kappa grad e
 #pragma unroll
-  for (int k = 0; k < NSG; ++k)
+  for (int r = 0; r < NPT; ++r)
 #pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-      for (int ob = 0; ob < NOB; ++ob) G[k][x][ob] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 9; ++j) tq[r][j] = 0.f;
   auto grad_dir = [&](auto Dc) {
     constexpr int d = decltype(Dc)::value;
 #pragma unroll
@@ -692,86 +724,47 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
     for (int k = 0; k < NSG; ++k)
 #pragma unroll
       for (int ob = 0; ob < NOB; ++ob) D[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
-    mdiv_chain<P, 4, NSG, d>(A, B, D, tid);
-    const float g0 = (float)mt[3 * d], g1 = (float)mt[3 * d + 1], g2 = (float)mt[3 * d + 2];
-#pragma unroll
-    for (int k = 0; k < NSG; ++k)
-#pragma unroll
-      for (int ob = 0; ob < NOB; ++ob) {
-        G[k][0][ob] += g0 * D[k][ob];
-        G[k][1][ob] += g1 * D[k][ob];
-        G[k][2][ob] += g2 * D[k][ob];
-      }
-  };
-  grad_dir(IC<0>());
-  grad_dir(IC<1>());
-  grad_dir(IC<2>());
-  mface_project<P, 24>(Jq, A, B, tid);   // H in B: [4][6][L][L]
-#pragma unroll
-  for (int k = 0; k < NSG; ++k) {
-    const int Gi = tid + k * NT;
-    if (Gi < 4 * L2) {
-      const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        float cf[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) cf[s] = (float)geo.fnorm[3 * (e * 6 + s) + x];
-#pragma unroll
-        for (int l = 0; l < L; ++l) G[k][x][l / 4][l % 4] += mface_sum<P>(B + c * 6 * L * LP, l, m, nn, cf);
-      }
-    }
-  }
-  __syncthreads();
-
-  // nodal gradients: batch x holds d/dx_x of (u, v, w, e)
-  float gn[NPT][12];   // [3*prim + x]
-  auto grad_nodal = [&](auto Xc) {
-    constexpr int x = decltype(Xc)::value;
+    mdiv_chain<P, 4, NSG, d, NT>(A, B, D, tid);
+    const double g0d = mt[3 * d], g1d = mt[3 * d + 1], g2d = mt[3 * d + 2];
+    const float inv = (float)(1.0 / sqrt(g0d * g0d + g1d * g1d + g2d * g2d));
 #pragma unroll
     for (int k = 0; k < NSG; ++k) {
       const int Gi = tid + k * NT;
       if (Gi < 4 * L2) {
         const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
 #pragma unroll
-        for (int l = 0; l < L; ++l) A[((c * L + l) * L + m) * LP + nn] = G[k][x][l / 4][l % 4];
+        for (int l = 0; l < L; ++l)
+          A[((c * L + l) * L + m) * LP + nn] =
+              D[k][l / 4][l % 4] + mface_pair<P, d>(H + c * 6 * L * LP, l, m, nn, -inv, inv);
       }
     }
     __syncthreads();
-    mbackward<P, 4>(A, B, tid);
+    mbackward<P, 4, NT>(A, B, tid);
+    const float g0 = (float)g0d, g1 = (float)g1d, g2 = (float)g2d;
+    const float c43 = mu * (4.f / 3.f), c23 = mu * (2.f / 3.f);
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
       const int pn = S::nidx(n < Q3 ? n : 0);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) gn[r][3 * c + x] = n < Q3 ? B[c * FS + pn] : 0.f;
+      const float au = B[pn], av = B[FS + pn], aw = B[2 * FS + pn], ae = B[3 * FS + pn];
+      const float ux = g0 * au, uy = g1 * au, uz = g2 * au;
+      const float vx = g0 * av, vy = g1 * av, vz = g2 * av;
+      const float wx = g0 * aw, wy = g1 * aw, wz = g2 * aw;
+      tq[r][0] += c43 * ux - c23 * (vy + wz);
+      tq[r][1] += mu * (uy + vx);
+      tq[r][2] += mu * (uz + wx);
+      tq[r][3] += c43 * vy - c23 * (ux + wz);
+      tq[r][4] += mu * (vz + wy);
+      tq[r][5] += c43 * wz - c23 * (ux + vy);
+      tq[r][6] += kap * (g0 * ae);
+      tq[r][7] += kap * (g1 * ae);
+      tq[r][8] += kap * (g2 * ae);
     }
     __syncthreads();
   };
-  grad_nodal(IC<0>());
-  grad_nodal(IC<1>());
-  grad_nodal(IC<2>());
-
-  // viscous stress tensor and heat flux at the nodes: sg[r][3*eq + x]
-  const float mu = (float)ph.vis, kap = (float)(ph.vis * ph.gamma / ph.Pr);
-  float sg[NPT][12];
-#pragma unroll
-  for (int r = 0; r < NPT; ++r) {
-    const float ux = gn[r][0], uy = gn[r][1], uz = gn[r][2];
-    const float vx = gn[r][3], vy = gn[r][4], vz = gn[r][5];
-    const float wx = gn[r][6], wy = gn[r][7], wz = gn[r][8];
-    const float t12 = mu * (uy + vx), t13 = mu * (uz + wx), t23 = mu * (vz + wy);
-    const float t11 = mu * ((2.f / 3.f) * ((2.f * ux - vy) - wz));
-    const float t22 = mu * ((2.f / 3.f) * ((2.f * vy - ux) - wz));
-    const float t33 = mu * ((2.f / 3.f) * ((2.f * wz - ux) - vy));
-    const float u = uw[r][0], v = uw[r][1], w = uw[r][2];
-    sg[r][0] = t11; sg[r][1] = t12; sg[r][2] = t13;
-    sg[r][3] = t12; sg[r][4] = t22; sg[r][5] = t23;
-    sg[r][6] = t13; sg[r][7] = t23; sg[r][8] = t33;
-    sg[r][9] = t11 * u + t12 * v + t13 * w + kap * gn[r][9];
-    sg[r][10] = t12 * u + t22 * v + t23 * w + kap * gn[r][10];
-    sg[r][11] = t13 * u + t23 * v + t33 * w + kap * gn[r][11];
-  }
+  grad_dir(IC<0>());
+  grad_dir(IC<1>());
+  grad_dir(IC<2>());
 
   // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss points
   float *Lay = A;
@@ -793,7 +786,7 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
-              sg[r][3 * q] * n0 + sg[r][3 * q + 1] * n1 + sg[r][3 * q + 2] * n2;
+              mvisc_flux(tq[r], tq[r] + 6, uw[r][0], uw[r][1], uw[r][2], q, n0, n1, n2);
       }
     }
   }
@@ -824,11 +817,11 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
         const int pn = S::nidx(n);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          A[q * FS + pn] = m0 * sg[r][3 * q] + m1 * sg[r][3 * q + 1] + m2 * sg[r][3 * q + 2];
+          A[q * FS + pn] = mvisc_flux(tq[r], tq[r] + 6, uw[r][0], uw[r][1], uw[r][2], q, m0, m1, m2);
       }
     }
     __syncthreads();
-    mdiv_chain<P, 4, NSG, d>(A, B, acc, tid);
+    mdiv_chain<P, 4, NSG, d, NT>(A, B, acc, tid);
   };
   visc_dir(IC<0>());
   visc_dir(IC<1>());
@@ -864,9 +857,9 @@ template <int P>
 hipError_t launch_mixed3_p(const StageLaunch &s) {
   using S = M3<P>;
   if (s.op == 0) {
-    const int sh = S::lds_vterm;
+    const int sh = M3<P, kVterm3mThreads>::lds_vterm;
     (void)hipFuncSetAttribute((const void *)k_vterm3m<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3m<P>, dim3(s.ne), dim3(kStageThreads), sh, s.stream, s.U, s.T, s.Vfw, s.TVfw,
+    hipLaunchKernelGGL(k_vterm3m<P>, dim3(s.ne), dim3(kVterm3mThreads), sh, s.stream, s.U, s.T, s.Vfw, s.TVfw,
                        s.geo, s.phys, s.ne);
     return hipGetLastError();
   }

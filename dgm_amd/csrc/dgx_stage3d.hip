@@ -150,6 +150,10 @@ struct StageArgs {
 // every contraction phase is then a single pass, and a CU holds 4 waves per
 // SIMD instead of 2 within the same LDS.
 constexpr int kStage3Threads = 2 * kStageThreads;
+#ifndef DGX_VTERM_THREADS
+#define DGX_VTERM_THREADS 512
+#endif
+constexpr int kVterm3Threads = DGX_VTERM_THREADS;
 
 template <int P, bool NS, bool UPDATE>
 __global__ void __launch_bounds__(kStage3Threads)
@@ -188,21 +192,17 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
     const int qn = geo.perm[geo.permid[es] * QF2 + q];
     const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1], nz = geo.fnorm[3 * es + 2];
     const double jr = geo.fjr[es];
-    double UL[5], UR[5], FL[5], FR[5], lL, lR;
+    double UL[5], UR[5], FL[5];
     const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
     const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
     for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + q]; UR[v] = tr[v * QF2 + qn]; }
-    bool bl, br;
-    euler_normal_flux3(UL, nx, ny, nz, gm1, FL, lL, bl);
-    euler_normal_flux3(UR, nx, ny, nz, gm1, FR, lR, br);
-    bad |= bl | br;
-    const double lmax = fmax(lL, lR);
+    double Fh[5];
+    riemann3(UL, UR, nx, ny, nz, ph.gamma, ph.flux, FL, Fh, bad);
     const int qa = q / QF, qb = q - qa * QF;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-      double g = jr * (Fh - FL[v]);
+      double g = jr * (Fh[v] - FL[v]);
       if (NS && v > 0) {
         const double *vl = a.TV + ((size_t)e * 6 + s) * 4 * QF2;
         const double *vr = a.TV + ((size_t)nb * 6 + ns) * 4 * QF2;
@@ -329,13 +329,45 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   }
 }
 
+// Face-lift contribution of the two faces normal to reference direction D at
+// modal (l, m, n) of block set H ([6][L][L], pitch LP), with weights clo, chi
+// for the face at -1 and +1 (faces 4/2 for a, 1/3 for b, 0/5 for c).
+template <int P, int D>
+__device__ __forceinline__ double face_pair(const double *H, int l, int m, int n, double clo, double chi) {
+  constexpr int L = P + 1, LP = odd(L), HB = L * LP;
+  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
+  if (D == 0) return fma(clo * B0[l], H[4 * HB + m * LP + n], chi * B1[l] * H[2 * HB + m * LP + n]);
+  if (D == 1) return fma(clo * B0[m], H[1 * HB + l * LP + n], chi * B1[m] * H[3 * HB + l * LP + n]);
+  return fma(clo * B0[n], H[0 * HB + l * LP + m], chi * B1[n] * H[5 * HB + l * LP + m]);
+}
+
+// Viscous flux of equation q (1..3 momentum, 4 energy) contracted with the
+// vector (n0, n1, n2), from the stress tensor t = (t11 t12 t13 t22 t23 t33),
+// heat flux h = kappa grad e and the velocity (u, v, w).
+__device__ __forceinline__ double visc_flux(const double t[6], const double h[3], double u, double v,
+                                           double w, int q, double n0, double n1, double n2) {
+  const double r0 = t[0] * n0 + t[1] * n1 + t[2] * n2;   // row 1 of tau . n
+  const double r1 = t[1] * n0 + t[3] * n1 + t[4] * n2;
+  const double r2 = t[2] * n0 + t[4] * n1 + t[5] * n2;
+  if (q == 0) return r0;
+  if (q == 1) return r1;
+  if (q == 2) return r2;
+  return (r0 * u + r1 * v + r2 * w) + (h[0] * n0 + h[1] * n1 + h[2] * n2);
+}
+
 // BR1 viscous terms: V[e][eq][L3] = modal divergence of the viscous flux,
 // TV[e][s][eq][QF2] = viscous flux . n at the Gauss points of face s.
+// The lifted gradient is assembled one reference direction at a time: for an
+// affine hex the normals of the two faces across reference direction d are
+// -+ grad(xi_d)/|grad(xi_d)|, so the physical gradient is
+// sum_d dxi_d/dx (D_d + lift of those two faces / |grad xi_d|), and each
+// direction's nodal contribution goes straight into the stress tensor and heat
+// flux at the thread's nodes.
 template <int P>
-__global__ void __launch_bounds__(kStageThreads)
+__global__ void __launch_bounds__(kVterm3Threads)
 k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
          double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
-  using S = S3<P>;
+  using S = S3<P, kVterm3Threads>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
                 LP = S::LP, QP = S::QP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSG = S::NSG;
   extern __shared__ double sm[];
@@ -374,7 +406,7 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     for (int c = 0; c < 4; ++c) Jq[((c * 6 + s) * QF + qa) * FP + qb] = jr * (0.5 * (pl[c] + pq[c]) - pl[c]);
   }
   __syncthreads();
-  backward_batch<P, 5>(A, B, tid);
+  backward_batch<P, 5, NT>(A, B, tid);
   double pv[NPT][4];   // u, v, w, e at the thread's nodes
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
@@ -388,16 +420,20 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     pv[r][0] = u; pv[r][1] = v; pv[r][2] = w; pv[r][3] = -0.5 * wk + en;
   }
   __syncthreads();
+  // face lift of the jumps: (PW (x) PW) Jq -> H[4][6][L][L], kept in the Jq region
+  con_mid<24, QF, 1, QF, L, FP, FP, NT>(CT<P>::PW(), Jq, A, tid);     // [24][L][QF]
+  __syncthreads();
+  con_last<24 * L, QF, L, FP, LP, NT>(CT<P>::PW(), A, Jq, tid);     // [24][L][L]
+  __syncthreads();
+  const double *H = Jq;
 
-  // lifted, projected gradients: G[k][x][l] for the thread's pencils (prim, m, n)
   const double *mt = geo.metric + 9 * (size_t)e;
-  double G[NSG][3][L];
+  const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
+  double tq[NPT][9];   // t11 t12 t13 t22 t23 t33, kappa grad e
 #pragma unroll
-  for (int k = 0; k < NSG; ++k)
+  for (int r = 0; r < NPT; ++r)
 #pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-      for (int l = 0; l < L; ++l) G[k][x][l] = 0.0;
+    for (int j = 0; j < 9; ++j) tq[r][j] = 0.0;
   auto grad_dir = [&](auto Dc) {
     constexpr int d = decltype(Dc)::value;
 #pragma unroll
@@ -415,83 +451,46 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     for (int k = 0; k < NSG; ++k)
 #pragma unroll
       for (int l = 0; l < L; ++l) D[k][l] = 0.0;
-    div_chain<P, 4, NSG, d>(A, B, D, tid);
-#pragma unroll
-    for (int k = 0; k < NSG; ++k)
-#pragma unroll
-      for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int l = 0; l < L; ++l) G[k][x][l] = fma(mt[3 * d + x], D[k][l], G[k][x][l]);
-  };
-  grad_dir(IC<0>());
-  grad_dir(IC<1>());
-  grad_dir(IC<2>());
-  face_project<P, 24>(Jq, A, B, tid);   // H in B: [4][6][L][L]
-#pragma unroll
-  for (int k = 0; k < NSG; ++k) {
-    const int Gi = tid + k * NT;
-    if (Gi < 4 * L2) {
-      const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        double cf[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) cf[s] = geo.fnorm[3 * (e * 6 + s) + x];
-#pragma unroll
-        for (int l = 0; l < L; ++l) G[k][x][l] += face_sum<P>(B + c * 6 * L * LP, l, m, nn, cf);
-      }
-    }
-  }
-  __syncthreads();
-
-  // nodal gradients: batch x holds d/dx_x of (u, v, w, e)
-  double gn[NPT][12];   // [3*prim + x]
-  auto grad_nodal = [&](auto Xc) {
-    constexpr int x = decltype(Xc)::value;
+    div_chain<P, 4, NSG, d, NT>(A, B, D, tid);
+    const double g0 = mt[3 * d], g1 = mt[3 * d + 1], g2 = mt[3 * d + 2];
+    const double inv = 1.0 / sqrt(g0 * g0 + g1 * g1 + g2 * g2);
 #pragma unroll
     for (int k = 0; k < NSG; ++k) {
       const int Gi = tid + k * NT;
       if (Gi < 4 * L2) {
         const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
 #pragma unroll
-        for (int l = 0; l < L; ++l) A[((c * L + l) * L + m) * LP + nn] = G[k][x][l];
+        for (int l = 0; l < L; ++l)
+          A[((c * L + l) * L + m) * LP + nn] = D[k][l] + face_pair<P, d>(H + c * 6 * L * LP, l, m, nn, -inv, inv);
       }
     }
     __syncthreads();
-    backward_batch<P, 4>(A, B, tid);
+    backward_batch<P, 4, NT>(A, B, tid);
+    // nodal d/dxi_d of (u, v, w, e) -> stress and heat flux
+    const double c43 = mu * (4.0 / 3.0), c23 = mu * (2.0 / 3.0);
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       const int n = tid + r * NT;
       const int pn = S::nidx(n < Q3 ? n : 0);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) gn[r][3 * c + x] = n < Q3 ? B[c * FS + pn] : 0.0;
+      const double au = B[pn], av = B[FS + pn], aw = B[2 * FS + pn], ae = B[3 * FS + pn];
+      const double ux = g0 * au, uy = g1 * au, uz = g2 * au;
+      const double vx = g0 * av, vy = g1 * av, vz = g2 * av;
+      const double wx = g0 * aw, wy = g1 * aw, wz = g2 * aw;
+      tq[r][0] += c43 * ux - c23 * (vy + wz);
+      tq[r][1] += mu * (uy + vx);
+      tq[r][2] += mu * (uz + wx);
+      tq[r][3] += c43 * vy - c23 * (ux + wz);
+      tq[r][4] += mu * (vz + wy);
+      tq[r][5] += c43 * wz - c23 * (ux + vy);
+      tq[r][6] += kap * (g0 * ae);
+      tq[r][7] += kap * (g1 * ae);
+      tq[r][8] += kap * (g2 * ae);
     }
     __syncthreads();
   };
-  grad_nodal(IC<0>());
-  grad_nodal(IC<1>());
-  grad_nodal(IC<2>());
-
-  // viscous stress tensor and heat flux at the nodes: sg[r][3*eq + x]
-  const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
-  double sg[NPT][12];
-#pragma unroll
-  for (int r = 0; r < NPT; ++r) {
-    const double ux = gn[r][0], uy = gn[r][1], uz = gn[r][2];
-    const double vx = gn[r][3], vy = gn[r][4], vz = gn[r][5];
-    const double wx = gn[r][6], wy = gn[r][7], wz = gn[r][8];
-    const double t12 = mu * (uy + vx), t13 = mu * (uz + wx), t23 = mu * (vz + wy);
-    const double t11 = mu * ((2.0 / 3.0) * ((2.0 * ux - vy) - wz));
-    const double t22 = mu * ((2.0 / 3.0) * ((2.0 * vy - ux) - wz));
-    const double t33 = mu * ((2.0 / 3.0) * ((2.0 * wz - ux) - vy));
-    const double u = pv[r][0], v = pv[r][1], w = pv[r][2];
-    sg[r][0] = t11; sg[r][1] = t12; sg[r][2] = t13;
-    sg[r][3] = t12; sg[r][4] = t22; sg[r][5] = t23;
-    sg[r][6] = t13; sg[r][7] = t23; sg[r][8] = t33;
-    sg[r][9] = t11 * u + t12 * v + t13 * w + kap * gn[r][9];
-    sg[r][10] = t12 * u + t22 * v + t23 * w + kap * gn[r][10];
-    sg[r][11] = t13 * u + t23 * v + t33 * w + kap * gn[r][11];
-  }
+  grad_dir(IC<0>());
+  grad_dir(IC<1>());
+  grad_dir(IC<2>());
 
   // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss points
   double *Lay = A;
@@ -512,7 +511,7 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
-              sg[r][3 * q] * nn[0] + sg[r][3 * q + 1] * nn[1] + sg[r][3 * q + 2] * nn[2];
+              visc_flux(tq[r], tq[r] + 6, pv[r][0], pv[r][1], pv[r][2], q, nn[0], nn[1], nn[2]);
       }
     }
   }
@@ -543,11 +542,11 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
         const int pn = S::nidx(n);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          A[q * FS + pn] = m0 * sg[r][3 * q] + m1 * sg[r][3 * q + 1] + m2 * sg[r][3 * q + 2];
+          A[q * FS + pn] = visc_flux(tq[r], tq[r] + 6, pv[r][0], pv[r][1], pv[r][2], q, m0, m1, m2);
       }
     }
     __syncthreads();
-    div_chain<P, 4, NSG, d>(A, B, acc, tid);
+    div_chain<P, 4, NSG, d, NT>(A, B, acc, tid);
   };
   visc_dir(IC<0>());
   visc_dir(IC<1>());
@@ -570,9 +569,9 @@ static hipError_t launch_stage3_p(const StageLaunch &s) {
   a.T = s.T; a.V = s.V; a.TV = s.TV; a.U = s.U; a.Ut = s.Ut; a.R = s.R; a.Tn = s.Tn;
   a.ca = s.ca; a.cb = s.cb; a.dt = s.dt; a.ne = s.ne; a.flag = s.flag;
   if (s.op == 0) {
-    const int sh = S::lds_vterm;
+    const int sh = S3<P, kVterm3Threads>::lds_vterm;
     (void)hipFuncSetAttribute((const void *)k_vterm3<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(kStageThreads), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
+    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(kVterm3Threads), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
                        s.geo, s.phys, s.ne);
     return hipGetLastError();
   }

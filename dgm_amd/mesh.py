@@ -60,3 +60,22 @@ def periodic_hexes(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> Mesh:
         con[rows, 2] = le
         con[rows, 3] = ls
     return Mesh(nsd=3, vert=vert, con=con)
+
+
+def gll_points(q):
+    """The q Gauss-Lobatto-Legendre points on [-1, 1], ascending: +-1 and the
+    roots of P'_{q-1} (src/Polylib.cpp zwgll)."""
+    from numpy.polynomial import legendre
+    inner = legendre.Legendre.basis(q - 1).deriv().roots() if q > 2 else np.array([])
+    return np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
+
+
+def quad_nodes(mesh: Mesh, p: int) -> np.ndarray:
+    """Nodal coordinates [2][elem][q*q] of affine quads at the q = p+2 GLL
+    points, node i*q + j at (xi_i, eta_j), bilinear vertex map with vertices
+    counter-clockwise from (-1, -1) (src/Quad.cpp:993-1013)."""
+    z = gll_points(p + 2)
+    xi, eta = np.meshgrid(z, z, indexing="ij")
+    xi, eta = xi.ravel(), eta.ravel()
+    N = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)]) / 4
+    return np.einsum("edv,vq->deq", mesh.vert, N)

@@ -141,6 +141,70 @@ long long dgx_stage_count(dgx_handle h);
 int dgx_profile(dgx_handle h, int inttype, double dt, int nsteps, double *ms, int *count,
                 int nkinds);
 
+/* Device bytes a domain built from `desc` holds (state, RK registers,
+ * traces, viscous terms, nodal transfer buffer, geometry), without building
+ * it.  Used for admission against a memory budget. */
+int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop: continuous batching of independent forward solves.
+ *
+ * No counterpart in the reference.  It drives the step seam (dgx_advance) for
+ * optimisation and UQ ensembles.
+ *
+ * Admission and overlap:
+ *  - Requests are admitted FIFO as slots (max_active) and budget (hbm_budget,
+ *    by dgx_domain_bytes) free up.
+ *  - Each admitted request gets its own domain and HIP stream, so the work of
+ *    different requests overlaps on the device.
+ *
+ * Stepping:
+ *  - Each pass of dgx_server_progress enqueues the next slice of
+ *    steps_per_slice steps of every admitted request that has none in flight.
+ *  - The first slice of a request runs directly.
+ *  - The second slice is captured into a hipGraph, which later full slices
+ *    replay, so small meshes are not bound by launch overhead.
+ *  - A request's result does not depend on what else runs: each request's
+ *    kernels see only its own buffers.
+ *
+ * Not thread safe: one host thread drives a server. */
+typedef struct dgx_server *dgx_server_handle;
+
+typedef struct {
+  size_t hbm_budget;     /* device bytes the admitted requests may hold together */
+  int max_active;        /* admitted requests at once */
+  int steps_per_slice;   /* steps enqueued per request and pass */
+  int use_graphs;        /* replay slices from a captured hipGraph (1) or launch directly (0) */
+} dgx_server_config;
+
+enum dgx_request_state { DGX_QUEUED = 0, DGX_RUNNING = 1, DGX_DONE = 2, DGX_FAILED = 3 };
+
+int dgx_server_create(const dgx_server_config *cfg, dgx_server_handle *out);
+int dgx_server_destroy(dgx_server_handle s);
+
+/* Queue a forward solve of `nsteps` steps of size `dt` with integrator
+ * `inttype` from the initial state U0 (host memory, `layout` as in
+ * dgx_state_upload).  `desc` and U0 are copied.  Fails with DGX_ERR_ARGUMENT
+ * if the request alone exceeds the budget. */
+int dgx_server_submit(dgx_server_handle s, const dgx_domain_desc *desc, const double *U0, int layout,
+                      int inttype, double dt, int nsteps, long long *id);
+
+/* One scheduling pass (non-blocking): retire finished slices and requests,
+ * admit queued requests, enqueue slices.  *pending = requests not yet done or
+ * failed. */
+int dgx_server_progress(dgx_server_handle s, int *pending);
+
+/* State of a request (enum dgx_request_state). */
+int dgx_server_status(dgx_server_handle s, long long id, int *state);
+
+/* Result of a finished request: the final state in `layout` (may be NULL),
+ * and times[3] = seconds from server creation at submission, admission and
+ * completion (may be NULL). */
+int dgx_server_result(dgx_server_handle s, long long id, double *U, int layout, double *times);
+
+/* Peak device bytes held by admitted requests, and peak admitted count. */
+int dgx_server_peak(dgx_server_handle s, size_t *bytes, int *active);
+
 const char *dgx_last_error(void);
 const char *dgx_version(void);
 

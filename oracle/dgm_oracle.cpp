@@ -189,6 +189,175 @@ void mm_nt(int M, int N, int K, const double *A, int lda, const double *B, int l
     }
 }
 
+// 2D Riemann flux at one face point (src/Euler.cpp:399-757): physical normal
+// fluxes of both states (Fpl, Fmr, Cartesian) and the numerical flux Fh.
+// Returns 1 for a non-realizable state.
+int point_flux2d(double gamma, int flux_type, const double Ul[4], const double Ur[4], double nxl, double nyl,
+                 double Fpl[4], double Fmr[4], double Fh[4]) {
+  const double gm1 = gamma - 1.0, pt5 = 0.5;
+  int status = 0;
+  const double rl = Ul[0], rul = Ul[1] * nxl + Ul[2] * nyl, rvl = Ul[2] * nxl - Ul[1] * nyl, rEl = Ul[3];
+  const double irl = 1.0 / rl;
+  const double ul = irl * rul, vl = irl * rvl;
+  const double pl = gm1 * (rEl - pt5 * (rul * ul + rvl * vl));
+  const double Hl = irl * (rEl + pl);
+  if (!(rl > 0) || !(pl > 0)) status = 1;
+  Fpl[0] = rul;
+  Fpl[1] = (rul * ul + pl) * nxl - (rvl * ul) * nyl;
+  Fpl[2] = (rul * ul + pl) * nyl + (rvl * ul) * nxl;
+  Fpl[3] = ul * (rEl + pl);
+  const double rr = Ur[0], rur = Ur[1] * nxl + Ur[2] * nyl, rvr = Ur[2] * nxl - Ur[1] * nyl, rEr = Ur[3];
+  const double irr = 1.0 / rr;
+  const double ur = irr * rur, vr = irr * rvr;
+  const double pr = gm1 * (rEr - pt5 * (rur * ur + rvr * vr));
+  const double Hr = irr * (rEr + pr);
+  if (!(rr > 0) || !(pr > 0)) status = 1;
+  Fmr[0] = rr * ur;
+  Fmr[1] = (rur * ur + pr) * nxl - (rvr * ur) * nyl;
+  Fmr[2] = (rur * ur + pr) * nyl + (rvr * ur) * nxl;
+  Fmr[3] = ur * (rEr + pr);
+  if (flux_type == 0) {
+    double lmax = 0;
+    const double al = std::sqrt(gm1 * (Hl - pt5 * (ul * ul + vl * vl)));
+    const double laml[4] = {ul - al, ul, ul, ul + al};
+    for (int n = 0; n < 4; ++n) lmax = std::max(lmax, std::fabs(laml[n]));
+    const double ar = std::sqrt(gm1 * (Hr - pt5 * (ur * ur + vr * vr)));
+    const double lamr[4] = {ur - ar, ur, ur, ur + ar};
+    for (int n = 0; n < 4; ++n) lmax = std::max(lmax, std::fabs(lamr[n]));
+    for (int n = 0; n < 4; ++n) Fh[n] = pt5 * (Fpl[n] + Fmr[n] + lmax * (Ul[n] - Ur[n]));
+  } else {
+    // Roe average flux without entropy fix (src/Euler.cpp:682-741)
+    const double R = std::sqrt(rr / rl);
+    const double iRp1 = 1.0 / (1.0 + R);
+    const double uu = (ul + R * ur) * iRp1, vv = (vl + R * vr) * iRp1, H = (Hl + R * Hr) * iRp1;
+    const double a2 = gm1 * (H - pt5 * (uu * uu + vv * vv));
+    const double a = std::sqrt(a2), ainv = 1.0 / a;
+    const double c1 = gm1 * ainv * ainv, c1inv = 1.0 / c1;
+    double alpha[4], lam[4], Gh[4];
+    alpha[2] = (rvr - rvl) - vv * (rr - rl);
+    alpha[1] = (gamma - 1.0) * ainv * ainv * ((rr - rl) * (H - uu * uu) + uu * (rur - rul) -
+                                             ((rEr - rEl) - ((rvr - rvl) - vv * (rr - rl)) * vv));
+    alpha[0] = pt5 * ainv * ((rr - rl) * (uu + a) - (rur - rul) - a * alpha[1]);
+    alpha[3] = (rr - rl) - (alpha[0] + alpha[1]);
+    lam[0] = std::min(uu - a, 0.0); lam[1] = std::min(uu, 0.0);
+    lam[2] = std::min(uu, 0.0);     lam[3] = std::min(uu + a, 0.0);
+    Gh[0] = lam[0] * alpha[0];
+    Gh[1] = lam[0] * (uu - a) * alpha[0];
+    Gh[2] = lam[0] * vv * alpha[0];
+    Gh[3] = lam[0] * (H - uu * a) * alpha[0];
+    Gh[0] += lam[1] * alpha[1];
+    Gh[1] += lam[1] * uu * alpha[1];
+    Gh[2] += lam[1] * vv * alpha[1];
+    Gh[3] += lam[1] * (H - c1inv) * alpha[1];
+    Gh[2] += lam[2] * alpha[2];
+    Gh[3] += lam[2] * vv * alpha[2];
+    Gh[0] += lam[3] * alpha[3];
+    Gh[1] += lam[3] * (uu + a) * alpha[3];
+    Gh[2] += lam[3] * vv * alpha[3];
+    Gh[3] += lam[3] * (H + uu * a) * alpha[3];
+    Fh[0] = Fpl[0] + Gh[0];
+    Fh[1] = Fpl[1] + Gh[1] * nxl - Gh[2] * nyl;
+    Fh[2] = Fpl[2] + Gh[1] * nyl + Gh[2] * nxl;
+    Fh[3] = Fpl[3] + Gh[3];
+  }
+  return status;
+}
+
+// 3D Riemann flux at one face point in the rotated frame
+// (src/Euler3d.cpp:584-817); as point_flux2d.
+int point_flux3d(double gamma, int flux_type, const double Ul[5], const double Ur[5], double nx, double ny,
+                 double nz, double Fpl[5], double Fmr[5], double Fh[5]) {
+  const double gm1 = gamma - 1.0, pt5 = 0.5;
+  int status = 0;
+  double sx = ny, sy = nz, sz = nx;
+  const double sdn = nx * sx + ny * sy + nz * sz;
+  sx -= sdn * nx; sy -= sdn * ny; sz -= sdn * nz;
+  const double snrm_inv = 1.0 / std::sqrt(sx * sx + sy * sy + sz * sz);
+  sx *= snrm_inv; sy *= snrm_inv; sz *= snrm_inv;
+  double tx = ny * sz - sy * nz, ty = nz * sx - sz * nx, tz = nx * sy - sx * ny;
+  const double tnrm_inv = 1.0 / std::sqrt(tx * tx + ty * ty + tz * tz);
+  tx *= tnrm_inv; ty *= tnrm_inv; tz *= tnrm_inv;
+  const double detA = (nx * sy * tz + sx * ty * nz + tx * sz * ny) - (nz * sy * tx + sz * ty * nx + tz * sx * ny);
+  const double idetA = 1.0 / detA;
+  const double xn = (sy * tz - sz * ty) * idetA, yn = -(sx * tz - sz * tx) * idetA, zn = (sx * ty - sy * tx) * idetA;
+  const double xs = -(ny * tz - nz * ty) * idetA, ys = (nx * tz - nz * tx) * idetA, zs = -(nx * ty - ny * tx) * idetA;
+  const double xt = (ny * sz - nz * sy) * idetA, yt = -(nx * sz - nz * sx) * idetA, zt = (nx * sy - ny * sx) * idetA;
+  const double rl = Ul[0];
+  const double rul = Ul[1] * nx + Ul[2] * ny + Ul[3] * nz;
+  const double rvl = Ul[1] * sx + Ul[2] * sy + Ul[3] * sz;
+  const double rwl = Ul[1] * tx + Ul[2] * ty + Ul[3] * tz;
+  const double rEl = Ul[4];
+  const double irl = 1.0 / rl;
+  const double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+  const double pl = gm1 * (rEl - pt5 * (rul * ul + rvl * vl + rwl * wl));
+  const double Hl = irl * (rEl + pl);
+  if (!(rl > 0) || !(pl > 0)) status = 1;
+  Fpl[0] = rl * ul;
+  Fpl[1] = (rul * ul + pl) * xn + (rvl * ul) * xs + (rwl * ul) * xt;
+  Fpl[2] = (rul * ul + pl) * yn + (rvl * ul) * ys + (rwl * ul) * yt;
+  Fpl[3] = (rul * ul + pl) * zn + (rvl * ul) * zs + (rwl * ul) * zt;
+  Fpl[4] = ul * (rEl + pl);
+  const double rr = Ur[0];
+  const double rur = Ur[1] * nx + Ur[2] * ny + Ur[3] * nz;
+  const double rvr = Ur[1] * sx + Ur[2] * sy + Ur[3] * sz;
+  const double rwr = Ur[1] * tx + Ur[2] * ty + Ur[3] * tz;
+  const double rEr = Ur[4];
+  const double irr = 1.0 / rr;
+  const double ur = irr * rur, vr = irr * rvr, wr = irr * rwr;
+  const double pr = gm1 * (rEr - pt5 * (rur * ur + rvr * vr + rwr * wr));
+  const double Hr = irr * (rEr + pr);
+  if (!(rr > 0) || !(pr > 0)) status = 1;
+  Fmr[0] = rr * ur;
+  Fmr[1] = (rur * ur + pr) * xn + (rvr * ur) * xs + (rwr * ur) * xt;
+  Fmr[2] = (rur * ur + pr) * yn + (rvr * ur) * ys + (rwr * ur) * yt;
+  Fmr[3] = (rur * ur + pr) * zn + (rvr * ur) * zs + (rwr * ur) * zt;
+  Fmr[4] = ur * (rEr + pr);
+  if (flux_type == 0) {
+    double lmax = 0;
+    const double al = std::sqrt(gm1 * (Hl - pt5 * (ul * ul + vl * vl + wl * wl)));
+    const double laml[5] = {ul - al, ul, ul, ul, ul + al};
+    for (int n = 0; n < 5; ++n) lmax = std::max(lmax, std::fabs(laml[n]));
+    const double ar = std::sqrt(gm1 * (Hr - pt5 * (ur * ur + vr * vr + wr * wr)));
+    const double lamr[5] = {ur - ar, ur, ur, ur, ur + ar};
+    for (int n = 0; n < 5; ++n) lmax = std::max(lmax, std::fabs(lamr[n]));
+    for (int n = 0; n < 5; ++n) Fh[n] = pt5 * (Fpl[n] + Fmr[n] + lmax * (Ul[n] - Ur[n]));
+  } else {
+    // Roe flux without entropy fix: the reference's 2D Roe flux
+    // (src/Euler.cpp:682-741) in the rotated frame (n, s, t) of
+    // src/Euler3d.cpp:584-817, with a second shear wave for t.  The
+    // reference has no 3D Roe flux: parity unpinned, pinned by
+    // consistency, conservation and its reduction to the 2D flux.
+    const double R = std::sqrt(rr / rl);
+    const double iRp1 = 1.0 / (1.0 + R);
+    const double uu = (ul + R * ur) * iRp1, vv = (vl + R * vr) * iRp1, ww = (wl + R * wr) * iRp1;
+    const double H = (Hl + R * Hr) * iRp1;
+    const double a2 = gm1 * (H - pt5 * (uu * uu + vv * vv + ww * ww));
+    const double a = std::sqrt(a2), ainv = 1.0 / a;
+    const double c1inv = a2 / gm1;
+    double alpha[5], lam[5], Gh[5];
+    alpha[2] = (rvr - rvl) - vv * (rr - rl);
+    alpha[3] = (rwr - rwl) - ww * (rr - rl);
+    alpha[1] = gm1 * ainv * ainv * ((rr - rl) * (H - uu * uu) + uu * (rur - rul) -
+                                    ((rEr - rEl) - alpha[2] * vv - alpha[3] * ww));
+    alpha[0] = pt5 * ainv * ((rr - rl) * (uu + a) - (rur - rul) - a * alpha[1]);
+    alpha[4] = (rr - rl) - (alpha[0] + alpha[1]);
+    lam[0] = std::min(uu - a, 0.0); lam[1] = std::min(uu, 0.0); lam[2] = lam[1];
+    lam[3] = lam[1];                lam[4] = std::min(uu + a, 0.0);
+    Gh[0] = lam[0] * alpha[0] + lam[1] * alpha[1] + lam[4] * alpha[4];
+    Gh[1] = lam[0] * (uu - a) * alpha[0] + lam[1] * uu * alpha[1] + lam[4] * (uu + a) * alpha[4];
+    Gh[2] = lam[0] * vv * alpha[0] + lam[1] * vv * alpha[1] + lam[2] * alpha[2] + lam[4] * vv * alpha[4];
+    Gh[3] = lam[0] * ww * alpha[0] + lam[1] * ww * alpha[1] + lam[3] * alpha[3] + lam[4] * ww * alpha[4];
+    Gh[4] = lam[0] * (H - uu * a) * alpha[0] + lam[1] * (H - c1inv) * alpha[1] +
+            lam[2] * vv * alpha[2] + lam[3] * ww * alpha[3] + lam[4] * (H + uu * a) * alpha[4];
+    Fh[0] = Fpl[0] + Gh[0];
+    Fh[1] = Fpl[1] + Gh[1] * xn + Gh[2] * xs + Gh[3] * xt;
+    Fh[2] = Fpl[2] + Gh[1] * yn + Gh[2] * ys + Gh[3] * yt;
+    Fh[3] = Fpl[3] + Gh[1] * zn + Gh[2] * zs + Gh[3] * zt;
+    Fh[4] = Fpl[4] + Gh[4];
+  }
+  return status;
+}
+
 // ---------------------------------------------------------------------------
 // Domain
 // ---------------------------------------------------------------------------
@@ -701,11 +870,10 @@ struct Domain {
       for (int n = 0; n < nv; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
   }
 
-  int flux_type = 0;   // 0 Lax-Friedrichs, 1 Roe (2D only)
+  int flux_type = 0;   // 0 Lax-Friedrichs, 1 Roe
 
   // 2D master-side Riemann flux (src/Euler.cpp:399-757)
   void convective_flux2d(VF &Us, VF &Uf) {
-    const double gm1 = gamma - 1.0, pt5 = 0.5;
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
@@ -715,73 +883,10 @@ struct Domain {
           const double nxl = S.nx[ql], nyl = S.ny[ql];
           double Ul[4], Ur[4], Fpl[4], Fmr[4], Fh[4];
           for (int n = 0; n < 4; ++n) Ul[n] = es(Us, n, e, s)[ql];
-          const double rl = Ul[0], rul = Ul[1] * nxl + Ul[2] * nyl, rvl = Ul[2] * nxl - Ul[1] * nyl, rEl = Ul[3];
-          const double irl = 1.0 / rl;
-          const double ul = irl * rul, vl = irl * rvl;
-          const double pl = gm1 * (rEl - pt5 * (rul * ul + rvl * vl));
-          const double Hl = irl * (rEl + pl);
-          if (!(rl > 0) || !(pl > 0)) status = 1;
-          Fpl[0] = rul;
-          Fpl[1] = (rul * ul + pl) * nxl - (rvl * ul) * nyl;
-          Fpl[2] = (rul * ul + pl) * nyl + (rvl * ul) * nxl;
-          Fpl[3] = ul * (rEl + pl);
-          for (int n = 0; n < 4; ++n) es(Us, n, e, s)[ql] = Fpl[n];
           for (int n = 0; n < 4; ++n) Ur[n] = es(Us, n, S.le, S.ls)[qr];
-          const double rr = Ur[0], rur = Ur[1] * nxl + Ur[2] * nyl, rvr = Ur[2] * nxl - Ur[1] * nyl, rEr = Ur[3];
-          const double irr = 1.0 / rr;
-          const double ur = irr * rur, vr = irr * rvr;
-          const double pr = gm1 * (rEr - pt5 * (rur * ur + rvr * vr));
-          const double Hr = irr * (rEr + pr);
-          if (!(rr > 0) || !(pr > 0)) status = 1;
-          Fmr[0] = rr * ur;
-          Fmr[1] = (rur * ur + pr) * nxl - (rvr * ur) * nyl;
-          Fmr[2] = (rur * ur + pr) * nyl + (rvr * ur) * nxl;
-          Fmr[3] = ur * (rEr + pr);
+          status |= point_flux2d(gamma, flux_type, Ul, Ur, nxl, nyl, Fpl, Fmr, Fh);
+          for (int n = 0; n < 4; ++n) es(Us, n, e, s)[ql] = Fpl[n];
           for (int n = 0; n < 4; ++n) es(Us, n, S.le, S.ls)[qr] = Fmr[n];
-          if (flux_type == 0) {
-            double lmax = 0;
-            const double al = std::sqrt(gm1 * (Hl - pt5 * (ul * ul + vl * vl)));
-            const double laml[4] = {ul - al, ul, ul, ul + al};
-            for (int n = 0; n < 4; ++n) lmax = std::max(lmax, std::fabs(laml[n]));
-            const double ar = std::sqrt(gm1 * (Hr - pt5 * (ur * ur + vr * vr)));
-            const double lamr[4] = {ur - ar, ur, ur, ur + ar};
-            for (int n = 0; n < 4; ++n) lmax = std::max(lmax, std::fabs(lamr[n]));
-            for (int n = 0; n < 4; ++n) Fh[n] = pt5 * (Fpl[n] + Fmr[n] + lmax * (Ul[n] - Ur[n]));
-          } else {
-            // Roe average flux without entropy fix (src/Euler.cpp:682-741)
-            const double R = std::sqrt(rr / rl);
-            const double iRp1 = 1.0 / (1.0 + R);
-            const double uu = (ul + R * ur) * iRp1, vv = (vl + R * vr) * iRp1, H = (Hl + R * Hr) * iRp1;
-            const double a2 = gm1 * (H - pt5 * (uu * uu + vv * vv));
-            const double a = std::sqrt(a2), ainv = 1.0 / a;
-            const double c1 = gm1 * ainv * ainv, c1inv = 1.0 / c1;
-            double alpha[4], lam[4], Gh[4];
-            alpha[2] = (rvr - rvl) - vv * (rr - rl);
-            alpha[1] = (gamma - 1.0) * ainv * ainv * ((rr - rl) * (H - uu * uu) + uu * (rur - rul) -
-                                                     ((rEr - rEl) - ((rvr - rvl) - vv * (rr - rl)) * vv));
-            alpha[0] = pt5 * ainv * ((rr - rl) * (uu + a) - (rur - rul) - a * alpha[1]);
-            alpha[3] = (rr - rl) - (alpha[0] + alpha[1]);
-            lam[0] = std::min(uu - a, 0.0); lam[1] = std::min(uu, 0.0);
-            lam[2] = std::min(uu, 0.0);     lam[3] = std::min(uu + a, 0.0);
-            Gh[0] = lam[0] * alpha[0];
-            Gh[1] = lam[0] * (uu - a) * alpha[0];
-            Gh[2] = lam[0] * vv * alpha[0];
-            Gh[3] = lam[0] * (H - uu * a) * alpha[0];
-            Gh[0] += lam[1] * alpha[1];
-            Gh[1] += lam[1] * uu * alpha[1];
-            Gh[2] += lam[1] * vv * alpha[1];
-            Gh[3] += lam[1] * (H - c1inv) * alpha[1];
-            Gh[2] += lam[2] * alpha[2];
-            Gh[3] += lam[2] * vv * alpha[2];
-            Gh[0] += lam[3] * alpha[3];
-            Gh[1] += lam[3] * (uu + a) * alpha[3];
-            Gh[2] += lam[3] * vv * alpha[3];
-            Gh[3] += lam[3] * (H + uu * a) * alpha[3];
-            Fh[0] = Fpl[0] + Gh[0];
-            Fh[1] = Fpl[1] + Gh[1] * nxl - Gh[2] * nyl;
-            Fh[2] = Fpl[2] + Gh[1] * nyl + Gh[2] * nxl;
-            Fh[3] = Fpl[3] + Gh[3];
-          }
           for (int n = 0; n < 4; ++n) {
             es(Us, n, S.le, S.ls)[qr] *= -1.0;
             es(Uf, n, e, s)[ql] = Fh[n];
@@ -852,9 +957,8 @@ struct Domain {
       for (int n = 0; n < nv; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
   }
 
-  // 3D master-side Lax-Friedrichs flux in the rotated frame (src/Euler3d.cpp:584-817)
+  // 3D master-side Lax-Friedrichs (or Roe) flux in the rotated frame (src/Euler3d.cpp:584-817)
   void convective_flux3d(VF &Us, VF &Uf) {
-    const double gm1 = gamma - 1.0, pt5 = 0.5;
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
@@ -862,62 +966,12 @@ struct Domain {
         for (int g = 0; g < qs; ++g) {
           const int ql = sidemap(S, g), qr = sidemap_slave(g);
           const double nx = S.nx[ql], ny = S.ny[ql], nz = S.nz[ql];
-          double sx = ny, sy = nz, sz = nx;
-          const double sdn = nx * sx + ny * sy + nz * sz;
-          sx -= sdn * nx; sy -= sdn * ny; sz -= sdn * nz;
-          const double snrm_inv = 1.0 / std::sqrt(sx * sx + sy * sy + sz * sz);
-          sx *= snrm_inv; sy *= snrm_inv; sz *= snrm_inv;
-          double tx = ny * sz - sy * nz, ty = nz * sx - sz * nx, tz = nx * sy - sx * ny;
-          const double tnrm_inv = 1.0 / std::sqrt(tx * tx + ty * ty + tz * tz);
-          tx *= tnrm_inv; ty *= tnrm_inv; tz *= tnrm_inv;
-          const double detA = (nx * sy * tz + sx * ty * nz + tx * sz * ny) - (nz * sy * tx + sz * ty * nx + tz * sx * ny);
-          const double idetA = 1.0 / detA;
-          const double xn = (sy * tz - sz * ty) * idetA, yn = -(sx * tz - sz * tx) * idetA, zn = (sx * ty - sy * tx) * idetA;
-          const double xs = -(ny * tz - nz * ty) * idetA, ys = (nx * tz - nz * tx) * idetA, zs = -(nx * ty - ny * tx) * idetA;
-          const double xt = (ny * sz - nz * sy) * idetA, yt = -(nx * sz - nz * sx) * idetA, zt = (nx * sy - ny * sx) * idetA;
           double Ul[5], Ur[5], Fpl[5], Fmr[5], Fh[5];
           for (int n = 0; n < 5; ++n) Ul[n] = es(Us, n, e, s)[ql];
-          const double rl = Ul[0];
-          const double rul = Ul[1] * nx + Ul[2] * ny + Ul[3] * nz;
-          const double rvl = Ul[1] * sx + Ul[2] * sy + Ul[3] * sz;
-          const double rwl = Ul[1] * tx + Ul[2] * ty + Ul[3] * tz;
-          const double rEl = Ul[4];
-          const double irl = 1.0 / rl;
-          const double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
-          const double pl = gm1 * (rEl - pt5 * (rul * ul + rvl * vl + rwl * wl));
-          const double Hl = irl * (rEl + pl);
-          if (!(rl > 0) || !(pl > 0)) status = 1;
-          Fpl[0] = rl * ul;
-          Fpl[1] = (rul * ul + pl) * xn + (rvl * ul) * xs + (rwl * ul) * xt;
-          Fpl[2] = (rul * ul + pl) * yn + (rvl * ul) * ys + (rwl * ul) * yt;
-          Fpl[3] = (rul * ul + pl) * zn + (rvl * ul) * zs + (rwl * ul) * zt;
-          Fpl[4] = ul * (rEl + pl);
-          for (int n = 0; n < 5; ++n) es(Us, n, e, s)[ql] = Fpl[n];
           for (int n = 0; n < 5; ++n) Ur[n] = es(Us, n, S.le, S.ls)[qr];
-          const double rr = Ur[0];
-          const double rur = Ur[1] * nx + Ur[2] * ny + Ur[3] * nz;
-          const double rvr = Ur[1] * sx + Ur[2] * sy + Ur[3] * sz;
-          const double rwr = Ur[1] * tx + Ur[2] * ty + Ur[3] * tz;
-          const double rEr = Ur[4];
-          const double irr = 1.0 / rr;
-          const double ur = irr * rur, vr = irr * rvr, wr = irr * rwr;
-          const double pr = gm1 * (rEr - pt5 * (rur * ur + rvr * vr + rwr * wr));
-          const double Hr = irr * (rEr + pr);
-          if (!(rr > 0) || !(pr > 0)) status = 1;
-          Fmr[0] = rr * ur;
-          Fmr[1] = (rur * ur + pr) * xn + (rvr * ur) * xs + (rwr * ur) * xt;
-          Fmr[2] = (rur * ur + pr) * yn + (rvr * ur) * ys + (rwr * ur) * yt;
-          Fmr[3] = (rur * ur + pr) * zn + (rvr * ur) * zs + (rwr * ur) * zt;
-          Fmr[4] = ur * (rEr + pr);
+          status |= point_flux3d(gamma, flux_type, Ul, Ur, nx, ny, nz, Fpl, Fmr, Fh);
+          for (int n = 0; n < 5; ++n) es(Us, n, e, s)[ql] = Fpl[n];
           for (int n = 0; n < 5; ++n) es(Us, n, S.le, S.ls)[qr] = Fmr[n];
-          double lmax = 0;
-          const double al = std::sqrt(gm1 * (Hl - pt5 * (ul * ul + vl * vl + wl * wl)));
-          const double laml[5] = {ul - al, ul, ul, ul, ul + al};
-          for (int n = 0; n < 5; ++n) lmax = std::max(lmax, std::fabs(laml[n]));
-          const double ar = std::sqrt(gm1 * (Hr - pt5 * (ur * ur + vr * vr + wr * wr)));
-          const double lamr[5] = {ur - ar, ur, ur, ur, ur + ar};
-          for (int n = 0; n < 5; ++n) lmax = std::max(lmax, std::fabs(lamr[n]));
-          for (int n = 0; n < 5; ++n) Fh[n] = pt5 * (Fpl[n] + Fmr[n] + lmax * (Ul[n] - Ur[n]));
           for (int n = 0; n < 5; ++n) {
             es(Us, n, S.le, S.ls)[qr] *= -1.0;
             es(Uf, n, e, s)[ql] = Fh[n];
@@ -1202,6 +1256,14 @@ void *orc_create(int nsd, int ne, int p, const double *vert, int ncon, const int
   return d;
 }
 void orc_destroy(void *h) { delete (Domain *)h; }
+// Numerical flux at one face point: nsd 2 (U, F: 4 values) or 3 (5 values);
+// flux 0 Lax-Friedrichs, 1 Roe.  Returns 1 for a non-realizable state.
+int orc_riemann(int nsd, int flux, double gamma, const double *Ul, const double *Ur, const double *n,
+                double *Fh) {
+  double Fpl[5], Fmr[5];
+  if (nsd == 2) return point_flux2d(gamma, flux, Ul, Ur, n[0], n[1], Fpl, Fmr, Fh);
+  return point_flux3d(gamma, flux, Ul, Ur, n[0], n[1], n[2], Fpl, Fmr, Fh);
+}
 int orc_qtot(void *h) { return ((Domain *)h)->qtot; }
 int orc_nmodes(void *h) { return ((Domain *)h)->nm; }
 int orc_status(void *h) { return ((Domain *)h)->status; }

@@ -45,6 +45,8 @@ def _load():
     lib.orc_rhs.restype = ctypes.c_int
     lib.orc_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     lib.orc_advance.restype = ctypes.c_int
+    lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+    lib.orc_riemann.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -139,6 +141,20 @@ class Oracle:
         D, B = np.zeros((q, q)), np.zeros((L, q))
         self.lib.orc_basis(self.h, _ptr(z), _ptr(w), _ptr(zg), _ptr(wg), _ptr(D), _ptr(B))
         return dict(z=z, w=w, zg=zg, wg=wg, D=D, B=B)
+
+
+def riemann(nsd, flux, UL, UR, n, gamma=1.4):
+    """Numerical flux of the oracle at one face point (Cartesian components)."""
+    lib = _load()
+    UL = np.ascontiguousarray(UL, dtype=np.float64)
+    UR = np.ascontiguousarray(UR, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    if UL.shape != (nsd + 2,) or UR.shape != (nsd + 2,) or n.shape != (nsd,):
+        raise ValueError("riemann: state or normal size")
+    F = np.zeros(nsd + 2)
+    if lib.orc_riemann(nsd, flux, gamma, _ptr(UL), _ptr(UR), _ptr(n), _ptr(F)):
+        raise FloatingPointError("non-realizable state")
+    return F
 
 
 def dgm_diff(oracle_a: Oracle, Uh_ref, Uh_test):

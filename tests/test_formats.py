@@ -83,3 +83,12 @@ def test_periodic_quads_connectivity():
         c1 = m.vert[le][:, edges[ls]].mean(axis=1)
         d = (c0 - c1) / np.array([3.0, 1.0])
         assert np.allclose(d - np.round(d), 0.0)
+
+
+@pytest.mark.parametrize("p", [2, 5, 8])
+def test_quad_nodes_match_oracle(p):
+    from oracle_lib import Oracle
+    m = meshgen.periodic_quads(3, 2, 0.0, 3.0, -1.0, 1.5)
+    x = meshgen.quad_nodes(m, p)
+    ref = Oracle(m, p, 1).coords()
+    assert np.allclose(x, ref, rtol=0, atol=1e-13)

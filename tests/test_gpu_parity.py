@@ -74,6 +74,18 @@ def test_rhs_taylor_green_ns3d(p):
     check_rhs(m, p, 9, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), vis=1.0 / 100.0)
 
 
+@pytest.mark.parametrize("p", [3, 7, 10])
+def test_rhs_roe_euler3d(p):
+    """3D Roe flux (fused kernels for p <= 9, unfused for p = 10)."""
+    m = meshgen.periodic_hexes(3, 2, 2, (0, 0, 0), (2 * np.pi,) * 3)
+    check_rhs(m, p, 8, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), flux=1)
+
+
+def test_rhs_roe_ns3d():
+    m = meshgen.periodic_hexes(3, 3, 2, (0, 0, 0), (2 * np.pi,) * 3)
+    check_rhs(m, 4, 9, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), vis=0.01, flux=1)
+
+
 def test_rhs_vquad_ns2d_p10():
     prm, m, icd = load_case("vquad")
     check_rhs(m, 10, 2, lambda x: perturbed(initial_state(icd, x, 2), x), vis=0.02)

@@ -62,7 +62,7 @@ def _desc(m, p, eqn, **kw):
     (dict(p=0), b"p must be"),
     (dict(p=11), b"p must be"),
     (dict(eqn=1), b"eqntype"),
-    (dict(flux=1), b"Roe"),
+    (dict(flux=7), b"flux must be"),
     (dict(p=10, eqn=9), b"p <= 9"),
     (dict(p=9, eqn=8, precision=1), b"DGX_MIXED"),
     (dict(p=3, eqn=8, precision=7), b"precision"),
@@ -92,3 +92,36 @@ def test_null_arguments_rejected(lib):
     assert lib.dgx_rhs(None) == -1
     assert lib.dgx_advance(None, 14, 1e-3, 1) == -1
     assert lib.dgx_domain_destroy(None) == 0
+
+
+def test_server_argument_checks(lib):
+    from dgm_amd import serve
+    serve._bind(lib)
+    h = ctypes.c_void_p()
+    bad = serve.ServerConfig(0, 4, 5, 1)
+    assert lib.dgx_server_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+    assert b"positive" in lib.dgx_last_error()
+    good = serve.ServerConfig(1 << 30, 4, 5, 1)
+    assert lib.dgx_server_create(ctypes.byref(good), ctypes.byref(h)) == 0
+    m = meshgen.periodic_quads(2, 2)
+    d, keep = _desc(m, 3, 1)
+    U = np.zeros((4, 4, 25))
+    rid = ctypes.c_longlong()
+    dp = U.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.dgx_server_submit(h, ctypes.byref(d), dp, 0, 14, 1e-3, 0, ctypes.byref(rid)) == -1
+    assert lib.dgx_server_submit(h, ctypes.byref(d), dp, 0, 5, 1e-3, 10, ctypes.byref(rid)) == -1
+    assert lib.dgx_server_submit(h, ctypes.byref(d), dp, 0, 14, 1e-3, 10, ctypes.byref(rid)) == 0
+    st = ctypes.c_int()
+    assert lib.dgx_server_status(h, rid.value, ctypes.byref(st)) == 0 and st.value == 0   # queued
+    assert lib.dgx_server_status(h, 999, ctypes.byref(st)) == -1
+    assert lib.dgx_server_destroy(h) == 0
+
+
+def test_domain_bytes_estimate(lib):
+    from dgm_amd import serve
+    m = meshgen.periodic_hexes(4, 4, 4)
+    b = serve.domain_bytes(m, 7, 9)
+    state = 5 * 64 * 512 * 8
+    assert 5 * state < b < 20 * state          # registers, traces, viscous terms, nodal buffer
+    bm = serve.domain_bytes(m, 7, 9, precision=1)
+    assert bm < b                               # FP32 viscous terms

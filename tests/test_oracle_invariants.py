@@ -95,7 +95,7 @@ def test_free_stream_preserved(eqn, nsd):
     assert np.max(np.abs(F)) < 1e-12
 
 
-@pytest.mark.parametrize("eqn,flux", [(1, 0), (1, 1), (2, 0), (8, 0), (9, 0)])
+@pytest.mark.parametrize("eqn,flux", [(1, 0), (1, 1), (2, 0), (8, 0), (8, 1), (9, 0), (9, 1)])
 def test_conservation_periodic(eqn, flux):
     vis = 0.02 if eqn in (2, 9) else 0.0
     rng = np.random.default_rng(11)
@@ -107,7 +107,7 @@ def test_conservation_periodic(eqn, flux):
         U = np.stack([1.0 + pert[0], 0.2 + pert[1], -0.1 + pert[2], 2.5 + pert[3]])
     else:
         m = meshgen.periodic_hexes(3, 2, 2, hi=(2 * np.pi,) * 3)
-        orc = Oracle(m, 4, eqn, vis=vis)
+        orc = Oracle(m, 4, eqn, vis=vis, flux=flux)
         x = orc.coords()
         pert = [ic.fourier_modes(x, (2 * np.pi,) * 3, 2, 0.05, rng) for _ in range(5)]
         U = np.stack([1.0 + pert[0], 0.2 + pert[1], -0.1 + pert[2], 0.15 + pert[3], 2.5 + pert[4]])
