@@ -62,8 +62,9 @@ BYTES_PER_DOF_STAGE = 32.0
 FLOPS_PER_DOF_STAGE = {7: {"residual": 256.0, "visc": 531.0}}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
-# (profiles/tgv_ns3d_32cubed_p7_fp64.md)
-MEASURED_TRAFFIC = {("fp64", 32, 7, "visc"): 2.50e9, ("fp64", 32, 7, "residual"): 5.93e9}
+# (profiles/tgv_ns3d_32cubed_p7.md)
+MEASURED_TRAFFIC = {("fp64", 32, 7, "visc"): 2.301e9, ("fp64", 32, 7, "residual"): 5.755e9,
+                    ("mixed", 32, 7, "visc"): 1.831e9, ("mixed", 32, 7, "residual"): 5.175e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 
 
