@@ -3,8 +3,9 @@
 Workload: 3D compressible Navier-Stokes Taylor-Green vortex on the periodic
 box [0, 2pi]^3, 32^3 affine hexes at p = 7 (8^3 modes per element, 16.8 M
 modal DOF per field, 5 fields), Re = 1600 (mu = 1/1600), Ma = 0.1, Pr = 0.71,
-Lax-Friedrichs flux with BR1 viscous terms (the reference's Navier_Stokes3d
-operator), LSRK14 stepping (14 stages per step).
+Roe flux with BR2 viscous terms by default (--flux lf --viscous br1 selects
+the reference's own Navier_Stokes3d operator), LSRK14 stepping (14 stages per
+step).
 
 Both paths run on the same initial state: the FP64 path (`value`) and the
 mixed path (FP32 MFMA contractions, FP64 state and registers; `mixed`), whose
@@ -21,7 +22,8 @@ Also printed in the same JSON line:
                 MI355X's 78.6 TFLOP/s FP64 peak (157.3 TFLOP/s FP32 matrix for the
                 mixed path).  Algorithmic flops per DOF-stage at p = 7 (minimal
                 sum-factorized operator): 787 in all, 256 of them in the inviscid
-                residual kernel, the other 531 in the BR1 viscous kernel.
+                residual kernel, the other 531 in the BR1 viscous kernel; BR2's
+                face pass adds 85 to the viscous kernel.
                 `traffic` is the kernel's HBM bytes per launch measured with
                 rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of
                 known size).
@@ -58,14 +60,22 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 ARITHMETIC = {"fp64": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
               "mixed": "FP32 MFMA contractions with FP64 pointwise work; peak is the FP32 matrix rate"}
 BYTES_PER_DOF_STAGE = 32.0
-# algorithmic flops per DOF-stage by kernel kind, per order (3D NS, BR1)
+# algorithmic flops per DOF-stage by kernel kind, per order (3D NS, BR1); BR2
+# adds its face pass to the viscous kernel: nodal face lifts (2 x 29,376 FMA),
+# the per-face-node correction (486 nodes x ~154 flops), the lift of the flux
+# difference (29,376 FMA) and its face sum (12,288 FMA) per element of 2,560 DOF
 FLOPS_PER_DOF_STAGE = {7: {"residual": 256.0, "visc": 531.0}}
+BR2_VISC_FLOPS = {7: 85.0}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
 # (profiles/tgv_ns3d_32cubed_p7.md)
-MEASURED_TRAFFIC = {("fp64", 32, 7, "visc"): 2.301e9, ("fp64", 32, 7, "residual"): 5.755e9,
-                    ("mixed", 32, 7, "visc"): 1.831e9, ("mixed", 32, 7, "residual"): 5.175e9}
+# keys: path, n, p, kernel, flux, viscous scheme
+MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "lf", "br1"): 2.301e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.755e9,
+                    ("mixed", 32, 7, "visc", "lf", "br1"): 1.831e9, ("mixed", 32, 7, "residual", "lf", "br1"): 5.175e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
+FLUXES = {"lf": dgx.DGX_FLUX_LF, "roe": dgx.DGX_FLUX_ROE}
+VISCOUS = {"br1": dgx.DGX_VISCOUS_BR1, "br2": dgx.DGX_VISCOUS_BR2}
+OPNAME = {"lf": "LF", "roe": "Roe"}
 
 
 def tgv_domain(n, p, mu):
@@ -81,12 +91,12 @@ def tgv_dt(n, p, gamma=1.4, Ma=0.1, cfl=0.5):
 
 def _oracle_worker(args):
     """One CPU worker: the oracle's LSRK14 on its own 3^3-element box for `seconds`."""
-    p, mu, seconds = args
+    p, mu, seconds, flux, viscous = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
     n = 3
     m = tgv_domain(n, p, mu)
-    orc = Oracle(m, p, 9, vis=mu, Pr=0.71)
+    orc = Oracle(m, p, 9, vis=mu, Pr=0.71, flux=flux, viscous=viscous)
     x = orc.coords()
     orc.set_state(ic.taylor_green(x[0], x[1], x[2]))
     orc.project_state()
@@ -100,7 +110,7 @@ def _oracle_worker(args):
     return dofs * 14 * steps, time.perf_counter() - t0, steps
 
 
-def cpu_baseline(p, mu, seconds=15.0, workers=None):
+def cpu_baseline(p, mu, seconds=15.0, workers=None, flux=0, viscous=0):
     """The oracle on the host cores: one independent worker per core (the
     reference's one-MPI-rank-per-core layout with single-threaded BLAS, here
     without halo exchange, so an upper bound of its throughput), run in spawned
@@ -109,7 +119,7 @@ def cpu_baseline(p, mu, seconds=15.0, workers=None):
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     workers = workers or max(1, min(16, ncpu))
     with mp.get_context("spawn").Pool(workers) as pool:
-        res = pool.map(_oracle_worker, [(p, mu, seconds)] * workers)
+        res = pool.map(_oracle_worker, [(p, mu, seconds, flux, viscous)] * workers)
     total = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     per_core = [r[0] / r[1] for r in res]
@@ -167,7 +177,8 @@ def serving_sample(args, seed=1234):
 
 def run_path(m, n, p, mu, precision, args):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
-    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision)
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision,
+                        flux=FLUXES[args.flux], viscous=VISCOUS[args.viscous])
     x = gpu.coords()
     U = ic.taylor_green(x[0], x[1], x[2])
     del x
@@ -203,13 +214,16 @@ def run_path(m, n, p, mu, precision, args):
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     mean_s = kern[dom]["ms_mean"] / 1e3
     flops = FLOPS_PER_DOF_STAGE.get(p, {}).get(dom)
+    if flops is not None and dom == "visc" and args.viscous == "br2":
+        flops += BR2_VISC_FLOPS[p]
     roofline = None
     if flops is not None:
         algo_flops = flops * dofs
         achieved = algo_flops / mean_s / 1e12
         peak = FP64_PEAK_TFLOPS if precision == dgx.DGX_FP64 else FP32_MFMA_PEAK_TFLOPS
         roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak, "traffic": MEASURED_TRAFFIC.get((name, n, p, dom)),
+                    "frac": achieved / peak,
+                    "traffic": MEASURED_TRAFFIC.get((name, n, p, dom, args.flux, args.viscous)),
                     "kernel": dom, "arithmetic": ARITHMETIC[name],
                     "algorithmic_flops_per_launch": algo_flops, "ms_mean": kern[dom]["ms_mean"]}
     stage_s = sum(kern[k]["ms_total"] for k in kern) / 1e3 / (14 * prof_steps)
@@ -241,6 +255,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=32, help="elements per direction")
     ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--flux", choices=sorted(FLUXES), default="roe", help="Riemann flux")
+    ap.add_argument("--viscous", choices=sorted(VISCOUS), default="br2", help="viscous face gradient")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="paths to run; with both, the mixed path is also compared with the FP64 one")
     ap.add_argument("--no-cpu", action="store_true")
@@ -263,7 +279,8 @@ def main():
         sys.exit(2)
 
     n, p, mu = args.n, args.p, 1.0 / 1600.0
-    cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers)
+    cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers, FLUXES[args.flux],
+                                                VISCOUS[args.viscous])
     m = tgv_domain(n, p, mu)
     runs, states = {}, {}
     if args.precision in ("both", "fp64"):
@@ -287,7 +304,8 @@ def main():
         "dtype": "f64" if head_name == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"3D Navier-Stokes Taylor-Green vortex, periodic [0,2pi]^3, {n}^3 hexes, "
-                               f"p={p}, Re=1600, Ma=0.1, Pr=0.71, LF + BR1, LSRK14, "
+                               f"p={p}, Re=1600, Ma=0.1, Pr=0.71, {OPNAME[args.flux]} + {args.viscous.upper()}, "
+                               f"LSRK14, "
                                f"{'FP64' if head_name == 'fp64' else 'mixed'}",
                    "dofs": head["dofs"], "stages_per_step": 14, "dt": head["dt"], "path": head_name,
                    "state_finite": head["state_finite"]},

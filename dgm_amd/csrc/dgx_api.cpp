@@ -99,7 +99,7 @@ ld normleg(int l, ld x) { ld P, dP; leg(l, x, P, dP); return P * sqrtl((2 * l + 
 
 struct HostBasis {
   int p, L, Q, QF;
-  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig;
+  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE;
 };
 
 HostBasis make_basis(int p) {
@@ -144,6 +144,22 @@ HostBasis make_basis(int p) {
   }
   for (int a = 0; a < QF; ++a)
     for (int i = 0; i < Q; ++i) b.Ig[a * Q + i] = (double)lagrange(z, i, zg[a]);
+  b.PI.resize(L * Q); b.KE.resize(2 * Q);
+  for (int l = 0; l < L; ++l)
+    for (int i = 0; i < Q; ++i) {
+      ld s = 0;
+      for (int a = 0; a < QF; ++a) s += wg[a] * normleg(l, zg[a]) * lagrange(z, i, zg[a]);
+      b.PI[l * Q + i] = (double)s;
+    }
+  for (int i = 0; i < Q; ++i) {
+    ld lo = 0, hi = 0;
+    for (int l = 0; l < L; ++l) {
+      lo += normleg(l, -1.0L) * normleg(l, z[i]);
+      hi += normleg(l, 1.0L) * normleg(l, z[i]);
+    }
+    b.KE[i] = (double)lo;
+    b.KE[Q + i] = (double)hi;
+  }
   return b;
 }
 
@@ -156,7 +172,7 @@ int upload_all_tables() {
   for (int p = 1; p <= dgx::kMaxOrder; ++p) {
     HostBasis b = make_basis(p);
     size_t off = dgx::tab_offset(p);
-    for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig}) {
+    for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig, &b.PI, &b.KE}) {
       std::copy(t->begin(), t->end(), packed.begin() + off);
       off += t->size();
     }
@@ -531,6 +547,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   d->geo.metric = d->d_metric; d->geo.fnorm = d->d_fnorm; d->geo.fjr = d->d_fjr;
   d->geo.nbr = d->d_nbr; d->geo.nside = d->d_nside; d->geo.permid = d->d_permid; d->geo.perm = d->d_perm;
   d->phys.gamma = d->gamma; d->phys.vis = d->vis; d->phys.Pr = d->Pr; d->phys.flux = d->flux;
+  d->phys.visc = desc->viscous;
+  d->phys.eta = desc->br2_eta > 0.0 ? desc->br2_eta : 6.0;
 
   const size_t n = d->state_len();
   if ((rc = alloc(&d->U, n)) || (rc = alloc(&d->Ut, n)) || (rc = alloc(&d->R0, n)) || (rc = alloc(&d->R1, n)))
@@ -613,6 +631,12 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "flux must be DGX_FLUX_LAX_FRIEDRICHS or DGX_FLUX_ROE");
   if (desc->precision != DGX_FP64 && desc->precision != DGX_MIXED)
     return fail(DGX_ERR_ARGUMENT, "precision must be DGX_FP64 or DGX_MIXED");
+  if (desc->viscous != DGX_VISCOUS_BR1 && desc->viscous != DGX_VISCOUS_BR2)
+    return fail(DGX_ERR_ARGUMENT, "viscous must be DGX_VISCOUS_BR1 or DGX_VISCOUS_BR2");
+  if (desc->viscous == DGX_VISCOUS_BR2 && desc->eqntype != DGX_NAVIER_STOKES3D)
+    return fail(DGX_ERR_ARGUMENT, "DGX_VISCOUS_BR2 is available for Navier_Stokes3d");
+  if (!(desc->br2_eta >= 0.0) || !std::isfinite(desc->br2_eta))
+    return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
   if (desc->eqntype == DGX_NAVIER_STOKES3D && !dgx::stage3_supported(desc->p) &&

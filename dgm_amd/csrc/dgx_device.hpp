@@ -21,10 +21,14 @@ struct Tab {
 
 // The same tables for every supported order p = 1..10 in constant memory, so
 // that uniform coefficient reads become scalar loads.  Layout per order:
-// BT, BW, QD, Bg, Bend, PW, Ig (sizes above), orders packed in ascending p.
+// BT, BW, QD, Bg, Bend, PW, Ig (sizes above), then PI = PW Ig [L][Q] (GLL
+// face layer -> modal face coefficients) and KE [2][Q] (KE[s][i] =
+// sum_l Bend[s][l] BT[i][l], the end traces sum_l phi_l(-+1) phi_l(xi_i)),
+// orders packed in ascending p.
 constexpr int kMaxOrder = 10;
 constexpr int tab_size(int p) {
-  return 3 * (p + 2) * (p + 1) + 2 * (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1) * (p + 2);
+  return 3 * (p + 2) * (p + 1) + 2 * (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1) * (p + 2) +
+         (p + 1) * (p + 2) + 2 * (p + 2);
 }
 constexpr int tab_offset(int p) {
   int s = 0;
@@ -45,6 +49,8 @@ template <int P> struct CT {
   __device__ static __forceinline__ const double *Bend() { return c_tab + b + 3 * Q * L + QF * L; }
   __device__ static __forceinline__ const double *PW() { return c_tab + b + 3 * Q * L + QF * L + 2 * L; }
   __device__ static __forceinline__ const double *Ig() { return c_tab + b + 3 * Q * L + 2 * QF * L + 2 * L; }
+  __device__ static __forceinline__ const double *PI() { return Ig() + QF * Q; }
+  __device__ static __forceinline__ const double *KE() { return PI() + L * Q; }
 };
 
 // Per-element affine geometry and face links (device memory).
@@ -61,6 +67,8 @@ struct Geo {
 struct Phys {
   double gamma, vis, Pr;
   int flux;                      // 0 Lax-Friedrichs, 1 Roe
+  int visc;                      // viscous face gradient: 0 BR1, 1 BR2
+  double eta;                    // BR2 penalty
 };
 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last

@@ -46,7 +46,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int mq_size(int no, int n) { return ((no + 3) / 4) * 4 * ((n + 3) / 4 * 4); }
 constexpr int mq_order_size(int p) {
   return mq_size(p + 2, p + 1) + 2 * mq_size(p + 1, p + 2) + mq_size(p + 1, p + 1) + mq_size(2, p + 1) +
-         mq_size(p + 1, p + 1) + mq_size(p + 1, p + 2);
+         mq_size(p + 1, p + 1) + mq_size(p + 1, p + 2) + mq_size(p + 1, p + 2);
 }
 constexpr int mq_order_offset(int p) {
   int s = 0;
@@ -66,6 +66,7 @@ template <int P> struct MT {
   static constexpr int oBend = oBg + mq_size(QF, L);        // [2][L]   modal -> +-1
   static constexpr int oPW = oBend + mq_size(2, L);         // [L][QF]  Gauss inner product
   static constexpr int oIg = oPW + mq_size(L, QF);          // [QF][Q]  GLL -> Gauss
+  static constexpr int oPI = oIg + mq_size(QF, Q);          // [L][Q]   GLL face layer -> modal face
 };
 
 // sqrt(1/2)^3: the constant normalized Legendre mode of a hex
@@ -184,6 +185,9 @@ template <int P, int NTH = kStageThreads> struct M3 {
   static constexpr int EC_CN = 0, EC_LIN = 5, EC_CF = 20, EC_H = 35, EC_B1 = 80, EC_N = 80 + Q;
   static constexpr int lds_stage = (2 * NA + NG) * 4 + EC_N * 8 + 64;
   static constexpr int lds_vterm = (2 * NA + 24 * QF * FP) * 4 + 64;
+  // BR2: face-lift nodal values, metric rows, end traces (+ layer velocity)
+  static constexpr bool br2_vel_in_b = 42 * Q * QP <= NA;
+  static constexpr int lds_vterm_br2 = (2 * NA + 24 * Q * QP + 16 + 2 * Q + (br2_vel_in_b ? 0 : 18 * Q * QP)) * 4 + 64;
   static constexpr int FS = Q * Q * QP;
   static __device__ __forceinline__ int nidx(int n) { return (n / Q) * QP + (n - (n / Q) * Q); }
   // FP64 face traces of the updated state, in batches of TB fields over the
@@ -627,8 +631,10 @@ __device__ __forceinline__ float mvisc_flux(const float t[6], const float h[3], 
 // points of face s.  The lifted gradient is assembled one reference direction
 // at a time (affine faces: normals -+ grad xi_d / |grad xi_d|) straight into
 // the stress tensor and heat flux at the thread's nodes.
-template <int P>
-__global__ void __launch_bounds__(kVterm3mThreads)
+// At p = 7 the BR2 variant needs a few registers more than the 128 that keep
+// two 512-thread workgroups per CU: capped there (4 registers spill).
+template <int P, bool BR2>
+__global__ void __launch_bounds__(kVterm3mThreads) __attribute__((amdgpu_waves_per_eu((BR2 && P == 7) ? 4 : 1)))
 k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__restrict__ V,
           float *__restrict__ TV, Geo geo, Phys ph, int ne) {
   using S = M3<P, kVterm3mThreads>;
@@ -766,7 +772,28 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
   grad_dir(IC<1>());
   grad_dir(IC<2>());
 
-  // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss points
+  f4 acc[NSG][NOB];   // modal viscous volume term of the 4 equations
+  // BR2 scratch: Hn [24][Q][Q] and gs[9], KE[2][Q] in the Jq region; the
+  // (BR2 - BR1) flux difference Df [4][6][Q][Q] in B, the face-layer velocity
+  // Vel [6][3][Q][Q] after it (or after KE when B is too small)
+  constexpr int HN = 24 * Q * QP, VOFF = S::br2_vel_in_b ? 0 : HN + 16 + 2 * Q;
+  float *Df = B, *Vel = S::br2_vel_in_b ? B + HN : Jq + VOFF;
+  if constexpr (BR2) {
+    // nodal values of the face lifts at each face's own GLL points: Hn[24][Q][Q]
+    mcon_mid<24, L, 1, L, Q, LP, LP, NT>(T::oBT, Jq, A, tid);   // [24][Q][L]
+    if (tid < 9) {   // unit rows of the metric
+      const int d = tid / 3;
+      Jq[HN + tid] = (float)(mt[tid] / sqrt(mt[3 * d] * mt[3 * d] + mt[3 * d + 1] * mt[3 * d + 1] +
+                                            mt[3 * d + 2] * mt[3 * d + 2]));
+    }
+    if (tid >= 16 && tid < 16 + 2 * Q) Jq[HN + tid] = (float)CT<P>::KE()[tid - 16];
+    __syncthreads();
+    mcon_last<24 * Q, L, Q, LP, QP, NT>(T::oBT, A, Jq, tid);    // [24][Q][Q]
+    __syncthreads();
+  }
+
+  // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss
+  // points (with BR2, also the velocity on the layers)
   float *Lay = A;
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
@@ -787,13 +814,66 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
         for (int q = 0; q < 4; ++q)
           Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
               mvisc_flux(tq[r], tq[r] + 6, uw[r][0], uw[r][1], uw[r][2], q, n0, n1, n2);
+        if constexpr (BR2) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) Vel[((fs[c] * 3 + q) * Q + l1[c]) * QP + l2[c]] = uw[r][q];
+        }
       }
     }
   }
   __syncthreads();
-  mcon_mid<24, Q, 1, Q, QF, QP, QP, NT>(T::oIg, Lay, B, tid);   // [24][QF][Q]
+  if constexpr (BR2) {
+    // BR2 correction, one thread per face point: flux difference Df and Lay += Df
+    const float eta = (float)ph.eta;
+    for (int idx = tid; idx < 6 * Q * Q; idx += NT) {
+      const int f = idx / (Q * Q), rem = idx - f * (Q * Q), l1 = rem / Q, l2 = rem - l1 * Q;
+      int i, j, k;
+      if (f == 0 || f == 5) { i = l1; j = l2; k = f == 0 ? 0 : Q - 1; }
+      else if (f == 1 || f == 3) { i = l1; k = l2; j = f == 1 ? 0 : Q - 1; }
+      else { j = l1; k = l2; i = f == 4 ? 0 : Q - 1; }
+      const double *nn = geo.fnorm + 3 * (e * 6 + f);
+      const float *vl = Vel + (f * 3 * Q + l1) * QP + l2;
+      float fd[4];
+      br2_flux<Q, QP>(fd, Jq, Jq + HN + 16, f, i, j, k, Jq + HN, mu, kap, eta, vl[0], vl[Q * QP],
+                      vl[2 * Q * QP], (float)nn[0], (float)nn[1], (float)nn[2]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Lay[((f * 4 + q) * Q + l1) * QP + l2] += fd[q];
+        Df[((q * 6 + f) * Q + l1) * QP + l2] = fd[q];
+      }
+    }
+    __syncthreads();
+  }
+  float *Gs = B;   // scratch of the GLL -> Gauss interpolation
+  if constexpr (BR2) {
+    // lift of the flux difference on the element's own faces, into the volume term
+    mcon_mid<24, Q, 1, Q, L, QP, QP, NT>(T::oPI, B, Jq, tid);   // [24][L][Q]
+    __syncthreads();
+    mcon_last<24 * L, Q, L, QP, LP, NT>(T::oPI, Jq, B, tid);    // [24][L][L]
+    __syncthreads();
+    float jr6[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) jr6[s] = (float)geo.fjr[e * 6 + s];
+#pragma unroll
+    for (int k = 0; k < NSG; ++k) {
+      const int Gi = tid + k * NT;
+      const int q = Gi / L2, pp = Gi - q * L2, m = pp / L, nn = pp - m * L;
+#pragma unroll
+      for (int ob = 0; ob < NOB; ++ob) acc[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+        acc[k][l / 4][l % 4] = Gi < 4 * L2 ? mface_sum<P>(B + q * 6 * L * LP, l, m, nn, jr6) : 0.f;
+    }
+    Gs = Jq;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NSG; ++k)
+#pragma unroll
+      for (int ob = 0; ob < NOB; ++ob) acc[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+  mcon_mid<24, Q, 1, Q, QF, QP, QP, NT>(T::oIg, Lay, Gs, tid);   // [24][QF][Q]
   __syncthreads();
-  mcon_last<24 * QF, Q, QF, QP, FP, NT>(T::oIg, B, A, tid);     // [24][QF][QF]
+  mcon_last<24 * QF, Q, QF, QP, FP, NT>(T::oIg, Gs, A, tid);     // [24][QF][QF]
   __syncthreads();
   for (int i = tid; i < 24 * QF2; i += NT) {
     const int sq = i / QF2, q = i - sq * QF2;
@@ -802,11 +882,6 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
   __syncthreads();
 
   // viscous volume divergence of the 4 equations
-  f4 acc[NSG][NOB];
-#pragma unroll
-  for (int k = 0; k < NSG; ++k)
-#pragma unroll
-    for (int ob = 0; ob < NOB; ++ob) acc[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
   auto visc_dir = [&](auto Dc) {
     constexpr int d = decltype(Dc)::value;
     const float m0 = (float)mt[3 * d], m1 = (float)mt[3 * d + 1], m2 = (float)mt[3 * d + 2];
@@ -857,10 +932,17 @@ template <int P>
 hipError_t launch_mixed3_p(const StageLaunch &s) {
   using S = M3<P>;
   if (s.op == 0) {
-    const int sh = M3<P, kVterm3mThreads>::lds_vterm;
-    (void)hipFuncSetAttribute((const void *)k_vterm3m<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3m<P>, dim3(s.ne), dim3(kVterm3mThreads), sh, s.stream, s.U, s.T, s.Vfw, s.TVfw,
-                       s.geo, s.phys, s.ne);
+    if (s.phys.visc == 1) {
+      const int sh = M3<P, kVterm3mThreads>::lds_vterm_br2;
+      (void)hipFuncSetAttribute((const void *)k_vterm3m<P, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+      hipLaunchKernelGGL((k_vterm3m<P, true>), dim3(s.ne), dim3(kVterm3mThreads), sh, s.stream, s.U, s.T, s.Vfw,
+                         s.TVfw, s.geo, s.phys, s.ne);
+    } else {
+      const int sh = M3<P, kVterm3mThreads>::lds_vterm;
+      (void)hipFuncSetAttribute((const void *)k_vterm3m<P, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+      hipLaunchKernelGGL((k_vterm3m<P, false>), dim3(s.ne), dim3(kVterm3mThreads), sh, s.stream, s.U, s.T, s.Vfw,
+                         s.TVfw, s.geo, s.phys, s.ne);
+    }
     return hipGetLastError();
   }
   MixedArgs a;
@@ -918,6 +1000,7 @@ hipError_t upload_tables_m3d(const double *h) {
     pack_mq(Bend, 2, L, o); o += mq_size(2, L);
     pack_mq(PW, L, QF, o); o += mq_size(L, QF);
     pack_mq(Ig, QF, Q, o); o += mq_size(QF, Q);
+    pack_mq(Ig + QF * Q, L, Q, o); o += mq_size(L, Q);   // PI follows Ig in the FP64 tables
     if (o != mq.data() + mq_order_offset(p + 1)) return hipErrorInvalidValue;
   }
   return hipMemcpyToSymbol(HIP_SYMBOL(c_mq), mq.data(), sizeof(float) * kMqTotal);

@@ -43,6 +43,9 @@ template <int P, int NTH = kStageThreads> struct S3 {
   static constexpr int NG = 30 * QF * FP;               // face corrections
   static constexpr int lds_stage = (2 * NA + NG) * 8;
   static constexpr int lds_vterm = (2 * NA + 24 * QF * FP) * 8;
+  // BR2: face-lift nodal values, metric rows, end traces (+ layer velocity)
+  static constexpr bool br2_vel_in_b = 42 * Q * QP <= NA;
+  static constexpr int lds_vterm_br2 = (2 * NA + 24 * Q * QP + 16 + 2 * Q + (br2_vel_in_b ? 0 : 18 * Q * QP)) * 8;
   // padded position of nodal index n = (i*Q + j)*Q + k
   static __device__ __forceinline__ int nidx(int n) { return (n / Q) * QP + (n - (n / Q) * Q); }
 };
@@ -357,13 +360,17 @@ __device__ __forceinline__ double visc_flux(const double t[6], const double h[3]
 
 // BR1 viscous terms: V[e][eq][L3] = modal divergence of the viscous flux,
 // TV[e][s][eq][QF2] = viscous flux . n at the Gauss points of face s.
+// With BR2, TV holds the BR2 face flux (stresses of grad_h q + eta r_s on face
+// s) and V also carries the lift of (BR2 - BR1) flux . n on the element's own
+// faces, so that the stage kernel's face correction, 1/2 (TV- + TV+) lifted,
+// completes the strong form -div F(G) - lift({F_BR2}.n - F(G)-.n).
 // The lifted gradient is assembled one reference direction at a time: for an
 // affine hex the normals of the two faces across reference direction d are
 // -+ grad(xi_d)/|grad(xi_d)|, so the physical gradient is
 // sum_d dxi_d/dx (D_d + lift of those two faces / |grad xi_d|), and each
 // direction's nodal contribution goes straight into the stress tensor and heat
 // flux at the thread's nodes.
-template <int P>
+template <int P, bool BR2>
 __global__ void __launch_bounds__(kVterm3Threads)
 k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
          double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
@@ -492,7 +499,28 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   grad_dir(IC<1>());
   grad_dir(IC<2>());
 
-  // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss points
+  double acc[NSG][L];   // modal viscous volume term of the 4 equations
+  // BR2 scratch: Hn [24][Q][Q] and gs[9], KE[2][Q] in the Jq region; the
+  // (BR2 - BR1) flux difference Df [4][6][Q][Q] in B, the face-layer velocity
+  // Vel [6][3][Q][Q] after it (or after KE when B is too small)
+  constexpr int HN = 24 * Q * QP, VOFF = S::br2_vel_in_b ? 0 : HN + 16 + 2 * Q;
+  double *Df = B, *Vel = S::br2_vel_in_b ? B + HN : Jq + VOFF;
+  if constexpr (BR2) {
+    // nodal values of the face lifts at each face's own GLL points: Hn[24][Q][Q]
+    con_mid<24, L, 1, L, Q, LP, LP, NT>(CT<P>::BT(), Jq, A, tid);   // [24][Q][L]
+    if (tid < 9) {   // unit rows of the metric
+      const int d = tid / 3;
+      Jq[HN + tid] = mt[tid] / sqrt(mt[3 * d] * mt[3 * d] + mt[3 * d + 1] * mt[3 * d + 1] +
+                                    mt[3 * d + 2] * mt[3 * d + 2]);
+    }
+    if (tid >= 16 && tid < 16 + 2 * Q) Jq[HN + tid] = CT<P>::KE()[tid - 16];
+    __syncthreads();
+    con_last<24 * Q, L, Q, LP, QP, NT>(CT<P>::BT(), A, Jq, tid);    // [24][Q][Q]
+    __syncthreads();
+  }
+
+  // normal viscous flux on the GLL face layers [6][4][Q][Q] (pitch QP) -> Gauss
+  // points (with BR2, also the velocity on the layers)
   double *Lay = A;
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
@@ -512,13 +540,63 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
         for (int q = 0; q < 4; ++q)
           Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
               visc_flux(tq[r], tq[r] + 6, pv[r][0], pv[r][1], pv[r][2], q, nn[0], nn[1], nn[2]);
+        if constexpr (BR2) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) Vel[((fs[c] * 3 + q) * Q + l1[c]) * QP + l2[c]] = pv[r][q];
+        }
       }
     }
   }
   __syncthreads();
-  con_mid<24, Q, 1, Q, QF, QP, QP, NT>(CT<P>::Ig(), Lay, B, tid);   // [24][QF][Q]
+  if constexpr (BR2) {
+    // BR2 correction, one thread per face point: flux difference Df and Lay += Df
+    for (int idx = tid; idx < 6 * Q * Q; idx += NT) {
+      const int f = idx / (Q * Q), rem = idx - f * (Q * Q), l1 = rem / Q, l2 = rem - l1 * Q;
+      int i, j, k;
+      if (f == 0 || f == 5) { i = l1; j = l2; k = f == 0 ? 0 : Q - 1; }
+      else if (f == 1 || f == 3) { i = l1; k = l2; j = f == 1 ? 0 : Q - 1; }
+      else { j = l1; k = l2; i = f == 4 ? 0 : Q - 1; }
+      const double *nn = geo.fnorm + 3 * (e * 6 + f);
+      const double *vl = Vel + (f * 3 * Q + l1) * QP + l2;
+      double fd[4];
+      br2_flux<Q, QP>(fd, Jq, Jq + HN + 16, f, i, j, k, Jq + HN, mu, kap, ph.eta, vl[0], vl[Q * QP],
+                      vl[2 * Q * QP], nn[0], nn[1], nn[2]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Lay[((f * 4 + q) * Q + l1) * QP + l2] += fd[q];
+        Df[((q * 6 + f) * Q + l1) * QP + l2] = fd[q];
+      }
+    }
+    __syncthreads();
+  }
+  double *Gs = B;   // scratch of the GLL -> Gauss interpolation
+  if constexpr (BR2) {
+    // lift of the flux difference on the element's own faces, into the volume term:
+    // (PW Ig (x) PW Ig) difference -> modal face blocks [4][6][L][L] in B
+    con_mid<24, Q, 1, Q, L, QP, QP, NT>(CT<P>::PI(), B, Jq, tid);   // [24][L][Q]
+    __syncthreads();
+    con_last<24 * L, Q, L, QP, LP, NT>(CT<P>::PI(), Jq, B, tid);    // [24][L][L]
+    __syncthreads();
+    double jr6[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) jr6[s] = geo.fjr[e * 6 + s];
+#pragma unroll
+    for (int k = 0; k < NSG; ++k) {
+      const int Gi = tid + k * NT;
+      const int q = Gi / L2, pp = Gi - q * L2, m = pp / L, nn = pp - m * L;
+#pragma unroll
+      for (int l = 0; l < L; ++l) acc[k][l] = Gi < 4 * L2 ? face_sum<P>(B + q * 6 * L * LP, l, m, nn, jr6) : 0.0;
+    }
+    Gs = Jq;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NSG; ++k)
+#pragma unroll
+      for (int l = 0; l < L; ++l) acc[k][l] = 0.0;
+  }
+  con_mid<24, Q, 1, Q, QF, QP, QP, NT>(CT<P>::Ig(), Lay, Gs, tid);   // [24][QF][Q]
   __syncthreads();
-  con_last<24 * QF, Q, QF, QP, FP, NT>(CT<P>::Ig(), B, A, tid);     // [24][QF][QF]
+  con_last<24 * QF, Q, QF, QP, FP, NT>(CT<P>::Ig(), Gs, A, tid);     // [24][QF][QF]
   __syncthreads();
   for (int i = tid; i < 24 * QF2; i += NT) {
     const int sq = i / QF2, q = i - sq * QF2;
@@ -527,11 +605,6 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   __syncthreads();
 
   // viscous volume divergence of the 4 equations
-  double acc[NSG][L];
-#pragma unroll
-  for (int k = 0; k < NSG; ++k)
-#pragma unroll
-    for (int l = 0; l < L; ++l) acc[k][l] = 0.0;
   auto visc_dir = [&](auto Dc) {
     constexpr int d = decltype(Dc)::value;
     const double m0 = mt[3 * d], m1 = mt[3 * d + 1], m2 = mt[3 * d + 2];
@@ -569,10 +642,17 @@ static hipError_t launch_stage3_p(const StageLaunch &s) {
   a.T = s.T; a.V = s.V; a.TV = s.TV; a.U = s.U; a.Ut = s.Ut; a.R = s.R; a.Tn = s.Tn;
   a.ca = s.ca; a.cb = s.cb; a.dt = s.dt; a.ne = s.ne; a.flag = s.flag;
   if (s.op == 0) {
-    const int sh = S3<P, kVterm3Threads>::lds_vterm;
-    (void)hipFuncSetAttribute((const void *)k_vterm3<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL(k_vterm3<P>, dim3(s.ne), dim3(kVterm3Threads), sh, s.stream, s.U, s.T, s.Vw, s.TVw, s.tab,
-                       s.geo, s.phys, s.ne);
+    if (s.phys.visc == 1) {
+      const int sh = S3<P, kVterm3Threads>::lds_vterm_br2;
+      (void)hipFuncSetAttribute((const void *)k_vterm3<P, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+      hipLaunchKernelGGL((k_vterm3<P, true>), dim3(s.ne), dim3(kVterm3Threads), sh, s.stream, s.U, s.T, s.Vw,
+                         s.TVw, s.tab, s.geo, s.phys, s.ne);
+    } else {
+      const int sh = S3<P, kVterm3Threads>::lds_vterm;
+      (void)hipFuncSetAttribute((const void *)k_vterm3<P, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+      hipLaunchKernelGGL((k_vterm3<P, false>), dim3(s.ne), dim3(kVterm3Threads), sh, s.stream, s.U, s.T, s.Vw,
+                         s.TVw, s.tab, s.geo, s.phys, s.ne);
+    }
     return hipGetLastError();
   }
   const int sh = S3<P, kStage3Threads>::lds_stage;

@@ -57,13 +57,13 @@ def _check(lib, rc):
         raise dgx.DgxError(rc, lib.dgx_last_error().decode())
 
 
-def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision):
+def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0):
     vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
     con = np.ascontiguousarray(mesh.con, dtype=np.int32)
     d = dgx.DomainDesc(int(mesh.nsd), int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
                        float(gamma), float(vis), float(Pr),
                        vert.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(con),
-                       con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                       con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(viscous), float(br2_eta))
     return d, (vert, con)
 
 
@@ -108,8 +108,9 @@ class Server:
             pass
 
     def submit(self, mesh, p, eqntype, U0, inttype, dt, nsteps, gamma=1.4, vis=0.0, Pr=0.72,
-               flux=dgx.DGX_FLUX_LF, precision=dgx.DGX_FP64, layout=dgx.DGX_NODAL) -> int:
-        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision)
+               flux=dgx.DGX_FLUX_LF, precision=dgx.DGX_FP64, layout=dgx.DGX_NODAL,
+               viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0) -> int:
+        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous, br2_eta)
         U0 = np.ascontiguousarray(U0, dtype=np.float64)
         nf, L, Q = mesh.nsd + 2, p + 1, p + 2
         per = (Q if layout == dgx.DGX_NODAL else L) ** mesh.nsd

@@ -53,6 +53,13 @@ enum dgx_eqntype {
 /* Numerical flux (src/Euler.cpp:102-113). */
 enum dgx_flux { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE = 1 };
 
+/* Viscous face gradient of Navier-Stokes.  BR1 is the reference's
+ * (src/Navier_Stokes3d.cpp:182-391); BR2 (hexes only) has no reference
+ * counterpart: on each face the viscous flux uses the element gradient plus
+ * br2_eta times the lifting of that face's jump alone, instead of the gradient
+ * lifted over all faces. */
+enum dgx_viscous { DGX_VISCOUS_BR1 = 0, DGX_VISCOUS_BR2 = 1 };
+
 /* Arithmetic of the residual. */
 enum dgx_precision { DGX_FP64 = 0, DGX_MIXED = 1 };
 
@@ -83,6 +90,8 @@ typedef struct {
   const double *vert;
   int ncon;
   const int *con;
+  int viscous;      /* enum dgx_viscous (Navier-Stokes; 0 = BR1) */
+  double br2_eta;   /* BR2 penalty; 0 selects 6, the number of hex faces */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

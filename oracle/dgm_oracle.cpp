@@ -25,6 +25,9 @@
 //   src/Euler3d.cpp:468-817, 1144-1199            3D convection, LF flux
 //   src/Navier_Stokes.cpp:113-258                 2D BR1 diffusion
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
+//   (no reference counterpart)                    3D BR2 face gradient: its parity
+//                                                 against the reference is unpinned; the
+//                                                 BR2 tests pin it by invariants
 //   src/Domain.cpp:1883-1937                      average_flux
 //   src/Field.cpp:87-90, 416-668, 3478-3556       links, master sides, flips
 //   src/BaseQuad.hpp:53-57, src/BaseHex.hpp:291-313  sideMap
@@ -388,7 +391,7 @@ struct Domain {
   int status = 0;
   std::vector<double> z, w, zg, wg, B, D, Dt, Ig, Ii;
   std::vector<Elem> el;
-  VF U, Ut, R0, R1, Uv;
+  VF U, Ut, R0, R1, Uv, Ub;   // Ub: BR2 normal viscous flux on the sides
 
   double *eu(VF &F, int v, int e) { return &F.u[((size_t)v * ne + e) * qtot]; }
   double *es(VF &F, int v, int e, int s) {
@@ -752,33 +755,33 @@ struct Domain {
       for (int e = 0; e < ne; ++e) fill_sides(F, v, e, eu(F, v, e));
   }
   // normal-flux side fill (src/Quad.cpp:894-934, src/Hex.cpp:950-973)
-  void fill_sides_flux(VF &F, int v, int e, const double *fx, const double *fy, const double *fz) {
-    const Elem &E = el[e];
+  void side_normal_flux(int e, int s, const double *fx, const double *fy, const double *fz, double *su) {
+    const SideInfo &S = el[e].side[s];
     std::vector<double> wk(nsd == 2 ? q : q * q), tmp(qs);
-    for (int s = 0; s < nsides; ++s) {
-      double *su = es(F, v, e, s);
-      const SideInfo &S = E.side[s];
-      get_side(fx, s, wk.data()); to_side(wk.data(), su);
-      if (nsd == 2) {
-        get_side(fy, s, wk.data()); to_side(wk.data(), tmp.data());
-        for (int g = 0; g < qs; ++g) su[g] = S.nx[g] * su[g];
-        for (int g = 0; g < qs; ++g) su[g] = S.ny[g] * tmp[g] + su[g];
-      } else {
-        for (int g = 0; g < qs; ++g) su[g] = S.nx[g] * su[g];
-        get_side(fy, s, wk.data()); to_side(wk.data(), tmp.data());
-        for (int g = 0; g < qs; ++g) su[g] = S.ny[g] * tmp[g] + su[g];
-        get_side(fz, s, wk.data()); to_side(wk.data(), tmp.data());
-        for (int g = 0; g < qs; ++g) su[g] = S.nz[g] * tmp[g] + su[g];
-      }
+    get_side(fx, s, wk.data()); to_side(wk.data(), su);
+    if (nsd == 2) {
+      get_side(fy, s, wk.data()); to_side(wk.data(), tmp.data());
+      for (int g = 0; g < qs; ++g) su[g] = S.nx[g] * su[g];
+      for (int g = 0; g < qs; ++g) su[g] = S.ny[g] * tmp[g] + su[g];
+    } else {
+      for (int g = 0; g < qs; ++g) su[g] = S.nx[g] * su[g];
+      get_side(fy, s, wk.data()); to_side(wk.data(), tmp.data());
+      for (int g = 0; g < qs; ++g) su[g] = S.ny[g] * tmp[g] + su[g];
+      get_side(fz, s, wk.data()); to_side(wk.data(), tmp.data());
+      for (int g = 0; g < qs; ++g) su[g] = S.nz[g] * tmp[g] + su[g];
     }
+  }
+  void fill_sides_flux(VF &F, int v, int e, const double *fx, const double *fy, const double *fz) {
+    for (int s = 0; s < nsides; ++s) side_normal_flux(e, s, fx, fy, fz, es(F, v, e, s));
   }
 
   // lift (src/Quad.cpp:1971-2038, src/Hex.cpp:1786-1952):
   //   in += G2GL[(F_side - U_side) * J_side/J_vol (* n_dir)] / w_boundary
-  void lift(VF &Us, VF &Fs, int v, int e, double *in, char dir) {
+  void lift(VF &Us, VF &Fs, int v, int e, double *in, char dir, int only = -1) {
     const Elem &E = el[e];
     std::vector<double> f(qs), fi(nsd == 2 ? q : q * q);
     for (int s = 0; s < nsides; ++s) {
+      if (only >= 0 && s != only) continue;
       const SideInfo &S = E.side[s];
       const double *fu = es(Fs, v, e, s), *uu = es(Us, v, e, s);
       for (int g = 0; g < qs; ++g) f[g] = fu[g] - uu[g];
@@ -1080,6 +1083,64 @@ struct Domain {
       }
   }
 
+  int visc_scheme = 0;   // 0 BR1, 1 BR2
+  double br2_eta = 6.0;  // BR2 penalty
+
+  // BR2 normal viscous flux on every side of hex k (no reference counterpart:
+  // the reference's diffusion is BR1 only).  On side s the stresses and heat
+  // flux come from grad_h q + eta r_s(q) instead of the BR1 lifted gradient
+  // grad_h q + sum_s' r_s'(q), where grad_h q is the projected element gradient
+  // and r_s the projected lifting of side s's jump ({q} - q-) alone, both
+  // built from the same gradient / lift / project steps as BR1
+  // (src/Euler3d.cpp:1144-1160).  out[(eq - 1) * nsides + s][qs], eq = 1..4.
+  void br2_sides3d(VF &Us, VF &Fv, int k, const double *u, const double *v, const double *w,
+                   const double *en, std::vector<double> &out) {
+    const int n = qtot;
+    const double *prim[4] = {u, v, w, en};
+    std::vector<double> bg(12 * n), rl(12 * n), G(12 * n);
+    for (int c = 0; c < 4; ++c) {
+      double *gx = &bg[(3 * c) * n], *gy = &bg[(3 * c + 1) * n], *gz = &bg[(3 * c + 2) * n];
+      gradient_all(k, prim[c], gx, gy, gz);
+      project_vec(k, gx); project_vec(k, gy); project_vec(k, gz);
+    }
+    const double mu = vis, gp = gamma / Pr;
+    std::vector<double> t(6 * n), fx(n), fy(n), fz(n);
+    out.assign(4 * nsides * qs, 0.0);
+    for (int s = 0; s < nsides; ++s) {
+      std::fill(rl.begin(), rl.end(), 0.0);
+      for (int c = 0; c < 4; ++c) {
+        const char dirs[3] = {'x', 'y', 'z'};
+        for (int x = 0; x < 3; ++x) {
+          double *r = &rl[(3 * c + x) * n];
+          lift(Us, Fv, c + 1, k, r, dirs[x], s);
+          project_vec(k, r);
+        }
+      }
+      for (int i = 0; i < 12 * n; ++i) G[i] = br2_eta * rl[i] + bg[i];
+      const double *ux = &G[0], *uy = &G[n], *uz = &G[2 * n], *vx = &G[3 * n], *vy = &G[4 * n],
+                   *vz = &G[5 * n], *wx = &G[6 * n], *wy = &G[7 * n], *wz = &G[8 * n];
+      const double *ex = &G[9 * n], *ey = &G[10 * n], *ez = &G[11 * n];
+      double *t11 = &t[0], *t12 = &t[n], *t13 = &t[2 * n], *t22 = &t[3 * n], *t23 = &t[4 * n], *t33 = &t[5 * n];
+      for (int i = 0; i < n; ++i) {
+        t12[i] = mu * (uy[i] + vx[i]);
+        t13[i] = mu * (uz[i] + wx[i]);
+        t23[i] = mu * (vz[i] + wy[i]);
+        t11[i] = mu * ((2.0 / 3.0) * (2.0 * ux[i] - vy[i] - wz[i]));
+        t22[i] = mu * ((2.0 / 3.0) * (2.0 * vy[i] - ux[i] - wz[i]));
+        t33[i] = mu * ((2.0 / 3.0) * (2.0 * wz[i] - ux[i] - vy[i]));
+      }
+      side_normal_flux(k, s, t11, t12, t13, &out[(0 * nsides + s) * qs]);
+      side_normal_flux(k, s, t12, t22, t23, &out[(1 * nsides + s) * qs]);
+      side_normal_flux(k, s, t13, t23, t33, &out[(2 * nsides + s) * qs]);
+      for (int i = 0; i < n; ++i) {
+        fx[i] = t11[i] * u[i] + t12[i] * v[i] + t13[i] * w[i] + gp * (mu * ex[i]);
+        fy[i] = t12[i] * u[i] + t22[i] * v[i] + t23[i] * w[i] + gp * (mu * ey[i]);
+        fz[i] = t13[i] * u[i] + t23[i] * v[i] + t33[i] * w[i] + gp * (mu * ez[i]);
+      }
+      side_normal_flux(k, s, fx.data(), fy.data(), fz.data(), &out[(3 * nsides + s) * qs]);
+    }
+  }
+
   // 3D BR1 diffusion (src/Navier_Stokes3d.cpp:182-391)
   void diffusion3d(VF &Us, VF &Uf, VF &Ur) {
     fill_sides_all(Us);
@@ -1088,7 +1149,7 @@ struct Domain {
     const int n = qtot;
     std::vector<double> u(n), v(n), w_(n), e_(n), wk(n);
     std::vector<double> ux(n), uy(n), uz(n), vx(n), vy(n), vz(n), wx(n), wy(n), wz(n);
-    std::vector<double> t11(n), t12(n), t13(n), t22(n), t23(n), t33(n);
+    std::vector<double> t11(n), t12(n), t13(n), t22(n), t23(n), t33(n), br2;
     const double mu = vis;
     for (int k = 0; k < ne; ++k) {
       const double *r = eu(Us, 0, k), *ru = eu(Us, 1, k), *rv = eu(Us, 2, k), *rw = eu(Us, 3, k), *rE = eu(Us, 4, k);
@@ -1097,6 +1158,7 @@ struct Domain {
         wk[i] = u[i] * u[i]; wk[i] = v[i] * v[i] + wk[i]; wk[i] = w_[i] * w_[i] + wk[i];
         e_[i] = -0.5 * wk[i] + e_[i];
       }
+      if (visc_scheme == 1) br2_sides3d(Us, Uf, k, u.data(), v.data(), w_.data(), e_.data(), br2);
       lifted_gradient(Us, Uf, 1, k, u.data(), ux.data(), uy.data(), uz.data());
       lifted_gradient(Us, Uf, 2, k, v.data(), vx.data(), vy.data(), vz.data());
       lifted_gradient(Us, Uf, 3, k, w_.data(), wx.data(), wy.data(), wz.data());
@@ -1145,8 +1207,15 @@ struct Domain {
       for (int i = 0; i < n; ++i) F4[i] = wk[i] + ux[i];
       gradient_dir(k, uz.data(), wk.data(), 'z');
       for (int i = 0; i < n; ++i) F4[i] = F4[i] + wk[i];
+      if (visc_scheme == 1)
+        for (int eq = 0; eq < 4; ++eq)
+          for (int s = 0; s < nsides; ++s)
+            std::memcpy(es(Ub, eq + 1, k, s), &br2[(eq * nsides + s) * qs], qs * sizeof(double));
     }
-    average_flux(Us, Uf, '-');
+    // face flux: average of the side fluxes (BR1: of the lifted-gradient flux
+    // also used in the volume; BR2: of the per-side BR2 flux); the lift below
+    // corrects the volume term's own trace F(G)- . n to it
+    average_flux(visc_scheme == 1 ? Ub : Us, Uf, '-');
     for (int nn = 1; nn < nv; ++nn)
       for (int k = 0; k < ne; ++k) lift(Us, Uf, nn, k, eu(Uf, nn, k), 0);
     for (int nn = 1; nn < nv; ++nn)
@@ -1252,7 +1321,7 @@ void *orc_create(int nsd, int ne, int p, const double *vert, int ncon, const int
   d->build_basis();
   d->build_geometry(vert);
   d->build_links(ncon, con, vert);
-  d->alloc(d->U); d->alloc(d->Ut); d->alloc(d->R0); d->alloc(d->R1); d->alloc(d->Uv);
+  d->alloc(d->U); d->alloc(d->Ut); d->alloc(d->R0); d->alloc(d->R1); d->alloc(d->Uv); d->alloc(d->Ub);
   return d;
 }
 void orc_destroy(void *h) { delete (Domain *)h; }
@@ -1263,6 +1332,13 @@ int orc_riemann(int nsd, int flux, double gamma, const double *Ul, const double 
   double Fpl[5], Fmr[5];
   if (nsd == 2) return point_flux2d(gamma, flux, Ul, Ur, n[0], n[1], Fpl, Fmr, Fh);
   return point_flux3d(gamma, flux, Ul, Ur, n[0], n[1], n[2], Fpl, Fmr, Fh);
+}
+// viscous scheme of the Navier-Stokes residual: 0 BR1 (the reference's), 1 BR2
+// with penalty eta (3D only)
+void orc_set_viscous(void *h, int scheme, double eta) {
+  Domain *d = (Domain *)h;
+  d->visc_scheme = scheme;
+  d->br2_eta = eta;
 }
 int orc_qtot(void *h) { return ((Domain *)h)->qtot; }
 int orc_nmodes(void *h) { return ((Domain *)h)->nm; }

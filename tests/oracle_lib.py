@@ -47,6 +47,7 @@ def _load():
     lib.orc_advance.restype = ctypes.c_int
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
     lib.orc_riemann.restype = ctypes.c_int
+    lib.orc_set_viscous.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     _lib = lib
     return lib
 
@@ -58,7 +59,7 @@ def _ptr(a, t=_dp):
 class Oracle:
     """CPU restatement of one DGM Euler/NS domain (nodal GLL storage)."""
 
-    def __init__(self, mesh, p, eqn, gamma=1.4, vis=0.0, Pr=0.72, flux=0):
+    def __init__(self, mesh, p, eqn, gamma=1.4, vis=0.0, Pr=0.72, flux=0, viscous=0, br2_eta=6.0):
         self.lib = _load()
         self.nsd = mesh.nsd
         self.ne = mesh.ne
@@ -70,6 +71,8 @@ class Oracle:
                                      _ptr(self._con, _ip), eqn, gamma, vis, Pr, flux)
         self.qtot = self.lib.orc_qtot(self.h)
         self.nm = self.lib.orc_nmodes(self.h)
+        if viscous:
+            self.lib.orc_set_viscous(self.h, int(viscous), float(br2_eta))
 
     def __del__(self):
         try:

@@ -25,9 +25,10 @@ from dgm_amd import dgx, ic, mesh as meshgen
 pytestmark = pytest.mark.gpu
 
 
-def _tgv_domain(n, p, eqn, precision, vis=0.0, flux=dgx.DGX_FLUX_LF):
+def _tgv_domain(n, p, eqn, precision, vis=0.0, flux=dgx.DGX_FLUX_LF, viscous=dgx.DGX_VISCOUS_BR1):
     m = meshgen.periodic_hexes(n, n, n, (0.0, 0.0, 0.0), (2 * np.pi,) * 3)
-    return dgx.GPUDomain(m, p, eqn, gamma=1.4, vis=vis, Pr=0.71, precision=precision, flux=flux)
+    return dgx.GPUDomain(m, p, eqn, gamma=1.4, vis=vis, Pr=0.71, precision=precision, flux=flux,
+                         viscous=viscous)
 
 
 def _rel(a, b):
@@ -37,16 +38,19 @@ def _rel(a, b):
     return d / r
 
 
-@pytest.mark.parametrize("eqn,p,vis,flux", [(dgx.DGX_EULER3D, 3, 0.0, 0), (dgx.DGX_EULER3D, 7, 0.0, 0),
-                                            (dgx.DGX_NAVIER_STOKES3D, 4, 0.01, 0),
-                                            (dgx.DGX_NAVIER_STOKES3D, 7, 1.0 / 1600.0, 0),
-                                            (dgx.DGX_NAVIER_STOKES3D, 8, 0.01, 0),
-                                            (dgx.DGX_NAVIER_STOKES3D, 7, 1.0 / 1600.0, 1)])
-def test_mixed_residual(eqn, p, vis, flux):
-    print(f"[mixed] residual eqn={eqn} p={p} flux={flux}", flush=True)
+@pytest.mark.parametrize("eqn,p,vis,flux,visc", [(dgx.DGX_EULER3D, 3, 0.0, 0, 0), (dgx.DGX_EULER3D, 7, 0.0, 0, 0),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 4, 0.01, 0, 0),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 7, 1.0 / 1600.0, 0, 0),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 8, 0.01, 0, 0),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 7, 1.0 / 1600.0, 1, 0),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 4, 0.01, 0, 1),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 7, 1.0 / 1600.0, 1, 1),
+                                                 (dgx.DGX_NAVIER_STOKES3D, 8, 0.01, 1, 1)])
+def test_mixed_residual(eqn, p, vis, flux, visc):
+    print(f"[mixed] residual eqn={eqn} p={p} flux={flux} viscous={visc}", flush=True)
     out = {}
     for prec in (dgx.DGX_FP64, dgx.DGX_MIXED):
-        g = _tgv_domain(3, p, eqn, prec, vis, flux)
+        g = _tgv_domain(3, p, eqn, prec, vis, flux, visc)
         x = g.coords()
         U = ic.taylor_green(x[0], x[1], x[2])
         U[3] += 0.05 * np.sin(x[0] + 2 * x[2])          # non-zero w for all fields

@@ -41,12 +41,12 @@ def perturbed(U, x, amp=0.05):
     return out
 
 
-def check_rhs(m, p, eqn, U_fn, vis=0.0, flux=0, tol=TOL_RHS):
-    orc = Oracle(m, p, eqn, vis=vis, flux=flux)
+def check_rhs(m, p, eqn, U_fn, vis=0.0, flux=0, tol=TOL_RHS, viscous=0, br2_eta=6.0):
+    orc = Oracle(m, p, eqn, vis=vis, flux=flux, viscous=viscous, br2_eta=br2_eta)
     x = orc.coords()
     Up = projected_state(orc, U_fn(x))
     F_ref = orc.rhs(Up)
-    gpu = dgx.GPUDomain(m, p, eqn, vis=vis, flux=flux)
+    gpu = dgx.GPUDomain(m, p, eqn, vis=vis, flux=flux, viscous=viscous, br2_eta=br2_eta)
     gx = gpu.coords()
     assert np.allclose(gx, x, rtol=0, atol=1e-13)
     gpu.upload(Up)
@@ -84,6 +84,23 @@ def test_rhs_roe_euler3d(p):
 def test_rhs_roe_ns3d():
     m = meshgen.periodic_hexes(3, 3, 2, (0, 0, 0), (2 * np.pi,) * 3)
     check_rhs(m, 4, 9, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), vis=0.01, flux=1)
+
+
+@pytest.mark.parametrize("p,flux,eta", [(2, 0, 6.0), (3, 1, 6.0), (5, 0, 2.5), (7, 1, 6.0), (8, 0, 6.0),
+                                        (9, 1, 20.0)])
+def test_rhs_br2_ns3d(p, flux, eta):
+    """BR2 viscous face gradient (no reference counterpart: the oracle's BR2 is
+    pinned by tests/test_oracle_br2.py); anisotropic elements."""
+    print(f"[br2] p={p} flux={flux} eta={eta}", flush=True)
+    m = meshgen.periodic_hexes(3, 3, 2, (0, 0, 0), (2 * np.pi, 2 * np.pi, 4.0))
+    check_rhs(m, p, 9, lambda x: perturbed(ic.taylor_green(x[0], x[1], x[2]), x), vis=0.02, flux=flux,
+              viscous=1, br2_eta=eta)
+
+
+def test_rhs_br2_ns3d_vhex_mesh():
+    """BR2 on the vhex fixture's mesh (general side links and orientations)."""
+    prm, m, icd = load_case("vhex")
+    check_rhs(m, 4, 9, lambda x: perturbed(initial_state(icd, x, 3), x), vis=0.01, viscous=1)
 
 
 def test_rhs_vquad_ns2d_p10():

@@ -54,7 +54,8 @@ def _desc(m, p, eqn, **kw):
     d = dgx.DomainDesc(nsd=m.nsd, ne=m.ne, p=p, eqntype=eqn, flux=kw.get("flux", 0),
                        precision=kw.get("precision", 0), gamma=1.4, vis=kw.get("vis", 0.0),
                        Pr=0.72, vert=vert.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                       ncon=len(con), con=con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                       ncon=len(con), con=con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       viscous=kw.get("viscous", 0), br2_eta=kw.get("br2_eta", 0.0))
     return d, (vert, con)
 
 
@@ -66,11 +67,16 @@ def _desc(m, p, eqn, **kw):
     (dict(p=10, eqn=9), b"p <= 9"),
     (dict(p=9, eqn=8, precision=1), b"DGX_MIXED"),
     (dict(p=3, eqn=8, precision=7), b"precision"),
+    (dict(p=3, eqn=9, viscous=2), b"viscous must be"),
+    (dict(p=3, eqn=8, viscous=1), b"DGX_VISCOUS_BR2"),
+    (dict(p=3, eqn=9, viscous=1, br2_eta=-1.0), b"br2_eta"),
+    (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
 ])
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
     d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0),
-                    precision=case.get("precision", 0))
+                    precision=case.get("precision", 0), viscous=case.get("viscous", 0),
+                    br2_eta=case.get("br2_eta", 0.0))
     h = ctypes.c_void_p()
     rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
     assert rc == -1
