@@ -68,10 +68,12 @@ FLOPS_PER_DOF_STAGE = {7: {"residual": 256.0, "visc": 531.0}}
 BR2_VISC_FLOPS = {7: 85.0}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
-# (profiles/tgv_ns3d_32cubed_p7.md)
+# (profiles/tgv_ns3d_32cubed_p7_roe_br2.md, profiles/tgv_ns3d_32cubed_p7_lf_br1.md)
 # keys: path, n, p, kernel, flux, viscous scheme
 MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "lf", "br1"): 2.301e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.755e9,
-                    ("mixed", 32, 7, "visc", "lf", "br1"): 1.831e9, ("mixed", 32, 7, "residual", "lf", "br1"): 5.175e9}
+                    ("mixed", 32, 7, "visc", "lf", "br1"): 1.831e9, ("mixed", 32, 7, "residual", "lf", "br1"): 5.175e9,
+                    ("fp64", 32, 7, "visc", "roe", "br2"): 2.301e9, ("fp64", 32, 7, "residual", "roe", "br2"): 5.755e9,
+                    ("mixed", 32, 7, "visc", "roe", "br2"): 2.079e9, ("mixed", 32, 7, "residual", "roe", "br2"): 5.175e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 FLUXES = {"lf": dgx.DGX_FLUX_LF, "roe": dgx.DGX_FLUX_ROE}
 VISCOUS = {"br1": dgx.DGX_VISCOUS_BR1, "br2": dgx.DGX_VISCOUS_BR2}

@@ -25,6 +25,7 @@
 //   src/Euler3d.cpp:468-817, 1144-1199            3D convection, LF flux
 //   src/Navier_Stokes.cpp:113-258                 2D BR1 diffusion
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
+//   src/LinEuler_quasi.cpp:151-495                2D linearized Euler (uniform mean flow)
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
 //                                                 against the reference is unpinned; the
 //                                                 BR2 tests pin it by invariants
@@ -365,7 +366,7 @@ int point_flux3d(double gamma, int flux_type, const double Ul[5], const double U
 // Domain
 // ---------------------------------------------------------------------------
 
-enum Eqn { EULER2D = 1, NS2D = 2, EULER3D = 8, NS3D = 9 };
+enum Eqn { EULER2D = 1, NS2D = 2, LINEULER2D = 4, EULER3D = 8, NS3D = 9 };
 
 struct SideInfo {
   bool master = false;       // side->link set (src/SideLoop.hpp:46-50)
@@ -1225,6 +1226,64 @@ struct Domain {
       }
   }
 
+  // Linear hyperbolic system on quads, dU/dt + Ax dU/dx + Ay dU/dy = 0, with the
+  // Lax-Friedrichs flux 1/2 (An UL + An UR) + 1/2 lam (UL - UR): the
+  // reference's LinEuler_quasi residual (src/LinEuler_quasi.cpp:151-495:
+  // Jacobian-vector fluxes, gradient, Lax-Friedrichs with the largest wave
+  // speed, lift, project) for a uniform mean flow, whose Jacobians Ax, Ay and
+  // speed |u0 . n| + c0 the caller sets; with -Ax^T, -Ay^T it is the adjoint system.
+  double linA[2][16] = {{0}}, lin_u0 = 0.0, lin_v0 = 0.0, lin_c0 = 0.0;
+  void convection_lin2d(VF &Us, VF &Uf) {
+    fill_sides_all(Us);
+    std::vector<double> fx(qtot), fy(qtot), d(qtot);
+    for (int e = 0; e < ne; ++e) {
+      for (int i = 0; i < 4; ++i) {
+        for (int q = 0; q < qtot; ++q) {
+          double sx = 0.0, sy = 0.0;
+          for (int j = 0; j < 4; ++j) {
+            sx += linA[0][4 * i + j] * eu(Us, j, e)[q];
+            sy += linA[1][4 * i + j] * eu(Us, j, e)[q];
+          }
+          fx[q] = sx; fy[q] = sy;
+        }
+        double *F = eu(Uf, i, e);
+        gradient_dir(e, fx.data(), F, 'x');
+        gradient_dir(e, fy.data(), d.data(), 'y');
+        for (int q = 0; q < qtot; ++q) F[q] = F[q] + d[q];
+      }
+    }
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < nsides; ++s) {
+        const SideInfo &S = el[e].side[s];
+        if (!S.master) continue;
+        for (int g = 0; g < qs; ++g) {
+          const int ql = sidemap(S, g), qr = sidemap_slave(g);
+          const double nx = S.nx[ql], ny = S.ny[ql];
+          const double lin_lam = std::fabs(lin_u0 * nx + lin_v0 * ny) + lin_c0;
+          double UL[4], UR[4], FL[4], FR[4];
+          for (int n = 0; n < 4; ++n) { UL[n] = es(Us, n, e, s)[ql]; UR[n] = es(Us, n, S.le, S.ls)[qr]; }
+          for (int i = 0; i < 4; ++i) {
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < 4; ++j) {
+              const double An = nx * linA[0][4 * i + j] + ny * linA[1][4 * i + j];
+              a += An * UL[j];
+              b += An * UR[j];
+            }
+            FL[i] = a; FR[i] = b;
+          }
+          for (int n = 0; n < 4; ++n) {
+            const double Fh = 0.5 * (FL[n] + FR[n] + lin_lam * (UL[n] - UR[n]));
+            es(Us, n, e, s)[ql] = FL[n];
+            es(Us, n, S.le, S.ls)[qr] = -FR[n];
+            es(Uf, n, e, s)[ql] = Fh;
+            es(Uf, n, S.le, S.ls)[qr] = -Fh;
+          }
+        }
+      }
+    for (int e = 0; e < ne; ++e)
+      for (int n = 0; n < 4; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
+  }
+
   void project_all(VF &F) {
     for (int v = 0; v < nv; ++v)
       for (int e = 0; e < ne; ++e) project_vec(e, eu(F, v, e));
@@ -1233,6 +1292,7 @@ struct Domain {
   // time_derivative: F = -dU/dt (src/Euler.cpp:282-287, src/Euler3d.cpp:455-458,
   // src/Navier_Stokes.cpp:89-94, src/Navier_Stokes3d.cpp:135-161)
   void time_derivative(VF &Us, VF &Uf) {
+    if (eqn == LINEULER2D) { convection_lin2d(Us, Uf); project_all(Uf); return; }
     if (nsd == 2) convection2d(Us, Uf); else convection3d(Us, Uf);
     if (eqn == NS2D) diffusion2d(Us, Uv, Uf);
     if (eqn == NS3D) diffusion3d(Us, Uv, Uf);
@@ -1339,6 +1399,14 @@ void orc_set_viscous(void *h, int scheme, double eta) {
   Domain *d = (Domain *)h;
   d->visc_scheme = scheme;
   d->br2_eta = eta;
+}
+// matrices Ax, Ay (row-major 4x4) of the linear system (eqn 4) and its
+// Lax-Friedrichs speed |u0 nx + v0 ny| + c0
+void orc_set_linear(void *h, const double *Ax, const double *Ay, double u0, double v0, double c0) {
+  Domain *d = (Domain *)h;
+  std::memcpy(d->linA[0], Ax, 16 * sizeof(double));
+  std::memcpy(d->linA[1], Ay, 16 * sizeof(double));
+  d->lin_u0 = u0; d->lin_v0 = v0; d->lin_c0 = c0;
 }
 int orc_qtot(void *h) { return ((Domain *)h)->qtot; }
 int orc_nmodes(void *h) { return ((Domain *)h)->nm; }

@@ -48,6 +48,7 @@ def _load():
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
     lib.orc_riemann.restype = ctypes.c_int
     lib.orc_set_viscous.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    lib.orc_set_linear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     _lib = lib
     return lib
 
@@ -79,6 +80,12 @@ class Oracle:
             self.lib.orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_linear(self, Ax, Ay, u0, v0, c0):
+        """Matrices of the linear system (eqn 4); Lax-Friedrichs speed |u0 nx + v0 ny| + c0."""
+        self._Ax = np.ascontiguousarray(Ax, dtype=np.float64).reshape(16)
+        self._Ay = np.ascontiguousarray(Ay, dtype=np.float64).reshape(16)
+        self.lib.orc_set_linear(self.h, _ptr(self._Ax), _ptr(self._Ay), float(u0), float(v0), float(c0))
 
     def coords(self):
         x = np.zeros((self.nsd, self.ne, self.qtot))
