@@ -2,7 +2,7 @@
 // links and orientation flips, basis tables, device buffers, and the explicit
 // Runge-Kutta drivers that chain the residual kernels on one HIP stream.
 #include "../../include/dgx.h"
-#include "dgx_kernels.hpp"
+#include "dgx_domain.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -97,10 +97,7 @@ ld lagrange(const std::vector<ld> &nodes, int j, ld x) {
 
 ld normleg(int l, ld x) { ld P, dP; leg(l, x, P, dP); return P * sqrtl((2 * l + 1) / 2.0L); }
 
-struct HostBasis {
-  int p, L, Q, QF;
-  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE;
-};
+using dgx::HostBasis;
 
 HostBasis make_basis(int p) {
   HostBasis b;
@@ -197,36 +194,7 @@ struct Side {
 
 }  // namespace
 
-struct dgx_domain {
-  int nsd = 0, ne = 0, p = 0, L = 0, Q = 0, QF = 0, nf = 0, nm = 0, nq = 0, ns = 0, qs = 0;
-  int eqn = 0, flux = 0, prec = 0;
-  bool visc = false;
-  bool mixed = false;   // DGX_MIXED: FP32 MFMA contractions (dgx_mixed3d.hip)
-  double gamma = 1.4, vis = 0.0, Pr = 0.72;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  HostBasis hb;
-  std::vector<double> coords, metric, J;
-  // device
-  double *d_tab = nullptr, *d_metric = nullptr, *d_fnorm = nullptr, *d_fjr = nullptr;
-  int *d_nbr = nullptr, *d_nside = nullptr, *d_flag = nullptr;
-  uint8_t *d_permid = nullptr;
-  uint16_t *d_perm = nullptr;
-  double *U = nullptr, *Ut = nullptr, *R0 = nullptr, *R1 = nullptr, *T = nullptr;
-  double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr, *T2 = nullptr;
-  float *Vf = nullptr, *TVf = nullptr;   // viscous terms of the mixed path
-  bool fused = false;   // hex fused stage kernels available for this order
-  dgx::Tab tab;
-  dgx::Geo geo;
-  dgx::Phys phys;
-  long long stages = 0;
-  // per-kernel timing (dgx_profile): event pairs and the kernel kind of each
-  bool profiling = false;
-  std::vector<hipEvent_t> ev;
-  std::vector<int> ev_kind;
 
-  size_t state_len() const { return (size_t)nf * ne * nm; }
-};
 
 namespace {
 
@@ -761,15 +729,8 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
       BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
       BL(dgx::blas_axpby(2.0 / 3.0, h->R0, 1.0 / 3.0, h->U, h->U, n, s));
     } else if (inttype == DGX_LSRK14) {
-      // LSRK14::advance (src/TimeInt.cpp:1989-2013), coefficients :1870-1917
-      static const double A[14] = {0.0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
-          -0.8552979934029281, -3.9564138245774565, -1.5780575380587385, -2.0837094552574054,
-          -0.7483334182761610, -0.7032861106563359, 0.0013917096117681, -0.0932075369637460,
-          -0.9514200470875948, -7.1151571693922548};
-      static const double B[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
-          0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
-          0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
-          0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
+      // LSRK14::advance (src/TimeInt.cpp:1989-2013)
+      const double *A = dgx::kLsrkA, *B = dgx::kLsrkB;
       if (h->fused) {
         // one fused kernel per stage: residual, register update, next traces
         if (step == 0) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));

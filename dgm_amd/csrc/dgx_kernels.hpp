@@ -64,6 +64,37 @@ hipError_t launch_stage3(const StageLaunch &s);
 bool mixed3_supported(int p);
 hipError_t launch_mixed3(const StageLaunch &s);
 
+// Linear hyperbolic system on quads, dU/dt + Ax dU/dx + Ay dU/dy = s (the
+// linearized Euler equations about a uniform mean flow), fused with one
+// LSRK14 stage; or, with the adjoint system (-Ax^T, -Ay^T), one reversed
+// stage of its discrete adjoint (dgx_kernels2d.hip: k_lin2).
+struct LinArgs {
+  const double *X = nullptr;     // modal input [4][ne][L2]: U (forward) or Ubar_t (adjoint)
+  const double *T = nullptr;     // face traces of X
+  double *Y = nullptr;           // forward: U += cb Ut;  adjoint: Ubar += -dt J Fadj(X / J)
+  double *Yt = nullptr;          // forward: Ut = ca Ut - dt R;  adjoint: Ubar_t = ca X (may alias X)
+  const double *theta = nullptr; // forward: source field, one value per element (energy equation)
+  double *gtheta = nullptr;      // adjoint: dJ/dtheta accumulator per element
+  double Ax[16], Ay[16];         // row-major Jacobians (adjoint: -Ax^T, -Ay^T)
+  double u0 = 0, v0 = 0, c0 = 0; // Lax-Friedrichs speed |u0 nx + v0 ny| + c0
+  double ca = 0, cb = 0, dt = 0, g = 0;   // LSRK coefficients, source amplitude g(t_stage)
+  int ne = 0, p = 0;
+  bool adjoint = false;
+  Tab tab;
+  Geo geo;
+  hipStream_t stream = nullptr;
+};
+hipError_t launch_lin2(const LinArgs &a);
+
+// Receiver observations y_r = sum_f c_f sum_m w[r][m] U[f][e_r][m] into y[r]
+// (observe), or the adjoint of the objective 1/2 dt sum_r (y_r - d_r)^2:
+// Ubar[f][e_r][m] += dt (y_r - d_r) c_f w[r][m] and *J += 1/2 dt sum_r (y_r - d_r)^2.
+hipError_t launch_observe(const double *U, const int *rec_e, const double *rec_w, const double *cobs, int nrec,
+                          int ne, int L2, double *y, hipStream_t s);
+hipError_t launch_observe_adjoint(const double *U, double *Ubar, const int *rec_e, const double *rec_w,
+                                  const double *cobs, int nrec, int ne, int L2, const double *d, double dt,
+                                  double *J, hipStream_t s);
+
 // Error message of the calling thread's last failure (dgx_api.cpp).
 int set_error(int code, const std::string &msg);
 

@@ -5,6 +5,8 @@
 //   numerical flux   Euler::convective_flux (LF, Roe) src/Euler.cpp:399-757
 //   lift + project   Quad::lift, Element::project     src/Quad.cpp:1971-2038, src/Element.hpp:427-437
 //   BR1 viscous      Navier_Stokes::diffusion         src/Navier_Stokes.cpp:113-258
+//   linear system    LinEuler_quasi::convection       src/LinEuler_quasi.cpp:151-495
+//                    and its discrete adjoint
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -332,6 +334,193 @@ k_backward2(const double *__restrict__ Fm, double *__restrict__ Fn, Tab tb, int 
   __syncthreads();
   backward2<P>(tb, A, X, O, tid);
   copy_block<NT>(O, Fn + ((size_t)v * ne + e) * Q2, Q2, tid);
+}
+
+// Linear system residual, Lax-Friedrichs flux 1/2 (An UL + An UR) + 1/2 lam
+// (UL - UR), lam = |u0.n| + c0 (LinEuler_quasi::convection / convective_flux,
+// src/LinEuler_quasi.cpp:161-495, for a uniform mean flow), fused with its
+// LSRK14 stage.  ADJ: the same residual of the adjoint system applied to
+// X / J_e and multiplied by J_e is the transpose of the forward residual in
+// modal coordinates (exact quadrature throughout), and the kernel performs one
+// reversed stage: Ubar += -dt L^T X, dJ/dtheta_e += 2 g dt X[E][e][0],
+// Ubar_t = ca X.
+template <int P, bool ADJ>
+__global__ void __launch_bounds__(64)
+k_lin2(LinArgs a) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double Un[4 * Q2], gF[16 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const Geo &geo = a.geo;
+  const double *mt = geo.metric + 9 * (size_t)e;
+  const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
+  const double dm = rx * sy - ry * sx;   // 1 / J_e
+  const double sc = ADJ ? dm : 1.0;
+  for (int v = 0; v < 4; ++v) {
+    copy_block<NT>(a.X + ((size_t)v * ne + e) * L2, Rv, L2, tid);
+    __syncthreads();
+    backward2<P>(a.tab, Rv, X1, Un + v * Q2, tid);
+  }
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, q = i - s * QF;
+    const int es = e * 4 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int qn = geo.perm[geo.permid[es] * QF + q];
+    const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1];
+    const double jr = geo.fjr[es];
+    double scn = 1.0;
+    if (ADJ) {
+      const double *mn = geo.metric + 9 * (size_t)nb;
+      scn = mn[0] * mn[4] - mn[1] * mn[3];
+    }
+    const double *tl = a.T + ((size_t)e * 4 + s) * 4 * QF;
+    const double *tr = a.T + ((size_t)nb * 4 + ns) * 4 * QF;
+    double UL[4], UR[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { UL[v] = sc * tl[v * QF + q]; UR[v] = scn * tr[v * QF + qn]; }
+    const double lam = fabs(a.u0 * nx + a.v0 * ny) + a.c0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double fl = 0.0, fr = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double An = nx * a.Ax[4 * v + j] + ny * a.Ay[4 * v + j];
+        fl = fma(An, UL[j], fl);
+        fr = fma(An, UR[j], fr);
+      }
+      const double fh = 0.5 * (fl + fr + lam * (UL[v] - UR[v]));
+      gF[(v * 4 + s) * QF + q] = jr * (fh - fl);
+    }
+  }
+  __syncthreads();
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int v = 0; v < 4; ++v) {
+    for (int k = tid; k < Q2; k += NT) {
+      double fx = 0.0, fy = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double u = sc * Un[j * Q2 + k];
+        fx = fma(a.Ax[4 * v + j], u, fx);
+        fy = fma(a.Ay[4 * v + j], u, fy);
+      }
+      Fr[k] = rx * fx + ry * fy;
+      Fs[k] = sx * fx + sy * fy;
+    }
+    __syncthreads();
+    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
+    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
+    __syncthreads();
+    for (int k = tid; k < L2; k += NT) {
+      const int l = k / L, m = k - l * L;
+      double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
+      const size_t gi = ((size_t)v * ne + e) * L2 + k;
+      if (!ADJ) {
+        if (v == 3 && k == 0) r -= 2.0 * a.theta[e] * a.g;   // source: mode 0 of the constant theta_e
+        const double ut = a.ca * a.Yt[gi] - a.dt * r;
+        a.Yt[gi] = ut;
+        a.Y[gi] = fma(a.cb, ut, a.Y[gi]);
+      } else {
+        const double x = a.X[gi];
+        a.Y[gi] = fma(-a.dt, r / dm, a.Y[gi]);
+        if (v == 3 && k == 0) a.gtheta[e] += 2.0 * a.g * a.dt * x;
+        a.Yt[gi] = a.ca * x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Observations (one workgroup per receiver) and the objective's adjoint (one
+// workgroup, receivers in order, so that receivers sharing an element never
+// race).
+__global__ void __launch_bounds__(64)
+k_observe(const double *__restrict__ U, const int *__restrict__ rec_e, const double *__restrict__ rec_w,
+          const double *__restrict__ cobs, int ne, int L2, double *__restrict__ y) {
+  __shared__ double red[64];
+  const int r = blockIdx.x, tid = threadIdx.x, e = rec_e[r];
+  double s = 0.0;
+  for (int i = tid; i < 4 * L2; i += 64) {
+    const int f = i / L2, m = i - f * L2;
+    s = fma(cobs[f] * rec_w[(size_t)r * L2 + m], U[((size_t)f * ne + e) * L2 + m], s);
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) y[r] = red[0];
+}
+
+__global__ void __launch_bounds__(64)
+k_observe_adjoint(const double *__restrict__ U, double *__restrict__ Ubar, const int *__restrict__ rec_e,
+                  const double *__restrict__ rec_w, const double *__restrict__ cobs, int nrec, int ne, int L2,
+                  const double *__restrict__ d, double dt, double *__restrict__ J) {
+  __shared__ double red[64];
+  const int tid = threadIdx.x;
+  double jsum = 0.0;
+  for (int r = 0; r < nrec; ++r) {
+    const int e = rec_e[r];
+    double s = 0.0;
+    for (int i = tid; i < 4 * L2; i += 64) {
+      const int f = i / L2, m = i - f * L2;
+      s = fma(cobs[f] * rec_w[(size_t)r * L2 + m], U[((size_t)f * ne + e) * L2 + m], s);
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    const double res = red[0] - d[r];
+    __syncthreads();
+    jsum += res * res;
+    for (int i = tid; i < 4 * L2; i += 64) {
+      const int f = i / L2, m = i - f * L2;
+      const size_t gi = ((size_t)f * ne + e) * L2 + m;
+      Ubar[gi] = fma(dt * res * cobs[f], rec_w[(size_t)r * L2 + m], Ubar[gi]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *J += 0.5 * dt * jsum;
+}
+
+template <int P>
+static hipError_t launch_lin2_p(const LinArgs &a) {
+  if (a.adjoint)
+    hipLaunchKernelGGL((k_lin2<P, true>), dim3(a.ne), dim3(64), 0, a.stream, a);
+  else
+    hipLaunchKernelGGL((k_lin2<P, false>), dim3(a.ne), dim3(64), 0, a.stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_lin2(const LinArgs &a) {
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch_lin2_p<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9) DGX_CASE(10)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_observe(const double *U, const int *rec_e, const double *rec_w, const double *cobs, int nrec,
+                          int ne, int L2, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_observe, dim3(nrec), dim3(64), 0, s, U, rec_e, rec_w, cobs, ne, L2, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_observe_adjoint(const double *U, double *Ubar, const int *rec_e, const double *rec_w,
+                                  const double *cobs, int nrec, int ne, int L2, const double *d, double dt,
+                                  double *J, hipStream_t s) {
+  hipLaunchKernelGGL(k_observe_adjoint, dim3(1), dim3(64), 0, s, U, Ubar, rec_e, rec_w, cobs, nrec, ne, L2, d,
+                     dt, J);
+  return hipGetLastError();
 }
 
 template <int P>

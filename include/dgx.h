@@ -214,6 +214,60 @@ int dgx_server_result(dgx_server_handle s, long long id, double *U, int layout, 
 /* Peak device bytes held by admitted requests, and peak admitted count. */
 int dgx_server_peak(dgx_server_handle s, size_t *bytes, int *active);
 
+/* ---------------------------------------------------------------------------
+ * Acoustic inverse-source problem and its discrete adjoint.
+ *
+ * The linearized 2D Euler equations about a uniform mean flow, in conservative
+ * perturbation variables: the reference's LinEuler_quasi (eqntype 4,
+ * src/LinEuler_quasi.cpp:151-495) with a uniform mean, Lax-Friedrichs flux.
+ * LSRK14 from a zero state, driven by theta_e g(t) in the energy equation
+ * (theta constant per element, g a Ricker wavelet).  After every step the
+ * pressure perturbation p' = (gamma-1)(E' - u0 m' - v0 n' + |u0|^2 rho'/2) is
+ * observed at the receivers; J = 1/2 dt sum_steps sum_r (p'_r - d_r)^2.
+ * dJ/dtheta comes from the discrete adjoint of the whole time loop (the
+ * reversed LSRK14 recurrences, cf. LSRK14::adjoint_advance,
+ * src/TimeInt.cpp:2064-2092, with the transposed DG residual), the forward
+ * states being recomputed from binomial checkpoints (revolve; cf.
+ * DTK::Griewank, src/DTK_Griewank.hpp:42-125).  FP64 only.
+ * ------------------------------------------------------------------------- */
+enum { DGX_LINEULER = 4 };   /* eqntype of the mesh description */
+
+typedef struct dgx_acoustic *dgx_acoustic_handle;
+
+typedef struct {
+  double rho0, u0, v0, p0;   /* uniform mean flow */
+  int nrec;                  /* number of receivers */
+  const double *xrec;        /* receiver points [nrec][2] */
+  double f0, t0;             /* g(t) = (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2 */
+  double dt;                 /* time step */
+  int nsteps;                /* LSRK14 steps */
+  int nsnap;                 /* checkpoint slots, state 0 included (0: ceil(sqrt(nsteps)) + 1) */
+} dgx_acoustic_desc;
+
+/* `mesh`: nsd 2, eqntype DGX_LINEULER, precision DGX_FP64 (flux and vis are
+ * ignored).  Allocates nsnap + 4 states of 4 x ne x (p+1)^2 doubles. */
+int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad, void *stream,
+                        dgx_acoustic_handle *out);
+int dgx_acoustic_destroy(dgx_acoustic_handle a);
+
+/* Forward solve with source theta[ne]; y[nsteps][nrec] receives the
+ * observations (host memory, may be NULL). */
+int dgx_acoustic_forward(dgx_acoustic_handle a, const double *theta, double *y);
+
+/* Objective J and gradient grad[ne] = dJ/dtheta for data[nsteps][nrec]. */
+int dgx_acoustic_gradient(dgx_acoustic_handle a, const double *theta, const double *data, double *J,
+                          double *grad);
+
+/* Forward and adjoint steps taken so far, and the most checkpoint slots the
+ * last gradient held at once. */
+int dgx_acoustic_stats(dgx_acoustic_handle a, long long *forward_steps, long long *adjoint_steps,
+                       int *snapshots);
+
+/* The checkpointing schedule alone (host only): forward steps it advances to
+ * reverse nsteps steps with nsnap slots, the most slots it holds, and whether
+ * every step was reversed in order from its own forward state. */
+int dgx_revolve_count(int nsteps, int nsnap, long long *advances, int *max_slots, int *order_ok);
+
 const char *dgx_last_error(void);
 const char *dgx_version(void);
 

@@ -131,3 +131,27 @@ def test_domain_bytes_estimate(lib):
     assert 5 * state < b < 20 * state          # registers, traces, viscous terms, nodal buffer
     bm = serve.domain_bytes(m, 7, 9, precision=1)
     assert bm < b                               # FP32 viscous terms
+
+
+@pytest.mark.parametrize("case,msg", [
+    (dict(nsd=3), b"2D mesh"),
+    (dict(eqn=1), b"DGX_LINEULER"),
+    (dict(precision=1), b"DGX_FP64"),
+    (dict(rho0=0.0), b"mean flow"),
+    (dict(dt=0.0), b"dt > 0"),
+    (dict(nsteps=0), b"nsteps"),
+    (dict(nsnap=-1), b"nsnap"),
+])
+def test_acoustic_create_rejects_bad_arguments(lib, case, msg):
+    from dgm_amd import acoustic
+    acoustic._bind(lib)
+    m = meshgen.periodic_hexes(2, 2, 2) if case.get("nsd") == 3 else meshgen.periodic_quads(3, 3)
+    d, keep = _desc(m, 2, case.get("eqn", acoustic.DGX_LINEULER), precision=case.get("precision", 0))
+    xr = np.array([[0.5, 0.5]])
+    ad = acoustic.AcousticDesc(case.get("rho0", 1.0), 0.0, 0.0, 1.0, 1,
+                               xr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 10.0, 0.1,
+                               case.get("dt", 1e-3), case.get("nsteps", 5), case.get("nsnap", 0))
+    h = ctypes.c_void_p()
+    rc = lib.dgx_acoustic_create(ctypes.byref(d), ctypes.byref(ad), None, ctypes.byref(h))
+    assert rc == -1 and msg in lib.dgx_last_error()
+    assert not h.value
