@@ -30,6 +30,11 @@ Also printed in the same JSON line:
   roofline_hbm  the HBM view of a whole stage: 32 B per DOF-stage (read U,
                 read Ut, write Ut, write U of a fused 2N-storage stage) / the
                 mean stage time, against 8 TB/s
+  adjoint       the acoustic inverse-source workload (64x64 quads, p=5, 4096
+                source coefficients, 16 receivers, 2000 LSRK14 steps, 45
+                checkpoints): time of one forward solve and of one gradient
+                (checkpointed forward sweep + discrete adjoint), and the
+                gradient's agreement with central differences in 8 directions
   serving       p50/p99 latency and throughput of a serving-loop sample: Poisson
                 arrivals of 2D Euler forward solves of random size and order,
                 continuously batched (dgm_amd.serve)
@@ -177,6 +182,51 @@ def serving_sample(args, seed=1234):
                       f"{args.serve_active} slots, slices of {args.serve_slice} steps replayed from hipGraphs"}
 
 
+def adjoint_sample(args, seed=4096):
+    """The acoustic inverse-source workload: linearized 2D Euler about a rest
+    state on 64x64 periodic quads at p = 5, a source of one seeded Gaussian
+    coefficient per element (4096) with a Ricker time signature, 16 seeded
+    receivers, 2000 LSRK14 steps, ceil(sqrt(2000)) = 45 checkpoints besides
+    the initial state.  One gradient = one checkpointed forward sweep and the
+    adjoint sweep; checked against central differences of the objective in 8
+    seeded directions (J is quadratic in theta, so they are exact up to
+    rounding).  FP64."""
+    from dgm_amd import acoustic
+    rng = np.random.default_rng(seed)
+    n, p, nsteps = args.adj_n, 5, args.adj_steps
+    m = meshgen.periodic_quads(n, n, 0.0, 1.0, 0.0, 1.0)
+    rec = rng.uniform(0.0, 1.0, size=(16, 2))
+    dt = 0.3 * (1.0 / n) / ((p + 1) ** 2 * 1.0)
+    nsnap = int(np.ceil(np.sqrt(nsteps))) + 1
+    f0, t0 = 4.0 / (nsteps * dt), 0.3 * nsteps * dt
+    prob = acoustic.AcousticProblem(m, p, (1.0, 0.0, 0.0, 1.0 / 1.4), rec, f0, t0, dt, nsteps, nsnap=nsnap)
+    data = prob.forward(rng.standard_normal(m.ne))
+    theta = rng.standard_normal(m.ne)
+    t0w = time.perf_counter()
+    prob.forward(theta)
+    t_fwd = time.perf_counter() - t0w
+    t0w = time.perf_counter()
+    J, g = prob.gradient(theta, data)
+    t_grad = time.perf_counter() - t0w
+    st = prob.stats()
+    errs = []
+    h = 1e-1
+    for _ in range(args.adj_dirs):
+        d = rng.standard_normal(m.ne)
+        fd = (prob.objective(theta + h * d, data) - prob.objective(theta - h * d, data)) / (2 * h)
+        errs.append(abs(fd - g @ d) / max(abs(fd), 1e-300))
+    prob.close()
+    dofs = 4 * m.ne * (p + 1) ** 2
+    adv, _, _ = acoustic.revolve_count(nsteps, nsnap)
+    return {"workload": f"linearized 2D Euler acoustics, {n}x{n} quads, p={p}, {m.ne} source coefficients, "
+                        f"16 receivers, {nsteps} LSRK14 steps, {nsnap - 1} checkpoints + the initial state, FP64",
+            "forward_s": t_fwd, "gradient_s": t_grad, "J": J,
+            "forward_steps_per_gradient": adv, "adjoint_steps_per_gradient": nsteps,
+            "forward_dof_stage_per_s": dofs * 14 * nsteps / t_fwd,
+            "fd_directions": len(errs), "fd_rel_err_max": float(max(errs)) if errs else None,
+            "fd_rel_err": [float(e) for e in errs]}
+
+
 def run_path(m, n, p, mu, precision, args):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
     gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision,
@@ -265,6 +315,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline workers (0: one per core, at most 16)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse-source) sample")
+    ap.add_argument("--adj-n", type=int, default=64, help="quads per direction of the adjoint sample")
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--adj-dirs", type=int, default=8, help="finite-difference directions")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-nmax", type=int, default=64)
     ap.add_argument("--serve-steps", type=int, default=200)
@@ -339,6 +393,8 @@ def main():
         line["mixed"] = {"value": runs["mixed"]["value"]}
     if not args.no_serve:
         line["serving"] = serving_sample(args)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_sample(args)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
