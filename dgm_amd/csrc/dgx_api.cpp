@@ -189,6 +189,7 @@ int upload_all_tables() {
 struct Side {
   bool master = false;
   int le = -1, ls = -1;
+  int bc = -1;   // boundary side: DGX_BC_WALL or DGX_BC_STATE
   bool fa = false, fb = false, tr = false;
 };
 
@@ -400,6 +401,14 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   std::vector<Side> sides((size_t)ne * d->ns);
   for (int r = 0; r < desc->ncon; ++r) {
     const int e = desc->con[4 * r], s = desc->con[4 * r + 1], le = desc->con[4 * r + 2], ls = desc->con[4 * r + 3];
+    if (e >= 0 && e < ne && s >= 0 && s < d->ns && le == -1) {   // boundary side (e, s, -1, bc type)
+      if (ls != DGX_BC_WALL && ls != DGX_BC_STATE)
+        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
+      if (desc->eqntype != DGX_EULER3D)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d");
+      sides[(size_t)e * d->ns + s].bc = ls;
+      continue;
+    }
     if (e < 0 || e >= ne || le < 0 || le >= ne || s < 0 || s >= d->ns || ls < 0 || ls >= d->ns)
       return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + " out of range");
     Side &me = sides[(size_t)e * d->ns + s], &nb = sides[(size_t)le * d->ns + ls];
@@ -410,11 +419,14 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     me.master = true; me.le = le; me.ls = ls;
     nb.master = false; nb.le = e; nb.ls = s;
   }
-  for (size_t i = 0; i < sides.size(); ++i)
-    if (sides[i].le < 0)
+  for (size_t i = 0; i < sides.size(); ++i) {
+    if (sides[i].le < 0 && sides[i].bc < 0)
       return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(i / d->ns) + " side " +
-                                        std::to_string(i % d->ns) +
-                                        " has no neighbour (boundary conditions are not supported yet)");
+                                        std::to_string(i % d->ns) + " has neither a neighbour nor a boundary row");
+    if (sides[i].le >= 0 && sides[i].bc >= 0)
+      return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(i / d->ns) + " side " +
+                                        std::to_string(i % d->ns) + " has both a neighbour and a boundary row");
+  }
   static const int con2d[4][4] = {{1, 1, 0, 0}, {1, 1, 0, 0}, {0, 0, 1, 1}, {0, 0, 1, 1}};
   static const int svm[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5}, {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
   for (int e = 0; e < ne; ++e)
@@ -462,7 +474,9 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     for (int s = 0; s < d->ns; ++s) {
       const Side &S = sides[(size_t)e * d->ns + s];
       std::vector<uint16_t> m(qs);
-      if (S.master) {
+      if (S.bc >= 0) {
+        for (int g = 0; g < qs; ++g) m[g] = (uint16_t)g;
+      } else if (S.master) {
         for (int g = 0; g < qs; ++g) m[sidemap(S, g)] = (uint16_t)g;
       } else {
         const Side &M = sides[(size_t)S.le * d->ns + S.ls];
@@ -476,8 +490,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
         permtab.insert(permtab.end(), m.begin(), m.end());
       } else id = it->second;
       permid[(size_t)e * d->ns + s] = (uint8_t)id;
-      nbr[(size_t)e * d->ns + s] = S.le;
-      nside[(size_t)e * d->ns + s] = S.ls;
+      nbr[(size_t)e * d->ns + s] = S.bc >= 0 ? -1 - S.bc : S.le;   // boundary: -1 - type
+      nside[(size_t)e * d->ns + s] = S.bc >= 0 ? 0 : S.ls;
     }
 
   // device buffers
@@ -516,6 +530,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   d->geo.nbr = d->d_nbr; d->geo.nside = d->d_nside; d->geo.permid = d->d_permid; d->geo.perm = d->d_perm;
   d->phys.gamma = d->gamma; d->phys.vis = d->vis; d->phys.Pr = d->Pr; d->phys.flux = d->flux;
   d->phys.visc = desc->viscous;
+  for (int v = 0; v < 5; ++v) d->phys.bcU[v] = desc->bc_state[v];
   d->phys.eta = desc->br2_eta > 0.0 ? desc->br2_eta : 6.0;
 
   const size_t n = d->state_len();
@@ -607,6 +622,13 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
+  for (int r = 0; r < desc->ncon; ++r)
+    if (desc->con[4 * r + 2] == -1) {
+      if (desc->eqntype != DGX_EULER3D) return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d");
+      const int t = desc->con[4 * r + 3];
+      if (t != DGX_BC_WALL && t != DGX_BC_STATE)
+        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
+    }
   if (desc->eqntype == DGX_NAVIER_STOKES3D && !dgx::stage3_supported(desc->p) &&
       dgx::lds_bytes3(desc->p, 3) > 160 * 1024)
     return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d supports p <= 9 (LDS)");

@@ -69,7 +69,32 @@ struct Phys {
   int flux;                      // 0 Lax-Friedrichs, 1 Roe
   int visc;                      // viscous face gradient: 0 BR1, 1 BR2
   double eta;                    // BR2 penalty
+  double bcU[5];                 // conservative state of state boundaries
 };
+
+// Ghost values of a boundary side, as Domain::enforce_bc fills them
+// (src/Domain.cpp:1346-1350): bc 0 slip wall (Euler3d::WallBC::apply,
+// src/Euler3d.cpp:876-909: the normal momentum removed, the internal energy
+// kept), bc 1 state (StateBC::apply, src/BC.cpp:17-28).  A side whose
+// neighbour index nb is negative is a boundary of type -1 - nb.
+__device__ __forceinline__ void ghost3(int bc, const double UL[5], double nx, double ny, double nz,
+                                       const double bcU[5], double UB[5]) {
+  // no fused multiply-adds: the ghost energy is a difference of O(1/Ma^2)
+  // energies, and the wall's Riemann wave strengths are differences of it
+#pragma clang fp contract(off)
+  if (bc == 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) UB[v] = bcU[v];
+    return;
+  }
+  const double dot = UL[1] * nx + UL[2] * ny + UL[3] * nz;
+  UB[0] = UL[0];
+  UB[1] = UL[1] - dot * nx;
+  UB[2] = UL[2] - dot * ny;
+  UB[3] = UL[3] - dot * nz;
+  UB[4] = UL[4] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) / UL[0] +
+          0.5 * (UB[1] * UB[1] + UB[2] * UB[2] + UB[3] * UB[3]) / UB[0];
+}
 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last
 // index fastest) with the NO x NI matrix M; output has NO in place of NI.

@@ -171,9 +171,15 @@ k_rhs3(const double *__restrict__ U, const double *__restrict__ T, const double 
     const double jr = geo.fjr[es];
     double UL[5], UR[5], FL[5];
     const double *tl = T + ((size_t)e * 6 + s) * 5 * QF2;
-    const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + a]; UR[v] = tr[v * QF2 + an]; }
+    for (int v = 0; v < 5; ++v) UL[v] = tl[v * QF2 + a];
+    if (nb >= 0) {
+      const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) UR[v] = tr[v * QF2 + an];
+    } else {
+      ghost3(-1 - nb, UL, nx, ny, nz, ph.bcU, UR);
+    }
     double Fh[5];
     riemann3(UL, UR, nx, ny, nz, ph.gamma, ph.flux, FL, Fh, bad);
 #pragma unroll

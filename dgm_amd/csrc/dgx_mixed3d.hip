@@ -423,9 +423,15 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
     const double jr = geo.fjr[es];
     double UL[5], UR[5], FL[5];
     const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
-    const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { UL[v] = tl[v * QF2 + q]; UR[v] = tr[v * QF2 + qn]; }
+    for (int v = 0; v < 5; ++v) UL[v] = tl[v * QF2 + q];
+    if (nb >= 0) {
+      const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) UR[v] = tr[v * QF2 + qn];
+    } else {
+      ghost3(-1 - nb, UL, nx, ny, nz, ph.bcU, UR);
+    }
     double Fh[5];
     riemann3(UL, UR, nx, ny, nz, ph.gamma, ph.flux, FL, Fh, bad);
     const int qa = q / QF, qb = q - qa * QF;

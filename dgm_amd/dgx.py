@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("DGX_LIBRARY") or os.path.join(HERE, "libdgx.so")
 DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D = 1, 2, 8, 9
 DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
 DGX_VISCOUS_BR1, DGX_VISCOUS_BR2 = 0, 1
+DGX_BC_WALL, DGX_BC_STATE = 0, 1
 DGX_FP64, DGX_MIXED = 0, 1
 DGX_TVD, DGX_RK4, DGX_LSRK14 = 3, 4, 14
 DGX_NODAL, DGX_MODAL = 0, 1
@@ -28,7 +29,7 @@ class DomainDesc(ctypes.Structure):
                 ("eqntype", ctypes.c_int), ("flux", ctypes.c_int), ("precision", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("vis", ctypes.c_double), ("Pr", ctypes.c_double),
                 ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip),
-                ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double)]
+                ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5)]
 
 
 class DgxError(RuntimeError):
@@ -89,7 +90,7 @@ class GPUDomain:
     """A DG domain on the GPU: modal FP64 state, residual and RK stepping."""
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
-                 precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0):
+                 precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -97,7 +98,8 @@ class GPUDomain:
         self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
         desc = DomainDesc(self.nsd, int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
                           float(gamma), float(vis), float(Pr), _ptr(self._vert), len(self._con),
-                          self._con.ctypes.data_as(_ip), int(viscous), float(br2_eta))
+                          self._con.ctypes.data_as(_ip), int(viscous), float(br2_eta),
+                          (ctypes.c_double * 5)(*([0.0] * 5 if bc_state is None else [float(v) for v in bc_state])))
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))

@@ -62,6 +62,23 @@ def periodic_hexes(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> Mesh:
     return Mesh(nsd=3, vert=vert, con=con)
 
 
+def channel_hexes(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), bc_bottom=0, bc_top=1) -> Mesh:
+    """Hexes periodic in x and y with boundary sides on z = lo[2] (side 0 of
+    the bottom layer) and z = hi[2] (side 5 of the top layer).  Boundary rows
+    are (eid, sid, -1, bc type): 0 slip wall (Euler3d::WallBC,
+    src/Euler3d.cpp:876-909), 1 state (StateBC, src/BC.cpp:17-28)."""
+    m = periodic_hexes(nx, ny, nz, lo, hi)
+    con = m.con.copy()
+    for r in range(len(con)):
+        e, s = int(con[r, 0]), int(con[r, 1])
+        k = e // (nx * ny)
+        if s == 0 and k == 0:
+            con[r, 2], con[r, 3] = -1, bc_bottom
+        elif s == 5 and k == nz - 1:
+            con[r, 2], con[r, 3] = -1, bc_top
+    return Mesh(nsd=3, vert=m.vert, con=con)
+
+
 def gll_points(q):
     """The q Gauss-Lobatto-Legendre points on [-1, 1], ascending: +-1 and the
     roots of P'_{q-1} (src/Polylib.cpp zwgll)."""

@@ -72,6 +72,13 @@ enum dgx_layout { DGX_NODAL = 0, DGX_MODAL = 1 };
 
 typedef struct dgx_domain *dgx_handle;
 
+/* Boundary sides (Euler3d): a .con row (eid, sid, -1, type) marks side sid
+ * of element eid as a boundary instead of a link.  Slip wall as
+ * Euler3d::WallBC (src/Euler3d.cpp:876-909), imposed state as StateBC
+ * (src/BC.cpp:17-28); the ghost values meet the interior through the same
+ * Riemann flux as an interior face. */
+enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1 };
+
 /* Mesh and physics description.  `vert` holds, per element, nsd rows of
  * vertex coordinates exactly as in a root.msh file ([elem][dim][vertex],
  * 4 vertices for quads, 8 for hexes).  `con` holds `ncon` rows
@@ -92,6 +99,7 @@ typedef struct {
   const int *con;
   int viscous;      /* enum dgx_viscous (Navier-Stokes; 0 = BR1) */
   double br2_eta;   /* BR2 penalty; 0 selects 6, the number of hex faces */
+  double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

@@ -26,6 +26,7 @@
 //   src/Navier_Stokes.cpp:113-258                 2D BR1 diffusion
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
 //   src/LinEuler_quasi.cpp:151-495                2D linearized Euler (uniform mean flow)
+//   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
 //                                                 against the reference is unpinned; the
 //                                                 BR2 tests pin it by invariants
@@ -372,6 +373,7 @@ struct SideInfo {
   bool master = false;       // side->link set (src/SideLoop.hpp:46-50)
   int le = -1, ls = -1;      // linked element/side (both directions)
   bool flip_a = false, flip_b = false, transp = false;
+  int bc = -1;               // boundary side: 0 slip wall, 1 state (no link)
   std::vector<double> nx, ny, nz, J;   // Gauss-point normal and J_side/J_vol
 };
 
@@ -617,6 +619,7 @@ struct Domain {
   void build_links(int ncon, const int *con, const double *vert) {
     for (int r = 0; r < ncon; ++r) {
       const int e = con[4 * r], s = con[4 * r + 1], le = con[4 * r + 2], ls = con[4 * r + 3];
+      if (e >= 0 && le < 0) { el[e].side[s].bc = ls; continue; }   // boundary side: (e, s, -1, bc type)
       if (e < 0 || le < 0) continue;
       SideInfo &me = el[e].side[s], &nb = el[le].side[ls];
       if (nb.master) continue;         // reverse link already set
@@ -961,11 +964,37 @@ struct Domain {
       for (int n = 0; n < nv; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
   }
 
-  // 3D master-side Lax-Friedrichs (or Roe) flux in the rotated frame (src/Euler3d.cpp:584-817)
+  double bc_state[5] = {1.0, 0.0, 0.0, 0.0, 2.5};   // state boundary values (conservative)
+
+  // ghost side values of a boundary side (Domain::enforce_bc, src/Domain.cpp:1346-1350):
+  // slip wall Euler3d::WallBC::apply (src/Euler3d.cpp:876-909): the normal
+  // momentum removed, internal energy kept; state StateBC::apply (src/BC.cpp:17-28)
+  void ghost3d(int bc, const double *Ul, double nx, double ny, double nz, double *Ub) const {
+    if (bc == 1) { for (int n = 0; n < 5; ++n) Ub[n] = bc_state[n]; return; }
+    double up = Ul[1], vp = Ul[2], wp = Ul[3];
+    double dot = up * nx + vp * ny + wp * nz;
+    up = up - dot * nx; vp = vp - dot * ny; wp = wp - dot * nz;
+    Ub[0] = Ul[0]; Ub[1] = up; Ub[2] = vp; Ub[3] = wp;
+    Ub[4] = Ul[4] - 0.5 * (Ul[1] * Ul[1] + Ul[2] * Ul[2] + Ul[3] * Ul[3]) / Ul[0]
+          + 0.5 * (Ub[1] * Ub[1] + Ub[2] * Ub[2] + Ub[3] * Ub[3]) / Ub[0];
+  }
+
+  // 3D master-side Lax-Friedrichs (or Roe) flux in the rotated frame (src/Euler3d.cpp:584-817);
+  // boundary sides pair with their ghost values
   void convective_flux3d(VF &Us, VF &Uf) {
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
+        if (S.bc >= 0) {
+          for (int g = 0; g < qs; ++g) {
+            double Ul[5], Ub[5], Fpl[5], Fmr[5], Fh[5];
+            for (int n = 0; n < 5; ++n) Ul[n] = es(Us, n, e, s)[g];
+            ghost3d(S.bc, Ul, S.nx[g], S.ny[g], S.nz[g], Ub);
+            status |= point_flux3d(gamma, flux_type, Ul, Ub, S.nx[g], S.ny[g], S.nz[g], Fpl, Fmr, Fh);
+            for (int n = 0; n < 5; ++n) { es(Us, n, e, s)[g] = Fpl[n]; es(Uf, n, e, s)[g] = Fh[n]; }
+          }
+          continue;
+        }
         if (!S.master) continue;
         for (int g = 0; g < qs; ++g) {
           const int ql = sidemap(S, g), qr = sidemap_slave(g);
@@ -1392,6 +1421,11 @@ int orc_riemann(int nsd, int flux, double gamma, const double *Ul, const double 
   double Fpl[5], Fmr[5];
   if (nsd == 2) return point_flux2d(gamma, flux, Ul, Ur, n[0], n[1], Fpl, Fmr, Fh);
   return point_flux3d(gamma, flux, Ul, Ur, n[0], n[1], n[2], Fpl, Fmr, Fh);
+}
+// conservative state of the state boundary condition (3D Euler)
+void orc_set_bc_state(void *h, const double *U) {
+  Domain *d = (Domain *)h;
+  std::memcpy(d->bc_state, U, 5 * sizeof(double));
 }
 // viscous scheme of the Navier-Stokes residual: 0 BR1 (the reference's), 1 BR2
 // with penalty eta (3D only)

@@ -48,6 +48,7 @@ def _load():
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
     lib.orc_riemann.restype = ctypes.c_int
     lib.orc_set_viscous.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    lib.orc_set_bc_state.argtypes = [ctypes.c_void_p, _dp]
     lib.orc_set_linear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     _lib = lib
     return lib
@@ -80,6 +81,11 @@ class Oracle:
             self.lib.orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_bc_state(self, U):
+        """Conservative state imposed on state boundaries (3D Euler)."""
+        self._bcU = np.ascontiguousarray(U, dtype=np.float64).reshape(5)
+        self.lib.orc_set_bc_state(self.h, _ptr(self._bcU))
 
     def set_linear(self, Ax, Ay, u0, v0, c0):
         """Matrices of the linear system (eqn 4); Lax-Friedrichs speed |u0 nx + v0 ny| + c0."""

@@ -131,3 +131,21 @@ def test_mixed_rk4_matches_fp64():
     rel = _rel(st[dgx.DGX_MIXED], st[dgx.DGX_FP64])
     print(f"[mixed] RK4 state rel L2 per field: {rel}", flush=True)
     assert np.all(rel <= 1e-6), rel
+
+
+def test_mixed_residual_with_walls():
+    H = 2.0
+    m = meshgen.channel_hexes(3, 3, 3, (0.0, 0.0, 0.0), (2 * np.pi, 2 * np.pi, H), bc_bottom=0, bc_top=1)
+    bc_state = [1.0, 0.0, 0.0, 0.0, 1.0 / (1.4 * 0.4 * 0.01) + 0.25]
+    out = {}
+    for prec in (dgx.DGX_FP64, dgx.DGX_MIXED):
+        g = dgx.GPUDomain(m, 5, dgx.DGX_EULER3D, precision=prec, bc_state=bc_state)
+        x = g.coords()
+        U = ic.taylor_green(x[0], x[1], x[2])
+        U[3] += 0.05 * np.cos(np.pi * x[2] / H) * np.sin(x[0])
+        g.upload(U)
+        out[prec] = g.rhs(dgx.DGX_MODAL)
+        g.close()
+    rel = _rel(out[dgx.DGX_MIXED], out[dgx.DGX_FP64])
+    print(f"[mixed] walls: residual rel L2 per field: {rel}", flush=True)
+    assert np.all(rel < 1e-5), rel

@@ -71,9 +71,13 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=8, viscous=1), b"DGX_VISCOUS_BR2"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=-1.0), b"br2_eta"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
+    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler3d"),
+    (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
 ])
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
+    if "boundary" in case:
+        m = meshgen.channel_hexes(2, 2, 2, bc_bottom=case["boundary"], bc_top=0)
     d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0),
                     precision=case.get("precision", 0), viscous=case.get("viscous", 0),
                     br2_eta=case.get("br2_eta", 0.0))
