@@ -96,3 +96,36 @@ def quad_nodes(mesh: Mesh, p: int) -> np.ndarray:
     xi, eta = xi.ravel(), eta.ravel()
     N = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)]) / 4
     return np.einsum("edv,vq->deq", mesh.vert, N)
+
+
+def hex_nodes(mesh: Mesh, p: int) -> np.ndarray:
+    """Nodal coordinates [3][elem][q^3] of hexes at the q = p+2 GLL points,
+    node (i*q + j)*q + k at (xi_i, eta_j, zeta_k), trilinear vertex map (bottom
+    face 0-3 counter-clockwise, top face 4-7; src/Hex.cpp:817-866)."""
+    z = gll_points(p + 2)
+    a, b, c = np.meshgrid(z, z, z, indexing="ij")
+    a, b, c = a.ravel(), b.ravel(), c.ravel()
+    sa = [-1, 1, 1, -1, -1, 1, 1, -1]
+    sb = [-1, -1, 1, 1, -1, -1, 1, 1]
+    sc = [-1, -1, -1, -1, 1, 1, 1, 1]
+    N = np.stack([(1 + sa[n] * a) * (1 + sb[n] * b) * (1 + sc[n] * c) / 8 for n in range(8)])
+    return np.einsum("edv,vq->deq", mesh.vert, N)
+
+
+def mapped_nodes(mesh: Mesh, p: int, fn) -> np.ndarray:
+    """Curved hexes: the GLL nodes of the affine mesh moved by fn(x, y, z) ->
+    (x', y', z'), for the curved-element path (dgx_domain_desc.xnodes)."""
+    x = hex_nodes(mesh, p)
+    return np.stack(fn(x[0], x[1], x[2]))
+
+
+def bump_map(lo, hi, amp):
+    """The channel [lo, hi] with a smooth periodic bump of height `amp` on its
+    bottom wall, decaying linearly to the flat top: z' = z + amp b(x, y) (hi_z - z) / (hi_z - lo_z),
+    b = (1 - cos(2 pi x / Lx)) (1 - cos(2 pi y / Ly)) / 4."""
+    Lx, Ly, H = hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]
+
+    def fn(x, y, z):
+        b = (1 - np.cos(2 * np.pi * (x - lo[0]) / Lx)) * (1 - np.cos(2 * np.pi * (y - lo[1]) / Ly)) / 4
+        return x, y, z + amp * b * (hi[2] - z) / H
+    return fn

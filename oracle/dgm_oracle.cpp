@@ -27,6 +27,7 @@
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
 //   src/LinEuler_quasi.cpp:151-495                2D linearized Euler (uniform mean flow)
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
+//   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
 //                                                 against the reference is unpinned; the
 //                                                 BR2 tests pin it by invariants
@@ -382,6 +383,11 @@ struct Elem {
   double rx = 0, ry = 0, rz = 0, sx = 0, sy = 0, sz = 0, tx = 0, ty = 0, tz = 0;
   std::vector<double> wJ, cx, cy, cz;
   SideInfo side[6];
+  // hexes: metrics rx ry rz sx sy sz tx ty tz and J at the nodes (constant
+  // when affine); curved elements also keep the Cholesky factor of their mass
+  // matrix B^T diag(wJ) B (src/Hex.cpp:289-318)
+  bool curved = false;
+  std::vector<double> mn[9], Jn, chol;
 };
 
 struct VF {                  // vField: nodal element data + side data
@@ -568,23 +574,33 @@ struct Domain {
           for (int j = 0; j < q; ++j) for (int k = 0; k < q; ++k) E.wJ[(i * q + j) * q + k] *= w[k];
         }
         for (int i = 0; i < q; ++i) for (int jk = 0; jk < q * q; ++jk) E.wJ[i * q * q + jk] *= w[i];
-        // face geometry (src/Hex.cpp:1385-1494)
+        const double mc[9] = {E.rx, E.ry, E.rz, E.sx, E.sy, E.sz, E.tx, E.ty, E.tz};
+        for (int c = 0; c < 9; ++c) E.mn[c].assign(qtot, mc[c]);
+        E.Jn.assign(qtot, detJ);
+        face_geometry3(E);
+      }
+    }
+  }
+
+  // hex face geometry from the nodal metrics and coordinates (src/Hex.cpp:1385-1494):
+  // normals from the metric row of the face's direction interpolated to the
+  // Gauss points, J_side from the face tangents, divided by J interpolated alike
+  void face_geometry3(Elem &E) {
+    {
+      {
         const int q2 = q * q;
         std::vector<double> wk(q2), X(q2), Y(q2), Zc(q2), XR(q2), XS(q2), YR(q2), YS(q2), ZR(q2), ZS(q2);
         for (int s = 0; s < 6; ++s) {
           SideInfo &S = E.side[s];
           S.nx.assign(qs, 0.0); S.ny.assign(qs, 0.0); S.nz.assign(qs, 0.0); S.J.assign(qs, 0.0);
-          double NX, NY, NZ;
-          if (s == 0 || s == 5) { NX = E.tx; NY = E.ty; NZ = E.tz; }
-          else if (s == 1 || s == 3) { NX = E.sx; NY = E.sy; NZ = E.sz; }
-          else { NX = E.rx; NY = E.ry; NZ = E.rz; }
-          std::fill(wk.begin(), wk.end(), NX); to_side(wk.data(), S.nx.data());
-          std::fill(wk.begin(), wk.end(), NY); to_side(wk.data(), S.ny.data());
-          std::fill(wk.begin(), wk.end(), NZ); to_side(wk.data(), S.nz.data());
+          const int row = (s == 0 || s == 5) ? 6 : ((s == 1 || s == 3) ? 3 : 0);
+          get_side(E.mn[row].data(), s, wk.data()); to_side(wk.data(), S.nx.data());
+          get_side(E.mn[row + 1].data(), s, wk.data()); to_side(wk.data(), S.ny.data());
+          get_side(E.mn[row + 2].data(), s, wk.data()); to_side(wk.data(), S.nz.data());
           if (s == 0 || s == 1 || s == 4)
             for (int g = 0; g < qs; ++g) { S.nx[g] *= -1.0; S.ny[g] *= -1.0; S.nz[g] *= -1.0; }
           std::vector<double> iJ(qs);
-          std::fill(wk.begin(), wk.end(), detJ); to_side(wk.data(), iJ.data());
+          get_side(E.Jn.data(), s, wk.data()); to_side(wk.data(), iJ.data());
           for (int g = 0; g < qs; ++g) {
             double t = S.nx[g] * S.nx[g];
             t = S.ny[g] * S.ny[g] + t;
@@ -685,6 +701,13 @@ struct Domain {
     const Elem &E = el[e];
     std::vector<double> fr(qtot), fs(qtot), ft(qtot);
     grad_r(f, fr.data(), fs.data(), ft.data());
+    if (E.curved) {   // metrics at the nodes (src/Hex.cpp:2030-2071 with geom arrays)
+      const int c = dir == 'x' ? 0 : (dir == 'y' ? 1 : 2);
+      for (int i = 0; i < qtot; ++i) df[i] = E.mn[c][i] * fr[i];
+      for (int i = 0; i < qtot; ++i) df[i] = E.mn[3 + c][i] * fs[i] + df[i];
+      for (int i = 0; i < qtot; ++i) df[i] = E.mn[6 + c][i] * ft[i] + df[i];
+      return;
+    }
     double a, b, c = 0;
     if (dir == 'x') { a = E.rx; b = E.sx; c = E.tx; }
     else if (dir == 'y') { a = E.ry; b = E.sy; c = E.ty; }
@@ -695,6 +718,10 @@ struct Domain {
   }
   void gradient_all(int e, const double *f, double *fx, double *fy, double *fz) const {
     const Elem &E = el[e];
+    if (E.curved) {
+      gradient_dir(e, f, fx, 'x'); gradient_dir(e, f, fy, 'y'); gradient_dir(e, f, fz, 'z');
+      return;
+    }
     std::vector<double> fr(qtot), fs(qtot), ft(qtot);
     grad_r(f, fr.data(), fs.data(), ft.data());
     for (int i = 0; i < qtot; ++i) fx[i] = E.rx * fr[i];
@@ -725,8 +752,92 @@ struct Domain {
       for (int a = 0; a < L; ++a) mm(L, q, q, B.data(), q, H.data() + a * q2, q, Ha.data() + a * L * q, q);
       mm_nt(L * L, L, q, Ha.data(), q, B.data(), q, fh, L);
     }
+    if (E.curved) { chol_solve(E.chol, fh); return; }
     const double iJ = 1.0 / E.J;
     for (int i = 0; i < nm; ++i) fh[i] *= iJ;
+  }
+  // M = L L^T (lower factor, row-major nm x nm): solve in place
+  void chol_solve(const std::vector<double> &Lf, double *b) const {
+    for (int i = 0; i < nm; ++i) {
+      double t = b[i];
+      for (int k = 0; k < i; ++k) t -= Lf[(size_t)i * nm + k] * b[k];
+      b[i] = t / Lf[(size_t)i * nm + i];
+    }
+    for (int i = nm - 1; i >= 0; --i) {
+      double t = b[i];
+      for (int k = i + 1; k < nm; ++k) t -= Lf[(size_t)k * nm + i] * b[k];
+      b[i] = t / Lf[(size_t)i * nm + i];
+    }
+  }
+  // curved hex from nodal coordinates: Hex::compute_curved_geometry
+  // (src/Hex.cpp:3992-4080) at the nodes, the side geometry from those
+  // metrics, and the Cholesky factor of B^T diag(wJ) B for the projection
+  void set_curved(int e, const double *x, const double *y, const double *z) {
+    Elem &E = el[e];
+    E.cx.assign(x, x + qtot); E.cy.assign(y, y + qtot); E.cz.assign(z, z + qtot);
+    std::vector<double> xr(qtot), xs(qtot), xt(qtot), yr(qtot), ys(qtot), yt(qtot), zr(qtot), zs(qtot), zt(qtot);
+    grad_r(x, xr.data(), xs.data(), xt.data());
+    grad_r(y, yr.data(), ys.data(), yt.data());
+    grad_r(z, zr.data(), zs.data(), zt.data());
+    E.Jn.assign(qtot, 0.0);
+    for (int c = 0; c < 9; ++c) E.mn[c].assign(qtot, 0.0);
+    for (int i = 0; i < qtot; ++i) {
+      double t1 = xr[i] * ys[i]; t1 = t1 * zt[i];
+      double t2 = yr[i] * zs[i]; t1 = xt[i] * t2 + t1;
+      t2 = zr[i] * xs[i]; t1 = yt[i] * t2 + t1;
+      t2 = xt[i] * ys[i]; t1 = t1 - zr[i] * t2;
+      t2 = zt[i] * xs[i]; t1 = t1 - yr[i] * t2;
+      t2 = xr[i] * zs[i]; const double J = t1 - yt[i] * t2;
+      if (!(J > 0.0)) status |= 4;
+      E.Jn[i] = J;
+      E.mn[0][i] = (ys[i] * zt[i] - yt[i] * zs[i]) / J;
+      E.mn[1][i] = -(xs[i] * zt[i] - xt[i] * zs[i]) / J;
+      E.mn[2][i] = (xs[i] * yt[i] - xt[i] * ys[i]) / J;
+      E.mn[3][i] = -(yr[i] * zt[i] - yt[i] * zr[i]) / J;
+      E.mn[4][i] = (xr[i] * zt[i] - xt[i] * zr[i]) / J;
+      E.mn[5][i] = -(xr[i] * yt[i] - xt[i] * yr[i]) / J;
+      E.mn[6][i] = (yr[i] * zs[i] - ys[i] * zr[i]) / J;
+      E.mn[7][i] = -(xr[i] * zs[i] - xs[i] * zr[i]) / J;
+      E.mn[8][i] = (xr[i] * ys[i] - xs[i] * yr[i]) / J;
+    }
+    E.wJ.assign(qtot, 0.0);
+    for (int i = 0; i < q; ++i)
+      for (int j = 0; j < q; ++j)
+        for (int k = 0; k < q; ++k) {
+          const int m = (i * q + j) * q + k;
+          E.wJ[m] = w[i] * w[j] * w[k] * E.Jn[m];
+        }
+    E.curved = true;
+    face_geometry3(E);
+    // mass matrix M[a][b] = sum_n phi_a(n) wJ(n) phi_b(n), phi from B (L x q per direction)
+    std::vector<double> Phi((size_t)nm * qtot);
+    for (int a = 0; a < L; ++a)
+      for (int b = 0; b < L; ++b)
+        for (int c = 0; c < L; ++c)
+          for (int i = 0; i < q; ++i)
+            for (int j = 0; j < q; ++j)
+              for (int k = 0; k < q; ++k)
+                Phi[(size_t)((a * L + b) * L + c) * qtot + (i * q + j) * q + k] =
+                    B[a * q + i] * B[b * q + j] * B[c * q + k];
+    std::vector<double> M((size_t)nm * nm, 0.0);
+    for (int a = 0; a < nm; ++a)
+      for (int b = 0; b <= a; ++b) {
+        double t = 0.0;
+        for (int n = 0; n < qtot; ++n) t += Phi[(size_t)a * qtot + n] * E.wJ[n] * Phi[(size_t)b * qtot + n];
+        M[(size_t)a * nm + b] = t;
+      }
+    E.chol.assign((size_t)nm * nm, 0.0);
+    for (int i = 0; i < nm; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double t = M[(size_t)i * nm + j];
+        for (int k = 0; k < j; ++k) t -= E.chol[(size_t)i * nm + k] * E.chol[(size_t)j * nm + k];
+        if (i == j) {
+          if (!(t > 0.0)) status |= 4;
+          E.chol[(size_t)i * nm + i] = std::sqrt(t);
+        } else {
+          E.chol[(size_t)i * nm + j] = t / E.chol[(size_t)j * nm + j];
+        }
+      }
   }
   void backward(const double *fh, double *f) const {
     if (nsd == 2) {
@@ -1421,6 +1532,14 @@ int orc_riemann(int nsd, int flux, double gamma, const double *Ul, const double 
   double Fpl[5], Fmr[5];
   if (nsd == 2) return point_flux2d(gamma, flux, Ul, Ur, n[0], n[1], Fpl, Fmr, Fh);
   return point_flux3d(gamma, flux, Ul, Ur, n[0], n[1], n[2], Fpl, Fmr, Fh);
+}
+// curved hexes: nodal coordinates [3][ne][qtot] at the GLL points replace the
+// affine map of every element
+void orc_set_nodes(void *h, const double *xyz) {
+  Domain *d = (Domain *)h;
+  const size_t n = (size_t)d->qtot, ne = (size_t)d->ne;
+  for (int e = 0; e < d->ne; ++e)
+    d->set_curved(e, xyz + (0 * ne + e) * n, xyz + (1 * ne + e) * n, xyz + (2 * ne + e) * n);
 }
 // conservative state of the state boundary condition (3D Euler)
 void orc_set_bc_state(void *h, const double *U) {

@@ -49,6 +49,7 @@ def _load():
     lib.orc_riemann.restype = ctypes.c_int
     lib.orc_set_viscous.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     lib.orc_set_bc_state.argtypes = [ctypes.c_void_p, _dp]
+    lib.orc_set_nodes.argtypes = [ctypes.c_void_p, _dp]
     lib.orc_set_linear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     _lib = lib
     return lib
@@ -81,6 +82,12 @@ class Oracle:
             self.lib.orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_nodes(self, xyz):
+        """Curved hexes: nodal coordinates [3][ne][qtot] at the GLL points."""
+        self._xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        assert self._xyz.shape == (3, self.ne, self.qtot), self._xyz.shape
+        self.lib.orc_set_nodes(self.h, _ptr(self._xyz))
 
     def set_bc_state(self, U):
         """Conservative state imposed on state boundaries (3D Euler)."""

@@ -1,0 +1,105 @@
+"""Curved hexes in the oracle: nodal metrics (Hex::compute_curved_geometry,
+src/Hex.cpp:3992-4080), side geometry from them, and the projection through
+the element mass matrix B^T diag(wJ) B (src/Hex.cpp:289-318).
+
+The reference's curved fixtures (runs/cyl, scyl, cvquad) are 2D and need its
+curved-side files, so the 3D curved path is pinned by:
+* affine nodes fed through the curved path reproduce the affine residual,
+  which the golden fixtures pin (the mass matrix is then J I to rounding);
+* the projection of any modal field is exact: forward(backward(u)) = u, on
+  strongly curved elements (a wrong mass matrix or solve breaks it);
+* consistency on a curved mesh: the residual of a smooth state converges
+  spectrally to the exact flux divergence, under p-refinement.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from dgm_amd import mesh as meshgen
+from oracle_lib import Oracle
+
+GAMMA = 1.4
+
+
+def _state(x, y, z):
+    rho = 1.0 + 0.1 * np.sin(x) * np.cos(y) * np.cos(z)
+    u = 0.3 * np.sin(x) * np.cos(y)
+    v = -0.3 * np.cos(x) * np.sin(y) * np.cos(z)
+    w = 0.2 * np.sin(z) * np.cos(x)
+    p = 1.0 + 0.1 * np.cos(2 * x) * np.cos(y + z)
+    E = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, E])
+
+
+def _flux_div(x, y, z, h=1e-5):
+    """Exact div F(U) of the Euler flux by central differences of the analytic state (error ~1e-10)."""
+    def F(x, y, z):
+        U = _state(x, y, z)
+        r, mx, my, mz, E = U
+        u, v, w = mx / r, my / r, mz / r
+        p = (GAMMA - 1) * (E - 0.5 * (mx * u + my * v + mz * w))
+        fx = np.stack([mx, mx * u + p, my * u, mz * u, (E + p) * u])
+        fy = np.stack([my, mx * v, my * v + p, mz * v, (E + p) * v])
+        fz = np.stack([mz, mx * w, my * w, mz * w + p, (E + p) * w])
+        return fx, fy, fz
+    dx = (F(x + h, y, z)[0] - F(x - h, y, z)[0]) / (2 * h)
+    dy = (F(x, y + h, z)[1] - F(x, y - h, z)[1]) / (2 * h)
+    dz = (F(x, y, z + h)[2] - F(x, y, z - h)[2]) / (2 * h)
+    return dx + dy + dz
+
+
+def wavy_map(L, amp):
+    """Periodic-preserving interior deformation of [0, L]^3 (boundary faces stay flat)."""
+    def fn(x, y, z):
+        s = lambda t: np.sin(2 * np.pi * t / L)
+        return (x + amp * s(y) * s(z), y + amp * s(z) * s(x), z + amp * s(x) * s(y))
+    return fn
+
+
+def test_affine_nodes_reproduce_affine_path():
+    m = meshgen.channel_hexes(2, 2, 2, (0, 0, 0), (2 * np.pi, 2 * np.pi, 2.0), bc_bottom=0, bc_top=1)
+    bc = [1.0, 0.1, 0.0, 0.0, 2.6]
+    for flux in (0, 1):
+        a, b = Oracle(m, 4, 8, flux=flux), Oracle(m, 4, 8, flux=flux)
+        a.set_bc_state(bc); b.set_bc_state(bc)
+        b.set_nodes(meshgen.hex_nodes(m, 4))
+        x = a.coords()
+        U = _state(x[0], x[1], x[2])
+        Fa, Fb = a.rhs(U), b.rhs(U)
+        assert np.max(np.abs(Fa - Fb)) <= 1e-12 * np.max(np.abs(Fa)), np.max(np.abs(Fa - Fb))
+
+
+def test_projection_exact_on_curved_elements():
+    m = meshgen.channel_hexes(2, 2, 2, (0, 0, 0), (1.0, 1.0, 1.0))
+    p = 4
+    o = Oracle(m, p, 8)
+    o.set_nodes(meshgen.mapped_nodes(m, p, meshgen.bump_map((0, 0, 0), (1, 1, 1), 0.3)))
+    rng = np.random.default_rng(1)
+    Uh = rng.standard_normal((5, m.ne, (p + 1) ** 3))
+    o.set_modal(Uh)
+    back = o.modal()
+    assert np.max(np.abs(back - Uh)) < 1e-12, np.max(np.abs(back - Uh))
+
+
+def _consistency_error(p, n=2, amp=0.15):
+    L = 2 * np.pi
+    m = meshgen.periodic_hexes(n, n, n, (0, 0, 0), (L, L, L))
+    o = Oracle(m, p, 8)
+    xyz = meshgen.mapped_nodes(m, p, wavy_map(L, amp))
+    o.set_nodes(xyz)
+    U = _state(*xyz)
+    F = o.rhs(U)
+    o.set_state(_flux_div(*xyz))                        # the exact divergence, projected alike
+    o.project_state()
+    ex = o.state()
+    return np.sqrt(np.sum((F - ex) ** 2)) / np.sqrt(np.sum(ex ** 2))
+
+
+def test_curved_residual_converges_spectrally():
+    errs = [_consistency_error(p, n=4) for p in (3, 5, 7)]
+    assert errs[2] < 2e-5, errs
+    assert errs[0] / errs[1] > 10 and errs[1] / errs[2] > 10, errs
+    # the curvature adds little to the resolution error of the affine mesh;
+    # a wrong normal, metric or mass matrix would leave an O(1) inconsistency
+    flat = _consistency_error(5, n=4, amp=0.0)
+    assert errs[1] < 1.5 * flat, (errs[1], flat)
