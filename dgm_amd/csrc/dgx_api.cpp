@@ -141,7 +141,15 @@ HostBasis make_basis(int p) {
   }
   for (int a = 0; a < QF; ++a)
     for (int i = 0; i < Q; ++i) b.Ig[a * Q + i] = (double)lagrange(z, i, zg[a]);
-  b.PI.resize(L * Q); b.KE.resize(2 * Q);
+  b.PI.resize(L * Q); b.KE.resize(2 * Q); b.BL.resize(L * Q); b.GL.resize(Q * QF);
+  for (int l = 0; l < L; ++l)
+    for (int i = 0; i < Q; ++i) b.BL[l * Q + i] = (double)normleg(l, z[i]);
+  for (int i = 0; i < Q; ++i)
+    for (int a = 0; a < QF; ++a) {
+      ld s = 0;
+      for (int l = 0; l < L; ++l) s += normleg(l, z[i]) * wg[a] * normleg(l, zg[a]);
+      b.GL[i * QF + a] = (double)s;
+    }
   for (int l = 0; l < L; ++l)
     for (int i = 0; i < Q; ++i) {
       ld s = 0;
@@ -169,7 +177,8 @@ int upload_all_tables() {
   for (int p = 1; p <= dgx::kMaxOrder; ++p) {
     HostBasis b = make_basis(p);
     size_t off = dgx::tab_offset(p);
-    for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig, &b.PI, &b.KE}) {
+    for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig, &b.PI, &b.KE, &b.BL, &b.GL,
+                                         &b.D}) {
       std::copy(t->begin(), t->end(), packed.begin() + off);
       off += t->size();
     }
@@ -179,7 +188,8 @@ int upload_all_tables() {
   if ((e = dgx::upload_tables_2d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -231,6 +241,15 @@ int launch(dgx_domain *d, int op, const double *U, double *out) {
 }
 
 int transform(dgx_domain *d, int op, const double *x, double *y) {
+  if (d->curved && op == dgx::OP_FORWARD) {   // projection through the curved mass matrix
+    dgx::CurvedArgs ca;
+    ca.U = x; ca.R = y; ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
+    ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
+    ca.project = true; ca.stream = d->stream;
+    hipError_t e = dgx::launch_curved3(ca);
+    if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved projection: ") + hipGetErrorString(e));
+    return DGX_OK;
+  }
   dgx::LaunchArgs a;
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf;
   a.X = x; a.Y = y; a.tab = d->tab; a.stream = d->stream;
@@ -275,6 +294,16 @@ int residual(dgx_domain *d, const double *U, double *R) {
     return DGX_OK;
   }
   if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
+  if (d->curved) {
+    dgx::CurvedArgs ca;
+    ca.U = U; ca.T = d->T; ca.R = R; ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav;
+    ca.geo = d->geo; ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
+    ca.stream = d->stream;
+    hipError_t e = timed(d, 1, [&] { return dgx::launch_curved3(ca); });
+    if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved residual: ") + hipGetErrorString(e));
+    ++d->stages;
+    return DGX_OK;
+  }
   if (d->visc) {
     if ((rc = launch(d, dgx::OP_GRAD, U, nullptr))) return rc;
     if ((rc = launch(d, dgx::OP_VISC, U, nullptr))) return rc;
@@ -287,6 +316,133 @@ int residual(dgx_domain *d, const double *U, double *R) {
 int ensure_nodal(dgx_domain *d) {
   if (d->nodal) return DGX_OK;
   return alloc(&d->nodal, (size_t)d->nf * d->ne * d->nq);
+}
+
+// Curved hexes from nodal coordinates xn[3][ne][Q^3]: metrics and J at the
+// nodes as Hex::compute_curved_geometry (src/Hex.cpp:3992-4080), the face
+// normals and J_side/J_vol at the Gauss points from them as
+// Hex::compute_side_geometry (src/Hex.cpp:1385-1494), and per element the mean
+// J that starts the kernel's mass-matrix solve.
+int curved_geometry(dgx_domain *d, const double *xn) {
+  const int ne = d->ne, Q = d->Q, QF = d->QF, Q3 = d->nq, QF2 = QF * QF, Q2 = Q * Q;
+  const HostBasis &b = d->hb;
+  const std::vector<double> &D = b.D, &w = b.w, &Ig = b.Ig;
+  std::vector<double> gm((size_t)ne * 10 * Q3), fgeo((size_t)ne * 6 * QF2 * 4), jav(ne);
+  std::vector<double> der(9 * (size_t)Q3), mn(9 * (size_t)Q3), J(Q3), wJ(Q3);
+  auto face = [&](const double *f, int s, double *to) {   // GLL face values [Q][Q] (get_side)
+    for (int u = 0; u < Q; ++u)
+      for (int v = 0; v < Q; ++v) {
+        int i, j, k;
+        switch (s) {
+          case 0: i = u; j = v; k = 0; break;
+          case 1: i = u; j = 0; k = v; break;
+          case 2: i = Q - 1; j = u; k = v; break;
+          case 3: i = u; j = Q - 1; k = v; break;
+          case 4: i = 0; j = u; k = v; break;
+          default: i = u; j = v; k = Q - 1; break;
+        }
+        to[u * Q + v] = f[(i * Q + j) * Q + k];
+      }
+  };
+  auto gauss = [&](const double *f, double *to) {   // GLL face -> Gauss points (Ig (x) Ig)
+    std::vector<double> t(QF * Q);
+    for (int a2 = 0; a2 < QF; ++a2)
+      for (int v = 0; v < Q; ++v) {
+        double s = 0.0;
+        for (int u = 0; u < Q; ++u) s += Ig[a2 * Q + u] * f[u * Q + v];
+        t[a2 * Q + v] = s;
+      }
+    for (int a2 = 0; a2 < QF; ++a2)
+      for (int b2 = 0; b2 < QF; ++b2) {
+        double s = 0.0;
+        for (int v = 0; v < Q; ++v) s += t[a2 * Q + v] * Ig[b2 * Q + v];
+        to[a2 * QF + b2] = s;
+      }
+  };
+  std::vector<double> fv(Q2), fg(QF2), X(Q2), Y(Q2), Z(Q2), nrm(3 * QF2), jf(QF2), cr(Q2);
+  for (int e = 0; e < ne; ++e) {
+    const double *C[3] = {xn + ((size_t)0 * ne + e) * Q3, xn + ((size_t)1 * ne + e) * Q3,
+                          xn + ((size_t)2 * ne + e) * Q3};
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < Q; ++i)
+        for (int j = 0; j < Q; ++j)
+          for (int k = 0; k < Q; ++k) {
+            double r = 0.0, s = 0.0, t = 0.0;
+            for (int m = 0; m < Q; ++m) {
+              r += D[i * Q + m] * C[c][(m * Q + j) * Q + k];
+              s += D[j * Q + m] * C[c][(i * Q + m) * Q + k];
+              t += D[k * Q + m] * C[c][(i * Q + j) * Q + m];
+            }
+            const int n = (i * Q + j) * Q + k;
+            der[(3 * c + 0) * Q3 + n] = r; der[(3 * c + 1) * Q3 + n] = s; der[(3 * c + 2) * Q3 + n] = t;
+          }
+    double sw = 0.0, swJ = 0.0;
+    for (int n = 0; n < Q3; ++n) {
+      const double xr = der[n], xs = der[Q3 + n], xt = der[2 * Q3 + n];
+      const double yr = der[3 * Q3 + n], ys = der[4 * Q3 + n], yt = der[5 * Q3 + n];
+      const double zr = der[6 * Q3 + n], zs = der[7 * Q3 + n], zt = der[8 * Q3 + n];
+      const double Jn = xr * ys * zt + yr * zs * xt + zr * xs * yt - xt * ys * zr - zt * xs * yr - xr * zs * yt;
+      if (!(Jn > 0.0))
+        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has a nonpositive Jacobian");
+      J[n] = Jn;
+      mn[0 * Q3 + n] = (ys * zt - yt * zs) / Jn;
+      mn[1 * Q3 + n] = -(xs * zt - xt * zs) / Jn;
+      mn[2 * Q3 + n] = (xs * yt - xt * ys) / Jn;
+      mn[3 * Q3 + n] = -(yr * zt - yt * zr) / Jn;
+      mn[4 * Q3 + n] = (xr * zt - xt * zr) / Jn;
+      mn[5 * Q3 + n] = -(xr * yt - xt * yr) / Jn;
+      mn[6 * Q3 + n] = (yr * zs - ys * zr) / Jn;
+      mn[7 * Q3 + n] = -(xr * zs - xs * zr) / Jn;
+      mn[8 * Q3 + n] = (xr * ys - xs * yr) / Jn;
+      const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+      const double ww = w[i] * w[j] * w[k];
+      wJ[n] = ww * Jn;
+      sw += ww; swJ += wJ[n];
+      for (int c = 0; c < 9; ++c) gm[((size_t)e * 10 + c) * Q3 + n] = wJ[n] * mn[c * Q3 + n];
+      gm[((size_t)e * 10 + 9) * Q3 + n] = wJ[n];
+    }
+    jav[e] = swJ / sw;
+    for (int s = 0; s < 6; ++s) {
+      const int row = (s == 0 || s == 5) ? 6 : ((s == 1 || s == 3) ? 3 : 0);
+      const double sg = (s == 0 || s == 1 || s == 4) ? -1.0 : 1.0;
+      for (int c = 0; c < 3; ++c) {
+        face(mn.data() + (size_t)(row + c) * Q3, s, fv.data());
+        gauss(fv.data(), nrm.data() + c * QF2);
+      }
+      face(J.data(), s, fv.data());
+      gauss(fv.data(), jf.data());
+      face(C[0], s, X.data()); face(C[1], s, Y.data()); face(C[2], s, Z.data());
+      for (int u = 0; u < Q; ++u)
+        for (int v = 0; v < Q; ++v) {
+          double XR = 0, XS = 0, YR = 0, YS = 0, ZR = 0, ZS = 0;
+          for (int m = 0; m < Q; ++m) {
+            XR += D[u * Q + m] * X[m * Q + v]; XS += X[u * Q + m] * D[v * Q + m];
+            YR += D[u * Q + m] * Y[m * Q + v]; YS += Y[u * Q + m] * D[v * Q + m];
+            ZR += D[u * Q + m] * Z[m * Q + v]; ZS += Z[u * Q + m] * D[v * Q + m];
+          }
+          const double cx = YR * ZS - ZR * YS, cy = ZR * XS - XR * ZS, cz = XR * YS - XS * YR;
+          cr[u * Q + v] = std::sqrt(cx * cx + cy * cy + cz * cz);
+        }
+      gauss(cr.data(), fg.data());
+      for (int g = 0; g < QF2; ++g) {
+        double nx = sg * nrm[g], ny = sg * nrm[QF2 + g], nz = sg * nrm[2 * QF2 + g];
+        const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+        double *o = fgeo.data() + (((size_t)e * 6 + s) * QF2 + g) * 4;
+        o[0] = nx / len; o[1] = ny / len; o[2] = nz / len; o[3] = fg[g] / jf[g];
+      }
+    }
+  }
+  int rc;
+  if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
+    return rc;
+  HIPCHK(hipMemcpy(d->d_gm, gm.data(), gm.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_fgeo, fgeo.data(), fgeo.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void **)&d->d_iters, 64));
+  HIPCHK(hipMemset(d->d_iters, 0, 64));
+  std::copy(xn, xn + (size_t)3 * ne * Q3, d->coords.begin());   // the nodes are the element's coordinates
+  d->curved = true;
+  return DGX_OK;
 }
 
 int build(dgx_domain *d, const dgx_domain_desc *desc) {
@@ -541,7 +697,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   HIPCHK(hipMemset(d->R0, 0, n * 8));
   HIPCHK(hipMemset(d->R1, 0, n * 8));
   if ((rc = alloc(&d->T, (size_t)ne * d->ns * d->nf * qs))) return rc;
-  d->fused = (nsd == 3 && dgx::stage3_supported(p));
+  if (desc->xnodes && (rc = curved_geometry(d, desc->xnodes))) return rc;
+  d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
   d->mixed = d->prec == DGX_MIXED;
   if (d->visc && d->mixed) {
@@ -563,7 +720,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2, d->Vf, d->TVf};
+                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -596,6 +753,7 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   b += nf * ne * nq * 8;                                    // nodal transfer buffer
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
   b += 8 * qs * 2 + (size_t)(1 << 20);                      // orientation tables, tables, rounding
+  if (desc->xnodes) b += ne * (10 * nq + 24 * qs + 1) * 8;   // curved: nodal metrics, face geometry
   *bytes = b;
   return DGX_OK;
 }
@@ -622,6 +780,9 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
+  if (desc->xnodes && !(desc->nsd == 3 && desc->eqntype == DGX_EULER3D && desc->precision == DGX_FP64 &&
+                        dgx::curved3_supported(desc->p)))
+    return fail(DGX_ERR_ARGUMENT, "curved elements (xnodes) are available for Euler3d, DGX_FP64, p <= 6");
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
       if (desc->eqntype != DGX_EULER3D) return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d");

@@ -1,0 +1,351 @@
+// Curved hexes (Euler3d, p <= 6): the residual with metrics at the nodes and
+// the projection through the element mass matrix.
+//
+//   volume term   Euler3d::convection with Hex::gradient on curved geometry:
+//                 dF_x/dx = sum_d (dxi_d/dx)(node) D_d F_x at every node
+//                 (src/Euler3d.cpp:468-575, src/Hex.cpp:2030-2071, 3992-4080)
+//   faces         the Riemann flux with the normal of each Gauss point, lifted
+//                 at the face nodes: G2GL((Fhat - F-.n) J_side/J_vol) / w_end
+//                 (src/Hex.cpp:1786-1952)
+//   project       b = B^T (wJ f) and M x = b with M = B^T diag(wJ) B, the curved
+//                 element's mass matrix (src/Hex.cpp:289-318), solved by
+//                 conjugate gradients inside the workgroup: M is symmetric
+//                 positive definite and close to mean(J) I, so about a dozen
+//                 iterations reach rounding.
+#include "dgx_kernels.hpp"
+
+namespace dgx {
+
+template <int P> struct C3 {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
+  static constexpr int L2 = L * L, L3 = L2 * L, Q2 = Q * Q, Q3 = Q2 * Q, QF2 = QF * QF;
+  static constexpr int NT = 256, NW = NT / 64;
+  static constexpr int NPT = (Q3 + NT - 1) / NT;          // nodes per thread
+  static constexpr int NML = (5 * L3 + NT - 1) / NT;      // modal entries per thread (5 fields)
+  // LDS (doubles): Un, Fw [5][Q3]; scratch S1, S2 [SZ] (the largest intermediate:
+  // a transform of 5 fields or the 30 face blocks at the GLL points);
+  // gF [5][6][QF2]; Fc [3][Q3]; reductions [NW][8]
+  static constexpr int SZ = (5 * Q3 > 30 * Q2) ? 5 * Q3 : 30 * Q2;
+  static constexpr int lds = (10 * Q3 + 2 * SZ + 30 * QF2 + 3 * Q3 + 8 * NW) * 8;
+};
+
+// [5][N]^3 blocks (compact) through matrix M [NO][N] along all three dimensions
+template <int N, int NO, int NT>
+__device__ __forceinline__ void chain5(const double *M, const double *in, double *s1, double *s2, double *out,
+                                       int tid) {
+  contract<5, N, N * N, 1, NO, NT>(M, in, s1, tid);            // [5][NO][N N]
+  __syncthreads();
+  contract<5 * NO, N, N, 1, NO, NT>(M, s1, s2, tid);           // [5 NO][NO][N]
+  __syncthreads();
+  contract<5 * NO * NO, N, 1, 1, NO, NT>(M, s2, out, tid);     // [5 NO NO][NO]
+  __syncthreads();
+}
+
+// per-field sums of a[g] * b[g] over the 5 x L3 modal entries, reduced over the workgroup
+template <int P>
+__device__ __forceinline__ void field_dots(const double (&part)[5], double *red, double (&out)[5], int tid) {
+  using S = C3<P>;
+  double v[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    double t = part[f];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    v[f] = t;
+  }
+  const int w = tid >> 6;
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) red[w * 8 + f] = v[f];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < S::NW; ++k) t += red[k * 8 + f];
+    out[f] = t;
+  }
+  __syncthreads();
+}
+
+// PROJ: project the nodal fields a.U [5][ne][Q^3] instead (the forward
+// transform of a curved element: x = M^-1 B^T (wJ u)).
+template <int P, bool PROJ>
+__global__ void __launch_bounds__(256)
+k_rhs3c(CurvedArgs a) {
+  using S = C3<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
+  constexpr int NT = S::NT, NPT = S::NPT, NML = S::NML;
+  extern __shared__ double sm[];
+  double *Un = sm, *Fw = Un + 5 * Q3, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *gF = S2 + S::SZ;
+  static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ, "scratch too small");
+  static_assert(S::lds <= 160 * 1024, "LDS");
+  double *Fc = gF + 30 * QF2, *red = Fc + 3 * Q3;
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const Geo &geo = a.geo;
+  const double *ge = a.gm + (size_t)e * 10 * Q3;   // wJ dxi_d/dx_x (9 rows), wJ
+  const double gm1 = a.phys.gamma - 1.0;
+
+  if (PROJ) {
+    for (int i = tid; i < 5 * Q3; i += NT) {
+      const int v = i / Q3, n = i - v * Q3;
+      Fw[i] = ge[9 * Q3 + n] * a.U[((size_t)v * ne + e) * Q3 + n];
+    }
+    __syncthreads();
+  } else {
+  // modal state -> nodal (5 fields)
+  for (int i = tid; i < 5 * L3; i += NT) {
+    const int v = i / L3;
+    S1[i] = a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
+  }
+  __syncthreads();
+  chain5<L, Q, NT>(CT<P>::BT(), S1, S2, Fw, Un, tid);
+
+  // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points
+  bool bad = false;
+  for (int i = tid; i < 6 * QF2; i += NT) {
+    const int s = i / QF2, q = i - s * QF2;
+    const int es = e * 6 + s;
+    const double *fg = a.fgeo + ((size_t)es * QF2 + q) * 4;
+    const double nx = fg[0], ny = fg[1], nz = fg[2], jr = fg[3];
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    double UL[5], UR[5], FL[5], Fh[5];
+    const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) UL[v] = tl[v * QF2 + q];
+    if (nb >= 0) {
+      const int qn = geo.perm[geo.permid[es] * QF2 + q];
+      const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) UR[v] = tr[v * QF2 + qn];
+    } else {
+      ghost3(-1 - nb, UL, nx, ny, nz, a.phys.bcU, UR);
+    }
+    riemann3(UL, UR, nx, ny, nz, a.phys.gamma, a.phys.flux, FL, Fh, bad);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) gF[(v * 6 + s) * QF2 + q] = jr * (Fh[v] - FL[v]);
+  }
+  if (bad) atomicOr(a.flag, 1);
+
+  // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by field
+  for (int v = 0; v < 5; ++v) {
+    __syncthreads();
+    for (int n = tid; n < Q3; n += NT) {
+      const double r = Un[n], mx = Un[Q3 + n], my = Un[2 * Q3 + n], mz = Un[3 * Q3 + n], E = Un[4 * Q3 + n];
+      const double ir = 1.0 / r, u = mx * ir, vv = my * ir, w = mz * ir;
+      const double p = gm1 * (E - 0.5 * (mx * u + my * vv + mz * w));
+      double fx, fy, fz;
+      if (v == 0) { fx = mx; fy = my; fz = mz; }
+      else if (v == 1) { fx = mx * u + p; fy = mx * vv; fz = mx * w; }
+      else if (v == 2) { fx = my * u; fy = my * vv + p; fz = my * w; }
+      else if (v == 3) { fx = mz * u; fy = mz * vv; fz = mz * w + p; }
+      else { const double h = E + p; fx = h * u; fy = h * vv; fz = h * w; }
+      Fc[n] = fx; Fc[Q3 + n] = fy; Fc[2 * Q3 + n] = fz;
+    }
+    __syncthreads();
+    const double *Dn = CT<P>::DN();
+#pragma unroll
+    for (int rr = 0; rr < NPT; ++rr) {
+      const int n = tid + rr * NT;
+      if (n < Q3) {
+        const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+        double acc = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const double *F = Fc + x * Q3;
+          double dr = 0.0, ds = 0.0, dt = 0.0;
+#pragma unroll
+          for (int m = 0; m < Q; ++m) {
+            dr = fma(Dn[i * Q + m], F[(m * Q + j) * Q + k], dr);
+            ds = fma(Dn[j * Q + m], F[(i * Q + m) * Q + k], ds);
+            dt = fma(Dn[k * Q + m], F[(i * Q + j) * Q + m], dt);
+          }
+          acc = fma(ge[x * Q3 + n], dr, acc);
+          acc = fma(ge[(3 + x) * Q3 + n], ds, acc);
+          acc = fma(ge[(6 + x) * Q3 + n], dt, acc);
+        }
+        Fw[v * Q3 + n] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  // faces: Gauss -> GLL face nodes for 30 blocks, then wJ / w_end at the face nodes
+  contract<30, QF, QF, 1, Q, NT>(CT<P>::GL(), gF, S1, tid);   // [30][Q][QF]
+  __syncthreads();
+  contract<30, Q, QF, 2, Q, NT>(CT<P>::GL(), S1, S2, tid);    // [30][Q][Q]
+  __syncthreads();
+  {
+    const double wend = CT<P>::BW()[0] / CT<P>::BT()[0];   // GLL end weight: phi_0 w_0 / phi_0
+#pragma unroll
+    for (int rr = 0; rr < NPT; ++rr) {
+      const int n = tid + rr * NT;
+      if (n < Q3) {
+        const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+        const double c = ge[9 * Q3 + n] / wend;
+        int fs[3], t1[3], t2[3], cnt = 0;
+        if (k == 0) { fs[cnt] = 0; t1[cnt] = i; t2[cnt++] = j; }
+        if (k == Q - 1) { fs[cnt] = 5; t1[cnt] = i; t2[cnt++] = j; }
+        if (j == 0) { fs[cnt] = 1; t1[cnt] = i; t2[cnt++] = k; }
+        if (j == Q - 1) { fs[cnt] = 3; t1[cnt] = i; t2[cnt++] = k; }
+        if (i == 0) { fs[cnt] = 4; t1[cnt] = j; t2[cnt++] = k; }
+        if (i == Q - 1) { fs[cnt] = 2; t1[cnt] = j; t2[cnt++] = k; }
+        for (int f = 0; f < cnt; ++f)
+#pragma unroll
+          for (int v = 0; v < 5; ++v)
+            Fw[v * Q3 + n] = fma(c, S2[((v * 6 + fs[f]) * Q + t1[f]) * Q + t2[f]], Fw[v * Q3 + n]);
+      }
+    }
+  }
+  __syncthreads();
+  }   // !PROJ
+  // b = B^T (wJ f): into Un (compact [5][L3])
+  chain5<Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Un, tid);
+
+  // conjugate gradients on M x = b, M = B^T diag(wJ) B, five fields at once;
+  // x0 = b / mean(J).  Pv, AP (compact [5][L3]) reuse Fw and Un.
+  double *Pv = Fw, *AP = Un;
+  double x[NML], res[NML], bvec[NML];
+  double part[5], bb[5], rr[5], pap[5];
+  const double jinv = 1.0 / a.jav[e];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) part[f] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT;
+    x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
+    if (g < 5 * L3) {
+      bvec[k] = Un[g];
+      x[k] = bvec[k] * jinv;
+      part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT;
+    if (g < 5 * L3) Pv[g] = x[k];
+  }
+  field_dots<P>(part, red, bb, tid);
+  // AP = M Pv
+  auto mass = [&]() {
+    contract<5, L, L * L, 1, Q, NT>(CT<P>::BT(), Pv, S1, tid);        // [5][Q][L L]
+    __syncthreads();
+    contract<5 * Q, L, L, 1, Q, NT>(CT<P>::BT(), S1, S2, tid);        // [5 Q][Q][L]
+    __syncthreads();
+    contract<5 * Q * Q, L, 1, 1, Q, NT>(CT<P>::BT(), S2, S1, tid);    // nodal [5][Q3]
+    __syncthreads();
+    for (int i = tid; i < 5 * Q3; i += NT) S1[i] *= ge[9 * Q3 + (i % Q3)];
+    __syncthreads();
+    contract<5, Q, Q * Q, 1, L, NT>(CT<P>::BL(), S1, S2, tid);        // [5][L][Q Q]
+    __syncthreads();
+    contract<5 * L, Q, Q, 1, L, NT>(CT<P>::BL(), S2, S1, tid);        // [5 L][L][Q]
+    __syncthreads();
+    contract<5 * L * L, Q, 1, 1, L, NT>(CT<P>::BL(), S1, AP, tid);    // [5][L3]
+    __syncthreads();
+  };
+  mass();
+  // r0 = b - M x0, p0 = r0
+#pragma unroll
+  for (int f = 0; f < 5; ++f) part[f] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT;
+    if (g < 5 * L3) {
+      res[k] = bvec[k] - AP[g];
+      part[g / L3] = fma(res[k], res[k], part[g / L3]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT;
+    if (g < 5 * L3) Pv[g] = res[k];
+  }
+  field_dots<P>(part, red, rr, tid);
+  constexpr double tol2 = 1e-30;   // (1e-15 relative)^2
+  int it = 0;
+  for (; it < 60; ++it) {
+    bool conv = true;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) conv = conv && (rr[f] <= tol2 * bb[f]);
+    if (conv) break;
+    mass();
+#pragma unroll
+    for (int f = 0; f < 5; ++f) part[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      if (g < 5 * L3) part[g / L3] = fma(Pv[g], AP[g], part[g / L3]);
+    }
+    field_dots<P>(part, red, pap, tid);
+    double alpha[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) alpha[f] = (rr[f] > 0.0 && pap[f] > 0.0) ? rr[f] / pap[f] : 0.0;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) part[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      if (g < 5 * L3) {
+        const int f = g / L3;
+        x[k] = fma(alpha[f], Pv[g], x[k]);
+        res[k] = fma(-alpha[f], AP[g], res[k]);
+        part[f] = fma(res[k], res[k], part[f]);
+      }
+    }
+    double rn[5];
+    field_dots<P>(part, red, rn, tid);
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      if (g < 5 * L3) {
+        const int f = g / L3;
+        const double beta = rr[f] > 0.0 ? rn[f] / rr[f] : 0.0;
+        Pv[g] = fma(beta, Pv[g], res[k]);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rr[f] = rn[f];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT;
+    if (g < 5 * L3) {
+      const int v = g / L3;
+      a.R[((size_t)v * ne + e) * L3 + (g - v * L3)] = x[k];
+    }
+  }
+  if (tid == 0 && a.iters) atomicMax(a.iters, it);
+}
+
+template <int P>
+static hipError_t launch_c3(const CurvedArgs &a) {
+  const int sh = C3<P>::lds;
+  if (a.project) {
+    (void)hipFuncSetAttribute((const void *)k_rhs3c<P, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+    hipLaunchKernelGGL((k_rhs3c<P, true>), dim3(a.ne), dim3(C3<P>::NT), sh, a.stream, a);
+  } else {
+    (void)hipFuncSetAttribute((const void *)k_rhs3c<P, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+    hipLaunchKernelGGL((k_rhs3c<P, false>), dim3(a.ne), dim3(C3<P>::NT), sh, a.stream, a);
+  }
+  return hipGetLastError();
+}
+
+bool curved3_supported(int p) { return p >= 1 && p <= 6; }
+
+hipError_t launch_curved3(const CurvedArgs &a) {
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch_c3<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5) DGX_CASE(6)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t upload_tables_c3d(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+}
+
+}  // namespace dgx

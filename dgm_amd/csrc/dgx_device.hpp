@@ -22,13 +22,13 @@ struct Tab {
 // The same tables for every supported order p = 1..10 in constant memory, so
 // that uniform coefficient reads become scalar loads.  Layout per order:
 // BT, BW, QD, Bg, Bend, PW, Ig (sizes above), then PI = PW Ig [L][Q] (GLL
-// face layer -> modal face coefficients) and KE [2][Q] (KE[s][i] =
+// face layer -> modal face coefficients), KE [2][Q] (KE[s][i] =
 // sum_l Bend[s][l] BT[i][l], the end traces sum_l phi_l(-+1) phi_l(xi_i)),
-// orders packed in ascending p.
+// BL [L][Q], GL [Q][QF] and DN [Q][Q] (see CT), orders packed in ascending p.
 constexpr int kMaxOrder = 10;
 constexpr int tab_size(int p) {
   return 3 * (p + 2) * (p + 1) + 2 * (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1) * (p + 2) +
-         (p + 1) * (p + 2) + 2 * (p + 2);
+         (p + 1) * (p + 2) + 2 * (p + 2) + (p + 1) * (p + 2) + (p + 2) * (p + 1) + (p + 2) * (p + 2);
 }
 constexpr int tab_offset(int p) {
   int s = 0;
@@ -51,6 +51,11 @@ template <int P> struct CT {
   __device__ static __forceinline__ const double *Ig() { return c_tab + b + 3 * Q * L + 2 * QF * L + 2 * L; }
   __device__ static __forceinline__ const double *PI() { return Ig() + QF * Q; }
   __device__ static __forceinline__ const double *KE() { return PI() + L * Q; }
+  // curved elements: BL [L][Q] = phi_l(xi_i) (unweighted projection), GL [Q][QF]
+  // Gauss -> GLL interpolation (= BT PW), DN [Q][Q] GLL collocation derivative
+  __device__ static __forceinline__ const double *BL() { return KE() + 2 * Q; }
+  __device__ static __forceinline__ const double *GL() { return BL() + L * Q; }
+  __device__ static __forceinline__ const double *DN() { return GL() + Q * QF; }
 };
 
 // Per-element affine geometry and face links (device memory).

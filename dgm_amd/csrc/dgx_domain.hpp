@@ -26,7 +26,7 @@ inline constexpr double kLsrkC[14] = {0.0, 0.0367762454319673, 0.124968526272502
 // Basis tables of one order (host copies).
 struct HostBasis {
   int p, L, Q, QF;
-  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE;
+  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE, BL, GL;
 };
 
 }  // namespace dgx
@@ -50,6 +50,9 @@ struct dgx_domain {
   double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr, *T2 = nullptr;
   float *Vf = nullptr, *TVf = nullptr;   // viscous terms of the mixed path
   bool fused = false;   // hex fused stage kernels available for this order
+  bool curved = false;  // curved hexes (dgx_curved3d.hip): nodal metrics, mass-matrix projection
+  double *d_gm = nullptr, *d_fgeo = nullptr, *d_jav = nullptr;
+  int *d_iters = nullptr;
   dgx::Tab tab;
   dgx::Geo geo;
   dgx::Phys phys;

@@ -64,6 +64,25 @@ hipError_t launch_stage3(const StageLaunch &s);
 bool mixed3_supported(int p);
 hipError_t launch_mixed3(const StageLaunch &s);
 
+// Curved hexes, Euler3d, p <= 6 (dgx_curved3d.hip): residual R = -dU/dt with
+// metrics at the nodes and the element mass matrix.
+struct CurvedArgs {
+  const double *U = nullptr, *T = nullptr;   // modal state and its face traces
+  double *R = nullptr;                       // modal residual
+  const double *gm = nullptr;                // [ne][10][Q^3]: wJ dxi_d/dx_x (d-major), wJ
+  const double *fgeo = nullptr;              // [ne][6][QF^2][4]: normal, J_side/J_vol
+  const double *jav = nullptr;               // [ne]: mean J (first guess of the mass solve)
+  Geo geo;
+  Phys phys;
+  int ne = 0, p = 0;
+  int *flag = nullptr, *iters = nullptr;
+  bool project = false;                      // U nodal [5][ne][Q^3] -> R modal: M^-1 B^T (wJ U)
+  hipStream_t stream = nullptr;
+};
+bool curved3_supported(int p);
+hipError_t launch_curved3(const CurvedArgs &a);
+hipError_t upload_tables_c3d(const double *h);
+
 // Linear hyperbolic system on quads, dU/dt + Ax dU/dx + Ay dU/dy = s (the
 // linearized Euler equations about a uniform mean flow), fused with one
 // LSRK14 stage; or, with the adjoint system (-Ax^T, -Ay^T), one reversed

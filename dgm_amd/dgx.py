@@ -29,7 +29,8 @@ class DomainDesc(ctypes.Structure):
                 ("eqntype", ctypes.c_int), ("flux", ctypes.c_int), ("precision", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("vis", ctypes.c_double), ("Pr", ctypes.c_double),
                 ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip),
-                ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5)]
+                ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5),
+                ("xnodes", _dp)]
 
 
 class DgxError(RuntimeError):
@@ -90,7 +91,8 @@ class GPUDomain:
     """A DG domain on the GPU: modal FP64 state, residual and RK stepping."""
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
-                 precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None):
+                 precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None,
+                 xnodes=None):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -99,7 +101,8 @@ class GPUDomain:
         desc = DomainDesc(self.nsd, int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
                           float(gamma), float(vis), float(Pr), _ptr(self._vert), len(self._con),
                           self._con.ctypes.data_as(_ip), int(viscous), float(br2_eta),
-                          (ctypes.c_double * 5)(*([0.0] * 5 if bc_state is None else [float(v) for v in bc_state])))
+                          (ctypes.c_double * 5)(*([0.0] * 5 if bc_state is None else [float(v) for v in bc_state])),
+                          None if xnodes is None else _ptr(self._nodes(xnodes, mesh, p)))
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))
@@ -107,6 +110,13 @@ class GPUDomain:
         _check(lib.dgx_sizes(self.h, ctypes.byref(q), ctypes.byref(m), ctypes.byref(f), ctypes.byref(ne)))
         self.qtot, self.nm, self.nf, self.ne = q.value, m.value, f.value, ne.value
         self.p = int(p)
+
+    def _nodes(self, xnodes, mesh, p):
+        q = int(p) + 2
+        self._xn = np.ascontiguousarray(xnodes, dtype=np.float64)
+        if self._xn.shape != (3, int(mesh.ne), q ** 3):
+            raise ValueError(f"xnodes shape {self._xn.shape} != {(3, int(mesh.ne), q ** 3)}")
+        return self._xn
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -100,6 +100,8 @@ typedef struct {
   int viscous;      /* enum dgx_viscous (Navier-Stokes; 0 = BR1) */
   double br2_eta;   /* BR2 penalty; 0 selects 6, the number of hex faces */
   double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries */
+  const double *xnodes;  /* curved hexes (Euler3d, FP64, p <= 6): coordinates [3][ne][q^3] of
+                            every element's GLL nodes, node (i*q + j)*q + k; NULL: affine */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

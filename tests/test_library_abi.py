@@ -73,6 +73,9 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
     (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler3d"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
+    (dict(p=3, eqn=9, curved=True), b"curved elements"),
+    (dict(p=7, eqn=8, curved=True), b"curved elements"),
+    (dict(p=3, eqn=8, precision=1, curved=True), b"curved elements"),
 ])
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
@@ -81,6 +84,10 @@ def test_create_rejects_bad_arguments(lib, case, msg):
     d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0),
                     precision=case.get("precision", 0), viscous=case.get("viscous", 0),
                     br2_eta=case.get("br2_eta", 0.0))
+    if case.get("curved"):
+        xn = np.ascontiguousarray(meshgen.hex_nodes(m, case.get("p", 3)))
+        keep = keep + (xn,)
+        d.xnodes = xn.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     h = ctypes.c_void_p()
     rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
     assert rc == -1
