@@ -227,6 +227,56 @@ def adjoint_sample(args, seed=4096):
             "fd_rel_err": [float(e) for e in errs]}
 
 
+def curved_sample(args, seed=5):
+    """The curved-bump workload: 3D Euler in an n^3 hex channel whose bottom wall
+    carries a smooth bump of amplitude 0.05 (curved elements throughout,
+    metrics at the nodes), p = 5, slip wall on the bump, the free stream imposed
+    on the top boundary, periodic sides; free stream at Mach 0.5 plus seeded
+    density noise of 1e-3.  FP64, LSRK14; one warmup step, then `cv_steps`
+    timed steps."""
+    n, p = args.cv_n, 5
+    lo, hi = (0.0, 0.0, 0.0), (3.0, 3.0, 1.0)
+    m = meshgen.channel_hexes(n, n, n, lo, hi, bc_bottom=dgx.DGX_BC_WALL, bc_top=dgx.DGX_BC_STATE)
+    t0 = time.perf_counter()
+    xn = meshgen.mapped_nodes(m, p, meshgen.bump_map(lo, hi, 0.05))
+    rho0, u0, p0 = 1.0, 0.5, 1.0 / 1.4
+    E0 = p0 / 0.4 + 0.5 * rho0 * u0 * u0
+    rng = np.random.default_rng(seed)
+    U = np.empty((5,) + xn.shape[1:])
+    U[0] = rho0 + 1e-3 * rng.standard_normal(xn.shape[1:])
+    U[1] = rho0 * u0
+    U[2] = 0.0
+    U[3] = 0.0
+    U[4] = E0
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_EULER3D, gamma=1.4, bc_state=[rho0, rho0 * u0, 0.0, 0.0, E0], xnodes=xn)
+    del xn
+    gpu.upload(U)
+    del U
+    gpu.synchronize()
+    t_setup = time.perf_counter() - t0
+    dt = 0.3 * (hi[2] / n) / ((p + 1) ** 2 * (u0 + 1.0))
+    dofs = 5 * m.ne * (p + 1) ** 3
+    print(f"[bench] curved: {n}^3 p={p}: {dofs / 1e6:.1f} M DOF, setup {t_setup:.1f} s", file=sys.stderr, flush=True)
+    gpu.advance(dgx.DGX_LSRK14, dt, 1)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.advance(dgx.DGX_LSRK14, dt, args.cv_steps)
+    gpu.synchronize()
+    el = time.perf_counter() - t0
+    finite = bool(np.all(np.isfinite(gpu.norms())))
+    gpu.close()
+    value = dofs * 14 * args.cv_steps / el
+    # HBM bytes per DOF-stage: 32 for the 2N state streams, plus the per-node
+    # metrics read once per stage (10 doubles per node over the 5 fields' modes)
+    metric_b = 10 * 8 * (p + 2) ** 3 / (5 * (p + 1) ** 3)
+    return {"workload": f"3D Euler, {n}^3 curved hexes (bump of amplitude 0.05 on a slip wall, free stream "
+                        f"on top, periodic sides), p={p}, Mach 0.5 + 1e-3 seeded density noise, LSRK14, FP64",
+            "value": value, "unit": "DOF-stage/s", "dofs": dofs, "steps": args.cv_steps, "dt": dt,
+            "ms_per_step": el / args.cv_steps * 1e3, "setup_s": t_setup, "state_finite": finite,
+            "hbm_gbs_at_algorithmic_bytes": value * (32.0 + metric_b) / 1e9,
+            "algorithmic_bytes_per_dof_stage": 32.0 + metric_b}
+
+
 def run_path(m, n, p, mu, precision, args):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
     gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.71, precision=precision,
@@ -319,6 +369,9 @@ def main():
     ap.add_argument("--adj-n", type=int, default=64, help="quads per direction of the adjoint sample")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--adj-dirs", type=int, default=8, help="finite-difference directions")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curved-bump sample")
+    ap.add_argument("--cv-n", type=int, default=96, help="hexes per direction of the curved sample")
+    ap.add_argument("--cv-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-nmax", type=int, default=64)
     ap.add_argument("--serve-steps", type=int, default=200)
@@ -395,6 +448,8 @@ def main():
         line["serving"] = serving_sample(args)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_sample(args)
+    if not args.no_curved:
+        line["curved"] = curved_sample(args)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -328,6 +330,8 @@ int curved_geometry(dgx_domain *d, const double *xn) {
   const HostBasis &b = d->hb;
   const std::vector<double> &D = b.D, &w = b.w, &Ig = b.Ig;
   std::vector<double> gm((size_t)ne * 10 * Q3), fgeo((size_t)ne * 6 * QF2 * 4), jav(ne);
+  std::atomic<int> bad_elem(-1);
+  auto work = [&](int e0, int e1) {
   std::vector<double> der(9 * (size_t)Q3), mn(9 * (size_t)Q3), J(Q3), wJ(Q3);
   auto face = [&](const double *f, int s, double *to) {   // GLL face values [Q][Q] (get_side)
     for (int u = 0; u < Q; ++u)
@@ -360,7 +364,7 @@ int curved_geometry(dgx_domain *d, const double *xn) {
       }
   };
   std::vector<double> fv(Q2), fg(QF2), X(Q2), Y(Q2), Z(Q2), nrm(3 * QF2), jf(QF2), cr(Q2);
-  for (int e = 0; e < ne; ++e) {
+  for (int e = e0; e < e1; ++e) {
     const double *C[3] = {xn + ((size_t)0 * ne + e) * Q3, xn + ((size_t)1 * ne + e) * Q3,
                           xn + ((size_t)2 * ne + e) * Q3};
     for (int c = 0; c < 3; ++c)
@@ -382,8 +386,7 @@ int curved_geometry(dgx_domain *d, const double *xn) {
       const double yr = der[3 * Q3 + n], ys = der[4 * Q3 + n], yt = der[5 * Q3 + n];
       const double zr = der[6 * Q3 + n], zs = der[7 * Q3 + n], zt = der[8 * Q3 + n];
       const double Jn = xr * ys * zt + yr * zs * xt + zr * xs * yt - xt * ys * zr - zt * xs * yr - xr * zs * yt;
-      if (!(Jn > 0.0))
-        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has a nonpositive Jacobian");
+      if (!(Jn > 0.0)) { int none = -1; bad_elem.compare_exchange_strong(none, e); return; }
       J[n] = Jn;
       mn[0 * Q3 + n] = (ys * zt - yt * zs) / Jn;
       mn[1 * Q3 + n] = -(xs * zt - xt * zs) / Jn;
@@ -432,6 +435,17 @@ int curved_geometry(dgx_domain *d, const double *xn) {
       }
     }
   }
+  };
+  {
+    const int nth = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    const int chunk = (ne + nth - 1) / nth;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth && t * chunk < ne; ++t) pool.emplace_back(work, t * chunk, std::min(ne, (t + 1) * chunk));
+    for (auto &th : pool) th.join();
+  }
+  if (bad_elem.load() >= 0)
+    return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(bad_elem.load()) + " has a nonpositive Jacobian");
+
   int rc;
   if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
     return rc;
