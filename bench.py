@@ -140,24 +140,33 @@ def cpu_baseline(p, mu, seconds=15.0, workers=None, flux=0, viscous=0):
 
 def serving_sample(args, seed=1234):
     """A serving mix shaped like the ensemble workload: Poisson arrivals of 2D
-    Euler forward solves (freestream plus 8 seeded Fourier modes of amplitude
-    0.05), n x n periodic quads with n log-uniform in [8, nmax], p uniform in
-    {2..8}, 200 LSRK14 steps each, FP64, continuous batching on one GPU."""
+    forward solves, each shallow water about rest (phi = 1) or Euler about a
+    freestream (seeded choice), plus 8 seeded Fourier modes of amplitude 0.05;
+    n x n periodic quads with n log-uniform in [8, nmax], p uniform in {2..8},
+    200 LSRK14 steps each, FP64, continuous batching on one GPU.  The
+    shallow-water requests leave out the balance term, which would hold them
+    at their initial state."""
     from dgm_amd import serve
     from dgm_amd.mesh import quad_nodes
     rng = np.random.default_rng(seed)
     jobs, dofs = [], []
-    base = [1.0, 0.5, 0.0, (1.0 / 1.4) / 0.4 + 0.125]
+    base = {dgx.DGX_EULER: [1.0, 0.5, 0.0, (1.0 / 1.4) / 0.4 + 0.125], dgx.DGX_SHALLOW: [1.0, 0.0, 0.0]}
+    speed = {dgx.DGX_EULER: 0.5 + np.sqrt(1.4 * (1 / 1.4)), dgx.DGX_SHALLOW: 1.0}
+    nshallow = 0
     for _ in range(args.serve_requests):
         nn = int(round(np.exp(rng.uniform(np.log(8), np.log(args.serve_nmax)))))
         pp = int(rng.integers(2, 9))
+        eqn = dgx.DGX_SHALLOW if rng.integers(2) == 0 else dgx.DGX_EULER
+        nshallow += eqn == dgx.DGX_SHALLOW
         m = meshgen.periodic_quads(nn, nn, 0.0, 10.0, 0.0, 10.0)
         x = quad_nodes(m, pp)   # nodal GLL coordinates without a device domain
-        U = np.stack([b + ic.fourier_modes(x, (10.0, 10.0), 8, 0.05, rng) for b in base])
-        dt = 0.2 * (10.0 / nn) / ((pp + 1) ** 2 * (0.5 + np.sqrt(1.4 * (1 / 1.4))))
-        jobs.append(dict(mesh=m, p=pp, eqntype=dgx.DGX_EULER, U0=U, inttype=dgx.DGX_LSRK14, dt=dt,
-                         nsteps=args.serve_steps))
-        dofs.append(4 * nn * nn * (pp + 1) ** 2)
+        U = np.stack([b + ic.fourier_modes(x, (10.0, 10.0), 8, 0.05, rng) for b in base[eqn]])
+        dt = 0.2 * (10.0 / nn) / ((pp + 1) ** 2 * speed[eqn])
+        job = dict(mesh=m, p=pp, eqntype=eqn, U0=U, inttype=dgx.DGX_LSRK14, dt=dt, nsteps=args.serve_steps)
+        if eqn == dgx.DGX_SHALLOW:
+            job["balance"] = False
+        jobs.append(job)
+        dofs.append(len(base[eqn]) * nn * nn * (pp + 1) ** 2)
     srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active,
                        steps_per_slice=args.serve_slice)
     arrivals = serve.poisson_arrivals(len(jobs), rate=args.serve_rate, rng=rng)
@@ -177,7 +186,7 @@ def serving_sample(args, seed=1234):
             "mean_queue_wait_ms": float(np.mean(wait) * 1e3),
             "throughput_dof_stage_per_s": float(sum(dofs) * 14 * args.serve_steps / wall),
             "makespan_s": wall, "peak_active": peak_active, "peak_device_bytes": peak_bytes,
-            "sample": f"{len(jobs)} 2D Euler solves, n log-uniform in [8,{args.serve_nmax}], p in 2..8, "
+            "sample": f"{len(jobs)} 2D solves ({nshallow} shallow water, {len(jobs) - nshallow} Euler), n log-uniform in [8,{args.serve_nmax}], p in 2..8, "
                       f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, FP64, "
                       f"{args.serve_active} slots, slices of {args.serve_slice} steps replayed from hipGraphs"}
 

@@ -237,8 +237,10 @@ int launch(dgx_domain *d, int op, const double *U, double *out) {
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf; a.ns = d->visc;
   a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
   a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
+  a.eqn = d->eqn; a.Bal = d->Bal; a.bal = op == dgx::OP_RHS ? d->bal : 0;
   hipError_t e = timed(d, op, [&] { return (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  if (op == dgx::OP_RHS && d->bal == 1) d->bal = 2;   // the first residual is stored
   return DGX_OK;
 }
 
@@ -463,7 +465,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   const int nsd = desc->nsd, ne = desc->ne, p = desc->p;
   d->nsd = nsd; d->ne = ne; d->p = p;
   d->L = p + 1; d->Q = p + 2; d->QF = p + 1;
-  d->nf = nsd + 2;
+  d->nf = desc->eqntype == DGX_SHALLOW ? 3 : nsd + 2;
   d->nm = nsd == 2 ? d->L * d->L : d->L * d->L * d->L;
   d->nq = nsd == 2 ? d->Q * d->Q : d->Q * d->Q * d->Q;
   d->ns = nsd == 2 ? 4 : 6;
@@ -574,8 +576,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     if (e >= 0 && e < ne && s >= 0 && s < d->ns && le == -1) {   // boundary side (e, s, -1, bc type)
       if (ls != DGX_BC_WALL && ls != DGX_BC_STATE)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
-      if (desc->eqntype != DGX_EULER3D)
-        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d");
+      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
       sides[(size_t)e * d->ns + s].bc = ls;
       continue;
     }
@@ -711,6 +713,10 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   HIPCHK(hipMemset(d->R0, 0, n * 8));
   HIPCHK(hipMemset(d->R1, 0, n * 8));
   if ((rc = alloc(&d->T, (size_t)ne * d->ns * d->nf * qs))) return rc;
+  if (d->eqn == DGX_SHALLOW && !desc->no_balance) {
+    if ((rc = alloc(&d->Bal, n))) return rc;
+    d->bal = 1;
+  }
   if (desc->xnodes && (rc = curved_geometry(d, desc->xnodes))) return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
@@ -734,7 +740,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters};
+                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -752,7 +758,8 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   if ((desc->nsd != 2 && desc->nsd != 3) || desc->p < 1 || desc->p > 10 || desc->ne <= 0)
     return fail(DGX_ERR_ARGUMENT, "bad domain description");
   const size_t nsd = desc->nsd, ne = desc->ne, L = desc->p + 1, Q = desc->p + 2, QF = desc->p + 1;
-  const size_t nf = nsd + 2, nm = nsd == 2 ? L * L : L * L * L, nq = nsd == 2 ? Q * Q : Q * Q * Q;
+  const bool shallow = desc->eqntype == DGX_SHALLOW;
+  const size_t nf = shallow ? 3 : nsd + 2, nm = nsd == 2 ? L * L : L * L * L, nq = nsd == 2 ? Q * Q : Q * Q * Q;
   const size_t ns = nsd == 2 ? 4 : 6, qs = nsd == 2 ? QF : QF * QF;
   const bool visc = desc->eqntype == DGX_NAVIER_STOKES || desc->eqntype == DGX_NAVIER_STOKES3D;
   const bool fused = nsd == 3 && dgx::stage3_supported(desc->p);
@@ -768,6 +775,7 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
   b += 8 * qs * 2 + (size_t)(1 << 20);                      // orientation tables, tables, rounding
   if (desc->xnodes) b += ne * (10 * nq + 24 * qs + 1) * 8;   // curved: nodal metrics, face geometry
+  if (shallow && !desc->no_balance) b += nf * ne * nm * 8;    // balance term
   *bytes = b;
   return DGX_OK;
 }
@@ -778,10 +786,13 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
   if (desc->nsd != 2 && desc->nsd != 3) return fail(DGX_ERR_ARGUMENT, "nsd must be 2 or 3");
   if (desc->p < 1 || desc->p > 10) return fail(DGX_ERR_ARGUMENT, "p must be in [1, 10]");
   if (desc->ne <= 0 || !desc->vert || !desc->con) return fail(DGX_ERR_ARGUMENT, "empty mesh");
-  const bool euler = desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES;
+  const bool euler = desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
+                     desc->eqntype == DGX_SHALLOW;
   const bool euler3 = desc->eqntype == DGX_EULER3D || desc->eqntype == DGX_NAVIER_STOKES3D;
   if ((desc->nsd == 2 && !euler) || (desc->nsd == 3 && !euler3))
     return fail(DGX_ERR_ARGUMENT, "eqntype does not match nsd");
+  if (desc->eqntype == DGX_SHALLOW && desc->flux != DGX_FLUX_LAX_FRIEDRICHS)
+    return fail(DGX_ERR_ARGUMENT, "Shallow has the Lax-Friedrichs flux only");
   if (desc->flux != DGX_FLUX_LAX_FRIEDRICHS && desc->flux != DGX_FLUX_ROE)
     return fail(DGX_ERR_ARGUMENT, "flux must be DGX_FLUX_LAX_FRIEDRICHS or DGX_FLUX_ROE");
   if (desc->precision != DGX_FP64 && desc->precision != DGX_MIXED)
@@ -799,7 +810,8 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "curved elements (xnodes) are available for Euler3d, DGX_FP64, p <= 6");
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
-      if (desc->eqntype != DGX_EULER3D) return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d");
+      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
       const int t = desc->con[4 * r + 3];
       if (t != DGX_BC_WALL && t != DGX_BC_STATE)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");

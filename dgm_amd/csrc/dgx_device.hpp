@@ -77,6 +77,36 @@ struct Phys {
   double bcU[5];                 // conservative state of state boundaries
 };
 
+// Shallow water in (phi, uphi, vphi) (src/Shallow.cpp:357-525): Cartesian
+// normal flux of one state and its largest wave speed |u_n| + sqrt(phi);
+// returns true when phi or phi^2/2 is below aepsM = eps^(2/3), where the
+// reference stops.
+__device__ __forceinline__ bool shallow_normal_flux2(const double U[3], double nx, double ny, double F[3],
+                                                     double &lam) {
+  constexpr double aepsM = 3.666852862501036e-11;
+  const double phi = U[0], mn = U[1] * nx + U[2] * ny, mt = U[2] * nx - U[1] * ny;
+  const double un = (1.0 / phi) * mn, p = 0.5 * phi * phi;
+  F[0] = mn;
+  F[1] = (mn * un + p) * nx - (mt * un) * ny;
+  F[2] = (mn * un + p) * ny + (mt * un) * nx;
+  lam = fabs(un) + sqrt(phi);
+  return !(phi >= aepsM) || !(p >= aepsM);
+}
+
+// Shallow-water ghost values: Shallow::WallBC::apply (src/ShallowBC.hpp:45-62)
+// keeps phi and the tangential momentum; state boundaries hold bcU[0..2].
+__device__ __forceinline__ void ghost_shallow2(int bc, const double UL[3], double nx, double ny,
+                                               const double bcU[5], double UB[3]) {
+#pragma clang fp contract(off)
+  if (bc == 1) {
+    UB[0] = bcU[0]; UB[1] = bcU[1]; UB[2] = bcU[2];
+    return;
+  }
+  UB[0] = UL[0];
+  UB[1] = (UL[1] * ny - UL[2] * nx) * ny;
+  UB[2] = (UL[2] * nx - UL[1] * ny) * nx;
+}
+
 // Ghost values of a boundary side, as Domain::enforce_bc fills them
 // (src/Domain.cpp:1346-1350): bc 0 slip wall (Euler3d::WallBC::apply,
 // src/Euler3d.cpp:876-909: the normal momentum removed, the internal energy

@@ -53,6 +53,8 @@ struct dgx_domain {
   bool curved = false;  // curved hexes (dgx_curved3d.hip): nodal metrics, mass-matrix projection
   double *d_gm = nullptr, *d_fgeo = nullptr, *d_jav = nullptr;
   int *d_iters = nullptr;
+  double *Bal = nullptr;   // shallow water: the first residual (balance term)
+  int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
   dgx::Tab tab;
   dgx::Geo geo;
   dgx::Phys phys;

@@ -24,6 +24,9 @@ struct LaunchArgs {
   const double *X = nullptr;
   double *Y = nullptr;
   int *flag = nullptr;
+  int eqn = 0;                   // eqntype (shallow water: 30)
+  double *Bal = nullptr;         // shallow water: the stored first residual
+  int bal = 0;                   // 0 no balance term, 1 store the residual in Bal, 2 subtract Bal
   Tab tab;
   Geo geo;
   Phys phys;

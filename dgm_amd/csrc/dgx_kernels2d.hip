@@ -5,6 +5,7 @@
 //   numerical flux   Euler::convective_flux (LF, Roe) src/Euler.cpp:399-757
 //   lift + project   Quad::lift, Element::project     src/Quad.cpp:1971-2038, src/Element.hpp:427-437
 //   BR1 viscous      Navier_Stokes::diffusion         src/Navier_Stokes.cpp:113-258
+//   shallow water    Shallow::convection, LF flux     src/Shallow.cpp:251-525
 //   linear system    LinEuler_quasi::convection       src/LinEuler_quasi.cpp:151-495
 //                    and its discrete adjoint
 #include "dgx_kernels.hpp"
@@ -154,6 +155,93 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
       double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
       if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
       R[((size_t)v * ne + e) * L2 + k] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// Shallow water (Shallow::time_derivative, src/Shallow.cpp:255-336): flux
+// divergence at the GLL points, Lax-Friedrichs face flux, lift, projection.
+// The balance term (src/Shallow.cpp:251-266): bal 1 stores this residual in
+// Bal and returns R - Bal = 0, bal 2 returns R - Bal.
+template <int P>
+__global__ void __launch_bounds__(64)
+k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ R,
+           double *__restrict__ Bal, int bal, Tab tb, Geo geo, Phys ph, int ne, int *__restrict__ flag) {
+  using S = H2<P>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
+  __shared__ double Un[3 * Q2], gF[12 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int v = 0; v < 3; ++v) {
+    copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
+    __syncthreads();
+    backward2<P>(tb, Rv, X1, Un + v * Q2, tid);
+  }
+  bool bad = false;
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, a = i - s * QF;
+    const int es = e * 4 + s;
+    const int nb = geo.nbr[es];
+    const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1];
+    const double jr = geo.fjr[es];
+    double UL[3], UR[3], FL[3], FR[3], lL, lR;
+    const double *tl = T + ((size_t)e * 4 + s) * 3 * QF;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) UL[v] = tl[v * QF + a];
+    if (nb < 0) {
+      ghost_shallow2(-1 - nb, UL, nx, ny, ph.bcU, UR);
+    } else {
+      const int an = geo.perm[geo.permid[es] * QF + a];
+      const double *tr = T + ((size_t)nb * 4 + geo.nside[es]) * 3 * QF;
+#pragma unroll
+      for (int v = 0; v < 3; ++v) UR[v] = tr[v * QF + an];
+    }
+    bad |= shallow_normal_flux2(UL, nx, ny, FL, lL);
+    bad |= shallow_normal_flux2(UR, nx, ny, FR, lR);
+    const double lmax = fmax(lL, lR);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+      gF[(v * 4 + s) * QF + a] = jr * (Fh - FL[v]);
+    }
+  }
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  const double *mt = geo.metric + 9 * (size_t)e;
+  const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int v = 0; v < 3; ++v) {
+    for (int k = tid; k < Q2; k += NT) {
+      const double phi = Un[k], mx = Un[Q2 + k], my = Un[2 * Q2 + k];
+      const double u = mx / phi, vv = my / phi, p = 0.5 * (phi * phi);
+      double fx, fy;
+      if (v == 0) { fx = mx; fy = my; }
+      else if (v == 1) { fx = mx * u + p; fy = mx * vv; }
+      else { fx = my * u; fy = my * vv + p; }
+      Fr[k] = rx * fx + ry * fy;
+      Fs[k] = sx * fx + sy * fy;
+    }
+    __syncthreads();
+    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
+    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
+    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
+    __syncthreads();
+    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
+    __syncthreads();
+    for (int k = tid; k < L2; k += NT) {
+      const int l = k / L, m = k - l * L;
+      const double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
+      const size_t o = ((size_t)v * ne + e) * L2 + k;
+      if (bal == 0) {
+        R[o] = r;
+      } else {
+        double b;
+        if (bal == 1) { b = r; Bal[o] = r; }
+        else b = Bal[o];
+        R[o] = -1.0 * b + r;
+      }
     }
     __syncthreads();
   }
@@ -528,10 +616,13 @@ static hipError_t launch2(const LaunchArgs &a) {
   const int ne = a.ne;
   switch (a.op) {
     case OP_TRACES:
-      hipLaunchKernelGGL(k_traces2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, 4);
+      hipLaunchKernelGGL(k_traces2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, a.nfield);
       break;
     case OP_RHS:
-      if (a.ns)
+      if (a.eqn == 30)
+        hipLaunchKernelGGL(k_shallow2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.R, a.Bal, a.bal,
+                           a.tab, a.geo, a.phys, ne, a.flag);
+      else if (a.ns)
         hipLaunchKernelGGL((k_rhs2<P, true>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.V, a.TV,
                            a.R, a.tab, a.geo, a.phys, ne, a.flag);
       else

@@ -35,7 +35,7 @@ int sfail(int code, const std::string &msg) {
 struct Request {
   long long id = 0;
   dgx_domain_desc desc{};
-  std::vector<double> vert, U0, result;
+  std::vector<double> vert, xnodes, U0, result;
   std::vector<int> con;
   int layout = DGX_NODAL, inttype = DGX_LSRK14, nsteps = 0;
   double dt = 0.0;
@@ -188,7 +188,11 @@ int dgx_server_submit(dgx_server_handle s, const dgx_domain_desc *desc, const do
   r->desc = *desc;
   r->desc.vert = r->vert.data();
   r->desc.con = r->con.data();
-  const size_t L = desc->p + 1, Q = desc->p + 2, nf = desc->nsd + 2;
+  const size_t L = desc->p + 1, Q = desc->p + 2, nf = desc->eqntype == DGX_SHALLOW ? 3 : desc->nsd + 2;
+  if (desc->xnodes) {   // curved hexes: the nodes are kept with the request like the mesh
+    r->xnodes.assign(desc->xnodes, desc->xnodes + 3 * (size_t)desc->ne * Q * Q * Q);
+    r->desc.xnodes = r->xnodes.data();
+  }
   const size_t per = desc->nsd == 2 ? (layout == DGX_NODAL ? Q * Q : L * L)
                                     : (layout == DGX_NODAL ? Q * Q * Q : L * L * L);
   r->U0.assign(U0, U0 + nf * desc->ne * per);

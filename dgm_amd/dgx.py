@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DGX_LIBRARY") or os.path.join(HERE, "libdgx.so")
 
-DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D = 1, 2, 8, 9
+DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D, DGX_SHALLOW = 1, 2, 8, 9, 30
 DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
 DGX_VISCOUS_BR1, DGX_VISCOUS_BR2 = 0, 1
 DGX_BC_WALL, DGX_BC_STATE = 0, 1
@@ -30,7 +30,7 @@ class DomainDesc(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("vis", ctypes.c_double), ("Pr", ctypes.c_double),
                 ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip),
                 ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5),
-                ("xnodes", _dp)]
+                ("xnodes", _dp), ("no_balance", ctypes.c_int)]
 
 
 class DgxError(RuntimeError):
@@ -92,7 +92,7 @@ class GPUDomain:
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
                  precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None,
-                 xnodes=None):
+                 xnodes=None, balance=True):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -102,7 +102,8 @@ class GPUDomain:
                           float(gamma), float(vis), float(Pr), _ptr(self._vert), len(self._con),
                           self._con.ctypes.data_as(_ip), int(viscous), float(br2_eta),
                           (ctypes.c_double * 5)(*([0.0] * 5 if bc_state is None else [float(v) for v in bc_state])),
-                          None if xnodes is None else _ptr(self._nodes(xnodes, mesh, p)))
+                          None if xnodes is None else _ptr(self._nodes(xnodes, mesh, p)),
+                          0 if balance else 1)
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))

@@ -33,6 +33,24 @@ def periodic_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0) -> Mesh:
     return Mesh(nsd=2, vert=vert, con=np.asarray(con, dtype=np.int32))
 
 
+def box_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, bc=(None, None, None, None)) -> Mesh:
+    """Quads on [x0, x1] x [y0, y1]; bc[k] is the boundary type of the bottom
+    (k=0), right (1), top (2) and left (3) edge, or None for periodic in that
+    direction (bottom/top and left/right must agree on None).  Boundary rows
+    are (eid, sid, -1, type): 0 wall, 1 state, 2 function of (x, y, t)."""
+    if (bc[0] is None) != (bc[2] is None) or (bc[1] is None) != (bc[3] is None):
+        raise ValueError("opposite edges must both be periodic or both be boundaries")
+    m = periodic_quads(nx, ny, x0, x1, y0, y1)
+    con = m.con.copy()
+    for r in range(len(con)):
+        e, s = int(con[r, 0]), int(con[r, 1])
+        i, j = e % nx, e // nx
+        edge = {0: j == 0, 1: i == nx - 1, 2: j == ny - 1, 3: i == 0}[s]
+        if edge and bc[s] is not None:
+            con[r, 2], con[r, 3] = -1, bc[s]
+    return Mesh(nsd=2, vert=m.vert, con=con)
+
+
 def periodic_hexes(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> Mesh:
     xs = np.linspace(lo[0], hi[0], nx + 1)
     ys = np.linspace(lo[1], hi[1], ny + 1)

@@ -57,13 +57,15 @@ def _check(lib, rc):
         raise dgx.DgxError(rc, lib.dgx_last_error().decode())
 
 
-def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0):
+def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0,
+          balance=True):
     vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
     con = np.ascontiguousarray(mesh.con, dtype=np.int32)
     d = dgx.DomainDesc(int(mesh.nsd), int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
                        float(gamma), float(vis), float(Pr),
                        vert.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(con),
                        con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(viscous), float(br2_eta))
+    d.no_balance = 0 if balance else 1
     return d, (vert, con)
 
 
@@ -109,10 +111,11 @@ class Server:
 
     def submit(self, mesh, p, eqntype, U0, inttype, dt, nsteps, gamma=1.4, vis=0.0, Pr=0.72,
                flux=dgx.DGX_FLUX_LF, precision=dgx.DGX_FP64, layout=dgx.DGX_NODAL,
-               viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0) -> int:
-        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous, br2_eta)
+               viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0, balance=True) -> int:
+        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous, br2_eta, balance)
         U0 = np.ascontiguousarray(U0, dtype=np.float64)
-        nf, L, Q = mesh.nsd + 2, p + 1, p + 2
+        nf = 3 if eqntype == dgx.DGX_SHALLOW else mesh.nsd + 2
+        L, Q = p + 1, p + 2
         per = (Q if layout == dgx.DGX_NODAL else L) ** mesh.nsd
         if U0.shape != (nf, mesh.ne, per):
             raise ValueError(f"initial state shape {U0.shape} != {(nf, mesh.ne, per)}")

@@ -47,7 +47,9 @@ enum dgx_eqntype {
   DGX_EULER = 1,            /* Reo::Euler (2D)            */
   DGX_NAVIER_STOKES = 2,    /* Reo::Navier_Stokes (2D)    */
   DGX_EULER3D = 8,          /* Reo::Euler3d               */
-  DGX_NAVIER_STOKES3D = 9   /* Reo::Navier_Stokes3d       */
+  DGX_NAVIER_STOKES3D = 9,  /* Reo::Navier_Stokes3d       */
+  DGX_SHALLOW = 30          /* Reo::Shallow (2D, fields phi, uphi, vphi;
+                               Lax-Friedrichs flux, FP64) */
 };
 
 /* Numerical flux (src/Euler.cpp:102-113). */
@@ -72,11 +74,12 @@ enum dgx_layout { DGX_NODAL = 0, DGX_MODAL = 1 };
 
 typedef struct dgx_domain *dgx_handle;
 
-/* Boundary sides (Euler3d): a .con row (eid, sid, -1, type) marks side sid
- * of element eid as a boundary instead of a link.  Slip wall as
- * Euler3d::WallBC (src/Euler3d.cpp:876-909), imposed state as StateBC
- * (src/BC.cpp:17-28); the ghost values meet the interior through the same
- * Riemann flux as an interior face. */
+/* Boundary sides (Euler3d, Shallow): a .con row (eid, sid, -1, type) marks
+ * side sid of element eid as a boundary instead of a link.  Slip wall as
+ * Euler3d::WallBC (src/Euler3d.cpp:876-909) or Shallow::WallBC
+ * (src/ShallowBC.hpp:45-62), imposed state as StateBC (src/BC.cpp:17-28);
+ * the ghost values meet the interior through the same Riemann flux as an
+ * interior face. */
 enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1 };
 
 /* Mesh and physics description.  `vert` holds, per element, nsd rows of
@@ -99,9 +102,13 @@ typedef struct {
   const int *con;
   int viscous;      /* enum dgx_viscous (Navier-Stokes; 0 = BR1) */
   double br2_eta;   /* BR2 penalty; 0 selects 6, the number of hex faces */
-  double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries */
+  double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries (Shallow: the first 3) */
   const double *xnodes;  /* curved hexes (Euler3d, FP64, p <= 6): coordinates [3][ne][q^3] of
                             every element's GLL nodes, node (i*q + j)*q + k; NULL: affine */
+  int no_balance;   /* Shallow: 1 leaves out the balance term.  By default, as
+                       compiled in the reference (src/Shallow.cpp:251-266), the
+                       domain's first residual is stored and subtracted from it
+                       and from every later residual */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

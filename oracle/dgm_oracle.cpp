@@ -26,6 +26,9 @@
 //   src/Navier_Stokes.cpp:113-258                 2D BR1 diffusion
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
 //   src/LinEuler_quasi.cpp:151-495                2D linearized Euler (uniform mean flow)
+//   src/Shallow.cpp:251-525, src/ShallowBC.hpp:45-62  shallow water: convection, LF flux,
+//                                                 wall, balance term
+//   src/BC.cpp:95-200, src/Source.cpp:11-75       function boundaries and sources
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
 //   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
@@ -364,17 +367,48 @@ int point_flux3d(double gamma, int flux_type, const double Ul[5], const double U
   return status;
 }
 
+// Shallow-water Lax-Friedrichs flux at one face point (src/Shallow.cpp:357-525)
+// in (phi, uphi, vphi), phi the geopotential g(h + eta): normal fluxes of both
+// states (Cartesian) and the numerical flux; the dissipation speed is the
+// largest |u_n| + sqrt(phi) of the two states.  Returns 1 when phi or the
+// pressure phi^2/2 of either state is below aepsM = eps^(2/3)
+// (src/Constants.hpp:41-43), where the reference stops.
+int point_flux_shallow(const double Ul[3], const double Ur[3], double nxl, double nyl, double Fpl[3],
+                       double Fmr[3], double Fh[3]) {
+  const double pt5 = 0.5, aepsM = std::pow(2.220446049250313e-16, 0.6666666666666666667);
+  int status = 0;
+  const double phil = Ul[0], uphil = Ul[1] * nxl + Ul[2] * nyl, vphil = Ul[2] * nxl - Ul[1] * nyl;
+  const double iphil = 1.0 / phil, ul = iphil * uphil, pl = pt5 * phil * phil;
+  if (!(phil >= aepsM) || !(pl >= aepsM)) status = 1;
+  Fpl[0] = uphil;
+  Fpl[1] = (uphil * ul + pl) * nxl - (vphil * ul) * nyl;
+  Fpl[2] = (uphil * ul + pl) * nyl + (vphil * ul) * nxl;
+  const double phir = Ur[0], uphir = Ur[1] * nxl + Ur[2] * nyl, vphir = Ur[2] * nxl - Ur[1] * nyl;
+  const double iphir = 1.0 / phir, ur = iphir * uphir, pr = pt5 * phir * phir;
+  if (!(phir >= aepsM) || !(pr >= aepsM)) status = 1;
+  Fmr[0] = uphir;
+  Fmr[1] = (uphir * ur + pr) * nxl - (vphir * ur) * nyl;
+  Fmr[2] = (uphir * ur + pr) * nyl + (vphir * ur) * nxl;
+  double lmax = 0.0;
+  const double al = std::sqrt(phil), ar = std::sqrt(phir);
+  const double laml[3] = {ul - al, ul, ul + al}, lamr[3] = {ur - ar, ur, ur + ar};
+  for (int n = 0; n < 3; ++n) lmax = std::max(lmax, std::fabs(laml[n]));
+  for (int n = 0; n < 3; ++n) lmax = std::max(lmax, std::fabs(lamr[n]));
+  for (int n = 0; n < 3; ++n) Fh[n] = pt5 * (Fpl[n] + Fmr[n] + lmax * (Ul[n] - Ur[n]));
+  return status;
+}
+
 // ---------------------------------------------------------------------------
 // Domain
 // ---------------------------------------------------------------------------
 
-enum Eqn { EULER2D = 1, NS2D = 2, LINEULER2D = 4, EULER3D = 8, NS3D = 9 };
+enum Eqn { EULER2D = 1, NS2D = 2, LINEULER2D = 4, EULER3D = 8, NS3D = 9, SHALLOW2D = 30 };
 
 struct SideInfo {
   bool master = false;       // side->link set (src/SideLoop.hpp:46-50)
   int le = -1, ls = -1;      // linked element/side (both directions)
   bool flip_a = false, flip_b = false, transp = false;
-  int bc = -1;               // boundary side: 0 slip wall, 1 state (no link)
+  int bc = -1;               // boundary side: 0 slip wall, 1 state, 2 function of (x, y, t) (no link)
   std::vector<double> nx, ny, nz, J;   // Gauss-point normal and J_side/J_vol
 };
 
@@ -1424,6 +1458,111 @@ struct Domain {
       for (int n = 0; n < 4; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
   }
 
+  // ---- shallow water (src/Shallow.cpp) in (phi, uphi, vphi) ----
+  // Function boundaries and sources (FuncBC, src/BC.cpp:95-200; FuncSource,
+  // src/Source.cpp:11-75) are callbacks of (t, n, x, y) -> 3 x n values; the
+  // balance term is the first residual, subtracted from every residual
+  // (REO_SHALLOW_ENFORCE_BALANCE, src/Shallow.cpp:251-266).
+  typedef void (*field_fn)(double t, int n, const double *x, const double *y, double *out);
+  field_fn bc_fn = nullptr, src_fn = nullptr;
+  double t = 0.0;
+  bool balance = true, have_bal = false;
+  std::vector<double> Usrc;
+
+  // Gauss-point coordinates of side s of element e (Element::get_side_coordinates)
+  void side_coords(int e, int s, double *x, double *y) const {
+    std::vector<double> wk(q);
+    get_side(el[e].cx.data(), s, wk.data()); to_side(wk.data(), x);
+    get_side(el[e].cy.data(), s, wk.data()); to_side(wk.data(), y);
+  }
+  // ghost values of a shallow-water boundary side: Shallow::WallBC::apply
+  // (src/ShallowBC.hpp:45-62) keeps phi and the tangential momentum; state
+  // boundaries hold bc_state; function boundaries the callback's values
+  void ghost_shallow(int e, int s, const SideInfo &S, VF &Us, std::vector<double> &Ub) {
+    Ub.assign(3 * qs, 0.0);
+    if (S.bc == 0) {
+      for (int g = 0; g < qs; ++g) {
+        const double m1 = es(Us, 1, e, s)[g], m2 = es(Us, 2, e, s)[g], nx = S.nx[g], ny = S.ny[g];
+        Ub[g] = es(Us, 0, e, s)[g];
+        Ub[qs + g] = (m1 * ny - m2 * nx) * ny;
+        Ub[2 * qs + g] = (m2 * nx - m1 * ny) * nx;
+      }
+    } else if (S.bc == 1) {
+      for (int n = 0; n < 3; ++n) for (int g = 0; g < qs; ++g) Ub[n * qs + g] = bc_state[n];
+    } else {
+      std::vector<double> x(qs), y(qs);
+      side_coords(e, s, x.data(), y.data());
+      if (bc_fn) bc_fn(t, qs, x.data(), y.data(), Ub.data());
+    }
+  }
+  // Shallow::convection (src/Shallow.cpp:276-336): flux divergence at the
+  // quadrature points, master-side flux, lift
+  void convection_shallow2d(VF &Us, VF &Uf) {
+    fill_sides_all(Us);
+    std::vector<double> u(qtot), v(qtot), p_(qtot), d(qtot), d1(qtot);
+    for (int e = 0; e < ne; ++e) {
+      const double *phi = eu(Us, 0, e), *uphi = eu(Us, 1, e), *vphi = eu(Us, 2, e);
+      double *F0 = eu(Uf, 0, e), *F1 = eu(Uf, 1, e), *F2 = eu(Uf, 2, e);
+      for (int i = 0; i < qtot; ++i) { u[i] = uphi[i] / phi[i]; v[i] = vphi[i] / phi[i]; }
+      for (int i = 0; i < qtot; ++i) { p_[i] = phi[i] * phi[i]; p_[i] = 0.5 * p_[i]; }
+      gradient_dir(e, uphi, F0, 'x');
+      gradient_dir(e, vphi, d.data(), 'y');
+      for (int i = 0; i < qtot; ++i) F0[i] = d[i] + F0[i];
+      for (int i = 0; i < qtot; ++i) { d[i] = uphi[i] * u[i]; d[i] = d[i] + p_[i]; }
+      gradient_dir(e, d.data(), F1, 'x');
+      for (int i = 0; i < qtot; ++i) d[i] = uphi[i] * v[i];
+      gradient_dir(e, d.data(), d1.data(), 'y');
+      for (int i = 0; i < qtot; ++i) F1[i] = d1[i] + F1[i];
+      for (int i = 0; i < qtot; ++i) d[i] = vphi[i] * u[i];
+      gradient_dir(e, d.data(), F2, 'x');
+      for (int i = 0; i < qtot; ++i) { d[i] = vphi[i] * v[i]; d[i] = d[i] + p_[i]; }
+      gradient_dir(e, d.data(), d1.data(), 'y');
+      for (int i = 0; i < qtot; ++i) F2[i] = d1[i] + F2[i];
+    }
+    std::vector<double> Ub;
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < nsides; ++s) {
+        const SideInfo &S = el[e].side[s];
+        if (S.bc >= 0) {
+          ghost_shallow(e, s, S, Us, Ub);
+          for (int g = 0; g < qs; ++g) {
+            double Ul[3], Ur[3], Fpl[3], Fmr[3], Fh[3];
+            for (int n = 0; n < 3; ++n) { Ul[n] = es(Us, n, e, s)[g]; Ur[n] = Ub[n * qs + g]; }
+            status |= point_flux_shallow(Ul, Ur, S.nx[g], S.ny[g], Fpl, Fmr, Fh);
+            for (int n = 0; n < 3; ++n) { es(Us, n, e, s)[g] = Fpl[n]; es(Uf, n, e, s)[g] = Fh[n]; }
+          }
+          continue;
+        }
+        if (!S.master) continue;
+        for (int g = 0; g < qs; ++g) {
+          const int ql = sidemap(S, g), qr = sidemap_slave(g);
+          double Ul[3], Ur[3], Fpl[3], Fmr[3], Fh[3];
+          for (int n = 0; n < 3; ++n) { Ul[n] = es(Us, n, e, s)[ql]; Ur[n] = es(Us, n, S.le, S.ls)[qr]; }
+          status |= point_flux_shallow(Ul, Ur, S.nx[ql], S.ny[ql], Fpl, Fmr, Fh);
+          for (int n = 0; n < 3; ++n) {
+            es(Us, n, e, s)[ql] = Fpl[n];
+            es(Us, n, S.le, S.ls)[qr] = -Fmr[n];
+            es(Uf, n, e, s)[ql] = Fh[n];
+            es(Uf, n, S.le, S.ls)[qr] = -Fh[n];
+          }
+        }
+      }
+    for (int e = 0; e < ne; ++e)
+      for (int n = 0; n < 3; ++n) lift(Us, Uf, n, e, eu(Uf, n, e), 0);
+  }
+  // FuncSource::apply (src/Source.cpp:11-24): Uf -= s at the quadrature points
+  void add_source_shallow(VF &Uf) {
+    if (!src_fn) return;
+    std::vector<double> s(3 * qtot);
+    for (int e = 0; e < ne; ++e) {
+      src_fn(t, qtot, el[e].cx.data(), el[e].cy.data(), s.data());
+      for (int n = 0; n < 3; ++n) {
+        double *f = eu(Uf, n, e);
+        for (int i = 0; i < qtot; ++i) f[i] = f[i] - s[n * qtot + i];
+      }
+    }
+  }
+
   void project_all(VF &F) {
     for (int v = 0; v < nv; ++v)
       for (int e = 0; e < ne; ++e) project_vec(e, eu(F, v, e));
@@ -1433,6 +1572,16 @@ struct Domain {
   // src/Navier_Stokes.cpp:89-94, src/Navier_Stokes3d.cpp:135-161)
   void time_derivative(VF &Us, VF &Uf) {
     if (eqn == LINEULER2D) { convection_lin2d(Us, Uf); project_all(Uf); return; }
+    if (eqn == SHALLOW2D) {   // Shallow::time_derivative (src/Shallow.cpp:255-266)
+      convection_shallow2d(Us, Uf);
+      add_source_shallow(Uf);
+      project_all(Uf);
+      if (balance) {
+        if (!have_bal) { Usrc = Uf.u; have_bal = true; }
+        for (size_t i = 0; i < Uf.u.size(); ++i) Uf.u[i] = -1.0 * Usrc[i] + Uf.u[i];
+      }
+      return;
+    }
     if (nsd == 2) convection2d(Us, Uf); else convection3d(Us, Uf);
     if (eqn == NS2D) diffusion2d(Us, Uv, Uf);
     if (eqn == NS3D) diffusion3d(Us, Uv, Uf);
@@ -1460,28 +1609,34 @@ struct Domain {
     time_derivative(U, R0);
     axpy(-a[0] * dt, R0, U, Ut);
     scal(-b[0], R0);
+    t += 0.5 * dt;
     time_derivative(Ut, R1);
     axpy(-a[1] * dt, R1, U, Ut);
     axpy2(-b[1], R1, R0);
     time_derivative(Ut, R1);
     axpy(-a[2] * dt, R1, U, Ut);
     axpy2(-b[2], R1, R0);
+    t += 0.5 * dt;
     time_derivative(Ut, R1);
     axpy2(-b[3], R1, R0);
     axpy2(dt, R0, U);
   }
-  // TVD::advance (src/TimeInt.cpp:824-856)
+  // TVD::advance (src/TimeInt.cpp:824-856); stage times c = {0, 1, 1/2} (src/TimeInt.cpp:740)
   void tvd(double dt) {
     static const double a[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
     static const double b[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    const double tn = t;
     time_derivative(U, R0);
     axpy(-dt, R0, U, Ut);
+    t = tn + dt;
     time_derivative(Ut, R0);
     axpy(-dt, R0, Ut, R0);
     axpby(a[1], U, b[1], R0, Ut);
+    t = tn + 0.5 * dt;
     time_derivative(Ut, R0);
     axpy(-dt, R0, Ut, R0);
     axpby(b[2], R0, a[2], U, U);
+    t = tn + dt;
   }
   // LSRK14::advance (src/TimeInt.cpp:1989-2013); coefficients from
   // src/TimeInt.cpp:1870-1917
@@ -1494,11 +1649,17 @@ struct Domain {
       0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
       0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
       0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
+    static const double C[14] = {0.0, 0.0367762454319673, 0.1249685262725025, 0.2446177702277698,
+      0.2476149531070420, 0.2969311120382472, 0.3978149645802642, 0.5270854589440328, 0.6981269994175695,
+      0.8190890835352128, 0.8527059887098624, 0.8604711817462826, 0.8627060376969976, 0.8734213127600976};
+    const double tn = t;
     for (int s = 0; s < 14; ++s) {
+      t = tn + C[s] * dt;
       time_derivative(U, R0);
       axpby(-dt, R0, A[s], Ut, Ut);
       axpy2(Bc[s], Ut, U);
     }
+    t = tn + dt;
   }
 };
 
@@ -1510,14 +1671,14 @@ struct Domain {
 extern "C" {
 
 // vert: [e][dim][nvert] as laid out in a .msh file; con: ncon rows of
-// (eid, sid, leid, lsid) in .con file order.  eqn: 1 Euler, 2 NS (2D),
+// (eid, sid, leid, lsid) in .con file order.  eqn: 1 Euler, 2 NS (2D), 30 Shallow,
 // 8 Euler3d, 9 NS3d (src/Reo.hpp:33-61).
 void *orc_create(int nsd, int ne, int p, const double *vert, int ncon, const int *con,
                  int eqn, double gamma, double vis, double Pr, int flux) {
   Domain *d = new Domain();
   d->nsd = nsd; d->ne = ne; d->p = p; d->eqn = eqn;
   d->gamma = gamma; d->vis = vis; d->Pr = Pr; d->flux_type = flux;
-  d->nv = nsd + 2;
+  d->nv = eqn == SHALLOW2D ? 3 : nsd + 2;
   d->build_basis();
   d->build_geometry(vert);
   d->build_links(ncon, con, vert);
@@ -1560,6 +1721,20 @@ void orc_set_linear(void *h, const double *Ax, const double *Ay, double u0, doub
   std::memcpy(d->linA[0], Ax, 16 * sizeof(double));
   std::memcpy(d->linA[1], Ay, 16 * sizeof(double));
   d->lin_u0 = u0; d->lin_v0 = v0; d->lin_c0 = c0;
+}
+// Shallow water: boundary and source callbacks (either may be null), the
+// domain time, and the first-residual balance (on by default, as compiled in
+// the reference); turning it on again discards the stored residual.
+void orc_set_functions(void *h, void (*bc)(double, int, const double *, const double *, double *),
+                       void (*src)(double, int, const double *, const double *, double *)) {
+  Domain *d = (Domain *)h;
+  d->bc_fn = bc; d->src_fn = src;
+}
+void orc_set_time(void *h, double t) { ((Domain *)h)->t = t; }
+double orc_get_time(void *h) { return ((Domain *)h)->t; }
+void orc_set_balance(void *h, int on) {
+  Domain *d = (Domain *)h;
+  d->balance = on != 0; d->have_bal = false; d->Usrc.clear();
 }
 int orc_qtot(void *h) { return ((Domain *)h)->qtot; }
 int orc_nmodes(void *h) { return ((Domain *)h)->nm; }

@@ -16,6 +16,9 @@ LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lib = None
+# callback (t, n, x, y, out[3 x n]) of shallow-water boundaries and sources
+FIELD_FN = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp)
+SHALLOW = 30
 
 
 def _load():
@@ -51,6 +54,11 @@ def _load():
     lib.orc_set_bc_state.argtypes = [ctypes.c_void_p, _dp]
     lib.orc_set_nodes.argtypes = [ctypes.c_void_p, _dp]
     lib.orc_set_linear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.orc_set_functions.argtypes = [ctypes.c_void_p, FIELD_FN, FIELD_FN]
+    lib.orc_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.orc_get_time.argtypes = [ctypes.c_void_p]
+    lib.orc_get_time.restype = ctypes.c_double
+    lib.orc_set_balance.argtypes = [ctypes.c_void_p, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -67,7 +75,7 @@ class Oracle:
         self.nsd = mesh.nsd
         self.ne = mesh.ne
         self.p = p
-        self.nv = self.nsd + 2
+        self.nv = 3 if eqn == SHALLOW else self.nsd + 2
         self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
         self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
         self.h = self.lib.orc_create(self.nsd, self.ne, p, _ptr(self._vert), len(self._con),
@@ -90,9 +98,35 @@ class Oracle:
         self.lib.orc_set_nodes(self.h, _ptr(self._xyz))
 
     def set_bc_state(self, U):
-        """Conservative state imposed on state boundaries (3D Euler)."""
-        self._bcU = np.ascontiguousarray(U, dtype=np.float64).reshape(5)
+        """Conservative state imposed on state boundaries (3D Euler; shallow water uses 3 values)."""
+        U = np.ravel(np.asarray(U, dtype=np.float64))
+        self._bcU = np.zeros(5)
+        self._bcU[:len(U)] = U
         self.lib.orc_set_bc_state(self.h, _ptr(self._bcU))
+
+    def set_functions(self, bc=None, src=None):
+        """Shallow water: Python callables (t, x, y) -> array [3, n] for function
+        boundaries (type 2 rows) and sources; None leaves that part out."""
+        def wrap(fn):
+            if fn is None:
+                return FIELD_FN()
+            def cb(t, n, x, y, out):
+                xs = np.ctypeslib.as_array(x, shape=(n,)).copy()
+                ys = np.ctypeslib.as_array(y, shape=(n,)).copy()
+                np.ctypeslib.as_array(out, shape=(3 * n,))[:] = np.asarray(fn(t, xs, ys), dtype=np.float64).reshape(-1)
+            return FIELD_FN(cb)
+        self._cbs = (wrap(bc), wrap(src))      # kept alive with the oracle
+        self.lib.orc_set_functions(self.h, *self._cbs)
+
+    def set_time(self, t):
+        self.lib.orc_set_time(self.h, float(t))
+
+    def time(self):
+        return self.lib.orc_get_time(self.h)
+
+    def set_balance(self, on):
+        """Shallow water: subtract the first residual from every residual (default on)."""
+        self.lib.orc_set_balance(self.h, int(bool(on)))
 
     def set_linear(self, Ax, Ay, u0, v0, c0):
         """Matrices of the linear system (eqn 4); Lax-Friedrichs speed |u0 nx + v0 ny| + c0."""

@@ -6,6 +6,8 @@
   bitwise for the 2D path; to rounding for the fused 3D path, where each slice
   starts by recomputing the face traces from the state instead of taking them
   from the previous stage's epilogue (1e-12 FP64, 1e-6 mixed).
+* Shallow-water and Euler requests share the loop; a shallow-water request
+  with the balance term keeps its initial state, as the reference's does.
 * Admission respects the memory budget and the slot count.
 * A request larger than the budget on its own is rejected at submission.
 """
@@ -31,6 +33,15 @@ def _euler2d_job(rng, n, p, steps):
     return dict(mesh=m, p=p, eqntype=dgx.DGX_EULER, U0=U, inttype=dgx.DGX_LSRK14, dt=dt, nsteps=steps)
 
 
+def _shallow_job(rng, n, p, steps, balance=False):
+    m = meshgen.periodic_quads(n, n, 0.0, 1.0, 0.0, 1.0)
+    x = meshgen.quad_nodes(m, p)
+    U = np.stack([b + ic.fourier_modes(x, (1.0, 1.0), 8, 0.05, rng) for b in (1.0, 0.0, 0.0)])
+    dt = 0.2 * (1.0 / n) / ((p + 1) ** 2 * 1.2)
+    return dict(mesh=m, p=p, eqntype=dgx.DGX_SHALLOW, U0=U, inttype=dgx.DGX_LSRK14, dt=dt, nsteps=steps,
+                balance=balance)
+
+
 def _ns3d_job(rng, n, p, steps, precision=dgx.DGX_FP64):
     m = meshgen.periodic_hexes(n, n, n, (0.0,) * 3, (2 * np.pi,) * 3)
     g = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, vis=0.01, Pr=0.71)
@@ -53,7 +64,7 @@ def _served_alone(job, steps_per_slice=5):
 
 def _alone(job):
     g = dgx.GPUDomain(job["mesh"], job["p"], job["eqntype"], vis=job.get("vis", 0.0), Pr=job.get("Pr", 0.72),
-                      precision=job.get("precision", dgx.DGX_FP64))
+                      precision=job.get("precision", dgx.DGX_FP64), balance=job.get("balance", True))
     g.upload(job["U0"])
     g.advance(job["inttype"], job["dt"], job["nsteps"])
     g.synchronize()
@@ -68,6 +79,9 @@ def _mix():
     for n, p, steps in [(4, 2, 7), (8, 4, 13), (6, 3, 20), (12, 5, 11), (4, 6, 16), (10, 2, 9),
                         (8, 8, 12), (5, 4, 25)]:
         jobs.append(_euler2d_job(rng, n, p, steps))
+    for n, p, steps in [(6, 3, 14), (9, 5, 11), (4, 8, 17)]:
+        jobs.append(_shallow_job(rng, n, p, steps))
+    jobs.append(_shallow_job(rng, 5, 4, 12, balance=True))
     jobs.append(_ns3d_job(rng, 2, 4, 6))
     jobs.append(_ns3d_job(rng, 2, 7, 5))
     jobs.append(_ns3d_job(rng, 2, 5, 7, precision=dgx.DGX_MIXED))
@@ -114,6 +128,25 @@ def test_graph_replay_matches_direct_launches():
         out.append(srv.result(rid)[0])
         srv.close()
     assert np.array_equal(out[0], out[1])
+
+
+def test_shallow_balance_keeps_the_initial_state():
+    """With the balance term (on by default, as in the reference) the first
+    residual is subtracted from every residual, so the initial state is a
+    fixed point of the served solve; without it the state evolves."""
+    rng = np.random.default_rng(11)
+    for balance in (True, False):
+        job = _shallow_job(rng, 6, 4, 12, balance=balance)
+        U = _served_alone(job, steps_per_slice=4)
+        moved = np.max(np.abs(U - job["U0"]))
+        if balance:
+            g = dgx.GPUDomain(job["mesh"], job["p"], job["eqntype"], balance=False)
+            g.upload(job["U0"])
+            U0p = g.download()                           # the projected initial state
+            g.close()
+            assert np.max(np.abs(U - U0p)) < 1e-13, np.max(np.abs(U - U0p))
+        else:
+            assert moved > 1e-4, moved
 
 
 def test_oversized_request_rejected():

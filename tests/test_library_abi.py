@@ -71,7 +71,7 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=8, viscous=1), b"DGX_VISCOUS_BR2"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=-1.0), b"br2_eta"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
-    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler3d"),
+    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler3d and Shallow"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
     (dict(p=3, eqn=9, curved=True), b"curved elements"),
     (dict(p=7, eqn=8, curved=True), b"curved elements"),
@@ -92,6 +92,28 @@ def test_create_rejects_bad_arguments(lib, case, msg):
     rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
     assert rc == -1
     assert msg in lib.dgx_last_error()
+    assert not h.value
+
+
+@pytest.mark.parametrize("case,msg", [
+    (dict(flux=1), b"Lax-Friedrichs flux only"),
+    (dict(precision=1), b"DGX_MIXED"),
+    (dict(nsd3=True), b"eqntype does not match nsd"),
+    (dict(boundary=4), b"unknown boundary type"),
+    (dict(boundary=0, eqn=1), b"boundary sides are available"),
+])
+def test_shallow_create_rejects_bad_arguments(lib, case, msg):
+    if case.get("nsd3"):
+        m = meshgen.periodic_hexes(2, 2, 2)
+    elif "boundary" in case:
+        m = meshgen.box_quads(2, 2, bc=(case["boundary"], None, 0, None))
+    else:
+        m = meshgen.periodic_quads(2, 2)
+    d, keep = _desc(m, 3, case.get("eqn", 30), flux=case.get("flux", 0), precision=case.get("precision", 0))
+    h = ctypes.c_void_p()
+    rc = lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h))
+    assert rc == -1
+    assert msg in lib.dgx_last_error(), lib.dgx_last_error()
     assert not h.value
 
 
