@@ -96,6 +96,30 @@ def read_mesh(root: str) -> Mesh:
     return Mesh(nsd=vert.shape[1], vert=vert, con=con)
 
 
+def read_bc(path: str) -> List[tuple]:
+    """Boundary sides of ``root.bc``: rows (tag, eid, sid) after the count line
+    (Field::read_bc, src/Field.cpp:4469-4495).  A side listed here is a
+    boundary even where ``root.con`` also links it: Field::initialize_bc
+    relinks it to its boundary side (src/Field.cpp:4426-4462)."""
+    rows: List[tuple] = []
+    with open(path) as f:
+        f.readline()
+        n = int(f.readline().split()[0])
+        for _ in range(n):
+            toks = f.readline().split()
+            rows.append((toks[0], int(toks[1]), int(toks[2])))
+    return rows
+
+
+def boundary_con(con: np.ndarray, bc_rows, types: Dict[str, int]) -> np.ndarray:
+    """Connectivity with the boundary sides of ``bc_rows`` turned into rows
+    (eid, sid, -1, type), ``types`` mapping each tag to its boundary type."""
+    bset = {(e, s): types[t] for t, e, s in bc_rows}
+    out = [r for r in con.tolist() if (r[0], r[1]) not in bset]
+    out += [[e, s, -1, t] for (e, s), t in sorted(bset.items())]
+    return np.asarray(out, dtype=np.int32).reshape(-1, 4)
+
+
 @dataclass
 class InitialCondition:
     code: str

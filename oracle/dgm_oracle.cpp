@@ -29,6 +29,8 @@
 //   src/Shallow.cpp:251-525, src/ShallowBC.hpp:45-62  shallow water: convection, LF flux,
 //                                                 wall, balance term
 //   src/BC.cpp:95-200, src/Source.cpp:11-75       function boundaries and sources
+//   src/Navier_Stokes3d.cpp:472-512, src/Channel.hpp:117-163  isothermal walls, bulk forcing:
+//                                                 parity unpinned (runs/chan is not reproduced)
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
 //   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
@@ -408,7 +410,8 @@ struct SideInfo {
   bool master = false;       // side->link set (src/SideLoop.hpp:46-50)
   int le = -1, ls = -1;      // linked element/side (both directions)
   bool flip_a = false, flip_b = false, transp = false;
-  int bc = -1;               // boundary side: 0 slip wall, 1 state, 2 function of (x, y, t) (no link)
+  int bc = -1;               // boundary side: 0 slip wall, 1 state, 2 function of (x, y, t),
+                             // 3 isothermal wall (no link)
   std::vector<double> nx, ny, nz, J;   // Gauss-point normal and J_side/J_vol
 };
 
@@ -1114,8 +1117,16 @@ struct Domain {
   // ghost side values of a boundary side (Domain::enforce_bc, src/Domain.cpp:1346-1350):
   // slip wall Euler3d::WallBC::apply (src/Euler3d.cpp:876-909): the normal
   // momentum removed, internal energy kept; state StateBC::apply (src/BC.cpp:17-28)
+  // isothermal wall Navier_Stokes3d::Isothermal_WallBC::apply (src/Navier_Stokes3d.cpp:472-492):
+  // density kept, no velocity, energy (Eref - 1/2) rho Tw
+  double iso_Tw = 1.0, iso_Eref = 0.0;
   void ghost3d(int bc, const double *Ul, double nx, double ny, double nz, double *Ub) const {
     if (bc == 1) { for (int n = 0; n < 5; ++n) Ub[n] = bc_state[n]; return; }
+    if (bc == 3) {
+      Ub[0] = Ul[0]; Ub[1] = 0.0; Ub[2] = 0.0; Ub[3] = 0.0;
+      Ub[4] = (iso_Eref - 0.5) * Ub[0] * iso_Tw;
+      return;
+    }
     double up = Ul[1], vp = Ul[2], wp = Ul[3];
     double dot = up * nx + vp * ny + wp * nz;
     up = up - dot * nx; vp = vp - dot * ny; wp = wp - dot * nz;
@@ -1166,6 +1177,15 @@ struct Domain {
       for (int e = 0; e < ne; ++e)
         for (int s = 0; s < nsides; ++s) {
           const SideInfo &S = el[e].side[s];
+          if (S.bc >= 0 && nsd == 3) {   // boundary: ghost primitives ('+') or -S ('-')
+            const double *ul = es(Us, v, e, s);
+            double *fl = es(Fs, v, e, s);
+            for (int g = 0; g < qs; ++g) {
+              const double ub = (sign == '+') ? gprim[(((size_t)v * ne + e) * nsides + s) * qs + g] : -ul[g];
+              fl[g] = 0.5 * (ul[g] + fact * ub);
+            }
+            continue;
+          }
           if (!S.master) continue;
           const double *ul = es(Us, v, e, s), *ur = es(Us, v, S.le, S.ls);
           double *fl = es(Fs, v, e, s), *fr = es(Fs, v, S.le, S.ls);
@@ -1178,11 +1198,49 @@ struct Domain {
   }
 
   // conservative -> primitive side values (src/Euler.cpp:1420-1456, src/Euler3d.cpp:1162-1199)
-  void primitive_sides(VF &Us) {
-    const int d = nsd;
+  // Boundary sides of Navier-Stokes 3D (src/Field.cpp:4451 links them to their
+  // ghost side): the ghost's primitive values, set with the interior's
+  // (Domain::enforce_bc then Euler3d::primitive_sides); the viscous flux's
+  // ghost is -S (Isothermal_WallBC::apply_flux, StateBC::apply_flux,
+  // src/Navier_Stokes3d.cpp:494-512, src/BC.cpp:31-40), so its '-' average is S.
+  std::vector<double> gprim;   // [v][e][side][qs] ghost primitives of boundary sides
+  void primitive_ghosts(VF &Us) {
+    gprim.assign((size_t)nv * ne * nsides * qs, 0.0);
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
+        if (S.bc < 0) continue;
+        if (S.bc == 0) status |= 2;   // Euler3d::WallBC has no viscous flux (src/Euler3d.cpp:911-913)
+        for (int g = 0; g < qs; ++g) {
+          double Ul[5], Ub[5];
+          for (int n = 0; n < 5; ++n) Ul[n] = es(Us, n, e, s)[g];
+          ghost3d(S.bc, Ul, S.nx[g], S.ny[g], S.nz[g], Ub);
+          double u = Ub[1] / Ub[0], v = Ub[2] / Ub[0], w = Ub[3] / Ub[0], en = Ub[4] / Ub[0];
+          double wk = u * u; wk = v * v + wk; wk = w * w + wk;
+          en = -0.5 * wk + en;
+          const double pr[5] = {Ub[0], u, v, w, en};
+          for (int n = 0; n < 5; ++n) gprim[(((size_t)n * ne + e) * nsides + s) * qs + g] = pr[n];
+        }
+      }
+  }
+  void primitive_sides(VF &Us) {
+    const int d = nsd;
+    if (d == 3) primitive_ghosts(Us);
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < nsides; ++s) {
+        const SideInfo &S = el[e].side[s];
+        if (d == 3 && S.bc >= 0) {   // the interior side of a boundary
+          double *r = es(Us, 0, e, s), *uu = es(Us, 1, e, s), *vv = es(Us, 2, e, s), *ww = es(Us, 3, e, s);
+          double *en = es(Us, 4, e, s);
+          for (int g = 0; g < qs; ++g) {
+            uu[g] = uu[g] / r[g]; vv[g] = vv[g] / r[g]; ww[g] = ww[g] / r[g]; en[g] = en[g] / r[g];
+            double wk = uu[g] * uu[g];
+            wk = vv[g] * vv[g] + wk;
+            wk = ww[g] * ww[g] + wk;
+            en[g] = -0.5 * wk + en[g];
+          }
+          continue;
+        }
         if (!S.master) continue;
         for (int side = 0; side < 2; ++side) {
           const int ee = side ? S.le : e, ss = side ? S.ls : s;
@@ -1586,6 +1644,33 @@ struct Domain {
     if (eqn == NS2D) diffusion2d(Us, Uv, Uf);
     if (eqn == NS3D) diffusion3d(Us, Uv, Uf);
     project_all(Uf);
+    if (channel) hold_bulk(Uf);
+  }
+
+  // Channel::holdBulkMassFlow (src/Channel.hpp:117-163): subtract the domain
+  // means of the density, x-momentum and energy residuals
+  bool channel = false;
+  void hold_bulk(VF &Uf) {
+    const int fields[3] = {0, 1, nv - 1};
+    double mean[3] = {0, 0, 0}, vol = 0.0;
+    for (int e = 0; e < ne; ++e) {
+      double lv = 0.0;
+      for (int i = 0; i < qtot; ++i) lv += el[e].wJ[i];
+      for (int k = 0; k < 3; ++k) {
+        const double *f = eu(Uf, fields[k], e);
+        double a = 0.0;
+        for (int i = 0; i < qtot; ++i) a += el[e].wJ[i] * f[i];
+        mean[k] += a;
+      }
+      vol += lv;
+    }
+    for (int k = 0; k < 3; ++k) {
+      mean[k] /= vol;
+      for (int e = 0; e < ne; ++e) {
+        double *f = eu(Uf, fields[k], e);
+        for (int i = 0; i < qtot; ++i) f[i] -= mean[k];
+      }
+    }
   }
 
   // vField BLAS-1 (src/vField.hpp:1002-1031) on element data
@@ -1730,6 +1815,13 @@ void orc_set_functions(void *h, void (*bc)(double, int, const double *, const do
   Domain *d = (Domain *)h;
   d->bc_fn = bc; d->src_fn = src;
 }
+// Navier-Stokes 3D: isothermal walls (boundary type 3) with wall temperature
+// Tw and reference energy Eref, and the channel's bulk forcing (on/off).
+void orc_set_isothermal(void *h, double Tw, double Eref) {
+  Domain *d = (Domain *)h;
+  d->iso_Tw = Tw; d->iso_Eref = Eref;
+}
+void orc_set_channel(void *h, int on) { ((Domain *)h)->channel = on != 0; }
 void orc_set_time(void *h, double t) { ((Domain *)h)->t = t; }
 double orc_get_time(void *h) { return ((Domain *)h)->t; }
 void orc_set_balance(void *h, int on) {

@@ -59,6 +59,8 @@ def _load():
     lib.orc_get_time.argtypes = [ctypes.c_void_p]
     lib.orc_get_time.restype = ctypes.c_double
     lib.orc_set_balance.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_set_isothermal.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    lib.orc_set_channel.argtypes = [ctypes.c_void_p, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -123,6 +125,14 @@ class Oracle:
 
     def time(self):
         return self.lib.orc_get_time(self.h)
+
+    def set_isothermal(self, Tw, Eref):
+        """Navier-Stokes 3D isothermal walls (boundary type 3): wall temperature and reference energy."""
+        self.lib.orc_set_isothermal(self.h, float(Tw), float(Eref))
+
+    def set_channel(self, on):
+        """Navier-Stokes 3D: subtract the domain means of the rho, rho u and E residuals (bulk forcing)."""
+        self.lib.orc_set_channel(self.h, int(bool(on)))
 
     def set_balance(self, on):
         """Shallow water: subtract the first residual from every residual (default on)."""
