@@ -143,9 +143,10 @@ def serving_sample(args, seed=1234):
     forward solves, each shallow water about rest (phi = 1) or Euler about a
     freestream (seeded choice), plus 8 seeded Fourier modes of amplitude 0.05;
     n x n periodic quads with n log-uniform in [8, nmax], p uniform in {2..8},
-    200 LSRK14 steps each, FP64, continuous batching on one GPU.  The
-    shallow-water requests leave out the balance term, which would hold them
-    at their initial state."""
+    200 LSRK14 steps each, FP64, continuous batching on one GPU (requests of
+    one order and physics share their launches unless --serve-unbatched).
+    The shallow-water requests leave out the balance term, which would hold
+    them at their initial state."""
     from dgm_amd import serve
     from dgm_amd.mesh import quad_nodes
     rng = np.random.default_rng(seed)
@@ -167,7 +168,7 @@ def serving_sample(args, seed=1234):
             job["balance"] = False
         jobs.append(job)
         dofs.append(len(base[eqn]) * nn * nn * (pp + 1) ** 2)
-    srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active,
+    srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active, batch=not args.serve_unbatched,
                        steps_per_slice=args.serve_slice)
     arrivals = serve.poisson_arrivals(len(jobs), rate=args.serve_rate, rng=rng)
     t0 = time.perf_counter()
@@ -188,7 +189,9 @@ def serving_sample(args, seed=1234):
             "makespan_s": wall, "peak_active": peak_active, "peak_device_bytes": peak_bytes,
             "sample": f"{len(jobs)} 2D solves ({nshallow} shallow water, {len(jobs) - nshallow} Euler), n log-uniform in [8,{args.serve_nmax}], p in 2..8, "
                       f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, FP64, "
-                      f"{args.serve_active} slots, slices of {args.serve_slice} steps replayed from hipGraphs"}
+                      f"{args.serve_active} slots, slices of {args.serve_slice} steps, "
+                      + ("per-request launches replayed from hipGraphs" if args.serve_unbatched
+                         else "requests of one order and physics batched into shared launches")}
 
 
 def adjoint_sample(args, seed=4096):
@@ -387,6 +390,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=32.0, help="arrivals per second")
     ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--serve-slice", type=int, default=20)
+    ap.add_argument("--serve-unbatched", action="store_true", help="serve every request on its own launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
                          "kernel of known size for scaling HBM counters)")

@@ -13,13 +13,13 @@ __global__ void __launch_bounds__(256) k_axpy(double a, const double *__restrict
     z[i] = fma(a, x[i], y[i]);
 }
 
-// z = a*x + b*y
+// z = a*x + b*y, as fma(a, x, b*y) (the batched serving kernels' fused update rounds alike)
 __global__ void __launch_bounds__(256) k_axpby(double a, const double *__restrict__ x, double b,
                                                const double *__restrict__ y,
                                                double *__restrict__ z, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    z[i] = a * x[i] + b * y[i];
+    z[i] = fma(a, x[i], b * y[i]);
 }
 
 __global__ void __launch_bounds__(256) k_scale(double a, double *__restrict__ x, size_t n) {

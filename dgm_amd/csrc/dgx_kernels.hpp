@@ -33,6 +33,32 @@ struct LaunchArgs {
   hipStream_t stream = nullptr;
 };
 
+// One member request of a batched 2D group (serving): its modal state U,
+// LSRK register Ut, face traces T, balance term, error flag, geometry and
+// physics; off = its first work item in the group's flat element list.
+struct Batch2 {
+  double *U = nullptr, *Ut = nullptr, *T = nullptr, *Bal = nullptr;
+  int *flag = nullptr;
+  Geo geo{};
+  Phys phys{};
+  int ne = 0, off = 0, bal = 0;
+  double dt = 0.0;
+};
+
+// Workgroups of a batched launch: enough for every CU, each walking the work list.
+constexpr int kBatchGrid = 8192;
+
+struct BatchLaunch {
+  int op = 0;                    // 0: face traces, 1: residual + LSRK14 register update
+  int p = 0, nreq = 0, nwork = 0;
+  bool shallow = false;          // shallow water (3 fields) or Euler (4 fields)
+  const Batch2 *b = nullptr;     // device array of nreq members
+  double A = 0.0, B = 0.0;       // LSRK14 coefficients of the stage
+  hipStream_t stream = nullptr;
+};
+
+hipError_t launch_batch2(const BatchLaunch &a);
+
 // Fill each translation unit's constant-memory basis tables (kTabTotal doubles).
 hipError_t upload_tables_2d(const double *h);
 hipError_t upload_tables_3d(const double *h);

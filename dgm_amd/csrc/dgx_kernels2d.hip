@@ -8,6 +8,8 @@
 //   shallow water    Shallow::convection, LF flux     src/Shallow.cpp:251-525
 //   linear system    LinEuler_quasi::convection       src/LinEuler_quasi.cpp:151-495
 //                    and its discrete adjoint
+#include <algorithm>
+
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -42,13 +44,14 @@ __device__ __forceinline__ double face_add2(const double *__restrict__ Bend, con
   return s;
 }
 
+// Gauss-point traces of the nf fields of element e on its four edges.
 template <int P>
-__global__ void __launch_bounds__(64)
-k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, int nf) {
+__device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne,
+                                             int nf, int e) {
   using S = H2<P>;
   constexpr int L = S::L, QF = S::QF, L2 = S::L2, NT = S::NT;
   __shared__ double A[L2], Kb[2 * L], Ka[2 * L], O[2 * QF];
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   double *Te = T + (size_t)e * 4 * nf * QF;
   for (int v = 0; v < nf; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L2, A, L2, tid);
@@ -73,15 +76,24 @@ k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, 
   }
 }
 
-template <int P, bool NS>
+template <int P>
 __global__ void __launch_bounds__(64)
-k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double *__restrict__ V,
-       const double *__restrict__ TV, double *__restrict__ R, Tab tb, Geo geo, Phys ph, int ne,
-       int *__restrict__ flag) {
+k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, int nf) {
+  traces2_elem<P>(U, T, ne, nf, blockIdx.x);
+}
+
+// Residual of element e (Euler, or with NS the BR1 viscous terms from V, TV);
+// out(o, r) receives the modal residual r at offset o = (v*ne + e)*L2 + k.
+template <int P, bool NS, class Out>
+__device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const double *__restrict__ T,
+                                          const double *__restrict__ V, const double *__restrict__ TV,
+                                          const Geo &geo, const Phys &ph, int ne, int e, int *__restrict__ flag,
+                                          Out out) {
   using S = H2<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
   __shared__ double Un[4 * Q2], gF[16 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
+  const Tab tb{};
   const double gm1 = ph.gamma - 1.0;
   for (int v = 0; v < 4; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
@@ -154,24 +166,33 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
       const int l = k / L, m = k - l * L;
       double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
       if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
-      R[((size_t)v * ne + e) * L2 + k] = r;
+      out(((size_t)v * ne + e) * L2 + k, r);
     }
     __syncthreads();
   }
+}
+
+template <int P, bool NS>
+__global__ void __launch_bounds__(64)
+k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double *__restrict__ V,
+       const double *__restrict__ TV, double *__restrict__ R, Tab tb, Geo geo, Phys ph, int ne,
+       int *__restrict__ flag) {
+  rhs2_elem<P, NS>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
 }
 
 // Shallow water (Shallow::time_derivative, src/Shallow.cpp:255-336): flux
 // divergence at the GLL points, Lax-Friedrichs face flux, lift, projection.
 // The balance term (src/Shallow.cpp:251-266): bal 1 stores this residual in
 // Bal and returns R - Bal = 0, bal 2 returns R - Bal.
-template <int P>
-__global__ void __launch_bounds__(64)
-k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ R,
-           double *__restrict__ Bal, int bal, Tab tb, Geo geo, Phys ph, int ne, int *__restrict__ flag) {
+template <int P, class Out>
+__device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, const double *__restrict__ T,
+                                              double *__restrict__ Bal, int bal, const Geo &geo, const Phys &ph,
+                                              int ne, int e, int *__restrict__ flag, Out out) {
   using S = H2<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
   __shared__ double Un[3 * Q2], gF[12 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
+  const Tab tb{};
   for (int v = 0; v < 3; ++v) {
     copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
     __syncthreads();
@@ -235,16 +256,23 @@ k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *_
       const double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
       const size_t o = ((size_t)v * ne + e) * L2 + k;
       if (bal == 0) {
-        R[o] = r;
+        out(o, r);
       } else {
         double b;
         if (bal == 1) { b = r; Bal[o] = r; }
         else b = Bal[o];
-        R[o] = -1.0 * b + r;
+        out(o, -1.0 * b + r);
       }
     }
     __syncthreads();
   }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ R,
+           double *__restrict__ Bal, int bal, Tab tb, Geo geo, Phys ph, int ne, int *__restrict__ flag) {
+  shallow2_elem<P>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
 }
 
 // lifted, projected gradients of (u, v, e): G[e][2*prim + dir][L2]
@@ -576,6 +604,74 @@ k_observe_adjoint(const double *__restrict__ U, double *__restrict__ Ubar, const
     __syncthreads();
   }
   if (tid == 0) *J += 0.5 * dt * jsum;
+}
+
+// Batched serving (dgx_server with batching): one launch per stage runs every
+// member request of a group (same order and physics).  A fixed grid of
+// workgroups walks the flat list of (request, element) work items; members
+// are in work-item order, b[r].off their first item.  The step kernel fuses
+// the LSRK14 register update of the direct path (vField axpby, axpy:
+// Ut = A Ut - dt R, U += B Ut) into the residual's store.
+template <int P, int NF>
+__global__ void __launch_bounds__(64)
+k_batch_traces2(const Batch2 *__restrict__ b, int nreq, int nwork) {
+  int r = 0;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    while (r + 1 < nreq && b[r + 1].off <= w) ++r;
+    traces2_elem<P>(b[r].U, b[r].T, b[r].ne, NF, w - b[r].off);
+  }
+}
+
+template <int P, bool SHALLOW>
+__global__ void __launch_bounds__(64)
+k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, double B) {
+  int r = 0;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    while (r + 1 < nreq && b[r + 1].off <= w) ++r;
+    double *U = b[r].U, *Ut = b[r].Ut;
+    const double mdt = -b[r].dt;
+    auto update = [=](size_t o, double rr) {
+      const double ut = fma(mdt, rr, A * Ut[o]);   // as the direct path's axpby
+      Ut[o] = ut;
+      U[o] = fma(B, ut, U[o]);
+    };
+    const Geo geo = b[r].geo;
+    const Phys ph = b[r].phys;
+    if (SHALLOW)
+      shallow2_elem<P>(U, b[r].T, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+    else
+      rhs2_elem<P, false>(U, b[r].T, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+  }
+}
+
+template <int P>
+static hipError_t launch_batch2_p(const BatchLaunch &a) {
+  const int grid = std::max(1, std::min(a.nwork, kBatchGrid));
+  if (a.op == 0) {
+    if (a.shallow)
+      hipLaunchKernelGGL((k_batch_traces2<P, 3>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
+    else
+      hipLaunchKernelGGL((k_batch_traces2<P, 4>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
+  } else {
+    if (a.shallow)
+      hipLaunchKernelGGL((k_batch_step2<P, true>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
+                         a.A, a.B);
+    else
+      hipLaunchKernelGGL((k_batch_step2<P, false>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
+                         a.A, a.B);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch2(const BatchLaunch &a) {
+  if (a.nreq < 1 || a.nwork < 1 || !a.b) return hipErrorInvalidValue;
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch_batch2_p<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9) DGX_CASE(10)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <int P>

@@ -8,6 +8,13 @@
 // device.  A slice is `steps_per_slice` steps: the first runs directly; the
 // second is captured into a hipGraph (no memset nodes: the stepping kernels
 // clear nothing), which later full slices replay.
+//
+// With batching, 2D Euler and shallow-water FP64 LSRK14 requests of the same
+// order join a group instead: each stage of a group is two launches (face
+// traces; residual with the fused register update) over the elements of all
+// its members, on fixed grids of workgroups that walk the members' elements.
+// Members step in lockstep slices of min(steps_per_slice, fewest remaining
+// steps); they join and leave at slice boundaries.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +26,7 @@
 #include <vector>
 
 #include "../../include/dgx.h"
+#include "dgx_domain.hpp"
 #include "dgx_kernels.hpp"
 
 namespace {
@@ -50,6 +58,21 @@ struct Request {
   hipGraphExec_t exec = nullptr;
   double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
   std::string msg;
+  struct Group *g = nullptr;   // batched group, if any
+};
+
+// Batched requests of one (physics, order): members, the device copy of their
+// descriptors, and the stream and event of the group's slices.
+struct Group {
+  int eqn = 0, p = 0;
+  std::vector<Request *> members;
+  std::vector<dgx::Batch2> hd;
+  dgx::Batch2 *dd = nullptr;
+  size_t cap = 0;
+  int nwork = 0, k = 0;
+  bool dirty = true, inflight = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
 };
 
 }  // namespace
@@ -61,6 +84,7 @@ struct dgx_server {
   std::deque<Request *> queue;
   std::vector<Request *> running;
   std::map<long long, std::unique_ptr<Request>> all;
+  std::vector<std::unique_ptr<Group>> groups;
   size_t used = 0, peak_bytes = 0;
   int peak_active = 0;
 
@@ -130,6 +154,94 @@ int enqueue_slice(dgx_server *s, Request *r) {
   return DGX_OK;
 }
 
+bool batchable(const dgx_server *s, const Request *r) {
+  const dgx_domain_desc &d = r->desc;
+  return s->cfg.batch && d.nsd == 2 && (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) &&
+         d.precision == DGX_FP64 && r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
+}
+
+// Join r (admitted, state uploaded) to the group of its physics and order.
+int join_group(dgx_server *s, Request *r) {
+  Group *g = nullptr;
+  for (auto &x : s->groups)
+    if (x->eqn == r->desc.eqntype && x->p == r->desc.p) g = x.get();
+  if (!g) {
+    auto ng = std::make_unique<Group>();
+    ng->eqn = r->desc.eqntype; ng->p = r->desc.p;
+    if (hipStreamCreate(&ng->stream) != hipSuccess) return sfail(DGX_ERR_DEVICE, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&ng->ev, hipEventDisableTiming) != hipSuccess)
+      return sfail(DGX_ERR_DEVICE, "hipEventCreate failed");
+    g = ng.get();
+    s->groups.push_back(std::move(ng));
+  }
+  int rc;
+  // the balance term of shallow water is the first residual: store it now, as
+  // the first stage would (its residual is then R(U0) - R(U0) = 0 there too)
+  if (r->h->bal == 1 && (rc = dgx_rhs(r->h))) return sfail(rc, dgx_last_error());
+  if (hipStreamSynchronize(r->stream) != hipSuccess) return sfail(DGX_ERR_DEVICE, "hipStreamSynchronize failed");
+  r->g = g;
+  g->members.push_back(r);
+  g->dirty = true;
+  return DGX_OK;
+}
+
+void leave_group(Request *r) {
+  Group *g = r->g;
+  if (!g) return;
+  g->members.erase(std::find(g->members.begin(), g->members.end(), r));
+  g->dirty = true;
+  r->g = nullptr;
+}
+
+// Enqueue the group's next slice: k LSRK14 steps of every member.
+int enqueue_group(dgx_server *s, Group *g) {
+  int k = s->cfg.steps_per_slice;
+  for (Request *r : g->members) k = std::min(k, r->nsteps - r->done_steps);
+  if (k <= 0) return DGX_OK;
+  if (g->dirty) {
+    g->hd.resize(g->members.size());
+    int off = 0;
+    for (size_t i = 0; i < g->members.size(); ++i) {
+      const dgx_domain *d = g->members[i]->h;
+      dgx::Batch2 &b = g->hd[i];
+      b.U = d->U; b.Ut = d->Ut; b.T = d->T; b.Bal = d->Bal; b.flag = d->d_flag;
+      b.geo = d->geo; b.phys = d->phys; b.ne = d->ne; b.off = off; b.bal = d->bal;
+      b.dt = g->members[i]->dt;
+      off += d->ne;
+    }
+    g->nwork = off;
+    if (g->hd.size() > g->cap) {
+      if (g->dd) (void)hipFree(g->dd);
+      g->cap = std::max<size_t>(g->hd.size(), 2 * g->cap);
+      if (hipMalloc((void **)&g->dd, g->cap * sizeof(dgx::Batch2)) != hipSuccess) {
+        g->dd = nullptr; g->cap = 0;
+        return sfail(DGX_ERR_MEMORY, "hipMalloc of the batch descriptors failed");
+      }
+    }
+    if (hipMemcpyAsync(g->dd, g->hd.data(), g->hd.size() * sizeof(dgx::Batch2), hipMemcpyHostToDevice,
+                       g->stream) != hipSuccess)
+      return sfail(DGX_ERR_DEVICE, "descriptor upload failed");
+    g->dirty = false;
+  }
+  dgx::BatchLaunch a;
+  a.p = g->p; a.nreq = (int)g->hd.size(); a.nwork = g->nwork; a.shallow = g->eqn == DGX_SHALLOW;
+  a.b = g->dd; a.stream = g->stream;
+  for (int step = 0; step < k; ++step)
+    for (int st = 0; st < 14; ++st) {
+      a.op = 0;
+      hipError_t e = dgx::launch_batch2(a);
+      if (e == hipSuccess) {
+        a.op = 1; a.A = dgx::kLsrkA[st]; a.B = dgx::kLsrkB[st];
+        e = dgx::launch_batch2(a);
+      }
+      if (e != hipSuccess) return sfail(DGX_ERR_DEVICE, std::string("batched launch: ") + hipGetErrorString(e));
+    }
+  if (hipEventRecord(g->ev, g->stream) != hipSuccess) return sfail(DGX_ERR_DEVICE, "hipEventRecord failed");
+  g->k = k;
+  g->inflight = true;
+  return DGX_OK;
+}
+
 // Final state to host memory, flag check, resources freed.
 int finish(dgx_server *s, Request *r) {
   int rc = dgx_synchronize(r->h);
@@ -161,9 +273,16 @@ int dgx_server_create(const dgx_server_config *cfg, dgx_server_handle *out) {
 
 int dgx_server_destroy(dgx_server_handle s) {
   if (!s) return DGX_OK;
+  for (auto &g : s->groups)
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
   for (Request *r : s->running) {
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     release(r);
+  }
+  for (auto &g : s->groups) {
+    if (g->dd) (void)hipFree(g->dd);
+    if (g->ev) (void)hipEventDestroy(g->ev);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
   }
   delete s;
   return DGX_OK;
@@ -210,11 +329,28 @@ int dgx_server_submit(dgx_server_handle s, const dgx_domain_desc *desc, const do
 
 int dgx_server_progress(dgx_server_handle s, int *pending) {
   if (!s) return sfail(DGX_ERR_ARGUMENT, "null handle");
+  // finished group slices
+  for (auto &g : s->groups) {
+    if (!g->inflight) continue;
+    hipError_t e = hipEventQuery(g->ev);
+    if (e == hipErrorNotReady) continue;
+    g->inflight = false;
+    for (Request *r : g->members) {
+      if (e == hipSuccess) r->done_steps += g->k;
+      else r->msg = std::string("batched slice failed: ") + hipGetErrorString(e);
+    }
+  }
   // retire finished slices and requests
   for (size_t i = 0; i < s->running.size();) {
     Request *r = s->running[i];
     bool retire = false;
-    if (r->inflight) {
+    if (r->g) {
+      if (!r->msg.empty()) {
+        leave_group(r);
+        fail_request(s, r, r->msg);
+        retire = true;
+      }
+    } else if (r->inflight) {
       hipError_t e = hipEventQuery(r->ev);
       if (e == hipSuccess) {
         r->inflight = false;
@@ -224,7 +360,9 @@ int dgx_server_progress(dgx_server_handle s, int *pending) {
         retire = true;
       }
     }
-    if (!retire && !r->inflight && r->done_steps >= r->nsteps) {
+    const bool busy = r->g ? r->g->inflight : r->inflight;
+    if (!retire && !busy && r->done_steps >= r->nsteps) {
+      leave_group(r);
       if (finish(s, r)) fail_request(s, r, s_err);
       retire = true;
     }
@@ -241,7 +379,8 @@ int dgx_server_progress(dgx_server_handle s, int *pending) {
     if ((int)s->running.size() >= s->cfg.max_active || s->used + r->bytes > s->cfg.hbm_budget) break;
     s->queue.pop_front();
     r->t_admit = s->now();
-    if (admit(s, r)) {
+    if (admit(s, r) || (batchable(s, r) && join_group(s, r))) {
+      leave_group(r);
       fail_request(s, r, s_err);
       continue;
     }
@@ -251,10 +390,22 @@ int dgx_server_progress(dgx_server_handle s, int *pending) {
     if (s->used > s->peak_bytes) s->peak_bytes = s->used;
     if ((int)s->running.size() > s->peak_active) s->peak_active = (int)s->running.size();
   }
-  // next slice of every admitted request with none in flight
+  // next slice of every group and every unbatched request with none in flight
+  for (auto &g : s->groups) {
+    if (g->inflight || g->members.empty()) continue;
+    if (enqueue_group(s, g.get())) {
+      const std::string msg = s_err;
+      for (Request *r : std::vector<Request *>(g->members)) {
+        leave_group(r);
+        fail_request(s, r, msg);
+        s->used -= r->bytes;
+        s->running.erase(std::find(s->running.begin(), s->running.end(), r));
+      }
+    }
+  }
   for (size_t i = 0; i < s->running.size();) {
     Request *r = s->running[i];
-    if (!r->inflight && r->done_steps < r->nsteps && enqueue_slice(s, r)) {
+    if (!r->g && !r->inflight && r->done_steps < r->nsteps && enqueue_slice(s, r)) {
       fail_request(s, r, s_err);
       s->used -= r->bytes;
       s->running.erase(s->running.begin() + i);

@@ -190,8 +190,13 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes);
  *  - The first slice of a request runs directly.
  *  - The second slice is captured into a hipGraph, which later full slices
  *    replay, so small meshes are not bound by launch overhead.
- *  - A request's result does not depend on what else runs: each request's
- *    kernels see only its own buffers.
+ *  - With `batch`, 2D Euler and shallow-water FP64 LSRK14 requests of one
+ *    order form a group: each stage is one face-trace launch and one
+ *    residual-and-update launch over the elements of every member, on a fixed
+ *    grid of workgroups that walk the members' elements.  Members step in
+ *    lockstep slices and join or leave at slice boundaries.
+ *  - A request's result does not depend on what else runs: each element's
+ *    arithmetic sees only its own request's buffers.
  *
  * Not thread safe: one host thread drives a server. */
 typedef struct dgx_server *dgx_server_handle;
@@ -201,6 +206,8 @@ typedef struct {
   int max_active;        /* admitted requests at once */
   int steps_per_slice;   /* steps enqueued per request and pass */
   int use_graphs;        /* replay slices from a captured hipGraph (1) or launch directly (0) */
+  int batch;             /* 1: batch 2D Euler / shallow-water FP64 LSRK14 requests of one order
+                            into shared launches (one per stage and kind for all of them) */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_QUEUED = 0, DGX_RUNNING = 1, DGX_DONE = 2, DGX_FAILED = 3 };
