@@ -191,7 +191,8 @@ int upload_all_tables() {
       (e = dgx::upload_tables_3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_c2d(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -244,13 +245,23 @@ int launch(dgx_domain *d, int op, const double *U, double *out) {
   return DGX_OK;
 }
 
+dgx::CurvedArgs curved_args(dgx_domain *d) {
+  dgx::CurvedArgs ca;
+  ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
+  ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
+  ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
+  return ca;
+}
+
+hipError_t launch_curved(const dgx_domain *d, const dgx::CurvedArgs &ca) {
+  return d->nsd == 2 ? dgx::launch_curved2(ca) : dgx::launch_curved3(ca);
+}
+
 int transform(dgx_domain *d, int op, const double *x, double *y) {
   if (d->curved && op == dgx::OP_FORWARD) {   // projection through the curved mass matrix
-    dgx::CurvedArgs ca;
-    ca.U = x; ca.R = y; ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
-    ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
-    ca.project = true; ca.stream = d->stream;
-    hipError_t e = dgx::launch_curved3(ca);
+    dgx::CurvedArgs ca = curved_args(d);
+    ca.U = x; ca.R = y; ca.project = true;
+    hipError_t e = launch_curved(d, ca);
     if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved projection: ") + hipGetErrorString(e));
     return DGX_OK;
   }
@@ -299,11 +310,9 @@ int residual(dgx_domain *d, const double *U, double *R) {
   }
   if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
   if (d->curved) {
-    dgx::CurvedArgs ca;
-    ca.U = U; ca.T = d->T; ca.R = R; ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav;
-    ca.geo = d->geo; ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
-    ca.stream = d->stream;
-    hipError_t e = timed(d, 1, [&] { return dgx::launch_curved3(ca); });
+    dgx::CurvedArgs ca = curved_args(d);
+    ca.U = U; ca.T = d->T; ca.R = R;
+    hipError_t e = timed(d, 1, [&] { return launch_curved(d, ca); });
     if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved residual: ") + hipGetErrorString(e));
     ++d->stages;
     return DGX_OK;
@@ -327,6 +336,89 @@ int ensure_nodal(dgx_domain *d) {
 // normals and J_side/J_vol at the Gauss points from them as
 // Hex::compute_side_geometry (src/Hex.cpp:1385-1494), and per element the mean
 // J that starts the kernel's mass-matrix solve.
+// Curved quads from nodal coordinates xn[2][ne][Q^2]: metrics and J at the
+// nodes as Quad::compute_curved_geometry (src/Quad.cpp:1309-1363), the edge
+// normals and J_side/J_vol at the Gauss points from them as
+// Quad::compute_side_geometry (src/Quad.cpp:1367-1452), and per element the
+// mean J that starts the kernel's mass-matrix solve.
+int curved_geometry2(dgx_domain *d, const double *xn) {
+  const int ne = d->ne, Q = d->Q, QF = d->QF, Q2 = d->nq;
+  const HostBasis &b = d->hb;
+  const std::vector<double> &D = b.D, &w = b.w, &Ig = b.Ig;
+  std::vector<double> gm((size_t)ne * 5 * Q2), fgeo((size_t)ne * 4 * QF * 4), jav(ne);
+  std::vector<double> xr(Q2), xs(Q2), yr(Q2), ys(Q2), J(Q2), mr[2] = {std::vector<double>(Q2), std::vector<double>(Q2)},
+      ms[2] = {std::vector<double>(Q2), std::vector<double>(Q2)};
+  std::vector<double> ex(Q), ey(Q), en0(Q), en1(Q), ej(Q), dl(Q);
+  auto gauss = [&](const std::vector<double> &f, double *to) {
+    for (int a = 0; a < QF; ++a) {
+      double t = 0.0;
+      for (int m = 0; m < Q; ++m) t += Ig[a * Q + m] * f[m];
+      to[a] = t;
+    }
+  };
+  for (int e = 0; e < ne; ++e) {
+    const double *X = xn + (size_t)e * Q2, *Y = xn + ((size_t)ne + e) * Q2;
+    for (int i = 0; i < Q; ++i)
+      for (int j = 0; j < Q; ++j) {
+        double a0 = 0, a1 = 0, c0 = 0, c1 = 0;
+        for (int m = 0; m < Q; ++m) {
+          a0 += D[i * Q + m] * X[m * Q + j]; a1 += D[j * Q + m] * X[i * Q + m];
+          c0 += D[i * Q + m] * Y[m * Q + j]; c1 += D[j * Q + m] * Y[i * Q + m];
+        }
+        const int n = i * Q + j;
+        xr[n] = a0; xs[n] = a1; yr[n] = c0; ys[n] = c1;
+      }
+    double sw = 0.0, swJ = 0.0;
+    for (int n = 0; n < Q2; ++n) {
+      const double Jn = xr[n] * ys[n] - xs[n] * yr[n];
+      if (!(Jn > 0.0)) return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has a nonpositive Jacobian");
+      J[n] = Jn;
+      mr[0][n] = ys[n] / Jn; mr[1][n] = -(xs[n] / Jn);
+      ms[0][n] = -(yr[n] / Jn); ms[1][n] = xr[n] / Jn;
+      const int i = n / Q, j = n % Q;
+      const double wJ = J[n] * w[i] * w[j];
+      sw += w[i] * w[j]; swJ += wJ;
+      double *g = gm.data() + (size_t)e * 5 * Q2;
+      g[0 * Q2 + n] = wJ * mr[0][n]; g[1 * Q2 + n] = wJ * mr[1][n];
+      g[2 * Q2 + n] = wJ * ms[0][n]; g[3 * Q2 + n] = wJ * ms[1][n];
+      g[4 * Q2 + n] = wJ;
+    }
+    jav[e] = swJ / sw;
+    for (int s = 0; s < 4; ++s) {
+      const std::vector<double> *M = (s == 0 || s == 2) ? ms : mr;
+      for (int t = 0; t < Q; ++t) {
+        const int n = s == 0 ? t * Q : (s == 1 ? (Q - 1) * Q + t : (s == 2 ? t * Q + Q - 1 : t));
+        en0[t] = M[0][n]; en1[t] = M[1][n]; ej[t] = J[n]; ex[t] = X[n]; ey[t] = Y[n];
+      }
+      double gx[16], gy[16], gj[16], gl[16];
+      gauss(en0, gx); gauss(en1, gy); gauss(ej, gj);
+      for (int t = 0; t < Q; ++t) {
+        double dx = 0, dy = 0;
+        for (int m = 0; m < Q; ++m) { dx += D[t * Q + m] * ex[m]; dy += D[t * Q + m] * ey[m]; }
+        dl[t] = std::sqrt(dx * dx + dy * dy);
+      }
+      gauss(dl, gl);
+      const double sg = (s == 0 || s == 3) ? -1.0 : 1.0;
+      for (int a = 0; a < QF; ++a) {
+        const double nx = sg * gx[a], ny = sg * gy[a], len = std::sqrt(nx * nx + ny * ny);
+        double *o = fgeo.data() + (((size_t)e * 4 + s) * QF + a) * 4;
+        o[0] = nx / len; o[1] = ny / len; o[2] = 0.0; o[3] = gl[a] / gj[a];
+      }
+    }
+  }
+  int rc;
+  if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
+    return rc;
+  HIPCHK(hipMemcpy(d->d_gm, gm.data(), gm.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_fgeo, fgeo.data(), fgeo.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void **)&d->d_iters, 64));
+  HIPCHK(hipMemset(d->d_iters, 0, 64));
+  std::copy(xn, xn + (size_t)2 * ne * Q2, d->coords.begin());
+  d->curved = true;
+  return DGX_OK;
+}
+
 int curved_geometry(dgx_domain *d, const double *xn) {
   const int ne = d->ne, Q = d->Q, QF = d->QF, Q3 = d->nq, QF2 = QF * QF, Q2 = Q * Q;
   const HostBasis &b = d->hb;
@@ -640,7 +732,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   };
   std::map<std::vector<uint16_t>, int> perms;
   std::vector<uint16_t> permtab;
-  std::vector<uint8_t> permid((size_t)ne * d->ns);
+  std::vector<uint8_t> permid((size_t)ne * d->ns), smaster((size_t)ne * d->ns);
   std::vector<int> nbr((size_t)ne * d->ns), nside((size_t)ne * d->ns);
   for (int e = 0; e < ne; ++e)
     for (int s = 0; s < d->ns; ++s) {
@@ -662,6 +754,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
         permtab.insert(permtab.end(), m.begin(), m.end());
       } else id = it->second;
       permid[(size_t)e * d->ns + s] = (uint8_t)id;
+      smaster[(size_t)e * d->ns + s] = (uint8_t)(S.bc >= 0 || S.master);
       nbr[(size_t)e * d->ns + s] = S.bc >= 0 ? -1 - S.bc : S.le;   // boundary: -1 - type
       nside[(size_t)e * d->ns + s] = S.bc >= 0 ? 0 : S.ls;
     }
@@ -691,15 +784,18 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   HIPCHK(hipMalloc((void **)&d->d_nbr, nbr.size() * sizeof(int)));
   HIPCHK(hipMalloc((void **)&d->d_nside, nside.size() * sizeof(int)));
   HIPCHK(hipMalloc((void **)&d->d_permid, permid.size()));
+  HIPCHK(hipMalloc((void **)&d->d_master, smaster.size()));
   HIPCHK(hipMalloc((void **)&d->d_perm, permtab.size() * sizeof(uint16_t)));
   HIPCHK(hipMalloc((void **)&d->d_flag, 64));
   HIPCHK(hipMemcpy(d->d_nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_nside, nside.data(), nside.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_permid, permid.data(), permid.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_master, smaster.data(), smaster.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_perm, permtab.data(), permtab.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(d->d_flag, 0, 64));
   d->geo.metric = d->d_metric; d->geo.fnorm = d->d_fnorm; d->geo.fjr = d->d_fjr;
   d->geo.nbr = d->d_nbr; d->geo.nside = d->d_nside; d->geo.permid = d->d_permid; d->geo.perm = d->d_perm;
+  d->geo.master = d->d_master;
   d->phys.gamma = d->gamma; d->phys.vis = d->vis; d->phys.Pr = d->Pr; d->phys.flux = d->flux;
   d->phys.visc = desc->viscous;
   for (int v = 0; v < 5; ++v) d->phys.bcU[v] = desc->bc_state[v];
@@ -717,7 +813,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     if ((rc = alloc(&d->Bal, n))) return rc;
     d->bal = 1;
   }
-  if (desc->xnodes && (rc = curved_geometry(d, desc->xnodes))) return rc;
+  if (desc->xnodes && (rc = (nsd == 2 ? curved_geometry2(d, desc->xnodes) : curved_geometry(d, desc->xnodes))))
+    return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
   d->mixed = d->prec == DGX_MIXED;
@@ -740,7 +837,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal};
+                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -774,7 +871,8 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   b += nf * ne * nq * 8;                                    // nodal transfer buffer
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
   b += 8 * qs * 2 + (size_t)(1 << 20);                      // orientation tables, tables, rounding
-  if (desc->xnodes) b += ne * (10 * nq + 24 * qs + 1) * 8;   // curved: nodal metrics, face geometry
+  if (desc->xnodes)   // curved: nodal metrics, face geometry
+    b += nsd == 2 ? ne * (5 * nq + 16 * qs + 1) * 8 : ne * (10 * nq + 24 * qs + 1) * 8;
   if (shallow && !desc->no_balance) b += nf * ne * nm * 8;    // balance term
   *bytes = b;
   return DGX_OK;
@@ -805,9 +903,10 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
-  if (desc->xnodes && !(desc->nsd == 3 && desc->eqntype == DGX_EULER3D && desc->precision == DGX_FP64 &&
-                        dgx::curved3_supported(desc->p)))
-    return fail(DGX_ERR_ARGUMENT, "curved elements (xnodes) are available for Euler3d, DGX_FP64, p <= 6");
+  if (desc->xnodes && !(desc->precision == DGX_FP64 &&
+                        ((desc->nsd == 3 && desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p)) ||
+                         (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_SHALLOW)))))
+    return fail(DGX_ERR_ARGUMENT, "curved elements (xnodes) are available for Euler3d (p <= 6), Euler and Shallow, DGX_FP64");
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
       if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)

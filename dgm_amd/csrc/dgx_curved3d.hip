@@ -4,9 +4,10 @@
 //   volume term   Euler3d::convection with Hex::gradient on curved geometry:
 //                 dF_x/dx = sum_d (dxi_d/dx)(node) D_d F_x at every node
 //                 (src/Euler3d.cpp:468-575, src/Hex.cpp:2030-2071, 3992-4080)
-//   faces         the Riemann flux with the normal of each Gauss point, lifted
-//                 at the face nodes: G2GL((Fhat - F-.n) J_side/J_vol) / w_end
-//                 (src/Hex.cpp:1786-1952)
+//   faces         the Riemann flux with the master side's normal at each Gauss
+//                 point (Euler3d::convective_flux evaluates a pair once, on its
+//                 master side), lifted at the face nodes:
+//                 G2GL((Fhat - F-.n) J_side/J_vol) / w_end (src/Hex.cpp:1786-1952)
 //   project       b = B^T (wJ f) and M x = b with M = B^T diag(wJ) B, the curved
 //                 element's mass matrix (src/Hex.cpp:289-318), solved by
 //                 conjugate gradients inside the workgroup: M is symmetric
@@ -108,7 +109,8 @@ k_rhs3c(CurvedArgs a) {
     const int s = i / QF2, q = i - s * QF2;
     const int es = e * 6 + s;
     const double *fg = a.fgeo + ((size_t)es * QF2 + q) * 4;
-    const double nx = fg[0], ny = fg[1], nz = fg[2], jr = fg[3];
+    double nx = fg[0], ny = fg[1], nz = fg[2];
+    const double jr = fg[3];
     const int nb = geo.nbr[es], ns = geo.nside[es];
     double UL[5], UR[5], FL[5], Fh[5];
     const double *tl = a.T + ((size_t)e * 6 + s) * 5 * QF2;
@@ -116,6 +118,11 @@ k_rhs3c(CurvedArgs a) {
     for (int v = 0; v < 5; ++v) UL[v] = tl[v * QF2 + q];
     if (nb >= 0) {
       const int qn = geo.perm[geo.permid[es] * QF2 + q];
+      if (!geo.master[es]) {   // the pair's flux uses the master side's normal (both sides' normals
+                               // come from their own interpolated metrics and differ slightly)
+        const double *fm = a.fgeo + (((size_t)nb * 6 + ns) * QF2 + qn) * 4;
+        nx = -fm[0]; ny = -fm[1]; nz = -fm[2];
+      }
       const double *tr = a.T + ((size_t)nb * 6 + ns) * 5 * QF2;
 #pragma unroll
       for (int v = 0; v < 5; ++v) UR[v] = tr[v * QF2 + qn];

@@ -67,6 +67,7 @@ struct Geo {
   const int *nside;              // [ne][nsides] neighbour side
   const uint8_t *permid;         // [ne][nsides] row of `perm`
   const uint16_t *perm;          // [nperm][QF^(d-1)] own point -> neighbour point
+  const uint8_t *master;         // [ne][nsides] 1: master or boundary side, 0: slave
 };
 
 struct Phys {

@@ -44,7 +44,7 @@ struct dgx_domain {
   // device
   double *d_tab = nullptr, *d_metric = nullptr, *d_fnorm = nullptr, *d_fjr = nullptr;
   int *d_nbr = nullptr, *d_nside = nullptr, *d_flag = nullptr;
-  uint8_t *d_permid = nullptr;
+  uint8_t *d_permid = nullptr, *d_master = nullptr;
   uint16_t *d_perm = nullptr;
   double *U = nullptr, *Ut = nullptr, *R0 = nullptr, *R1 = nullptr, *T = nullptr;
   double *G = nullptr, *V = nullptr, *TV = nullptr, *nodal = nullptr, *T2 = nullptr;

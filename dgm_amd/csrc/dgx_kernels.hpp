@@ -106,8 +106,13 @@ struct CurvedArgs {
   int ne = 0, p = 0;
   int *flag = nullptr, *iters = nullptr;
   bool project = false;                      // U nodal [5][ne][Q^3] -> R modal: M^-1 B^T (wJ U)
+  bool shallow = false;                      // quads: shallow water (3 fields) instead of Euler
   hipStream_t stream = nullptr;
 };
+// Quads use the same arguments with gm [ne][5][Q^2] (wJ dxi_d/dx_x, wJ),
+// fgeo [ne][4][QF][4] (normal x, y, 0, J_side/J_vol) and NF = 3 or 4 fields.
+hipError_t launch_curved2(const CurvedArgs &a);
+hipError_t upload_tables_c2d(const double *h);
 bool curved3_supported(int p);
 hipError_t launch_curved3(const CurvedArgs &a);
 hipError_t upload_tables_c3d(const double *h);

@@ -156,7 +156,7 @@ int enqueue_slice(dgx_server *s, Request *r) {
 
 bool batchable(const dgx_server *s, const Request *r) {
   const dgx_domain_desc &d = r->desc;
-  return s->cfg.batch && d.nsd == 2 && (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) &&
+  return s->cfg.batch && d.nsd == 2 && !d.xnodes && (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) &&
          d.precision == DGX_FP64 && r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
 }
 

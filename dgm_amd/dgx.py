@@ -113,10 +113,10 @@ class GPUDomain:
         self.p = int(p)
 
     def _nodes(self, xnodes, mesh, p):
-        q = int(p) + 2
+        q, nsd = int(p) + 2, int(mesh.nsd)
         self._xn = np.ascontiguousarray(xnodes, dtype=np.float64)
-        if self._xn.shape != (3, int(mesh.ne), q ** 3):
-            raise ValueError(f"xnodes shape {self._xn.shape} != {(3, int(mesh.ne), q ** 3)}")
+        if self._xn.shape != (nsd, int(mesh.ne), q ** nsd):
+            raise ValueError(f"xnodes shape {self._xn.shape} != {(nsd, int(mesh.ne), q ** nsd)}")
         return self._xn
 
     def close(self):

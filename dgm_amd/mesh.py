@@ -131,10 +131,11 @@ def hex_nodes(mesh: Mesh, p: int) -> np.ndarray:
 
 
 def mapped_nodes(mesh: Mesh, p: int, fn) -> np.ndarray:
-    """Curved hexes: the GLL nodes of the affine mesh moved by fn(x, y, z) ->
-    (x', y', z'), for the curved-element path (dgx_domain_desc.xnodes)."""
-    x = hex_nodes(mesh, p)
-    return np.stack(fn(x[0], x[1], x[2]))
+    """Curved elements: the GLL nodes of the affine mesh moved by fn(x, y, z)
+    -> (x', y', z') (hexes) or fn(x, y) -> (x', y') (quads), for the
+    curved-element path (dgx_domain_desc.xnodes)."""
+    x = hex_nodes(mesh, p) if mesh.nsd == 3 else quad_nodes(mesh, p)
+    return np.stack(fn(*x))
 
 
 def bump_map(lo, hi, amp):

@@ -33,6 +33,7 @@
 //                                                 parity unpinned (runs/chan is not reproduced)
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
 //   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
+//   src/Quad.cpp:263-300, 1309-1452               curved quads: nodal metrics, edges, mass matrix
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
 //                                                 against the reference is unpinned; the
 //                                                 BR2 tests pin it by invariants
@@ -738,11 +739,11 @@ struct Domain {
     const Elem &E = el[e];
     std::vector<double> fr(qtot), fs(qtot), ft(qtot);
     grad_r(f, fr.data(), fs.data(), ft.data());
-    if (E.curved) {   // metrics at the nodes (src/Hex.cpp:2030-2071 with geom arrays)
+    if (E.curved) {   // metrics at the nodes (src/Quad.cpp:2309-2330, src/Hex.cpp:2030-2071 with geom arrays)
       const int c = dir == 'x' ? 0 : (dir == 'y' ? 1 : 2);
       for (int i = 0; i < qtot; ++i) df[i] = E.mn[c][i] * fr[i];
       for (int i = 0; i < qtot; ++i) df[i] = E.mn[3 + c][i] * fs[i] + df[i];
-      for (int i = 0; i < qtot; ++i) df[i] = E.mn[6 + c][i] * ft[i] + df[i];
+      if (nsd == 3) for (int i = 0; i < qtot; ++i) df[i] = E.mn[6 + c][i] * ft[i] + df[i];
       return;
     }
     double a, b, c = 0;
@@ -756,7 +757,8 @@ struct Domain {
   void gradient_all(int e, const double *f, double *fx, double *fy, double *fz) const {
     const Elem &E = el[e];
     if (E.curved) {
-      gradient_dir(e, f, fx, 'x'); gradient_dir(e, f, fy, 'y'); gradient_dir(e, f, fz, 'z');
+      gradient_dir(e, f, fx, 'x'); gradient_dir(e, f, fy, 'y');
+      if (nsd == 3) gradient_dir(e, f, fz, 'z');
       return;
     }
     std::vector<double> fr(qtot), fs(qtot), ft(qtot);
@@ -846,16 +848,28 @@ struct Domain {
         }
     E.curved = true;
     face_geometry3(E);
-    // mass matrix M[a][b] = sum_n phi_a(n) wJ(n) phi_b(n), phi from B (L x q per direction)
+    mass_factor(E);
+  }
+  // Cholesky factor of the element mass matrix M[a][b] = sum_n phi_a(n) wJ(n)
+  // phi_b(n), phi from B (L x q per direction) (src/Quad.cpp:263-300,
+  // src/Hex.cpp:289-318 with DGM_CHOLESKY_MASS_MATRIX)
+  void mass_factor(Elem &E) {
     std::vector<double> Phi((size_t)nm * qtot);
-    for (int a = 0; a < L; ++a)
-      for (int b = 0; b < L; ++b)
-        for (int c = 0; c < L; ++c)
+    if (nsd == 2) {
+      for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b)
           for (int i = 0; i < q; ++i)
-            for (int j = 0; j < q; ++j)
-              for (int k = 0; k < q; ++k)
-                Phi[(size_t)((a * L + b) * L + c) * qtot + (i * q + j) * q + k] =
-                    B[a * q + i] * B[b * q + j] * B[c * q + k];
+            for (int j = 0; j < q; ++j) Phi[(size_t)(a * L + b) * qtot + i * q + j] = B[a * q + i] * B[b * q + j];
+    } else {
+      for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b)
+          for (int c = 0; c < L; ++c)
+            for (int i = 0; i < q; ++i)
+              for (int j = 0; j < q; ++j)
+                for (int k = 0; k < q; ++k)
+                  Phi[(size_t)((a * L + b) * L + c) * qtot + (i * q + j) * q + k] =
+                      B[a * q + i] * B[b * q + j] * B[c * q + k];
+    }
     std::vector<double> M((size_t)nm * nm, 0.0);
     for (int a = 0; a < nm; ++a)
       for (int b = 0; b <= a; ++b) {
@@ -875,6 +889,62 @@ struct Domain {
           E.chol[(size_t)i * nm + j] = t / E.chol[(size_t)j * nm + j];
         }
       }
+  }
+  // curved quad from nodal coordinates: Quad::compute_curved_geometry
+  // (src/Quad.cpp:1309-1363) at the nodes, the edge geometry from those metrics
+  // (Quad::compute_side_geometry, src/Quad.cpp:1367-1452) and the mass matrix
+  void set_curved2(int e, const double *x, const double *y) {
+    Elem &E = el[e];
+    E.cx.assign(x, x + qtot); E.cy.assign(y, y + qtot);
+    std::vector<double> xr(qtot), xs(qtot), yr(qtot), ys(qtot), dummy(qtot);
+    grad_r(x, xr.data(), xs.data(), dummy.data());
+    grad_r(y, yr.data(), ys.data(), dummy.data());
+    E.Jn.assign(qtot, 0.0);
+    for (int c = 0; c < 9; ++c) E.mn[c].assign(qtot, 0.0);
+    for (int i = 0; i < qtot; ++i) {
+      double J = xs[i] * yr[i];
+      J = -1.0 * J;
+      J = xr[i] * ys[i] + J;
+      if (!(J > 0.0)) status |= 4;
+      E.Jn[i] = J;
+      E.mn[0][i] = ys[i] / J;            // rx
+      E.mn[1][i] = -1.0 * (xs[i] / J);   // ry
+      E.mn[3][i] = -1.0 * (yr[i] / J);   // sx
+      E.mn[4][i] = xr[i] / J;            // sy
+    }
+    E.wJ = E.Jn;
+    for (int i = 0; i < q; ++i) for (int j = 0; j < q; ++j) E.wJ[i * q + j] *= w[i];
+    for (int i = 0; i < q; ++i) for (int j = 0; j < q; ++j) E.wJ[i * q + j] *= w[j];
+    E.curved = true;
+    std::vector<double> wkx(q), wky(q), tmp(q), dx(q), dy(q);
+    for (int s = 0; s < 4; ++s) {
+      SideInfo &S = E.side[s];
+      S.nx.assign(qf, 0.0); S.ny.assign(qf, 0.0); S.J.assign(qf, 0.0);
+      const int row = (s == 0 || s == 2) ? 3 : 0;
+      get_side(E.mn[row].data(), s, wkx.data());
+      get_side(E.mn[row + 1].data(), s, wky.data());
+      to_side(wkx.data(), S.nx.data());
+      to_side(wky.data(), S.ny.data());
+      if (s == 0 || s == 3)
+        for (int g = 0; g < qf; ++g) { S.nx[g] *= -1.0; S.ny[g] *= -1.0; }
+      std::vector<double> iJ(qf);
+      get_side(E.Jn.data(), s, tmp.data());
+      to_side(tmp.data(), iJ.data());
+      for (int g = 0; g < qf; ++g) {
+        const double len = std::sqrt(S.nx[g] * S.nx[g] + S.ny[g] * S.ny[g]);
+        S.nx[g] /= len; S.ny[g] /= len;
+      }
+      get_side(E.cx.data(), s, tmp.data());
+      mm(q, 1, q, D.data(), q, tmp.data(), 1, dx.data(), 1);
+      get_side(E.cy.data(), s, tmp.data());
+      mm(q, 1, q, D.data(), q, tmp.data(), 1, dy.data(), 1);
+      for (int i = 0; i < q; ++i) dx[i] = dx[i] * dx[i];
+      for (int i = 0; i < q; ++i) dx[i] = dy[i] * dy[i] + dx[i];
+      for (int i = 0; i < q; ++i) tmp[i] = std::sqrt(dx[i]);
+      to_side(tmp.data(), S.J.data());
+      for (int g = 0; g < qf; ++g) S.J[g] = S.J[g] / iJ[g];
+    }
+    mass_factor(E);
   }
   void backward(const double *fh, double *f) const {
     if (nsd == 2) {
@@ -1779,13 +1849,15 @@ int orc_riemann(int nsd, int flux, double gamma, const double *Ul, const double 
   if (nsd == 2) return point_flux2d(gamma, flux, Ul, Ur, n[0], n[1], Fpl, Fmr, Fh);
   return point_flux3d(gamma, flux, Ul, Ur, n[0], n[1], n[2], Fpl, Fmr, Fh);
 }
-// curved hexes: nodal coordinates [3][ne][qtot] at the GLL points replace the
-// affine map of every element
+// curved elements: nodal coordinates [nsd][ne][qtot] at the GLL points replace
+// the affine map of every element
 void orc_set_nodes(void *h, const double *xyz) {
   Domain *d = (Domain *)h;
   const size_t n = (size_t)d->qtot, ne = (size_t)d->ne;
-  for (int e = 0; e < d->ne; ++e)
-    d->set_curved(e, xyz + (0 * ne + e) * n, xyz + (1 * ne + e) * n, xyz + (2 * ne + e) * n);
+  for (int e = 0; e < d->ne; ++e) {
+    if (d->nsd == 2) d->set_curved2(e, xyz + (0 * ne + e) * n, xyz + (1 * ne + e) * n);
+    else d->set_curved(e, xyz + (0 * ne + e) * n, xyz + (1 * ne + e) * n, xyz + (2 * ne + e) * n);
+  }
 }
 // conservative state of the state boundary condition (3D Euler)
 void orc_set_bc_state(void *h, const double *U) {

@@ -94,9 +94,9 @@ class Oracle:
             pass
 
     def set_nodes(self, xyz):
-        """Curved hexes: nodal coordinates [3][ne][qtot] at the GLL points."""
+        """Curved elements: nodal coordinates [nsd][ne][qtot] at the GLL points."""
         self._xyz = np.ascontiguousarray(xyz, dtype=np.float64)
-        assert self._xyz.shape == (3, self.ne, self.qtot), self._xyz.shape
+        assert self._xyz.shape == (self.nsd, self.ne, self.qtot), self._xyz.shape
         self.lib.orc_set_nodes(self.h, _ptr(self._xyz))
 
     def set_bc_state(self, U):

@@ -103,3 +103,80 @@ def test_curved_residual_converges_spectrally():
     # a wrong normal, metric or mass matrix would leave an O(1) inconsistency
     flat = _consistency_error(5, n=4, amp=0.0)
     assert errs[1] < 1.5 * flat, (errs[1], flat)
+
+
+# ---- curved quads (Quad::compute_curved_geometry, src/Quad.cpp:1309-1452) ----
+
+def _state2(x, y):
+    rho = 1.0 + 0.1 * np.sin(x) * np.cos(y)
+    u = 0.3 * np.sin(x) * np.cos(y)
+    v = -0.3 * np.cos(x) * np.sin(y)
+    p = 1.0 + 0.1 * np.cos(2 * x) * np.cos(y)
+    E = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v)
+    return np.stack([rho, rho * u, rho * v, E])
+
+
+def _flux_div2(x, y, h=1e-5):
+    def F(x, y):
+        r, mx, my, E = _state2(x, y)
+        u, v = mx / r, my / r
+        p = (GAMMA - 1) * (E - 0.5 * (mx * u + my * v))
+        return np.stack([mx, mx * u + p, my * u, (E + p) * u]), np.stack([my, mx * v, my * v + p, (E + p) * v])
+    return (F(x + h, y)[0] - F(x - h, y)[0]) / (2 * h) + (F(x, y + h)[1] - F(x, y - h)[1]) / (2 * h)
+
+
+def wavy_map2(L, amp):
+    def fn(x, y):
+        s = lambda t: np.sin(2 * np.pi * t / L)
+        return x + amp * s(y) * s(x), y + amp * s(x) * s(y) * 0.7
+    return fn
+
+
+def test_quads_affine_nodes_reproduce_affine_path():
+    m = meshgen.periodic_quads(3, 2, 0.0, 2 * np.pi, 0.0, 1.5)
+    for eqn, kw in ((1, dict(flux=0)), (1, dict(flux=1)), (2, dict(vis=0.05)), (30, {})):
+        a, b = Oracle(m, 5, eqn, **kw), Oracle(m, 5, eqn, **kw)
+        if eqn == 30:
+            a.set_balance(False); b.set_balance(False)
+        b.set_nodes(meshgen.quad_nodes(m, 5))
+        x = a.coords()
+        U = _state2(x[0], x[1])
+        if eqn == 30:
+            U = np.stack([1.0 + 0.1 * np.sin(x[0]), 0.2 * np.cos(x[1]), 0.1 + 0 * x[0]])
+        Fa, Fb = a.rhs(U), b.rhs(U)
+        assert np.max(np.abs(Fa - Fb)) <= 1e-12 * np.max(np.abs(Fa)), (eqn, np.max(np.abs(Fa - Fb)))
+
+
+def test_quads_projection_exact_and_free_stream():
+    m = meshgen.periodic_quads(3, 3, 0.0, 1.0, 0.0, 1.0)
+    p = 6
+    o = Oracle(m, p, 1)
+    o.set_nodes(meshgen.mapped_nodes(m, p, wavy_map2(1.0, 0.08)))
+    rng = np.random.default_rng(4)
+    Uh = rng.standard_normal((4, m.ne, (p + 1) ** 2))
+    o.set_modal(Uh)
+    assert np.max(np.abs(o.modal() - Uh)) < 1e-12
+    x = o.coords()
+    U = np.stack([np.full_like(x[0], v) for v in (1.2, 0.3, -0.2, 2.9)])
+    assert np.max(np.abs(o.rhs(U))) < 1e-12
+
+
+def _consistency_error2(p, n=4, amp=0.15):
+    L = 2 * np.pi
+    m = meshgen.periodic_quads(n, n, 0.0, L, 0.0, L)
+    o = Oracle(m, p, 1)
+    xy = meshgen.mapped_nodes(m, p, wavy_map2(L, amp))
+    o.set_nodes(xy)
+    F = o.rhs(_state2(*xy))
+    o.set_state(_flux_div2(*xy))
+    o.project_state()
+    ex = o.state()
+    return np.sqrt(np.sum((F - ex) ** 2)) / np.sqrt(np.sum(ex ** 2))
+
+
+def test_quads_curved_residual_converges_spectrally():
+    errs = [_consistency_error2(p) for p in (3, 5, 7)]
+    assert errs[2] < 1e-4, errs
+    assert errs[0] / errs[1] > 10 and errs[1] / errs[2] > 10, errs
+    flat = _consistency_error2(5, amp=0.0)
+    assert errs[1] < 1.5 * flat, (errs[1], flat)
