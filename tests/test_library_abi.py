@@ -117,6 +117,16 @@ def test_shallow_create_rejects_bad_arguments(lib, case, msg):
     assert not h.value
 
 
+def test_curved_quads_rejected_for_navier_stokes(lib):
+    m = meshgen.periodic_quads(2, 2)
+    d, keep = _desc(m, 3, 2, vis=0.01)
+    xn = np.ascontiguousarray(meshgen.quad_nodes(m, 3))
+    d.xnodes = xn.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    h = ctypes.c_void_p()
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert b"curved elements" in lib.dgx_last_error()
+
+
 def test_mixed_rejected_in_2d(lib):
     m = meshgen.periodic_quads(2, 2)
     d, keep = _desc(m, 3, 1, precision=1)
