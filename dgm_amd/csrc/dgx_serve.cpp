@@ -66,6 +66,7 @@ struct Request {
 struct Group {
   int eqn = 0, p = 0;
   std::vector<Request *> members;
+  std::vector<Request *> stepping;   // the members of the slice in flight
   std::vector<dgx::Batch2> hd;
   dgx::Batch2 *dd = nullptr;
   size_t cap = 0;
@@ -238,6 +239,7 @@ int enqueue_group(dgx_server *s, Group *g) {
     }
   if (hipEventRecord(g->ev, g->stream) != hipSuccess) return sfail(DGX_ERR_DEVICE, "hipEventRecord failed");
   g->k = k;
+  g->stepping = g->members;
   g->inflight = true;
   return DGX_OK;
 }
@@ -335,10 +337,11 @@ int dgx_server_progress(dgx_server_handle s, int *pending) {
     hipError_t e = hipEventQuery(g->ev);
     if (e == hipErrorNotReady) continue;
     g->inflight = false;
-    for (Request *r : g->members) {
+    for (Request *r : g->stepping) {   // members that joined during the slice did not step
       if (e == hipSuccess) r->done_steps += g->k;
       else r->msg = std::string("batched slice failed: ") + hipGetErrorString(e);
     }
+    g->stepping.clear();
   }
   // retire finished slices and requests
   for (size_t i = 0; i < s->running.size();) {

@@ -142,6 +142,23 @@ def test_batched_groups_match_unbatched_service():
         assert rel <= 1e-13, rel
 
 
+def test_member_joining_during_a_slice_steps_from_the_next():
+    """A request admitted while its group's slice is in flight joins the group
+    at the next slice and still takes exactly its own number of steps."""
+    rng = np.random.default_rng(33)
+    first, late = _euler2d_job(rng, 16, 4, 30), _euler2d_job(rng, 12, 4, 17)
+    srv = serve.Server(hbm_budget=1 << 32, max_active=4, steps_per_slice=10)
+    a = srv.submit(**first)
+    srv.progress()                                      # admits `first` and enqueues its first slice
+    b = srv.submit(**late)
+    srv.progress()                                      # `late` joins while that slice may still run
+    srv.run_until_done(timeout_s=60.0)
+    Ua, Ub = srv.result(a)[0], srv.result(b)[0]
+    srv.close()
+    assert np.array_equal(Ua, _served_alone(first))
+    assert np.array_equal(Ub, _served_alone(late))
+
+
 def test_graph_replay_matches_direct_launches():
     rng = np.random.default_rng(3)
     job = _euler2d_job(rng, 6, 4, 23)
