@@ -391,6 +391,8 @@ def main():
     ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--serve-slice", type=int, default=20)
     ap.add_argument("--serve-unbatched", action="store_true", help="serve every request on its own launches")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving sample and print its JSON (for profiling it alone)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -399,6 +401,9 @@ def main():
         print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
               "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.serve_only:
+        print(json.dumps(serving_sample(args)), flush=True)
+        return
 
     n, p, mu = args.n, args.p, 1.0 / 1600.0
     cpu = None if args.no_cpu else cpu_baseline(p, mu, args.cpu_seconds, args.cpu_workers, FLUXES[args.flux],
