@@ -44,62 +44,120 @@ __device__ __forceinline__ double face_add2(const double *__restrict__ Bend, con
   return s;
 }
 
-// Gauss-point traces of the nf fields of element e on its four edges.
-template <int P>
-__device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne,
-                                             int nf, int e) {
+// Gauss-point traces of the NF fields of element e on its four edges, all
+// fields per contraction (one load phase, four contraction phases).
+template <int P, int NF>
+__device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne, int e) {
   using S = H2<P>;
   constexpr int L = S::L, QF = S::QF, L2 = S::L2, NT = S::NT;
-  __shared__ double A[L2], Kb[2 * L], Ka[2 * L], O[2 * QF];
+  __shared__ double A[NF * L2], Kb[NF * 2 * L], Ka[NF * 2 * L], O[NF * 2 * QF];
   const int tid = threadIdx.x;
-  double *Te = T + (size_t)e * 4 * nf * QF;
-  for (int v = 0; v < nf; ++v) {
-    copy_block<NT>(U + ((size_t)v * ne + e) * L2, A, L2, tid);
-    __syncthreads();
-    contract<1, L, L, 2, 2, NT>(CT<P>::Bend(), A, Kb, tid);   // [L][2]
-    contract<1, L, L, 1, 2, NT>(CT<P>::Bend(), A, Ka, tid);   // [2][L]
-    __syncthreads();
-    contract<1, L, 2, 1, QF, NT>(CT<P>::Bg(), Kb, O, tid);    // [QF][2]: edges 0 / 2
-    __syncthreads();
-    for (int i = tid; i < 2 * QF; i += NT) {
-      const int a = i >> 1, end = i & 1;
-      Te[((end ? 2 : 0) * nf + v) * QF + a] = O[i];
-    }
-    __syncthreads();
-    contract<1, 2, L, 2, QF, NT>(CT<P>::Bg(), Ka, O, tid);    // [2][QF]: edges 3 / 1
-    __syncthreads();
-    for (int i = tid; i < 2 * QF; i += NT) {
-      const int end = i / QF, b = i - end * QF;
-      Te[((end ? 1 : 3) * nf + v) * QF + b] = O[i];
-    }
-    __syncthreads();
+  double *Te = T + (size_t)e * 4 * NF * QF;
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    A[i] = U[((size_t)v * ne + e) * L2 + (i - v * L2)];
   }
+  __syncthreads();
+  contract<NF, L, L, 2, 2, NT>(CT<P>::Bend(), A, Kb, tid);   // [NF][L][2]
+  contract<NF, L, L, 1, 2, NT>(CT<P>::Bend(), A, Ka, tid);   // [NF][2][L]
+  __syncthreads();
+  contract<NF, L, 2, 1, QF, NT>(CT<P>::Bg(), Kb, O, tid);    // [NF][QF][2]: edges 0 / 2
+  __syncthreads();
+  for (int i = tid; i < NF * 2 * QF; i += NT) {
+    const int v = i / (2 * QF), r = i - v * 2 * QF, a = r >> 1, end = r & 1;
+    Te[((end ? 2 : 0) * NF + v) * QF + a] = O[i];
+  }
+  __syncthreads();
+  contract<NF, 2, L, 2, QF, NT>(CT<P>::Bg(), Ka, O, tid);    // [NF][2][QF]: edges 3 / 1
+  __syncthreads();
+  for (int i = tid; i < NF * 2 * QF; i += NT) {
+    const int v = i / (2 * QF), r = i - v * 2 * QF, end = r / QF, b = r - end * QF;
+    Te[((end ? 1 : 3) * NF + v) * QF + b] = O[i];
+  }
+  __syncthreads();
 }
 
 template <int P>
 __global__ void __launch_bounds__(64)
 k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, int nf) {
-  traces2_elem<P>(U, T, ne, nf, blockIdx.x);
+  if (nf == 3) traces2_elem<P, 3>(U, T, ne, blockIdx.x);
+  else traces2_elem<P, 4>(U, T, ne, blockIdx.x);
 }
 
 // Residual of element e (Euler, or with NS the BR1 viscous terms from V, TV);
 // out(o, r) receives the modal residual r at offset o = (v*ne + e)*L2 + k.
+// Every contraction runs on all fields of the element at once, so a pass
+// keeps NF x Q pencils busy instead of Q.
+template <int P, int NF>
+struct H2F {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q;
+  // LDS (doubles): M [NF][L2] (modal, then the modal residual), X1, X2 [NF][Q][L],
+  // N1, N2 [NF][Q2] (nodal state, then the contravariant fluxes), gF [NF][4][QF], H [NF][4][L]
+  static constexpr int NM = NF * L2, NX = NF * Q * L, NN = NF * Q2;
+};
+
+template <int P, int NF>
+__device__ __forceinline__ double *rhs2_smem() {
+  using F = H2F<P, NF>;
+  __shared__ double sm[F::NM + 2 * F::NX + 2 * F::NN + NF * 4 * F::QF + NF * 4 * F::L];
+  return sm;
+}
+
+// Element state modal -> nodal for all NF fields; each thread keeps its nodes
+// k = tid + r*NT in registers (un[r][v]).
+template <int P, int NF, int NPT>
+__device__ __forceinline__ void load_nodal2(const double *__restrict__ U, int ne, int e, double *M, double *X,
+                                            double *N, double (&un)[NPT][NF], int tid) {
+  using F = H2F<P, NF>;
+  constexpr int L = F::L, Q = F::Q, L2 = F::L2, Q2 = F::Q2, NT = 64;
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    M[i] = U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), M, X, tid);   // [NF][Q][L]
+  __syncthreads();
+  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), X, N, tid);   // [NF][Q][Q]
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int k = tid + r * NT;
+#pragma unroll
+    for (int v = 0; v < NF; ++v) un[r][v] = k < Q2 ? N[v * Q2 + k] : 1.0;
+  }
+}
+
+// Projected strong-form residual of all NF fields from the contravariant
+// fluxes in N1 (r) and N2 (s) and the face terms gF; result in M [NF][L2].
+template <int P, int NF>
+__device__ __forceinline__ void divergence2(double *M, double *X1, double *X2, double *N1, double *N2,
+                                            const double *gF, double *H, int tid) {
+  using F = H2F<P, NF>;
+  constexpr int L = F::L, Q = F::Q, QF = F::QF, NT = 64;
+  contract<NF, Q, Q, 2, L, NT>(CT<P>::BW(), N1, X1, tid);
+  contract<NF, Q, Q, 2, L, NT>(CT<P>::QD(), N2, X2, tid);
+  contract<NF, 4, QF, 2, L, NT>(CT<P>::PW(), gF, H, tid);
+  __syncthreads();
+  contract<NF, Q, L, 1, L, NT>(CT<P>::QD(), X1, M, tid);
+  __syncthreads();
+  contract<NF, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, M, tid);
+  __syncthreads();
+}
+
 template <int P, bool NS, class Out>
 __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const double *__restrict__ T,
                                           const double *__restrict__ V, const double *__restrict__ TV,
                                           const Geo &geo, const Phys &ph, int ne, int e, int *__restrict__ flag,
                                           Out out) {
-  using S = H2<P>;
-  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
-  __shared__ double Un[4 * Q2], gF[16 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  using F = H2F<P, 4>;
+  constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
+  constexpr int NPT = (Q2 + NT - 1) / NT;
+  double *M = rhs2_smem<P, 4>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  double *gF = N2 + F::NN, *H = gF + 16 * QF;
   const int tid = threadIdx.x;
-  const Tab tb{};
   const double gm1 = ph.gamma - 1.0;
-  for (int v = 0; v < 4; ++v) {
-    copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
-    __syncthreads();
-    backward2<P>(tb, Rv, X1, Un + v * Q2, tid);
-  }
+  double un[NPT][4];
+  load_nodal2<P, 4, NPT>(U, ne, e, M, X1, N1, un, tid);
   bool bad = false;
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, a = i - s * QF;
@@ -136,40 +194,33 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
     }
   }
   if (bad) atomicOr(flag, 1);
-  __syncthreads();
   const double *mt = geo.metric + 9 * (size_t)e;
   const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
-  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
-  for (int v = 0; v < 4; ++v) {
-    for (int k = tid; k < Q2; k += NT) {
-      const double r = Un[k], ru = Un[Q2 + k], rv = Un[2 * Q2 + k], rE = Un[3 * Q2 + k];
-      const double ir = 1.0 / r, u = ru * ir, vv = rv * ir;
-      const double p = gm1 * (rE - 0.5 * (ru * u + rv * vv));
-      double fx, fy;
-      if (v == 0) { fx = ru; fy = rv; }
-      else if (v == 1) { fx = ru * u + p; fy = ru * vv; }
-      else if (v == 2) { fx = rv * u; fy = rv * vv + p; }
-      else { const double h = rE + p; fx = u * h; fy = vv * h; }
-      Fr[k] = rx * fx + ry * fy;
-      Fs[k] = sx * fx + sy * fy;
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int k = tid + rr * NT;
+    if (k >= Q2) continue;
+    const double r = un[rr][0], ru = un[rr][1], rv = un[rr][2], rE = un[rr][3];
+    const double ir = 1.0 / r, u = ru * ir, vv = rv * ir;
+    const double p = gm1 * (rE - 0.5 * (ru * u + rv * vv));
+    const double h = rE + p;
+    const double fx[4] = {ru, ru * u + p, rv * u, u * h}, fy[4] = {rv, ru * vv, rv * vv + p, vv * h};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      N1[v * Q2 + k] = rx * fx[v] + ry * fy[v];
+      N2[v * Q2 + k] = sx * fx[v] + sy * fy[v];
     }
-    __syncthreads();
-    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
-    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
-    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
-    __syncthreads();
-    for (int k = tid; k < L2; k += NT) {
-      const int l = k / L, m = k - l * L;
-      double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
-      if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
-      out(((size_t)v * ne + e) * L2 + k, r);
-    }
-    __syncthreads();
   }
+  __syncthreads();
+  divergence2<P, 4>(M, X1, X2, N1, N2, gF, H, tid);
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int i = tid; i < 4 * L2; i += NT) {
+    const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
+    double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
+    out(((size_t)v * ne + e) * L2 + k, r);
+  }
+  __syncthreads();
 }
 
 template <int P, bool NS>
@@ -188,16 +239,14 @@ template <int P, class Out>
 __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, const double *__restrict__ T,
                                               double *__restrict__ Bal, int bal, const Geo &geo, const Phys &ph,
                                               int ne, int e, int *__restrict__ flag, Out out) {
-  using S = H2<P>;
-  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
-  __shared__ double Un[3 * Q2], gF[12 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  using F = H2F<P, 3>;
+  constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
+  constexpr int NPT = (Q2 + NT - 1) / NT;
+  double *M = rhs2_smem<P, 3>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  double *gF = N2 + F::NN, *H = gF + 12 * QF;
   const int tid = threadIdx.x;
-  const Tab tb{};
-  for (int v = 0; v < 3; ++v) {
-    copy_block<NT>(U + ((size_t)v * ne + e) * L2, Rv, L2, tid);
-    __syncthreads();
-    backward2<P>(tb, Rv, X1, Un + v * Q2, tid);
-  }
+  double un[NPT][3];
+  load_nodal2<P, 3, NPT>(U, ne, e, M, X1, N1, un, tid);
   bool bad = false;
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, a = i - s * QF;
@@ -227,45 +276,38 @@ __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, cons
     }
   }
   if (bad) atomicOr(flag, 1);
-  __syncthreads();
   const double *mt = geo.metric + 9 * (size_t)e;
   const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
-  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
-  for (int v = 0; v < 3; ++v) {
-    for (int k = tid; k < Q2; k += NT) {
-      const double phi = Un[k], mx = Un[Q2 + k], my = Un[2 * Q2 + k];
-      const double u = mx / phi, vv = my / phi, p = 0.5 * (phi * phi);
-      double fx, fy;
-      if (v == 0) { fx = mx; fy = my; }
-      else if (v == 1) { fx = mx * u + p; fy = mx * vv; }
-      else { fx = my * u; fy = my * vv + p; }
-      Fr[k] = rx * fx + ry * fy;
-      Fs[k] = sx * fx + sy * fy;
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int k = tid + rr * NT;
+    if (k >= Q2) continue;
+    const double phi = un[rr][0], mx = un[rr][1], my = un[rr][2];
+    const double u = mx / phi, vv = my / phi, p = 0.5 * (phi * phi);
+    const double fx[3] = {mx, mx * u + p, my * u}, fy[3] = {my, mx * vv, my * vv + p};
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      N1[v * Q2 + k] = rx * fx[v] + ry * fy[v];
+      N2[v * Q2 + k] = sx * fx[v] + sy * fy[v];
     }
-    __syncthreads();
-    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
-    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
-    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
-    __syncthreads();
-    for (int k = tid; k < L2; k += NT) {
-      const int l = k / L, m = k - l * L;
-      const double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
-      const size_t o = ((size_t)v * ne + e) * L2 + k;
-      if (bal == 0) {
-        out(o, r);
-      } else {
-        double b;
-        if (bal == 1) { b = r; Bal[o] = r; }
-        else b = Bal[o];
-        out(o, -1.0 * b + r);
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();
+  divergence2<P, 3>(M, X1, X2, N1, N2, gF, H, tid);
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int i = tid; i < 3 * L2; i += NT) {
+    const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
+    const double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    const size_t o = ((size_t)v * ne + e) * L2 + k;
+    if (bal == 0) {
+      out(o, r);
+    } else {
+      double b;
+      if (bal == 1) { b = r; Bal[o] = r; }
+      else b = Bal[o];
+      out(o, -1.0 * b + r);
+    }
+  }
+  __syncthreads();
 }
 
 template <int P>
@@ -618,7 +660,7 @@ k_batch_traces2(const Batch2 *__restrict__ b, int nreq, int nwork) {
   int r = 0;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     while (r + 1 < nreq && b[r + 1].off <= w) ++r;
-    traces2_elem<P>(b[r].U, b[r].T, b[r].ne, NF, w - b[r].off);
+    traces2_elem<P, NF>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
   }
 }
 
@@ -637,7 +679,7 @@ k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, doubl
     };
     const Geo geo = b[r].geo;
     const Phys ph = b[r].phys;
-    if (SHALLOW)
+    if constexpr (SHALLOW)
       shallow2_elem<P>(U, b[r].T, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
     else
       rhs2_elem<P, false>(U, b[r].T, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
