@@ -505,20 +505,19 @@ k_backward2(const double *__restrict__ Fm, double *__restrict__ Fn, Tab tb, int 
 template <int P, bool ADJ>
 __global__ void __launch_bounds__(64)
 k_lin2(LinArgs a) {
-  using S = H2<P>;
-  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2, NT = S::NT;
-  __shared__ double Un[4 * Q2], gF[16 * QF], Fr[Q2], Fs[Q2], X1[Q * L], X2[Q * L], Rv[L2], H[4 * L];
+  using F = H2F<P, 4>;
+  constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
+  constexpr int NPT = (Q2 + NT - 1) / NT;
+  double *M = rhs2_smem<P, 4>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  double *gF = N2 + F::NN, *H = gF + 16 * QF;
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const Geo &geo = a.geo;
   const double *mt = geo.metric + 9 * (size_t)e;
   const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
   const double dm = rx * sy - ry * sx;   // 1 / J_e
   const double sc = ADJ ? dm : 1.0;
-  for (int v = 0; v < 4; ++v) {
-    copy_block<NT>(a.X + ((size_t)v * ne + e) * L2, Rv, L2, tid);
-    __syncthreads();
-    backward2<P>(a.tab, Rv, X1, Un + v * Q2, tid);
-  }
+  double un[NPT][4];
+  load_nodal2<P, 4, NPT>(a.X, ne, e, M, X1, N1, un, tid);
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, q = i - s * QF;
     const int es = e * 4 + s;
@@ -550,46 +549,41 @@ k_lin2(LinArgs a) {
       gF[(v * 4 + s) * QF + q] = jr * (fh - fl);
     }
   }
-  __syncthreads();
-  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
-  for (int v = 0; v < 4; ++v) {
-    for (int k = tid; k < Q2; k += NT) {
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int k = tid + rr * NT;
+    if (k >= Q2) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
       double fx = 0.0, fy = 0.0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double u = sc * Un[j * Q2 + k];
+        const double u = sc * un[rr][j];
         fx = fma(a.Ax[4 * v + j], u, fx);
         fy = fma(a.Ay[4 * v + j], u, fy);
       }
-      Fr[k] = rx * fx + ry * fy;
-      Fs[k] = sx * fx + sy * fy;
+      N1[v * Q2 + k] = rx * fx + ry * fy;
+      N2[v * Q2 + k] = sx * fx + sy * fy;
     }
-    __syncthreads();
-    contract<1, Q, Q, 2, L, NT>(CT<P>::BW(), Fr, X1, tid);
-    contract<1, Q, Q, 2, L, NT>(CT<P>::QD(), Fs, X2, tid);
-    contract<1, 4, QF, 2, L, NT>(CT<P>::PW(), gF + v * 4 * QF, H, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT>(CT<P>::QD(), X1, Rv, tid);
-    __syncthreads();
-    contract<1, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, Rv, tid);
-    __syncthreads();
-    for (int k = tid; k < L2; k += NT) {
-      const int l = k / L, m = k - l * L;
-      double r = Rv[k] + face_add2<P>(CT<P>::Bend(), H, l, m, cw);
-      const size_t gi = ((size_t)v * ne + e) * L2 + k;
-      if (!ADJ) {
-        if (v == 3 && k == 0) r -= 2.0 * a.theta[e] * a.g;   // source: mode 0 of the constant theta_e
-        const double ut = a.ca * a.Yt[gi] - a.dt * r;
-        a.Yt[gi] = ut;
-        a.Y[gi] = fma(a.cb, ut, a.Y[gi]);
-      } else {
-        const double x = a.X[gi];
-        a.Y[gi] = fma(-a.dt, r / dm, a.Y[gi]);
-        if (v == 3 && k == 0) a.gtheta[e] += 2.0 * a.g * a.dt * x;
-        a.Yt[gi] = a.ca * x;
-      }
+  }
+  __syncthreads();
+  divergence2<P, 4>(M, X1, X2, N1, N2, gF, H, tid);
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int i = tid; i < 4 * L2; i += NT) {
+    const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
+    double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    const size_t gi = ((size_t)v * ne + e) * L2 + k;
+    if (!ADJ) {
+      if (v == 3 && k == 0) r -= 2.0 * a.theta[e] * a.g;   // source: mode 0 of the constant theta_e
+      const double ut = a.ca * a.Yt[gi] - a.dt * r;
+      a.Yt[gi] = ut;
+      a.Y[gi] = fma(a.cb, ut, a.Y[gi]);
+    } else {
+      const double x = a.X[gi];
+      a.Y[gi] = fma(-a.dt, r / dm, a.Y[gi]);
+      if (v == 3 && k == 0) a.gtheta[e] += 2.0 * a.g * a.dt * x;
+      a.Yt[gi] = a.ca * x;
     }
-    __syncthreads();
   }
 }
 
