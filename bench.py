@@ -168,11 +168,14 @@ def serving_sample(args, seed=1234):
             job["balance"] = False
         jobs.append(job)
         dofs.append(len(base[eqn]) * nn * nn * (pp + 1) ** 2)
+    if args.serve_requests > 64:
+        print(f"[serve] built {len(jobs)} requests", flush=True)
     srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active, batch=not args.serve_unbatched,
                        steps_per_slice=args.serve_slice)
     arrivals = serve.poisson_arrivals(len(jobs), rate=args.serve_rate, rng=rng)
     t0 = time.perf_counter()
-    ids = serve.serve_timeline(srv, jobs, arrivals)
+    ids = serve.serve_timeline(srv, jobs, arrivals, timeout_s=1200.0,
+                               report_every=32 if args.serve_requests > 64 else 0)
     wall = time.perf_counter() - t0
     lat, wait = [], []
     for rid in ids:

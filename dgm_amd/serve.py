@@ -165,19 +165,26 @@ def poisson_arrivals(n: int, rate: float, rng) -> List[float]:
     return list(np.cumsum(rng.exponential(1.0 / rate, size=n)))
 
 
-def serve_timeline(server: Server, jobs, arrivals, poll_s: float = 0.0002, timeout_s: float = 600.0):
+def serve_timeline(server: Server, jobs, arrivals, poll_s: float = 0.0002, timeout_s: float = 600.0,
+                   report_every: int = 0):
     """Submit job i at arrivals[i] seconds after the start (jobs[i] is a dict of
     Server.submit keyword arguments) while driving the scheduler; returns the
-    request ids in submission order."""
+    request ids in submission order.  report_every > 0 prints a line each time
+    that many more requests have finished."""
     ids: List[int] = []
     t0 = time.perf_counter()
     i = 0
+    reported = 0
     while True:
         now = time.perf_counter() - t0
         while i < len(jobs) and arrivals[i] <= now:
             ids.append(server.submit(**jobs[i]))
             i += 1
         pending = server.progress()
+        done = i - pending
+        if report_every > 0 and done >= reported + report_every:
+            reported = done - done % report_every
+            print(f"[serve] {done} of {len(jobs)} requests finished at {now:.1f} s", flush=True)
         if i == len(jobs) and pending == 0:
             return ids
         if now > timeout_s:
