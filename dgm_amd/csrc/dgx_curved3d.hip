@@ -23,11 +23,14 @@ template <int P> struct C3 {
   static constexpr int NT = 256, NW = NT / 64;
   static constexpr int NPT = (Q3 + NT - 1) / NT;          // nodes per thread
   static constexpr int NML = (5 * L3 + NT - 1) / NT;      // modal entries per thread (5 fields)
-  // LDS (doubles): Un, Fw [5][Q3]; scratch S1, S2 [SZ] (the largest intermediate:
-  // a transform of 5 fields or the 30 face blocks at the GLL points);
-  // gF [5][6][QF2]; Fc [3][Q3]; reductions [NW][8]
+  // LDS (doubles): Fw [5][Q3] (nodal residual, then the CG direction); scratch
+  // S1, S2 [SZ] (the largest intermediate: a transform of 5 fields or the 30
+  // face blocks at the GLL points; S1 also holds the 3 flux components of the
+  // volume term); G [GZ] (the face terms gF [5][6][QF2], then b and M p
+  // [5][L3]); reductions [NW][8].  The nodal state lives in registers.
   static constexpr int SZ = (5 * Q3 > 30 * Q2) ? 5 * Q3 : 30 * Q2;
-  static constexpr int lds = (10 * Q3 + 2 * SZ + 30 * QF2 + 3 * Q3 + 8 * NW) * 8;
+  static constexpr int GZ = (30 * QF2 > 5 * L3) ? 30 * QF2 : 5 * L3;
+  static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 8 * NW) * 8;
 };
 
 // [5][N]^3 blocks (compact) through matrix M [NO][N] along all three dimensions
@@ -79,10 +82,10 @@ k_rhs3c(CurvedArgs a) {
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
   constexpr int NT = S::NT, NPT = S::NPT, NML = S::NML;
   extern __shared__ double sm[];
-  double *Un = sm, *Fw = Un + 5 * Q3, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *gF = S2 + S::SZ;
-  static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ, "scratch too small");
+  double *Fw = sm, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *G = S2 + S::SZ, *red = G + S::GZ;
+  double *gF = G, *Fc = S1, *Bv = G;
+  static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ && 3 * Q3 <= S::SZ && 5 * Q * L * L <= S::SZ, "scratch");
   static_assert(S::lds <= 160 * 1024, "LDS");
-  double *Fc = gF + 30 * QF2, *red = Fc + 3 * Q3;
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const Geo &geo = a.geo;
   const double *ge = a.gm + (size_t)e * 10 * Q3;   // wJ dxi_d/dx_x (9 rows), wJ
@@ -95,13 +98,21 @@ k_rhs3c(CurvedArgs a) {
     }
     __syncthreads();
   } else {
-  // modal state -> nodal (5 fields)
+  // modal state -> nodal (5 fields), kept in registers: node n = tid + r NT
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3;
     S1[i] = a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
   }
   __syncthreads();
-  chain5<L, Q, NT>(CT<P>::BT(), S1, S2, Fw, Un, tid);
+  chain5<L, Q, NT>(CT<P>::BT(), S1, S2, S1, Fw, tid);
+  double un[NPT][5];
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int n = tid + rr * NT;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) un[rr][v] = n < Q3 ? Fw[v * Q3 + n] : 1.0;
+  }
+  __syncthreads();
 
   // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points
   bool bad = false;
@@ -138,8 +149,11 @@ k_rhs3c(CurvedArgs a) {
   // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by field
   for (int v = 0; v < 5; ++v) {
     __syncthreads();
-    for (int n = tid; n < Q3; n += NT) {
-      const double r = Un[n], mx = Un[Q3 + n], my = Un[2 * Q3 + n], mz = Un[3 * Q3 + n], E = Un[4 * Q3 + n];
+#pragma unroll
+    for (int rr = 0; rr < NPT; ++rr) {
+      const int n = tid + rr * NT;
+      if (n >= Q3) continue;
+      const double r = un[rr][0], mx = un[rr][1], my = un[rr][2], mz = un[rr][3], E = un[rr][4];
       const double ir = 1.0 / r, u = mx * ir, vv = my * ir, w = mz * ir;
       const double p = gm1 * (E - 0.5 * (mx * u + my * vv + mz * w));
       double fx, fy, fz;
@@ -206,12 +220,12 @@ k_rhs3c(CurvedArgs a) {
   }
   __syncthreads();
   }   // !PROJ
-  // b = B^T (wJ f): into Un (compact [5][L3])
-  chain5<Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Un, tid);
+  // b = B^T (wJ f): into Bv (compact [5][L3], the face-term region, free now)
+  chain5<Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Bv, tid);
 
   // conjugate gradients on M x = b, M = B^T diag(wJ) B, five fields at once;
-  // x0 = b / mean(J).  Pv, AP (compact [5][L3]) reuse Fw and Un.
-  double *Pv = Fw, *AP = Un;
+  // x0 = b / mean(J).  Pv, AP (compact [5][L3]) reuse Fw and Bv.
+  double *Pv = Fw, *AP = Bv;
   double x[NML], res[NML], bvec[NML];
   double part[5], bb[5], rr[5], pap[5];
   const double jinv = 1.0 / a.jav[e];
@@ -222,7 +236,7 @@ k_rhs3c(CurvedArgs a) {
     const int g = tid + k * NT;
     x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
     if (g < 5 * L3) {
-      bvec[k] = Un[g];
+      bvec[k] = Bv[g];
       x[k] = bvec[k] * jinv;
       part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
     }
