@@ -111,6 +111,41 @@ def read_bc(path: str) -> List[tuple]:
     return rows
 
 
+def read_crv(path: str):
+    """Curved sides of ``root.crv`` in the old text format (Curve::read_old and
+    Curve::read_sides, src/Curve.cpp:185-330): a header line, the number of
+    curve types, each as ``code tag [parameters]`` (Arc: the radius), then the
+    number of curved sides and rows ``eid sid tag``.  Returns (types, sides):
+    types maps each tag to (code, parameters), sides lists (eid, sid, tag).  A
+    later row for the same element replaces the earlier one, as the
+    reference keeps one curve per element."""
+    with open(path) as f:
+        f.readline()
+        toks = f.read().split()
+    pos = 0
+    ntypes = int(toks[pos]); pos += 1
+    while not toks[pos][0].isalpha():
+        pos += 1
+    while toks[pos].lower() in ("number", "of", "curve", "type(s)", "types"):
+        pos += 1
+    types = {}
+    for _ in range(ntypes):
+        code = toks[pos].lower(); tag = toks[pos + 1]; pos += 2
+        params = []
+        if code in ("arc", "circle"):
+            params = [float(toks[pos])]; pos += 1
+        elif code not in ("straight", "nonaffine", "skewed", "affine"):
+            raise ValueError(f"{path}: unsupported curve type {code}")
+        types[tag] = (code, params)
+    nsides = int(toks[pos]); pos += 1
+    while pos < len(toks) and not toks[pos].lstrip("-").isdigit():
+        pos += 1
+    sides = []
+    for _ in range(nsides):
+        sides.append((int(toks[pos]), int(toks[pos + 1]), toks[pos + 2])); pos += 3
+    return types, sides
+
+
 def boundary_con(con: np.ndarray, bc_rows, types: Dict[str, int]) -> np.ndarray:
     """Connectivity with the boundary sides of ``bc_rows`` turned into rows
     (eid, sid, -1, type), ``types`` mapping each tag to its boundary type."""

@@ -138,6 +138,85 @@ def mapped_nodes(mesh: Mesh, p: int, fn) -> np.ndarray:
     return np.stack(fn(*x))
 
 
+def _arc_points(p0, p1, radius, orientation, z):
+    """Points of a circular arc through p0 and p1 at the local coordinates z,
+    as Arc::generate (src/Curve.cpp:523-590) places them."""
+    arclength = np.hypot(p1[0] - p0[0], p1[1] - p0[1])
+    if abs(arclength / radius) > 2.0:
+        raise ValueError("arc radius too small for its side")
+    pm = ((p0[0] + p1[0]) * 0.5, (p0[1] + p1[1]) * 0.5)
+    dist = np.sqrt(radius * radius - arclength * arclength * 0.25) * (-1.0 if radius > 0.0 else 1.0)
+    pc = (pm[0] + dist * (p1[1] - p0[1]) / arclength, pm[1] + dist * (p0[0] - p1[0]) / arclength)
+    adis, arad = abs(dist), abs(radius)
+    pd = ((pm[0] - pc[0]) * (arad / adis), (pm[1] - pc[1]) * (arad / adis))
+    theta = np.arctan(0.5 * arclength / adis) * (1.0 if radius > 0 else -1.0) * orientation
+    beta = theta * z
+    return (pc[0] + pd[0] * np.cos(beta) - pd[1] * np.sin(beta),
+            pc[1] + pd[0] * np.sin(beta) + pd[1] * np.cos(beta))
+
+
+def _edge_deviation(f, z, w, L):
+    """Element::generate_curve for one coordinate of a curved side
+    (Quad::forward_transform_side and Quad::get_1d_mass_matrix,
+    src/Quad.cpp:1455-1515): the side values less their linear interpolant,
+    projected with the GLL rule onto the L - 2 edge modes (1-z)(1+z)/4
+    P_i^(1,1)(z); returned at the points z."""
+    from numpy.polynomial import polynomial as npoly
+    lin = f[0] * (1 - z) / 2 + f[-1] * (1 + z) / 2
+    d = f - lin
+    if L <= 2:
+        return np.zeros_like(z)
+    from scipy.special import eval_jacobi
+    E = np.stack([(1 - z) * (1 + z) / 4 * eval_jacobi(i, 1.0, 1.0, z) for i in range(L - 2)])
+    M = (E[:, 1:-1] * w[1:-1]) @ E[:, 1:-1].T
+    rhs = E @ (w * d)
+    return np.linalg.solve(M, rhs) @ E
+
+
+def curved_quad_nodes(mesh: Mesh, p: int, crv):
+    """GLL nodal coordinates [2][elem][q*q] of quads with curved sides from a
+    ``root.crv`` description (formats.read_crv): Quad::get_curved_coordinates
+    (src/Quad.cpp:1018-1130) evaluates the modified C0 expansion of the
+    vertices and the curved side's edge modes at the nodes, so the element map
+    is the bilinear vertex map plus the side's projected deviation from its
+    chord, carried into the element by the linear vertex function across it.
+    Returns (nodes, curved element ids)."""
+    types, sides = crv
+    q, L = p + 2, p + 1
+    z = gll_points(q)
+    from numpy.polynomial import legendre
+    P = legendre.legval(z, np.eye(q)[q - 1])
+    w = 2.0 / (q * (q - 1) * P * P)
+    x = quad_nodes(mesh, p).copy()
+    curve = {}
+    for eid, sid, tag in sides:
+        curve[eid] = (sid, tag)
+    curved = []
+    for eid, (sid, tag) in sorted(curve.items()):
+        code, params = types[tag]
+        curved.append(eid)
+        if code not in ("arc", "circle"):
+            continue                     # straight (non-affine) sides: the vertex map already
+        v = mesh.vert[eid]
+        p0 = (v[0][sid], v[1][sid])
+        p1 = (v[0][(sid + 1) % 4], v[1][(sid + 1) % 4])
+        orientation = 1 if sid in (0, 1) else -1
+        ax, ay = _arc_points(p0, p1, params[0], orientation, z)
+        dx, dy = _edge_deviation(ax, z, w, L), _edge_deviation(ay, z, w, L)
+        a, b = np.meshgrid(z, z, indexing="ij")
+        if sid == 0:
+            fac, dev = (1 - b) / 2, (dx[:, None], dy[:, None])
+        elif sid == 2:
+            fac, dev = (1 + b) / 2, (dx[:, None], dy[:, None])
+        elif sid == 1:
+            fac, dev = (1 + a) / 2, (dx[None, :], dy[None, :])
+        else:
+            fac, dev = (1 - a) / 2, (dx[None, :], dy[None, :])
+        x[0, eid] += (fac * dev[0]).ravel()
+        x[1, eid] += (fac * dev[1]).ravel()
+    return x, curved
+
+
 def bump_map(lo, hi, amp):
     """The channel [lo, hi] with a smooth periodic bump of height `amp` on its
     bottom wall, decaying linearly to the flat top: z' = z + amp b(x, y) (hi_z - z) / (hi_z - lo_z),

@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 
-from dgm_amd import formats, ic
+from dgm_amd import formats, ic, mesh as meshgen
 from oracle_lib import Oracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -27,7 +27,12 @@ def run_oracle_case(root, nsteps=None):
     """Run the case as Domain::solve does (restart writes every Ntout steps
     project the state, src/Domain.cpp:898-947) and return (oracle, modal state)."""
     prm, mesh, icd = load_case(root)
-    o = Oracle(mesh, int(prm["p"]), int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    p = int(prm["p"])
+    o = Oracle(mesh, p, int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    crv = os.path.join(GOLDEN, root + ".crv")
+    if os.path.exists(crv):                       # curved sides (Domain::setup reads root.crv)
+        xn, _ = meshgen.curved_quad_nodes(mesh, p, formats.read_crv(crv))
+        o.set_nodes(xn)
     x = o.coords()
     o.set_state(initial_state(icd, x, mesh.nsd))
     o.project_state()

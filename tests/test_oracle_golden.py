@@ -1,7 +1,7 @@
 """Pin the CPU oracle against the reference's golden restart files.
 
 Each case is run exactly as the reference's regression test does
-(runs/vquad.tst, runs/vhex.tst) and compared with ``dgm_diff`` semantics
+(runs/vquad.tst, runs/vhex.tst, runs/cvquad.tst) and compared with ``dgm_diff`` semantics
 (src/Domain.cpp:1000-1071): every field must pass the absolute tolerance and,
 where the reference norm exceeds repsM = sqrt(eps), the relative tolerance.
 """
@@ -36,8 +36,41 @@ def test_vhex_3d_euler_lf_tvd():
     _check("vhex", 5e-13, 5e-10)
 
 
+def test_cvquad_2d_ns_curved_arcs():
+    # runs/cvquad.tst:22-23 aeps = reps = 5e-13; four quads with circular-arc
+    # sides (runs/cvquad.crv), p = 10, so the curved metrics and the curved
+    # mass-matrix projection are both on the path
+    _check("cvquad", 5e-13, 5e-13)
+
+
+def test_crv_reader_and_arc_nodes():
+    from dgm_amd import mesh as meshgen
+    from fixture_runs import load_case
+    types, sides = formats.read_crv(f"{GOLDEN}/cvquad.crv")
+    assert types == {"arc": ("arc", [-4.0]), "arc2": ("arc", [4.0])}
+    assert sides == [(0, 0, "arc"), (1, 0, "arc2"), (2, 2, "arc2"), (3, 2, "arc")]
+    prm, m, _ = load_case("cvquad")
+    p = int(prm["p"])
+    q = p + 2
+    xn, curved = meshgen.curved_quad_nodes(m, p, (types, sides))
+    assert curved == [0, 1, 2, 3]
+    # element 0, side 0 runs (0,0) -> (2,0) on a circle of radius 4; the
+    # polynomial side of degree p = 10 follows it to the projection error
+    e0 = xn[:, 0].reshape(2, q, q)
+    sx, sy = e0[0, :, 0], e0[1, :, 0]
+    dist = np.sqrt(16.0 - 1.0)
+    centre = (1.0, -dist)                    # radius < 0 puts the centre outside: the side bows inward
+    r = np.hypot(sx - centre[0], sy - centre[1])
+    assert np.max(np.abs(r - 4.0)) < 1e-10
+    sagitta = 4.0 - dist
+    assert np.all(sy[1:-1] > 0.0) and 0.95 * sagitta < np.max(sy) <= sagitta
+    # sides shared across the periodic seam coincide after the shift by 4
+    e2 = xn[:, 2].reshape(2, q, q)
+    assert np.allclose(e2[0, :, -1], sx, atol=1e-12) and np.allclose(e2[1, :, -1] - 4.0, sy, atol=1e-12)
+
+
 def test_restart_reader_consumes_whole_files():
-    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25)):
+    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25), ("cvquad", 4)):
         r = formats.read_restart(f"{GOLDEN}/{root}.ref", ne)
         assert r.data.shape[1] == ne
         assert np.all(np.isfinite(r.data))
