@@ -250,6 +250,7 @@ dgx::CurvedArgs curved_args(dgx_domain *d) {
   ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
   ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
   ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
+  ca.ns = d->visc; ca.G = d->G; ca.V = d->V; ca.TV = d->TV;
   return ca;
 }
 
@@ -312,6 +313,13 @@ int residual(dgx_domain *d, const double *U, double *R) {
   if (d->curved) {
     dgx::CurvedArgs ca = curved_args(d);
     ca.U = U; ca.T = d->T; ca.R = R;
+    for (int op : {1, 2}) {   // Navier-Stokes: lifted gradients, then the viscous terms
+      if (!d->visc) break;
+      ca.op = op;
+      hipError_t e = timed(d, op == 1 ? dgx::OP_GRAD : dgx::OP_VISC, [&] { return launch_curved(d, ca); });
+      if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved viscous terms: ") + hipGetErrorString(e));
+    }
+    ca.op = 0;
     hipError_t e = timed(d, 1, [&] { return launch_curved(d, ca); });
     if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved residual: ") + hipGetErrorString(e));
     ++d->stages;
@@ -828,7 +836,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   } else if (d->visc) {
     const int ng = nsd == 2 ? 6 : 12;
     if (!d->fused && (rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
-    if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * d->nm))) return rc;
+    // curved quads keep the weighted nodal viscous divergence (Q^2 per field)
+    if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * (d->curved ? d->nq : d->nm)))) return rc;
     if ((rc = alloc(&d->TV, (size_t)ne * d->ns * (d->nf - 1) * qs))) return rc;
   }
   return DGX_OK;
@@ -866,7 +875,7 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   if (visc) {
     const size_t w = mixed ? 4 : 8;
     if (!fused && !mixed) b += ne * (nsd == 2 ? 6 : 12) * nm * 8;
-    b += ne * (nf - 1) * nm * w + ne * ns * (nf - 1) * qs * w;
+    b += ne * (nf - 1) * (desc->xnodes && nsd == 2 ? nq : nm) * w + ne * ns * (nf - 1) * qs * w;
   }
   b += nf * ne * nq * 8;                                    // nodal transfer buffer
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
@@ -905,8 +914,10 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
   if (desc->xnodes && !(desc->precision == DGX_FP64 &&
                         ((desc->nsd == 3 && desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p)) ||
-                         (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_SHALLOW)))))
-    return fail(DGX_ERR_ARGUMENT, "curved elements (xnodes) are available for Euler3d (p <= 6), Euler and Shallow, DGX_FP64");
+                         (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
+                                             desc->eqntype == DGX_SHALLOW)))))
+    return fail(DGX_ERR_ARGUMENT,
+                "curved elements (xnodes) are available for Euler3d (p <= 6), Euler, Navier_Stokes and Shallow, DGX_FP64");
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
       if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)

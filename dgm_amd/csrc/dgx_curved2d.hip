@@ -1,6 +1,6 @@
-// Curved quads (Euler 2D and shallow water, p <= 10): the residual with
-// metrics at the nodes and the projection through the element mass matrix,
-// the quad counterpart of dgx_curved3d.hip.
+// Curved quads (Euler 2D, Navier-Stokes 2D and shallow water, p <= 10): the
+// residual with metrics at the nodes and the projection through the element
+// mass matrix, the quad counterpart of dgx_curved3d.hip.
 //
 //   volume term   Euler::convection / Shallow::convection with Quad::gradient
 //                 on curved geometry: dF_x/dx = sum_d (dxi_d/dx)(node) D_d F_x
@@ -13,6 +13,12 @@
 //   project       b = B^T (wJ f) and M x = b with M = B^T diag(wJ) B
 //                 (src/Quad.cpp:263-300), by conjugate gradients inside the
 //                 workgroup, all fields at once.
+//   viscous       BR1 (Navier_Stokes::diffusion, src/Navier_Stokes.cpp:113-258):
+//                 k_grad2c lifts and projects the nodal gradients of (u, v, e)
+//                 with each side's own normal, k_visc2c forms the stresses and
+//                 heat flux at the nodes, their weighted divergence and each
+//                 side's normal flux, and the residual adds -div F_v and the
+//                 lifted 1/2 (F_v.n own + F_v.n neighbour) before projecting.
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -55,179 +61,25 @@ __device__ __forceinline__ void field_dots2(const double (&part)[NF], double *re
   __syncthreads();
 }
 
-// PROJ: project the nodal fields a.U [NF][ne][Q^2] instead (the forward
-// transform of a curved element: x = M^-1 B^T (wJ u)).
-template <int P, bool SHALLOW, bool PROJ>
-__global__ void __launch_bounds__(128)
-k_rhs2c(CurvedArgs a) {
-  constexpr int NF = SHALLOW ? 3 : 4;
-  using S = C2<P, NF>;
-  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2;
-  constexpr int NT = S::NT, NPT = S::NPT, NML = S::NML, NW = S::NW;
-  extern __shared__ double sm[];
-  double *Un = sm, *Fw = Un + NF * Q2, *S1 = Fw + NF * Q2, *S2 = S1 + S::SZ, *gF = S2 + S::SZ;
-  double *Fc = gF + 4 * NF * QF, *red = Fc + 2 * Q2;
-  static_assert(NF * L * Q <= S::SZ && NF * L2 <= S::SZ, "scratch too small");
-  static_assert(S::lds <= 64 * 1024, "LDS");
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
-  const Geo &geo = a.geo;
-  const double *ge = a.gm + (size_t)e * 5 * Q2;   // wJ dxi_d/dx_x (rows r, s), wJ
-  const double gm1 = a.phys.gamma - 1.0;
-
-  if (PROJ) {
-    for (int i = tid; i < NF * Q2; i += NT) {
-      const int v = i / Q2, n = i - v * Q2;
-      Fw[i] = ge[4 * Q2 + n] * a.U[((size_t)v * ne + e) * Q2 + n];
-    }
-    __syncthreads();
-  } else {
-  // modal state -> nodal
-  for (int i = tid; i < NF * L2; i += NT) {
-    const int v = i / L2;
-    S1[i] = a.U[((size_t)v * ne + e) * L2 + (i - v * L2)];
-  }
-  __syncthreads();
-  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), S1, S2, tid);   // [NF][Q][L]
-  __syncthreads();
-  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), S2, Un, tid);   // [NF][Q][Q]
-
-  // edge corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points
-  bool bad = false;
-  for (int i = tid; i < 4 * QF; i += NT) {
-    const int s = i / QF, q = i - s * QF;
-    const int es = e * 4 + s;
-    const double *fg = a.fgeo + ((size_t)es * QF + q) * 4;
-    double nx = fg[0], ny = fg[1];
-    const double jr = fg[3];
-    const int nb = geo.nbr[es];
-    double UL[4], UR[4], FL[4], FR[4], Fh[4], lL, lR;
-    const double *tl = a.T + ((size_t)e * 4 + s) * NF * QF;
-#pragma unroll
-    for (int v = 0; v < NF; ++v) UL[v] = tl[v * QF + q];
-    if (nb >= 0) {
-      const int qn = geo.perm[geo.permid[es] * QF + q];
-      if (!geo.master[es]) {   // the pair's flux uses the master side's normal (both sides' normals
-                               // come from their own interpolated metrics and differ slightly)
-        const double *fm = a.fgeo + (((size_t)nb * 4 + geo.nside[es]) * QF + qn) * 4;
-        nx = -fm[0]; ny = -fm[1];
-      }
-      const double *tr = a.T + ((size_t)nb * 4 + geo.nside[es]) * NF * QF;
-#pragma unroll
-      for (int v = 0; v < NF; ++v) UR[v] = tr[v * QF + qn];
-    } else if (SHALLOW) {
-      ghost_shallow2(-1 - nb, UL, nx, ny, a.phys.bcU, UR);
-    } else {   // no Euler boundaries on quads (rejected at creation)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) UR[v] = UL[v];
-    }
-    if (SHALLOW) {
-      bad |= shallow_normal_flux2(UL, nx, ny, FL, lL);
-      bad |= shallow_normal_flux2(UR, nx, ny, FR, lR);
-      const double lmax = fmax(lL, lR);
-#pragma unroll
-      for (int v = 0; v < 3; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-    } else {
-      bool bl, br;
-      euler_normal_flux2(UL, nx, ny, gm1, FL, lL, bl);
-      euler_normal_flux2(UR, nx, ny, gm1, FR, lR, br);
-      bad |= bl | br;
-      if (a.phys.flux == 1) {
-        roe_flux2(UL, UR, nx, ny, a.phys.gamma, FL, Fh);
-      } else {
-        const double lmax = fmax(lL, lR);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < NF; ++v) gF[(v * 4 + s) * QF + q] = jr * (Fh[v] - FL[v]);
-  }
-  if (bad) atomicOr(a.flag, 1);
-
-  // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by field
-  for (int v = 0; v < NF; ++v) {
-    __syncthreads();
-    for (int n = tid; n < Q2; n += NT) {
-      double fx, fy;
-      if (SHALLOW) {
-        const double phi = Un[n], mx = Un[Q2 + n], my = Un[2 * Q2 + n];
-        const double u = mx / phi, vv = my / phi, p = 0.5 * (phi * phi);
-        if (v == 0) { fx = mx; fy = my; }
-        else if (v == 1) { fx = mx * u + p; fy = mx * vv; }
-        else { fx = my * u; fy = my * vv + p; }
-      } else {
-        const double r = Un[n], ru = Un[Q2 + n], rv = Un[2 * Q2 + n], rE = Un[3 * Q2 + n];
-        const double ir = 1.0 / r, u = ru * ir, vv = rv * ir;
-        const double p = gm1 * (rE - 0.5 * (ru * u + rv * vv));
-        if (v == 0) { fx = ru; fy = rv; }
-        else if (v == 1) { fx = ru * u + p; fy = ru * vv; }
-        else if (v == 2) { fx = rv * u; fy = rv * vv + p; }
-        else { const double h = rE + p; fx = u * h; fy = vv * h; }
-      }
-      Fc[n] = fx; Fc[Q2 + n] = fy;
-    }
-    __syncthreads();
-    const double *Dn = CT<P>::DN();
-#pragma unroll
-    for (int rr = 0; rr < NPT; ++rr) {
-      const int n = tid + rr * NT;
-      if (n < Q2) {
-        const int i = n / Q, j = n - i * Q;
-        double acc = 0.0;
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-          const double *F = Fc + x * Q2;
-          double dr = 0.0, ds = 0.0;
-#pragma unroll
-          for (int m = 0; m < Q; ++m) {
-            dr = fma(Dn[i * Q + m], F[m * Q + j], dr);
-            ds = fma(Dn[j * Q + m], F[i * Q + m], ds);
-          }
-          acc = fma(ge[x * Q2 + n], dr, acc);
-          acc = fma(ge[(2 + x) * Q2 + n], ds, acc);
-        }
-        Fw[v * Q2 + n] = acc;
-      }
-    }
-  }
-  __syncthreads();
-  // edges: Gauss -> GLL edge nodes for NF x 4 blocks, then wJ / w_end at the edge nodes
-  contract<4 * NF, QF, 1, 1, Q, NT>(CT<P>::GL(), gF, S1, tid);   // [NF 4][Q]
-  __syncthreads();
-  {
-    const double wend = CT<P>::BW()[0] / CT<P>::BT()[0];   // GLL end weight: phi_0 w_0 / phi_0
-#pragma unroll
-    for (int rr = 0; rr < NPT; ++rr) {
-      const int n = tid + rr * NT;
-      if (n < Q2) {
-        const int i = n / Q, j = n - i * Q;
-        const double c = ge[4 * Q2 + n] / wend;
-        // edge 0: j = 0 (along i), 1: i = Q-1 (along j), 2: j = Q-1 (along i), 3: i = 0 (along j)
-        int es[2], t[2], cnt = 0;
-        if (j == 0) { es[cnt] = 0; t[cnt++] = i; }
-        if (i == Q - 1) { es[cnt] = 1; t[cnt++] = j; }
-        if (j == Q - 1) { es[cnt] = 2; t[cnt++] = i; }
-        if (i == 0) { es[cnt] = 3; t[cnt++] = j; }
-        for (int f = 0; f < cnt; ++f)
-#pragma unroll
-          for (int v = 0; v < NF; ++v)
-            Fw[v * Q2 + n] = fma(c, S1[(v * 4 + es[f]) * Q + t[f]], Fw[v * Q2 + n]);
-      }
-    }
-  }
-  __syncthreads();
-  }   // !PROJ
-  // b = B^T (wJ f): into Un (compact [NF][L2])
+// x = M^-1 B^T Fw for NF fields, Fw [NF][Q2] the weighted nodal values (wJ f),
+// M = B^T diag(wJ) B, by conjugate gradients on all fields at once from
+// x0 = b / mean(J); x goes to out[v * ostride + k].  Scratch: S1, S2 [NF Q2],
+// AP [NF L2]; Fw is overwritten.
+template <int P, int NF, int NT>
+__device__ void curved_project2(double *Fw, double *S1, double *S2, double *AP, double *red, const double *ge,
+                                double jav, double *out, size_t ostride, int *iters, int tid) {
+  constexpr int L = P + 1, Q = P + 2, L2 = L * L, Q2 = Q * Q;
+  constexpr int NW = NT / 64, NML = (NF * L2 + NT - 1) / NT;
+  // b = B^T (wJ f): into AP (compact [NF][L2])
   contract<NF, Q, Q, 1, L, NT>(CT<P>::BL(), Fw, S1, tid);   // [NF][L][Q]
   __syncthreads();
-  contract<NF, L, Q, 2, L, NT>(CT<P>::BL(), S1, Un, tid);   // [NF][L][L]
+  contract<NF, L, Q, 2, L, NT>(CT<P>::BL(), S1, AP, tid);   // [NF][L][L]
   __syncthreads();
 
-  // conjugate gradients on M x = b, all fields at once; x0 = b / mean(J)
-  double *Pv = Fw, *AP = Un;
+  double *Pv = Fw;
   double x[NML], res[NML], bvec[NML];
   double part[NF], bb[NF], rs[NF], pap[NF];
-  const double jinv = 1.0 / a.jav[e];
+  const double jinv = 1.0 / jav;
 #pragma unroll
   for (int f = 0; f < NF; ++f) part[f] = 0.0;
 #pragma unroll
@@ -235,7 +87,7 @@ k_rhs2c(CurvedArgs a) {
     const int g = tid + k * NT;
     x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
     if (g < NF * L2) {
-      bvec[k] = Un[g];
+      bvec[k] = AP[g];
       x[k] = bvec[k] * jinv;
       part[g / L2] = fma(bvec[k], bvec[k], part[g / L2]);
     }
@@ -328,28 +180,403 @@ k_rhs2c(CurvedArgs a) {
     const int g = tid + k * NT;
     if (g < NF * L2) {
       const int v = g / L2;
-      a.R[((size_t)v * ne + e) * L2 + (g - v * L2)] = x[k];
+      out[v * ostride + (g - v * L2)] = x[k];
     }
   }
-  if (tid == 0 && a.iters) atomicMax(a.iters, it);
+  if (tid == 0 && iters) atomicMax(iters, it);
 }
 
-template <int P, bool SHALLOW>
-static hipError_t launch_c2s(const CurvedArgs &a) {
-  constexpr int sh = C2<P, SHALLOW ? 3 : 4>::lds;
-  if (a.project) {
-    (void)hipFuncSetAttribute((const void *)k_rhs2c<P, SHALLOW, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL((k_rhs2c<P, SHALLOW, true>), dim3(a.ne), dim3(128), sh, a.stream, a);
-  } else {
-    (void)hipFuncSetAttribute((const void *)k_rhs2c<P, SHALLOW, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-    hipLaunchKernelGGL((k_rhs2c<P, SHALLOW, false>), dim3(a.ne), dim3(128), sh, a.stream, a);
+// edge corrections gF [NF][4][QF] (Gauss points) lifted to the edge nodes and
+// added to Fw [NF][Q2] with weight wJ / w_end: G2GL(g) / w_end at the edge
+// nodes (src/Quad.cpp:1971-2038), before the wJ of the inner product
+template <int P, int NF, int NT>
+__device__ __forceinline__ void lift_edges2(const double *gF, double *S1, double *Fw, const double *ge, int tid) {
+  constexpr int Q = P + 2, Q2 = Q * Q, QF = P + 1, NPT = (Q2 + NT - 1) / NT;
+  contract<4 * NF, QF, 1, 1, Q, NT>(CT<P>::GL(), gF, S1, tid);   // [NF 4][Q]
+  __syncthreads();
+  const double wend = CT<P>::BW()[0] / CT<P>::BT()[0];   // GLL end weight: phi_0 w_0 / phi_0
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int n = tid + rr * NT;
+    if (n < Q2) {
+      const int i = n / Q, j = n - i * Q;
+      const double c = ge[4 * Q2 + n] / wend;
+      // edge 0: j = 0 (along i), 1: i = Q-1 (along j), 2: j = Q-1 (along i), 3: i = 0 (along j)
+      int es[2], t[2], cnt = 0;
+      if (j == 0) { es[cnt] = 0; t[cnt++] = i; }
+      if (i == Q - 1) { es[cnt] = 1; t[cnt++] = j; }
+      if (j == Q - 1) { es[cnt] = 2; t[cnt++] = i; }
+      if (i == 0) { es[cnt] = 3; t[cnt++] = j; }
+      for (int f = 0; f < cnt; ++f)
+#pragma unroll
+        for (int v = 0; v < NF; ++v)
+          Fw[v * Q2 + n] = fma(c, S1[(v * 4 + es[f]) * Q + t[f]], Fw[v * Q2 + n]);
+    }
   }
-  return hipGetLastError();
+  __syncthreads();
+}
+
+// modal fields M [NF][L2] (LDS) -> nodal Out [NF][Q2], through S [NF][Q][L]
+template <int P, int NF, int NT>
+__device__ __forceinline__ void nodal2(const double *M, double *S, double *Out, int tid) {
+  constexpr int L = P + 1, Q = P + 2;
+  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), M, S, tid);   // [NF][Q][L]
+  __syncthreads();
+  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), S, Out, tid);   // [NF][Q][Q]
+  __syncthreads();
+}
+
+// weighted divergence wJ (d/dx Fx + d/dy Fy) at every node of an element:
+// sum_x sum_d (wJ dxi_d/dx_x) D_d F_x (src/Quad.cpp:2309-2330 with the curved metrics)
+template <int P, int NT>
+__device__ __forceinline__ void divergence2c(const double *Fc, const double *ge, double *out, int tid) {
+  constexpr int Q = P + 2, Q2 = Q * Q, NPT = (Q2 + NT - 1) / NT;
+  const double *Dn = CT<P>::DN();
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int n = tid + rr * NT;
+    if (n < Q2) {
+      const int i = n / Q, j = n - i * Q;
+      double acc = 0.0;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const double *F = Fc + x * Q2;
+        double dr = 0.0, ds = 0.0;
+#pragma unroll
+        for (int m = 0; m < Q; ++m) {
+          dr = fma(Dn[i * Q + m], F[m * Q + j], dr);
+          ds = fma(Dn[j * Q + m], F[i * Q + m], ds);
+        }
+        acc = fma(ge[x * Q2 + n], dr, acc);
+        acc = fma(ge[(2 + x) * Q2 + n], ds, acc);
+      }
+      out[n] = acc;
+    }
+  }
+}
+
+// PROJ: project the nodal fields a.U [NF][ne][Q^2] instead (the forward
+// transform of a curved element: x = M^-1 B^T (wJ u)).
+// NS: add the viscous terms of V and TV (k_visc2c) to the Euler residual.
+template <int P, bool SHALLOW, bool PROJ, bool NS>
+__global__ void __launch_bounds__(128)
+k_rhs2c(CurvedArgs a) {
+  constexpr int NF = SHALLOW ? 3 : 4;
+  using S = C2<P, NF>;
+  constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, Q2 = S::Q2;
+  constexpr int NT = S::NT;
+  extern __shared__ double sm[];
+  double *Un = sm, *Fw = Un + NF * Q2, *S1 = Fw + NF * Q2, *S2 = S1 + S::SZ, *gF = S2 + S::SZ;
+  double *Fc = gF + 4 * NF * QF, *red = Fc + 2 * Q2;
+  static_assert(NF * L * Q <= S::SZ && NF * L2 <= S::SZ, "scratch too small");
+  static_assert(S::lds <= 64 * 1024, "LDS");
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const Geo &geo = a.geo;
+  const double *ge = a.gm + (size_t)e * 5 * Q2;   // wJ dxi_d/dx_x (rows r, s), wJ
+  const double gm1 = a.phys.gamma - 1.0;
+
+  if (PROJ) {
+    for (int i = tid; i < NF * Q2; i += NT) {
+      const int v = i / Q2, n = i - v * Q2;
+      Fw[i] = ge[4 * Q2 + n] * a.U[((size_t)v * ne + e) * Q2 + n];
+    }
+    __syncthreads();
+  } else {
+  // modal state -> nodal
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    S1[i] = a.U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), S1, S2, tid);   // [NF][Q][L]
+  __syncthreads();
+  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), S2, Un, tid);   // [NF][Q][Q]
+
+  // edge corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points
+  bool bad = false;
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, q = i - s * QF;
+    const int es = e * 4 + s;
+    const double *fg = a.fgeo + ((size_t)es * QF + q) * 4;
+    double nx = fg[0], ny = fg[1];
+    const double jr = fg[3];
+    const int nb = geo.nbr[es];
+    double UL[4], UR[4], FL[4], FR[4], Fh[4], lL, lR;
+    const double *tl = a.T + ((size_t)e * 4 + s) * NF * QF;
+#pragma unroll
+    for (int v = 0; v < NF; ++v) UL[v] = tl[v * QF + q];
+    if (nb >= 0) {
+      const int qn = geo.perm[geo.permid[es] * QF + q];
+      if (!geo.master[es]) {   // the pair's flux uses the master side's normal (both sides' normals
+                               // come from their own interpolated metrics and differ slightly)
+        const double *fm = a.fgeo + (((size_t)nb * 4 + geo.nside[es]) * QF + qn) * 4;
+        nx = -fm[0]; ny = -fm[1];
+      }
+      const double *tr = a.T + ((size_t)nb * 4 + geo.nside[es]) * NF * QF;
+#pragma unroll
+      for (int v = 0; v < NF; ++v) UR[v] = tr[v * QF + qn];
+    } else if (SHALLOW) {
+      ghost_shallow2(-1 - nb, UL, nx, ny, a.phys.bcU, UR);
+    } else {   // no Euler boundaries on quads (rejected at creation)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) UR[v] = UL[v];
+    }
+    double gv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (NS && nb >= 0) {   // viscous: own normal flux less the average 1/2 (own - neighbour's), negated
+      const int qn = geo.perm[geo.permid[es] * QF + q];
+      const double *to = a.TV + ((size_t)es * 3) * QF;
+      const double *tn = a.TV + (((size_t)nb * 4 + geo.nside[es]) * 3) * QF;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double own = to[c * QF + q], avg = 0.5 * (own + -1.0 * tn[c * QF + qn]);
+        gv[1 + c] = jr * (avg - own);
+      }
+    }
+    if (SHALLOW) {
+      bad |= shallow_normal_flux2(UL, nx, ny, FL, lL);
+      bad |= shallow_normal_flux2(UR, nx, ny, FR, lR);
+      const double lmax = fmax(lL, lR);
+#pragma unroll
+      for (int v = 0; v < 3; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+    } else {
+      bool bl, br;
+      euler_normal_flux2(UL, nx, ny, gm1, FL, lL, bl);
+      euler_normal_flux2(UR, nx, ny, gm1, FR, lR, br);
+      bad |= bl | br;
+      if (a.phys.flux == 1) {
+        roe_flux2(UL, UR, nx, ny, a.phys.gamma, FL, Fh);
+      } else {
+        const double lmax = fmax(lL, lR);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NF; ++v) gF[(v * 4 + s) * QF + q] = NS ? jr * (Fh[v] - FL[v]) - gv[v] : jr * (Fh[v] - FL[v]);
+  }
+  if (bad) atomicOr(a.flag, 1);
+
+  // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by field
+  for (int v = 0; v < NF; ++v) {
+    __syncthreads();
+    for (int n = tid; n < Q2; n += NT) {
+      double fx, fy;
+      if (SHALLOW) {
+        const double phi = Un[n], mx = Un[Q2 + n], my = Un[2 * Q2 + n];
+        const double u = mx / phi, vv = my / phi, p = 0.5 * (phi * phi);
+        if (v == 0) { fx = mx; fy = my; }
+        else if (v == 1) { fx = mx * u + p; fy = mx * vv; }
+        else { fx = my * u; fy = my * vv + p; }
+      } else {
+        const double r = Un[n], ru = Un[Q2 + n], rv = Un[2 * Q2 + n], rE = Un[3 * Q2 + n];
+        const double ir = 1.0 / r, u = ru * ir, vv = rv * ir;
+        const double p = gm1 * (rE - 0.5 * (ru * u + rv * vv));
+        if (v == 0) { fx = ru; fy = rv; }
+        else if (v == 1) { fx = ru * u + p; fy = ru * vv; }
+        else if (v == 2) { fx = rv * u; fy = rv * vv + p; }
+        else { const double h = rE + p; fx = u * h; fy = vv * h; }
+      }
+      Fc[n] = fx; Fc[Q2 + n] = fy;
+    }
+    __syncthreads();
+    divergence2c<P, NT>(Fc, ge, Fw + v * Q2, tid);
+    if (NS && v > 0) {
+      const double *vn = a.V + ((size_t)e * 3 + (v - 1)) * Q2;
+      for (int n = tid; n < Q2; n += NT) Fw[v * Q2 + n] -= vn[n];
+    }
+  }
+  __syncthreads();
+  lift_edges2<P, NF, NT>(gF, S1, Fw, ge, tid);
+  }   // !PROJ
+  curved_project2<P, NF, NT>(Fw, S1, S2, Un, red, ge, a.jav[e], a.R + (size_t)e * L2, (size_t)ne * L2, a.iters, tid);
+}
+
+// Lifted, projected gradients of the primitives (u, v, e) of a curved quad
+// (Euler::gradient_lift path of Navier_Stokes::diffusion, src/Euler.cpp:1406-1456):
+// at the nodes g_x = dxi_d/dx_x D_d q, plus on each side G2GL(J_side/J_vol
+// ({q} - q-) n_x) / w_end with the side's own normal, then projected through
+// the curved mass matrix; G [ne][6][L2] in the order (u_x, u_y, v_x, v_y, e_x, e_y).
+template <int P>
+__global__ void __launch_bounds__(128)
+k_grad2c(CurvedArgs a) {
+  constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q, NT = 128;
+  constexpr int NPT = (Q2 + NT - 1) / NT;
+  extern __shared__ double sm[];
+  double *Un = sm, *Fw = Un + 4 * Q2, *S1 = Fw + 6 * Q2, *S2 = S1 + 6 * Q2, *AP = S2 + 6 * Q2;
+  double *gF = AP + 6 * L2, *Pq = gF + 6 * 4 * QF, *red = Pq + Q2;
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const Geo &geo = a.geo;
+  const double *ge = a.gm + (size_t)e * 5 * Q2;
+  for (int i = tid; i < 4 * L2; i += NT) {
+    const int v = i / L2;
+    S2[i] = a.U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  nodal2<P, 4, NT>(S2, S1, Un, tid);
+  // jumps of the primitives at the Gauss points: J_side/J_vol ({q} - q-) n
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, q = i - s * QF;
+    const int es = e * 4 + s;
+    const double *fg = a.fgeo + ((size_t)es * QF + q) * 4;
+    const double nx = fg[0], ny = fg[1], jr = fg[3];
+    const int nb = geo.nbr[es];   // Navier-Stokes quads have no boundary sides (rejected at creation)
+    const int qn = nb >= 0 ? geo.perm[geo.permid[es] * QF + q] : q;
+    const double *tl = a.T + ((size_t)es) * 4 * QF;
+    const double *tr = nb >= 0 ? a.T + ((size_t)nb * 4 + geo.nside[es]) * 4 * QF : tl;
+    double pl[3], pr[3];
+    {
+      const double r = tl[q], u = tl[QF + q] / r, v = tl[2 * QF + q] / r;
+      double en = tl[3 * QF + q] / r;
+      double wk = u * u; wk = v * v + wk;
+      pl[0] = u; pl[1] = v; pl[2] = -0.5 * wk + en;
+    }
+    {
+      const double r = tr[qn], u = tr[QF + qn] / r, v = tr[2 * QF + qn] / r;
+      double en = tr[3 * QF + qn] / r;
+      double wk = u * u; wk = v * v + wk;
+      pr[0] = u; pr[1] = v; pr[2] = -0.5 * wk + en;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double jmp = jr * (0.5 * (pl[c] + pr[c]) - pl[c]);
+      gF[((2 * c) * 4 + s) * QF + q] = nx * jmp;
+      gF[((2 * c + 1) * 4 + s) * QF + q] = ny * jmp;
+    }
+  }
+  // nodal gradients, weighted: wJ dq/dx_x = sum_d (wJ dxi_d/dx_x) D_d q
+  const double *Dn = CT<P>::DN();
+  for (int c = 0; c < 3; ++c) {
+    for (int n = tid; n < Q2; n += NT) {
+      const double r = Un[n], u = Un[Q2 + n] / r, v = Un[2 * Q2 + n] / r;
+      double val;
+      if (c == 0) val = u;
+      else if (c == 1) val = v;
+      else {
+        double en = Un[3 * Q2 + n] / r;
+        double wk = u * u; wk = v * v + wk;
+        val = -0.5 * wk + en;
+      }
+      Pq[n] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < NPT; ++rr) {
+      const int n = tid + rr * NT;
+      if (n < Q2) {
+        const int i = n / Q, j = n - i * Q;
+        double dr = 0.0, ds = 0.0;
+#pragma unroll
+        for (int m = 0; m < Q; ++m) {
+          dr = fma(Dn[i * Q + m], Pq[m * Q + j], dr);
+          ds = fma(Dn[j * Q + m], Pq[i * Q + m], ds);
+        }
+        Fw[(2 * c) * Q2 + n] = fma(ge[2 * Q2 + n], ds, ge[n] * dr);
+        Fw[(2 * c + 1) * Q2 + n] = fma(ge[3 * Q2 + n], ds, ge[Q2 + n] * dr);
+      }
+    }
+    __syncthreads();
+  }
+  lift_edges2<P, 6, NT>(gF, S1, Fw, ge, tid);
+  curved_project2<P, 6, NT>(Fw, S1, S2, AP, red, ge, a.jav[e], a.G + (size_t)e * 6 * L2, L2, a.iters, tid);
+}
+
+// Viscous terms of a curved quad from the state and the lifted gradients G:
+// the stresses and heat flux at the nodes (src/Navier_Stokes.cpp:113-258),
+// V [ne][3][Q2] = wJ div F_v for (rho u, rho v, rho E), and each side's normal
+// flux F_v.n at its Gauss points with its own normal into TV [ne][4][3][QF]
+// (interpolated first, then dotted, as the reference's side flux is formed).
+template <int P>
+__global__ void __launch_bounds__(128)
+k_visc2c(CurvedArgs a) {
+  constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q, NT = 128;
+  extern __shared__ double sm[];
+  double *Un = sm, *Gn = Un + 4 * Q2, *Fv = Gn + 6 * Q2, *M = Fv + 6 * Q2, *S = M + 6 * L2;
+  double *Lay = S + 6 * Q * L, *Wf = Lay + 24 * Q;
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const double *ge = a.gm + (size_t)e * 5 * Q2;
+  for (int i = tid; i < 4 * L2; i += NT) {
+    const int v = i / L2;
+    M[i] = a.U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  nodal2<P, 4, NT>(M, S, Un, tid);
+  for (int i = tid; i < 6 * L2; i += NT) M[i] = a.G[(size_t)e * 6 * L2 + i];
+  __syncthreads();
+  nodal2<P, 6, NT>(M, S, Gn, tid);
+  const double mu = a.phys.vis, gp = a.phys.gamma / a.phys.Pr;
+  for (int n = tid; n < Q2; n += NT) {
+    const double r = Un[n], u = Un[Q2 + n] / r, v = Un[2 * Q2 + n] / r;
+    const double ux = Gn[n], uy = Gn[Q2 + n], vx = Gn[2 * Q2 + n], vy = Gn[3 * Q2 + n];
+    const double ex = Gn[4 * Q2 + n], ey = Gn[5 * Q2 + n];
+    const double t12 = mu * (uy + vx);
+    const double t11 = mu * ((2.0 / 3.0) * (2.0 * ux - vy));
+    const double t22 = mu * ((2.0 / 3.0) * (2.0 * vy - ux));
+    Fv[n] = t11; Fv[Q2 + n] = t12;                      // x momentum: (t11, t12)
+    Fv[2 * Q2 + n] = t12; Fv[3 * Q2 + n] = t22;         // y momentum: (t12, t22)
+    Fv[4 * Q2 + n] = gp * (mu * ex) + fma(t12, v, t11 * u);
+    Fv[5 * Q2 + n] = gp * (mu * ey) + fma(t22, v, t12 * u);
+  }
+  __syncthreads();
+  // side values at the edge nodes -> Gauss points, then n . F with the side's normal
+  for (int i = tid; i < 4 * Q; i += NT) {
+    const int s = i / Q, j = i - s * Q;
+    int k;
+    switch (s) {
+      case 0: k = j * Q; break;
+      case 1: k = (Q - 1) * Q + j; break;
+      case 2: k = j * Q + Q - 1; break;
+      default: k = j; break;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) Lay[(s * 6 + c) * Q + j] = Fv[c * Q2 + k];
+  }
+  __syncthreads();
+  contract<1, 24, Q, 2, QF, NT>(CT<P>::Ig(), Lay, Wf, tid);
+  __syncthreads();
+  for (int i = tid; i < 4 * 3 * QF; i += NT) {
+    const int s = i / (3 * QF), r = i - s * 3 * QF, c = r / QF, q = r - c * QF;
+    const double *fg = a.fgeo + (((size_t)e * 4 + s) * QF + q) * 4;
+    a.TV[((size_t)e * 4 + s) * 3 * QF + r] =
+        fma(fg[1], Wf[(s * 6 + 2 * c + 1) * QF + q], fg[0] * Wf[(s * 6 + 2 * c) * QF + q]);
+  }
+  for (int c = 0; c < 3; ++c) divergence2c<P, NT>(Fv + 2 * c * Q2, ge, a.V + ((size_t)e * 3 + c) * Q2, tid);
+}
+
+template <int P> struct C2NS {   // LDS bytes of the viscous kernels
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q;
+  static constexpr int grad = (4 * Q2 + 18 * Q2 + 6 * L2 + 24 * QF + Q2 + 8 * 2) * 8;
+  static constexpr int visc = (16 * Q2 + 6 * L2 + 6 * Q * L + 24 * Q + 24 * QF) * 8;
+  static_assert(grad <= 64 * 1024 && visc <= 64 * 1024, "LDS");
+};
+
+template <int P, bool SHALLOW, bool PROJ, bool NS>
+static void launch_c2k(const CurvedArgs &a) {
+  constexpr int sh = C2<P, SHALLOW ? 3 : 4>::lds;
+  (void)hipFuncSetAttribute((const void *)k_rhs2c<P, SHALLOW, PROJ, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+  hipLaunchKernelGGL((k_rhs2c<P, SHALLOW, PROJ, NS>), dim3(a.ne), dim3(128), sh, a.stream, a);
 }
 
 template <int P>
 static hipError_t launch_c2(const CurvedArgs &a) {
-  return a.shallow ? launch_c2s<P, true>(a) : launch_c2s<P, false>(a);
+  if (a.project) {
+    if (a.shallow) launch_c2k<P, true, true, false>(a);
+    else launch_c2k<P, false, true, false>(a);
+  } else if (a.shallow) {
+    launch_c2k<P, true, false, false>(a);
+  } else if (!a.ns) {
+    launch_c2k<P, false, false, false>(a);
+  } else if (a.op == 1) {
+    constexpr int sh = C2NS<P>::grad;
+    (void)hipFuncSetAttribute((const void *)k_grad2c<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+    hipLaunchKernelGGL(k_grad2c<P>, dim3(a.ne), dim3(128), sh, a.stream, a);
+  } else if (a.op == 2) {
+    constexpr int sh = C2NS<P>::visc;
+    (void)hipFuncSetAttribute((const void *)k_visc2c<P>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+    hipLaunchKernelGGL(k_visc2c<P>, dim3(a.ne), dim3(128), sh, a.stream, a);
+  } else {
+    launch_c2k<P, false, false, true>(a);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_curved2(const CurvedArgs &a) {

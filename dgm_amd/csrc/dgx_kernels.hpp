@@ -107,6 +107,13 @@ struct CurvedArgs {
   int *flag = nullptr, *iters = nullptr;
   bool project = false;                      // U nodal [5][ne][Q^3] -> R modal: M^-1 B^T (wJ U)
   bool shallow = false;                      // quads: shallow water (3 fields) instead of Euler
+  // quads, Navier-Stokes (BR1): op 1 writes the lifted, projected gradients of
+  // (u, v, e) into G [ne][6][L^2]; op 2 the weighted nodal viscous divergence
+  // into V [ne][3][Q^2] and each side's normal viscous flux into TV
+  // [ne][4][3][QF]; op 0 (the residual) then adds both.
+  bool ns = false;
+  int op = 0;
+  double *G = nullptr, *V = nullptr, *TV = nullptr;
   hipStream_t stream = nullptr;
 };
 // Quads use the same arguments with gm [ne][5][Q^2] (wJ dxi_d/dx_x, wJ),

@@ -105,7 +105,7 @@ typedef struct {
   double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries (Shallow: the first 3) */
   const double *xnodes;  /* curved elements, FP64: hexes (Euler3d, p <= 6) coordinates [3][ne][q^3]
                             of every element's GLL nodes, node (i*q + j)*q + k; quads (Euler,
-                            Shallow) [2][ne][q^2], node i*q + j; NULL: affine */
+                            Navier_Stokes, Shallow) [2][ne][q^2], node i*q + j; NULL: affine */
   int no_balance;   /* Shallow: 1 leaves out the balance term.  By default, as
                        compiled in the reference (src/Shallow.cpp:251-266), the
                        domain's first residual is stored and subtracted from it

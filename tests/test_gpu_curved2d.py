@@ -2,7 +2,9 @@
 projection by in-workgroup conjugate gradients) against the oracle's
 restatement of Quad::compute_curved_geometry (src/Quad.cpp:1309-1452).
 
-Bar: relative L2 per field <= 1e-12 for one residual, <= 1e-11 after steps.
+Bar: relative L2 per field <= 1e-12 for one residual, <= 1e-11 after steps;
+the reference's cvquad run (2D Navier-Stokes on arc-sided quads) within its
+own dgm_diff tolerances.
 At p >= 8 on strongly curved quads the residual bar is 2.5e-12: the GPU solves
 the curved mass matrix by conjugate gradients, the oracle (as the reference)
 by its Cholesky factor, and at those orders their roundings differ by up to
@@ -11,8 +13,9 @@ by its Cholesky factor, and at those orders their roundings differ by up to
 import numpy as np
 import pytest
 
-from dgm_amd import dgx, mesh as meshgen
-from oracle_lib import SHALLOW, Oracle
+from dgm_amd import dgx, formats, mesh as meshgen
+from fixture_runs import GOLDEN, initial_state, load_case
+from oracle_lib import SHALLOW, Oracle, dgm_diff
 from test_gpu_parity import TOL_RHS, TOL_STEP, projected_state, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -125,3 +128,58 @@ def test_advance_curved2d_lsrk14():
     print(f"[curved2d] LSRK14 3 steps {err}", flush=True)
     assert np.all(err <= TOL_STEP), err
     gpu.close()
+
+
+@pytest.mark.parametrize("p", [2, 5, 8, 10])
+def test_rhs_ns2d_curved_periodic(p):
+    """BR1 Navier-Stokes on curved quads: lifted gradients, viscous flux with
+    each side's own normal, one projection of the sum."""
+    print(f"[curved2d] navier-stokes p={p}", flush=True)
+    vis = 0.05
+    m = meshgen.periodic_quads(4, 3, 0.0, TWO_PI, 0.0, TWO_PI)
+    xn = meshgen.mapped_nodes(m, p, _wavy(TWO_PI, 0.25))
+    orc = Oracle(m, p, 2, vis=vis)
+    orc.set_nodes(xn)
+    Up = projected_state(orc, _euler_state(xn))
+    F_ref = orc.rhs(Up)
+    eul = Oracle(m, p, 1)
+    eul.set_nodes(xn)
+    visc = rel_l2(eul.rhs(Up), F_ref)
+    assert np.all(visc[1:] > 1e-3), visc                 # the viscous terms are resolved by the bar
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES, vis=vis, xnodes=xn)
+    gpu.upload(Up)
+    err = rel_l2(gpu.rhs(), F_ref)
+    print(f"[curved2d] rel L2 per field: {err}", flush=True)
+    assert np.all(err <= (TOL_RHS if p < 8 else 2.5e-12)), err
+    gpu.close()
+
+
+def test_cvquad_golden_run():
+    """runs/cvquad end to end on the GPU: 2D Navier-Stokes, TVD-RK3, p = 10,
+    four quads with circular-arc sides, 1000 steps with the state projected at
+    each restart write (every 500 steps), against cvquad.ref with the
+    reference's own aeps = reps = 5e-13 (runs/cvquad.tst:22-23)."""
+    prm, m, icd = load_case("cvquad")
+    p = int(prm["p"])
+    xn, _ = meshgen.curved_quad_nodes(m, p, formats.read_crv(f"{GOLDEN}/cvquad.crv"))
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES, vis=float(prm["vis"]), xnodes=xn)
+    gpu.upload(initial_state(icd, gpu.coords(), 2))
+    gpu.upload(gpu.download())                           # Domain::solve projects the initial state
+    Nt, Ntout, dt, it = int(prm["Nt"]), int(prm["Ntout"]), float(prm["dt"]), int(prm["inttype"])
+    done = 0
+    while done < Nt:
+        n = min(Ntout, Nt - done)
+        gpu.advance(it, dt, n)
+        done += n
+        gpu.upload(gpu.download())                       # each restart write projects the state
+        print(f"[curved2d] cvquad {done} steps", flush=True)
+    Uh = gpu.download(dgx.DGX_MODAL)
+    gpu.close()
+    o = Oracle(m, p, 2, vis=float(prm["vis"]))
+    o.set_nodes(xn)
+    ref = formats.read_restart(f"{GOLDEN}/cvquad.ref", m.ne)
+    rn, dn = dgm_diff(o, ref.data, Uh)
+    print(f"[curved2d] cvquad dgm_diff absolute {dn.max():.3e}", flush=True)
+    assert np.all(dn <= 5e-13), dn
+    mask = rn > np.sqrt(np.finfo(float).eps)
+    assert np.all(np.where(mask, dn / np.where(mask, rn, 1.0), 0.0) <= 5e-13)

@@ -117,14 +117,14 @@ def test_shallow_create_rejects_bad_arguments(lib, case, msg):
     assert not h.value
 
 
-def test_curved_quads_rejected_for_navier_stokes(lib):
-    m = meshgen.periodic_quads(2, 2)
+def test_curved_navier_stokes_quads_rejected_with_boundaries(lib):
+    m = meshgen.box_quads(2, 2, bc=(0, None, 0, None))
     d, keep = _desc(m, 3, 2, vis=0.01)
     xn = np.ascontiguousarray(meshgen.quad_nodes(m, 3))
     d.xnodes = xn.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     h = ctypes.c_void_p()
     assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
-    assert b"curved elements" in lib.dgx_last_error()
+    assert b"boundary sides are available" in lib.dgx_last_error()
 
 
 def test_mixed_rejected_in_2d(lib):
