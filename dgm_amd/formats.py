@@ -155,6 +155,158 @@ def boundary_con(con: np.ndarray, bc_rows, types: Dict[str, int]) -> np.ndarray:
     return np.asarray(out, dtype=np.int32).reshape(-1, 4)
 
 
+_FUNCS = {"sin": np.sin, "cos": np.cos, "tan": np.tan, "asin": np.arcsin, "acos": np.arccos,
+          "atan": np.arctan, "sinh": np.sinh, "cosh": np.cosh, "tanh": np.tanh, "exp": np.exp,
+          "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "abs": np.abs, "fabs": np.abs,
+          "atan2": np.arctan2, "pow": np.power}
+_CONSTS = {"PI": np.pi, "pi": np.pi}
+
+
+def _tokens(text: str):
+    import re
+    spec = r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|([A-Za-z_]\w*)|(:=|[-+*/^(),;{}]))"
+    pos, out = 0, []
+    text = text.rstrip()
+    while pos < len(text):
+        m = re.compile(spec).match(text, pos)
+        if not m or m.end() == pos:
+            raise ValueError(f"bad character in function text near {text[pos:pos + 20]!r}")
+        num, name, op = m.groups()
+        out.append(("n", float(num)) if num else (("v", name) if name else ("o", op)))
+        pos = m.end()
+        while pos < len(text) and text[pos].isspace():
+            pos += 1
+    return out
+
+
+class _ExprParser:
+    """Arithmetic expressions of the reference's function definitions
+    (src/Parser.y): + - * / ^ (right associative, above unary minus),
+    parentheses, the usual elementary functions, PI, and the variables x, y,
+    z, t.  Compiles to a closure over numpy; nothing in the text is executed."""
+
+    def __init__(self, toks):
+        self.t, self.i = toks, 0
+
+    def peek(self):
+        return self.t[self.i] if self.i < len(self.t) else ("o", None)
+
+    def take(self, op=None):
+        tok = self.peek()
+        if op is not None and tok != ("o", op):
+            raise ValueError(f"expected {op!r}, found {tok[1]!r}")
+        self.i += 1
+        return tok
+
+    def expr(self):
+        f = self.term()
+        while self.peek() in (("o", "+"), ("o", "-")):
+            op = self.take()[1]
+            g = self.term()
+            f = (lambda a, b: lambda env: a(env) + b(env))(f, g) if op == "+" else \
+                (lambda a, b: lambda env: a(env) - b(env))(f, g)
+        return f
+
+    def term(self):
+        f = self.unary()
+        while self.peek() in (("o", "*"), ("o", "/")):
+            op = self.take()[1]
+            g = self.unary()
+            f = (lambda a, b: lambda env: a(env) * b(env))(f, g) if op == "*" else \
+                (lambda a, b: lambda env: a(env) / b(env))(f, g)
+        return f
+
+    def unary(self):
+        if self.peek() == ("o", "-"):
+            self.take()
+            f = self.unary()
+            return lambda env: -f(env)
+        if self.peek() == ("o", "+"):
+            self.take()
+            return self.unary()
+        return self.power()
+
+    def power(self):
+        f = self.atom()
+        if self.peek() == ("o", "^"):
+            self.take()
+            g = self.unary()
+            return lambda env: np.power(f(env), g(env))
+        return f
+
+    def atom(self):
+        kind, val = self.take()
+        if kind == "n":
+            return lambda env: val
+        if kind == "v":
+            if self.peek() == ("o", "("):
+                if val not in _FUNCS:
+                    raise ValueError(f"unknown function {val}")
+                fn, args = _FUNCS[val], []
+                self.take("(")
+                args.append(self.expr())
+                while self.peek() == ("o", ","):
+                    self.take()
+                    args.append(self.expr())
+                self.take(")")
+                return lambda env: fn(*[a(env) for a in args])
+            if val in _CONSTS:
+                c = _CONSTS[val]
+                return lambda env: c
+            return lambda env: env[val]
+        if val == "(":
+            f = self.expr()
+            self.take(")")
+            return f
+        raise ValueError(f"unexpected {val!r}")
+
+
+def read_functions(path: str) -> Dict[str, Dict[str, object]]:
+    """Function definitions of a boundary-type or source file (``root.bct``,
+    ``root.src``): blocks ``function NAME { var := expression; ... }``, the
+    form FuncBC and FuncSource read (src/BC.hpp, src/Source.hpp:190-230).
+    Returns {NAME: {var: f(env)}} where env maps x, y, z, t to arrays; the
+    components keep their order in the block."""
+    with open(path) as f:
+        text = f.read().replace("\\\n", " ")
+    toks = _tokens(text)
+    out: Dict[str, Dict[str, object]] = {}
+    i = 0
+    while i < len(toks):
+        if toks[i] == ("v", "function") or toks[i] == ("v", "Function"):
+            name = toks[i + 1][1]
+            if toks[i + 2] != ("o", "{"):
+                raise ValueError(f"{path}: expected '{{' after function {name}")
+            i += 3
+            comps: Dict[str, object] = {}
+            while toks[i] != ("o", "}"):
+                var = toks[i][1]
+                if toks[i + 1] != ("o", ":="):
+                    raise ValueError(f"{path}: expected ':=' after {var}")
+                j = i + 2
+                while toks[j] != ("o", ";"):
+                    j += 1
+                p = _ExprParser(toks[i + 2:j])
+                comps[var] = p.expr()
+                if p.i != j - (i + 2):
+                    raise ValueError(f"{path}: trailing tokens in {var}")
+                i = j + 1
+            out[name] = comps
+            i += 1
+        else:
+            i += 1
+    return out
+
+
+def field_function(comps: Dict[str, object]):
+    """(t, x, y) -> array [ncomp, n] of a read_functions block."""
+    def fn(t, x, y):
+        env = {"t": t, "x": x, "y": y, "z": np.zeros_like(x)}
+        return np.stack([np.broadcast_to(np.asarray(f(env), dtype=np.float64), np.shape(x))
+                         for f in comps.values()])
+    return fn
+
+
 @dataclass
 class InitialCondition:
     code: str
@@ -182,6 +334,8 @@ def read_ic(path: str) -> InitialCondition:
                 if m:
                     vals.append(float(m.group(1)))
             return InitialCondition(code, vals)
+        if lc == "mmsinitialcondition":                  # no parameters (Shallow::set_ic)
+            return InitialCondition(code, [])
     raise ValueError(f"{path}: unsupported IC code {code}")
 
 

@@ -12,6 +12,8 @@ the coordinate arrays.  Reference counterparts:
 * ``isentropic_vortex`` the 2D vortex of the C1 workload: freestream plus a
   Gaussian isentropic vortex of given strength (Shu's form)
 * ``fourier_modes`` smooth seeded perturbation about a uniform state
+* ``shallow_mms``    ``MMSInitialCondition`` IC code of the shallow-water
+  manufactured solution (src/ShallowIC.hpp:235-252, src/Shallow.cpp:697-705)
 * ``from_code``      dispatch on a parsed ``root.ic`` block (Euler::set_ic,
   src/Euler.cpp:880-1190)
 """
@@ -115,6 +117,18 @@ def fourier_modes(coords, lengths, nmodes, amp, rng):
     return amp * out / max(nmodes, 1)
 
 
+def shallow_mms(x, y):
+    """(phi, uphi, vphi) of the shallow-water manufactured solution at t = 0,
+    with the reference's literal 3.141592654 for pi (its boundary and source
+    functions use the parser's PI instead)."""
+    pi = 3.141592654
+    a = np.sin(pi / 2.0) * np.cos(x) * np.sin(y)
+    phi = (49.0 * a) / 5.0 + 539.0 / 5.0
+    uphi = (49.0 * x * y * (a + 11.0)) / (20.0 * pi * pi)
+    vphi = -1.0 * (49.0 * x * y * (a + 11.0)) / (20.0 * pi * pi)
+    return np.stack([phi, uphi, vphi])
+
+
 def from_code(icd, x, gamma=1.4):
     """Evaluate a parsed ``root.ic`` block (dgm_amd.formats.read_ic) at nodal coordinates ``x``."""
     code = icd.code.lower()
@@ -126,4 +140,6 @@ def from_code(icd, x, gamma=1.4):
         return acoustic_pulse(x[0], *icd.values, gamma=gamma)
     if code == "given":
         return given_constant(np.shape(x[0]), icd.values)
+    if code == "mmsinitialcondition":
+        return shallow_mms(x[0], x[1])
     raise ValueError(f"unsupported initial condition code {icd.code}")

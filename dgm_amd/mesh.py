@@ -138,6 +138,28 @@ def mapped_nodes(mesh: Mesh, p: int, fn) -> np.ndarray:
     return np.stack(fn(*x))
 
 
+def affine_quads(mesh: Mesh) -> np.ndarray:
+    """Per quad, whether it is a parallelogram within the reference's
+    tolerances (Quad::is_affine, src/Quad.cpp:1184-1205: each half edge vector
+    equal to its opposite within REL = sqrt(eps) or ABS = eps^(2/3)).  The
+    reference gives every other quad a straight-sided curve, so it takes the
+    curved path with nodal metrics (Field::read_curve_pc, src/Field.cpp:1226-1240)."""
+    eps = np.finfo(float).eps
+    rel, tol = np.sqrt(eps), eps ** 0.6666666666666666667
+
+    def close(u, v):
+        d = np.abs(u - v)
+        return (u == v) | (d <= tol) | (d <= rel * np.maximum(np.abs(u), np.abs(v)))
+
+    v = np.asarray(mesh.vert, dtype=np.float64)           # [ne][2][4]
+    out = np.ones(v.shape[0], dtype=bool)
+    for c in range(2):
+        a = v[:, c]
+        out &= close((a[:, 1] - a[:, 0]) * 0.5, (a[:, 2] - a[:, 3]) * 0.5)
+        out &= close((a[:, 3] - a[:, 0]) * 0.5, (a[:, 2] - a[:, 1]) * 0.5)
+    return out
+
+
 def _arc_points(p0, p1, radius, orientation, z):
     """Points of a circular arc through p0 and p1 at the local coordinates z,
     as Arc::generate (src/Curve.cpp:523-590) places them."""

@@ -1,7 +1,7 @@
 """Pin the CPU oracle against the reference's golden restart files.
 
 Each case is run exactly as the reference's regression test does
-(runs/vquad.tst, runs/vhex.tst, runs/cvquad.tst) and compared with ``dgm_diff`` semantics
+(runs/vquad.tst, runs/vhex.tst, runs/cvquad.tst, runs/shallow_mms/shallow_mms.tst) and compared with ``dgm_diff`` semantics
 (src/Domain.cpp:1000-1071): every field must pass the absolute tolerance and,
 where the reference norm exceeds repsM = sqrt(eps), the relative tolerance.
 """
@@ -43,6 +43,38 @@ def test_cvquad_2d_ns_curved_arcs():
     _check("cvquad", 5e-13, 5e-13)
 
 
+def test_shallow_mms_function_boundaries_and_source():
+    # runs/shallow_mms.tst:23-24 aeps = 1e-9, reps = 1e-10: shallow water on
+    # 400 non-parallelogram quads (the curved path), p = 1, RK4, 3 steps, with
+    # function boundaries (shallow_mms.bct) and a function source
+    # (shallow_mms.src) evaluated at each stage's time, and the balance term as
+    # the reference compiles it; measured 4e-13 absolute
+    dn, rel = _check("shallow_mms", 1e-9, 1e-10)
+    assert dn.max() < 1e-11
+
+
+def test_function_file_parser():
+    fns = formats.read_functions(f"{GOLDEN}/shallow_mms.bct")
+    f = formats.field_function(fns["dansMMSFunction"])
+    x, y, t = np.array([0.3, 1.2]), np.array([-0.5, 2.0]), 0.1
+    a = np.sin(np.pi / 2 + t) * np.cos(x) * np.sin(y)
+    want = np.stack([49.0 * a / 5.0 + 539.0 / 5.0, 49.0 * x * y * (a + 11.0) / (20.0 * np.pi ** 2),
+                     -49.0 * x * y * (a + 11.0) / (20.0 * np.pi ** 2)])
+    assert np.allclose(f(t, x, y), want, rtol=1e-15, atol=0)
+    ev = lambda text: formats.field_function({"v": formats._ExprParser(formats._tokens(text)).expr()})(0.0, x, y)[0]
+    assert np.all(ev("-2^2") == -4.0) and np.all(ev("2^3^2") == 512.0) and np.all(ev("2*3-4/2+1") == 5.0)
+    assert np.allclose(ev("atan2(y, x) + sqrt(abs(-x))"), np.arctan2(y, x) + np.sqrt(x))
+    with pytest.raises(ValueError):
+        ev("__import__('os')")
+
+
+def test_affine_detection():
+    from dgm_amd import mesh as meshgen
+    prm, m, _ = __import__("fixture_runs").load_case("shallow_mms")
+    assert not np.any(meshgen.affine_quads(m))
+    assert np.all(meshgen.affine_quads(meshgen.periodic_quads(3, 2, 0.0, 1.0, 0.0, 2.0)))
+
+
 def test_crv_reader_and_arc_nodes():
     from dgm_amd import mesh as meshgen
     from fixture_runs import load_case
@@ -70,7 +102,7 @@ def test_crv_reader_and_arc_nodes():
 
 
 def test_restart_reader_consumes_whole_files():
-    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25), ("cvquad", 4)):
+    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25), ("cvquad", 4), ("shallow_mms", 400)):
         r = formats.read_restart(f"{GOLDEN}/{root}.ref", ne)
         assert r.data.shape[1] == ne
         assert np.all(np.isfinite(r.data))
