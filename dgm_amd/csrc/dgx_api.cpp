@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -251,6 +252,7 @@ dgx::CurvedArgs curved_args(dgx_domain *d) {
   ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
   ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
   ca.ns = d->visc; ca.G = d->G; ca.V = d->V; ca.TV = d->TV;
+  ca.fn = d->fn; ca.t = d->time;
   return ca;
 }
 
@@ -320,8 +322,10 @@ int residual(dgx_domain *d, const double *U, double *R) {
       if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved viscous terms: ") + hipGetErrorString(e));
     }
     ca.op = 0;
+    ca.Bal = d->Bal; ca.bal = d->bal;
     hipError_t e = timed(d, 1, [&] { return launch_curved(d, ca); });
     if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved residual: ") + hipGetErrorString(e));
+    if (d->bal == 1) d->bal = 2;   // the first residual is stored
     ++d->stages;
     return DGX_OK;
   }
@@ -349,6 +353,110 @@ int ensure_nodal(dgx_domain *d) {
 // normals and J_side/J_vol at the Gauss points from them as
 // Quad::compute_side_geometry (src/Quad.cpp:1367-1452), and per element the
 // mean J that starts the kernel's mass-matrix solve.
+// Quad::is_affine (src/Quad.cpp:1184-1205): each half edge vector equals its
+// opposite within REL = sqrt(eps) or ABS = eps^(2/3) (CMC::approx_equal,
+// src/Vector.hpp:1253-1264).  Other quads are curved in the reference
+// (Field::read_curve_pc, src/Field.cpp:1226-1240).
+bool quad_affine(const double *vx, const double *vy) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double REL = std::sqrt(eps), ABS = std::pow(eps, 0.6666666666666666667);
+  auto close = [&](double u, double v) {
+    if (u == v) return true;
+    const double diff = std::fabs(u - v);
+    if (diff <= ABS) return true;
+    return diff <= REL * std::max(std::fabs(u), std::fabs(v));
+  };
+  return close((vx[1] - vx[0]) * 0.5, (vx[2] - vx[3]) * 0.5) && close((vx[3] - vx[0]) * 0.5, (vx[2] - vx[1]) * 0.5) &&
+         close((vy[1] - vy[0]) * 0.5, (vy[2] - vy[3]) * 0.5) && close((vy[3] - vy[0]) * 0.5, (vy[2] - vy[1]) * 0.5);
+}
+
+bool all_quads_affine(const dgx_domain_desc *desc) {
+  if (desc->nsd != 2 || !desc->vert) return true;
+  for (int e = 0; e < desc->ne; ++e) {
+    const double *vx = desc->vert + (size_t)e * 8;
+    if (!quad_affine(vx, vx + 4)) return false;
+  }
+  return true;
+}
+
+// A dgx_function with ncomp components whose programs all run within the
+// stack and leave one value (the device evaluator relies on it).
+bool check_function(const dgx_function *f, int ncomp, std::string *why) {
+  if (f->ncomp != ncomp) { *why = "must have " + std::to_string(ncomp) + " components"; return false; }
+  if (!f->start || !f->code || (f->nconst > 0 && !f->consts) || f->nconst < 0) { *why = "null arrays"; return false; }
+  if (f->start[0] != 0) { *why = "start[0] must be 0"; return false; }
+  for (int c = 0; c < ncomp; ++c) {
+    if (f->start[c + 1] <= f->start[c] || f->start[c + 1] > (1 << 24)) {
+      *why = "component " + std::to_string(c) + " is empty or too long"; return false;
+    }
+    int sp = 0;
+    for (int i = f->start[c]; i < f->start[c + 1]; ++i) {
+      const int ins = f->code[i], op = ins & 255;
+      if (ins < 0 || op > DGX_OP_ATAN2 || (op != DGX_OP_CONST && (ins >> 8) != 0)) {
+        *why = "bad instruction " + std::to_string(ins); return false;
+      }
+      if (op == DGX_OP_CONST && (ins >> 8) >= f->nconst) { *why = "constant index out of range"; return false; }
+      const int need = (op <= DGX_OP_T) ? 0 : ((op <= DGX_OP_POW || op == DGX_OP_ATAN2) ? 2 : 1);
+      if (sp < need) { *why = "stack underflow in component " + std::to_string(c); return false; }
+      sp += (op <= DGX_OP_T) ? 1 : ((need == 2) ? -1 : 0);
+      if (sp > DGX_FN_STACK) { *why = "stack deeper than DGX_FN_STACK"; return false; }
+    }
+    if (sp != 1) { *why = "component " + std::to_string(c) + " does not leave one value"; return false; }
+  }
+  return true;
+}
+
+// Device copies of the boundary and source programs (one code and constant
+// table, the source's indices shifted behind the boundary's) and the
+// coordinates they are evaluated at: every side's Gauss points (the GLL edge
+// coordinates interpolated, as Element::get_side_coordinates does) and the nodes.
+int setup_functions(dgx_domain *d, const dgx_domain_desc *desc) {
+  const dgx_function *fs[2] = {desc->bc_function, desc->source};
+  std::vector<int> code;
+  std::vector<double> consts;
+  for (int k = 0; k < 2; ++k) {
+    if (!fs[k]) continue;
+    const dgx_function *f = fs[k];
+    int *bounds = k == 0 ? d->fn.bc : d->fn.src;
+    const int base = (int)code.size(), cbase = (int)consts.size();
+    for (int c = 0; c <= 3; ++c) bounds[c] = base + f->start[c];
+    for (int i = 0; i < f->start[3]; ++i) {
+      const int ins = f->code[i];
+      code.push_back((ins & 255) == DGX_OP_CONST ? (((ins >> 8) + cbase) << 8) : ins);
+    }
+    consts.insert(consts.end(), f->consts, f->consts + f->nconst);
+    (k == 0 ? d->fn.has_bc : d->fn.has_src) = 1;
+  }
+  if (code.empty()) return DGX_OK;
+  if (consts.empty()) consts.push_back(0.0);
+  const int ne = d->ne, Q = d->Q, QF = d->QF, Q2 = d->nq;
+  std::vector<double> fxy((size_t)ne * 4 * QF * 2);
+  const std::vector<double> &Ig = d->hb.Ig;
+  for (int e = 0; e < ne; ++e)
+    for (int s = 0; s < 4; ++s)
+      for (int a = 0; a < QF; ++a) {
+        double x = 0.0, y = 0.0;
+        for (int t = 0; t < Q; ++t) {
+          const int n = s == 0 ? t * Q : (s == 1 ? (Q - 1) * Q + t : (s == 2 ? t * Q + Q - 1 : t));
+          x += Ig[a * Q + t] * d->coords[(size_t)e * Q2 + n];
+          y += Ig[a * Q + t] * d->coords[((size_t)ne + e) * Q2 + n];
+        }
+        double *o = fxy.data() + (((size_t)e * 4 + s) * QF + a) * 2;
+        o[0] = x; o[1] = y;
+      }
+  int rc;
+  HIPCHK(hipMalloc((void **)&d->d_fcode, code.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(d->d_fcode, code.data(), code.size() * sizeof(int), hipMemcpyHostToDevice));
+  if ((rc = alloc(&d->d_fconst, consts.size())) || (rc = alloc(&d->d_fxy, fxy.size())) ||
+      (rc = alloc(&d->d_xn, d->coords.size())))
+    return rc;
+  HIPCHK(hipMemcpy(d->d_fconst, consts.data(), consts.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_fxy, fxy.data(), fxy.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_xn, d->coords.data(), d->coords.size() * 8, hipMemcpyHostToDevice));
+  d->fn.code = d->d_fcode; d->fn.consts = d->d_fconst; d->fn.fxy = d->d_fxy; d->fn.xn = d->d_xn;
+  return DGX_OK;
+}
+
 int curved_geometry2(dgx_domain *d, const double *xn) {
   const int ne = d->ne, Q = d->Q, QF = d->QF, Q2 = d->nq;
   const HostBasis &b = d->hb;
@@ -418,6 +526,7 @@ int curved_geometry2(dgx_domain *d, const double *xn) {
   if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
     return rc;
   HIPCHK(hipMemcpy(d->d_gm, gm.data(), gm.size() * 8, hipMemcpyHostToDevice));
+  d->gm_host = gm;   // the norms integrate with the nodal metrics
   HIPCHK(hipMemcpy(d->d_fgeo, fgeo.data(), fgeo.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&d->d_iters, 64));
@@ -552,6 +661,7 @@ int curved_geometry(dgx_domain *d, const double *xn) {
   if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
     return rc;
   HIPCHK(hipMemcpy(d->d_gm, gm.data(), gm.size() * 8, hipMemcpyHostToDevice));
+  d->gm_host = gm;   // the norms integrate with the nodal metrics
   HIPCHK(hipMemcpy(d->d_fgeo, fgeo.data(), fgeo.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&d->d_iters, 64));
@@ -586,6 +696,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   d->J.assign(ne, 0.0);
   std::vector<double> fnorm((size_t)ne * d->ns * 3, 0.0), fjr((size_t)ne * d->ns, 0.0);
   auto vmode = [](int which, double x) { return which ? 0.5 * (1.0 + x) : 0.5 * (1.0 - x); };
+  bool nonaffine = false;
   for (int e = 0; e < ne; ++e) {
     const double *vx = desc->vert + (size_t)e * nsd * nvert, *vy = vx + nvert;
     const double *vz = nsd == 3 ? vy + nvert : nullptr;
@@ -603,10 +714,10 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
           }
       const double xr = (vx[1] - vx[0]) * 0.5, xs = (vx[3] - vx[0]) * 0.5;
       const double yr = (vy[1] - vy[0]) * 0.5, ys = (vy[3] - vy[0]) * 0.5;
-      const double tol = 1e-10 * (std::fabs(xr) + std::fabs(xs) + std::fabs(yr) + std::fabs(ys));
-      if (std::fabs(xr - (vx[2] - vx[3]) * 0.5) > tol || std::fabs(xs - (vx[2] - vx[1]) * 0.5) > tol ||
-          std::fabs(yr - (vy[2] - vy[3]) * 0.5) > tol || std::fabs(ys - (vy[2] - vy[1]) * 0.5) > tol)
-        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " is not affine");
+      if (!quad_affine(vx, vy)) {   // the curved path replaces this element's geometry
+        nonaffine = true;
+        continue;
+      }
       const double det = xr * ys - xs * yr;
       if (!(det > 0)) return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(e) + " has J <= 0");
       d->J[e] = det;
@@ -674,7 +785,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   for (int r = 0; r < desc->ncon; ++r) {
     const int e = desc->con[4 * r], s = desc->con[4 * r + 1], le = desc->con[4 * r + 2], ls = desc->con[4 * r + 3];
     if (e >= 0 && e < ne && s >= 0 && s < d->ns && le == -1) {   // boundary side (e, s, -1, bc type)
-      if (ls != DGX_BC_WALL && ls != DGX_BC_STATE)
+      if (ls != DGX_BC_WALL && ls != DGX_BC_STATE && ls != DGX_BC_FUNCTION)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
       if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
         return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
@@ -823,6 +934,13 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   }
   if (desc->xnodes && (rc = (nsd == 2 ? curved_geometry2(d, desc->xnodes) : curved_geometry(d, desc->xnodes))))
     return rc;
+  if (!desc->xnodes && (nonaffine || desc->bc_function || desc->source)) {
+    // the curved path on the vertices' bilinear map (the element coordinates),
+    // as the reference gives non-parallelogram quads a straight-sided curve
+    const std::vector<double> xn = d->coords;
+    if ((rc = curved_geometry2(d, xn.data()))) return rc;
+  }
+  if ((desc->bc_function || desc->source) && (rc = setup_functions(d, desc))) return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
   d->mixed = d->prec == DGX_MIXED;
@@ -846,7 +964,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master};
+                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
+                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -880,8 +999,15 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   b += nf * ne * nq * 8;                                    // nodal transfer buffer
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
   b += 8 * qs * 2 + (size_t)(1 << 20);                      // orientation tables, tables, rounding
-  if (desc->xnodes)   // curved: nodal metrics, face geometry
+  const bool fns = desc->bc_function || desc->source;
+  if (desc->xnodes || fns || !all_quads_affine(desc))   // curved: nodal metrics, face geometry
     b += nsd == 2 ? ne * (5 * nq + 16 * qs + 1) * 8 : ne * (10 * nq + 24 * qs + 1) * 8;
+  if (fns) {                 // programs, side Gauss-point and nodal coordinates
+    for (const dgx_function *f : {desc->bc_function, desc->source})
+      if (f && f->start && f->ncomp > 0 && f->ncomp <= 16 && f->start[f->ncomp] > 0 && f->nconst >= 0)
+        b += (size_t)f->start[f->ncomp] * 4 + (size_t)f->nconst * 8;
+    b += ne * (8 * qs + 2 * nq) * 8;
+  }
   if (shallow && !desc->no_balance) b += nf * ne * nm * 8;    // balance term
   *bytes = b;
   return DGX_OK;
@@ -912,6 +1038,8 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
   if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
+  if (!desc->xnodes && !all_quads_affine(desc) && desc->precision != DGX_FP64)
+    return fail(DGX_ERR_ARGUMENT, "non-parallelogram quads take the curved path, which is DGX_FP64");
   if (desc->xnodes && !(desc->precision == DGX_FP64 &&
                         ((desc->nsd == 3 && desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p)) ||
                          (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
@@ -923,9 +1051,21 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
       if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
         return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
       const int t = desc->con[4 * r + 3];
-      if (t != DGX_BC_WALL && t != DGX_BC_STATE)
+      if (t != DGX_BC_WALL && t != DGX_BC_STATE && t != DGX_BC_FUNCTION)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
+      if (t == DGX_BC_FUNCTION && !desc->bc_function)
+        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) +
+                                          ": DGX_BC_FUNCTION boundary without bc_function");
     }
+  if ((desc->bc_function || desc->source) && desc->eqntype != DGX_SHALLOW)
+    return fail(DGX_ERR_ARGUMENT, "boundary and source functions are available for Shallow");
+  {
+    std::string why;
+    if (desc->bc_function && !check_function(desc->bc_function, 3, &why))
+      return fail(DGX_ERR_ARGUMENT, "bc_function: " + why);
+    if (desc->source && !check_function(desc->source, 3, &why))
+      return fail(DGX_ERR_ARGUMENT, "source: " + why);
+  }
   if (desc->eqntype == DGX_NAVIER_STOKES3D && !dgx::stage3_supported(desc->p) &&
       dgx::lds_bytes3(desc->p, 3) > 160 * 1024)
     return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d supports p <= 9 (LDS)");
@@ -946,6 +1086,19 @@ int dgx_domain_destroy(dgx_handle h) {
   (void)hipStreamSynchronize(h->stream);
   release(h);
   delete h;
+  return DGX_OK;
+}
+
+int dgx_set_time(dgx_handle h, double t) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  if (!std::isfinite(t)) return fail(DGX_ERR_ARGUMENT, "time must be finite");
+  h->time = t;
+  return DGX_OK;
+}
+
+int dgx_get_time(dgx_handle h, double *t) {
+  if (!h || !t) return fail(DGX_ERR_ARGUMENT, "null argument");
+  *t = h->time;
   return DGX_OK;
 }
 
@@ -1028,28 +1181,35 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpy(-a[0] * dt, h->R0, h->U, h->Ut, n, s));
       BL(dgx::blas_scale(-b[0], h->R0, n, s));
+      h->time += 0.5 * dt;
       RUN(residual(h, h->Ut, h->R1));
       BL(dgx::blas_axpy(-a[1] * dt, h->R1, h->U, h->Ut, n, s));
       BL(dgx::blas_axpy(-b[1], h->R1, h->R0, h->R0, n, s));
       RUN(residual(h, h->Ut, h->R1));
       BL(dgx::blas_axpy(-a[2] * dt, h->R1, h->U, h->Ut, n, s));
       BL(dgx::blas_axpy(-b[2], h->R1, h->R0, h->R0, n, s));
+      h->time += 0.5 * dt;
       RUN(residual(h, h->Ut, h->R1));
       BL(dgx::blas_axpy(-b[3], h->R1, h->R0, h->R0, n, s));
       BL(dgx::blas_axpy(dt, h->R0, h->U, h->U, n, s));
     } else if (inttype == DGX_TVD) {
-      // TVD::advance (src/TimeInt.cpp:824-856)
+      // TVD::advance (src/TimeInt.cpp:824-856), stage times c = 0, 1, 1/2 (src/TimeInt.cpp:740)
+      const double tn = h->time;
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->U, h->Ut, n, s));
+      h->time = tn + dt;
       RUN(residual(h, h->Ut, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
       BL(dgx::blas_axpby(3.0 / 4.0, h->U, 1.0 / 4.0, h->R0, h->Ut, n, s));
+      h->time = tn + 0.5 * dt;
       RUN(residual(h, h->Ut, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
       BL(dgx::blas_axpby(2.0 / 3.0, h->R0, 1.0 / 3.0, h->U, h->U, n, s));
+      h->time = tn + dt;
     } else if (inttype == DGX_LSRK14) {
       // LSRK14::advance (src/TimeInt.cpp:1989-2013)
       const double *A = dgx::kLsrkA, *B = dgx::kLsrkB;
+      const double tn = h->time;
       if (h->fused) {
         // one fused kernel per stage: residual, register update, next traces
         if (step == 0) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));
@@ -1063,13 +1223,16 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
           std::swap(h->T, h->T2);
           ++h->stages;
         }
+        h->time = tn + dt;
         continue;
       }
       for (int st = 0; st < 14; ++st) {
+        h->time = tn + dgx::kLsrkC[st] * dt;
         RUN(residual(h, h->U, h->R0));
         BL(dgx::blas_axpby(-dt, h->R0, A[st], h->Ut, h->Ut, n, s));
         BL(dgx::blas_axpy(B[st], h->Ut, h->U, h->U, n, s));
       }
+      h->time = tn + dt;
     } else {
       return fail(DGX_ERR_ARGUMENT, "unsupported inttype " + std::to_string(inttype));
     }
@@ -1146,13 +1309,24 @@ int dgx_norms(dgx_handle h, double *out) {
                 ft[(i * Q + j) * Q + k] += D[k * Q + m] * u[(i * Q + j) * Q + m];
               }
       }
+      const double *gm = h->curved ? &h->gm_host[(size_t)e * (nsd == 2 ? 5 : 10) * nq] : nullptr;
       for (int n = 0; n < nq; ++n) {
-        double wj = h->J[e];
-        if (nsd == 2) wj *= w[n / Q] * w[n % Q];
-        else wj *= w[n / (Q * Q)] * w[(n / Q) % Q] * w[n % Q];
-        const double gx = mt[0] * fr[n] + mt[3] * fs[n] + mt[6] * ft[n];
-        const double gy = mt[1] * fr[n] + mt[4] * fs[n] + mt[7] * ft[n];
-        const double gz = mt[2] * fr[n] + mt[5] * fs[n] + mt[8] * ft[n];
+        double wj, gx, gy, gz = 0.0;
+        if (gm) {   // nodal metrics: gm [d * nsd + x] = wJ dxi_d/dx_x, then wJ
+          wj = gm[(size_t)nsd * nsd * nq + n];
+          const double f[3] = {fr[n], fs[n], ft[n]};
+          double g[3] = {0.0, 0.0, 0.0};
+          for (int dd = 0; dd < nsd; ++dd)
+            for (int x = 0; x < nsd; ++x) g[x] += gm[(size_t)(dd * nsd + x) * nq + n] * f[dd];
+          gx = g[0] / wj; gy = g[1] / wj; gz = g[2] / wj;
+        } else {
+          wj = h->J[e];
+          if (nsd == 2) wj *= w[n / Q] * w[n % Q];
+          else wj *= w[n / (Q * Q)] * w[(n / Q) % Q] * w[n % Q];
+          gx = mt[0] * fr[n] + mt[3] * fs[n] + mt[6] * ft[n];
+          gy = mt[1] * fr[n] + mt[4] * fs[n] + mt[7] * ft[n];
+          gz = mt[2] * fr[n] + mt[5] * fs[n] + mt[8] * ft[n];
+        }
         Linf = std::max(Linf, std::fabs(u[n]));
         L2 += wj * u[n] * u[n];
         H1 += wj * (u[n] * u[n] + gx * gx + gy * gy + gz * gz);

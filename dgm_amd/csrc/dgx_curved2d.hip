@@ -67,7 +67,8 @@ __device__ __forceinline__ void field_dots2(const double (&part)[NF], double *re
 // AP [NF L2]; Fw is overwritten.
 template <int P, int NF, int NT>
 __device__ void curved_project2(double *Fw, double *S1, double *S2, double *AP, double *red, const double *ge,
-                                double jav, double *out, size_t ostride, int *iters, int tid) {
+                                double jav, double *out, size_t ostride, int *iters, int tid,
+                                double *Bal = nullptr, int bal = 0) {
   constexpr int L = P + 1, Q = P + 2, L2 = L * L, Q2 = Q * Q;
   constexpr int NW = NT / 64, NML = (NF * L2 + NT - 1) / NT;
   // b = B^T (wJ f): into AP (compact [NF][L2])
@@ -180,7 +181,11 @@ __device__ void curved_project2(double *Fw, double *S1, double *S2, double *AP, 
     const int g = tid + k * NT;
     if (g < NF * L2) {
       const int v = g / L2;
-      out[v * ostride + (g - v * L2)] = x[k];
+      const size_t o = v * ostride + (g - v * L2);
+      double r = x[k];
+      if (bal == 1) { Bal[o] = r; r = -1.0 * r + r; }   // the balance term (src/Shallow.cpp:251-266)
+      else if (bal == 2) r = -1.0 * Bal[o] + r;
+      out[o] = r;
     }
   }
   if (tid == 0 && iters) atomicMax(iters, it);
@@ -315,6 +320,10 @@ k_rhs2c(CurvedArgs a) {
       const double *tr = a.T + ((size_t)nb * 4 + geo.nside[es]) * NF * QF;
 #pragma unroll
       for (int v = 0; v < NF; ++v) UR[v] = tr[v * QF + qn];
+    } else if (SHALLOW && -1 - nb == 2) {   // function boundary (FuncBC) at the Gauss point and time t
+      const double *xy = a.fn.fxy + ((size_t)es * QF + q) * 2;
+#pragma unroll
+      for (int v = 0; v < 3; ++v) UR[v] = eval_program(a.fn.code, a.fn.bc[v], a.fn.bc[v + 1], a.fn.consts, xy[0], xy[1], a.t);
     } else if (SHALLOW) {
       ghost_shallow2(-1 - nb, UL, nx, ny, a.phys.bcU, UR);
     } else {   // no Euler boundaries on quads (rejected at creation)
@@ -380,6 +389,12 @@ k_rhs2c(CurvedArgs a) {
     }
     __syncthreads();
     divergence2c<P, NT>(Fc, ge, Fw + v * Q2, tid);
+    if (SHALLOW && a.fn.has_src) {   // FuncSource::apply: F -= s at the nodes (src/Source.cpp:11-24)
+      const double *xx = a.fn.xn + (size_t)e * Q2, *yy = a.fn.xn + ((size_t)ne + e) * Q2;
+      for (int n = tid; n < Q2; n += NT)
+        Fw[v * Q2 + n] -= ge[4 * Q2 + n] * eval_program(a.fn.code, a.fn.src[v], a.fn.src[v + 1], a.fn.consts,
+                                                         xx[n], yy[n], a.t);
+    }
     if (NS && v > 0) {
       const double *vn = a.V + ((size_t)e * 3 + (v - 1)) * Q2;
       for (int n = tid; n < Q2; n += NT) Fw[v * Q2 + n] -= vn[n];
@@ -388,7 +403,9 @@ k_rhs2c(CurvedArgs a) {
   __syncthreads();
   lift_edges2<P, NF, NT>(gF, S1, Fw, ge, tid);
   }   // !PROJ
-  curved_project2<P, NF, NT>(Fw, S1, S2, Un, red, ge, a.jav[e], a.R + (size_t)e * L2, (size_t)ne * L2, a.iters, tid);
+  curved_project2<P, NF, NT>(Fw, S1, S2, Un, red, ge, a.jav[e], a.R + (size_t)e * L2, (size_t)ne * L2, a.iters, tid,
+                             (SHALLOW && !PROJ && a.bal) ? a.Bal + (size_t)e * L2 : nullptr,
+                             (SHALLOW && !PROJ) ? a.bal : 0);
 }
 
 // Lifted, projected gradients of the primitives (u, v, e) of a curved quad

@@ -108,6 +108,57 @@ __device__ __forceinline__ void ghost_shallow2(int bc, const double UL[3], doubl
   UB[2] = (UL[2] * nx - UL[1] * ny) * nx;
 }
 
+// One component of a boundary or source function at (x, y, t): the postfix
+// program code[b] .. code[e] (dgx_function).  The host checks every program at
+// domain creation (opcodes, constant indices, a stack depth within 16 that
+// ends at one value), so the stack index stays in bounds.
+__device__ __noinline__ double eval_program(const int *__restrict__ code, int b, int e,
+                                            const double *__restrict__ consts, double x, double y, double t) {
+#pragma clang fp contract(off)
+  double st[16];
+  int sp = 0;
+  for (int i = b; i < e; ++i) {
+    const int ins = code[i], op = ins & 255;
+    if (op <= 4) {
+      st[sp++] = op == 0 ? consts[ins >> 8] : (op == 1 ? x : (op == 2 ? y : (op == 3 ? 0.0 : t)));
+    } else if (op <= 9 || op == 25) {
+      const double bb = st[--sp], a = st[sp - 1];
+      double r;
+      switch (op) {
+        case 5: r = a + bb; break;
+        case 6: r = a - bb; break;
+        case 7: r = a * bb; break;
+        case 8: r = a / bb; break;
+        case 9: r = pow(a, bb); break;
+        default: r = atan2(a, bb); break;
+      }
+      st[sp - 1] = r;
+    } else {
+      const double a = st[sp - 1];
+      double r;
+      switch (op) {
+        case 10: r = -a; break;
+        case 11: r = sin(a); break;
+        case 12: r = cos(a); break;
+        case 13: r = tan(a); break;
+        case 14: r = asin(a); break;
+        case 15: r = acos(a); break;
+        case 16: r = atan(a); break;
+        case 17: r = sinh(a); break;
+        case 18: r = cosh(a); break;
+        case 19: r = tanh(a); break;
+        case 20: r = exp(a); break;
+        case 21: r = log(a); break;
+        case 22: r = log10(a); break;
+        case 23: r = sqrt(a); break;
+        default: r = fabs(a); break;
+      }
+      st[sp - 1] = r;
+    }
+  }
+  return st[0];
+}
+
 // Ghost values of a boundary side, as Domain::enforce_bc fills them
 // (src/Domain.cpp:1346-1350): bc 0 slip wall (Euler3d::WallBC::apply,
 // src/Euler3d.cpp:876-909: the normal momentum removed, the internal energy

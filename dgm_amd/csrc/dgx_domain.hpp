@@ -41,6 +41,7 @@ struct dgx_domain {
   bool own_stream = false;
   dgx::HostBasis hb;
   std::vector<double> coords, metric, J;
+  std::vector<double> gm_host;   // curved: host copy of the nodal metrics d_gm
   // device
   double *d_tab = nullptr, *d_metric = nullptr, *d_fnorm = nullptr, *d_fjr = nullptr;
   int *d_nbr = nullptr, *d_nside = nullptr, *d_flag = nullptr;
@@ -53,6 +54,10 @@ struct dgx_domain {
   bool curved = false;  // curved hexes (dgx_curved3d.hip): nodal metrics, mass-matrix projection
   double *d_gm = nullptr, *d_fgeo = nullptr, *d_jav = nullptr;
   int *d_iters = nullptr;
+  double time = 0.0;       // solution time: boundary and source functions are evaluated at it
+  dgx::FnArgs fn;          // boundary and source functions (device programs and coordinates)
+  int *d_fcode = nullptr;
+  double *d_fconst = nullptr, *d_fxy = nullptr, *d_xn = nullptr;
   double *Bal = nullptr;   // shallow water: the first residual (balance term)
   int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
   dgx::Tab tab;

@@ -95,6 +95,17 @@ hipError_t launch_mixed3(const StageLaunch &s);
 
 // Curved hexes, Euler3d, p <= 6 (dgx_curved3d.hip): residual R = -dU/dt with
 // metrics at the nodes and the element mass matrix.
+// Boundary and source functions of (x, y, t) as postfix programs (dgx_function):
+// bc[c] .. bc[c+1] and src[c] .. src[c+1] bound component c's instructions in code.
+struct FnArgs {
+  const int *code = nullptr;
+  const double *consts = nullptr;
+  int bc[4] = {0, 0, 0, 0}, src[4] = {0, 0, 0, 0};
+  int has_bc = 0, has_src = 0;
+  const double *fxy = nullptr;   // [ne][4][QF][2] Gauss-point coordinates of every side
+  const double *xn = nullptr;    // [2][ne][Q^2] nodal coordinates
+};
+
 struct CurvedArgs {
   const double *U = nullptr, *T = nullptr;   // modal state and its face traces
   double *R = nullptr;                       // modal residual
@@ -114,6 +125,12 @@ struct CurvedArgs {
   bool ns = false;
   int op = 0;
   double *G = nullptr, *V = nullptr, *TV = nullptr;
+  // shallow water: functions at time t; the balance term (Bal: bal 1 stores
+  // the residual and returns R - R = 0, bal 2 returns R - Bal)
+  FnArgs fn;
+  double t = 0.0;
+  double *Bal = nullptr;
+  int bal = 0;
   hipStream_t stream = nullptr;
 };
 // Quads use the same arguments with gm [ne][5][Q^2] (wJ dxi_d/dx_x, wJ),

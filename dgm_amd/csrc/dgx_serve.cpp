@@ -299,6 +299,8 @@ int dgx_server_submit(dgx_server_handle s, const dgx_domain_desc *desc, const do
   if (nsteps < 1 || !(dt > 0.0)) return sfail(DGX_ERR_ARGUMENT, "nsteps and dt must be positive");
   if (!desc->vert || !desc->con || desc->ne <= 0 || desc->ncon <= 0)
     return sfail(DGX_ERR_ARGUMENT, "empty mesh");
+  if (desc->bc_function || desc->source)
+    return sfail(DGX_ERR_ARGUMENT, "boundary and source functions are not served; create the domain directly");
   auto r = std::make_unique<Request>();
   int rc = dgx_domain_bytes(desc, &r->bytes);
   if (rc) return sfail(rc, dgx_last_error());
@@ -310,8 +312,9 @@ int dgx_server_submit(dgx_server_handle s, const dgx_domain_desc *desc, const do
   r->desc.vert = r->vert.data();
   r->desc.con = r->con.data();
   const size_t L = desc->p + 1, Q = desc->p + 2, nf = desc->eqntype == DGX_SHALLOW ? 3 : desc->nsd + 2;
-  if (desc->xnodes) {   // curved hexes: the nodes are kept with the request like the mesh
-    r->xnodes.assign(desc->xnodes, desc->xnodes + 3 * (size_t)desc->ne * Q * Q * Q);
+  if (desc->xnodes) {   // curved elements: the nodes are kept with the request like the mesh
+    const size_t nx = desc->nsd == 2 ? 2 * (size_t)desc->ne * Q * Q : 3 * (size_t)desc->ne * Q * Q * Q;
+    r->xnodes.assign(desc->xnodes, desc->xnodes + nx);
     r->desc.xnodes = r->xnodes.data();
   }
   const size_t per = desc->nsd == 2 ? (layout == DGX_NODAL ? Q * Q : L * L)

@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("DGX_LIBRARY") or os.path.join(HERE, "libdgx.so")
 DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D, DGX_SHALLOW = 1, 2, 8, 9, 30
 DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
 DGX_VISCOUS_BR1, DGX_VISCOUS_BR2 = 0, 1
-DGX_BC_WALL, DGX_BC_STATE = 0, 1
+DGX_BC_WALL, DGX_BC_STATE, DGX_BC_FUNCTION = 0, 1, 2
 DGX_FP64, DGX_MIXED = 0, 1
 DGX_TVD, DGX_RK4, DGX_LSRK14 = 3, 4, 14
 DGX_NODAL, DGX_MODAL = 0, 1
@@ -24,13 +24,20 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+class Function(ctypes.Structure):
+    """dgx_function: per-component postfix programs of a function of (x, y, z, t)."""
+    _fields_ = [("ncomp", ctypes.c_int), ("start", _ip), ("code", _ip), ("nconst", ctypes.c_int),
+                ("consts", _dp)]
+
+
 class DomainDesc(ctypes.Structure):
     _fields_ = [("nsd", ctypes.c_int), ("ne", ctypes.c_int), ("p", ctypes.c_int),
                 ("eqntype", ctypes.c_int), ("flux", ctypes.c_int), ("precision", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("vis", ctypes.c_double), ("Pr", ctypes.c_double),
                 ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip),
                 ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5),
-                ("xnodes", _dp), ("no_balance", ctypes.c_int)]
+                ("xnodes", _dp), ("no_balance", ctypes.c_int),
+                ("bc_function", ctypes.POINTER(Function)), ("source", ctypes.POINTER(Function))]
 
 
 class DgxError(RuntimeError):
@@ -60,6 +67,8 @@ def load():
         "dgx_state_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_state_device_ptr": ([h, ctypes.POINTER(_dp)], ctypes.c_int),
         "dgx_rhs": ([h], ctypes.c_int),
+        "dgx_set_time": ([h, ctypes.c_double], ctypes.c_int),
+        "dgx_get_time": ([h, _dp], ctypes.c_int),
         "dgx_residual_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_advance": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
         "dgx_synchronize": ([h], ctypes.c_int),
@@ -87,12 +96,38 @@ def _check(rc):
         raise DgxError(rc, load().dgx_last_error().decode())
 
 
+def function_arrays(fn):
+    """(start, code, consts) int32/int32/float64 arrays of a function: a
+    formats.read_functions block (name -> expression tree) or the arrays
+    themselves."""
+    if isinstance(fn, dict):
+        from .formats import compile_function
+        return compile_function(fn)
+    start, code, consts = fn
+    return (np.ascontiguousarray(start, dtype=np.int32), np.ascontiguousarray(code, dtype=np.int32),
+            np.ascontiguousarray(consts, dtype=np.float64))
+
+
+def make_function(fn, keep):
+    """A dgx_function pointer for fn (None: NULL); keep holds the arrays alive."""
+    if fn is None:
+        return None
+    start, code, consts = function_arrays(fn)
+    if consts.size == 0:
+        consts = np.zeros(1)
+    keep += [start, code, consts]
+    f = Function(len(start) - 1, start.ctypes.data_as(_ip), code.ctypes.data_as(_ip), int(consts.size),
+                 consts.ctypes.data_as(_dp))
+    keep.append(f)
+    return ctypes.pointer(f)
+
+
 class GPUDomain:
     """A DG domain on the GPU: modal FP64 state, residual and RK stepping."""
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
                  precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None,
-                 xnodes=None, balance=True):
+                 xnodes=None, balance=True, bc_function=None, source=None):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -104,6 +139,9 @@ class GPUDomain:
                           (ctypes.c_double * 5)(*([0.0] * 5 if bc_state is None else [float(v) for v in bc_state])),
                           None if xnodes is None else _ptr(self._nodes(xnodes, mesh, p)),
                           0 if balance else 1)
+        self._fkeep = []
+        desc.bc_function = make_function(bc_function, self._fkeep)
+        desc.source = make_function(source, self._fkeep)
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))
@@ -155,6 +193,15 @@ class GPUDomain:
         _check(self.lib.dgx_residual_download(self.h, _ptr(F), layout, 0))
         self.synchronize()
         return F
+
+    def set_time(self, t):
+        _check(self.lib.dgx_set_time(self.h, float(t)))
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        _check(self.lib.dgx_get_time(self.h, ctypes.byref(t)))
+        return t.value
 
     def advance(self, inttype, dt, nsteps):
         _check(self.lib.dgx_advance(self.h, int(inttype), float(dt), int(nsteps)))

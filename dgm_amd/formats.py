@@ -183,7 +183,7 @@ class _ExprParser:
     """Arithmetic expressions of the reference's function definitions
     (src/Parser.y): + - * / ^ (right associative, above unary minus),
     parentheses, the usual elementary functions, PI, and the variables x, y,
-    z, t.  Compiles to a closure over numpy; nothing in the text is executed."""
+    z, t.  Builds a tree of tuples; nothing in the text is executed."""
 
     def __init__(self, toks):
         self.t, self.i = toks, 0
@@ -201,26 +201,19 @@ class _ExprParser:
     def expr(self):
         f = self.term()
         while self.peek() in (("o", "+"), ("o", "-")):
-            op = self.take()[1]
-            g = self.term()
-            f = (lambda a, b: lambda env: a(env) + b(env))(f, g) if op == "+" else \
-                (lambda a, b: lambda env: a(env) - b(env))(f, g)
+            f = ("bin", self.take()[1], f, self.term())
         return f
 
     def term(self):
         f = self.unary()
         while self.peek() in (("o", "*"), ("o", "/")):
-            op = self.take()[1]
-            g = self.unary()
-            f = (lambda a, b: lambda env: a(env) * b(env))(f, g) if op == "*" else \
-                (lambda a, b: lambda env: a(env) / b(env))(f, g)
+            f = ("bin", self.take()[1], f, self.unary())
         return f
 
     def unary(self):
         if self.peek() == ("o", "-"):
             self.take()
-            f = self.unary()
-            return lambda env: -f(env)
+            return ("neg", self.unary())
         if self.peek() == ("o", "+"):
             self.take()
             return self.unary()
@@ -230,30 +223,31 @@ class _ExprParser:
         f = self.atom()
         if self.peek() == ("o", "^"):
             self.take()
-            g = self.unary()
-            return lambda env: np.power(f(env), g(env))
+            return ("bin", "^", f, self.unary())
         return f
 
     def atom(self):
         kind, val = self.take()
         if kind == "n":
-            return lambda env: val
+            return ("num", val)
         if kind == "v":
             if self.peek() == ("o", "("):
                 if val not in _FUNCS:
                     raise ValueError(f"unknown function {val}")
-                fn, args = _FUNCS[val], []
                 self.take("(")
-                args.append(self.expr())
+                args = [self.expr()]
                 while self.peek() == ("o", ","):
                     self.take()
                     args.append(self.expr())
                 self.take(")")
-                return lambda env: fn(*[a(env) for a in args])
+                if len(args) != (2 if val in ("atan2", "pow") else 1):
+                    raise ValueError(f"{val} takes {2 if val in ('atan2', 'pow') else 1} arguments")
+                return ("call", val, args)
             if val in _CONSTS:
-                c = _CONSTS[val]
-                return lambda env: c
-            return lambda env: env[val]
+                return ("num", _CONSTS[val])
+            if val not in ("x", "y", "z", "t"):
+                raise ValueError(f"unknown variable {val}")
+            return ("var", val)
         if val == "(":
             f = self.expr()
             self.take(")")
@@ -261,11 +255,70 @@ class _ExprParser:
         raise ValueError(f"unexpected {val!r}")
 
 
+_BINOPS = {"+": np.add, "-": np.subtract, "*": np.multiply, "/": np.divide, "^": np.power}
+
+
+def eval_tree(node, env):
+    """Value of an expression tree at env (x, y, z, t arrays or scalars)."""
+    kind = node[0]
+    if kind == "num":
+        return node[1]
+    if kind == "var":
+        return env[node[1]]
+    if kind == "neg":
+        return -eval_tree(node[1], env)
+    if kind == "bin":
+        return _BINOPS[node[1]](eval_tree(node[2], env), eval_tree(node[3], env))
+    return _FUNCS[node[1]](*[eval_tree(a, env) for a in node[2]])
+
+
+# postfix instruction set of dgx_function (include/dgx.h): opcode in the low
+# 8 bits, constant-table index above them for DGX_OP_CONST
+OPCODES = {"const": 0, "x": 1, "y": 2, "z": 3, "t": 4, "+": 5, "-": 6, "*": 7, "/": 8, "^": 9,
+           "neg": 10, "sin": 11, "cos": 12, "tan": 13, "asin": 14, "acos": 15, "atan": 16,
+           "sinh": 17, "cosh": 18, "tanh": 19, "exp": 20, "log": 21, "log10": 22, "sqrt": 23,
+           "abs": 24, "fabs": 24, "atan2": 25, "pow": 9}
+
+
+def emit_tree(node, code, consts):
+    """Append the postfix program of an expression tree to code."""
+    kind = node[0]
+    if kind == "num":
+        consts.append(float(node[1]))
+        code.append(OPCODES["const"] | ((len(consts) - 1) << 8))
+    elif kind == "var":
+        code.append(OPCODES[node[1]])
+    elif kind == "neg":
+        emit_tree(node[1], code, consts)
+        code.append(OPCODES["neg"])
+    elif kind == "bin":
+        emit_tree(node[2], code, consts)
+        emit_tree(node[3], code, consts)
+        code.append(OPCODES[node[1]])
+    else:
+        for a in node[2]:
+            emit_tree(a, code, consts)
+        code.append(OPCODES[node[1]])
+
+
+def compile_function(comps: Dict[str, object]):
+    """Postfix programs of a read_functions block, one per component, in the
+    layout of dgx_function: (start [ncomp + 1], code, consts)."""
+    code: List[int] = []
+    consts: List[float] = []
+    start = [0]
+    for tree in comps.values():
+        emit_tree(tree, code, consts)
+        start.append(len(code))
+    return (np.asarray(start, dtype=np.int32), np.asarray(code, dtype=np.int32),
+            np.asarray(consts, dtype=np.float64))
+
+
 def read_functions(path: str) -> Dict[str, Dict[str, object]]:
     """Function definitions of a boundary-type or source file (``root.bct``,
     ``root.src``): blocks ``function NAME { var := expression; ... }``, the
     form FuncBC and FuncSource read (src/BC.hpp, src/Source.hpp:190-230).
-    Returns {NAME: {var: f(env)}} where env maps x, y, z, t to arrays; the
+    Returns {NAME: {var: expression tree}} (eval_tree, compile_function); the
     components keep their order in the block."""
     with open(path) as f:
         text = f.read().replace("\\\n", " ")
@@ -302,7 +355,7 @@ def field_function(comps: Dict[str, object]):
     """(t, x, y) -> array [ncomp, n] of a read_functions block."""
     def fn(t, x, y):
         env = {"t": t, "x": x, "y": y, "z": np.zeros_like(x)}
-        return np.stack([np.broadcast_to(np.asarray(f(env), dtype=np.float64), np.shape(x))
+        return np.stack([np.broadcast_to(np.asarray(eval_tree(f, env), dtype=np.float64), np.shape(x))
                          for f in comps.values()])
     return fn
 

@@ -80,7 +80,31 @@ typedef struct dgx_domain *dgx_handle;
  * (src/ShallowBC.hpp:45-62), imposed state as StateBC (src/BC.cpp:17-28);
  * the ghost values meet the interior through the same Riemann flux as an
  * interior face. */
-enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1 };
+enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1, DGX_BC_FUNCTION = 2 };
+
+/* A function of (x, y, z, t) with ncomp components, as the reference's
+ * FuncBC and FuncSource read them from root.bct / root.src and evaluate them
+ * with its expression parser (src/BC.hpp, src/Source.hpp:190-230,
+ * src/Parser.y).  Each component is a postfix program: code[start[c]] up to
+ * code[start[c+1]] for c < ncomp.  An instruction holds its opcode in the low
+ * 8 bits; DGX_OP_CONST pushes consts[code >> 8].  Binary operators pop b, then
+ * a, and push a op b; a program leaves exactly one value and needs at most
+ * DGX_FN_STACK slots.  The library copies the programs at creation. */
+enum dgx_op {
+  DGX_OP_CONST = 0, DGX_OP_X = 1, DGX_OP_Y = 2, DGX_OP_Z = 3, DGX_OP_T = 4,
+  DGX_OP_ADD = 5, DGX_OP_SUB = 6, DGX_OP_MUL = 7, DGX_OP_DIV = 8, DGX_OP_POW = 9, DGX_OP_NEG = 10,
+  DGX_OP_SIN = 11, DGX_OP_COS = 12, DGX_OP_TAN = 13, DGX_OP_ASIN = 14, DGX_OP_ACOS = 15,
+  DGX_OP_ATAN = 16, DGX_OP_SINH = 17, DGX_OP_COSH = 18, DGX_OP_TANH = 19, DGX_OP_EXP = 20,
+  DGX_OP_LOG = 21, DGX_OP_LOG10 = 22, DGX_OP_SQRT = 23, DGX_OP_ABS = 24, DGX_OP_ATAN2 = 25
+};
+#define DGX_FN_STACK 16
+typedef struct {
+  int ncomp;
+  const int *start;      /* [ncomp + 1] */
+  const int *code;       /* [start[ncomp]] */
+  int nconst;
+  const double *consts;  /* [nconst] */
+} dgx_function;
 
 /* Mesh and physics description.  `vert` holds, per element, nsd rows of
  * vertex coordinates exactly as in a root.msh file ([elem][dim][vertex],
@@ -110,6 +134,14 @@ typedef struct {
                        compiled in the reference (src/Shallow.cpp:251-266), the
                        domain's first residual is stored and subtracted from it
                        and from every later residual */
+  const dgx_function *bc_function;  /* Shallow: (phi, uphi, vphi) of DGX_BC_FUNCTION
+                                       boundaries at the side's Gauss points and the
+                                       stage's time (FuncBC); NULL: none */
+  const dgx_function *source;       /* Shallow: source s(x, y, t) at the nodes, F -= s
+                                       (FuncSource::apply, src/Source.cpp:11-24); NULL: none.
+                                       With either function, quads take the curved-element
+                                       path (nodal metrics from xnodes, or from the
+                                       vertices' bilinear map when xnodes is NULL) */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device
@@ -141,6 +173,13 @@ int dgx_state_device_ptr(dgx_handle h, double **U);
  * Navier_Stokes(3d)::time_derivative src/Navier_Stokes.cpp:89-94,
  * src/Navier_Stokes3d.cpp:135-161). */
 int dgx_rhs(dgx_handle h);
+
+/* Solution time (Domain::t): the time at which dgx_rhs evaluates boundary and
+ * source functions.  dgx_advance moves it through each stage as the
+ * reference's integrators do (RK4: t + dt/2 after the first and third
+ * stages; TVD-RK3: c = 0, 1, 1/2; LSRK14: t_n + c_s dt) and leaves t_n + dt. */
+int dgx_set_time(dgx_handle h, double t);
+int dgx_get_time(dgx_handle h, double *t);
 int dgx_residual_download(dgx_handle h, double *F, int layout, int on_device);
 
 /* Advance nsteps steps of size dt with integrator `inttype`

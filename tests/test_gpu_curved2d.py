@@ -10,6 +10,8 @@ the curved mass matrix by conjugate gradients, the oracle (as the reference)
 by its Cholesky factor, and at those orders their roundings differ by up to
 1.5e-12 (measured; the same elements fed affine nodes agree to 1e-13).
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -183,3 +185,31 @@ def test_cvquad_golden_run():
     assert np.all(dn <= 5e-13), dn
     mask = rn > np.sqrt(np.finfo(float).eps)
     assert np.all(np.where(mask, dn / np.where(mask, rn, 1.0), 0.0) <= 5e-13)
+
+
+def test_non_parallelogram_quads_take_the_curved_path():
+    """Quads that are not parallelograms, given by their vertices only, are
+    curved as in the reference (a straight-sided curve on the bilinear map):
+    the residual and the norms match the oracle fed the bilinear nodes."""
+    p = 4
+    m = meshgen.periodic_quads(4, 4, 0.0, TWO_PI, 0.0, TWO_PI)
+    v = np.array(m.vert, dtype=np.float64)
+    # move interior vertices consistently (shared vertices move together)
+    def shift(x, y):
+        return x + 0.15 * np.sin(x) * np.sin(y), y
+    v[:, 0, :], v[:, 1, :] = shift(v[:, 0, :], v[:, 1, :])
+    m2 = dataclasses.replace(m, vert=v)
+    assert not np.all(meshgen.affine_quads(m2))
+    xn = meshgen.quad_nodes(m2, p)
+    orc = Oracle(m2, p, 1)
+    orc.set_nodes(xn)
+    Up = projected_state(orc, _euler_state(xn))
+    gpu = dgx.GPUDomain(m2, p, dgx.DGX_EULER)
+    assert np.allclose(gpu.coords(), xn, rtol=0, atol=1e-13)
+    gpu.upload(Up)
+    err = rel_l2(gpu.rhs(), orc.rhs(Up))
+    print(f"[curved2d] non-parallelogram rel L2 {err}", flush=True)
+    assert np.all(err <= TOL_RHS), err
+    orc.set_state(Up)
+    assert np.allclose(gpu.norms(), orc.norms(), rtol=1e-12, atol=0)
+    gpu.close()

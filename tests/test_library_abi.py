@@ -198,3 +198,62 @@ def test_acoustic_create_rejects_bad_arguments(lib, case, msg):
     rc = lib.dgx_acoustic_create(ctypes.byref(d), ctypes.byref(ad), None, ctypes.byref(h))
     assert rc == -1 and msg in lib.dgx_last_error()
     assert not h.value
+
+
+def _fn(start, code, consts, keep):
+    start = np.asarray(start, dtype=np.int32)
+    code = np.asarray(code, dtype=np.int32)
+    consts = np.asarray(consts if len(consts) else [0.0], dtype=np.float64)
+    keep += [start, code, consts]
+    f = dgx.Function(len(start) - 1, start.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                     code.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(consts),
+                     consts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    keep.append(f)
+    return ctypes.pointer(f)
+
+
+X, Y, T, ADD, SIN = 1, 2, 4, 5, 11
+
+
+@pytest.mark.parametrize("which,start,code,consts,msg", [
+    ("source", [0, 1, 2], [X, Y], [], b"3 components"),
+    ("source", [0, 1, 2, 2], [X, Y], [], b"empty"),
+    ("source", [0, 1, 2, 3], [X, Y, 99], [], b"bad instruction"),
+    ("source", [0, 1, 2, 3], [X, Y, 0 | (3 << 8)], [1.0], b"constant index"),
+    ("bc", [0, 1, 2, 3], [X, Y, ADD], [], b"stack underflow"),
+    ("bc", [0, 1, 2, 4], [X, Y, X, Y], [], b"one value"),
+    ("bc", [0, 1, 2, 20], [X, Y] + [T] * 17 + [SIN], [], b"DGX_FN_STACK"),
+    ("bc", [0, 1, 2, 3], [X, Y, SIN | (1 << 8)], [], b"bad instruction"),
+])
+def test_functions_validated(lib, which, start, code, consts, msg):
+    m = meshgen.box_quads(2, 2, bc=(2, None, 2, None))
+    d, kd = _desc(m, 2, 30)
+    keep = []
+    good = _fn([0, 1, 2, 3], [X, Y, T], [], keep)
+    bad = _fn(start, code, consts, keep)
+    d.bc_function = bad if which == "bc" else good
+    d.source = bad if which == "source" else None
+    h = ctypes.c_void_p()
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert msg in lib.dgx_last_error(), lib.dgx_last_error()
+    assert not h.value
+
+
+def test_functions_need_shallow_and_function_rows_need_a_function(lib):
+    keep = []
+    m = meshgen.periodic_quads(2, 2)
+    d, k1 = _desc(m, 2, 1)
+    d.source = _fn([0, 1, 2, 3], [X, Y, T], [], keep)
+    h = ctypes.c_void_p()
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert b"available for Shallow" in lib.dgx_last_error()
+    m = meshgen.box_quads(2, 2, bc=(2, None, 0, None))
+    d, k2 = _desc(m, 2, 30)
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert b"without bc_function" in lib.dgx_last_error()
+
+
+def test_time_entry_points_reject_null(lib):
+    t = ctypes.c_double()
+    assert lib.dgx_set_time(None, 0.0) == -1
+    assert lib.dgx_get_time(None, ctypes.byref(t)) == -1
