@@ -787,8 +787,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     if (e >= 0 && e < ne && s >= 0 && s < d->ns && le == -1) {   // boundary side (e, s, -1, bc type)
       if (ls != DGX_BC_WALL && ls != DGX_BC_STATE && ls != DGX_BC_FUNCTION)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
-      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
-        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
+      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW && desc->eqntype != DGX_EULER)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler, Euler3d and Shallow");
       sides[(size_t)e * d->ns + s].bc = ls;
       continue;
     }
@@ -1048,8 +1048,8 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
                 "curved elements (xnodes) are available for Euler3d (p <= 6), Euler, Navier_Stokes and Shallow, DGX_FP64");
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
-      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW)
-        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler3d and Shallow");
+      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW && desc->eqntype != DGX_EULER)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler, Euler3d and Shallow");
       const int t = desc->con[4 * r + 3];
       if (t != DGX_BC_WALL && t != DGX_BC_STATE && t != DGX_BC_FUNCTION)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");

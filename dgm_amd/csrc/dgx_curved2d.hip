@@ -326,9 +326,8 @@ k_rhs2c(CurvedArgs a) {
       for (int v = 0; v < 3; ++v) UR[v] = eval_program(a.fn.code, a.fn.bc[v], a.fn.bc[v + 1], a.fn.consts, xy[0], xy[1], a.t);
     } else if (SHALLOW) {
       ghost_shallow2(-1 - nb, UL, nx, ny, a.phys.bcU, UR);
-    } else {   // no Euler boundaries on quads (rejected at creation)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) UR[v] = UL[v];
+    } else {   // Euler boundary (Navier-Stokes meshes have none: rejected at creation)
+      ghost2(-1 - nb, UL, nx, ny, a.phys.bcU, UR);
     }
     double gv[4] = {0.0, 0.0, 0.0, 0.0};
     if (NS && nb >= 0) {   // viscous: own normal flux less the average 1/2 (own - neighbour's), negated

@@ -108,6 +108,23 @@ __device__ __forceinline__ void ghost_shallow2(int bc, const double UL[3], doubl
   UB[2] = (UL[2] * nx - UL[1] * ny) * nx;
 }
 
+// Ghost values of an Euler (2D) boundary side: bc 0 slip wall
+// (Euler::WallBC::apply, src/EulerBC.hpp:43-62: the density kept, the momentum
+// reflected onto the tangent, the energy moved to the ghost's kinetic energy),
+// bc 1 state (StateBC::apply, src/BC.cpp:17-28).
+__device__ __forceinline__ void ghost2(int bc, const double UL[4], double nx, double ny, const double bcU[5],
+                                       double UB[4]) {
+#pragma clang fp contract(off)
+  if (bc == 1) {
+    UB[0] = bcU[0]; UB[1] = bcU[1]; UB[2] = bcU[2]; UB[3] = bcU[3];
+    return;
+  }
+  UB[0] = UL[0];
+  UB[1] = (UL[1] * ny - UL[2] * nx) * ny;
+  UB[2] = (UL[2] * nx - UL[1] * ny) * nx;
+  UB[3] = UL[3] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2]) / UL[0] + 0.5 * (UB[1] * UB[1] + UB[2] * UB[2]) / UB[0];
+}
+
 // One component of a boundary or source function at (x, y, t): the postfix
 // program code[b] .. code[e] (dgx_function).  The host checks every program at
 // domain creation (opcodes, constant indices, a stack depth within 16 that

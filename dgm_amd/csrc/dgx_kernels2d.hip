@@ -163,14 +163,20 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
     const int s = i / QF, a = i - s * QF;
     const int es = e * 4 + s;
     const int nb = geo.nbr[es], ns = geo.nside[es];
-    const int an = geo.perm[geo.permid[es] * QF + a];
+    const int an = nb >= 0 ? geo.perm[geo.permid[es] * QF + a] : a;
     const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1];
     const double jr = geo.fjr[es];
     double UL[4], UR[4], FL[4], FR[4], Fh[4], lL, lR;
     const double *tl = T + ((size_t)e * 4 + s) * 4 * QF;
-    const double *tr = T + ((size_t)nb * 4 + ns) * 4 * QF;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) { UL[v] = tl[v * QF + a]; UR[v] = tr[v * QF + an]; }
+    for (int v = 0; v < 4; ++v) UL[v] = tl[v * QF + a];
+    if (nb >= 0) {
+      const double *tr = T + ((size_t)nb * 4 + ns) * 4 * QF;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) UR[v] = tr[v * QF + an];
+    } else {   // boundary side of type -1 - nb (Navier-Stokes meshes have none: rejected at creation)
+      ghost2(-1 - nb, UL, nx, ny, ph.bcU, UR);
+    }
     bool bl, br;
     euler_normal_flux2(UL, nx, ny, gm1, FL, lL, bl);
     euler_normal_flux2(UR, nx, ny, gm1, FR, lR, br);
@@ -185,7 +191,7 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double g = jr * (Fh[v] - FL[v]);
-      if (NS && v > 0) {
+      if (NS && v > 0 && nb >= 0) {
         const double *vl = TV + ((size_t)e * 4 + s) * 3 * QF;
         const double *vr = TV + ((size_t)nb * 4 + ns) * 3 * QF;
         g += 0.5 * jr * (vl[(v - 1) * QF + a] + vr[(v - 1) * QF + an]);

@@ -58,7 +58,7 @@ def _check(lib, rc):
 
 
 def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0,
-          balance=True):
+          balance=True, bc_state=None):
     vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
     con = np.ascontiguousarray(mesh.con, dtype=np.int32)
     d = dgx.DomainDesc(int(mesh.nsd), int(mesh.ne), int(p), int(eqntype), int(flux), int(precision),
@@ -66,6 +66,9 @@ def _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous=dgx.DGX_VIS
                        vert.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(con),
                        con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(viscous), float(br2_eta))
     d.no_balance = 0 if balance else 1
+    if bc_state is not None:
+        vals = [float(v) for v in bc_state] + [0.0] * (5 - len(bc_state))
+        d.bc_state = (ctypes.c_double * 5)(*vals[:5])
     return d, (vert, con)
 
 
@@ -112,8 +115,8 @@ class Server:
 
     def submit(self, mesh, p, eqntype, U0, inttype, dt, nsteps, gamma=1.4, vis=0.0, Pr=0.72,
                flux=dgx.DGX_FLUX_LF, precision=dgx.DGX_FP64, layout=dgx.DGX_NODAL,
-               viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0, balance=True) -> int:
-        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous, br2_eta, balance)
+               viscous=dgx.DGX_VISCOUS_BR1, br2_eta=0.0, balance=True, bc_state=None) -> int:
+        d, keep = _desc(mesh, p, eqntype, gamma, vis, Pr, flux, precision, viscous, br2_eta, balance, bc_state)
         U0 = np.ascontiguousarray(U0, dtype=np.float64)
         nf = 3 if eqntype == dgx.DGX_SHALLOW else mesh.nsd + 2
         L, Q = p + 1, p + 2

@@ -1097,11 +1097,35 @@ struct Domain {
 
   int flux_type = 0;   // 0 Lax-Friedrichs, 1 Roe
 
-  // 2D master-side Riemann flux (src/Euler.cpp:399-757)
+  // Euler (2D) ghost values: Euler::WallBC::apply (src/EulerBC.hpp:43-62)
+  // keeps the density, reflects the momentum onto the tangent and adjusts the
+  // energy to the ghost's kinetic energy; StateBC (src/BC.cpp:17-28) holds
+  // bc_state
+  void ghost2d(int bc, const double *Ul, double nx, double ny, double *Ub) const {
+    if (bc == 1) { for (int n = 0; n < 4; ++n) Ub[n] = bc_state[n]; return; }
+    Ub[0] = Ul[0];
+    Ub[1] = (Ul[1] * ny - Ul[2] * nx) * ny;
+    Ub[2] = (Ul[2] * nx - Ul[1] * ny) * nx;
+    Ub[3] = Ul[3] - 0.5 * (Ul[1] * Ul[1] + Ul[2] * Ul[2]) / Ul[0] + 0.5 * (Ub[1] * Ub[1] + Ub[2] * Ub[2]) / Ub[0];
+  }
+
+  // 2D master-side Riemann flux (src/Euler.cpp:399-757); boundary sides pair
+  // with their ghost values
   void convective_flux2d(VF &Us, VF &Uf) {
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
+        if (S.bc >= 0) {
+          if (S.bc > 1) status |= 2;   // Euler has wall and state boundaries here
+          for (int g = 0; g < qs; ++g) {
+            double Ul[4], Ub[4], Fpl[4], Fmr[4], Fh[4];
+            for (int n = 0; n < 4; ++n) Ul[n] = es(Us, n, e, s)[g];
+            ghost2d(S.bc, Ul, S.nx[g], S.ny[g], Ub);
+            status |= point_flux2d(gamma, flux_type, Ul, Ub, S.nx[g], S.ny[g], Fpl, Fmr, Fh);
+            for (int n = 0; n < 4; ++n) { es(Us, n, e, s)[g] = Fpl[n]; es(Uf, n, e, s)[g] = Fh[n]; }
+          }
+          continue;
+        }
         if (!S.master) continue;
         for (int g = 0; g < qs; ++g) {
           const int ql = sidemap(S, g), qr = sidemap_slave(g);
@@ -1337,8 +1361,11 @@ struct Domain {
     if (nsd == 3) { lift(Us, Fv, v, e, fz, 'z'); project_vec(e, fz); }
   }
 
-  // 2D BR1 diffusion (src/Navier_Stokes.cpp:113-258)
+  // 2D BR1 diffusion (src/Navier_Stokes.cpp:113-258); periodic meshes only
   void diffusion2d(VF &Us, VF &Uf, VF &Ur) {
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < nsides; ++s)
+        if (el[e].side[s].bc >= 0) status |= 2;   // no viscous boundaries restated in 2D
     fill_sides_all(Us);
     primitive_sides(Us);
     average_flux(Us, Uf, '+');

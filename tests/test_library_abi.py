@@ -71,7 +71,7 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=8, viscous=1), b"DGX_VISCOUS_BR2"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=-1.0), b"br2_eta"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
-    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler3d and Shallow"),
+    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler, Euler3d and Shallow"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
     (dict(p=3, eqn=9, curved=True), b"curved elements"),
     (dict(p=7, eqn=8, curved=True), b"curved elements"),
@@ -100,7 +100,7 @@ def test_create_rejects_bad_arguments(lib, case, msg):
     (dict(precision=1), b"DGX_MIXED"),
     (dict(nsd3=True), b"eqntype does not match nsd"),
     (dict(boundary=4), b"unknown boundary type"),
-    (dict(boundary=0, eqn=1), b"boundary sides are available"),
+    (dict(boundary=0, eqn=2), b"boundary sides are available"),
 ])
 def test_shallow_create_rejects_bad_arguments(lib, case, msg):
     if case.get("nsd3"):
