@@ -155,10 +155,13 @@ int enqueue_slice(dgx_server *s, Request *r) {
   return DGX_OK;
 }
 
+// Affine quads only: the batched kernels use constant metrics, so domains on
+// the curved path (xnodes, non-parallelogram quads) step on their own.
 bool batchable(const dgx_server *s, const Request *r) {
   const dgx_domain_desc &d = r->desc;
-  return s->cfg.batch && d.nsd == 2 && !d.xnodes && (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) &&
-         d.precision == DGX_FP64 && r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
+  return s->cfg.batch && d.nsd == 2 && !d.xnodes && r->h && !r->h->curved &&
+         (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) && d.precision == DGX_FP64 &&
+         r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
 }
 
 // Join r (admitted, state uploaded) to the group of its physics and order.

@@ -213,3 +213,29 @@ def test_non_parallelogram_quads_take_the_curved_path():
     orc.set_state(Up)
     assert np.allclose(gpu.norms(), orc.norms(), rtol=1e-12, atol=0)
     gpu.close()
+
+
+def test_served_non_parallelogram_request_steps_on_its_own():
+    """A request on non-parallelogram quads is curved, so the batching server
+    steps it on its own; its result equals the direct domain's."""
+    from dgm_amd import serve
+    p, dt, n = 3, 1e-3, 10
+    m = meshgen.periodic_quads(4, 4, 0.0, TWO_PI, 0.0, TWO_PI)
+    v = np.array(m.vert, dtype=np.float64)
+    v[:, 0, :] = v[:, 0, :] + 0.15 * np.sin(v[:, 0, :]) * np.sin(v[:, 1, :])
+    m2 = dataclasses.replace(m, vert=v)
+    xn = meshgen.quad_nodes(m2, p)
+    U = _euler_state(xn)
+    gpu = dgx.GPUDomain(m2, p, dgx.DGX_EULER)
+    gpu.upload(U)
+    gpu.advance(dgx.DGX_LSRK14, dt, n)
+    want = gpu.download()
+    gpu.close()
+    srv = serve.Server(hbm_budget=1 << 34, max_active=2, steps_per_slice=5, batch=True)
+    rid = srv.submit(m2, p, dgx.DGX_EULER, U, dgx.DGX_LSRK14, dt, n)
+    srv.run_until_done(timeout_s=60.0)
+    out = srv.result(rid)[0]
+    srv.close()
+    err = rel_l2(out, want)
+    print(f"[curved2d] served non-parallelogram rel L2 {err}", flush=True)
+    assert np.all(err <= 1e-13), err
