@@ -30,6 +30,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import dgx, formats, ic
+from . import mesh as meshgen
 
 DGM_SUCCESS = 0
 
@@ -112,18 +113,47 @@ class ReoProblem:
         prm.update(self.table)
         return prm
 
+    def _boundaries(self, mesh, kw):
+        """root.bc sides whose tag root.bct defines as a function (FuncBC),
+        and the first function of root.src as the source (FuncSource)."""
+        import dataclasses
+        if os.path.exists(self.root + ".bc"):
+            rows = formats.read_bc(self.root + ".bc")
+            funcs = formats.read_functions(self.root + ".bct") if os.path.exists(self.root + ".bct") else {}
+            tags = sorted({t for t, _, _ in rows})
+            if any(t not in funcs for t in tags) or len(tags) > 1:
+                raise NotImplementedError(f"Reo::Problem: boundary tags {tags}: one function boundary "
+                                          "defined in root.bct is read from run directories")
+            kw["bc_function"] = funcs[tags[0]]
+            mesh = dataclasses.replace(mesh, con=formats.boundary_con(mesh.con, rows,
+                                                                      {tags[0]: dgx.DGX_BC_FUNCTION}))
+        if os.path.exists(self.root + ".src"):
+            kw["source"] = next(iter(formats.read_functions(self.root + ".src").values()))
+        return mesh
+
     def setup(self) -> dgx.GPUDomain:
         prm = self.params
         eqn = int(prm.get("eqntype", 0))
         if eqn not in (dgx.DGX_EULER, dgx.DGX_NAVIER_STOKES, dgx.DGX_EULER3D,
-                       dgx.DGX_NAVIER_STOKES3D):
+                       dgx.DGX_NAVIER_STOKES3D, dgx.DGX_SHALLOW):
             raise NotImplementedError(f"Reo::Problem: equation type {eqn} has no GPU path")
-        mesh = formats.read_mesh(self.root)
+        p = int(prm["p"])
+        kw = {}
+        mesh = self._boundaries(formats.read_mesh(self.root), kw)
+        self.xnodes = None
+        if os.path.exists(self.root + ".crv"):     # curved sides (Field::read_curve)
+            if mesh.nsd != 2:
+                raise NotImplementedError("Reo::Problem: root.crv is read for quads; pass hex nodes as xnodes")
+            self.xnodes, _ = meshgen.curved_quad_nodes(mesh, p, formats.read_crv(self.root + ".crv"))
+        elif mesh.nsd == 2 and not np.all(meshgen.affine_quads(mesh)):
+            self.xnodes = meshgen.quad_nodes(mesh, p)   # what the library does by itself, kept for the caller
         self.mesh = mesh
+        self.eqn = eqn
         vis = float(prm["vis"]) if eqn in (dgx.DGX_NAVIER_STOKES, dgx.DGX_NAVIER_STOKES3D) else 0.0
-        dom = dgx.GPUDomain(mesh, int(prm["p"]), eqn, gamma=float(prm["gamma"]), vis=vis,
+        dom = dgx.GPUDomain(mesh, p, eqn, gamma=float(prm["gamma"]), vis=vis,
                             Pr=float(prm["Pr"]), flux=int(prm["flux"]),
-                            precision=int(prm["precision"]))
+                            precision=int(prm["precision"]), xnodes=self.xnodes, **kw)
+        dom.set_time(self.t)
         icd = formats.read_ic(self.root + ".ic")
         x = dom.coords()
         dom.upload(ic.from_code(icd, x, float(prm["gamma"])))
@@ -135,7 +165,7 @@ class ReoProblem:
         return {"Name": os.path.basename(self.root), "Nsd": self.mesh.nsd,
                 "Lmax": int(p["p"]) + 1, "Step": self.step, "Time": self.t,
                 "Time step": float(p["dt"]), "Viscosity": float(p["vis"]),
-                "Field names": _FIELDS[self.mesh.nsd]}
+                "Field names": ["phi", "uphi", "vphi"] if self.eqn == dgx.DGX_SHALLOW else _FIELDS[self.mesh.nsd]}
 
     def write(self, fname: str) -> None:
         """Restart file of the current state (modal, JSON header)."""

@@ -1,7 +1,9 @@
 """End to end on the GPU through the pydgm-style driver (dgm_amd.problem).
 
 ``ReoProblem(args, table, comm).solve()`` runs the reference's own regression
-run directories (runs/vquad, runs/vhex: mesh, connectivity, IC, parameters)
+run directories (runs/vquad, runs/vhex, runs/cvquad with its arc sides,
+runs/shallow_mms with its function boundaries and source: mesh, connectivity,
+IC, parameters)
 on libdgx.so and writes ``root.rst``; that file must match the reference's
 golden ``.ref`` under the reference's own diff rule and tolerances
 (runs/vquad.tst:23-24, runs/vhex.tst:23-24; src/Domain.cpp:1000-1071).
@@ -25,8 +27,9 @@ REPS_M = np.sqrt(np.finfo(float).eps)
 
 
 def _run(tmp_path, root, extra=()):
-    for ext in (".inp", ".msh", ".con", ".ic"):
-        shutil.copy(os.path.join(GOLDEN, root + ext), tmp_path / (root + ext))
+    for ext in (".inp", ".msh", ".con", ".ic", ".crv", ".bc", ".bct", ".src"):
+        if os.path.exists(os.path.join(GOLDEN, root + ext)):
+            shutil.copy(os.path.join(GOLDEN, root + ext), tmp_path / (root + ext))
     base = str(tmp_path / root)
     log = io.StringIO()
     prob = dgm.ReoProblem(dgm.ArgList(list(extra) + [base]), dgm.Table(), dgm.SerialComm(), out=log)
@@ -38,7 +41,8 @@ def _run(tmp_path, root, extra=()):
     return base, prob
 
 
-@pytest.mark.parametrize("root,aeps,reps", [("vquad", 5e-12, 5e-12), ("vhex", 5e-13, 5e-10)])
+@pytest.mark.parametrize("root,aeps,reps", [("vquad", 5e-12, 5e-12), ("vhex", 5e-13, 5e-10),
+                                            ("cvquad", 5e-13, 5e-13), ("shallow_mms", 1e-9, 1e-10)])
 def test_golden_run_directory(tmp_path, root, aeps, reps):
     print(f"[problem] {root}: full run through ReoProblem.solve()", flush=True)
     base, prob = _run(tmp_path, root)
@@ -48,6 +52,8 @@ def test_golden_run_directory(tmp_path, root, aeps, reps):
     ref = formats.read_restart(os.path.join(GOLDEN, root + ".ref"), ne)
     assert out.header["Step"] == int(prm["Nt"])
     o = Oracle(prob.mesh, int(prm["p"]), int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    if prob.xnodes is not None:                           # the norms integrate over the curved elements
+        o.set_nodes(prob.xnodes)
     rn, dn = dgm_diff(o, ref.data, out.data)
     assert np.all(dn <= aeps), f"{root}: absolute diff {dn}"
     mask = rn > REPS_M
