@@ -27,13 +27,16 @@
 //   src/Navier_Stokes3d.cpp:182-391               3D BR1 diffusion
 //   src/LinEuler_quasi.cpp:151-495                2D linearized Euler (uniform mean flow)
 //   src/Shallow.cpp:251-525, src/ShallowBC.hpp:45-62  shallow water: convection, LF flux,
-//                                                 wall, balance term
-//   src/BC.cpp:95-200, src/Source.cpp:11-75       function boundaries and sources
+//                                                 wall, balance term (pinned by runs/shallow_mms)
+//   src/BC.cpp:95-200, src/Source.cpp:11-75       function boundaries and sources (pinned
+//                                                 by runs/shallow_mms)
 //   src/Navier_Stokes3d.cpp:472-512, src/Channel.hpp:117-163  isothermal walls, bulk forcing:
 //                                                 parity unpinned (runs/chan is not reproduced)
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
+//   src/EulerBC.hpp:43-62                         2D slip wall
 //   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
 //   src/Quad.cpp:263-300, 1309-1452               curved quads: nodal metrics, edges, mass matrix
+//                                                 (pinned by runs/cvquad and runs/shallow_mms)
 //   (no reference counterpart)                    3D BR2 face gradient: its parity
 //                                                 against the reference is unpinned; the
 //                                                 BR2 tests pin it by invariants
