@@ -351,6 +351,40 @@ def read_functions(path: str) -> Dict[str, Dict[str, object]]:
     return out
 
 
+def read_bc_types(path: str) -> Dict[str, tuple]:
+    """Boundary types of ``root.bct`` with numeric parameters: blocks
+    ``code tag { v0 v1 ... }`` as Euler3d::make_BC and Navier_Stokes3d::make_BC
+    read them (src/Euler3d.cpp:849-868, src/Navier_Stokes3d.cpp:431-455;
+    e.g. ``Isothermal_Wall T { Tw Eref }``, ``State S { r ru rv rw rE }``).
+    Returns {tag: (lower-case code, [values])}.  ``function`` blocks are left
+    to read_functions."""
+    with open(path) as f:
+        toks = f.read().replace("{", " { ").replace("}", " } ").split()
+    out: Dict[str, tuple] = {}
+    i = 0
+    while i < len(toks):
+        code = toks[i].lower()
+        if code == "function":
+            depth, i = 0, i + 2
+            while i < len(toks):
+                depth += (toks[i] == "{") - (toks[i] == "}")
+                i += 1
+                if depth == 0:
+                    break
+            continue
+        if i + 2 >= len(toks) or toks[i + 2] != "{":
+            raise ValueError(f"{path}: expected 'code tag {{ ... }}' at {toks[i:i + 3]}")
+        tag = toks[i + 1]
+        j = i + 3
+        vals: List[float] = []
+        while toks[j] != "}":
+            vals.append(float(toks[j]))
+            j += 1
+        out[tag] = (code, vals)
+        i = j + 1
+    return out
+
+
 def field_function(comps: Dict[str, object]):
     """(t, x, y) -> array [ncomp, n] of a read_functions block."""
     def fn(t, x, y):

@@ -30,8 +30,8 @@
 //                                                 wall, balance term (pinned by runs/shallow_mms)
 //   src/BC.cpp:95-200, src/Source.cpp:11-75       function boundaries and sources (pinned
 //                                                 by runs/shallow_mms)
-//   src/Navier_Stokes3d.cpp:472-512, src/Channel.hpp:117-163  isothermal walls, bulk forcing:
-//                                                 parity unpinned (runs/chan is not reproduced)
+//   src/Navier_Stokes3d.cpp:472-512, src/Channel.hpp:117-163  isothermal walls, bulk forcing
+//                                                 (pinned by runs/chan: 3D NS BR1, p = 3)
 //   src/Euler3d.cpp:876-909, src/BC.cpp:17-28     3D slip wall and state boundaries
 //   src/EulerBC.hpp:43-62                         2D slip wall
 //   src/Hex.cpp:289-318, 3992-4080                curved hexes: nodal metrics, mass matrix
@@ -1233,16 +1233,23 @@ struct Domain {
   }
 
   // 3D master-side Lax-Friedrichs (or Roe) flux in the rotated frame (src/Euler3d.cpp:584-817);
-  // boundary sides pair with their ghost values
+  // boundary sides pair with their ghost values.  The inviscid ghost comes from
+  // Domain::BC_db, which Euler3d::setup_BCdb fills (src/Euler3d.cpp:136, 824-868):
+  // Euler3d::make_BC turns every code containing "wall" -- Isothermal_Wall
+  // included -- into the slip Euler3d::WallBC (:857-860), and Euler3d::convection
+  // applies that database (:476, :930-933).  The isothermal ghost lives in
+  // Navier_Stokes3d's own BC_db (src/Navier_Stokes3d.hpp:83, .cpp:398-465) and
+  // only the viscous terms see it (primitive_ghosts, average_flux).
   void convective_flux3d(VF &Us, VF &Uf) {
     for (int e = 0; e < ne; ++e)
       for (int s = 0; s < nsides; ++s) {
         const SideInfo &S = el[e].side[s];
         if (S.bc >= 0) {
+          const int inviscid_bc = (S.bc == 3) ? 0 : S.bc;
           for (int g = 0; g < qs; ++g) {
             double Ul[5], Ub[5], Fpl[5], Fmr[5], Fh[5];
             for (int n = 0; n < 5; ++n) Ul[n] = es(Us, n, e, s)[g];
-            ghost3d(S.bc, Ul, S.nx[g], S.ny[g], S.nz[g], Ub);
+            ghost3d(inviscid_bc, Ul, S.nx[g], S.ny[g], S.nz[g], Ub);
             status |= point_flux3d(gamma, flux_type, Ul, Ub, S.nx[g], S.ny[g], S.nz[g], Fpl, Fmr, Fh);
             for (int n = 0; n < 5; ++n) { es(Us, n, e, s)[g] = Fpl[n]; es(Uf, n, e, s)[g] = Fh[n]; }
           }

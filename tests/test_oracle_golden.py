@@ -106,3 +106,49 @@ def test_restart_reader_consumes_whole_files():
         r = formats.read_restart(f"{GOLDEN}/{root}.ref", ne)
         assert r.data.shape[1] == ne
         assert np.all(np.isfinite(r.data))
+
+
+def _chan_oracle():
+    """runs/chan as the reference runs it (runs/chan/chan.tst): 3D Navier-Stokes
+    (BR1, LF), p = 3 on 512 hexes periodic in x and y, isothermal walls at
+    z = 0 and 2 (``Isothermal_Wall T {Tw Eref}`` of chan.bct), the channel's bulk
+    forcing (Channel::holdBulkMassFlow, src/Channel.hpp:117-163), restarted
+    from chan.ini and stepped 100 TVD-RK3 steps with a restart write (a
+    projection) every Ntout = 50 steps."""
+    import dataclasses
+    from oracle_lib import Oracle
+    prm = formats.read_inp(f"{GOLDEN}/chan.inp")
+    mesh = formats.read_mesh(f"{GOLDEN}/chan")
+    rows = formats.read_bc(f"{GOLDEN}/chan.bc")
+    assert {t for t, _, _ in rows} == {"T"}
+    code, (tw, eref) = formats.read_bc_types(f"{GOLDEN}/chan.bct")["T"]
+    assert code == "isothermal_wall"
+    mesh = dataclasses.replace(mesh, con=formats.boundary_con(mesh.con, rows, {"T": 3}))
+    p = int(prm["p"])
+    o = Oracle(mesh, p, int(prm["eqntype"]), vis=float(prm["vis"]))
+    o.set_isothermal(tw, eref)
+    o.set_channel(True)
+    ini = formats.read_restart(f"{GOLDEN}/chan.ini", mesh.ne)
+    o.set_modal(ini.data)
+    Nt, Ntout, dt = int(prm["Nt"]), int(prm["Ntout"]), float(prm["dt"])
+    done = 0
+    while done < Nt:
+        n = min(Ntout, Nt - done)
+        o.advance(int(prm["inttype"]), dt, n)
+        done += n
+        o.project_state()
+    return o, o.modal()
+
+
+def test_chan_3d_ns_isothermal_walls_bulk_forcing():
+    # runs/chan/chan.tst: aeps = 5e-10, reps = 1e-8.  The inviscid wall flux
+    # takes the slip ghost of Euler3d::WallBC (Domain::BC_db) and only the
+    # viscous terms take the isothermal ghost (Navier_Stokes3d::BC_db); with
+    # the isothermal ghost in both the difference is up to 7e-2
+    o, Uh = _chan_oracle()
+    ref = formats.read_restart(f"{GOLDEN}/chan.ref", o.ne)
+    rn, dn = dgm_diff(o, ref.data, Uh)
+    assert np.all(dn <= 5e-10), f"chan: absolute diff {dn}"
+    mask = rn > REPS_M
+    rel = np.where(mask, dn / np.where(mask, rn, 1.0), 0.0)
+    assert np.all(rel <= 1e-8), f"chan: relative diff {rel}"
