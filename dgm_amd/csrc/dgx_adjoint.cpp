@@ -240,10 +240,20 @@ int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad
   dd.eqntype = DGX_EULER;   // geometry, links, tables and work buffers of a 4-field quad domain
   dd.flux = DGX_FLUX_LAX_FRIEDRICHS;
   dd.vis = 0.0;
+  if (mesh->xnodes || mesh->bc_function || mesh->source)
+    return fail(DGX_ERR_ARGUMENT, "the acoustic problem runs on affine quads (no xnodes or functions)");
+  for (int r = 0; r < mesh->ncon; ++r)
+    if (mesh->con && mesh->con[4 * r + 2] == -1)
+      return fail(DGX_ERR_ARGUMENT, "the acoustic problem runs on periodic meshes (no boundary rows)");
   dgx_acoustic *a = new dgx_acoustic();
   int rc = dgx_domain_create(&dd, stream, &a->dom);
   if (rc) { delete a; return rc; }
   dgx_domain *d = a->dom;
+  if (d->curved) {   // a non-parallelogram quad: k_lin2 uses constant per-element metrics
+    dgx_domain_destroy(d);
+    delete a;
+    return fail(DGX_ERR_ARGUMENT, "the acoustic problem needs parallelogram quads (constant Jacobians)");
+  }
   a->p = d->p; a->ne = d->ne; a->nm = d->nm; a->nrec = ad->nrec; a->nsteps = ad->nsteps;
   a->dt = ad->dt; a->f0 = ad->f0; a->t0 = ad->t0; a->u0 = ad->u0; a->v0 = ad->v0;
   const double g = d->gamma, g1 = g - 1.0, rho0 = ad->rho0, u0 = ad->u0, v0 = ad->v0, p0 = ad->p0;

@@ -298,8 +298,19 @@ int vterm(dgx_domain *d, const double *U, const double *T) {
   return launch_stage(d, s);
 }
 
-// Residual of state `U` into `R` (Domain::time_derivative).
+int residual_core(dgx_domain *d, const double *U, double *R);
+
+// Residual of state `U` into `R` (Domain::time_derivative), then the channel's
+// bulk forcing where the domain has it (src/Navier_Stokes3d.cpp:160).
 int residual(dgx_domain *d, const double *U, double *R) {
+  int rc = residual_core(d, U, R);
+  if (rc || !d->bulk) return rc;
+  hipError_t e = timed(d, 6, [&] { return dgx::bulk_hold(R, d->d_J, d->ne, d->nm, d->nf, d->stream); });
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("bulk forcing: ") + hipGetErrorString(e));
+  return DGX_OK;
+}
+
+int residual_core(dgx_domain *d, const double *U, double *R) {
   int rc;
   if (d->fused) {
     if ((rc = launch(d, dgx::OP_TRACES, U, nullptr))) return rc;
@@ -785,11 +796,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   for (int r = 0; r < desc->ncon; ++r) {
     const int e = desc->con[4 * r], s = desc->con[4 * r + 1], le = desc->con[4 * r + 2], ls = desc->con[4 * r + 3];
     if (e >= 0 && e < ne && s >= 0 && s < d->ns && le == -1) {   // boundary side (e, s, -1, bc type)
-      if (ls != DGX_BC_WALL && ls != DGX_BC_STATE && ls != DGX_BC_FUNCTION)
-        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
-      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW && desc->eqntype != DGX_EULER)
-        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler, Euler3d and Shallow");
-      sides[(size_t)e * d->ns + s].bc = ls;
+      sides[(size_t)e * d->ns + s].bc = ls;   // types checked in dgx_domain_create
       continue;
     }
     if (e < 0 || e >= ne || le < 0 || le >= ne || s < 0 || s >= d->ns || ls < 0 || ls >= d->ns)
@@ -918,6 +925,8 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   d->phys.gamma = d->gamma; d->phys.vis = d->vis; d->phys.Pr = d->Pr; d->phys.flux = d->flux;
   d->phys.visc = desc->viscous;
   for (int v = 0; v < 5; ++v) d->phys.bcU[v] = desc->bc_state[v];
+  d->phys.wallT = desc->wall_T;
+  d->phys.wallE = desc->wall_Eref;
   d->phys.eta = desc->br2_eta > 0.0 ? desc->br2_eta : 6.0;
 
   const size_t n = d->state_len();
@@ -942,6 +951,11 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   }
   if ((desc->bc_function || desc->source) && (rc = setup_functions(d, desc))) return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
+  if (desc->bulk_forcing) {
+    d->bulk = true;
+    if ((rc = alloc(&d->d_J, ne))) return rc;
+    HIPCHK(hipMemcpy(d->d_J, d->J.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
   d->mixed = d->prec == DGX_MIXED;
   if (d->visc && d->mixed) {
@@ -965,7 +979,7 @@ void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
                   d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
-                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn};
+                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -1046,17 +1060,33 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
                                              desc->eqntype == DGX_SHALLOW)))))
     return fail(DGX_ERR_ARGUMENT,
                 "curved elements (xnodes) are available for Euler3d (p <= 6), Euler, Navier_Stokes and Shallow, DGX_FP64");
+  const bool ns3 = desc->eqntype == DGX_NAVIER_STOKES3D;
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
-      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW && desc->eqntype != DGX_EULER)
-        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler, Euler3d and Shallow");
+      if (desc->eqntype != DGX_EULER3D && desc->eqntype != DGX_SHALLOW && desc->eqntype != DGX_EULER && !ns3)
+        return fail(DGX_ERR_ARGUMENT, "boundary sides are available for Euler, Euler3d, Navier_Stokes3d and Shallow");
       const int t = desc->con[4 * r + 3];
-      if (t != DGX_BC_WALL && t != DGX_BC_STATE && t != DGX_BC_FUNCTION)
+      if (t != DGX_BC_WALL && t != DGX_BC_STATE && t != DGX_BC_FUNCTION && t != DGX_BC_ISOTHERMAL)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) + ": unknown boundary type");
+      if (t == DGX_BC_ISOTHERMAL && !ns3)
+        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) +
+                                          ": DGX_BC_ISOTHERMAL is a Navier_Stokes3d boundary");
+      if (ns3 && t == DGX_BC_WALL)
+        return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) +
+                                          ": Navier_Stokes3d has no slip-wall viscous flux "
+                                          "(Euler3d::WallBC::apply_flux); use DGX_BC_ISOTHERMAL");
+      if (ns3 && (desc->precision != DGX_FP64 || desc->viscous != DGX_VISCOUS_BR1 ||
+                  !dgx::stage3_supported(desc->p) || desc->xnodes))
+        return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d boundaries are available for affine hexes, p <= 9, "
+                                      "DGX_FP64 and DGX_VISCOUS_BR1");
+      if (t == DGX_BC_ISOTHERMAL && !(std::isfinite(desc->wall_T) && std::isfinite(desc->wall_Eref)))
+        return fail(DGX_ERR_ARGUMENT, "wall_T and wall_Eref must be finite");
       if (t == DGX_BC_FUNCTION && !desc->bc_function)
         return fail(DGX_ERR_ARGUMENT, "connectivity row " + std::to_string(r) +
                                           ": DGX_BC_FUNCTION boundary without bc_function");
     }
+  if (desc->bulk_forcing && !(desc->nsd == 3 && desc->precision == DGX_FP64 && !desc->xnodes))
+    return fail(DGX_ERR_ARGUMENT, "bulk_forcing is available for affine hexes, DGX_FP64");
   if ((desc->bc_function || desc->source) && desc->eqntype != DGX_SHALLOW)
     return fail(DGX_ERR_ARGUMENT, "boundary and source functions are available for Shallow");
   {
@@ -1210,8 +1240,9 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
       // LSRK14::advance (src/TimeInt.cpp:1989-2013)
       const double *A = dgx::kLsrkA, *B = dgx::kLsrkB;
       const double tn = h->time;
-      if (h->fused) {
+      if (h->fused && !h->bulk) {
         // one fused kernel per stage: residual, register update, next traces
+        // (the bulk forcing needs the whole residual before any update)
         if (step == 0) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));
         for (int st = 0; st < 14; ++st) {
           if (h->visc) RUN(vterm(h, h->U, h->T));

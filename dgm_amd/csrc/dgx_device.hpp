@@ -76,6 +76,7 @@ struct Phys {
   int visc;                      // viscous face gradient: 0 BR1, 1 BR2
   double eta;                    // BR2 penalty
   double bcU[5];                 // conservative state of state boundaries
+  double wallT, wallE;           // isothermal walls: temperature Tw and reference energy Eref
 };
 
 // Shallow water in (phi, uphi, vphi) (src/Shallow.cpp:357-525): Cartesian
@@ -179,8 +180,11 @@ __device__ __noinline__ double eval_program(const int *__restrict__ code, int b,
 // Ghost values of a boundary side, as Domain::enforce_bc fills them
 // (src/Domain.cpp:1346-1350): bc 0 slip wall (Euler3d::WallBC::apply,
 // src/Euler3d.cpp:876-909: the normal momentum removed, the internal energy
-// kept), bc 1 state (StateBC::apply, src/BC.cpp:17-28).  A side whose
-// neighbour index nb is negative is a boundary of type -1 - nb.
+// kept), bc 1 state (StateBC::apply, src/BC.cpp:17-28).  Isothermal walls
+// (bc 3) take the slip wall here: Domain::BC_db holds the Euler3d::WallBC that
+// Euler3d::make_BC makes of every "wall" code (src/Euler3d.cpp:857-860), and
+// the isothermal ghost enters only the viscous terms (ghost_prim3).  A side
+// whose neighbour index nb is negative is a boundary of type -1 - nb.
 __device__ __forceinline__ void ghost3(int bc, const double UL[5], double nx, double ny, double nz,
                                        const double bcU[5], double UB[5]) {
   // no fused multiply-adds: the ghost energy is a difference of O(1/Ma^2)
@@ -198,6 +202,22 @@ __device__ __forceinline__ void ghost3(int bc, const double UL[5], double nx, do
   UB[3] = UL[3] - dot * nz;
   UB[4] = UL[4] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) / UL[0] +
           0.5 * (UB[1] * UB[1] + UB[2] * UB[2] + UB[3] * UB[3]) / UB[0];
+}
+
+// Primitive values (u, v, w, e) of the ghost of a Navier-Stokes boundary side
+// at one face point, as Navier_Stokes3d::enforce_bc and Euler3d::primitive_sides
+// give them (src/Navier_Stokes3d.cpp:457-492, src/Euler3d.cpp:1162-1199):
+// bc 3 isothermal wall (rho-, 0, 0, 0, (Eref - 1/2) rho- Tw), bc 1 state bcU.
+__device__ __forceinline__ void ghost_prim3(int bc, double rho, const Phys &ph, double pq[4]) {
+#pragma clang fp contract(off)
+  if (bc == 3) {
+    const double E = (ph.wallE - 0.5) * rho * ph.wallT;
+    pq[0] = 0.0; pq[1] = 0.0; pq[2] = 0.0; pq[3] = E / rho;
+    return;
+  }
+  const double r = ph.bcU[0], u = ph.bcU[1] / r, v = ph.bcU[2] / r, w = ph.bcU[3] / r;
+  double wk = u * u; wk = v * v + wk; wk = w * w + wk;
+  pq[0] = u; pq[1] = v; pq[2] = w; pq[3] = -0.5 * wk + ph.bcU[4] / r;
 }
 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last

@@ -59,6 +59,8 @@ struct dgx_domain {
   int *d_fcode = nullptr;
   double *d_fconst = nullptr, *d_fxy = nullptr, *d_xn = nullptr;
   double *Bal = nullptr;   // shallow water: the first residual (balance term)
+  bool bulk = false;       // channel bulk forcing (Channel::holdBulkMassFlow) after every residual
+  double *d_J = nullptr;   // [ne] element Jacobians (bulk forcing)
   int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
   dgx::Tab tab;
   dgx::Geo geo;

@@ -180,5 +180,7 @@ hipError_t blas_axpy(double a, const double *x, const double *y, double *z, size
 hipError_t blas_axpby(double a, const double *x, double b, const double *y, double *z, size_t n,
                       hipStream_t s);
 hipError_t blas_scale(double a, double *x, size_t n, hipStream_t s);
+// Channel bulk forcing on a modal residual [nf][ne][nm] of affine elements with Jacobians J[ne].
+hipError_t bulk_hold(double *R, const double *J, int ne, int nm, int nf, hipStream_t s);
 
 }  // namespace dgx

@@ -212,7 +212,7 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       double g = jr * (Fh[v] - FL[v]);
-      if (NS && v > 0) {
+      if (NS && v > 0 && nb >= 0) {   // boundary: ghost flux -S, so the face flux is the own one
         const double *vl = a.TV + ((size_t)e * 6 + s) * 4 * QF2;
         const double *vr = a.TV + ((size_t)nb * 6 + ns) * 4 * QF2;
         g += 0.5 * jr * (vl[(v - 1) * QF2 + q] + vr[(v - 1) * QF2 + qn]);
@@ -400,7 +400,6 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     const int qn = geo.perm[geo.permid[es] * QF2 + q];
     const double jr = geo.fjr[es];
     const double *tl = T + ((size_t)e * 6 + s) * 5 * QF2;
-    const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
     double pl[4], pq[4];
     {
       const double r = tl[q], u = tl[QF2 + q] / r, v = tl[2 * QF2 + q] / r, w = tl[3 * QF2 + q] / r;
@@ -408,7 +407,10 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
       double wk = u * u; wk = v * v + wk; wk = w * w + wk;
       pl[0] = u; pl[1] = v; pl[2] = w; pl[3] = -0.5 * wk + en;
     }
-    {
+    if (nb < 0) {
+      ghost_prim3(-1 - nb, tl[q], ph, pq);
+    } else {
+      const double *tr = T + ((size_t)nb * 6 + ns) * 5 * QF2;
       const double r = tr[qn], u = tr[QF2 + qn] / r, v = tr[2 * QF2 + qn] / r, w = tr[3 * QF2 + qn] / r;
       double en = tr[4 * QF2 + qn] / r;
       double wk = u * u; wk = v * v + wk; wk = w * w + wk;

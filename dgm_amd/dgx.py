@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("DGX_LIBRARY") or os.path.join(HERE, "libdgx.so")
 DGX_EULER, DGX_NAVIER_STOKES, DGX_EULER3D, DGX_NAVIER_STOKES3D, DGX_SHALLOW = 1, 2, 8, 9, 30
 DGX_FLUX_LF, DGX_FLUX_ROE = 0, 1
 DGX_VISCOUS_BR1, DGX_VISCOUS_BR2 = 0, 1
-DGX_BC_WALL, DGX_BC_STATE, DGX_BC_FUNCTION = 0, 1, 2
+DGX_BC_WALL, DGX_BC_STATE, DGX_BC_FUNCTION, DGX_BC_ISOTHERMAL = 0, 1, 2, 3
 DGX_FP64, DGX_MIXED = 0, 1
 DGX_TVD, DGX_RK4, DGX_LSRK14 = 3, 4, 14
 DGX_NODAL, DGX_MODAL = 0, 1
@@ -37,7 +37,8 @@ class DomainDesc(ctypes.Structure):
                 ("vert", _dp), ("ncon", ctypes.c_int), ("con", _ip),
                 ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5),
                 ("xnodes", _dp), ("no_balance", ctypes.c_int),
-                ("bc_function", ctypes.POINTER(Function)), ("source", ctypes.POINTER(Function))]
+                ("bc_function", ctypes.POINTER(Function)), ("source", ctypes.POINTER(Function)),
+                ("wall_T", ctypes.c_double), ("wall_Eref", ctypes.c_double), ("bulk_forcing", ctypes.c_int)]
 
 
 class DgxError(RuntimeError):
@@ -127,7 +128,7 @@ class GPUDomain:
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
                  precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None,
-                 xnodes=None, balance=True, bc_function=None, source=None):
+                 xnodes=None, balance=True, bc_function=None, source=None, wall=None, bulk_forcing=False):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -142,6 +143,9 @@ class GPUDomain:
         self._fkeep = []
         desc.bc_function = make_function(bc_function, self._fkeep)
         desc.source = make_function(source, self._fkeep)
+        if wall is not None:       # isothermal walls: (Tw, Eref)
+            desc.wall_T, desc.wall_Eref = float(wall[0]), float(wall[1])
+        desc.bulk_forcing = int(bool(bulk_forcing))
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))

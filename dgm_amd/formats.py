@@ -398,6 +398,7 @@ def field_function(comps: Dict[str, object]):
 class InitialCondition:
     code: str
     values: List[float] = field(default_factory=list)
+    file: str = ""               # Restart: the restart file to start from
 
 
 def read_ic(path: str) -> InitialCondition:
@@ -423,6 +424,8 @@ def read_ic(path: str) -> InitialCondition:
             return InitialCondition(code, vals)
         if lc == "mmsinitialcondition":                  # no parameters (Shallow::set_ic)
             return InitialCondition(code, [])
+        if lc == "restart":                              # Euler3d::set_ic (src/Euler3d.cpp:957-965)
+            return InitialCondition(code, [], f.readline().split()[0])
     raise ValueError(f"{path}: unsupported IC code {code}")
 
 

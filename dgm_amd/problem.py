@@ -113,20 +113,76 @@ class ReoProblem:
         prm.update(self.table)
         return prm
 
-    def _boundaries(self, mesh, kw):
-        """root.bc sides whose tag root.bct defines as a function (FuncBC),
-        and the first function of root.src as the source (FuncSource)."""
+    def _bc_database(self, eqn):
+        """Boundary types by tag, as each domain's setup_BCdb builds its BC
+        database: the defaults, then root.bct (Euler::setup_BCdb
+        src/Euler.cpp:1199-1215, Euler3d::setup_BCdb src/Euler3d.cpp:824-841,
+        Navier_Stokes3d::setup_BCdb src/Navier_Stokes3d.cpp:398-416,
+        Shallow::setup_BCdb src/Shallow.cpp:745-760).  Returns {tag: (type,
+        parameters)}.  Every code containing "wall" is a slip wall for Euler,
+        Euler3d and Shallow (Euler3d::make_BC, src/Euler3d.cpp:857-860); for
+        Navier_Stokes3d an Isothermal_Wall is isothermal, as its own make_BC
+        reads it (src/Navier_Stokes3d.cpp:438-441)."""
+        ma03 = 20.34126984126985                  # Ma = 0.3 default state energy
+        db: Dict[str, tuple] = {}
+        if eqn == dgx.DGX_EULER:
+            db = {"S": (dgx.DGX_BC_STATE, [1.0, 1.0, 0.0, ma03]), "W": (dgx.DGX_BC_WALL, [])}
+        elif eqn == dgx.DGX_EULER3D:
+            db = {"S": (dgx.DGX_BC_STATE, [1.0, 1.0, 0.0, 0.0, ma03]), "W": (dgx.DGX_BC_WALL, [])}
+        elif eqn == dgx.DGX_NAVIER_STOKES3D:
+            db = {"S": (dgx.DGX_BC_STATE, [1.0, 1.0, 0.0, 0.0, ma03]),
+                  "W": (dgx.DGX_BC_ISOTHERMAL, [1.0, ma03])}
+        elif eqn == dgx.DGX_SHALLOW:
+            db = {"W": (dgx.DGX_BC_WALL, [])}
+        bct = self.root + ".bct"
+        if os.path.exists(bct):
+            for tag, comps in formats.read_functions(bct).items():
+                db[tag] = (dgx.DGX_BC_FUNCTION, comps)
+            for tag, (code, vals) in formats.read_bc_types(bct).items():
+                if code == "state":
+                    db[tag] = (dgx.DGX_BC_STATE, vals)
+                elif eqn == dgx.DGX_NAVIER_STOKES3D and code == "isothermal_wall":
+                    db[tag] = (dgx.DGX_BC_ISOTHERMAL, vals[:2])
+                elif "wall" in code and eqn != dgx.DGX_NAVIER_STOKES3D:
+                    db[tag] = (dgx.DGX_BC_WALL, [])
+                else:
+                    db[tag] = (None, code)
+        return db
+
+    def _boundaries(self, mesh, eqn, kw):
+        """Boundary sides of root.bc by the type their tag has in the domain's
+        BC database (state, slip wall, isothermal wall, function), and the
+        first function of root.src as the source (FuncSource).  A root.bc
+        without rows (``0 nbc``) leaves every side to root.con."""
         import dataclasses
         if os.path.exists(self.root + ".bc"):
             rows = formats.read_bc(self.root + ".bc")
-            funcs = formats.read_functions(self.root + ".bct") if os.path.exists(self.root + ".bct") else {}
-            tags = sorted({t for t, _, _ in rows})
-            if any(t not in funcs for t in tags) or len(tags) > 1:
-                raise NotImplementedError(f"Reo::Problem: boundary tags {tags}: one function boundary "
-                                          "defined in root.bct is read from run directories")
-            kw["bc_function"] = funcs[tags[0]]
-            mesh = dataclasses.replace(mesh, con=formats.boundary_con(mesh.con, rows,
-                                                                      {tags[0]: dgx.DGX_BC_FUNCTION}))
+            if rows:
+                db = self._bc_database(eqn)
+                types = {}
+                states, walls, fns = set(), set(), set()
+                for t in sorted({t for t, _, _ in rows}):
+                    if t not in db or db[t][0] is None:
+                        what = db[t][1] if t in db else "not defined"
+                        raise NotImplementedError(f"Reo::Problem: boundary tag {t} ({what}) has no GPU path")
+                    kind, prm = db[t]
+                    types[t] = kind
+                    if kind == dgx.DGX_BC_STATE:
+                        states.add(tuple(prm))
+                    elif kind == dgx.DGX_BC_ISOTHERMAL:
+                        walls.add(tuple(prm))
+                    elif kind == dgx.DGX_BC_FUNCTION:
+                        fns.add(t)
+                if len(states) > 1 or len(walls) > 1 or len(fns) > 1:
+                    raise NotImplementedError("Reo::Problem: one state, one isothermal wall and one function "
+                                              "boundary type per domain")
+                if states:
+                    kw["bc_state"] = list(next(iter(states)))
+                if walls:
+                    kw["wall"] = next(iter(walls))
+                if fns:
+                    kw["bc_function"] = db[next(iter(fns))][1]
+                mesh = dataclasses.replace(mesh, con=formats.boundary_con(mesh.con, rows, types))
         if os.path.exists(self.root + ".src"):
             kw["source"] = next(iter(formats.read_functions(self.root + ".src").values()))
         return mesh
@@ -138,8 +194,14 @@ class ReoProblem:
                        dgx.DGX_NAVIER_STOKES3D, dgx.DGX_SHALLOW):
             raise NotImplementedError(f"Reo::Problem: equation type {eqn} has no GPU path")
         p = int(prm["p"])
+        if os.path.exists(self.root + ".ord"):
+            # variable order per element (Field::read_order, src/Field.cpp:87-90)
+            raise NotImplementedError("Reo::Problem: root.ord (variable order per element) has no GPU path; "
+                                      "the run would silently take the uniform order p")
         kw = {}
-        mesh = self._boundaries(formats.read_mesh(self.root), kw)
+        mesh = self._boundaries(formats.read_mesh(self.root), eqn, kw)
+        if os.path.exists(self.root + ".chn"):   # Channel::exists (src/Euler3d.cpp:75-77)
+            kw["bulk_forcing"] = True
         self.xnodes = None
         if os.path.exists(self.root + ".crv"):     # curved sides (Field::read_curve)
             if mesh.nsd != 2:
@@ -153,10 +215,22 @@ class ReoProblem:
         dom = dgx.GPUDomain(mesh, p, eqn, gamma=float(prm["gamma"]), vis=vis,
                             Pr=float(prm["Pr"]), flux=int(prm["flux"]),
                             precision=int(prm["precision"]), xnodes=self.xnodes, **kw)
-        dom.set_time(self.t)
         icd = formats.read_ic(self.root + ".ic")
-        x = dom.coords()
-        dom.upload(ic.from_code(icd, x, float(prm["gamma"])))
+        if icd.code.lower() == "restart":
+            # Euler3d::set_ic "Restart" (src/Euler3d.cpp:957-965): the state, step
+            # and time of a restart file (Domain::read_header, src/Domain.cpp:1652-1653)
+            path = os.path.join(os.path.dirname(self.root) or ".", icd.file)
+            rst = formats.read_restart(path, mesh.ne)
+            if rst.data.shape[2] != dom.nm:
+                raise ValueError(f"{path}: {rst.data.shape[2]} modes per element, the domain has {dom.nm}")
+            self.step = int(rst.header.get("Step", 0))
+            self.t = float(rst.header.get("Time", 0.0))
+            dom.set_time(self.t)
+            dom.upload(rst.data, dgx.DGX_MODAL)
+        else:
+            dom.set_time(self.t)
+            x = dom.coords()
+            dom.upload(ic.from_code(icd, x, float(prm["gamma"])))
         self.domain = dom
         return dom
 
@@ -192,7 +266,7 @@ class ReoProblem:
             el = (time.perf_counter() - t0) / n
             done += n
             self.step += n
-            self.t += n * dt
+            self.t = dom.time
             if done % ntout == 0 or done == Nt:
                 print(f"{self.step:8d}{self.t:12.4e}{dt:12.4e}{0.0:12.4e}{el:12.4e}", file=self.out)
             if Ntout == 0 or self.step % Ntout == 0:

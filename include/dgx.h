@@ -80,7 +80,17 @@ typedef struct dgx_domain *dgx_handle;
  * (src/ShallowBC.hpp:45-62), imposed state as StateBC (src/BC.cpp:17-28);
  * the ghost values meet the interior through the same Riemann flux as an
  * interior face. */
-enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1, DGX_BC_FUNCTION = 2 };
+enum dgx_bc { DGX_BC_WALL = 0, DGX_BC_STATE = 1, DGX_BC_FUNCTION = 2, DGX_BC_ISOTHERMAL = 3 };
+/* DGX_BC_ISOTHERMAL (Navier_Stokes3d): Isothermal_WallBC
+ * (src/Navier_Stokes3d.cpp:472-512).  As in the reference, the inviscid flux
+ * sees the slip Euler3d::WallBC that Euler3d::make_BC makes of every "wall"
+ * code (src/Euler3d.cpp:857-860); the viscous terms see the ghost (rho-, 0, 0,
+ * 0, (wall_Eref - 1/2) rho- wall_T) and the ghost viscous flux -S, so the face
+ * viscous flux is the interior one.  Navier_Stokes3d state boundaries take
+ * bc_state in both, with the same -S viscous ghost (StateBC::apply_flux,
+ * src/BC.cpp:31-40).  Navier_Stokes3d has no slip-wall viscous flux
+ * (Euler3d::WallBC::apply_flux raises, src/Euler3d.cpp:911-913), so
+ * DGX_BC_WALL is rejected there. */
 
 /* A function of (x, y, z, t) with ncomp components, as the reference's
  * FuncBC and FuncSource read them from root.bct / root.src and evaluate them
@@ -142,6 +152,14 @@ typedef struct {
                                        With either function, quads take the curved-element
                                        path (nodal metrics from xnodes, or from the
                                        vertices' bilinear map when xnodes is NULL) */
+  double wall_T;    /* Navier_Stokes3d DGX_BC_ISOTHERMAL: wall temperature Tw and */
+  double wall_Eref; /* reference energy Eref (Isothermal_Wall tag { Tw Eref } in root.bct) */
+  int bulk_forcing; /* 1: hold the bulk density, x-momentum and energy, as a run
+                       directory with root.chn does (Channel::holdBulkMassFlow,
+                       src/Channel.hpp:117-163, called at src/Navier_Stokes3d.cpp:160):
+                       after the projection the domain means of the rho, rho u and
+                       rho E residuals are subtracted from them.  Affine hexes,
+                       DGX_FP64 */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

@@ -33,8 +33,8 @@ def case_setup(root, prm, mesh):
     import dataclasses
     base = os.path.join(GOLDEN, root)
     bc_fn = src_fn = None
-    if os.path.exists(base + ".bc"):
-        rows = formats.read_bc(base + ".bc")
+    rows = formats.read_bc(base + ".bc") if os.path.exists(base + ".bc") else []
+    if rows:                                      # "0 nbc" files hold no boundary sides
         funcs = formats.read_functions(base + ".bct") if os.path.exists(base + ".bct") else {}
         tags = sorted({t for t, _, _ in rows})
         unknown = [t for t in tags if t not in funcs]

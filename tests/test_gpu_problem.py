@@ -2,8 +2,9 @@
 
 ``ReoProblem(args, table, comm).solve()`` runs the reference's own regression
 run directories (runs/vquad, runs/vhex, runs/cvquad with its arc sides,
-runs/shallow_mms with its function boundaries and source: mesh, connectivity,
-IC, parameters)
+runs/shallow_mms with its function boundaries and source, runs/chan with its
+isothermal walls, bulk forcing and restart initial condition: mesh,
+connectivity, IC, parameters)
 on libdgx.so and writes ``root.rst``; that file must match the reference's
 golden ``.ref`` under the reference's own diff rule and tolerances
 (runs/vquad.tst:23-24, runs/vhex.tst:23-24; src/Domain.cpp:1000-1071).
@@ -27,7 +28,7 @@ REPS_M = np.sqrt(np.finfo(float).eps)
 
 
 def _run(tmp_path, root, extra=()):
-    for ext in (".inp", ".msh", ".con", ".ic", ".crv", ".bc", ".bct", ".src"):
+    for ext in (".inp", ".msh", ".con", ".ic", ".crv", ".bc", ".bct", ".src", ".chn", ".ini"):
         if os.path.exists(os.path.join(GOLDEN, root + ext)):
             shutil.copy(os.path.join(GOLDEN, root + ext), tmp_path / (root + ext))
     base = str(tmp_path / root)
@@ -42,7 +43,8 @@ def _run(tmp_path, root, extra=()):
 
 
 @pytest.mark.parametrize("root,aeps,reps", [("vquad", 5e-12, 5e-12), ("vhex", 5e-13, 5e-10),
-                                            ("cvquad", 5e-13, 5e-13), ("shallow_mms", 1e-9, 1e-10)])
+                                            ("cvquad", 5e-13, 5e-13), ("shallow_mms", 1e-9, 1e-10),
+                                            ("chan", 5e-10, 1e-8)])
 def test_golden_run_directory(tmp_path, root, aeps, reps):
     print(f"[problem] {root}: full run through ReoProblem.solve()", flush=True)
     base, prob = _run(tmp_path, root)

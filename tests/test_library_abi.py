@@ -71,7 +71,12 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=8, viscous=1), b"DGX_VISCOUS_BR2"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=-1.0), b"br2_eta"),
     (dict(p=3, eqn=9, viscous=1, br2_eta=float("nan")), b"br2_eta"),
-    (dict(p=3, eqn=9, boundary=0), b"boundary sides are available for Euler, Euler3d and Shallow"),
+    (dict(p=3, eqn=9, boundary=0), b"no slip-wall viscous flux"),
+    (dict(p=3, eqn=8, boundary=3), b"DGX_BC_ISOTHERMAL is a Navier_Stokes3d boundary"),
+    (dict(p=3, eqn=9, boundary=3, top=3, precision=1), b"Navier_Stokes3d boundaries are available"),
+    (dict(p=3, eqn=9, boundary=3, top=3, viscous=1), b"Navier_Stokes3d boundaries are available"),
+    (dict(p=10, eqn=9, boundary=3, top=3), b"Navier_Stokes3d boundaries are available"),
+    (dict(p=3, eqn=9, bulk=1, precision=1), b"bulk_forcing is available"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
     (dict(p=3, eqn=9, curved=True), b"curved elements"),
     (dict(p=7, eqn=8, curved=True), b"curved elements"),
@@ -80,10 +85,11 @@ def _desc(m, p, eqn, **kw):
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
     if "boundary" in case:
-        m = meshgen.channel_hexes(2, 2, 2, bc_bottom=case["boundary"], bc_top=0)
+        m = meshgen.channel_hexes(2, 2, 2, bc_bottom=case["boundary"], bc_top=case.get("top", 0))
     d, keep = _desc(m, case.get("p", 3), case.get("eqn", 8), flux=case.get("flux", 0),
                     precision=case.get("precision", 0), viscous=case.get("viscous", 0),
                     br2_eta=case.get("br2_eta", 0.0))
+    d.bulk_forcing = case.get("bulk", 0)
     if case.get("curved"):
         xn = np.ascontiguousarray(meshgen.hex_nodes(m, case.get("p", 3)))
         keep = keep + (xn,)
