@@ -136,3 +136,56 @@ class AcousticProblem:
         f, a, s = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int()
         _check(self.lib, self.lib.dgx_acoustic_stats(self.h, ctypes.byref(f), ctypes.byref(a), ctypes.byref(s)))
         return {"forward_steps": int(f.value), "adjoint_steps": int(a.value), "snapshots": int(s.value)}
+
+
+class ReoOptProblem:
+    """The optimisation seam of pydgm (``ReoOptProblem.computeState``,
+    ``computeAdjoint``, ``getGradient``, ``getControl``; python/dgm.pyx:322-355,
+    Reo::OptProblem) over the acoustic inverse-source problem: the control is
+    the source field theta, the state a forward solve, the objective
+    J = 1/2 dt sum (p' - d)^2 against `data`, the gradient the checkpointed
+    discrete adjoint's.  computeState returns J for the current control;
+    computeAdjoint evaluates dJ/dtheta, which getGradient then returns."""
+
+    def __init__(self, problem: AcousticProblem, data, theta0=None):
+        self.problem = problem
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        self.control = np.zeros(problem.ne) if theta0 is None else np.array(theta0, dtype=np.float64)
+        self.gradient = np.zeros(problem.ne)
+        self.J = None
+
+    def getControl(self) -> np.ndarray:
+        return self.control
+
+    def computeState(self) -> float:
+        self.J = self.problem.objective(self.control, self.data)
+        return self.J
+
+    def computeAdjoint(self) -> None:
+        self.J, self.gradient = self.problem.gradient(self.control, self.data)
+
+    def getGradient(self) -> np.ndarray:
+        return self.gradient
+
+    def getObjective(self) -> float:
+        return self.J
+
+    def solve(self, iterations: int = 10, step: float = 1.0) -> float:
+        """Steepest descent with a backtracking line search (the role of
+        OptProblem::solve); returns the final objective."""
+        J = self.computeState()
+        for _ in range(iterations):
+            self.computeAdjoint()
+            g = self.gradient
+            a = step
+            while a > 1e-12:
+                trial = self.control - a * g
+                Jt = self.problem.objective(trial, self.data)
+                if Jt < J:
+                    self.control, J = trial, Jt
+                    break
+                a *= 0.5
+            else:
+                break
+        self.J = J
+        return J

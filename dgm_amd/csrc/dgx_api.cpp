@@ -193,7 +193,8 @@ int upload_all_tables() {
       (e = dgx::upload_tables_s3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_c2d(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_c2d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_cfl(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -346,6 +347,39 @@ int residual_core(dgx_domain *d, const double *U, double *R) {
   }
   if ((rc = launch(d, dgx::OP_RHS, U, R))) return rc;
   ++d->stages;
+  return DGX_OK;
+}
+
+// Largest idt of Euler::set_time_step over the domain's nodes (dgx_cfl.hip),
+// with `part` as the kernels' scratch (ne + 1 doubles); waits for the stream.
+int cfl_max(dgx_domain *d, double *part, double *m) {
+  if (!d->d_cgeo)
+    return fail(DGX_ERR_ARGUMENT, "the CFL time step is available for 2D Euler, Navier_Stokes and Shallow domains");
+  dgx::CflArgs a;
+  a.U = d->U; a.cgeo = d->d_cgeo; a.part = part; a.out = part + d->ne;
+  for (int i = 0; i < d->Q; ++i) {   // GLL spacings (src/Element.cpp:513-527)
+    const std::vector<double> &z = d->hb.z;
+    a.dz[i] = i == 0 ? z[1] - z[0] : (i == d->Q - 1 ? z[i] - z[i - 1] : 0.5 * (z[i + 1] - z[i - 1]));
+  }
+  a.gamma = d->gamma; a.ne = d->ne; a.p = d->p;
+  a.shallow = d->eqn == DGX_SHALLOW; a.curved = d->curved; a.stream = d->stream;
+  hipError_t e = dgx::launch_cfl2(a);
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("CFL kernels: ") + hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(m, a.out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (!(*m == *m)) return fail(DGX_ERR_NONREALIZABLE, "non-realizable state in the CFL evaluation");
+  return DGX_OK;
+}
+
+// The time step rule of Euler::set_time_step (src/Euler.cpp:861-874) from the
+// largest idt: CFL of dt, new dt under CFLmax (and dtmax), CFL of the new dt.
+int cfl_finish(double dt, double CFLmax, double dtmax, double m, double *cfl, double *dt_new) {
+  const double c = dt * m;
+  double dn = dt;
+  if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / c);
+  else if (CFLmax != 0) dn = CFLmax * dt / c;
+  *dt_new = dn;
+  *cfl = c * dn / dt;
   return DGX_OK;
 }
 
@@ -951,6 +985,30 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   }
   if ((desc->bc_function || desc->source) && (rc = setup_functions(d, desc))) return rc;
   d->fused = (nsd == 3 && dgx::stage3_supported(p) && !d->curved);
+  if (nsd == 2) {
+    // CFL geometry: the coordinate derivatives x_r, x_s, y_r, y_s and J of
+    // Element::compute_metrics_2d_tensor_prod (src/Element.cpp:485-555), from
+    // the collocation derivative of the nodal coordinates as local_gradient
+    // takes them; constant on parallelograms, per node on curved quads
+    const int Q2 = d->nq, S = d->curved ? Q2 : 1;
+    std::vector<double> cg((size_t)ne * 5 * S);
+    const std::vector<double> &D = b.D;
+    for (int e = 0; e < ne; ++e) {
+      const double *X = &d->coords[(size_t)e * Q2], *Y = &d->coords[((size_t)ne + e) * Q2];
+      for (int n = 0; n < S; ++n) {
+        const int i = n / Q, j = n % Q;
+        double xr = 0, xs = 0, yr = 0, ys = 0;
+        for (int m = 0; m < Q; ++m) {
+          xr += D[i * Q + m] * X[m * Q + j]; xs += D[j * Q + m] * X[i * Q + m];
+          yr += D[i * Q + m] * Y[m * Q + j]; ys += D[j * Q + m] * Y[i * Q + m];
+        }
+        double *o = &cg[(size_t)e * 5 * S + n];
+        o[0] = xr; o[S] = xs; o[2 * S] = yr; o[3 * S] = ys; o[4 * S] = xr * ys - xs * yr;
+      }
+    }
+    if ((rc = alloc(&d->d_cgeo, cg.size()))) return rc;
+    HIPCHK(hipMemcpy(d->d_cgeo, cg.data(), cg.size() * 8, hipMemcpyHostToDevice));
+  }
   if (desc->bulk_forcing) {
     d->bulk = true;
     if ((rc = alloc(&d->d_J, ne))) return rc;
@@ -979,7 +1037,7 @@ void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
                   d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
-                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J};
+                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -1023,6 +1081,8 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
     b += ne * (8 * qs + 2 * nq) * 8;
   }
   if (shallow && !desc->no_balance) b += nf * ne * nm * 8;    // balance term
+  if (nsd == 2) b += ne * 5 * ((desc->xnodes || fns || !all_quads_affine(desc)) ? nq : 1) * 8;   // CFL geometry
+  if (desc->bulk_forcing) b += ne * 8;
   *bytes = b;
   return DGX_OK;
 }
@@ -1141,6 +1201,17 @@ int dgx_sizes(dgx_handle h, int *qtot, int *nmodes, int *nfields, int *ne) {
   return DGX_OK;
 }
 
+int dgx_vfield_sizes(dgx_handle h, int *size, long long *qtot, long long *nmodes, long long *sdof, int *nsd) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  const long long nf = h->nf, ne = h->ne;
+  if (size) *size = h->nf;
+  if (qtot) *qtot = nf * ne * h->nq;
+  if (nmodes) *nmodes = nf * ne * h->nm;
+  if (sdof) *sdof = nf * ne * h->ns * h->qs;
+  if (nsd) *nsd = h->nsd;
+  return DGX_OK;
+}
+
 int dgx_coordinates(dgx_handle h, double *xyz) {
   if (!h || !xyz) return fail(DGX_ERR_ARGUMENT, "null argument");
   std::memcpy(xyz, h->coords.data(), h->coords.size() * sizeof(double));
@@ -1195,82 +1266,140 @@ int dgx_residual_download(dgx_handle h, double *F, int layout, int on_device) {
   return download(h, h->R0, F, layout, on_device);
 }
 
-int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
-  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+}  // extern "C"
+
+namespace {
+
+int nstages(int inttype) {
+  return inttype == DGX_RK4 ? 4 : (inttype == DGX_TVD ? 3 : (inttype == DGX_LSRK14 ? 14 : 0));
+}
+
+// Stage `st` of one step of integrator `inttype`, as the reference's
+// TimeInt::advance runs it (RungeKutta src/TimeInt.cpp:267-302, TVD :824-856,
+// LSRK14 :1989-2013); the step starts at the domain time of stage 0.
+// `traces_ready`: the fused LSRK14 path may take the face traces the previous
+// stage wrote instead of evaluating them.
+int run_stage(dgx_domain *h, int inttype, int st, double dt, bool traces_ready) {
   const size_t n = h->state_len();
   hipStream_t s = h->stream;
   int rc;
 #define RUN(x) do { if ((rc = (x))) return rc; } while (0)
 #define BL(x) do { hipError_t _e = timed(h, 6, [&] { return (x); }); if (_e != hipSuccess) return fail(DGX_ERR_DEVICE, hipGetErrorString(_e)); } while (0)
-  for (int step = 0; step < nsteps; ++step) {
-    if (inttype == DGX_RK4) {
-      // RungeKutta::advance (src/TimeInt.cpp:267-302)
-      static const double a[3] = {0.5, 0.5, 1.0};
-      static const double b[4] = {0.16666666666666667, 0.33333333333333333, 0.33333333333333333,
-                                  0.16666666666666667};
+  if (st == 0) h->t_step = h->time;
+  const double tn = h->t_step;
+  if (inttype == DGX_RK4) {
+    static const double a[3] = {0.5, 0.5, 1.0};
+    static const double b[4] = {0.16666666666666667, 0.33333333333333333, 0.33333333333333333,
+                                0.16666666666666667};
+    if (st == 0) {
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpy(-a[0] * dt, h->R0, h->U, h->Ut, n, s));
       BL(dgx::blas_scale(-b[0], h->R0, n, s));
-      h->time += 0.5 * dt;
+      h->time = tn + 0.5 * dt;
+    } else if (st < 3) {
       RUN(residual(h, h->Ut, h->R1));
-      BL(dgx::blas_axpy(-a[1] * dt, h->R1, h->U, h->Ut, n, s));
-      BL(dgx::blas_axpy(-b[1], h->R1, h->R0, h->R0, n, s));
-      RUN(residual(h, h->Ut, h->R1));
-      BL(dgx::blas_axpy(-a[2] * dt, h->R1, h->U, h->Ut, n, s));
-      BL(dgx::blas_axpy(-b[2], h->R1, h->R0, h->R0, n, s));
-      h->time += 0.5 * dt;
+      BL(dgx::blas_axpy(-a[st] * dt, h->R1, h->U, h->Ut, n, s));
+      BL(dgx::blas_axpy(-b[st], h->R1, h->R0, h->R0, n, s));
+      if (st == 2) h->time = tn + dt;
+    } else {
       RUN(residual(h, h->Ut, h->R1));
       BL(dgx::blas_axpy(-b[3], h->R1, h->R0, h->R0, n, s));
       BL(dgx::blas_axpy(dt, h->R0, h->U, h->U, n, s));
-    } else if (inttype == DGX_TVD) {
-      // TVD::advance (src/TimeInt.cpp:824-856), stage times c = 0, 1, 1/2 (src/TimeInt.cpp:740)
-      const double tn = h->time;
+    }
+  } else if (inttype == DGX_TVD) {
+    // stage times c = 0, 1, 1/2 (src/TimeInt.cpp:740)
+    if (st == 0) {
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->U, h->Ut, n, s));
       h->time = tn + dt;
+    } else if (st == 1) {
       RUN(residual(h, h->Ut, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
       BL(dgx::blas_axpby(3.0 / 4.0, h->U, 1.0 / 4.0, h->R0, h->Ut, n, s));
       h->time = tn + 0.5 * dt;
+    } else {
       RUN(residual(h, h->Ut, h->R0));
       BL(dgx::blas_axpy(-dt, h->R0, h->Ut, h->R0, n, s));
       BL(dgx::blas_axpby(2.0 / 3.0, h->R0, 1.0 / 3.0, h->U, h->U, n, s));
       h->time = tn + dt;
-    } else if (inttype == DGX_LSRK14) {
-      // LSRK14::advance (src/TimeInt.cpp:1989-2013)
-      const double *A = dgx::kLsrkA, *B = dgx::kLsrkB;
-      const double tn = h->time;
-      if (h->fused && !h->bulk) {
-        // one fused kernel per stage: residual, register update, next traces
-        // (the bulk forcing needs the whole residual before any update)
-        if (step == 0) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));
-        for (int st = 0; st < 14; ++st) {
-          if (h->visc) RUN(vterm(h, h->U, h->T));
-          dgx::StageLaunch sa = stage_args(h);
-          sa.op = 1; sa.update = true; sa.U = h->U; sa.Ut = h->Ut; sa.T = h->T; sa.Tn = h->T2;
-          sa.V = h->V; sa.TV = h->TV; sa.Vf = h->Vf; sa.TVf = h->TVf;
-          sa.ca = A[st]; sa.cb = B[st]; sa.dt = dt;
-          RUN(launch_stage(h, sa));
-          std::swap(h->T, h->T2);
-          ++h->stages;
-        }
-        h->time = tn + dt;
-        continue;
-      }
-      for (int st = 0; st < 14; ++st) {
-        h->time = tn + dgx::kLsrkC[st] * dt;
-        RUN(residual(h, h->U, h->R0));
-        BL(dgx::blas_axpby(-dt, h->R0, A[st], h->Ut, h->Ut, n, s));
-        BL(dgx::blas_axpy(B[st], h->Ut, h->U, h->U, n, s));
-      }
-      h->time = tn + dt;
-    } else {
-      return fail(DGX_ERR_ARGUMENT, "unsupported inttype " + std::to_string(inttype));
     }
+  } else {
+    const double A = dgx::kLsrkA[st], B = dgx::kLsrkB[st];
+    h->time = tn + dgx::kLsrkC[st] * dt;
+    if (h->fused && !h->bulk) {
+      // one fused kernel per stage: residual, register update, next traces
+      // (the bulk forcing needs the whole residual before any update)
+      if (!traces_ready) RUN(launch(h, dgx::OP_TRACES, h->U, nullptr));
+      if (h->visc) RUN(vterm(h, h->U, h->T));
+      dgx::StageLaunch sa = stage_args(h);
+      sa.op = 1; sa.update = true; sa.U = h->U; sa.Ut = h->Ut; sa.T = h->T; sa.Tn = h->T2;
+      sa.V = h->V; sa.TV = h->TV; sa.Vf = h->Vf; sa.TVf = h->TVf;
+      sa.ca = A; sa.cb = B; sa.dt = dt;
+      RUN(launch_stage(h, sa));
+      std::swap(h->T, h->T2);
+      ++h->stages;
+    } else {
+      RUN(residual(h, h->U, h->R0));
+      BL(dgx::blas_axpby(-dt, h->R0, A, h->Ut, h->Ut, n, s));
+      BL(dgx::blas_axpy(B, h->Ut, h->U, h->U, n, s));
+    }
+    if (st == 13) h->time = tn + dt;
   }
 #undef RUN
 #undef BL
   return DGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  const int ns = nstages(inttype);
+  if (!ns) return fail(DGX_ERR_ARGUMENT, "unsupported inttype " + std::to_string(inttype));
+  if (h->next_stage != 0)
+    return fail(DGX_ERR_ARGUMENT, "a step begun with dgx_stage is unfinished (next stage " +
+                                      std::to_string(h->next_stage) + ")");
+  for (int step = 0; step < nsteps; ++step)
+    for (int st = 0; st < ns; ++st) {
+      int rc = run_stage(h, inttype, st, dt, st > 0 || step > 0);
+      if (rc) return rc;
+    }
+  return DGX_OK;
+}
+
+int dgx_stage(dgx_handle h, int inttype, int stage, double dt) {
+  if (!h) return fail(DGX_ERR_ARGUMENT, "null handle");
+  const int ns = nstages(inttype);
+  if (!ns) return fail(DGX_ERR_ARGUMENT, "unsupported inttype " + std::to_string(inttype));
+  if (stage != h->next_stage || (stage > 0 && inttype != h->stage_int))
+    return fail(DGX_ERR_ARGUMENT, "stages run in order 0 .. " + std::to_string(ns - 1) +
+                                      " of one integrator; expected stage " + std::to_string(h->next_stage));
+  int rc = run_stage(h, inttype, stage, dt, stage > 0);
+  if (rc) return rc;
+  h->stage_int = inttype;
+  h->next_stage = stage + 1 == ns ? 0 : stage + 1;
+  return DGX_OK;
+}
+
+int dgx_set_time_step(dgx_handle h, double dt, double CFLmax, double dtmax, double *cfl, double *dt_new) {
+  if (!h || !cfl || !dt_new) return fail(DGX_ERR_ARGUMENT, "null argument");
+  if (!(dt > 0) || !std::isfinite(dt) || !(CFLmax >= 0) || !(dtmax >= 0))
+    return fail(DGX_ERR_ARGUMENT, "dt must be positive and CFLmax, dtmax >= 0");
+  if (h->nsd == 3) {   // Euler3d::set_time_step (src/Euler3d.cpp:462-465) returns dt and keeps it
+    *cfl = dt;
+    *dt_new = dt;
+    return DGX_OK;
+  }
+  if (!h->d_part) {
+    int rc = alloc(&h->d_part, (size_t)h->ne + 1);
+    if (rc) return rc;
+  }
+  double m = 0.0;
+  int rc = cfl_max(h, h->d_part, &m);
+  if (rc) return rc;
+  return cfl_finish(dt, CFLmax, dtmax, m, cfl, dt_new);
 }
 
 int dgx_synchronize(dgx_handle h) {

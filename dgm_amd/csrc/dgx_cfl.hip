@@ -1,0 +1,132 @@
+// CFL time step of the 2D Euler, Navier-Stokes and shallow-water domains
+// (Euler::set_time_step, src/Euler.cpp:804-877; Shallow::set_time_step,
+// src/Shallow.cpp:573-637) with the stability metrics of tensor-product quads
+// (Element::compute_metrics_2d_tensor_prod, src/Element.cpp:485-555).
+//
+// At every GLL node of every element, from the nodal state,
+//   idt = (|u dxr + v dyr| + |u dxs + v dys| + c acoustic) ivol,
+// with dxr = x_r dr_i, dyr = y_r dr_i, dxs = x_s ds_j, dys = y_s ds_j (dr_i,
+// ds_j the GLL node spacings: one-sided at the ends, centred inside),
+// ivol = 1 / (dr_i ds_j J), acoustic = sqrt((dxr + dyr)^2 + (dxs + dys)^2),
+// and c the sound speed sqrt(gamma (gamma-1) (E - 1/2 (rho u u + rho v v)) / rho)
+// (Euler) or sqrt(phi) (shallow water).  The CFL number of dt is
+// dt * max idt over the domain.
+//
+// k_cfl2 evaluates the nodal state from the modes in LDS (one wavefront per
+// element) and writes each element's largest idt to a scratch array of ne
+// values; k_max1 reduces that array in one workgroup, in a fixed order.
+#include "dgx_kernels.hpp"
+
+namespace dgx {
+
+__device__ constexpr double kNaN = __builtin_nan("");
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_cfl2(CflArgs a) {
+  constexpr int L = P + 1, Q = P + 2, L2 = L * L, Q2 = Q * Q, NT = 64;
+  constexpr int NF = 4;   // at most 4 fields are read (shallow water: 3)
+  __shared__ double A[NF * L2], X[NF * Q * L], O[NF * Q2], red[NT];
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne, nf = a.shallow ? 3 : 4;
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    A[i] = v < nf ? a.U[((size_t)v * ne + e) * L2 + (i - v * L2)] : 0.0;
+  }
+  __syncthreads();
+  // modal -> GLL nodes, all fields at once: [NF][L][L] -> [NF][Q][L] -> [NF][Q][Q]
+  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), A, X, tid);
+  __syncthreads();
+  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), X, O, tid);
+  __syncthreads();
+  const double g = a.gamma;
+  double m = 0.0;
+  bool bad = false;   // a NaN idt (non-realizable state) makes the element's value NaN
+  for (int n = tid; n < Q2; n += NT) {
+    const int i = n / Q, j = n - i * Q;
+    const int S = a.curved ? Q2 : 1, o = a.curved ? n : 0;
+    const double *cg = a.cgeo + (size_t)e * 5 * S + o;
+    const double xr = cg[0], xs = cg[S], yr = cg[2 * S], ys = cg[3 * S], J = cg[4 * S];
+    const double dr = a.dz[i], ds = a.dz[j];
+    const double dxr = xr * dr, dxs = xs * ds, dyr = yr * dr, dys = ys * ds;
+    const double ivol = 1.0 / (dr * ds * J);
+    const double acoustic = sqrt((dxr + dyr) * (dxr + dyr) + (dxs + dys) * (dxs + dys));
+    const double r = O[n], ru = O[Q2 + n], rv = O[2 * Q2 + n];
+    const double u = ru / r, v = rv / r;
+    double c;
+    if (a.shallow) {
+      c = sqrt(r);
+    } else {
+      c = u * ru;
+      c = fma(v, rv, c);
+      c = -0.5 * c + O[3 * Q2 + n];
+      c = (g * (g - 1.0)) * c;
+      c = sqrt(c / r);
+    }
+    double idt = fabs(fma(v, dyr, u * dxr));
+    idt += fabs(fma(v, dys, u * dxs));
+    idt = fma(c, acoustic, idt);
+    idt = ivol * idt;
+    bad |= !(idt == idt);
+    m = fmax(m, idt);
+  }
+  red[tid] = bad ? kNaN : m;
+  __syncthreads();
+  if (tid == 0) {
+    double b = 0.0;
+    bool nan = false;
+    for (int k = 0; k < NT; ++k) { nan |= !(red[k] == red[k]); b = fmax(b, red[k]); }
+    a.part[e] = nan ? kNaN : b;
+  }
+}
+
+// out[0] = max of part[0 .. n-1] (NaN if any is NaN); one workgroup.
+constexpr int kMaxThreads = 1024;
+__global__ void __launch_bounds__(kMaxThreads) k_max1(const double *__restrict__ part, int n, double *out) {
+  __shared__ double red[kMaxThreads];
+  __shared__ int bad[kMaxThreads];
+  const int tid = threadIdx.x;
+  double m = 0.0;
+  int nan = 0;
+  for (int i = tid; i < n; i += kMaxThreads) {
+    const double x = part[i];
+    nan |= !(x == x);
+    m = fmax(m, x);
+  }
+  red[tid] = m;
+  bad[tid] = nan;
+  __syncthreads();
+  for (int w = kMaxThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[tid] = fmax(red[tid], red[tid + w]);
+      bad[tid] |= bad[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) out[0] = bad[0] ? kNaN : red[0];
+}
+
+template <int P>
+static hipError_t launch_cfl2_p(const CflArgs &a) {
+  hipLaunchKernelGGL(k_cfl2<P>, dim3(a.ne), dim3(64), 0, a.stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_max1, dim3(1), dim3(kMaxThreads), 0, a.stream, a.part, a.ne, a.out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl2(const CflArgs &a) {
+  if (a.ne <= 0 || !a.part || !a.out || !a.U || !a.cgeo) return hipErrorInvalidValue;
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch_cfl2_p<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9) DGX_CASE(10)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t upload_tables_cfl(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+}
+
+}  // namespace dgx

@@ -55,12 +55,16 @@ struct dgx_domain {
   double *d_gm = nullptr, *d_fgeo = nullptr, *d_jav = nullptr;
   int *d_iters = nullptr;
   double time = 0.0;       // solution time: boundary and source functions are evaluated at it
+  double t_step = 0.0;     // time at the start of the current step
+  int next_stage = 0, stage_int = 0;   // dgx_stage: the next stage and its integrator
   dgx::FnArgs fn;          // boundary and source functions (device programs and coordinates)
   int *d_fcode = nullptr;
   double *d_fconst = nullptr, *d_fxy = nullptr, *d_xn = nullptr;
   double *Bal = nullptr;   // shallow water: the first residual (balance term)
   bool bulk = false;       // channel bulk forcing (Channel::holdBulkMassFlow) after every residual
   double *d_J = nullptr;   // [ne] element Jacobians (bulk forcing)
+  double *d_cgeo = nullptr;  // 2D: CFL geometry (x_r, x_s, y_r, y_s, J) per element or, curved, per node
+  double *d_part = nullptr;  // 2D: CFL scratch, ne per-element maxima and the domain maximum
   int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
   dgx::Tab tab;
   dgx::Geo geo;

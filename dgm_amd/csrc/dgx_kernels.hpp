@@ -172,6 +172,24 @@ hipError_t launch_observe_adjoint(const double *U, double *Ubar, const int *rec_
                                   const double *cobs, int nrec, int ne, int L2, const double *d, double dt,
                                   double *J, hipStream_t s);
 
+// CFL metric of a 2D domain (dgx_cfl.hip): part[e] = max over element e's GLL
+// nodes of idt (Euler::set_time_step), out[0] = max over elements.  cgeo holds
+// (x_r, x_s, y_r, y_s, J) per element [ne][5] (affine) or per node [ne][5][Q^2]
+// (curved); dz[i] the GLL node spacings.
+struct CflArgs {
+  const double *U = nullptr;     // modal state [nf][ne][L^2]
+  const double *cgeo = nullptr;
+  double *part = nullptr;        // [ne] scratch
+  double *out = nullptr;         // [1]
+  double dz[kMaxOrder + 2] = {};
+  double gamma = 1.4;
+  int ne = 0, p = 0;
+  bool shallow = false, curved = false;
+  hipStream_t stream = nullptr;
+};
+hipError_t launch_cfl2(const CflArgs &a);
+hipError_t upload_tables_cfl(const double *h);
+
 // Error message of the calling thread's last failure (dgx_api.cpp).
 int set_error(int code, const std::string &msg);
 

@@ -64,6 +64,8 @@ def load():
         "dgx_domain_destroy": ([h], ctypes.c_int),
         "dgx_sizes": ([h, _ip, _ip, _ip, _ip], ctypes.c_int),
         "dgx_coordinates": ([h, _dp], ctypes.c_int),
+        "dgx_vfield_sizes": ([h, _ip, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                              ctypes.POINTER(ctypes.c_longlong), _ip], ctypes.c_int),
         "dgx_state_upload": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_state_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_state_device_ptr": ([h, ctypes.POINTER(_dp)], ctypes.c_int),
@@ -72,7 +74,9 @@ def load():
         "dgx_get_time": ([h, _dp], ctypes.c_int),
         "dgx_residual_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_advance": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "dgx_stage": ([h, ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "dgx_synchronize": ([h], ctypes.c_int),
+        "dgx_set_time_step": ([h, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
         "dgx_norms": ([h, _dp], ctypes.c_int),
         "dgx_stage_count": ([h], ctypes.c_longlong),
         "dgx_profile": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip, ctypes.c_int],
@@ -172,6 +176,14 @@ class GPUDomain:
         except Exception:
             pass
 
+    def vfield_sizes(self):
+        """dict of the state's vField totals: size, qtot, nmodes, sdof, nsd."""
+        sz, nsd = ctypes.c_int(), ctypes.c_int()
+        q, m, s = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(self.lib.dgx_vfield_sizes(self.h, ctypes.byref(sz), ctypes.byref(q), ctypes.byref(m), ctypes.byref(s),
+                                         ctypes.byref(nsd)))
+        return {"size": sz.value, "qtot": q.value, "nmodes": m.value, "sdof": s.value, "nsd": nsd.value}
+
     def coords(self):
         x = np.zeros((self.nsd, self.ne, self.qtot))
         _check(self.lib.dgx_coordinates(self.h, _ptr(x)))
@@ -209,6 +221,17 @@ class GPUDomain:
 
     def advance(self, inttype, dt, nsteps):
         _check(self.lib.dgx_advance(self.h, int(inttype), float(dt), int(nsteps)))
+
+    def set_time_step(self, dt, CFLmax=0.0, dtmax=0.0):
+        """(CFL number, new dt) of the current state (Domain::set_time_step)."""
+        c, d = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.dgx_set_time_step(self.h, float(dt), float(CFLmax), float(dtmax), ctypes.byref(c),
+                                          ctypes.byref(d)))
+        return c.value, d.value
+
+    def stage(self, inttype, stage, dt):
+        """Stage `stage` of one step (dgx_stage)."""
+        _check(self.lib.dgx_stage(self.h, int(inttype), int(stage), float(dt)))
 
     def synchronize(self):
         _check(self.lib.dgx_synchronize(self.h))

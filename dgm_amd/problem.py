@@ -36,12 +36,12 @@ DGM_SUCCESS = 0
 
 _DEFAULTS: Dict[str, object] = {
     "inttype": 3, "p": 5, "Nt": 100, "Ntout": 50, "ntout": 1, "dt": 0.01,
-    "gamma": 1.4, "vis": 0.0, "Pr": 0.72, "flux": 0, "precision": 0,
+    "gamma": 1.4, "vis": 0.0, "Pr": 0.72, "flux": 0, "precision": 0, "CFLmax": 0.0, "dtmax": 0.0,
 }
 
 # command-line options that take one value (src/Domain.cpp:375-470)
 _OPTIONS = {"-p": int, "-dt": float, "-Nt": int, "-Ntout": int, "-ntout": int,
-            "-inttype": int, "-vis": float}
+            "-inttype": int, "-vis": float, "-CFLmax": float, "-dtmax": float}
 
 _FIELDS = {2: ["r", "ru", "rv", "E"], 3: ["r", "ru", "rv", "rw", "E"]}
 
@@ -79,6 +79,40 @@ class ArgList(list):
         return len(self)
 
 
+class vField:
+    """Sizes and contents of a domain's conserved state, as pydgm's vField
+    binds DGM::vField (python/dgm.pyx:297-318; src/vField.hpp:330-376).  The
+    data stay on the GPU; ``numpy`` copies them to the host."""
+
+    def __init__(self, domain: "dgx.GPUDomain"):
+        self.domain = domain
+        self._s = domain.vfield_sizes()
+
+    def size(self) -> int:
+        return self._s["size"]
+
+    def qtot(self) -> int:
+        return self._s["qtot"]
+
+    def nmodes(self) -> int:
+        return self._s["nmodes"]
+
+    def ndof(self) -> int:
+        return self._s["nmodes"]        # modal bases: vField::ndof is nmodes
+
+    def sdof(self) -> int:
+        return self._s["sdof"]
+
+    def nsd(self) -> int:
+        return self._s["nsd"]
+
+    def zero(self) -> None:
+        self.domain.upload(np.zeros((self.domain.nf, self.domain.ne, self.domain.nm)), dgx.DGX_MODAL)
+
+    def numpy(self, layout: int = dgx.DGX_NODAL) -> np.ndarray:
+        return self.domain.download(layout)
+
+
 class ReoProblem:
     """Forward problem of a DGM run directory, solved on one GPU."""
 
@@ -94,6 +128,7 @@ class ReoProblem:
         self.domain: Optional[dgx.GPUDomain] = None
         self.step = 0
         self.t = 0.0
+        self.dt = None       # the step Domain::set_time_step last chose (CFLmax runs)
 
     def _parameters(self) -> Dict[str, object]:
         prm = dict(_DEFAULTS)
@@ -234,11 +269,17 @@ class ReoProblem:
         self.domain = dom
         return dom
 
+    def getState(self) -> vField:
+        """The conserved state as a vField (sizes; host copies on request)."""
+        if self.domain is None:
+            self.setup()
+        return vField(self.domain)
+
     def _header(self) -> Dict[str, object]:
         p = self.params
         return {"Name": os.path.basename(self.root), "Nsd": self.mesh.nsd,
                 "Lmax": int(p["p"]) + 1, "Step": self.step, "Time": self.t,
-                "Time step": float(p["dt"]), "Viscosity": float(p["vis"]),
+                "Time step": float(self.dt if self.dt is not None else p["dt"]), "Viscosity": float(p["vis"]),
                 "Field names": ["phi", "uphi", "vphi"] if self.eqn == dgx.DGX_SHALLOW else _FIELDS[self.mesh.nsd]}
 
     def write(self, fname: str) -> None:
@@ -255,20 +296,37 @@ class ReoProblem:
             self.write(f"{self.root}.{self.step}.rst")
         print(f"Advancing solution {Nt} time steps", file=self.out)
         print("    Step      t           dt         CFL         dCPU", file=self.out)
-        chunk = ntout if Ntout == 0 else math.gcd(ntout, Ntout)
+        # Domain::solve calls set_time_step before every step (src/Domain.cpp:931):
+        # with CFLmax the step follows the CFL condition and the run goes step by
+        # step; otherwise dt is fixed and the CFL number is evaluated only for
+        # the steps whose line is printed
+        CFLmax, dtmax = float(prm["CFLmax"]), float(prm["dtmax"])
+        chunk = 1 if CFLmax != 0 else (ntout if Ntout == 0 else math.gcd(ntout, Ntout))
         wall = time.perf_counter()
         done = 0
+        cfl = 0.0
         while done < Nt:
             n = min(chunk, Nt - done)
+            printed = (done + n) % ntout == 0 or done + n == Nt
             t0 = time.perf_counter()
-            dom.advance(inttype, dt, n)
+            if CFLmax != 0:
+                cfl, dt = dom.set_time_step(dt, CFLmax, dtmax)
+                self.dt = dt
+                dom.advance(inttype, dt, n)
+            elif printed:
+                if n > 1:
+                    dom.advance(inttype, dt, n - 1)
+                cfl, _ = dom.set_time_step(dt)
+                dom.advance(inttype, dt, 1)
+            else:
+                dom.advance(inttype, dt, n)
             dom.synchronize()
             el = (time.perf_counter() - t0) / n
             done += n
             self.step += n
             self.t = dom.time
             if done % ntout == 0 or done == Nt:
-                print(f"{self.step:8d}{self.t:12.4e}{dt:12.4e}{0.0:12.4e}{el:12.4e}", file=self.out)
+                print(f"{self.step:8d}{self.t:12.4e}{dt:12.4e}{cfl:12.4e}{el:12.4e}", file=self.out)
             if Ntout == 0 or self.step % Ntout == 0:
                 self.write(f"{self.root}.{self.step}.rst")
         print(f"Finished solve in {time.perf_counter() - wall:.4e} sec", file=self.out)

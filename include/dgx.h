@@ -172,6 +172,13 @@ int dgx_domain_destroy(dgx_handle h);
 /* Sizes: nodes and modes per element, fields, elements. */
 int dgx_sizes(dgx_handle h, int *qtot, int *nmodes, int *nfields, int *ne);
 
+/* vField totals of the domain's state (vField::size, qtot, nmodes, ndof,
+ * sdof, nsd; src/vField.hpp:330-376, bound in python/dgm.pyx:297-318): fields;
+ * quadrature points, modes and side quadrature points summed over all fields
+ * and elements; space dimensions.  ndof equals nmodes (modal bases).  Any
+ * pointer may be NULL. */
+int dgx_vfield_sizes(dgx_handle h, int *size, long long *qtot, long long *nmodes, long long *sdof, int *nsd);
+
 /* Nodal coordinates [dim][elem][node] to host memory (Element::C,
  * src/Quad.cpp:993-1013, src/Hex.cpp:991-1039). */
 int dgx_coordinates(dgx_handle h, double *xyz);
@@ -204,6 +211,31 @@ int dgx_residual_download(dgx_handle h, double *F, int layout, int on_device);
  * (TimeInt::advance: RungeKutta src/TimeInt.cpp:267-302, TVD :824-856,
  * LSRK14 :1989-2013).  Asynchronous on the domain's stream. */
 int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps);
+
+/* CFL time step (Domain::set_time_step, called before every step by
+ * Domain::solve, src/Domain.cpp:931).  For 2D domains (Euler, Navier_Stokes,
+ * Shallow) as Euler::set_time_step (src/Euler.cpp:804-877) and
+ * Shallow::set_time_step (src/Shallow.cpp:573-637) with DGM_STORE_METRICS:
+ * from the current state at the GLL nodes,
+ *   idt = (|u dxr + v dyr| + |u dxs + v dys| + c acoustic) ivol
+ * with the metrics of Element::compute_metrics_2d_tensor_prod
+ * (src/Element.cpp:485-555); *cfl = dt * max idt.  With CFLmax != 0 the new
+ * step is CFLmax * dt / cfl (no larger than dtmax when dtmax != 0), else dt;
+ * *cfl is then rescaled to the new step.  For hexes the reference's
+ * Euler3d::set_time_step (src/Euler3d.cpp:462-465) keeps dt and returns it as
+ * the CFL number, and so does this call.  Waits for the domain's stream;
+ * DGX_ERR_NONREALIZABLE if the state gives no finite sound speed. */
+int dgx_set_time_step(dgx_handle h, double dt, double CFLmax, double dtmax, double *cfl, double *dt_new);
+
+/* One stage of a step: stage `stage` (0 .. 3 RK4, 0 .. 2 TVD-RK3, 0 .. 13
+ * LSRK14) of integrator `inttype`, exactly as dgx_advance runs it, so a
+ * device-resident TimeInt can do host work between stages (the per-stage body
+ * of RungeKutta::advance src/TimeInt.cpp:267-302, TVD::advance :824-856,
+ * LSRK14::advance :1989-2013: residual at the stage time, then the register
+ * update).  Stages must come in order from 0, with one integrator and dt, and
+ * the last one completes the step; dgx_advance is refused while a step begun
+ * here is unfinished.  Asynchronous on the domain's stream. */
+int dgx_stage(dgx_handle h, int inttype, int stage, double dt);
 
 /* Wait for the domain's stream; returns DGX_ERR_NONREALIZABLE if any
  * residual met a non-realizable state since the last check. */

@@ -1754,6 +1754,75 @@ struct Domain {
     if (channel) hold_bulk(Uf);
   }
 
+  // Euler::set_time_step (src/Euler.cpp:804-877) and Shallow::set_time_step
+  // (src/Shallow.cpp:573-637) with the stability metrics of
+  // Element::compute_metrics_2d_tensor_prod (src/Element.cpp:485-555): the
+  // largest idt over the GLL nodes of the current state, then the step rule.
+  // Hexes: Euler3d::set_time_step (src/Euler3d.cpp:462-465) returns dt.
+  // out = {cfl, new dt}
+  void set_time_step(double dt, double CFLmax, double dtmax, double *out) {
+    if (nsd == 3) { out[0] = dt; out[1] = dt; return; }
+    double cfl = 0.0;
+    std::vector<double> xr(qtot), xs(qtot), yr(qtot), ys(qtot);
+    for (int e = 0; e < ne; ++e) {
+      const Elem &E = el[e];
+      // local_gradient(C->x, xr, xs), local_gradient(C->y, yr, ys)
+      for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j) {
+          double a0 = 0, a1 = 0, c0 = 0, c1 = 0;
+          for (int m = 0; m < q; ++m) {
+            a0 += D[i * q + m] * E.cx[m * q + j]; a1 += D[j * q + m] * E.cx[i * q + m];
+            c0 += D[i * q + m] * E.cy[m * q + j]; c1 += D[j * q + m] * E.cy[i * q + m];
+          }
+          xr[i * q + j] = a0; xs[i * q + j] = a1; yr[i * q + j] = c0; ys[i * q + j] = c1;
+        }
+      const double *r = eu(U, 0, e), *ru = eu(U, 1, e), *rv = eu(U, 2, e);
+      const double *rE = eqn == SHALLOW2D ? nullptr : eu(U, 3, e);
+      double emax = 0.0;
+      for (int i = 0; i < q; ++i) {
+        const double dr = i == 0 ? z[i + 1] - z[i] : (i == q - 1 ? z[i] - z[i - 1] : 0.5 * (z[i + 1] - z[i - 1]));
+        for (int j = 0; j < q; ++j) {
+          const double ds = j == 0 ? z[j + 1] - z[j] : (j == q - 1 ? z[j] - z[j - 1] : 0.5 * (z[j + 1] - z[j - 1]));
+          const int n = j + i * q;
+          const double Jn = E.curved ? xr[n] * ys[n] - xs[n] * yr[n] : E.J;
+          const double dxr = xr[n] * dr, dxs = xs[n] * ds, dyr = yr[n] * dr, dys = ys[n] * ds;
+          const double ivol = 1.0 / (dr * ds * Jn);
+          const double acoustic = std::sqrt((dxr + dyr) * (dxr + dyr) + (dxs + dys) * (dxs + dys));
+          const double u = ru[n] / r[n], v = rv[n] / r[n];
+          double c;
+          if (eqn == SHALLOW2D) {
+            c = std::sqrt(1.0 * r[n]);
+          } else {   // c = sqrt(gamma (gamma-1) (rE - 1/2 (r u u + r v v)) / r)
+            c = u * ru[n];
+            c = v * rv[n] + c;
+            c = -0.5 * c;
+            c = rE[n] + c;
+            c = gamma * (gamma - 1.0) * c;
+            c = c / r[n];
+            c = std::sqrt(c);
+          }
+          double idt = u * dxr;
+          idt = v * dyr + idt;
+          idt = std::fabs(idt);
+          double wk = u * dxs;
+          wk = v * dys + wk;
+          wk = std::fabs(wk);
+          idt = wk + idt;
+          idt = c * acoustic + idt;
+          idt = ivol * idt;
+          if (!(idt == idt)) status |= 1;
+          emax = std::max(emax, idt);
+        }
+      }
+      cfl = std::max(cfl, dt * emax);
+    }
+    double dn = dt;
+    if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / cfl);
+    else if (CFLmax != 0) dn = CFLmax * dt / cfl;
+    out[0] = cfl * dn / dt;
+    out[1] = dn;
+  }
+
   // Channel::holdBulkMassFlow (src/Channel.hpp:117-163): subtract the domain
   // means of the density, x-momentum and energy residuals
   bool channel = false;
@@ -2021,6 +2090,12 @@ int orc_advance(void *h, int inttype, double dt, int nsteps) {
   return d->status;
 }
 // Linf, L2, H1 per field of the current state (src/Field.cpp:4675-4697)
+int orc_set_time_step(void *h, double dt, double CFLmax, double dtmax, double *out) {
+  Domain *d = (Domain *)h;
+  d->status = 0;
+  d->set_time_step(dt, CFLmax, dtmax, out);
+  return d->status;
+}
 void orc_norms(void *h, double *out) {
   Domain *d = (Domain *)h;
   const int n = d->qtot;

@@ -61,6 +61,8 @@ def _load():
     lib.orc_set_balance.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_isothermal.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     lib.orc_set_channel.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_set_time_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_set_time_step.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -190,6 +192,13 @@ class Oracle:
         st = self.lib.orc_advance(self.h, inttype, dt, nsteps)
         if st:
             raise FloatingPointError(f"oracle advance failed ({st})")
+
+    def set_time_step(self, dt, CFLmax=0.0, dtmax=0.0):
+        """(CFL, new dt) of the current state (Euler::set_time_step)."""
+        out = np.zeros(2)
+        if self.lib.orc_set_time_step(self.h, float(dt), float(CFLmax), float(dtmax), _ptr(out)):
+            raise FloatingPointError("oracle: non-realizable state in the CFL evaluation")
+        return float(out[0]), float(out[1])
 
     def norms(self):
         out = np.zeros((self.nv, 3))
