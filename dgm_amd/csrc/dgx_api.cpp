@@ -27,6 +27,48 @@ int set_error(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+
+#define HIPCHK_X(expr)                                                                \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return set_error(DGX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+// Largest idt of Euler::set_time_step over the domain's nodes (dgx_cfl.hip),
+// with `part` as the kernels' scratch (ne + 1 doubles); waits for the stream.
+int cfl_max(dgx_domain *d, double *part, double *m) {
+  auto fail = [](int c, const std::string &msg) { return set_error(c, msg); };
+  if (!d->d_cgeo)
+    return fail(DGX_ERR_ARGUMENT, "the CFL time step is available for 2D Euler, Navier_Stokes and Shallow domains");
+  dgx::CflArgs a;
+  a.U = d->U; a.cgeo = d->d_cgeo; a.part = part; a.out = part + d->ne;
+  for (int i = 0; i < d->Q; ++i) {   // GLL spacings (src/Element.cpp:513-527)
+    const std::vector<double> &z = d->hb.z;
+    a.dz[i] = i == 0 ? z[1] - z[0] : (i == d->Q - 1 ? z[i] - z[i - 1] : 0.5 * (z[i + 1] - z[i - 1]));
+  }
+  a.gamma = d->gamma; a.ne = d->ne; a.p = d->p;
+  a.shallow = d->eqn == DGX_SHALLOW; a.curved = d->curved; a.stream = d->stream;
+  hipError_t e = dgx::launch_cfl2(a);
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("CFL kernels: ") + hipGetErrorString(e));
+  HIPCHK_X(hipMemcpyAsync(m, a.out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK_X(hipStreamSynchronize(d->stream));
+  if (!(*m == *m)) return fail(DGX_ERR_NONREALIZABLE, "non-realizable state in the CFL evaluation");
+  return DGX_OK;
+}
+
+// The time step rule of Euler::set_time_step (src/Euler.cpp:861-874) from the
+// largest idt: CFL of dt, new dt under CFLmax (and dtmax), CFL of the new dt.
+int cfl_finish(double dt, double CFLmax, double dtmax, double m, double *cfl, double *dt_new) {
+  const double c = dt * m;
+  double dn = dt;
+  if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / c);
+  else if (CFLmax != 0) dn = CFLmax * dt / c;
+  *dt_new = dn;
+  *cfl = c * dn / dt;
+  return DGX_OK;
+}
+
+#undef HIPCHK_X
 }  // namespace dgx
 
 namespace {
@@ -250,6 +292,7 @@ int launch(dgx_domain *d, int op, const double *U, double *out) {
 dgx::CurvedArgs curved_args(dgx_domain *d) {
   dgx::CurvedArgs ca;
   ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
+  ca.chol = d->nsd == 2 ? d->d_chol : nullptr;
   ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
   ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
   ca.ns = d->visc; ca.G = d->G; ca.V = d->V; ca.TV = d->TV;
@@ -347,39 +390,6 @@ int residual_core(dgx_domain *d, const double *U, double *R) {
   }
   if ((rc = launch(d, dgx::OP_RHS, U, R))) return rc;
   ++d->stages;
-  return DGX_OK;
-}
-
-// Largest idt of Euler::set_time_step over the domain's nodes (dgx_cfl.hip),
-// with `part` as the kernels' scratch (ne + 1 doubles); waits for the stream.
-int cfl_max(dgx_domain *d, double *part, double *m) {
-  if (!d->d_cgeo)
-    return fail(DGX_ERR_ARGUMENT, "the CFL time step is available for 2D Euler, Navier_Stokes and Shallow domains");
-  dgx::CflArgs a;
-  a.U = d->U; a.cgeo = d->d_cgeo; a.part = part; a.out = part + d->ne;
-  for (int i = 0; i < d->Q; ++i) {   // GLL spacings (src/Element.cpp:513-527)
-    const std::vector<double> &z = d->hb.z;
-    a.dz[i] = i == 0 ? z[1] - z[0] : (i == d->Q - 1 ? z[i] - z[i - 1] : 0.5 * (z[i + 1] - z[i - 1]));
-  }
-  a.gamma = d->gamma; a.ne = d->ne; a.p = d->p;
-  a.shallow = d->eqn == DGX_SHALLOW; a.curved = d->curved; a.stream = d->stream;
-  hipError_t e = dgx::launch_cfl2(a);
-  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("CFL kernels: ") + hipGetErrorString(e));
-  HIPCHK(hipMemcpyAsync(m, a.out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIPCHK(hipStreamSynchronize(d->stream));
-  if (!(*m == *m)) return fail(DGX_ERR_NONREALIZABLE, "non-realizable state in the CFL evaluation");
-  return DGX_OK;
-}
-
-// The time step rule of Euler::set_time_step (src/Euler.cpp:861-874) from the
-// largest idt: CFL of dt, new dt under CFLmax (and dtmax), CFL of the new dt.
-int cfl_finish(double dt, double CFLmax, double dtmax, double m, double *cfl, double *dt_new) {
-  const double c = dt * m;
-  double dn = dt;
-  if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / c);
-  else if (CFLmax != 0) dn = CFLmax * dt / c;
-  *dt_new = dn;
-  *cfl = c * dn / dt;
   return DGX_OK;
 }
 
@@ -576,6 +586,57 @@ int curved_geometry2(dgx_domain *d, const double *xn) {
   HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&d->d_iters, 64));
   HIPCHK(hipMemset(d->d_iters, 0, 64));
+  // Cholesky factors of the element mass matrices M[a][b] = sum_n phi_a wJ phi_b
+  // over the GLL nodes (Quad::get_mass_matrix and its factorization with
+  // DGM_CHOLESKY_MASS_MATRIX, src/Quad.cpp:263-300), rows of the lower factor
+  // packed; the projection then solves with them instead of iterating.  Held
+  // while they take at most 1 GiB (p = 10: 59 KB per element); beyond that the
+  // kernels keep the in-workgroup conjugate gradients.
+  const int L = d->L, nm = L * L;
+  const size_t pk = (size_t)nm * (nm + 1) / 2;
+  if ((size_t)ne * pk * 8 <= (size_t(1) << 30)) {
+    std::vector<double> cf((size_t)ne * pk);
+    const std::vector<double> &BT = b.BT;   // [Q][L]
+    auto factor = [&](int e0, int e1) {
+      std::vector<double> Phi((size_t)nm * Q2), M((size_t)nm * nm), C((size_t)nm * nm);
+      for (int a = 0; a < L; ++a)
+        for (int c = 0; c < L; ++c)
+          for (int i = 0; i < Q; ++i)
+            for (int j = 0; j < Q; ++j) Phi[(size_t)(a * L + c) * Q2 + i * Q + j] = BT[i * L + a] * BT[j * L + c];
+      for (int e = e0; e < e1; ++e) {
+        const double *wJ = gm.data() + (size_t)e * 5 * Q2 + 4 * Q2;
+        for (int r = 0; r < nm; ++r)
+          for (int c = 0; c <= r; ++c) {
+            double t = 0.0;
+            for (int n = 0; n < Q2; ++n) t += Phi[(size_t)r * Q2 + n] * wJ[n] * Phi[(size_t)c * Q2 + n];
+            M[(size_t)r * nm + c] = t;
+          }
+        for (int r = 0; r < nm; ++r)
+          for (int c = 0; c <= r; ++c) {
+            double t = M[(size_t)r * nm + c];
+            for (int k = 0; k < c; ++k) t -= C[(size_t)r * nm + k] * C[(size_t)c * nm + k];
+            C[(size_t)r * nm + c] = (r == c) ? std::sqrt(t) : t / C[(size_t)c * nm + c];
+          }
+        double *o = cf.data() + (size_t)e * pk;
+        for (int r = 0; r < nm; ++r)
+          for (int c = 0; c <= r; ++c) o[(size_t)r * (r + 1) / 2 + c] = C[(size_t)r * nm + c];
+      }
+    };
+    const int nth = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    const int chunk = (ne + nth - 1) / nth;
+    for (int t = 0; t < nth; ++t) {
+      const int e0 = t * chunk, e1 = std::min(ne, e0 + chunk);
+      if (e0 < e1) pool.emplace_back(factor, e0, e1);
+    }
+    for (auto &th : pool) th.join();
+    for (size_t k = 0; k < cf.size(); k += pk)
+      if (!(cf[k] > 0.0) || !std::isfinite(cf[k + pk - 1]) || !(cf[k + pk - 1] > 0.0))
+        return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(k / pk) + ": mass matrix not positive definite");
+    int rc2;
+    if ((rc2 = alloc(&d->d_chol, cf.size()))) return rc2;
+    HIPCHK(hipMemcpy(d->d_chol, cf.data(), cf.size() * 8, hipMemcpyHostToDevice));
+  }
   std::copy(xn, xn + (size_t)2 * ne * Q2, d->coords.begin());
   d->curved = true;
   return DGX_OK;
@@ -731,6 +792,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   d->vis = desc->vis;
   d->Pr = desc->Pr > 0 ? desc->Pr : 0.72;
   d->hb = make_basis(p);
+  (void)hipGetDevice(&d->device);
   const HostBasis &b = d->hb;
   const int Q = d->Q, nvert = nsd == 2 ? 4 : 8;
 
@@ -1037,7 +1099,7 @@ void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
                   d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
-                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part};
+                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part, d->d_chol};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -1082,6 +1144,8 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   }
   if (shallow && !desc->no_balance) b += nf * ne * nm * 8;    // balance term
   if (nsd == 2) b += ne * 5 * ((desc->xnodes || fns || !all_quads_affine(desc)) ? nq : 1) * 8;   // CFL geometry
+  if (nsd == 2 && (desc->xnodes || fns || !all_quads_affine(desc)) && ne * nm * (nm + 1) / 2 * 8 <= (size_t(1) << 30))
+    b += ne * nm * (nm + 1) / 2 * 8;                        // curved quads: mass-matrix factors
   if (desc->bulk_forcing) b += ne * 8;
   *bytes = b;
   return DGX_OK;
@@ -1397,9 +1461,9 @@ int dgx_set_time_step(dgx_handle h, double dt, double CFLmax, double dtmax, doub
     if (rc) return rc;
   }
   double m = 0.0;
-  int rc = cfl_max(h, h->d_part, &m);
+  int rc = dgx::cfl_max(h, h->d_part, &m);
   if (rc) return rc;
-  return cfl_finish(dt, CFLmax, dtmax, m, cfl, dt_new);
+  return dgx::cfl_finish(dt, CFLmax, dtmax, m, cfl, dt_new);
 }
 
 int dgx_synchronize(dgx_handle h) {

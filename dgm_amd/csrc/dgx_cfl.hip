@@ -15,6 +15,11 @@
 // k_cfl2 evaluates the nodal state from the modes in LDS (one wavefront per
 // element) and writes each element's largest idt to a scratch array of ne
 // values; k_max1 reduces that array in one workgroup, in a fixed order.
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "dgx_domain.hpp"
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -129,4 +134,63 @@ hipError_t upload_tables_cfl(const double *h) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free CFL evaluation (dgx_set_time_step_noalloc).  The kernels'
+// scratch (ne per-element maxima and the domain maximum) is this file-scope
+// device array instead of a buffer the library allocates; every GPU the
+// library runs on holds its own copy until the process ends.  Its device
+// address is looked up once per device, with that device current, and kept.
+// Calls on one GPU share the array: calls that overlap on one GPU race.  The
+// kernels write every value they later read, so nothing needs clearing.
+constexpr size_t kCflScratch = size_t(1) << 19;   // doubles (4 MiB): ne + 1 <= 524288
+static __device__ __align__(256) double scratch[kCflScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return set_error(DGX_ERR_ARGUMENT, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(DGX_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return DGX_OK;
+}
+}  // namespace
+
 }  // namespace dgx
+
+extern "C" int dgx_set_time_step_noalloc(dgx_handle h, double dt, double CFLmax, double dtmax, double *cfl,
+                                         double *dt_new) {
+  using dgx::set_error;
+  if (!h || !cfl || !dt_new) return set_error(DGX_ERR_ARGUMENT, "null argument");
+  if (!(dt > 0) || !std::isfinite(dt) || !(CFLmax >= 0) || !(dtmax >= 0))
+    return set_error(DGX_ERR_ARGUMENT, "dt must be positive and CFLmax, dtmax >= 0");
+  if (h->nsd == 3) {   // Euler3d::set_time_step keeps dt and returns it
+    *cfl = dt;
+    *dt_new = dt;
+    return DGX_OK;
+  }
+  if ((size_t)h->ne + 1 > dgx::kCflScratch)
+    return set_error(DGX_ERR_ARGUMENT, "dgx_set_time_step_noalloc: " + std::to_string(h->ne) +
+                                           " elements exceed its scratch of " +
+                                           std::to_string(dgx::kCflScratch - 1) + "; use dgx_set_time_step");
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) return set_error(DGX_ERR_DEVICE, "hipGetDevice failed");
+  if (cur != h->device && hipSetDevice(h->device) != hipSuccess)
+    return set_error(DGX_ERR_DEVICE, "hipSetDevice failed");
+  double *ptr = nullptr;
+  double m = 0.0;
+  int rc = dgx::scratch_ptr(h->device, &ptr);
+  if (!rc) rc = dgx::cfl_max(h, ptr, &m);
+  if (cur != h->device) (void)hipSetDevice(cur);
+  if (rc) return rc;
+  return dgx::cfl_finish(dt, CFLmax, dtmax, m, cfl, dt_new);
+}

@@ -23,48 +23,62 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <int NI> constexpr int out_chunk() { return NI * 3 <= 40 ? 3 : 2; }
 
 // [A][N][B][C] (last-dim pitch PI) -> [A][NO][B][C] (pitch PO), contracting N.
+// When every thread owns at most one pencil (NP <= NT), the pencil's inputs
+// are read from LDS once and all output chunks are computed from registers;
+// otherwise the chunks are outermost (inputs re-read per chunk) so that only
+// one chunk of coefficient rows is live in scalar registers across the
+// pencil loop.  Both orders give bitwise the same sums.
 template <int A, int N, int B, int C, int NO, int PI, int PO, int NT, bool ACC = false>
 __device__ __forceinline__ void con_mid(const double *__restrict__ M, const double *in,
                                         double *out, int tid) {
   constexpr int NP = A * B * C, OC = out_chunk<N>();
+  auto pencil = [&](int pc, int o0, int o1) {
+    const int c = pc % C, t = pc / C, b = t % B, a = t / B;
+    double x[N];
 #pragma unroll
-  for (int o0 = 0; o0 < NO; o0 += OC) {
-    for (int pc = tid; pc < NP; pc += NT) {
-      const int c = pc % C, t = pc / C, b = t % B, a = t / B;
-      double x[N];
+    for (int i = 0; i < N; ++i) x[i] = in[((a * N + i) * B + b) * PI + c];
 #pragma unroll
-      for (int i = 0; i < N; ++i) x[i] = in[((a * N + i) * B + b) * PI + c];
+    for (int o = o0; o < o1; ++o) {
+      double s = 0.0;
 #pragma unroll
-      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) s = fma(M[o * N + i], x[i], s);
-        double *dst = out + ((a * NO + o) * B + b) * PO + c;
-        if (ACC) *dst += s; else *dst = s;
-      }
+      for (int i = 0; i < N; ++i) s = fma(M[o * N + i], x[i], s);
+      double *dst = out + ((a * NO + o) * B + b) * PO + c;
+      if (ACC) *dst += s; else *dst = s;
     }
+  };
+  if constexpr (NP <= NT) {
+    if (tid < NP) pencil(tid, 0, NO);
+  } else {
+#pragma unroll
+    for (int o0 = 0; o0 < NO; o0 += OC)
+      for (int pc = tid; pc < NP; pc += NT) pencil(pc, o0, o0 + OC < NO ? o0 + OC : NO);
   }
 }
 
-// [R][NI] (pitch PI) -> [R][NO] (pitch PO), contracting the last dimension.
+// [R][NI] (pitch PI) -> [R][NO] (pitch PO), contracting the last dimension
+// (loop order as in con_mid).
 template <int R, int NI, int NO, int PI, int PO, int NT>
 __device__ __forceinline__ void con_last(const double *__restrict__ M, const double *in, double *out,
                                          int tid) {
   constexpr int OC = out_chunk<NI>();
+  auto pencil = [&](int r, int o0, int o1) {
+    double x[NI];
 #pragma unroll
-  for (int o0 = 0; o0 < NO; o0 += OC) {
-    for (int r = tid; r < R; r += NT) {
-      double x[NI];
+    for (int i = 0; i < NI; ++i) x[i] = in[r * PI + i];
 #pragma unroll
-      for (int i = 0; i < NI; ++i) x[i] = in[r * PI + i];
+    for (int o = o0; o < o1; ++o) {
+      double s = 0.0;
 #pragma unroll
-      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
-        out[r * PO + o] = s;
-      }
+      for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
+      out[r * PO + o] = s;
     }
+  };
+  if constexpr (R <= NT) {
+    if (tid < R) pencil(tid, 0, NO);
+  } else {
+#pragma unroll
+    for (int o0 = 0; o0 < NO; o0 += OC)
+      for (int r = tid; r < R; r += NT) pencil(r, o0, o0 + OC < NO ? o0 + OC : NO);
   }
 }
 

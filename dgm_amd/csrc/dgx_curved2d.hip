@@ -68,7 +68,7 @@ __device__ __forceinline__ void field_dots2(const double (&part)[NF], double *re
 template <int P, int NF, int NT>
 __device__ void curved_project2(double *Fw, double *S1, double *S2, double *AP, double *red, const double *ge,
                                 double jav, double *out, size_t ostride, int *iters, int tid,
-                                double *Bal = nullptr, int bal = 0) {
+                                double *Bal = nullptr, int bal = 0, const double *cf = nullptr) {
   constexpr int L = P + 1, Q = P + 2, L2 = L * L, Q2 = Q * Q;
   constexpr int NW = NT / 64, NML = (NF * L2 + NT - 1) / NT;
   // b = B^T (wJ f): into AP (compact [NF][L2])
@@ -76,6 +76,37 @@ __device__ void curved_project2(double *Fw, double *S1, double *S2, double *AP, 
   __syncthreads();
   contract<NF, L, Q, 2, L, NT>(CT<P>::BL(), S1, AP, tid);   // [NF][L][L]
   __syncthreads();
+  if (cf) {
+    // M x = b with the element's Cholesky factor M = C C^T (rows of C packed),
+    // forward then backward substitution, one thread per field, in the order
+    // of the reference's factored solve (DGM_CHOLESKY_MASS_MATRIX,
+    // src/Quad.cpp:263-300) and without contraction of its products
+    if (tid < NF) {
+#pragma clang fp contract(off)
+      double *x = AP + tid * L2;
+      for (int i = 0; i < L2; ++i) {
+        const double *row = cf + (size_t)i * (i + 1) / 2;
+        double t = x[i];
+        for (int k = 0; k < i; ++k) t -= row[k] * x[k];
+        x[i] = t / row[i];
+      }
+      for (int i = L2 - 1; i >= 0; --i) {
+        double t = x[i];
+        for (int k = i + 1; k < L2; ++k) t -= cf[(size_t)k * (k + 1) / 2 + i] * x[k];
+        x[i] = t / cf[(size_t)i * (i + 1) / 2 + i];
+      }
+    }
+    __syncthreads();
+    for (int g = tid; g < NF * L2; g += NT) {
+      const int v = g / L2;
+      const size_t o = v * ostride + (g - v * L2);
+      double r = AP[g];
+      if (bal == 1) { Bal[o] = r; r = -1.0 * r + r; }   // the balance term (src/Shallow.cpp:251-266)
+      else if (bal == 2) r = -1.0 * Bal[o] + r;
+      out[o] = r;
+    }
+    return;
+  }
 
   double *Pv = Fw;
   double x[NML], res[NML], bvec[NML];
@@ -404,7 +435,8 @@ k_rhs2c(CurvedArgs a) {
   }   // !PROJ
   curved_project2<P, NF, NT>(Fw, S1, S2, Un, red, ge, a.jav[e], a.R + (size_t)e * L2, (size_t)ne * L2, a.iters, tid,
                              (SHALLOW && !PROJ && a.bal) ? a.Bal + (size_t)e * L2 : nullptr,
-                             (SHALLOW && !PROJ) ? a.bal : 0);
+                             (SHALLOW && !PROJ) ? a.bal : 0,
+                             a.chol ? a.chol + (size_t)e * (L2 * (L2 + 1) / 2) : nullptr);
 }
 
 // Lifted, projected gradients of the primitives (u, v, e) of a curved quad
@@ -493,7 +525,8 @@ k_grad2c(CurvedArgs a) {
     __syncthreads();
   }
   lift_edges2<P, 6, NT>(gF, S1, Fw, ge, tid);
-  curved_project2<P, 6, NT>(Fw, S1, S2, AP, red, ge, a.jav[e], a.G + (size_t)e * 6 * L2, L2, a.iters, tid);
+  curved_project2<P, 6, NT>(Fw, S1, S2, AP, red, ge, a.jav[e], a.G + (size_t)e * 6 * L2, L2, a.iters, tid, nullptr, 0,
+                            a.chol ? a.chol + (size_t)e * (L2 * (L2 + 1) / 2) : nullptr);
 }
 
 // Viscous terms of a curved quad from the state and the lifted gradients G:

@@ -223,9 +223,11 @@ __device__ __forceinline__ void ghost_prim3(int bc, double rho, const Phys &ph, 
 // Contract dimension DIMC of the LDS tensor `in` (shape N0 x N1 x N2, last
 // index fastest) with the NO x NI matrix M; output has NO in place of NI.
 // One thread owns one pencil at a time and computes OC outputs per pass from
-// its NI inputs (register blocking).  Passes over output chunks are outermost
-// so that only OC x NI coefficients are live in scalar registers at a time.
-// ACC adds into `out`.
+// its NI inputs (register blocking).  When every thread owns at most one
+// pencil, its inputs are read once and all outputs computed from registers;
+// otherwise passes over output chunks are outermost so that only OC x NI
+// coefficients are live in scalar registers at a time.  Both orders give
+// bitwise the same sums.  ACC adds into `out`.
 template <int N0, int N1, int N2, int DIMC, int NO, int NT, bool ACC = false>
 __device__ __forceinline__ void contract(const double *__restrict__ M, const double *in,
                                          double *out, int tid) {
@@ -236,27 +238,31 @@ __device__ __forceinline__ void contract(const double *__restrict__ M, const dou
   constexpr int SI = DIMC == 0 ? N1 * N2 : (DIMC == 1 ? N2 : 1);
   constexpr int SO = DIMC == 0 ? O1 * O2 : (DIMC == 1 ? O2 : 1);
   constexpr int OC = NI * 3 <= 40 ? 3 : 2;    // coefficients live per pass <= ~40 doubles
+  auto pencil = [&](int pc, int o0, int o1) {
+    int i0, i1, i2;
+    if (DIMC == 0) { i0 = 0; i1 = pc / N2; i2 = pc - i1 * N2; }
+    else if (DIMC == 1) { i0 = pc / N2; i1 = 0; i2 = pc - i0 * N2; }
+    else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
+    const int bi = (i0 * N1 + i1) * N2 + i2;
+    const int bo = (i0 * O1 + i1) * O2 + i2;
+    double x[NI];
 #pragma unroll
-  for (int o0 = 0; o0 < NO; o0 += OC) {
-    for (int pc = tid; pc < NP; pc += NT) {
-      int i0, i1, i2;
-      if (DIMC == 0) { i0 = 0; i1 = pc / N2; i2 = pc - i1 * N2; }
-      else if (DIMC == 1) { i0 = pc / N2; i1 = 0; i2 = pc - i0 * N2; }
-      else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
-      const int bi = (i0 * N1 + i1) * N2 + i2;
-      const int bo = (i0 * O1 + i1) * O2 + i2;
-      double x[NI];
+    for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
 #pragma unroll
-      for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
+    for (int o = o0; o < o1; ++o) {
+      double s = 0.0;
 #pragma unroll
-      for (int o = o0; o < (o0 + OC < NO ? o0 + OC : NO); ++o) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
-        if (ACC) out[bo + o * SO] += s;
-        else out[bo + o * SO] = s;
-      }
+      for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
+      if (ACC) out[bo + o * SO] += s;
+      else out[bo + o * SO] = s;
     }
+  };
+  if constexpr (NP <= NT) {
+    if (tid < NP) pencil(tid, 0, NO);
+  } else {
+#pragma unroll
+    for (int o0 = 0; o0 < NO; o0 += OC)
+      for (int pc = tid; pc < NP; pc += NT) pencil(pc, o0, o0 + OC < NO ? o0 + OC : NO);
   }
 }
 

@@ -64,6 +64,7 @@ struct dgx_domain {
   bool bulk = false;       // channel bulk forcing (Channel::holdBulkMassFlow) after every residual
   double *d_J = nullptr;   // [ne] element Jacobians (bulk forcing)
   double *d_cgeo = nullptr;  // 2D: CFL geometry (x_r, x_s, y_r, y_s, J) per element or, curved, per node
+  double *d_chol = nullptr;  // curved quads: packed Cholesky factors of the element mass matrices
   double *d_part = nullptr;  // 2D: CFL scratch, ne per-element maxima and the domain maximum
   int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
   dgx::Tab tab;
@@ -75,5 +76,15 @@ struct dgx_domain {
   std::vector<hipEvent_t> ev;
   std::vector<int> ev_kind;
 
+  int device = 0;           // the GPU the domain's buffers live on
+
   size_t state_len() const { return (size_t)nf * ne * nm; }
 };
+
+namespace dgx {
+// CFL evaluation of a 2D domain (dgx_api.cpp): the largest idt of
+// Euler::set_time_step with `part` (ne + 1 doubles of device memory) as the
+// kernels' scratch, waiting for the domain's stream; and the step rule.
+int cfl_max(dgx_domain *d, double *part, double *m);
+int cfl_finish(double dt, double CFLmax, double dtmax, double m, double *cfl, double *dt_new);
+}  // namespace dgx

@@ -112,6 +112,8 @@ struct CurvedArgs {
   const double *gm = nullptr;                // [ne][10][Q^3]: wJ dxi_d/dx_x (d-major), wJ
   const double *fgeo = nullptr;              // [ne][6][QF^2][4]: normal, J_side/J_vol
   const double *jav = nullptr;               // [ne]: mean J (first guess of the mass solve)
+  const double *chol = nullptr;              // quads: lower Cholesky factors of the element mass
+                                             // matrices, rows packed, [ne][L2 (L2 + 1) / 2]
   Geo geo;
   Phys phys;
   int ne = 0, p = 0;

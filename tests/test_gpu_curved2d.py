@@ -1,14 +1,11 @@
 """Curved quads on the GPU (k_rhs2c: metrics at the nodes, mass-matrix
-projection by in-workgroup conjugate gradients) against the oracle's
-restatement of Quad::compute_curved_geometry (src/Quad.cpp:1309-1452).
+projection with each element's Cholesky factor, as the reference solves it)
+against the oracle's restatement of Quad::compute_curved_geometry
+(src/Quad.cpp:1309-1452).
 
-Bar: relative L2 per field <= 1e-12 for one residual, <= 1e-11 after steps;
-the reference's cvquad run (2D Navier-Stokes on arc-sided quads) within its
-own dgm_diff tolerances.
-At p >= 8 on strongly curved quads the residual bar is 2.5e-12: the GPU solves
-the curved mass matrix by conjugate gradients, the oracle (as the reference)
-by its Cholesky factor, and at those orders their roundings differ by up to
-1.5e-12 (measured; the same elements fed affine nodes agree to 1e-13).
+Bar: relative L2 per field <= 1e-12 for one residual at every order, <= 1e-11
+after steps; the reference's cvquad run (2D Navier-Stokes on arc-sided quads)
+within its own dgm_diff tolerances.
 """
 import dataclasses
 
@@ -68,7 +65,7 @@ def test_rhs_euler2d_curved_periodic(p, flux):
     gpu.upload(Up)
     err = rel_l2(gpu.rhs(), F_ref)
     print(f"[curved2d] rel L2 per field: {err}", flush=True)
-    assert np.all(err <= (TOL_RHS if p < 8 else 2.5e-12)), err
+    assert np.all(err <= TOL_RHS), err
     gpu.close()
 
 
@@ -152,7 +149,7 @@ def test_rhs_ns2d_curved_periodic(p):
     gpu.upload(Up)
     err = rel_l2(gpu.rhs(), F_ref)
     print(f"[curved2d] rel L2 per field: {err}", flush=True)
-    assert np.all(err <= (TOL_RHS if p < 8 else 2.5e-12)), err
+    assert np.all(err <= TOL_RHS), err
     gpu.close()
 
 
