@@ -3,8 +3,10 @@ projection with each element's Cholesky factor, as the reference solves it)
 against the oracle's restatement of Quad::compute_curved_geometry
 (src/Quad.cpp:1309-1452).
 
-Bar: relative L2 per field <= 1e-12 for one residual at every order, <= 1e-11
-after steps; the reference's cvquad run (2D Navier-Stokes on arc-sided quads)
+Bar: relative L2 per field <= 1e-12 for one residual at every order (for
+Navier-Stokes, or three times the residual's own rounding floor where that is
+larger: at p = 10 a one-ulp perturbation of the input state moves the energy
+residual by about 8e-13), <= 1e-11 after steps; the reference's cvquad run (2D Navier-Stokes on arc-sided quads)
 within its own dgm_diff tolerances.
 """
 import dataclasses
@@ -145,11 +147,20 @@ def test_rhs_ns2d_curved_periodic(p):
     eul.set_nodes(xn)
     visc = rel_l2(eul.rhs(Up), F_ref)
     assert np.all(visc[1:] > 1e-3), visc                 # the viscous terms are resolved by the bar
+    # The residual's own rounding floor: its change when the input state is
+    # perturbed by one ulp (random relative 2.2e-16), on the oracle.  At p = 10
+    # the second derivatives of the viscous terms put it near 8e-13 for the
+    # energy, so two FP64 evaluations in different operation orders can differ
+    # by more than 1e-12 there; the bar is 1e-12 or three times that floor.
+    rng = np.random.default_rng(p)
+    floor = np.max([rel_l2(orc.rhs(Up * (1 + 2.2e-16 * rng.standard_normal(Up.shape))), F_ref)
+                    for _ in range(2)], axis=0)
+    bar = np.maximum(TOL_RHS, 3 * floor)
     gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES, vis=vis, xnodes=xn)
     gpu.upload(Up)
     err = rel_l2(gpu.rhs(), F_ref)
-    print(f"[curved2d] rel L2 per field: {err}", flush=True)
-    assert np.all(err <= TOL_RHS), err
+    print(f"[curved2d] rel L2 per field: {err}; rounding floor {floor}; bar {bar}", flush=True)
+    assert np.all(err <= bar), err
     gpu.close()
 
 
