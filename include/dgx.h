@@ -227,6 +227,17 @@ int dgx_advance(dgx_handle h, int inttype, double dt, int nsteps);
  * DGX_ERR_NONREALIZABLE if the state gives no finite sound speed. */
 int dgx_set_time_step(dgx_handle h, double dt, double CFLmax, double dtmax, double *cfl, double *dt_new);
 
+/* dgx_set_time_step without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * kernels' scratch (one maximum per element plus the domain maximum) is a
+ * file-scope device array of 2^19 doubles (4 MiB, held by every GPU the
+ * library runs on until the process ends), so domains of more than 524,287
+ * elements are refused with DGX_ERR_ARGUMENT.  Same kernels and results as
+ * dgx_set_time_step.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race. */
+int dgx_set_time_step_noalloc(dgx_handle h, double dt, double CFLmax, double dtmax, double *cfl, double *dt_new);
+
 /* One stage of a step: stage `stage` (0 .. 3 RK4, 0 .. 2 TVD-RK3, 0 .. 13
  * LSRK14) of integrator `inttype`, exactly as dgx_advance runs it, so a
  * device-resident TimeInt can do host work between stages (the per-stage body
