@@ -38,6 +38,9 @@ Also printed in the same JSON line:
   serving       p50/p99 latency and throughput of a serving-loop sample: Poisson
                 arrivals of 2D Euler forward solves of random size and order,
                 continuously batched (dgm_amd.serve)
+  element_block the element-block seam (GpGpu::hex_kernels' inner_product and
+                backward_transform) on 32768 hexes at 8 modes / 9 points per
+                direction, device-resident, with its HBM GB/s
   cpu_baseline  the CPU oracle on the host cores (one worker per core, at
                 most 16) on a bounded sample of the same workload: same p and
                 physics on a 3^3 periodic box per worker, ~15 s
@@ -301,6 +304,39 @@ def curved_sample(args, seed=5):
             "algorithmic_bytes_per_dof_stage": 32.0 + metric_b}
 
 
+def element_block_sample(args, seed=11):
+    """The element-block seam (GpGpu::hex_kernels: inner_product and
+    backward_transform of a block of hexes sharing one basis) on the TGV mesh's
+    element count and order: 32768 hexes, 8 modes and 9 GLL points per
+    direction, device-resident slots (no PCIe), random basis and wJ.  HBM view:
+    each transform reads and writes each element's slot once, (729 + 512) x 8 B
+    per element."""
+    from dgm_amd.hexblock import DeviceBuffer, HexKernels
+    ne, nm, q, reps = args.hb_ne, 8, 9, args.hb_reps
+    rng = np.random.default_rng(seed)
+    hk = HexKernels().initialize(ne, nm, q, q, q)
+    hk.Basis(*[rng.standard_normal((nm, q)) for _ in range(3)])
+    hk.wJ(rng.uniform(0.5, 2.0, q ** 3))
+    d = DeviceBuffer(ne * q ** 3)
+    d.upload(rng.standard_normal(ne * q ** 3))
+    out = {}
+    for name, fn in (("inner_product", hk.inner_product), ("backward_transform", hk.backward_transform)):
+        fn(d.address)
+        hk.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(d.address)
+        hk.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        nbytes = ne * (q ** 3 + nm ** 3) * 8
+        out[name] = {"ms_per_call": el * 1e3, "hbm_gbs": nbytes / el / 1e9, "frac_of_hbm_peak": nbytes / el / 1e9 / HBM_PEAK_GBS}
+    hk.close()
+    d.free()
+    out["workload"] = (f"{ne} hexes, {nm} modes and {q} GLL points per direction, FP64, device-resident slots, "
+                       f"{reps} calls each; bytes per call = ne x (q^3 + nm^3) x 8")
+    return out
+
+
 def run_path(m, n, p, mu, precision, args, flux=None, viscous=None, compare=True):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
     flux = flux or args.flux
@@ -409,6 +445,9 @@ def main():
     ap.add_argument("--no-curved", action="store_true", help="skip the curved-bump sample")
     ap.add_argument("--cv-n", type=int, default=96, help="hexes per direction of the curved sample")
     ap.add_argument("--cv-steps", type=int, default=2)
+    ap.add_argument("--no-hexblock", action="store_true", help="skip the element-block seam sample")
+    ap.add_argument("--hb-ne", type=int, default=32768, help="elements of the element-block sample")
+    ap.add_argument("--hb-reps", type=int, default=50)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-nmax", type=int, default=64)
     ap.add_argument("--serve-steps", type=int, default=200)
@@ -500,6 +539,8 @@ def main():
         line["adjoint"] = adjoint_sample(args)
     if not args.no_curved:
         line["curved"] = curved_sample(args)
+    if not args.no_hexblock:
+        line["element_block"] = element_block_sample(args)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

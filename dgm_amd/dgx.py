@@ -81,6 +81,14 @@ def load():
         "dgx_stage_count": ([h], ctypes.c_longlong),
         "dgx_profile": ([h, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip, ctypes.c_int],
                         ctypes.c_int),
+        "dgx_hexblock_create": ([ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.POINTER(h)], ctypes.c_int),
+        "dgx_hexblock_destroy": ([h], ctypes.c_int),
+        "dgx_hexblock_set_wJ": ([h, _dp, ctypes.c_int], ctypes.c_int),
+        "dgx_hexblock_set_basis": ([h, _dp, _dp, _dp], ctypes.c_int),
+        "dgx_hexblock_inner_product": ([h, _dp, ctypes.c_int], ctypes.c_int),
+        "dgx_hexblock_backward_transform": ([h, _dp, ctypes.c_int], ctypes.c_int),
+        "dgx_hexblock_synchronize": ([h], ctypes.c_int),
         "dgx_last_error": ([], ctypes.c_char_p),
         "dgx_version": ([], ctypes.c_char_p),
     }

@@ -393,6 +393,53 @@ int dgx_acoustic_stats(dgx_acoustic_handle a, long long *forward_steps, long lon
  * every step was reversed in order from its own forward state. */
 int dgx_revolve_count(int nsteps, int nsnap, long long *advances, int *max_slots, int *order_ok);
 
+/* ---------------------------------------------------------------------------
+ * Element-block seam: the batched hex transforms of GpGpu::hex_kernels
+ * (src/GpGpu.hpp:24-57, src/GpGpu.cu:174-420 and 470-800), which
+ * HexBlock::initialize_gpu / forward_transform / backward_transform drive for
+ * a block of hexes that share one basis (src/HexBlock.cpp:35-130).
+ *
+ * Every element owns a slot of qa*qb*qc doubles, [elem][a][b][c] with c
+ * fastest; its nmodes^3 modal coefficients are the first values of the slot,
+ * [i][j][k] with k fastest.  Both transforms work in place, as hex_kernels'
+ * do, on host memory (copied in and out on the block's stream, as the
+ * reference does on every call) or on device memory (`on_device` = 1: no
+ * copies; the library runs on the caller's buffer).
+ * ------------------------------------------------------------------------- */
+typedef struct dgx_hexblock *dgx_hexblock_handle;
+
+/* hex_kernels::hex_kernels + initialize(ne, nmodes, qa, qb, qc): sizes
+ * (1 <= nmodes <= qa, qb, qc <= 16) and the block's device buffers.
+ * `stream` is a hipStream_t or NULL (the library creates one). */
+int dgx_hexblock_create(long long ne, int nmodes, int qa, int qb, int qc, void *stream, dgx_hexblock_handle *out);
+/* hex_kernels::~hex_kernels */
+int dgx_hexblock_destroy(dgx_hexblock_handle h);
+
+/* hex_kernels::wJ: the weighted Jacobian wJ [qa][qb][qc] (host memory), one
+ * for the whole block as HexBlock::initialize_gpu passes it (affine hexes),
+ * or with per_element = 1 one per element [ne][qa][qb][qc]. */
+int dgx_hexblock_set_wJ(dgx_hexblock_handle h, const double *wJ, int per_element);
+
+/* hex_kernels::Basis: Ba [nmodes][qa], Bb [nmodes][qb], Bc [nmodes][qc]
+ * (host memory, row-major, the reference's psi->Ba/Bb/Bc). */
+int dgx_hexblock_set_basis(dgx_hexblock_handle h, const double *Ba, const double *Bb, const double *Bc);
+
+/* hex_kernels::inner_product (Hex::inner_product, src/Hex.cpp:703-747):
+ * f *= wJ, then the a, b and c contractions with Ba, Bb, Bc; each slot's
+ * first nmodes^3 values become the inner products and the rest of the slot
+ * is left as it was.  Synchronous for host memory, asynchronous on the
+ * block's stream for device memory. */
+int dgx_hexblock_inner_product(dgx_hexblock_handle h, double *u, int on_device);
+
+/* hex_kernels::backward_transform (Hex::backward_transform,
+ * src/Hex.cpp:796-812): the first nmodes^3 values of each slot (modal) to the
+ * slot's qa*qb*qc values at the points.  Synchronous for host memory,
+ * asynchronous on the block's stream for device memory. */
+int dgx_hexblock_backward_transform(dgx_hexblock_handle h, double *u, int on_device);
+
+/* Wait for the block's stream. */
+int dgx_hexblock_synchronize(dgx_hexblock_handle h);
+
 const char *dgx_last_error(void);
 const char *dgx_version(void);
 

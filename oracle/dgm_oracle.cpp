@@ -45,6 +45,8 @@
 //   src/BaseQuad.hpp:53-57, src/BaseHex.hpp:291-313  sideMap
 //   src/TimeInt.cpp:208-302, 738-858, 1870-2013   RK4, TVD-RK3, LSRK14
 //   src/Quad.cpp:2556-2632, src/Field.cpp:4675-4697  norms
+//   src/GpGpu.cu:236-262, 505-750, src/Hex.cpp:703-812  element-block transforms (pinned by
+//                                                 the GLL mass identity and brute-force sums)
 //
 // Compile with -ffp-contract=off so that every multiply and add rounds
 // separately, as in the reference's plain BLAS-1 loops.
@@ -2121,6 +2123,73 @@ void orc_norms(void *h, double *out) {
     out[3 * v + 0] = Linf;
     out[3 * v + 1] = std::sqrt(std::fabs(L2 / area));
     out[3 * v + 2] = std::sqrt(std::fabs(H1 / area));
+  }
+}
+
+// Element-block transforms (GpGpu::hex_kernels, src/GpGpu.cu:236-262 and
+// 505-750, used by HexBlock::forward_transform / backward_transform,
+// src/HexBlock.cpp:54-130), element by element as Hex::inner_product
+// (src/Hex.cpp:703-747: f *= wJ, H = Ba F, Ha = Bb H per a, fh = Ha Bc^T)
+// and Hex::backward_transform (src/Hex.cpp:796-812: H = Ba^T fh,
+// Ha = Bb^T H per a, f = Ha Bc).  Slots of qa*qb*qc doubles per element, the
+// nm^3 modes first; in place.  wj_stride 0: one wJ for all elements.
+void oracle_hexblock_inner_product(long long ne, int nm, int qa, int qb, int qc, const double *wJ,
+                                   int wj_stride, const double *Ba, const double *Bb, const double *Bc,
+                                   double *u) {
+  const int qt = qa * qb * qc, bc = qb * qc;
+  std::vector<double> f(qt), H((size_t)nm * bc), Ha((size_t)nm * nm * qc);
+  for (long long e = 0; e < ne; ++e) {
+    double *ue = u + e * qt;
+    const double *w = wJ + e * wj_stride;
+    for (int i = 0; i < qt; ++i) f[i] = ue[i] * w[i];
+    for (int i = 0; i < nm; ++i)
+      for (int r = 0; r < bc; ++r) {
+        double s = 0.0;
+        for (int a = 0; a < qa; ++a) s = s + Ba[i * qa + a] * f[a * bc + r];
+        H[i * bc + r] = s;
+      }
+    for (int i = 0; i < nm; ++i)
+      for (int j = 0; j < nm; ++j)
+        for (int c = 0; c < qc; ++c) {
+          double s = 0.0;
+          for (int b = 0; b < qb; ++b) s = s + Bb[j * qb + b] * H[i * bc + b * qc + c];
+          Ha[(i * nm + j) * qc + c] = s;
+        }
+    for (int ij = 0; ij < nm * nm; ++ij)
+      for (int k = 0; k < nm; ++k) {
+        double s = 0.0;
+        for (int c = 0; c < qc; ++c) s = s + Ha[ij * qc + c] * Bc[k * qc + c];
+        ue[ij * nm + k] = s;
+      }
+  }
+}
+
+void oracle_hexblock_backward_transform(long long ne, int nm, int qa, int qb, int qc, const double *Ba,
+                                        const double *Bb, const double *Bc, double *u) {
+  const int qt = qa * qb * qc, n2 = nm * nm, n3 = n2 * nm;
+  std::vector<double> fh(n3), H((size_t)qa * n2), Ha((size_t)qa * qb * nm);
+  for (long long e = 0; e < ne; ++e) {
+    double *ue = u + e * qt;
+    for (int i = 0; i < n3; ++i) fh[i] = ue[i];
+    for (int a = 0; a < qa; ++a)
+      for (int r = 0; r < n2; ++r) {
+        double s = 0.0;
+        for (int i = 0; i < nm; ++i) s = s + Ba[i * qa + a] * fh[i * n2 + r];
+        H[a * n2 + r] = s;
+      }
+    for (int a = 0; a < qa; ++a)
+      for (int b = 0; b < qb; ++b)
+        for (int k = 0; k < nm; ++k) {
+          double s = 0.0;
+          for (int j = 0; j < nm; ++j) s = s + Bb[j * qb + b] * H[a * n2 + j * nm + k];
+          Ha[(a * qb + b) * nm + k] = s;
+        }
+    for (int ab = 0; ab < qa * qb; ++ab)
+      for (int c = 0; c < qc; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < nm; ++k) s = s + Ha[ab * nm + k] * Bc[k * qc + c];
+        ue[ab * qc + c] = s;
+      }
   }
 }
 

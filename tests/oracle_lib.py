@@ -63,6 +63,11 @@ def _load():
     lib.orc_set_channel.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_time_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
     lib.orc_set_time_step.restype = ctypes.c_int
+    lib.oracle_hexblock_inner_product.argtypes = [ctypes.c_longlong] + [ctypes.c_int] * 4 + [_dp, ctypes.c_int] + \
+        [_dp] * 4
+    lib.oracle_hexblock_inner_product.restype = None
+    lib.oracle_hexblock_backward_transform.argtypes = [ctypes.c_longlong] + [ctypes.c_int] * 4 + [_dp] * 4
+    lib.oracle_hexblock_backward_transform.restype = None
     _lib = lib
     return lib
 
@@ -240,3 +245,31 @@ def dgm_diff(oracle_a: Oracle, Uh_ref, Uh_test):
     oracle_a.set_modal(Uh_ref - Uh_test)
     dif = oracle_a.norms()
     return ref, dif
+
+
+def hexblock_inner_product(u, Ba, Bb, Bc, wJ, per_element=False):
+    """The oracle's element-block inner product (GpGpu::hex_kernels /
+    Hex::inner_product), on a copy: u [ne][qa*qb*qc] slots; returns the slots
+    with each element's nm^3 inner products first."""
+    lib = _load()
+    nm, qa = Ba.shape
+    qb, qc = Bb.shape[1], Bc.shape[1]
+    out = np.array(u, dtype=np.float64, order="C", copy=True)
+    wJ = np.ascontiguousarray(wJ, dtype=np.float64)
+    B = [np.ascontiguousarray(x, dtype=np.float64) for x in (Ba, Bb, Bc)]
+    lib.oracle_hexblock_inner_product(out.shape[0], nm, qa, qb, qc, _ptr(wJ), qa * qb * qc if per_element else 0,
+                                      _ptr(B[0]), _ptr(B[1]), _ptr(B[2]), _ptr(out))
+    return out
+
+
+def hexblock_backward_transform(u, Ba, Bb, Bc):
+    """The oracle's element-block backward transform (Hex::backward_transform),
+    on a copy of the slots u [ne][qa*qb*qc]."""
+    lib = _load()
+    nm, qa = Ba.shape
+    qb, qc = Bb.shape[1], Bc.shape[1]
+    out = np.array(u, dtype=np.float64, order="C", copy=True)
+    B = [np.ascontiguousarray(x, dtype=np.float64) for x in (Ba, Bb, Bc)]
+    lib.oracle_hexblock_backward_transform(out.shape[0], nm, qa, qb, qc, _ptr(B[0]), _ptr(B[1]), _ptr(B[2]),
+                                           _ptr(out))
+    return out

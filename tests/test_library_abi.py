@@ -263,3 +263,27 @@ def test_time_entry_points_reject_null(lib):
     t = ctypes.c_double()
     assert lib.dgx_set_time(None, 0.0) == -1
     assert lib.dgx_get_time(None, ctypes.byref(t)) == -1
+
+
+@pytest.mark.parametrize("args,msg", [((0, 3, 4, 4, 4), b"ne must be positive"),
+                                      ((8, 0, 4, 4, 4), b"must be in 1..16"),
+                                      ((8, 3, 17, 4, 4), b"must be in 1..16"),
+                                      ((8, 5, 4, 6, 6), b"must not exceed qa"),
+                                      ((8, 4, 6, 6, 3), b"must not exceed qa")])
+def test_hexblock_create_rejects_bad_sizes(lib, args, msg):
+    h = ctypes.c_void_p()
+    assert lib.dgx_hexblock_create(*args, None, ctypes.byref(h)) == -1
+    assert msg in lib.dgx_last_error()
+    assert not h.value
+
+
+def test_hexblock_entry_points_reject_null(lib):
+    x = np.zeros(4)
+    p = x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.dgx_hexblock_create(8, 3, 4, 4, 4, None, None) == -1
+    assert lib.dgx_hexblock_set_wJ(None, p, 0) == -1
+    assert lib.dgx_hexblock_set_basis(None, p, p, p) == -1
+    assert lib.dgx_hexblock_inner_product(None, p, 0) == -1
+    assert lib.dgx_hexblock_backward_transform(None, p, 1) == -1
+    assert lib.dgx_hexblock_synchronize(None) == -1
+    assert lib.dgx_hexblock_destroy(None) == 0
