@@ -24,9 +24,12 @@ Also printed in the same JSON line:
                 sum-factorized operator): 787 in all, 256 of them in the inviscid
                 residual kernel, the other 531 in the BR1 viscous kernel; BR2's
                 face pass adds 85 to the viscous kernel.
-                `traffic` is the kernel's HBM bytes per launch measured with
-                rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of
-                known size).
+                `bound` is "valu" for the FP64 kernels (every flop on the FP64
+                vector pipe) and "mfma" for the mixed ones.  `traffic` is the
+                kernel's HBM bytes per launch measured with rocprofv3 FETCH_SIZE
+                and WRITE_SIZE (scaled by a plain copy of known size) in a
+                separate profiling run of this workload: a fixed calibration,
+                not read from this run's counters.
   roofline_hbm  the HBM view of a whole stage: 32 B per DOF-stage (read U,
                 read Ut, write Ut, write U of a fused 2N-storage stage) / the
                 mean stage time, against 8 TB/s
@@ -385,9 +388,14 @@ def run_path(m, n, p, mu, precision, args, flux=None, viscous=None, compare=True
         algo_flops = flops * dofs
         achieved = algo_flops / mean_s / 1e12
         peak = FP64_PEAK_TFLOPS if precision == dgx.DGX_FP64 else FP32_MFMA_PEAK_TFLOPS
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak,
+        # the FP64 kernels run every flop on the FP64 vector pipe (no MFMA); the
+        # mixed kernels run their contractions on MFMA
+        roofline = {"bound": "valu" if precision == dgx.DGX_FP64 else "mfma", "achieved": achieved, "peak": peak,
+                    "unit": "TFLOP/s", "frac": achieved / peak,
                     "traffic": MEASURED_TRAFFIC.get((name, n, p, dom, flux, viscous)),
+                    "traffic_source": "fixed calibration: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this "
+                                      "workload, scaled by a plain copy (profiles/tgv_ns3d_32cubed_p7_*.md), "
+                                      "not measured in this run",
                     "kernel": dom, "arithmetic": ARITHMETIC[name],
                     "algorithmic_flops_per_launch": algo_flops, "ms_mean": kern[dom]["ms_mean"]}
     stage_s = sum(kern[k]["ms_total"] for k in kern) / 1e3 / (14 * prof_steps)
