@@ -96,6 +96,28 @@ def read_mesh(root: str) -> Mesh:
     return Mesh(nsd=vert.shape[1], vert=vert, con=con)
 
 
+def read_ord(path: str, ne: int, p: int) -> np.ndarray:
+    """Per-element polynomial order from ``root.ord`` (Field::Field,
+    src/Field.cpp:2290-2320): a title line, then rows ``eid p q``; elements
+    not listed keep the default order ``p``.  A nonzero ``q`` sets the element's
+    quadrature points; only the default q = p + 2 is supported here."""
+    orders = np.full(ne, int(p), dtype=np.int32)
+    with open(path) as f:
+        f.readline()
+        for line in f:
+            toks = line.split()
+            if len(toks) < 3:
+                continue
+            e, pl, ql = int(toks[0]), int(toks[1]), int(toks[2])
+            if e < 0 or e >= ne:
+                raise ValueError(f"{path}: illegal element number {e}")
+            if ql not in (0, pl + 2):
+                raise ValueError(f"{path}: element {e} asks for q = {ql} quadrature points; "
+                                 f"only the default q = p + 2 is supported")
+            orders[e] = pl
+    return orders
+
+
 def read_bc(path: str) -> List[tuple]:
     """Boundary sides of ``root.bc``: rows (tag, eid, sid) after the count line
     (Field::read_bc, src/Field.cpp:4469-4495).  A side listed here is a
@@ -439,7 +461,8 @@ _TYPES = {"int32": "i", "int64": "q", "uint32": "I", "uint64": "Q", "float64": "
 class Restart:
     header: Dict[str, object]
     nmodes: np.ndarray           # per-element modal counts
-    data: np.ndarray             # [field][e][nmodes] (uniform order) as float64
+    data: np.ndarray             # [field][e][max nmodes] as float64; with variable order each
+                                 # element's nmodes[e] modes come first, zeros after
 
 
 def read_restart(path: str, ne: int) -> Restart:
@@ -489,20 +512,44 @@ def read_restart(path: str, ne: int) -> Restart:
         off += tot * ssz
         if nmodes is None:
             nmodes = nm
-        if not np.all(nm == nm[0]):
-            raise ValueError(f"{path}: variable-order restart files are not supported")
-        fields.append(vals.reshape(ne, int(nm[0])))
+        if np.all(nm == nm[0]):
+            fields.append(vals.reshape(ne, int(nm[0])))
+        else:   # variable order: each element's modes first in a slot of the largest count
+            pad = np.zeros((ne, int(nm.max())))
+            starts = np.concatenate([[0], np.cumsum(nm)[:-1]])
+            for e in range(ne):
+                pad[e, :nm[e]] = vals[starts[e]:starts[e] + nm[e]]
+            fields.append(pad)
     if off != len(raw):
         raise ValueError(f"{path}: {len(raw) - off} trailing bytes after {nf} fields")
     return Restart(header=header, nmodes=nmodes, data=np.stack(fields))
 
 
-def write_restart(path: str, data: np.ndarray, header: Dict[str, object]) -> None:
+def write_restart(path: str, data: np.ndarray, header: Dict[str, object], nmodes=None) -> None:
     """Write a JSON-header restart file (int32 sizes, float64 modes) readable by DGM.
 
-    ``data`` is ``[field][e][nmodes]``.
+    ``data`` is ``[field][e][nmodes]``; with variable order ``nmodes[e]`` gives
+    each element's count, its modes first in the slot.
     """
     nf, ne, nm = data.shape
+    if nmodes is not None and not np.all(np.asarray(nmodes) == nm):
+        nmodes = np.asarray(nmodes, dtype=np.int64)
+        hdr = dict(header)
+        hdr.setdefault("Basis type", "Normalized")
+        hdr["Nel"] = ne
+        hdr["Number of fields"] = nf
+        hdr["Types"] = {"ByteOrder": "LittleEndian", "GlobalSize": "int32",
+                        "LocalSize": "int32", "Scalar": "float64"}
+        disp = np.concatenate([[0], np.cumsum(nmodes)[:-1]])
+        with open(path, "wb") as f:
+            f.write((json.dumps(hdr, indent=3, sort_keys=True) + "\n").encode())
+            for v in range(nf):
+                f.write(struct.pack("<i", int(nmodes.sum())))
+                f.write(nmodes.astype("<i4").tobytes())
+                f.write(disp.astype("<i4").tobytes())
+                for e in range(ne):
+                    f.write(np.ascontiguousarray(data[v, e, :nmodes[e]], dtype="<f8").tobytes())
+        return
     hdr = dict(header)
     hdr.setdefault("Basis type", "Normalized")
     hdr["Nel"] = ne

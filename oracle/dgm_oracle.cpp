@@ -412,6 +412,19 @@ int point_flux_shallow(const double Ul[3], const double Ur[3], double nxl, doubl
 
 enum Eqn { EULER2D = 1, NS2D = 2, LINEULER2D = 4, EULER3D = 8, NS3D = 9, SHALLOW2D = 30 };
 
+// LSRK14 coefficients (src/TimeInt.cpp:1870-1917)
+const double kLsrkA[15] = {0.0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
+  -0.8552979934029281, -3.9564138245774565, -1.5780575380587385, -2.0837094552574054,
+  -0.7483334182761610, -0.7032861106563359, 0.0013917096117681, -0.0932075369637460,
+  -0.9514200470875948, -7.1151571693922548, 0.0};
+const double kLsrkB[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
+  0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
+  0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
+  0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
+const double kLsrkC[14] = {0.0, 0.0367762454319673, 0.1249685262725025, 0.2446177702277698,
+  0.2476149531070420, 0.2969311120382472, 0.3978149645802642, 0.5270854589440328, 0.6981269994175695,
+  0.8190890835352128, 0.8527059887098624, 0.8604711817462826, 0.8627060376969976, 0.8734213127600976};
+
 struct SideInfo {
   bool master = false;       // side->link set (src/SideLoop.hpp:46-50)
   int le = -1, ls = -1;      // linked element/side (both directions)
@@ -1904,23 +1917,266 @@ struct Domain {
   // LSRK14::advance (src/TimeInt.cpp:1989-2013); coefficients from
   // src/TimeInt.cpp:1870-1917
   void lsrk14(double dt) {
-    static const double A[15] = {0.0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
-      -0.8552979934029281, -3.9564138245774565, -1.5780575380587385, -2.0837094552574054,
-      -0.7483334182761610, -0.7032861106563359, 0.0013917096117681, -0.0932075369637460,
-      -0.9514200470875948, -7.1151571693922548, 0.0};
-    static const double Bc[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
-      0.0030097086818182, 0.3326293790646110, 0.2440251405350864, 0.3718879239592277,
-      0.6204126221582444, 0.1524043173028741, 0.0760894927419266, 0.0077604214040978,
-      0.0024647284755382, 0.0780348340049386, 5.5059777270269628};
-    static const double C[14] = {0.0, 0.0367762454319673, 0.1249685262725025, 0.2446177702277698,
-      0.2476149531070420, 0.2969311120382472, 0.3978149645802642, 0.5270854589440328, 0.6981269994175695,
-      0.8190890835352128, 0.8527059887098624, 0.8604711817462826, 0.8627060376969976, 0.8734213127600976};
     const double tn = t;
     for (int s = 0; s < 14; ++s) {
-      t = tn + C[s] * dt;
+      t = tn + kLsrkC[s] * dt;
       time_derivative(U, R0);
-      axpby(-dt, R0, A[s], Ut, Ut);
-      axpy2(Bc[s], Ut, U);
+      axpby(-dt, R0, kLsrkA[s], Ut, Ut);
+      axpy2(kLsrkB[s], Ut, U);
+    }
+    t = tn + dt;
+  }
+};
+
+// Variable-order quads (runs/vortex): 2D Euler with an order per element, as
+// the reference builds it from root.ord (Field::Field, src/Field.cpp:2290-2320).
+// Every element keeps its own basis (q_e = p_e + 2 GLL points); the two sides
+// of a link share max(L_left, L_right) Gauss points (Side::match,
+// src/Side.cpp:106-117, called by Field::adjust_side_order,
+// src/Field.cpp:3208-3230).  fill_sides interpolates the element's GLL edge
+// values to the side's Gauss points (Quad::fill_sides, src/Quad.cpp:845-869:
+// GL2G of size qs x q_e); the lift interpolates back with G2GL (q_e x qs) and
+// scales by the element's end weight (Quad::local_lift, src/Quad.cpp:1971-2038:
+// the reference's default build has no DGM_SIDE_FLUX_TO_VOLUME, so var_p_lift
+// is never taken).  Affine quads; the per-element work reuses one uniform-order
+// Domain per order present (same mesh, same links).
+struct VarDomain2 {
+  int ne = 0, nv = 4, pmax = 0, Lmax = 0, qmax = 0;
+  double gamma = 1.4;
+  int status = 0;
+  std::vector<int> pe;                           // order of each element
+  std::vector<Domain *> dom;                     // [p] uniform-order domain (nullptr if absent)
+  std::vector<size_t> off;                       // nodal offset of element e within a field
+  size_t ntot = 0;                               // nodes per field
+  std::vector<int> qs;                           // [e*4 + s] Gauss points of the side
+  std::vector<std::vector<double>> gl2g, g2gl;   // [e*4 + s] GL2G (qs x q_e), G2GL (q_e x qs)
+  VF U, Ut, R0, R1, Sf;                          // flat nodal [v][ntot]; sides in .s of Us/Uf
+  double t = 0.0;
+
+  ~VarDomain2() { for (Domain *d : dom) delete d; }
+  Domain &D(int e) const { return *dom[pe[e]]; }
+  int q_of(int e) const { return pe[e] + 2; }
+  double *nod(VF &F, int v, int e) { return &F.u[(size_t)v * ntot + off[e]]; }
+  double *sid(VF &F, int v, int e, int s) { return &F.s[(((size_t)v * ne + e) * 4 + s) * Lmax]; }
+  void alloc(VF &F) {
+    F.u.assign((size_t)nv * ntot, 0.0);
+    F.s.assign((size_t)nv * ne * 4 * Lmax, 0.0);
+  }
+
+  void build(int ne_, const int *orders, const double *vert, int ncon, const int *con, double g, int flux) {
+    ne = ne_; gamma = g;
+    pe.assign(orders, orders + ne);
+    pmax = *std::max_element(pe.begin(), pe.end());
+    Lmax = pmax + 1; qmax = pmax + 2;
+    dom.assign(pmax + 1, nullptr);
+    for (int e = 0; e < ne; ++e) {
+      const int p = pe[e];
+      if (dom[p]) continue;
+      Domain *d = new Domain();
+      d->nsd = 2; d->ne = ne; d->p = p; d->eqn = 1; d->gamma = g; d->vis = 0.0; d->Pr = 0.72;
+      d->flux_type = flux; d->nv = 4;
+      d->build_basis();
+      d->build_geometry(vert);
+      d->build_links(ncon, con, vert);
+      dom[p] = d;
+    }
+    off.assign(ne, 0);
+    ntot = 0;
+    for (int e = 0; e < ne; ++e) { off[e] = ntot; ntot += (size_t)q_of(e) * q_of(e); }
+    // side point counts and interpolation matrices
+    qs.assign(4 * (size_t)ne, 0);
+    gl2g.assign(4 * (size_t)ne, {});
+    g2gl.assign(4 * (size_t)ne, {});
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < 4; ++s) {
+        const SideInfo &S = D(e).el[e].side[s];
+        int n = pe[e] + 1;
+        if (S.bc < 0 && S.le >= 0) n = std::max(n, pe[S.le] + 1);
+        qs[4 * e + s] = n;
+        std::vector<double> zg, wg;
+        zwgl(n, zg, wg);
+        interp_matrix(D(e).z, zg, gl2g[4 * e + s]);   // n x q_e
+        interp_matrix(zg, D(e).z, g2gl[4 * e + s]);   // q_e x n
+      }
+    alloc(U); alloc(Ut); alloc(R0); alloc(R1); alloc(Sf);
+  }
+
+  void fill_sides(VF &F) {
+    std::vector<double> wk(qmax);
+    for (int v = 0; v < nv; ++v)
+      for (int e = 0; e < ne; ++e) {
+        const int q = q_of(e);
+        for (int s = 0; s < 4; ++s) {
+          D(e).get_side(nod(F, v, e), s, wk.data());
+          mm(qs[4 * e + s], 1, q, gl2g[4 * e + s].data(), q, wk.data(), 1, sid(F, v, e, s), 1);
+        }
+      }
+  }
+
+  // Euler::convection volume term of one element (src/Euler.cpp:296-390)
+  void volume(VF &Us, VF &Uf, int e) {
+    const Domain &Dm = D(e);
+    const int qt = q_of(e) * q_of(e);
+    const double gm1 = gamma - 1.0, mpt5gm1 = -0.5 * gm1;
+    std::vector<double> u(qt), v(qt), p_(qt), d(qt), d1(qt);
+    const double *r = nod(Us, 0, e), *ru = nod(Us, 1, e), *rv = nod(Us, 2, e), *rE = nod(Us, 3, e);
+    double *F0 = nod(Uf, 0, e), *F1 = nod(Uf, 1, e), *F2 = nod(Uf, 2, e), *F3 = nod(Uf, 3, e);
+    for (int i = 0; i < qt; ++i) { u[i] = ru[i] / r[i]; v[i] = rv[i] / r[i]; }
+    for (int i = 0; i < qt; ++i) {
+      p_[i] = u[i] * ru[i]; d[i] = v[i] * rv[i]; p_[i] = p_[i] + d[i];
+      p_[i] = mpt5gm1 * p_[i]; p_[i] = gm1 * rE[i] + p_[i];
+    }
+    Dm.gradient_dir(e, ru, F0, 'x');
+    Dm.gradient_dir(e, rv, d.data(), 'y');
+    for (int i = 0; i < qt; ++i) F0[i] = F0[i] + d[i];
+    for (int i = 0; i < qt; ++i) { d[i] = ru[i] * u[i]; d[i] = d[i] + p_[i]; }
+    Dm.gradient_dir(e, d.data(), F1, 'x');
+    for (int i = 0; i < qt; ++i) d[i] = ru[i] * v[i];
+    Dm.gradient_dir(e, d.data(), d1.data(), 'y');
+    for (int i = 0; i < qt; ++i) F1[i] = F1[i] + d1[i];
+    for (int i = 0; i < qt; ++i) d[i] = rv[i] * u[i];
+    Dm.gradient_dir(e, d.data(), F2, 'x');
+    for (int i = 0; i < qt; ++i) { d[i] = rv[i] * v[i]; d[i] = d[i] + p_[i]; }
+    Dm.gradient_dir(e, d.data(), d1.data(), 'y');
+    for (int i = 0; i < qt; ++i) F2[i] = F2[i] + d1[i];
+    for (int i = 0; i < qt; ++i) p_[i] = p_[i] + rE[i];
+    for (int i = 0; i < qt; ++i) d[i] = u[i] * p_[i];
+    Dm.gradient_dir(e, d.data(), F3, 'x');
+    for (int i = 0; i < qt; ++i) d[i] = v[i] * p_[i];
+    Dm.gradient_dir(e, d.data(), d1.data(), 'y');
+    for (int i = 0; i < qt; ++i) F3[i] = F3[i] + d1[i];
+  }
+
+  // master-side Riemann flux on the shared Gauss points (src/Euler.cpp:399-757);
+  // a flipped side reverses the order of its qs points (sideMap,
+  // src/BaseQuad.hpp:53-57)
+  void faces(VF &Us, VF &Uf) {
+    for (int e = 0; e < ne; ++e)
+      for (int s = 0; s < 4; ++s) {
+        const SideInfo &S = D(e).el[e].side[s];
+        const int n = qs[4 * e + s];
+        if (S.bc >= 0) {
+          if (S.bc > 1) status |= 2;
+          for (int g = 0; g < n; ++g) {
+            double Ul[4], Ub[4], Fpl[4], Fmr[4], Fh[4];
+            for (int c = 0; c < 4; ++c) Ul[c] = sid(Us, c, e, s)[g];
+            D(e).ghost2d(S.bc, Ul, S.nx[0], S.ny[0], Ub);
+            status |= point_flux2d(gamma, D(e).flux_type, Ul, Ub, S.nx[0], S.ny[0], Fpl, Fmr, Fh);
+            for (int c = 0; c < 4; ++c) { sid(Us, c, e, s)[g] = Fpl[c]; sid(Uf, c, e, s)[g] = Fh[c]; }
+          }
+          continue;
+        }
+        if (!S.master) continue;
+        for (int g = 0; g < n; ++g) {
+          const int ql = S.flip_a ? (n - 1) - g : g, qr = g;
+          const double nxl = S.nx[0], nyl = S.ny[0];   // affine: constant along the side
+          double Ul[4], Ur[4], Fpl[4], Fmr[4], Fh[4];
+          for (int c = 0; c < 4; ++c) Ul[c] = sid(Us, c, e, s)[ql];
+          for (int c = 0; c < 4; ++c) Ur[c] = sid(Us, c, S.le, S.ls)[qr];
+          status |= point_flux2d(gamma, D(e).flux_type, Ul, Ur, nxl, nyl, Fpl, Fmr, Fh);
+          for (int c = 0; c < 4; ++c) sid(Us, c, e, s)[ql] = Fpl[c];
+          for (int c = 0; c < 4; ++c) sid(Us, c, S.le, S.ls)[qr] = Fmr[c];
+          for (int c = 0; c < 4; ++c) {
+            sid(Us, c, S.le, S.ls)[qr] *= -1.0;
+            sid(Uf, c, e, s)[ql] = Fh[c];
+            sid(Uf, c, S.le, S.ls)[qr] = -Fh[c];
+          }
+        }
+      }
+  }
+
+  // Quad::lift / local_lift with G2GL (q_e x qs) (src/Quad.cpp:1971-2038)
+  void lift(VF &Us, VF &Uf, int v, int e) {
+    const Domain &Dm = D(e);
+    const int q = q_of(e);
+    double *in = nod(Uf, v, e);
+    std::vector<double> f(Lmax), fi(q);
+    for (int s = 0; s < 4; ++s) {
+      const SideInfo &S = Dm.el[e].side[s];
+      const int n = qs[4 * e + s];
+      const double *fu = sid(Uf, v, e, s), *uu = sid(Us, v, e, s);
+      for (int g = 0; g < n; ++g) f[g] = fu[g] - uu[g];
+      for (int g = 0; g < n; ++g) f[g] = S.J[0] * f[g];
+      mm(q, 1, n, g2gl[4 * e + s].data(), n, f.data(), 1, fi.data(), 1);
+      double W;
+      switch (s) {
+        case 0: W = 1.0 / Dm.w[0];
+          for (int i = 0; i < q; ++i) fi[i] = W * fi[i];
+          for (int i = 0; i < q; ++i) in[i * q + 0] += fi[i];
+          break;
+        case 1: W = 1.0 / Dm.w[q - 1];
+          for (int j = 0; j < q; ++j) fi[j] = W * fi[j];
+          for (int j = 0; j < q; ++j) in[(q - 1) * q + j] += fi[j];
+          break;
+        case 2: W = 1.0 / Dm.w[q - 1];
+          for (int i = 0; i < q; ++i) fi[i] = W * fi[i];
+          for (int i = 0; i < q; ++i) in[i * q + (q - 1)] += fi[i];
+          break;
+        case 3: W = 1.0 / Dm.w[0];
+          for (int j = 0; j < q; ++j) fi[j] = W * fi[j];
+          for (int j = 0; j < q; ++j) in[0 * q + j] += fi[j];
+          break;
+      }
+    }
+  }
+
+  void project_all(VF &F) {
+    for (int v = 0; v < nv; ++v)
+      for (int e = 0; e < ne; ++e) D(e).project_vec(e, nod(F, v, e));
+  }
+
+  // Euler::time_derivative: F = -dU/dt (src/Euler.cpp:282-287)
+  void time_derivative(VF &Us, VF &Uf) {
+    fill_sides(Us);
+    for (int e = 0; e < ne; ++e) volume(Us, Uf, e);
+    faces(Us, Uf);
+    for (int e = 0; e < ne; ++e)
+      for (int v = 0; v < nv; ++v) lift(Us, Uf, v, e);
+    project_all(Uf);
+  }
+
+  // the steppers of Domain (RungeKutta, TVD, LSRK14) on the flat nodal arrays
+  void rk4(double dt) {
+    static const double a[3] = {0.5, 0.5, 1.0};
+    static const double b[4] = {0.16666666666666667, 0.33333333333333333, 0.33333333333333333, 0.16666666666666667};
+    time_derivative(U, R0);
+    Domain::axpy(-a[0] * dt, R0, U, Ut);
+    Domain::scal(-b[0], R0);
+    t += 0.5 * dt;
+    time_derivative(Ut, R1);
+    Domain::axpy(-a[1] * dt, R1, U, Ut);
+    Domain::axpy2(-b[1], R1, R0);
+    time_derivative(Ut, R1);
+    Domain::axpy(-a[2] * dt, R1, U, Ut);
+    Domain::axpy2(-b[2], R1, R0);
+    t += 0.5 * dt;
+    time_derivative(Ut, R1);
+    Domain::axpy2(-b[3], R1, R0);
+    Domain::axpy2(dt, R0, U);
+  }
+  void tvd(double dt) {
+    static const double a[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+    static const double b[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    const double tn = t;
+    time_derivative(U, R0);
+    Domain::axpy(-dt, R0, U, Ut);
+    t = tn + dt;
+    time_derivative(Ut, R0);
+    Domain::axpy(-dt, R0, Ut, R0);
+    Domain::axpby(a[1], U, b[1], R0, Ut);
+    t = tn + 0.5 * dt;
+    time_derivative(Ut, R0);
+    Domain::axpy(-dt, R0, Ut, R0);
+    Domain::axpby(b[2], R0, a[2], U, U);
+    t = tn + dt;
+  }
+  void lsrk14(double dt) {
+    const double tn = t;
+    for (int s = 0; s < 14; ++s) {
+      t = tn + kLsrkC[s] * dt;
+      time_derivative(U, R0);
+      Domain::axpby(-dt, R0, kLsrkA[s], Ut, Ut);
+      Domain::axpy2(kLsrkB[s], Ut, U);
     }
     t = tn + dt;
   }
@@ -2190,6 +2446,116 @@ void oracle_hexblock_backward_transform(long long ne, int nm, int qa, int qb, in
         for (int k = 0; k < nm; ++k) s = s + Ha[ab * nm + k] * Bc[k * qc + c];
         ue[ab * qc + c] = s;
       }
+  }
+}
+
+// Variable-order quads (VarDomain2).  Arrays are padded per element: nodal
+// [v][e][qmax^2] with element e's q_e^2 nodes first, modal [v][e][Lmax^2]
+// with its L_e^2 modes first (the rest zero on output, ignored on input).
+void *orc_varp_create(int ne, const int *orders, const double *vert, int ncon, const int *con, double gamma,
+                      int flux) {
+  VarDomain2 *d = new VarDomain2();
+  d->build(ne, orders, vert, ncon, con, gamma, flux);
+  return d;
+}
+void orc_varp_destroy(void *h) { delete (VarDomain2 *)h; }
+int orc_varp_qmax(void *h) { return ((VarDomain2 *)h)->qmax; }
+static void varp_pad_out(VarDomain2 *d, const VF &F, double *out, int nv) {
+  const size_t Q2 = (size_t)d->qmax * d->qmax;
+  std::fill(out, out + (size_t)nv * d->ne * Q2, 0.0);
+  for (int v = 0; v < nv; ++v)
+    for (int e = 0; e < d->ne; ++e) {
+      const size_t n = (size_t)d->q_of(e) * d->q_of(e);
+      std::memcpy(out + ((size_t)v * d->ne + e) * Q2, &F.u[(size_t)v * d->ntot + d->off[e]], n * sizeof(double));
+    }
+}
+static void varp_pad_in(VarDomain2 *d, const double *in, VF &F, int nv) {
+  const size_t Q2 = (size_t)d->qmax * d->qmax;
+  for (int v = 0; v < nv; ++v)
+    for (int e = 0; e < d->ne; ++e) {
+      const size_t n = (size_t)d->q_of(e) * d->q_of(e);
+      std::memcpy(&F.u[(size_t)v * d->ntot + d->off[e]], in + ((size_t)v * d->ne + e) * Q2, n * sizeof(double));
+    }
+}
+void orc_varp_coords(void *h, double *x) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  VF c;
+  c.u.assign(2 * d->ntot, 0.0);
+  for (int e = 0; e < d->ne; ++e) {
+    const Elem &E = d->D(e).el[e];
+    std::memcpy(&c.u[d->off[e]], E.cx.data(), E.cx.size() * sizeof(double));
+    std::memcpy(&c.u[d->ntot + d->off[e]], E.cy.data(), E.cy.size() * sizeof(double));
+  }
+  varp_pad_out(d, c, x, 2);
+}
+void orc_varp_set_state(void *h, const double *U) { VarDomain2 *d = (VarDomain2 *)h; varp_pad_in(d, U, d->U, 4); }
+void orc_varp_get_state(void *h, double *U) { VarDomain2 *d = (VarDomain2 *)h; varp_pad_out(d, d->U, U, 4); }
+void orc_varp_project_state(void *h) { VarDomain2 *d = (VarDomain2 *)h; d->project_all(d->U); }
+void orc_varp_modal(void *h, double *Uh) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  const size_t L2 = (size_t)d->Lmax * d->Lmax;
+  std::fill(Uh, Uh + 4 * d->ne * L2, 0.0);
+  std::vector<double> f((size_t)d->qmax * d->qmax);
+  for (int v = 0; v < 4; ++v)
+    for (int e = 0; e < d->ne; ++e) {
+      const int q = d->q_of(e);
+      std::memcpy(f.data(), d->nod(d->U, v, e), (size_t)q * q * sizeof(double));
+      d->D(e).forward(e, f.data(), Uh + ((size_t)v * d->ne + e) * L2);
+    }
+}
+void orc_varp_set_modal(void *h, const double *Uh) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  const size_t L2 = (size_t)d->Lmax * d->Lmax;
+  for (int v = 0; v < 4; ++v)
+    for (int e = 0; e < d->ne; ++e) d->D(e).backward(Uh + ((size_t)v * d->ne + e) * L2, d->nod(d->U, v, e));
+}
+int orc_varp_rhs(void *h, const double *U, double *F) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  d->status = 0;
+  varp_pad_in(d, U, d->Ut, 4);
+  d->time_derivative(d->Ut, d->R0);
+  varp_pad_out(d, d->R0, F, 4);
+  return d->status;
+}
+int orc_varp_advance(void *h, int inttype, double dt, int nsteps) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  d->status = 0;
+  for (int s = 0; s < nsteps && !d->status; ++s) {
+    if (inttype == 4) d->rk4(dt);
+    else if (inttype == 3) d->tvd(dt);
+    else if (inttype == 14) d->lsrk14(dt);
+    else return -1;
+  }
+  return d->status;
+}
+
+// Linf, L2, H1 per field of the current state, as orc_norms with each
+// element's own basis
+void orc_varp_norms(void *h, double *out) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  for (int v = 0; v < 4; ++v) {
+    double Linf = 0, L2 = 0, H1 = 0, area = 0;
+    for (int e = 0; e < d->ne; ++e) {
+      const Domain &Dm = d->D(e);
+      const int n = d->q_of(e) * d->q_of(e);
+      std::vector<double> fx(n), fy(n);
+      const double *u = d->nod(d->U, v, e);
+      const std::vector<double> &b = Dm.el[e].wJ;
+      double l2 = 0, ar = 0, h1 = 0, li = 0;
+      for (int i = 0; i < n; ++i) { l2 += b[i] * (u[i] * u[i]); ar += b[i]; li = std::max(li, std::fabs(u[i])); }
+      Dm.gradient_all(e, u, fx.data(), fy.data(), nullptr);
+      for (int i = 0; i < n; ++i) {
+        double g = fx[i] * fx[i];
+        g = fy[i] * fy[i] + g;
+        double f = u[i] * u[i];
+        f = f + g;
+        h1 += b[i] * f;
+      }
+      Linf = std::max(Linf, li); L2 += l2; H1 += h1; area += ar;
+    }
+    out[3 * v + 0] = Linf;
+    out[3 * v + 1] = std::sqrt(std::fabs(L2 / area));
+    out[3 * v + 2] = std::sqrt(std::fabs(H1 / area));
   }
 }
 

@@ -6,7 +6,7 @@ import os
 import numpy as np
 
 from dgm_amd import formats, ic, mesh as meshgen
-from oracle_lib import Oracle
+from oracle_lib import Oracle, VarOracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -55,11 +55,18 @@ def case_setup(root, prm, mesh):
 
 def run_oracle_case(root, nsteps=None):
     """Run the case as Domain::solve does (restart writes every Ntout steps
-    project the state, src/Domain.cpp:898-947) and return (oracle, modal state)."""
+    project the state, src/Domain.cpp:898-947) and return (oracle, modal state).
+    A case with a root.ord runs on the oracle's variable-order quads (Euler;
+    Navier-Stokes only without viscosity, whose diffusion term is then zero)."""
     prm, mesh, icd = load_case(root)
     p = int(prm["p"])
     mesh, bc_fn, src_fn, xn = case_setup(root, prm, mesh)
-    o = Oracle(mesh, p, int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    if os.path.exists(os.path.join(GOLDEN, root + ".ord")):
+        assert int(prm["eqntype"]) in (1, 2) and float(prm.get("vis", 0.0)) == 0.0 and xn is None
+        orders = formats.read_ord(os.path.join(GOLDEN, root + ".ord"), mesh.ne, p)
+        o = VarOracle(mesh, orders)
+    else:
+        o = Oracle(mesh, p, int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
     if bc_fn is not None or src_fn is not None:
         o.set_functions(bc=bc_fn, src=src_fn)
     if xn is not None:

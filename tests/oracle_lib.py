@@ -63,6 +63,22 @@ def _load():
     lib.orc_set_channel.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_time_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
     lib.orc_set_time_step.restype = ctypes.c_int
+    lib.orc_varp_create.restype = ctypes.c_void_p
+    lib.orc_varp_create.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int]
+    lib.orc_varp_destroy.argtypes = [ctypes.c_void_p]
+    lib.orc_varp_destroy.restype = None
+    lib.orc_varp_qmax.argtypes = [ctypes.c_void_p]
+    lib.orc_varp_qmax.restype = ctypes.c_int
+    for name in ("orc_varp_coords", "orc_varp_set_state", "orc_varp_get_state", "orc_varp_modal",
+                 "orc_varp_set_modal", "orc_varp_norms"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, _dp]
+        getattr(lib, name).restype = None
+    lib.orc_varp_project_state.argtypes = [ctypes.c_void_p]
+    lib.orc_varp_project_state.restype = None
+    lib.orc_varp_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.orc_varp_rhs.restype = ctypes.c_int
+    lib.orc_varp_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.orc_varp_advance.restype = ctypes.c_int
     lib.oracle_hexblock_inner_product.argtypes = [ctypes.c_longlong] + [ctypes.c_int] * 4 + [_dp, ctypes.c_int] + \
         [_dp] * 4
     lib.oracle_hexblock_inner_product.restype = None
@@ -222,6 +238,79 @@ class Oracle:
         D, B = np.zeros((q, q)), np.zeros((L, q))
         self.lib.orc_basis(self.h, _ptr(z), _ptr(w), _ptr(zg), _ptr(wg), _ptr(D), _ptr(B))
         return dict(z=z, w=w, zg=zg, wg=wg, D=D, B=B)
+
+
+class VarOracle:
+    """The oracle's variable-order quads (2D Euler, affine): one order per
+    element (root.ord).  Arrays are padded per element: nodal [4][ne][qmax^2]
+    with element e's (p_e+2)^2 nodes first, modal [4][ne][(pmax+1)^2] with its
+    (p_e+1)^2 modes first; the rest is zero."""
+
+    def __init__(self, mesh, orders, gamma=1.4, flux=0):
+        self.lib = _load()
+        if mesh.nsd != 2:
+            raise ValueError("variable order: quads only")
+        self.nsd, self.ne, self.nv = 2, mesh.ne, 4
+        self.orders = np.ascontiguousarray(orders, dtype=np.int32)
+        self.p = int(self.orders.max())
+        self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
+        self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
+        self.h = self.lib.orc_varp_create(self.ne, _ptr(self.orders, _ip), _ptr(self._vert), len(self._con),
+                                          _ptr(self._con, _ip), gamma, flux)
+        self.qtot = (self.p + 2) ** 2
+        self.nm = (self.p + 1) ** 2
+
+    def __del__(self):
+        try:
+            self.lib.orc_varp_destroy(self.h)
+        except Exception:
+            pass
+
+    def coords(self):
+        x = np.zeros((2, self.ne, self.qtot))
+        self.lib.orc_varp_coords(self.h, _ptr(x))
+        return x
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (4, self.ne, self.qtot), U.shape
+        self.lib.orc_varp_set_state(self.h, _ptr(U))
+
+    def state(self):
+        U = np.zeros((4, self.ne, self.qtot))
+        self.lib.orc_varp_get_state(self.h, _ptr(U))
+        return U
+
+    def project_state(self):
+        self.lib.orc_varp_project_state(self.h)
+
+    def modal(self):
+        Uh = np.zeros((4, self.ne, self.nm))
+        self.lib.orc_varp_modal(self.h, _ptr(Uh))
+        return Uh
+
+    def set_modal(self, Uh):
+        Uh = np.ascontiguousarray(Uh, dtype=np.float64)
+        assert Uh.shape == (4, self.ne, self.nm), Uh.shape
+        self.lib.orc_varp_set_modal(self.h, _ptr(Uh))
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (4, self.ne, self.qtot), U.shape
+        F = np.zeros_like(U)
+        if self.lib.orc_varp_rhs(self.h, _ptr(U), _ptr(F)):
+            raise FloatingPointError("oracle: non-realizable state")
+        return F
+
+    def advance(self, inttype, dt, nsteps):
+        st = self.lib.orc_varp_advance(self.h, inttype, dt, nsteps)
+        if st:
+            raise FloatingPointError(f"oracle advance failed ({st})")
+
+    def norms(self):
+        out = np.zeros((4, 3))
+        self.lib.orc_varp_norms(self.h, _ptr(out))
+        return out
 
 
 def riemann(nsd, flux, UL, UR, n, gamma=1.4):

@@ -8,6 +8,8 @@ import pytest
 
 from dgm_amd import formats, mesh as meshgen
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -92,3 +94,21 @@ def test_quad_nodes_match_oracle(p):
     x = meshgen.quad_nodes(m, p)
     ref = Oracle(m, p, 1).coords()
     assert np.allclose(x, ref, rtol=0, atol=1e-13)
+
+
+def test_variable_order_restart_round_trip(tmp_path):
+    """A variable-order restart (per-element nModes, as runs/vortex.ref) reads
+    into padded slots and writes back to the same bytes of data."""
+    ref = formats.read_restart(f"{GOLDEN}/vortex.ref", 25)
+    assert set(ref.nmodes.tolist()) == {36, 121} and ref.data.shape == (4, 25, 121)
+    for e in range(25):
+        assert np.all(ref.data[:, e, ref.nmodes[e]:] == 0.0)
+    out = tmp_path / "v.rst"
+    formats.write_restart(str(out), ref.data, {"Step": 100}, nmodes=ref.nmodes)
+    back = formats.read_restart(str(out), 25)
+    assert np.array_equal(back.nmodes, ref.nmodes) and np.array_equal(back.data, ref.data)
+
+
+def test_order_file():
+    orders = formats.read_ord(f"{GOLDEN}/vortex.ord", 25, 5)
+    assert orders.tolist() == [5, 10, 10, 10, 5] * 5

@@ -1,7 +1,8 @@
 """Pin the CPU oracle against the reference's golden restart files.
 
 Each case is run exactly as the reference's regression test does
-(runs/vquad.tst, runs/vhex.tst, runs/cvquad.tst, runs/shallow_mms/shallow_mms.tst) and compared with ``dgm_diff`` semantics
+(runs/vquad.tst, runs/vhex.tst, runs/vortex.tst, runs/cvquad.tst,
+runs/shallow_mms/shallow_mms.tst) and compared with ``dgm_diff`` semantics
 (src/Domain.cpp:1000-1071): every field must pass the absolute tolerance and,
 where the reference norm exceeds repsM = sqrt(eps), the relative tolerance.
 """
@@ -34,6 +35,14 @@ def test_vquad_2d_ns_br1_tvd():
 def test_vhex_3d_euler_lf_tvd():
     # runs/vhex.tst:23-24 aeps = 5e-13, reps = 5e-10
     _check("vhex", 5e-13, 5e-10)
+
+
+def test_vortex_variable_order():
+    """runs/vortex: 2D Navier-Stokes without viscosity (so the Euler residual)
+    on 25 periodic quads, p = 5 with a band of 15 elements at p = 10
+    (vortex.ord), RK4, 100 steps; sides between orders carry the larger
+    order's Gauss points.  Tolerances of runs/vortex.tst: 5e-12 / 5e-12."""
+    _check("vortex", 5e-12, 5e-12)
 
 
 def test_cvquad_2d_ns_curved_arcs():
@@ -102,7 +111,7 @@ def test_crv_reader_and_arc_nodes():
 
 
 def test_restart_reader_consumes_whole_files():
-    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25), ("cvquad", 4), ("shallow_mms", 400)):
+    for root, ne in (("vquad", 4), ("vhex", 8), ("nvortex", 25), ("vortex", 25), ("cvquad", 4), ("shallow_mms", 400)):
         r = formats.read_restart(f"{GOLDEN}/{root}.ref", ne)
         assert r.data.shape[1] == ne
         assert np.all(np.isfinite(r.data))
