@@ -48,7 +48,18 @@ int cfl_max(dgx_domain *d, double *part, double *m) {
   }
   a.gamma = d->gamma; a.ne = d->ne; a.p = d->p;
   a.shallow = d->eqn == DGX_SHALLOW; a.curved = d->curved; a.stream = d->stream;
-  hipError_t e = dgx::launch_cfl2(a);
+  hipError_t e = hipSuccess;
+  if (d->varp) {   // each order group with its own GLL spacings, then one reduction
+    for (size_t g = 0; g < d->vgrp_p.size() && e == hipSuccess; ++g) {
+      const int P = d->vgrp_p[g];
+      for (int i = 0; i < P + 2; ++i) a.dz[i] = d->vgrp_dz[g * (dgx::kMaxOrder + 2) + i];
+      a.p = P; a.elems = d->d_velems + d->vgrp_off[g]; a.nel = d->vgrp_n[g]; a.NM = d->nm;
+      e = dgx::launch_cfl2(a);
+    }
+    if (e == hipSuccess) e = dgx::launch_cfl_max1(part, d->ne, a.out, d->stream);
+  } else {
+    e = dgx::launch_cfl2(a);
+  }
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("CFL kernels: ") + hipGetErrorString(e));
   HIPCHK_X(hipMemcpyAsync(m, a.out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIPCHK_X(hipStreamSynchronize(d->stream));
@@ -236,7 +247,8 @@ int upload_all_tables() {
       (e = dgx::upload_tables_m3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_c2d(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_cfl(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_cfl(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_v2d(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -277,13 +289,18 @@ hipError_t timed(dgx_domain *d, int kind, Fn fn) {
   return e;
 }
 
+hipError_t launch_varp(const dgx_domain *d, int op, const double *in, double *out);
+
 int launch(dgx_domain *d, int op, const double *U, double *out) {
   dgx::LaunchArgs a;
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf; a.ns = d->visc;
   a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
   a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
   a.eqn = d->eqn; a.Bal = d->Bal; a.bal = op == dgx::OP_RHS ? d->bal : 0;
-  hipError_t e = timed(d, op, [&] { return (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a); });
+  hipError_t e = timed(d, op, [&] {
+    if (d->varp) return launch_varp(d, op, U, out);
+    return (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
+  });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   if (op == dgx::OP_RHS && d->bal == 1) d->bal = 2;   // the first residual is stored
   return DGX_OK;
@@ -315,7 +332,7 @@ int transform(dgx_domain *d, int op, const double *x, double *y) {
   dgx::LaunchArgs a;
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf;
   a.X = x; a.Y = y; a.tab = d->tab; a.stream = d->stream;
-  hipError_t e = (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
+  hipError_t e = d->varp ? launch_varp(d, op, x, y) : (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("transform: ") + hipGetErrorString(e));
   return DGX_OK;
 }
@@ -777,6 +794,118 @@ int curved_geometry(dgx_domain *d, const double *xn) {
   return DGX_OK;
 }
 
+// Variable order (root.ord): elements grouped by order, every side's Gauss
+// points max(L_e, L_neighbour) (Side::match, src/Side.cpp:106-117) and flip,
+// and the side tables of each (order, point count): Bgv = the normalized
+// Legendre modes at the side's Gauss points, PWv = (B diag(w)) G2GL, the
+// projected lift of Quad::local_lift (src/Quad.cpp:1971-2038).  Quads, Euler
+// (or Navier_Stokes without viscosity, whose viscous terms are then zero),
+// affine, FP64.
+// The orders were checked by dgx_domain_create.
+int varp_setup(dgx_domain *d, const dgx_domain_desc *desc, const std::vector<Side> &sides) {
+  const int ne = d->ne, p = d->p;
+  d->porder.assign(desc->porder, desc->porder + ne);
+  d->varp = true;
+  d->visc = false;   // Navier_Stokes with vis = 0: the diffusion term adds exact zeros
+  std::vector<int> elems;
+  for (int P = 1; P <= p; ++P) {
+    const int off = (int)elems.size();
+    for (int e = 0; e < ne; ++e)
+      if (d->porder[e] == P) elems.push_back(e);
+    if ((int)elems.size() > off) {
+      d->vgrp_p.push_back(P); d->vgrp_off.push_back(off); d->vgrp_n.push_back((int)elems.size() - off);
+      // GLL node spacings of the order, for the CFL metrics (src/Element.cpp:513-527)
+      const HostBasis hb = make_basis(P);
+      const int Q = P + 2;
+      for (int i = 0; i < dgx::kMaxOrder + 2; ++i)
+        d->vgrp_dz.push_back(i >= Q ? 0.0 : i == 0 ? hb.z[1] - hb.z[0]
+                                          : (i == Q - 1 ? hb.z[i] - hb.z[i - 1] : 0.5 * (hb.z[i + 1] - hb.z[i - 1])));
+    }
+  }
+  std::vector<int> qs((size_t)ne * 4);
+  std::vector<uint8_t> flip((size_t)ne * 4, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int s = 0; s < 4; ++s) {
+      const Side &S = sides[(size_t)e * 4 + s];
+      int n = d->porder[e] + 1;
+      if (S.bc < 0) {
+        n = std::max(n, d->porder[S.le] + 1);
+        // 2D flips reverse the side; the master holds the flag (src/Field.cpp:416-668)
+        flip[(size_t)e * 4 + s] = S.master ? S.fa : sides[(size_t)S.le * 4 + S.ls].fa;
+      }
+      qs[(size_t)e * 4 + s] = n;
+    }
+  std::vector<double> vt(dgx::kVpTab, 0.0);
+  for (int P = 1; P <= dgx::kMaxOrder; ++P) {
+    const int L = P + 1, Q = P + 2;
+    std::vector<ld> z, w;
+    gll_nodes(Q, z, w);
+    for (int n = L; n <= dgx::kMaxOrder + 1; ++n) {
+      std::vector<ld> zg, wg;
+      gauss_nodes(n, zg, wg);
+      double *Bg = &vt[dgx::vp_block(P, n, 0)], *PW = &vt[dgx::vp_block(P, n, 1)];
+      for (int g = 0; g < n; ++g)
+        for (int l = 0; l < L; ++l) Bg[g * L + l] = (double)normleg(l, zg[g]);
+      for (int l = 0; l < L; ++l)
+        for (int g = 0; g < n; ++g) {
+          ld acc = 0;
+          for (int i = 0; i < Q; ++i) acc += normleg(l, z[i]) * w[i] * lagrange(zg, g, z[i]);
+          PW[l * n + g] = (double)acc;
+        }
+    }
+  }
+  HIPCHK(hipMalloc((void **)&d->d_velems, elems.size() * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d->d_vqs, qs.size() * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d->d_vflip, flip.size()));
+  int rc;
+  if ((rc = alloc(&d->d_vtab, vt.size()))) return rc;
+  HIPCHK(hipMemcpy(d->d_velems, elems.data(), elems.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_vqs, qs.data(), qs.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_vflip, flip.data(), flip.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_vtab, vt.data(), vt.size() * sizeof(double), hipMemcpyHostToDevice));
+  // nodal coordinates at each element's own GLL points (padded slots)
+  std::fill(d->coords.begin(), d->coords.end(), 0.0);
+  for (int e = 0; e < ne; ++e) {
+    const int Q = d->porder[e] + 2;
+    std::vector<ld> z, w;
+    gll_nodes(Q, z, w);
+    const double *vx = desc->vert + (size_t)e * 2 * 4, *vy = vx + 4;
+    static const int ai[4] = {0, 1, 1, 0}, bi[4] = {0, 0, 1, 1};
+    double *cx = &d->coords[(size_t)e * d->nq], *cy = &d->coords[((size_t)ne + e) * d->nq];
+    for (int n = 0; n < 4; ++n)
+      for (int i = 0; i < Q; ++i)
+        for (int j = 0; j < Q; ++j) {
+          const double za = (double)z[i], zb = (double)z[j];
+          const double phi = (ai[n] ? 0.5 * (1.0 + za) : 0.5 * (1.0 - za)) * (bi[n] ? 0.5 * (1.0 + zb) : 0.5 * (1.0 - zb));
+          cx[i * Q + j] += vx[n] * phi;
+          cy[i * Q + j] += vy[n] * phi;
+        }
+  }
+  // traces in slots of kMaxOrder + 1 points per side and field
+  (void)hipFree(d->T);
+  d->T = nullptr;
+  if ((rc = alloc(&d->T, (size_t)ne * 4 * d->nf * (dgx::kMaxOrder + 1)))) return rc;
+  return DGX_OK;
+}
+
+// Launch `op` (OP_TRACES, OP_RHS, OP_FORWARD, OP_BACKWARD) of a variable-order
+// domain, one launch per order group.
+hipError_t launch_varp(const dgx_domain *d, int op, const double *in, double *out) {
+  for (size_t g = 0; g < d->vgrp_p.size(); ++g) {
+    dgx::VarpArgs a;
+    a.op = op; a.p = d->vgrp_p[g];
+    a.elems = d->d_velems + d->vgrp_off[g]; a.nel = d->vgrp_n[g]; a.ne = d->ne;
+    a.NM = d->nm; a.QM = d->nq; a.QFM = dgx::kMaxOrder + 1;
+    a.T = d->T; a.qs = d->d_vqs; a.flip = d->d_vflip; a.vtab = d->d_vtab; a.flag = d->d_flag;
+    a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
+    if (op == dgx::OP_TRACES || op == dgx::OP_RHS) { a.U = in; a.R = out; }
+    else { a.X = in; a.Y = out; }
+    const hipError_t e = dgx::launch_varp2(a);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int build(dgx_domain *d, const dgx_domain_desc *desc) {
   const int nsd = desc->nsd, ne = desc->ne, p = desc->p;
   d->nsd = nsd; d->ne = ne; d->p = p;
@@ -1092,6 +1221,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * (d->curved ? d->nq : d->nm)))) return rc;
     if ((rc = alloc(&d->TV, (size_t)ne * d->ns * (d->nf - 1) * qs))) return rc;
   }
+  if (desc->porder && (rc = varp_setup(d, desc, sides))) return rc;
   return DGX_OK;
 }
 
@@ -1099,7 +1229,8 @@ void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
                   d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
-                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part, d->d_chol};
+                  d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part, d->d_chol,
+                  d->d_velems, d->d_vqs, d->d_vflip, d->d_vtab};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
@@ -1213,6 +1344,20 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "bulk_forcing is available for affine hexes, DGX_FP64");
   if ((desc->bc_function || desc->source) && desc->eqntype != DGX_SHALLOW)
     return fail(DGX_ERR_ARGUMENT, "boundary and source functions are available for Shallow");
+  if (desc->porder) {   // variable order (root.ord)
+    if (desc->nsd != 2 || !(desc->eqntype == DGX_EULER || (desc->eqntype == DGX_NAVIER_STOKES && desc->vis == 0.0)))
+      return fail(DGX_ERR_ARGUMENT, "porder: variable order is built for quads, Euler (and Navier_Stokes with vis = 0)");
+    if (desc->precision != DGX_FP64 || desc->xnodes || desc->bc_function || desc->source || !all_quads_affine(desc))
+      return fail(DGX_ERR_ARGUMENT, "porder: variable order needs affine quads without functions, DGX_FP64");
+    int pmax = 0;
+    for (int e = 0; e < desc->ne; ++e) {
+      if (desc->porder[e] < 1 || desc->porder[e] > desc->p)
+        return fail(DGX_ERR_ARGUMENT, "porder: element " + std::to_string(e) + " has order " +
+                                          std::to_string(desc->porder[e]) + " outside 1..p");
+      pmax = std::max(pmax, desc->porder[e]);
+    }
+    if (pmax != desc->p) return fail(DGX_ERR_ARGUMENT, "porder: p must be the largest element order");
+  }
   {
     std::string why;
     if (desc->bc_function && !check_function(desc->bc_function, 3, &why))
@@ -1507,13 +1652,18 @@ int dgx_norms(dgx_handle h, double *out) {
   std::vector<double> Un((size_t)h->nf * h->ne * h->nq);
   int rc = dgx_state_download(h, Un.data(), DGX_NODAL, 0);
   if (rc) return rc;
-  const int Q = h->Q, nq = h->nq, nsd = h->nsd;
-  const std::vector<double> &w = h->hb.w, &D = h->hb.D;
-  std::vector<double> fr(nq), fs(nq), ft(nq);
+  const int nsd = h->nsd, NQ = h->nq;
+  std::vector<HostBasis> vb;   // variable order: each element's own basis
+  if (h->varp)
+    for (int P = 0; P <= h->p; ++P) vb.push_back(P ? make_basis(P) : HostBasis());
+  std::vector<double> fr(NQ), fs(NQ), ft(NQ);
   for (int v = 0; v < h->nf; ++v) {
     double Linf = 0, L2 = 0, H1 = 0, area = 0;
     for (int e = 0; e < h->ne; ++e) {
-      const double *u = &Un[((size_t)v * h->ne + e) * nq];
+      const HostBasis &eb = h->varp ? vb[h->porder[e]] : h->hb;
+      const int Q = eb.Q, nq = nsd == 2 ? Q * Q : Q * Q * Q;
+      const std::vector<double> &w = eb.w, &D = eb.D;
+      const double *u = &Un[((size_t)v * h->ne + e) * NQ];
       const double *mt = &h->metric[(size_t)e * 9];
       std::fill(fr.begin(), fr.end(), 0.0); std::fill(fs.begin(), fs.end(), 0.0); std::fill(ft.begin(), ft.end(), 0.0);
       if (nsd == 2) {

@@ -32,10 +32,11 @@ k_cfl2(CflArgs a) {
   constexpr int L = P + 1, Q = P + 2, L2 = L * L, Q2 = Q * Q, NT = 64;
   constexpr int NF = 4;   // at most 4 fields are read (shallow water: 3)
   __shared__ double A[NF * L2], X[NF * Q * L], O[NF * Q2], red[NT];
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne, nf = a.shallow ? 3 : 4;
+  const int e = a.elems ? a.elems[blockIdx.x] : blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const int nf = a.shallow ? 3 : 4, NM = a.elems ? a.NM : L2;
   for (int i = tid; i < NF * L2; i += NT) {
     const int v = i / L2;
-    A[i] = v < nf ? a.U[((size_t)v * ne + e) * L2 + (i - v * L2)] : 0.0;
+    A[i] = v < nf ? a.U[((size_t)v * ne + e) * NM + (i - v * L2)] : 0.0;
   }
   __syncthreads();
   // modal -> GLL nodes, all fields at once: [NF][L][L] -> [NF][Q][L] -> [NF][Q][Q]
@@ -112,15 +113,21 @@ __global__ void __launch_bounds__(kMaxThreads) k_max1(const double *__restrict__
 
 template <int P>
 static hipError_t launch_cfl2_p(const CflArgs &a) {
-  hipLaunchKernelGGL(k_cfl2<P>, dim3(a.ne), dim3(64), 0, a.stream, a);
+  hipLaunchKernelGGL(k_cfl2<P>, dim3(a.elems ? a.nel : a.ne), dim3(64), 0, a.stream, a);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || a.elems) return e;
   hipLaunchKernelGGL(k_max1, dim3(1), dim3(kMaxThreads), 0, a.stream, a.part, a.ne, a.out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl_max1(const double *part, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_max1, dim3(1), dim3(kMaxThreads), 0, s, part, n, out);
   return hipGetLastError();
 }
 
 hipError_t launch_cfl2(const CflArgs &a) {
   if (a.ne <= 0 || !a.part || !a.out || !a.U || !a.cgeo) return hipErrorInvalidValue;
+  if (a.elems && (a.nel < 1 || a.NM < (a.p + 1) * (a.p + 1))) return hipErrorInvalidValue;
   switch (a.p) {
 #define DGX_CASE(PP) case PP: return launch_cfl2_p<PP>(a);
     DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)

@@ -266,6 +266,20 @@ __device__ __forceinline__ void contract(const double *__restrict__ M, const dou
   }
 }
 
+// Edge contributions at modal (l,m): edges 0,2 normal to b, edges 1,3 normal to a.
+template <int P>
+__device__ __forceinline__ double face_add2(const double *__restrict__ Bend, const double *H, int l,
+                                            int m, const double c[4]) {
+  constexpr int L = P + 1;
+  const double *B0 = Bend, *B1 = Bend + L;
+  double s = 0.0;
+  s = fma(c[0] * B0[m], H[0 * L + l], s);
+  s = fma(c[2] * B1[m], H[2 * L + l], s);
+  s = fma(c[1] * B1[l], H[1 * L + m], s);
+  s = fma(c[3] * B0[l], H[3 * L + m], s);
+  return s;
+}
+
 // Contribution of the six face blocks H[s][.][.] at modal index (l,m,n),
 // weighted per face by c[s].  Face normal dims: 4,2 -> a; 1,3 -> b; 0,5 -> c.
 template <int P>

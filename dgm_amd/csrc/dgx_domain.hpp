@@ -67,6 +67,15 @@ struct dgx_domain {
   double *d_chol = nullptr;  // curved quads: packed Cholesky factors of the element mass matrices
   double *d_part = nullptr;  // 2D: CFL scratch, ne per-element maxima and the domain maximum
   int bal = 0;             // 0 no balance term, 1 store at the next residual, 2 subtract
+  // variable order (quads, dgx_varp2d.hip): per-element orders, elements
+  // grouped by order (d_velems, group g at vgrp_off[g] with vgrp_n[g] elements
+  // of order vgrp_p[g]), Gauss points and flips of every side, side tables
+  bool varp = false;
+  std::vector<int> porder, vgrp_p, vgrp_off, vgrp_n;
+  std::vector<double> vgrp_dz;   // [group][kMaxOrder + 2] GLL node spacings (CFL)
+  int *d_velems = nullptr, *d_vqs = nullptr;
+  uint8_t *d_vflip = nullptr;
+  double *d_vtab = nullptr;
   dgx::Tab tab;
   dgx::Geo geo;
   dgx::Phys phys;

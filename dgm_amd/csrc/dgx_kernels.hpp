@@ -187,10 +187,43 @@ struct CflArgs {
   double gamma = 1.4;
   int ne = 0, p = 0;
   bool shallow = false, curved = false;
+  // variable order: the group of elements of order p (elems[nel]) in slots of
+  // NM modes, and no reduction (launch_cfl_max1 follows the last group)
+  const int *elems = nullptr;
+  int nel = 0, NM = 0;
   hipStream_t stream = nullptr;
 };
 hipError_t launch_cfl2(const CflArgs &a);
+hipError_t launch_cfl_max1(const double *part, int n, double *out, hipStream_t s);
 hipError_t upload_tables_cfl(const double *h);
+
+// Variable-order quads (dgx_varp2d.hip), 2D Euler on affine quads, elements
+// launched in groups of one order.  Side tables for element order P and n side
+// points (P + 1 <= n <= kMaxOrder + 1): Bgv [n][L] (modal -> the side's Gauss
+// points) and PWv [L][n] (the lift of those points into the element's modes),
+// each in a block of 121 doubles at vp_block(P, n, 0 / 1).
+__host__ __device__ constexpr int vp_block(int P, int n, int which) { return ((P * 12 + n) * 2 + which) * 121; }
+constexpr int kVpTab = vp_block(kMaxOrder + 1, 0, 0);
+struct VarpArgs {
+  int op = 0, p = 0;             // OP_TRACES, OP_RHS, OP_FORWARD, OP_BACKWARD; the group's order
+  const int *elems = nullptr;    // the group's elements
+  int nel = 0, ne = 0;
+  int NM = 0, QM = 0, QFM = 0;   // padded modes and nodes per element, points per side slot
+  const double *U = nullptr;     // modal [4][ne][NM]
+  double *T = nullptr;           // traces [ne][4][4][QFM]
+  double *R = nullptr;           // modal residual [4][ne][NM]
+  const double *X = nullptr;     // transforms: input
+  double *Y = nullptr;           // transforms: output
+  const int *qs = nullptr;       // [ne][4] Gauss points of each side
+  const uint8_t *flip = nullptr; // [ne][4] 1: the neighbour's points run reversed
+  const double *vtab = nullptr;  // side tables (kVpTab doubles)
+  int *flag = nullptr;
+  Geo geo;
+  Phys phys;
+  hipStream_t stream = nullptr;
+};
+hipError_t launch_varp2(const VarpArgs &a);
+hipError_t upload_tables_v2d(const double *h);
 
 // Error message of the calling thread's last failure (dgx_api.cpp).
 int set_error(int code, const std::string &msg);

@@ -30,20 +30,6 @@ __device__ __forceinline__ void backward2(const Tab &tb, const double *m, double
   __syncthreads();
 }
 
-// Edge contributions at modal (l,m): edges 0,2 normal to b, edges 1,3 normal to a.
-template <int P>
-__device__ __forceinline__ double face_add2(const double *__restrict__ Bend, const double *H, int l,
-                                            int m, const double c[4]) {
-  constexpr int L = H2<P>::L;
-  const double *B0 = Bend, *B1 = Bend + L;
-  double s = 0.0;
-  s = fma(c[0] * B0[m], H[0 * L + l], s);
-  s = fma(c[2] * B1[m], H[2 * L + l], s);
-  s = fma(c[1] * B1[l], H[1 * L + m], s);
-  s = fma(c[3] * B0[l], H[3 * L + m], s);
-  return s;
-}
-
 // Gauss-point traces of the NF fields of element e on its four edges, all
 // fields per contraction (one load phase, four contraction phases).
 template <int P, int NF>

@@ -38,7 +38,8 @@ class DomainDesc(ctypes.Structure):
                 ("viscous", ctypes.c_int), ("br2_eta", ctypes.c_double), ("bc_state", ctypes.c_double * 5),
                 ("xnodes", _dp), ("no_balance", ctypes.c_int),
                 ("bc_function", ctypes.POINTER(Function)), ("source", ctypes.POINTER(Function)),
-                ("wall_T", ctypes.c_double), ("wall_Eref", ctypes.c_double), ("bulk_forcing", ctypes.c_int)]
+                ("wall_T", ctypes.c_double), ("wall_Eref", ctypes.c_double), ("bulk_forcing", ctypes.c_int),
+                ("porder", _ip)]
 
 
 class DgxError(RuntimeError):
@@ -140,7 +141,8 @@ class GPUDomain:
 
     def __init__(self, mesh, p, eqntype, gamma=1.4, vis=0.0, Pr=0.72, flux=DGX_FLUX_LF,
                  precision=DGX_FP64, stream=None, viscous=DGX_VISCOUS_BR1, br2_eta=0.0, bc_state=None,
-                 xnodes=None, balance=True, bc_function=None, source=None, wall=None, bulk_forcing=False):
+                 xnodes=None, balance=True, bc_function=None, source=None, wall=None, bulk_forcing=False,
+                 porder=None):
         lib = load()
         self.lib = lib
         self.nsd = int(mesh.nsd)
@@ -158,6 +160,10 @@ class GPUDomain:
         if wall is not None:       # isothermal walls: (Tw, Eref)
             desc.wall_T, desc.wall_Eref = float(wall[0]), float(wall[1])
         desc.bulk_forcing = int(bool(bulk_forcing))
+        self.porder = None
+        if porder is not None:     # variable order: one order per element, p the largest
+            self.porder = np.ascontiguousarray(porder, dtype=np.int32)
+            desc.porder = self.porder.ctypes.data_as(_ip)
         self.h = ctypes.c_void_p()
         _check(lib.dgx_domain_create(ctypes.byref(desc), ctypes.c_void_p(stream or 0),
                                      ctypes.byref(self.h)))

@@ -229,19 +229,24 @@ class ReoProblem:
                        dgx.DGX_NAVIER_STOKES3D, dgx.DGX_SHALLOW):
             raise NotImplementedError(f"Reo::Problem: equation type {eqn} has no GPU path")
         p = int(prm["p"])
-        if os.path.exists(self.root + ".ord"):
-            # variable order per element (Field::read_order, src/Field.cpp:87-90)
-            raise NotImplementedError("Reo::Problem: root.ord (variable order per element) has no GPU path; "
-                                      "the run would silently take the uniform order p")
         kw = {}
         mesh = self._boundaries(formats.read_mesh(self.root), eqn, kw)
+        self.porder = None
+        if os.path.exists(self.root + ".ord"):
+            # variable order per element (Field::Field, src/Field.cpp:2290-2320):
+            # the domain's p is the largest; the library rejects what it has
+            # no variable-order path for (curved, viscous, 3D, functions)
+            self.porder = formats.read_ord(self.root + ".ord", mesh.ne, p)
+            p = int(self.porder.max())
+            kw["porder"] = self.porder
         if os.path.exists(self.root + ".chn"):   # Channel::exists (src/Euler3d.cpp:75-77)
             kw["bulk_forcing"] = True
         self.xnodes = None
-        if os.path.exists(self.root + ".crv"):     # curved sides (Field::read_curve)
+        crv = formats.read_crv(self.root + ".crv") if os.path.exists(self.root + ".crv") else None
+        if crv is not None and crv[1]:             # curved sides (Field::read_curve)
             if mesh.nsd != 2:
                 raise NotImplementedError("Reo::Problem: root.crv is read for quads; pass hex nodes as xnodes")
-            self.xnodes, _ = meshgen.curved_quad_nodes(mesh, p, formats.read_crv(self.root + ".crv"))
+            self.xnodes, _ = meshgen.curved_quad_nodes(mesh, p, crv)
         elif mesh.nsd == 2 and not np.all(meshgen.affine_quads(mesh)):
             self.xnodes = meshgen.quad_nodes(mesh, p)   # what the library does by itself, kept for the caller
         self.mesh = mesh
@@ -277,14 +282,17 @@ class ReoProblem:
 
     def _header(self) -> Dict[str, object]:
         p = self.params
+        pmax = int(p["p"]) if self.porder is None else int(self.porder.max())
         return {"Name": os.path.basename(self.root), "Nsd": self.mesh.nsd,
-                "Lmax": int(p["p"]) + 1, "Step": self.step, "Time": self.t,
+                "Lmax": pmax + 1, "Step": self.step, "Time": self.t,
                 "Time step": float(self.dt if self.dt is not None else p["dt"]), "Viscosity": float(p["vis"]),
                 "Field names": ["phi", "uphi", "vphi"] if self.eqn == dgx.DGX_SHALLOW else _FIELDS[self.mesh.nsd]}
 
     def write(self, fname: str) -> None:
-        """Restart file of the current state (modal, JSON header)."""
-        formats.write_restart(fname, self.domain.download(dgx.DGX_MODAL), self._header())
+        """Restart file of the current state (modal, JSON header); with
+        variable order each element keeps its own number of modes."""
+        nmodes = None if self.porder is None else (self.porder.astype(np.int64) + 1) ** self.mesh.nsd
+        formats.write_restart(fname, self.domain.download(dgx.DGX_MODAL), self._header(), nmodes=nmodes)
 
     def solve(self) -> int:
         if self.domain is None:

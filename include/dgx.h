@@ -160,6 +160,18 @@ typedef struct {
                        after the projection the domain means of the rho, rho u and
                        rho E residuals are subtracted from them.  Affine hexes,
                        DGX_FP64 */
+  const int *porder; /* variable order, as root.ord gives it (Field::Field,
+                       src/Field.cpp:2290-2320): the order of every element [ne], p
+                       their largest; NULL: every element has order p.  Affine quads,
+                       Euler (or Navier_Stokes with vis = 0, whose diffusion adds
+                       exact zeros), DGX_FP64, fixed dt (no dgx_set_time_step).
+                       The two sides of a link carry max(L_left, L_right) Gauss
+                       points (Side::match, src/Side.cpp:106-117); the lift
+                       interpolates them to the element's GLL points
+                       (Quad::local_lift, src/Quad.cpp:1971-2038).  State layouts
+                       keep slots of the largest order: element e's (p_e+1)^2
+                       modes (or (p_e+2)^2 nodes) come first in its slot of
+                       (p+1)^2 (or (p+2)^2), zeros after */
 } dgx_domain_desc;
 
 /* Build a domain: geometry, side links and flips, basis tables, device

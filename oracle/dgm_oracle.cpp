@@ -1775,11 +1775,10 @@ struct Domain {
   // largest idt over the GLL nodes of the current state, then the step rule.
   // Hexes: Euler3d::set_time_step (src/Euler3d.cpp:462-465) returns dt.
   // out = {cfl, new dt}
-  void set_time_step(double dt, double CFLmax, double dtmax, double *out) {
-    if (nsd == 3) { out[0] = dt; out[1] = dt; return; }
-    double cfl = 0.0;
+  // largest idt over the GLL nodes of element e for the nodal state r, ru, rv, rE
+  double cfl_element(int e, const double *r, const double *ru, const double *rv, const double *rE) {
     std::vector<double> xr(qtot), xs(qtot), yr(qtot), ys(qtot);
-    for (int e = 0; e < ne; ++e) {
+    {
       const Elem &E = el[e];
       // local_gradient(C->x, xr, xs), local_gradient(C->y, yr, ys)
       for (int i = 0; i < q; ++i)
@@ -1791,8 +1790,6 @@ struct Domain {
           }
           xr[i * q + j] = a0; xs[i * q + j] = a1; yr[i * q + j] = c0; ys[i * q + j] = c1;
         }
-      const double *r = eu(U, 0, e), *ru = eu(U, 1, e), *rv = eu(U, 2, e);
-      const double *rE = eqn == SHALLOW2D ? nullptr : eu(U, 3, e);
       double emax = 0.0;
       for (int i = 0; i < q; ++i) {
         const double dr = i == 0 ? z[i + 1] - z[i] : (i == q - 1 ? z[i] - z[i - 1] : 0.5 * (z[i + 1] - z[i - 1]));
@@ -1829,7 +1826,15 @@ struct Domain {
           emax = std::max(emax, idt);
         }
       }
-      cfl = std::max(cfl, dt * emax);
+      return emax;
+    }
+  }
+  void set_time_step(double dt, double CFLmax, double dtmax, double *out) {
+    if (nsd == 3) { out[0] = dt; out[1] = dt; return; }
+    double cfl = 0.0;
+    for (int e = 0; e < ne; ++e) {
+      const double *rE = eqn == SHALLOW2D ? nullptr : eu(U, 3, e);
+      cfl = std::max(cfl, dt * cfl_element(e, eu(U, 0, e), eu(U, 1, e), eu(U, 2, e), rE));
     }
     double dn = dt;
     if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / cfl);
@@ -2133,6 +2138,21 @@ struct VarDomain2 {
     for (int e = 0; e < ne; ++e)
       for (int v = 0; v < nv; ++v) lift(Us, Uf, v, e);
     project_all(Uf);
+  }
+
+  // Euler::set_time_step (src/Euler.cpp:804-877) with each element's own nodes
+  void set_time_step(double dt, double CFLmax, double dtmax, double *out) {
+    double cfl = 0.0;
+    for (int e = 0; e < ne; ++e) {
+      Domain &Dm = *dom[pe[e]];
+      cfl = std::max(cfl, dt * Dm.cfl_element(e, nod(U, 0, e), nod(U, 1, e), nod(U, 2, e), nod(U, 3, e)));
+      status |= Dm.status;
+    }
+    double dn = dt;
+    if (CFLmax != 0 && dtmax != 0) dn = std::min(dtmax, CFLmax * dt / cfl);
+    else if (CFLmax != 0) dn = CFLmax * dt / cfl;
+    out[0] = cfl * dn / dt;
+    out[1] = dn;
   }
 
   // the steppers of Domain (RungeKutta, TVD, LSRK14) on the flat nodal arrays
@@ -2487,6 +2507,19 @@ void orc_varp_coords(void *h, double *x) {
     std::memcpy(&c.u[d->ntot + d->off[e]], E.cy.data(), E.cy.size() * sizeof(double));
   }
   varp_pad_out(d, c, x, 2);
+}
+int orc_varp_set_time_step(void *h, double dt, double CFLmax, double dtmax, double *out) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  d->status = 0;
+  for (Domain *dm : d->dom)
+    if (dm) dm->status = 0;
+  d->set_time_step(dt, CFLmax, dtmax, out);
+  return d->status;
+}
+void orc_varp_set_bc_state(void *h, const double *U) {
+  VarDomain2 *d = (VarDomain2 *)h;
+  for (Domain *dm : d->dom)
+    if (dm) std::memcpy(dm->bc_state, U, 5 * sizeof(double));
 }
 void orc_varp_set_state(void *h, const double *U) { VarDomain2 *d = (VarDomain2 *)h; varp_pad_in(d, U, d->U, 4); }
 void orc_varp_get_state(void *h, double *U) { VarDomain2 *d = (VarDomain2 *)h; varp_pad_out(d, d->U, U, 4); }

@@ -70,9 +70,11 @@ def _load():
     lib.orc_varp_qmax.argtypes = [ctypes.c_void_p]
     lib.orc_varp_qmax.restype = ctypes.c_int
     for name in ("orc_varp_coords", "orc_varp_set_state", "orc_varp_get_state", "orc_varp_modal",
-                 "orc_varp_set_modal", "orc_varp_norms"):
+                 "orc_varp_set_modal", "orc_varp_norms", "orc_varp_set_bc_state"):
         getattr(lib, name).argtypes = [ctypes.c_void_p, _dp]
         getattr(lib, name).restype = None
+    lib.orc_varp_set_time_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_varp_set_time_step.restype = ctypes.c_int
     lib.orc_varp_project_state.argtypes = [ctypes.c_void_p]
     lib.orc_varp_project_state.restype = None
     lib.orc_varp_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -271,6 +273,12 @@ class VarOracle:
         self.lib.orc_varp_coords(self.h, _ptr(x))
         return x
 
+    def set_bc_state(self, U):
+        """Conservative state of state boundaries (type 1 rows)."""
+        self._bcU = np.zeros(5)
+        self._bcU[:4] = np.ravel(np.asarray(U, dtype=np.float64))[:4]
+        self.lib.orc_varp_set_bc_state(self.h, _ptr(self._bcU))
+
     def set_state(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == (4, self.ne, self.qtot), U.shape
@@ -311,6 +319,13 @@ class VarOracle:
         out = np.zeros((4, 3))
         self.lib.orc_varp_norms(self.h, _ptr(out))
         return out
+
+    def set_time_step(self, dt, CFLmax=0.0, dtmax=0.0):
+        """(CFL, new dt) of the current state (Euler::set_time_step)."""
+        out = np.zeros(2)
+        if self.lib.orc_varp_set_time_step(self.h, float(dt), float(CFLmax), float(dtmax), _ptr(out)):
+            raise FloatingPointError("oracle: non-realizable state in the CFL evaluation")
+        return float(out[0]), float(out[1])
 
 
 def riemann(nsd, flux, UL, UR, n, gamma=1.4):

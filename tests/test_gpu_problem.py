@@ -3,7 +3,8 @@
 ``ReoProblem(args, table, comm).solve()`` runs the reference's own regression
 run directories (runs/vquad, runs/vhex, runs/cvquad with its arc sides,
 runs/shallow_mms with its function boundaries and source, runs/chan with its
-isothermal walls, bulk forcing and restart initial condition: mesh,
+isothermal walls, bulk forcing and restart initial condition, runs/vortex
+with its variable order per element: mesh,
 connectivity, IC, parameters)
 on libdgx.so and writes ``root.rst``; that file must match the reference's
 golden ``.ref`` under the reference's own diff rule and tolerances
@@ -21,14 +22,14 @@ import pytest
 from dgm_amd import formats
 from dgm_amd import problem as dgm
 from fixture_runs import GOLDEN
-from oracle_lib import Oracle, dgm_diff
+from oracle_lib import Oracle, VarOracle, dgm_diff
 
 pytestmark = pytest.mark.gpu
 REPS_M = np.sqrt(np.finfo(float).eps)
 
 
 def _run(tmp_path, root, extra=()):
-    for ext in (".inp", ".msh", ".con", ".ic", ".crv", ".bc", ".bct", ".src", ".chn", ".ini"):
+    for ext in (".inp", ".msh", ".con", ".ic", ".crv", ".bc", ".bct", ".src", ".chn", ".ini", ".ord"):
         if os.path.exists(os.path.join(GOLDEN, root + ext)):
             shutil.copy(os.path.join(GOLDEN, root + ext), tmp_path / (root + ext))
     base = str(tmp_path / root)
@@ -44,7 +45,7 @@ def _run(tmp_path, root, extra=()):
 
 @pytest.mark.parametrize("root,aeps,reps", [("vquad", 5e-12, 5e-12), ("vhex", 5e-13, 5e-10),
                                             ("cvquad", 5e-13, 5e-13), ("shallow_mms", 1e-9, 1e-10),
-                                            ("chan", 5e-10, 1e-8)])
+                                            ("chan", 5e-10, 1e-8), ("vortex", 5e-12, 5e-12)])
 def test_golden_run_directory(tmp_path, root, aeps, reps):
     print(f"[problem] {root}: full run through ReoProblem.solve()", flush=True)
     base, prob = _run(tmp_path, root)
@@ -53,7 +54,10 @@ def test_golden_run_directory(tmp_path, root, aeps, reps):
     out = formats.read_restart(base + ".rst", ne)
     ref = formats.read_restart(os.path.join(GOLDEN, root + ".ref"), ne)
     assert out.header["Step"] == int(prm["Nt"])
-    o = Oracle(prob.mesh, int(prm["p"]), int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
+    if prob.porder is not None:                           # variable order: each element's own basis
+        o = VarOracle(prob.mesh, prob.porder)
+    else:
+        o = Oracle(prob.mesh, int(prm["p"]), int(prm["eqntype"]), vis=float(prm.get("vis", 0.0)))
     if prob.xnodes is not None:                           # the norms integrate over the curved elements
         o.set_nodes(prob.xnodes)
     rn, dn = dgm_diff(o, ref.data, out.data)

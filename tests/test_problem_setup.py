@@ -69,11 +69,12 @@ def test_unknown_boundary_tag_raises(tmp_path):
         prob._boundaries(formats.read_mesh(base), dgx.DGX_NAVIER_STOKES3D, {})
 
 
-def test_variable_order_file_raises(tmp_path):
-    # a root.ord (variable order per element) must not run silently at uniform p
+def test_variable_order_without_a_path_raises(tmp_path):
+    # a root.ord is read; where the library has no variable-order path (here
+    # Navier-Stokes with viscosity) the run stops instead of taking the uniform p
     base = _copy(tmp_path, "vquad", (".inp", ".msh", ".con", ".bc", ".ic"))
     with open(base + ".ord", "w") as f:
-        f.write("**ORDER DATA**\n1 nord\n0 4\n")
+        f.write("**ORDER DATA**\n0 4 0\n")
     prob = dgm.ReoProblem(dgm.ArgList([base]))
-    with pytest.raises(NotImplementedError, match="ord"):
+    with pytest.raises(dgx.DgxError, match="porder"):
         prob.setup()
