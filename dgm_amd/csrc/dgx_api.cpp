@@ -221,6 +221,21 @@ HostBasis make_basis(int p) {
     b.KE[i] = (double)lo;
     b.KE[Q + i] = (double)hi;
   }
+  // NP1 = BT BW and NPD = BT QD: projection (and projected derivative) from
+  // GLL values back to GLL values, in long double
+  b.NP1.resize(Q * Q); b.NPD.resize(Q * Q);
+  for (int i = 0; i < Q; ++i)
+    for (int m = 0; m < Q; ++m) {
+      ld sp = 0, sd = 0;
+      for (int l = 0; l < L; ++l) {
+        ld qd = 0;
+        for (int j = 0; j < Q; ++j) qd += normleg(l, z[j]) * w[j] * Dl[j * Q + m];
+        sp += normleg(l, z[i]) * normleg(l, z[m]) * w[m];
+        sd += normleg(l, z[i]) * qd;
+      }
+      b.NP1[i * Q + m] = (double)sp;
+      b.NPD[i * Q + m] = (double)sd;
+    }
   return b;
 }
 
@@ -234,7 +249,7 @@ int upload_all_tables() {
     HostBasis b = make_basis(p);
     size_t off = dgx::tab_offset(p);
     for (const std::vector<double> *t : {&b.BT, &b.BW, &b.QD, &b.Bg, &b.Bend, &b.PW, &b.Ig, &b.PI, &b.KE, &b.BL, &b.GL,
-                                         &b.D}) {
+                                         &b.D, &b.NP1, &b.NPD}) {
       std::copy(t->begin(), t->end(), packed.begin() + off);
       off += t->size();
     }

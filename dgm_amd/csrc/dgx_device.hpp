@@ -24,11 +24,12 @@ struct Tab {
 // BT, BW, QD, Bg, Bend, PW, Ig (sizes above), then PI = PW Ig [L][Q] (GLL
 // face layer -> modal face coefficients), KE [2][Q] (KE[s][i] =
 // sum_l Bend[s][l] BT[i][l], the end traces sum_l phi_l(-+1) phi_l(xi_i)),
-// BL [L][Q], GL [Q][QF] and DN [Q][Q] (see CT), orders packed in ascending p.
+// BL [L][Q], GL [Q][QF], DN [Q][Q], NP1 [Q][Q] and NPD [Q][Q] (see CT), orders
+// packed in ascending p.
 constexpr int kMaxOrder = 10;
 constexpr int tab_size(int p) {
   return 3 * (p + 2) * (p + 1) + 2 * (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1) * (p + 2) +
-         (p + 1) * (p + 2) + 2 * (p + 2) + (p + 1) * (p + 2) + (p + 2) * (p + 1) + (p + 2) * (p + 2);
+         (p + 1) * (p + 2) + 2 * (p + 2) + (p + 1) * (p + 2) + (p + 2) * (p + 1) + 3 * (p + 2) * (p + 2);
 }
 constexpr int tab_offset(int p) {
   int s = 0;
@@ -56,6 +57,11 @@ template <int P> struct CT {
   __device__ static __forceinline__ const double *BL() { return KE() + 2 * Q; }
   __device__ static __forceinline__ const double *GL() { return BL() + L * Q; }
   __device__ static __forceinline__ const double *DN() { return GL() + Q * QF; }
+  // nodal-to-nodal [Q][Q]: NP1 = BT BW (the L2 projection onto the order-P
+  // modes, evaluated back at the GLL points) and NPD = BT QD (the projected
+  // derivative, at the GLL points)
+  __device__ static __forceinline__ const double *NP1() { return DN() + Q * Q; }
+  __device__ static __forceinline__ const double *NPD() { return NP1() + Q * Q; }
 };
 
 // Per-element affine geometry and face links (device memory).

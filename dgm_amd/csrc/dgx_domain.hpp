@@ -26,7 +26,7 @@ inline constexpr double kLsrkC[14] = {0.0, 0.0367762454319673, 0.124968526272502
 // Basis tables of one order (host copies).
 struct HostBasis {
   int p, L, Q, QF;
-  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE, BL, GL;
+  std::vector<double> z, w, D, BT, BW, QD, Bg, Bend, PW, Ig, PI, KE, BL, GL, NP1, NPD;
 };
 
 }  // namespace dgx

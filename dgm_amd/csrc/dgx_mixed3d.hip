@@ -608,8 +608,9 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   if constexpr (4 * S::TB < 5) batch(IC<4 * S::TB>());
 }
 
-// Face-lift contribution of the two faces normal to reference direction D (see
-// face_pair in dgx_stage3d.hip), FP32.
+// Face-lift contribution of the two faces normal to reference direction D at
+// modal (l, m, n) of block set H ([6][L][L], pitch LP), with weights clo, chi
+// for the face at -1 and +1 (faces 4/2 for a, 1/3 for b, 0/5 for c), FP32.
 template <int P, int D>
 __device__ __forceinline__ float mface_pair(const float *H, int l, int m, int n, float clo, float chi) {
   constexpr int L = P + 1, LP = odd(L), HB = L * LP;

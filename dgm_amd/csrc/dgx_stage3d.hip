@@ -42,7 +42,7 @@ template <int P, int NTH = kStageThreads> struct S3 {
   static constexpr int NSG = (4 * L2 + NT - 1) / NT;    // modal pencil slots (4 fields)
   static constexpr int NG = 30 * QF * FP;               // face corrections
   static constexpr int lds_stage = (2 * NA + NG) * 8;
-  static constexpr int lds_vterm = (2 * NA + 24 * QF * FP) * 8;
+  static constexpr int lds_vterm = (2 * NA + 24 * Q * QP) * 8;
   // BR2: face-lift nodal values, metric rows, end traces (+ layer velocity)
   static constexpr bool br2_vel_in_b = 42 * Q * QP <= NA;
   static constexpr int lds_vterm_br2 = (2 * NA + 24 * Q * QP + 16 + 2 * Q + (br2_vel_in_b ? 0 : 18 * Q * QP)) * 8;
@@ -78,21 +78,32 @@ __device__ __forceinline__ void chain_to_regs(const double *__restrict__ M, cons
   }
 }
 
-// Sum-factorized projected divergence chain for reference direction D of NF
-// nodal fields in A ([NF][Q][Q][Q], pitch QP); result accumulated into registers.
-// D = 0 (r): QD along a;  D = 1 (s): QD along b;  D = 2 (t): QD along c.
-template <int P, int NF, int NS, int D, int NTH = kStageThreads>
-__device__ __forceinline__ void div_chain(double *A, double *B, double (&acc)[NS][P + 1], int tid) {
+// Collocation derivative along reference direction D of NF nodal fields in A
+// ([NF][Q][Q][Q], pitch QP), written (ACC: added) to B.  Summed over the three
+// directions of the contravariant fluxes, B holds the nodal divergence, and
+// projecting it once (proj_to_regs) equals the three projected-derivative
+// chains: (BW D)_d (x) BW (x) BW = (BW (x) BW (x) BW)(D along d).
+template <int P, int NF, int D, bool ACC, int NTH = kStageThreads>
+__device__ __forceinline__ void nodal_deriv(const double *A, double *B, int tid) {
+  using S = S3<P, NTH>;
+  constexpr int Q = S::Q, QP = S::QP, NT = S::NT;
+  static_assert(D != 2 || !ACC, "the c derivative comes first and writes B");
+  if constexpr (D == 2) con_last<NF * Q * Q, Q, Q, QP, QP, NT>(CT<P>::DN(), A, B, tid);
+  else if constexpr (D == 1) con_mid<NF * Q, Q, 1, Q, Q, QP, QP, NT, ACC>(CT<P>::DN(), A, B, tid);
+  else con_mid<NF, Q, Q, Q, Q, QP, QP, NT, ACC>(CT<P>::DN(), A, B, tid);
+}
+
+// Projection BW (x) BW (x) BW of NF nodal fields in B, accumulated into the
+// modal registers acc (the pencils of chain_to_regs); A is scratch.
+template <int P, int NF, int NS, int NTH = kStageThreads>
+__device__ __forceinline__ void proj_to_regs(double *A, double *B, double (&acc)[NS][P + 1], int tid) {
   using S = S3<P, NTH>;
   constexpr int L = S::L, Q = S::Q, LP = S::LP, QP = S::QP, NT = S::NT;
-  const double *Mc = D == 2 ? CT<P>::QD() : CT<P>::BW();
-  const double *Mb = D == 1 ? CT<P>::QD() : CT<P>::BW();
-  const double *Ma = D == 0 ? CT<P>::QD() : CT<P>::BW();
-  con_last<NF * Q * Q, Q, L, QP, LP, NT>(Mc, A, B, tid);        // [NF Q Q][L]
+  con_last<NF * Q * Q, Q, L, QP, LP, NT>(CT<P>::BW(), B, A, tid);   // [NF Q Q][L]
   __syncthreads();
-  con_mid<NF * Q, Q, 1, L, L, LP, LP, NT>(Mb, B, A, tid);       // [NF Q][L][L]
+  con_mid<NF * Q, Q, 1, L, L, LP, LP, NT>(CT<P>::BW(), A, B, tid);  // [NF Q][L][L]
   __syncthreads();
-  chain_to_regs<P, NF, NS, NTH>(Ma, A, acc, tid);               // [NF][L][L][L]
+  chain_to_regs<P, NF, NS, NTH>(CT<P>::BW(), B, acc, tid);           // [NF][L][L][L]
   __syncthreads();
 }
 
@@ -253,11 +264,15 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
       }
     }
     __syncthreads();
-    div_chain<P, 5, NSL, d, kStage3Threads>(A, B, acc, tid);
+    nodal_deriv<P, 5, d, d != 2, kStage3Threads>(A, B, tid);
+    __syncthreads();
   };
-  vol_dir(IC<0>());
-  vol_dir(IC<1>());
+  // nodal divergence of the contravariant fluxes (c first: it writes B), then
+  // one projection to modes
   vol_dir(IC<2>());
+  vol_dir(IC<1>());
+  vol_dir(IC<0>());
+  proj_to_regs<P, 5, NSL, kStage3Threads>(A, B, acc, tid);
 
   // faces: H = (PW (x) PW) gF for 5 fields x 6 faces, in B
   face_project<P, 30, kStage3Threads>(gF, A, B, tid);
@@ -338,16 +353,38 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   }
 }
 
-// Face-lift contribution of the two faces normal to reference direction D at
-// modal (l, m, n) of block set H ([6][L][L], pitch LP), with weights clo, chi
-// for the face at -1 and +1 (faces 4/2 for a, 1/3 for b, 0/5 for c).
-template <int P, int D>
-__device__ __forceinline__ double face_pair(const double *H, int l, int m, int n, double clo, double chi) {
-  constexpr int L = P + 1, LP = odd(L), HB = L * LP;
-  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
-  if (D == 0) return fma(clo * B0[l], H[4 * HB + m * LP + n], chi * B1[l] * H[2 * HB + m * LP + n]);
-  if (D == 1) return fma(clo * B0[m], H[1 * HB + l * LP + n], chi * B1[m] * H[3 * HB + l * LP + n]);
-  return fma(clo * B0[n], H[0 * HB + l * LP + m], chi * B1[n] * H[5 * HB + l * LP + m]);
+// Last contraction of the lifted gradient along reference direction D for the
+// 4 primitives: out = NPD x along D plus clo KE_lo Hn[face lo] + chi KE_hi
+// Hn[face hi] (the two face lifts across D at their own GLL points; faces 4/2
+// for a, 1/3 for b, 0/5 for c).  One thread per pencil of 4 Q^2, all Q outputs.
+template <int P, int D, int NT>
+__device__ __forceinline__ void lifted_dir(const double *in, double *out, const double *Hn, double clo, double chi,
+                                           int tid) {
+  constexpr int Q = P + 2, QP = odd(Q), HB = Q * QP, NP = 4 * Q * Q;
+  constexpr int flo = D == 0 ? 4 : (D == 1 ? 1 : 0), fhi = D == 0 ? 2 : (D == 1 ? 3 : 5);
+  const double *M = CT<P>::NPD(), *KE = CT<P>::KE();
+  for (int pc = tid; pc < NP; pc += NT) {
+    const int f = pc / (Q * Q), t = pc - f * Q * Q, t1 = t / Q, t2 = t - t1 * Q;
+    // (t1, t2): the two other indices in order; base and stride of the pencil
+    int base, stride;
+    if (D == 0) { base = (f * Q * Q + t1) * QP + t2; stride = Q * QP; }          // (j, k)
+    else if (D == 1) { base = (f * Q * Q + t1 * Q) * QP + t2; stride = QP; }    // (i, k)
+    else { base = (f * Q * Q + t1 * Q + t2) * QP; stride = 1; }                 // (i, j)
+    double x[Q];
+#pragma unroll
+    for (int m = 0; m < Q; ++m) x[m] = in[base + m * stride];
+    const double hl = clo * Hn[(f * 6 + flo) * HB + t1 * QP + t2];
+    const double hh = chi * Hn[(f * 6 + fhi) * HB + t1 * QP + t2];
+#pragma unroll 3   // about 3 coefficient rows live in scalar registers at a time
+    for (int o = 0; o < Q; ++o) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < Q; ++m) s = fma(M[o * Q + m], x[m], s);
+      s = fma(KE[o], hl, s);
+      s = fma(KE[Q + o], hh, s);
+      out[base + o * stride] = s;
+    }
+  }
 }
 
 // Viscous flux of equation q (1..3 momentum, 4 energy) contracted with the
@@ -376,8 +413,12 @@ __device__ __forceinline__ double visc_flux(const double t[6], const double h[3]
 // sum_d dxi_d/dx (D_d + lift of those two faces / |grad xi_d|), and each
 // direction's nodal contribution goes straight into the stress tensor and heat
 // flux at the thread's nodes.
+// Two 512-thread workgroups per CU (4 waves per SIMD) where their LDS fits,
+// which needs at most 128 VGPRs.
 template <int P, bool BR2>
 __global__ void __launch_bounds__(kVterm3Threads)
+__attribute__((amdgpu_waves_per_eu((BR2 ? S3<P, kVterm3Threads>::lds_vterm_br2 : S3<P, kVterm3Threads>::lds_vterm) <=
+                                           80 * 1024 ? 4 : 1)))
 k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ V,
          double *__restrict__ TV, Tab tb, Geo geo, Phys ph, int ne) {
   using S = S3<P, kVterm3Threads>;
@@ -435,12 +476,18 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     pv[r][0] = u; pv[r][1] = v; pv[r][2] = w; pv[r][3] = -0.5 * wk + en;
   }
   __syncthreads();
-  // face lift of the jumps: (PW (x) PW) Jq -> H[4][6][L][L], kept in the Jq region
+  // face lift of the jumps: (PW (x) PW) Jq -> H[4][6][L][L], then its values at
+  // each face's own GLL points Hn[4][6][Q][Q] = (BT (x) BT) H, kept in the Jq region
   con_mid<24, QF, 1, QF, L, FP, FP, NT>(CT<P>::PW(), Jq, A, tid);     // [24][L][QF]
   __syncthreads();
   con_last<24 * L, QF, L, FP, LP, NT>(CT<P>::PW(), A, Jq, tid);     // [24][L][L]
   __syncthreads();
-  const double *H = Jq;
+  con_mid<24, L, 1, L, Q, LP, LP, NT>(CT<P>::BT(), Jq, A, tid);     // [24][Q][L]
+  __syncthreads();
+  con_last<24 * Q, L, Q, LP, QP, NT>(CT<P>::BT(), A, Jq, tid);      // [24][Q][Q]
+  __syncthreads();
+  const double *Hn = Jq;
+  constexpr int HB = Q * QP;   // one nodal face block
 
   const double *mt = geo.metric + 9 * (size_t)e;
   const double mu = ph.vis, kap = ph.vis * ph.gamma / ph.Pr;
@@ -449,6 +496,12 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   for (int r = 0; r < NPT; ++r)
 #pragma unroll
     for (int j = 0; j < 9; ++j) tq[r][j] = 0.0;
+  // The lifted gradient along reference direction d, at the GLL nodes: the
+  // projected derivative (BT BW along the two other directions, then BT QD
+  // along d: three contractions on nodal tensors) plus the nodal values of the
+  // two face lifts across d, folded into the last contraction as two more
+  // inputs with the end-trace columns of KE.  Equal to projecting D_d q and the
+  // face terms to modes and evaluating the result back at the nodes.
   auto grad_dir = [&](auto Dc) {
     constexpr int d = decltype(Dc)::value;
 #pragma unroll
@@ -461,26 +514,16 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
       }
     }
     __syncthreads();
-    double D[NSG][L];
-#pragma unroll
-    for (int k = 0; k < NSG; ++k)
-#pragma unroll
-      for (int l = 0; l < L; ++l) D[k][l] = 0.0;
-    div_chain<P, 4, NSG, d, NT>(A, B, D, tid);
+    if constexpr (d == 2) con_mid<4 * Q, Q, 1, Q, Q, QP, QP, NT>(CT<P>::NP1(), A, B, tid);   // along b
+    else con_last<4 * Q * Q, Q, Q, QP, QP, NT>(CT<P>::NP1(), A, B, tid);                    // along c
+    __syncthreads();
+    if constexpr (d == 0) con_mid<4 * Q, Q, 1, Q, Q, QP, QP, NT>(CT<P>::NP1(), B, A, tid);  // along b
+    else con_mid<4, Q, Q, Q, Q, QP, QP, NT>(CT<P>::NP1(), B, A, tid);                       // along a
+    __syncthreads();
     const double g0 = mt[3 * d], g1 = mt[3 * d + 1], g2 = mt[3 * d + 2];
     const double inv = 1.0 / sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-#pragma unroll
-    for (int k = 0; k < NSG; ++k) {
-      const int Gi = tid + k * NT;
-      if (Gi < 4 * L2) {
-        const int c = Gi / L2, pp = Gi - c * L2, m = pp / L, nn = pp - m * L;
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-          A[((c * L + l) * L + m) * LP + nn] = D[k][l] + face_pair<P, d>(H + c * 6 * L * LP, l, m, nn, -inv, inv);
-      }
-    }
+    lifted_dir<P, d, NT>(A, B, Hn, -inv, inv, tid);                                        // along d
     __syncthreads();
-    backward_batch<P, 4, NT>(A, B, tid);
     // nodal d/dxi_d of (u, v, w, e) -> stress and heat flux
     const double c43 = mu * (4.0 / 3.0), c23 = mu * (2.0 / 3.0);
 #pragma unroll
@@ -501,7 +544,6 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
       tq[r][7] += kap * (g1 * ae);
       tq[r][8] += kap * (g2 * ae);
     }
-    __syncthreads();
   };
   grad_dir(IC<0>());
   grad_dir(IC<1>());
@@ -514,16 +556,14 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   constexpr int HN = 24 * Q * QP, VOFF = S::br2_vel_in_b ? 0 : HN + 16 + 2 * Q;
   double *Df = B, *Vel = S::br2_vel_in_b ? B + HN : Jq + VOFF;
   if constexpr (BR2) {
-    // nodal values of the face lifts at each face's own GLL points: Hn[24][Q][Q]
-    con_mid<24, L, 1, L, Q, LP, LP, NT>(CT<P>::BT(), Jq, A, tid);   // [24][Q][L]
-    if (tid < 9) {   // unit rows of the metric
+    // the face lifts' nodal values Hn[24][Q][Q] are in place; after them the
+    // unit rows of the metric and the end-trace table
+    if (tid < 9) {
       const int d = tid / 3;
       Jq[HN + tid] = mt[tid] / sqrt(mt[3 * d] * mt[3 * d] + mt[3 * d + 1] * mt[3 * d + 1] +
                                     mt[3 * d + 2] * mt[3 * d + 2]);
     }
     if (tid >= 16 && tid < 16 + 2 * Q) Jq[HN + tid] = CT<P>::KE()[tid - 16];
-    __syncthreads();
-    con_last<24 * Q, L, Q, LP, QP, NT>(CT<P>::BT(), A, Jq, tid);    // [24][Q][Q]
     __syncthreads();
   }
 
@@ -627,11 +667,13 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
       }
     }
     __syncthreads();
-    div_chain<P, 4, NSG, d, NT>(A, B, acc, tid);
+    nodal_deriv<P, 4, d, d != 2, NT>(A, B, tid);
+    __syncthreads();
   };
-  visc_dir(IC<0>());
-  visc_dir(IC<1>());
   visc_dir(IC<2>());
+  visc_dir(IC<1>());
+  visc_dir(IC<0>());
+  proj_to_regs<P, 4, NSG, NT>(A, B, acc, tid);
 #pragma unroll
   for (int k = 0; k < NSG; ++k) {
     const int Gi = tid + k * NT;
