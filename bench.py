@@ -37,7 +37,10 @@ Also printed in the same JSON line:
                 source coefficients, 16 receivers, 2000 LSRK14 steps, 45
                 checkpoints): time of one forward solve and of one gradient
                 (checkpointed forward sweep + discrete adjoint), and the
-                gradient's agreement with central differences in 8 directions
+                gradient's agreement with central differences in 8 directions;
+                and the nonlinear Euler initial-state gradient (64x64 quads,
+                p=5, 16 pressure receivers, 200 LSRK14 steps, 15 checkpoints):
+                forward and gradient times, central differences in 2 directions
   serving       p50/p99 latency and throughput of a serving-loop sample: Poisson
                 arrivals of 2D Euler forward solves of random size and order,
                 continuously batched (dgm_amd.serve)
@@ -257,6 +260,69 @@ def adjoint_sample(args, seed=4096):
             "fd_rel_err": [float(e) for e in errs]}
 
 
+def euler_adjoint_sample(args, seed=77):
+    """Initial-state gradient through nonlinear 2D Euler: 64x64 periodic quads
+    on [0, 2 pi]^2 at p = 5, a smooth seeded initial flow, 16 seeded
+    receivers observing the pressure after every step, 200 LSRK14 steps at
+    CFL 0.5, ceil(sqrt(200)) = 15 checkpoints besides the initial state.  One
+    gradient = one checkpointed forward sweep, each reversed step recomputing
+    its 14 stage states, and the backward kernels (dgx_adjoint2d.hip) at every
+    stage; checked against central differences of the objective along smooth
+    seeded directions.  FP64."""
+    from dgm_amd import euler_adjoint
+    rng = np.random.default_rng(seed)
+    n, p, nsteps = args.eadj_n, 5, args.eadj_steps
+    L = 2 * np.pi
+    m = meshgen.periodic_quads(n, n, 0.0, L, 0.0, L)
+    dom = dgx.GPUDomain(m, p, dgx.DGX_EULER)
+    x = dom.coords()
+
+    def smooth(scale):
+        return np.stack([sum(scale[c] * rng.uniform(0.2, 1.0) *
+                             np.sin(rng.integers(0, 3) * x[0] + rng.integers(1, 3) * x[1] + rng.uniform(0, L))
+                             for _ in range(3)) for c in range(4)])
+
+    def modal(f):
+        dom.upload(f)
+        return dom.download(dgx.DGX_MODAL)
+
+    rho = 1.0 + 0.1 * np.sin(x[0]) * np.cos(x[1])
+    u, v = 0.4 + 0.1 * np.sin(x[0] + x[1]), 0.25 - 0.2 * np.cos(x[0]) * np.sin(2 * x[1])
+    pr = 1.0 / 1.4 + 0.05 * np.cos(2 * x[0]) * np.cos(x[1])
+    U0 = modal(np.stack([rho, rho * u, rho * v, pr / 0.4 + 0.5 * rho * (u * u + v * v)]))
+    dom.upload(U0, dgx.DGX_MODAL)
+    _, dt = dom.set_time_step(1e-3, 0.5)
+    nsnap = int(np.ceil(np.sqrt(nsteps))) + 1
+    rec = rng.uniform(0.0, L, size=(16, 2))
+    prob = euler_adjoint.EulerAdjointProblem(m, p, rec, dt, nsteps, nsnap=nsnap)
+    data, _ = prob.forward(U0 + 0.02 * modal(smooth((1.0, 0.4, 0.25, 2.5))))
+    t0w = time.perf_counter()
+    prob.forward(U0)
+    t_fwd = time.perf_counter() - t0w
+    t0w = time.perf_counter()
+    J, g = prob.gradient(U0, data)
+    t_grad = time.perf_counter() - t0w
+    st = prob.stats()
+    errs = []
+    h = 1e-5
+    for _ in range(args.adj_dirs // 4):
+        d = modal(smooth((1.0, 0.4, 0.25, 2.5)))
+        fd = (prob.objective(U0 + h * d, data) - prob.objective(U0 - h * d, data)) / (2 * h)
+        errs.append(abs(fd - float(np.sum(g * d))) / max(abs(fd), 1e-300))
+    prob.close()
+    dom.close()
+    dofs = 4 * m.ne * (p + 1) ** 2
+    fwd_per_grad = st["forward_steps"] - nsteps * (1 + 2 * len(errs))
+    return {"workload": f"nonlinear 2D Euler, {n}x{n} periodic quads, p={p}, initial-state gradient "
+                        f"({dofs} unknowns), 16 pressure receivers, {nsteps} LSRK14 steps at CFL 0.5, "
+                        f"{nsnap - 1} checkpoints + the initial state, FP64",
+            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd, "J": J,
+            "forward_steps_per_gradient": fwd_per_grad, "adjoint_steps_per_gradient": nsteps,
+            "forward_dof_stage_per_s": dofs * 14 * nsteps / t_fwd,
+            "fd_directions": len(errs), "fd_rel_err_max": float(max(errs)) if errs else None,
+            "fd_rel_err": [float(e) for e in errs]}
+
+
 def curved_sample(args, seed=5):
     """The curved-bump workload: 3D Euler in an n^3 hex channel whose bottom wall
     carries a smooth bump of amplitude 0.05 (curved elements throughout,
@@ -450,6 +516,8 @@ def main():
     ap.add_argument("--adj-n", type=int, default=64, help="quads per direction of the adjoint sample")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--adj-dirs", type=int, default=8, help="finite-difference directions")
+    ap.add_argument("--eadj-n", type=int, default=64, help="quads per direction of the Euler adjoint sample")
+    ap.add_argument("--eadj-steps", type=int, default=200)
     ap.add_argument("--no-curved", action="store_true", help="skip the curved-bump sample")
     ap.add_argument("--cv-n", type=int, default=96, help="hexes per direction of the curved sample")
     ap.add_argument("--cv-steps", type=int, default=2)
@@ -545,6 +613,7 @@ def main():
         line["serving"] = serving_sample(args)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_sample(args)
+        line["euler_adjoint"] = euler_adjoint_sample(args)
     if not args.no_curved:
         line["curved"] = curved_sample(args)
     if not args.no_hexblock:

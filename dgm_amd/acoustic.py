@@ -92,6 +92,7 @@ class AcousticProblem:
         _check(self.lib, self.lib.dgx_acoustic_create(ctypes.byref(md), ctypes.byref(ad),
                                                       ctypes.c_void_p(stream or 0), ctypes.byref(self.h)))
         self.dt = float(dt)
+        self.control_shape = (self.ne,)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -141,17 +142,20 @@ class AcousticProblem:
 class ReoOptProblem:
     """The optimisation seam of pydgm (``ReoOptProblem.computeState``,
     ``computeAdjoint``, ``getGradient``, ``getControl``; python/dgm.pyx:322-355,
-    Reo::OptProblem) over the acoustic inverse-source problem: the control is
-    the source field theta, the state a forward solve, the objective
-    J = 1/2 dt sum (p' - d)^2 against `data`, the gradient the checkpointed
-    discrete adjoint's.  computeState returns J for the current control;
-    computeAdjoint evaluates dJ/dtheta, which getGradient then returns."""
+    Reo::OptProblem) over an adjoint problem: the acoustic inverse-source
+    problem (control: the source field theta) or the nonlinear Euler problem
+    of dgm_amd.euler_adjoint (control: the initial modal state).  The state is
+    a forward solve, the objective J = 1/2 dt sum (p - d)^2 against `data`, the
+    gradient the checkpointed discrete adjoint's.  computeState returns J for
+    the current control; computeAdjoint evaluates dJ/dcontrol, which
+    getGradient then returns."""
 
-    def __init__(self, problem: AcousticProblem, data, theta0=None):
+    def __init__(self, problem, data, theta0=None):
         self.problem = problem
         self.data = np.ascontiguousarray(data, dtype=np.float64)
-        self.control = np.zeros(problem.ne) if theta0 is None else np.array(theta0, dtype=np.float64)
-        self.gradient = np.zeros(problem.ne)
+        shape = problem.control_shape
+        self.control = np.zeros(shape) if theta0 is None else np.array(theta0, dtype=np.float64)
+        self.gradient = np.zeros(shape)
         self.J = None
 
     def getControl(self) -> np.ndarray:

@@ -1,0 +1,353 @@
+// Backward kernels of the 2D Euler residual (quads, Lax-Friedrichs, affine,
+// periodic): the vector-Jacobian product out = (dR/dU)^T lam of the residual
+// R = -dU/dt that k_traces2 + k_rhs2 compute (Euler::convection,
+// convective_flux, lift and project, src/Euler.cpp:296-757,
+// src/Quad.cpp:1971-2038), and the pressure observation at receiver points
+// with its adjoint.  They drive the discrete adjoint of LSRK14 time stepping
+// (LSRK14::adjoint_advance, src/TimeInt.cpp:2064-2092) for gradients with
+// respect to the initial state (dgx_euler_adjoint.cpp).
+//
+// The forward residual of element e, in modal coordinates, is
+//   R = (QD (x) BW) N1 + (BW (x) QD) N2 + sum_s Bend_s (x) (PW g_s),
+//   N1 = F(u).(rx, ry), N2 = F(u).(sx, sy) at the GLL nodes, u = (BT (x) BT) U,
+//   g_s = J_side/J_vol (F^(u-, u+, n) - F(u-).n) at the Gauss points of side s,
+//   u- = the element's trace (Bend, Bg), u+ = the neighbour's.
+// Its transpose, applied to lam:
+//   volume  N1bar = (QD (x) BW)^T lam, N2bar = (BW (x) QD)^T lam, then
+//           ubar = A(u; rx, ry)^T N1bar + A(u; sx, sy)^T N2bar at every node
+//           (A(u; k) = d(F(u).k)/du) and Ubar += (BT (x) BT)^T ubar;
+//   faces   gbar_s = PW^T (the Bend collapse of lam): the adjoint traces, which
+//           k_adjtraces2 evaluates for every element first, since each side's
+//           trace enters both its own and its neighbour's face flux;
+//           u-bar = jr (dF^/du- - A(u-; n))^T gbar_own + jr' (dF^'/du+')^T gbar_nb
+//           with the neighbour's flux F^' = F^(u+, u-, n'), then the transpose
+//           of the trace evaluation into the modes.
+// The Lax-Friedrichs speed max(|u.n| + c) is differentiated on the side that
+// attains it.
+#include "dgx_kernels.hpp"
+
+namespace dgx {
+
+// A(U; k)^T w for the 2D Euler flux F(U).k with an unnormalized vector k:
+// F.k = (rho vn, m vn + p kx, n vn + p ky, (E + p) vn), vn = (m kx + n ky) / rho.
+__device__ __forceinline__ void euler_jac_t(const double U[4], double kx, double ky, double gm1, const double w[4],
+                                            double out[4]) {
+  const double ir = 1.0 / U[0], u = U[1] * ir, v = U[2] * ir;
+  const double q2 = u * u + v * v;
+  const double p = gm1 * (U[3] - 0.5 * U[0] * q2);
+  const double vn = u * kx + v * ky, H = (U[3] + p) * ir;
+  const double dp[4] = {0.5 * gm1 * q2, -gm1 * u, -gm1 * v, gm1};
+  // rows of A: r0 = (0, kx, ky, 0)
+  //            r1 = (-u vn, u kx + vn, u ky, 0) + kx dp
+  //            r2 = (-v vn, v kx, v ky + vn, 0) + ky dp
+  //            r3 = (vn (0.5 gm1 q2 - H), H kx - gm1 u vn, H ky - gm1 v vn, (gm1 + 1) vn)
+  out[0] = w[1] * (-u * vn + kx * dp[0]) + w[2] * (-v * vn + ky * dp[0]) + w[3] * (vn * (0.5 * gm1 * q2 - H));
+  out[1] = w[0] * kx + w[1] * (u * kx + vn + kx * dp[1]) + w[2] * (v * kx + ky * dp[1]) + w[3] * (H * kx - gm1 * u * vn);
+  out[2] = w[0] * ky + w[1] * (u * ky + kx * dp[2]) + w[2] * (v * ky + vn + ky * dp[2]) + w[3] * (H * ky - gm1 * v * vn);
+  out[3] = w[1] * kx * dp[3] + w[2] * ky * dp[3] + w[3] * ((gm1 + 1.0) * vn);
+}
+
+// gradient of the wave speed |u.n| + c with respect to U
+__device__ __forceinline__ void speed_grad(const double U[4], double nx, double ny, double gamma, double g[4]) {
+  const double gm1 = gamma - 1.0, ir = 1.0 / U[0], u = U[1] * ir, v = U[2] * ir;
+  const double p = gm1 * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+  const double un = u * nx + v * ny, c = sqrt(gamma * p * ir), s = un >= 0.0 ? 1.0 : -1.0;
+  // d(un)/dU = (-un/rho, nx/rho, ny/rho, 0); c = sqrt(gamma p / rho):
+  // dc/dU = gamma / (2 c rho) (dp/dU - p/rho e_rho)
+  const double dp[4] = {0.5 * gm1 * (u * u + v * v), -gm1 * u, -gm1 * v, gm1};
+  const double f = gamma / (2.0 * c) * ir;
+  g[0] = s * (-un * ir) + f * (dp[0] - p * ir);
+  g[1] = s * (nx * ir) + f * dp[1];
+  g[2] = s * (ny * ir) + f * dp[2];
+  g[3] = f * dp[3];
+}
+
+// Transposed Jacobians of the Lax-Friedrichs flux F^(UL, UR, n) =
+// 1/2 (F(UL).n + F(UR).n + lam (UL - UR)), lam = max(lam(UL), lam(UR)),
+// applied to w: aL = (dF^/dUL)^T w, aR = (dF^/dUR)^T w.
+__device__ __forceinline__ void lf_jac_t(const double UL[4], const double UR[4], double nx, double ny, double gamma,
+                                         const double w[4], double aL[4], double aR[4]) {
+  const double gm1 = gamma - 1.0;
+  double FL[4], FR[4], lL, lR;
+  bool bad;
+  euler_normal_flux2(UL, nx, ny, gm1, FL, lL, bad);
+  euler_normal_flux2(UR, nx, ny, gm1, FR, lR, bad);
+  const double lam = fmax(lL, lR);
+  double jl[4], jr[4];
+  euler_jac_t(UL, nx, ny, gm1, w, jl);
+  euler_jac_t(UR, nx, ny, gm1, w, jr);
+  // d lam / dU on the attaining side, times (UL - UR) . w
+  double dw = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) dw += (UL[v] - UR[v]) * w[v];
+  double gl[4] = {0, 0, 0, 0}, gr[4] = {0, 0, 0, 0};
+  if (lL >= lR) speed_grad(UL, nx, ny, gamma, gl);
+  else speed_grad(UR, nx, ny, gamma, gr);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    aL[v] = 0.5 * (jl[v] + lam * w[v] + dw * gl[v]);
+    aR[v] = 0.5 * (jr[v] - lam * w[v] + dw * gr[v]);
+  }
+}
+
+// Adjoint traces of lam: Lt[e][s][v][g] = sum_k PW[k][g] (Bend collapse of lam
+// on side s)[v][k] (the transpose of the lift's face integral), 4 fields.
+template <int P>
+__global__ void __launch_bounds__(64) k_adjtraces2(const double *__restrict__ lam, double *__restrict__ Lt, int ne) {
+  constexpr int L = P + 1, QF = P + 1, L2 = L * L, NT = 64, NF = 4;
+  __shared__ double A[NF * L2], K[NF * 4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    A[i] = lam[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
+  // K[v][s][k]: edges 0 / 2 collapse m with Bend(-1) / Bend(+1) (index l);
+  // edges 3 / 1 collapse l (index m) -- as face_add2 distributes them
+  for (int i = tid; i < NF * 4 * L; i += NT) {
+    const int v = i / (4 * L), s = (i / L) & 3, k = i % L;
+    const double *a = A + v * L2;
+    double acc = 0.0;
+    if (s == 0 || s == 2) {
+      const double *b = s == 0 ? B0 : B1;
+      for (int m = 0; m < L; ++m) acc = fma(a[k * L + m], b[m], acc);
+    } else {
+      const double *b = s == 3 ? B0 : B1;
+      for (int l = 0; l < L; ++l) acc = fma(a[l * L + k], b[l], acc);
+    }
+    K[i] = acc;
+  }
+  __syncthreads();
+  const double *PW = CT<P>::PW();   // [L][QF]
+  double *Le = Lt + (size_t)e * 4 * NF * QF;
+  for (int i = tid; i < 4 * NF * QF; i += NT) {
+    const int s = i / (NF * QF), v = (i / QF) % NF, g = i % QF;
+    const double *k = K + (v * 4 + s) * L;
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) acc = fma(PW[l * QF + g], k[l], acc);
+    Le[(s * NF + v) * QF + g] = acc;
+  }
+}
+
+// out = (dR/dU)^T lam for element e (see the file comment).  T: traces of U,
+// Lt: adjoint traces of lam (k_adjtraces2).
+template <int P>
+__global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
+  constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q, NT = 64, NF = 4;
+  constexpr int NPT = (Q2 + NT - 1) / NT;
+  __shared__ double M[NF * L2], X[NF * Q * L], N1[NF * Q2], N2[NF * Q2], Ub[NF * Q2], gT[NF * 4 * QF],
+      Kb[NF * 4 * L];
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const Geo &geo = a.geo;
+  const double gamma = a.gamma, gm1 = gamma - 1.0;
+  // nodal state u = (BT (x) BT) U
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    M[i] = a.U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), M, X, tid);   // [NF][Q][L]
+  __syncthreads();
+  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), X, N1, tid);  // [NF][Q][Q]
+  __syncthreads();
+  double un[NPT][NF];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int k = tid + r * NT;
+#pragma unroll
+    for (int v = 0; v < NF; ++v) un[r][v] = k < Q2 ? N1[v * Q2 + k] : 1.0;
+  }
+  __syncthreads();
+  // volume transpose: N1bar[i][j] = sum_l QD[l][i] sum_m BW[m][j] lam[l][m],
+  //                   N2bar[i][j] = sum_l BW[l][i] sum_m QD[m][j] lam[l][m]
+  for (int i = tid; i < NF * L2; i += NT) {
+    const int v = i / L2;
+    M[i] = a.lam[((size_t)v * ne + e) * L2 + (i - v * L2)];
+  }
+  __syncthreads();
+  const double *QD = CT<P>::QD(), *BW = CT<P>::BW();   // [L][Q]
+  for (int pass = 0; pass < 2; ++pass) {
+    const double *Mi = pass == 0 ? QD : BW, *Mj = pass == 0 ? BW : QD;
+    double *Nb = pass == 0 ? N1 : N2;
+    for (int t = tid; t < NF * L * Q; t += NT) {   // X[v][l][j] = sum_m Mj[m][j] lam[v][l][m]
+      const int v = t / (L * Q), l = (t / Q) % L, j = t % Q;
+      double acc = 0.0;
+      for (int m = 0; m < L; ++m) acc = fma(Mj[m * Q + j], M[v * L2 + l * L + m], acc);
+      X[t] = acc;
+    }
+    __syncthreads();
+    for (int t = tid; t < NF * Q2; t += NT) {   // Nb[v][i][j] = sum_l Mi[l][i] X[v][l][j]
+      const int v = t / Q2, i = (t / Q) % Q, j = t % Q;
+      double acc = 0.0;
+      for (int l = 0; l < L; ++l) acc = fma(Mi[l * Q + i], X[(v * L + l) * Q + j], acc);
+      Nb[t] = acc;
+    }
+    __syncthreads();
+  }
+  // pointwise: ubar = A(u; rx, ry)^T N1bar + A(u; sx, sy)^T N2bar
+  const double *mt = geo.metric + 9 * (size_t)e;
+  const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int k = tid + r * NT;
+    if (k >= Q2) continue;
+    double w1[4], w2[4], o1[4], o2[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { w1[v] = N1[v * Q2 + k]; w2[v] = N2[v * Q2 + k]; }
+    euler_jac_t(un[r], rx, ry, gm1, w1, o1);
+    euler_jac_t(un[r], sx, sy, gm1, w2, o2);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ub[v * Q2 + k] = o1[v] + o2[v];
+  }
+  // faces: adjoint of the own traces at the Gauss points
+  for (int i = tid; i < 4 * QF; i += NT) {
+    const int s = i / QF, g = i - s * QF;
+    const int es = e * 4 + s;
+    const int nb = geo.nbr[es], ns = geo.nside[es];
+    const int gn = geo.perm[geo.permid[es] * QF + g];
+    const double nx = geo.fnorm[3 * es], ny = geo.fnorm[3 * es + 1];
+    const double mx = geo.fnorm[3 * (nb * 4 + ns)], my = geo.fnorm[3 * (nb * 4 + ns) + 1];
+    const double jr = geo.fjr[es], jrn = geo.fjr[nb * 4 + ns];
+    const double *tl = a.T + ((size_t)es) * NF * QF, *tr = a.T + ((size_t)nb * 4 + ns) * NF * QF;
+    const double *ll = a.Lt + ((size_t)es) * NF * QF, *lr = a.Lt + ((size_t)nb * 4 + ns) * NF * QF;
+    double UL[4], UR[4], wo[4], wn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      UL[v] = tl[v * QF + g]; UR[v] = tr[v * QF + gn];
+      wo[v] = jr * ll[v * QF + g]; wn[v] = jrn * lr[v * QF + gn];
+    }
+    double aL[4], aR[4], jf[4], bL[4], bR[4];
+    lf_jac_t(UL, UR, nx, ny, gamma, wo, aL, aR);     // own face flux F^(UL, UR, n)
+    euler_jac_t(UL, nx, ny, gm1, wo, jf);             // minus F(UL).n
+    lf_jac_t(UR, UL, mx, my, gamma, wn, bL, bR);     // the neighbour's F^(UR, UL, n_nb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gT[(v * 4 + s) * QF + g] = aL[v] - jf[v] + bR[v];
+  }
+  __syncthreads();
+  // transpose of the trace evaluation: Kb[v][s][k] = sum_g Bg[g][k] gT[v][s][g]
+  const double *Bg = CT<P>::Bg();   // [QF][L]
+  for (int i = tid; i < NF * 4 * L; i += NT) {
+    const int vs = i / L, k = i % L;
+    double acc = 0.0;
+#pragma unroll
+    for (int g = 0; g < QF; ++g) acc = fma(Bg[g * L + k], gT[vs * QF + g], acc);
+    Kb[i] = acc;
+  }
+  // modal: (BT (x) BT)^T ubar + the Bend distribution of Kb (face_add2's pattern)
+  for (int t = tid; t < NF * Q * L; t += NT) {   // X[v][i][m] = sum_j BT[j][m] Ub[v][i][j]
+    const int v = t / (Q * L), i = (t / L) % Q, m = t % L;
+    double acc = 0.0;
+    for (int j = 0; j < Q; ++j) acc = fma(CT<P>::BT()[j * L + m], Ub[(v * Q + i) * Q + j], acc);
+    X[t] = acc;
+  }
+  __syncthreads();
+  const double cw[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int t = tid; t < NF * L2; t += NT) {
+    const int v = t / L2, k = t - v * L2, l = k / L, m = k - l * L;
+    double acc = 0.0;
+    for (int i = 0; i < Q; ++i) acc = fma(CT<P>::BT()[i * L + l], X[(v * Q + i) * L + m], acc);
+    acc += face_add2<P>(CT<P>::Bend(), Kb + v * 4 * L, l, m, cw);
+    const size_t o = ((size_t)v * ne + e) * L2 + k;
+    a.out[o] = a.accumulate ? fma(a.scale, acc, a.out[o]) : a.scale * acc;
+  }
+}
+
+// the state at receiver r: u[f] = sum_m w[r][m] U[f][e_r][m]
+__device__ __forceinline__ void receiver_state(const double *__restrict__ U, int e, const double *__restrict__ w,
+                                               int ne, int L2, double u[4]) {
+  for (int f = 0; f < 4; ++f) {
+    const double *Uf = U + ((size_t)f * ne + e) * L2;
+    double s = 0.0;
+    for (int m = 0; m < L2; ++m) s = fma(w[m], Uf[m], s);
+    u[f] = s;
+  }
+}
+
+// pressure (gamma - 1)(E - 1/2 (m u + n v)), u = m / rho, v = n / rho; the
+// forward observation and its adjoint evaluate it alike, so that data the
+// forward solve produced give a zero misfit
+__device__ __forceinline__ double receiver_pressure(const double u[4], double gm1, double &vx, double &vy) {
+#pragma clang fp contract(off)
+  const double iu = 1.0 / u[0];
+  vx = u[1] * iu;
+  vy = u[2] * iu;
+  return gm1 * (u[3] - 0.5 * (u[1] * vx + u[2] * vy));
+}
+
+// y[r] = the pressure of the state at receiver r; one thread per receiver
+__global__ void k_pobs2(const double *__restrict__ U, const int *__restrict__ rec_e, const double *__restrict__ rec_w,
+                        int nrec, int ne, int L2, double gamma, double *__restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  double u[4], vx, vy;
+  receiver_state(U, rec_e[r], rec_w + (size_t)r * L2, ne, L2, u);
+  y[r] = receiver_pressure(u, gamma - 1.0, vx, vy);
+}
+
+// adjoint of the objective 1/2 dt sum_r (p_r - d_r)^2 at one observation:
+// lam[f][e_r][m] += dt (p_r - d_r) dp/du_f w[r][m], J += 1/2 dt sum (p_r - d_r)^2;
+// one workgroup, receivers in order (no atomics: receivers may share elements)
+__global__ void __launch_bounds__(64) k_pobs2_adj(const double *__restrict__ U, double *__restrict__ lam,
+                                                  const int *__restrict__ rec_e, const double *__restrict__ rec_w,
+                                                  int nrec, int ne, int L2, double gamma, const double *__restrict__ d,
+                                                  double dt, double *__restrict__ J) {
+  __shared__ double c[4], res;
+  const int tid = threadIdx.x;
+  double jacc = 0.0;
+  for (int r = 0; r < nrec; ++r) {
+    const double *w = rec_w + (size_t)r * L2;
+    if (tid == 0) {
+      double u[4], vx, vy;
+      receiver_state(U, rec_e[r], w, ne, L2, u);
+      const double gm1 = gamma - 1.0, p = receiver_pressure(u, gm1, vx, vy);
+      const double dr = p - d[r];
+      jacc += 0.5 * dt * dr * dr;
+      res = dt * dr;
+      c[0] = 0.5 * gm1 * (vx * vx + vy * vy); c[1] = -gm1 * vx; c[2] = -gm1 * vy; c[3] = gm1;
+    }
+    __syncthreads();
+    for (int i = tid; i < 4 * L2; i += 64) {
+      const int f = i / L2, m = i - f * L2;
+      lam[((size_t)f * ne + rec_e[r]) * L2 + m] += res * c[f] * w[m];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) J[0] += jacc;
+}
+
+template <int P>
+static hipError_t launch_adj2_p(const AdjArgs &a) {
+  if (a.op == 0) hipLaunchKernelGGL(k_adjtraces2<P>, dim3(a.ne), dim3(64), 0, a.stream, a.lam, a.Lt, a.ne);
+  else hipLaunchKernelGGL(k_vjp2<P>, dim3(a.ne), dim3(64), 0, a.stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj2(const AdjArgs &a) {
+  switch (a.p) {
+#define DGX_CASE(PP) case PP: return launch_adj2_p<PP>(a);
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)
+    DGX_CASE(6) DGX_CASE(7) DGX_CASE(8) DGX_CASE(9) DGX_CASE(10)
+#undef DGX_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pobs2(const double *U, const int *rec_e, const double *rec_w, int nrec, int ne, int L2,
+                        double gamma, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_pobs2, dim3((nrec + 63) / 64), dim3(64), 0, s, U, rec_e, rec_w, nrec, ne, L2, gamma, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_pobs2_adj(const double *U, double *lam, const int *rec_e, const double *rec_w, int nrec, int ne,
+                            int L2, double gamma, const double *d, double dt, double *J, hipStream_t s) {
+  hipLaunchKernelGGL(k_pobs2_adj, dim3(1), dim3(64), 0, s, U, lam, rec_e, rec_w, nrec, ne, L2, gamma, d, dt, J);
+  return hipGetLastError();
+}
+
+hipError_t upload_tables_a2d(const double *h) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+}
+
+}  // namespace dgx

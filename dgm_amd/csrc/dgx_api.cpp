@@ -263,7 +263,8 @@ int upload_all_tables() {
       (e = dgx::upload_tables_c3d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_c2d(packed.data())) != hipSuccess ||
       (e = dgx::upload_tables_cfl(packed.data())) != hipSuccess ||
-      (e = dgx::upload_tables_v2d(packed.data())) != hipSuccess)
+      (e = dgx::upload_tables_v2d(packed.data())) != hipSuccess ||
+      (e = dgx::upload_tables_a2d(packed.data())) != hipSuccess)
     return fail(DGX_ERR_DEVICE, std::string("basis table upload: ") + hipGetErrorString(e));
   status = 0;
   return DGX_OK;
@@ -306,12 +307,17 @@ hipError_t timed(dgx_domain *d, int kind, Fn fn) {
 
 hipError_t launch_varp(const dgx_domain *d, int op, const double *in, double *out);
 
-int launch(dgx_domain *d, int op, const double *U, double *out) {
+dgx::LaunchArgs launch_args(dgx_domain *d, int op, const double *U, double *out) {
   dgx::LaunchArgs a;
   a.op = op; a.p = d->p; a.ne = d->ne; a.nfield = d->nf; a.ns = d->visc;
   a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
   a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
-  a.eqn = d->eqn; a.Bal = d->Bal; a.bal = op == dgx::OP_RHS ? d->bal : 0;
+  a.eqn = d->eqn; a.Bal = d->Bal; a.bal = (op == dgx::OP_RHS || op == dgx::OP_STEP) ? d->bal : 0;
+  return a;
+}
+
+int launch(dgx_domain *d, int op, const double *U, double *out) {
+  dgx::LaunchArgs a = launch_args(d, op, U, out);
   hipError_t e = timed(d, op, [&] {
     if (d->varp) return launch_varp(d, op, U, out);
     return (d->nsd == 2) ? dgx::launch_2d(a) : dgx::launch_3d(a);
@@ -421,6 +427,26 @@ int residual_core(dgx_domain *d, const double *U, double *R) {
     if ((rc = launch(d, dgx::OP_VISC, U, nullptr))) return rc;
   }
   if ((rc = launch(d, dgx::OP_RHS, U, R))) return rc;
+  ++d->stages;
+  return DGX_OK;
+}
+
+// One LSRK stage of an affine 2D domain (not curved, one order): the traces
+// (and the viscous terms), then the residual fused with the register update
+// Ut = A Ut - dt R, U += B Ut (k_step2), bitwise the same as the residual
+// followed by the axpby and axpy.
+int step2(dgx_domain *d, double A, double B, double dt) {
+  int rc;
+  if ((rc = launch(d, dgx::OP_TRACES, d->U, nullptr))) return rc;
+  if (d->visc) {
+    if ((rc = launch(d, dgx::OP_GRAD, d->U, nullptr))) return rc;
+    if ((rc = launch(d, dgx::OP_VISC, d->U, nullptr))) return rc;
+  }
+  dgx::LaunchArgs a = launch_args(d, dgx::OP_STEP, d->U, nullptr);
+  a.Us = d->U; a.Ut = d->Ut; a.ca = A; a.cb = B; a.dt = dt;
+  hipError_t e = timed(d, dgx::OP_RHS, [&] { return dgx::launch_2d(a); });
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("stage launch: ") + hipGetErrorString(e));
+  if (d->bal == 1) d->bal = 2;   // the first residual is stored
   ++d->stages;
   return DGX_OK;
 }
@@ -1562,6 +1588,8 @@ int run_stage(dgx_domain *h, int inttype, int st, double dt, bool traces_ready) 
       RUN(launch_stage(h, sa));
       std::swap(h->T, h->T2);
       ++h->stages;
+    } else if (h->nsd == 2 && !h->curved && !h->varp && !h->bulk) {
+      RUN(step2(h, A, B, dt));
     } else {
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpby(-dt, h->R0, A, h->Ut, h->Ut, n, s));

@@ -13,7 +13,8 @@ enum Op {
   OP_GRAD = 2,      // U, T -> lifted gradients G          (Navier-Stokes)
   OP_VISC = 3,      // U, G -> viscous volume V, traces TV  (Navier-Stokes)
   OP_FORWARD = 4,   // nodal X -> modal Y
-  OP_BACKWARD = 5   // modal X -> nodal Y
+  OP_BACKWARD = 5,  // modal X -> nodal Y
+  OP_STEP = 6       // 2D: U, T [, V, TV] -> one LSRK stage in place: Ut = ca Ut - dt R, U += cb Ut
 };
 
 struct LaunchArgs {
@@ -27,6 +28,8 @@ struct LaunchArgs {
   int eqn = 0;                   // eqntype (shallow water: 30)
   double *Bal = nullptr;         // shallow water: the stored first residual
   int bal = 0;                   // 0 no balance term, 1 store the residual in Bal, 2 subtract Bal
+  double *Us = nullptr, *Ut = nullptr;   // OP_STEP: the state (updated in place) and the LSRK register
+  double ca = 0.0, cb = 0.0, dt = 0.0;   // OP_STEP: LSRK coefficients and time step
   Tab tab;
   Geo geo;
   Phys phys;
@@ -164,6 +167,29 @@ struct LinArgs {
   hipStream_t stream = nullptr;
 };
 hipError_t launch_lin2(const LinArgs &a);
+
+// Backward kernels of the 2D Euler residual R = -dU/dt on periodic affine
+// quads with the Lax-Friedrichs flux (dgx_adjoint2d.hip).  op 0: adjoint
+// traces Lt [ne][4][4][QF] of lam; op 1: out = scale (dR/dU(U))^T lam, added
+// to out when accumulate, with T the face traces of U and Lt those of lam.
+struct AdjArgs {
+  int op = 0, p = 0, ne = 0;
+  const double *U = nullptr, *T = nullptr, *lam = nullptr;
+  double *Lt = nullptr, *out = nullptr;
+  double scale = 1.0, gamma = 1.4;
+  bool accumulate = false;
+  Geo geo;
+  hipStream_t stream = nullptr;
+};
+hipError_t launch_adj2(const AdjArgs &a);
+// Pressure (gamma-1)(E - |m|^2 / (2 rho)) of the state at receiver points,
+// y_r = p(sum_m w[r][m] U[.][e_r][m]); and the adjoint of the objective
+// 1/2 dt sum_r (y_r - d_r)^2 into lam, with J += the objective.
+hipError_t launch_pobs2(const double *U, const int *rec_e, const double *rec_w, int nrec, int ne, int L2,
+                        double gamma, double *y, hipStream_t s);
+hipError_t launch_pobs2_adj(const double *U, double *lam, const int *rec_e, const double *rec_w, int nrec, int ne,
+                            int L2, double gamma, const double *d, double dt, double *J, hipStream_t s);
+hipError_t upload_tables_a2d(const double *h);
 
 // Receiver observations y_r = sum_f c_f sum_m w[r][m] U[f][e_r][m] into y[r]
 // (observe), or the adjoint of the objective 1/2 dt sum_r (y_r - d_r)^2:

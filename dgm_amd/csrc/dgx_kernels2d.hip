@@ -223,6 +223,22 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
   rhs2_elem<P, NS>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
 }
 
+// One LSRK14 stage of an affine 2D domain in one kernel after the traces (and
+// the viscous terms): the residual of element e fused with the register
+// update Ut = A Ut - dt R, U += B Ut in its store, bitwise the same as the
+// residual followed by the axpby and axpy.  An element reads its own modes
+// only (its neighbours' through the traces), before its own store.
+template <int P, bool NS>
+__global__ void __launch_bounds__(64)
+k_step2(double *U, const double *__restrict__ T, const double *__restrict__ V, const double *__restrict__ TV,
+        double *__restrict__ Ut, Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
+  rhs2_elem<P, NS>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
+    const double ut = fma(mdt, r, A * Ut[o]);
+    Ut[o] = ut;
+    U[o] = fma(B, ut, U[o]);
+  });
+}
+
 // Shallow water (Shallow::time_derivative, src/Shallow.cpp:255-336): flux
 // divergence at the GLL points, Lax-Friedrichs face flux, lift, projection.
 // The balance term (src/Shallow.cpp:251-266): bal 1 stores this residual in
@@ -307,6 +323,17 @@ __global__ void __launch_bounds__(64)
 k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ R,
            double *__restrict__ Bal, int bal, Tab tb, Geo geo, Phys ph, int ne, int *__restrict__ flag) {
   shallow2_elem<P>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+k_shallow_step2(double *U, const double *__restrict__ T, double *__restrict__ Ut, double *__restrict__ Bal, int bal,
+                Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
+  shallow2_elem<P>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
+    const double ut = fma(mdt, r, A * Ut[o]);
+    Ut[o] = ut;
+    U[o] = fma(B, ut, U[o]);
+  });
 }
 
 // lifted, projected gradients of (u, v, e): G[e][2*prim + dir][L2]
@@ -752,6 +779,17 @@ static hipError_t launch2(const LaunchArgs &a) {
       else
         hipLaunchKernelGGL((k_rhs2<P, false>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.V, a.TV,
                            a.R, a.tab, a.geo, a.phys, ne, a.flag);
+      break;
+    case OP_STEP:
+      if (a.eqn == 30)
+        hipLaunchKernelGGL(k_shallow_step2<P>, dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.Ut, a.Bal, a.bal,
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+      else if (a.ns)
+        hipLaunchKernelGGL((k_step2<P, true>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV, a.Ut,
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+      else
+        hipLaunchKernelGGL((k_step2<P, false>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV, a.Ut,
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
       break;
     case OP_GRAD:
       hipLaunchKernelGGL(k_grad2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.G, a.tab, a.geo, ne);

@@ -369,7 +369,6 @@ template <int P, bool NS, bool UPDATE>
 __global__ void __launch_bounds__(kStageThreads)
 k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   using S = M3<P>;
-  using T = MT<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L2 = S::L2, L3 = S::L3, Q3 = S::Q3, QF2 = S::QF2,
                 LP = S::LP, FP = S::FP, NT = S::NT, NPT = S::NPT, NSL = S::NSL, NOB = S::NOB, FS = S::FS;
   extern __shared__ __attribute__((aligned(16))) float smf[];

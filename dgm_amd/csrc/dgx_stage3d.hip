@@ -687,7 +687,6 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
 
 template <int P>
 static hipError_t launch_stage3_p(const StageLaunch &s) {
-  using S = S3<P>;
   StageArgs a;
   a.T = s.T; a.V = s.V; a.TV = s.TV; a.U = s.U; a.Ut = s.Ut; a.R = s.R; a.Tn = s.Tn;
   a.ca = s.ca; a.cb = s.cb; a.dt = s.dt; a.ne = s.ne; a.flag = s.flag;

@@ -406,6 +406,55 @@ int dgx_acoustic_stats(dgx_acoustic_handle a, long long *forward_steps, long lon
 int dgx_revolve_count(int nsteps, int nsnap, long long *advances, int *max_slots, int *order_ok);
 
 /* ---------------------------------------------------------------------------
+ * Discrete adjoint of the nonlinear 2D Euler equations.
+ *
+ * LSRK14 steps of the 2D Euler residual (Euler::convection with the
+ * Lax-Friedrichs flux, lift and projection, src/Euler.cpp:296-757,
+ * src/Quad.cpp:1971-2038) from an initial modal state U0 [4][ne][(p+1)^2].
+ * After every step the pressure (gamma-1)(E - |m|^2 / (2 rho)) is observed at
+ * the receivers; J = 1/2 dt sum_steps sum_r (p_r - d_r)^2.  dJ/dU0 comes from
+ * the reversed LSRK14 recurrences (LSRK14::adjoint_advance,
+ * src/TimeInt.cpp:2064-2092) with the transposed residual Jacobian at every
+ * stage state (hand-written backward kernels of the residual), the forward
+ * states being recomputed from binomial checkpoints (cf. DTK::Griewank,
+ * src/DTK_Griewank.hpp:42-125).  Periodic affine quads, FP64.
+ * ------------------------------------------------------------------------- */
+typedef struct dgx_euler_adjoint *dgx_euler_adjoint_handle;
+
+typedef struct {
+  int nrec;                  /* number of receivers */
+  const double *xrec;        /* receiver points [nrec][2] */
+  double dt;                 /* time step */
+  int nsteps;                /* LSRK14 steps */
+  int nsnap;                 /* checkpoint slots, state 0 included (0: ceil(sqrt(nsteps)) + 1) */
+} dgx_euler_adjoint_desc;
+
+/* `mesh`: nsd 2, eqntype DGX_EULER, vis 0, DGX_FLUX_LAX_FRIEDRICHS,
+ * DGX_FP64, periodic parallelogram quads (no boundary rows, xnodes, functions
+ * or porder).  Allocates nsnap + 20 states of 4 x ne x (p+1)^2 doubles. */
+int dgx_euler_adjoint_create(const dgx_domain_desc *mesh, const dgx_euler_adjoint_desc *ad, void *stream,
+                             dgx_euler_adjoint_handle *out);
+int dgx_euler_adjoint_destroy(dgx_euler_adjoint_handle a);
+
+/* Forward solve from U0 (host); y[nsteps][nrec] receives the observations and
+ * UT the final modal state (host memory, each may be NULL).  Returns
+ * DGX_ERR_NONREALIZABLE if a density or pressure fell below aepsM. */
+int dgx_euler_adjoint_forward(dgx_euler_adjoint_handle a, const double *U0, double *y, double *UT);
+
+/* Objective J and gradient grad = dJ/dU0 [4][ne][(p+1)^2] for data[nsteps][nrec]. */
+int dgx_euler_adjoint_gradient(dgx_euler_adjoint_handle a, const double *U0, const double *data, double *J,
+                               double *grad);
+
+/* One transposed residual Jacobian: out = (dR/dU(U))^T lam, R = -dU/dt the
+ * modal residual of dgx_rhs (host arrays [4][ne][(p+1)^2]). */
+int dgx_euler_adjoint_vjp(dgx_euler_adjoint_handle a, const double *U, const double *lam, double *out);
+
+/* Forward and adjoint steps taken so far, and the most checkpoint slots the
+ * last gradient held at once. */
+int dgx_euler_adjoint_stats(dgx_euler_adjoint_handle a, long long *forward_steps, long long *adjoint_steps,
+                            int *snapshots);
+
+/* ---------------------------------------------------------------------------
  * Element-block seam: the batched hex transforms of GpGpu::hex_kernels
  * (src/GpGpu.hpp:24-57, src/GpGpu.cu:174-420 and 470-800), which
  * HexBlock::initialize_gpu / forward_transform / backward_transform drive for

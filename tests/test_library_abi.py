@@ -206,6 +206,38 @@ def test_acoustic_create_rejects_bad_arguments(lib, case, msg):
     assert not h.value
 
 
+@pytest.mark.parametrize("case,msg", [
+    (dict(nsd=3), b"2D mesh"),
+    (dict(eqn=30), b"DGX_EULER"),
+    (dict(vis=1e-3), b"vis = 0"),
+    (dict(precision=1), b"DGX_FP64"),
+    (dict(flux=1), b"Lax-Friedrichs"),
+    (dict(walls=True), b"periodic"),
+    (dict(dt=0.0), b"dt > 0"),
+    (dict(nsteps=0), b"nsteps"),
+    (dict(nrec=-1), b"receivers"),
+    (dict(nsnap=-1), b"nsnap"),
+])
+def test_euler_adjoint_create_rejects_bad_arguments(lib, case, msg):
+    from dgm_amd import euler_adjoint
+    euler_adjoint._bind(lib)
+    if case.get("nsd") == 3:
+        m = meshgen.periodic_hexes(2, 2, 2)
+    elif case.get("walls"):
+        m = meshgen.box_quads(3, 3, bc=(0, 1, 0, 1))
+    else:
+        m = meshgen.periodic_quads(3, 3)
+    d, keep = _desc(m, 2, case.get("eqn", 1), precision=case.get("precision", 0), flux=case.get("flux", 0),
+                    vis=case.get("vis", 0.0))
+    xr = np.array([[0.5, 0.5]])
+    ad = euler_adjoint.EulerAdjointDesc(case.get("nrec", 1), xr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        case.get("dt", 1e-3), case.get("nsteps", 5), case.get("nsnap", 0))
+    h = ctypes.c_void_p()
+    rc = lib.dgx_euler_adjoint_create(ctypes.byref(d), ctypes.byref(ad), None, ctypes.byref(h))
+    assert rc == -1 and msg in lib.dgx_last_error()
+    assert not h.value
+
+
 def _fn(start, code, consts, keep):
     start = np.asarray(start, dtype=np.int32)
     code = np.asarray(code, dtype=np.int32)
