@@ -24,6 +24,8 @@
 //           of the trace evaluation into the modes.
 // The Lax-Friedrichs speed max(|u.n| + c) is differentiated on the side that
 // attains it.
+#include <vector>
+
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -90,54 +92,76 @@ __device__ __forceinline__ void lf_jac_t(const double UL[4], const double UR[4],
   }
 }
 
-// Adjoint traces of lam: Lt[e][s][v][g] = sum_k PW[k][g] (Bend collapse of lam
-// on side s)[v][k] (the transpose of the lift's face integral), 4 fields.
+// Transposed basis tables of every order (this file's constant memory,
+// derived from the shared tables at upload): BTt [L][Q] = BT^T, BWt [Q][L] =
+// BW^T, QDt [Q][L] = QD^T, PWt [QF][L] = PW^T and Bgt [L][QF] = Bg^T, so that
+// every transposed contraction runs through the register-blocked `contract`.
+constexpr int adj_size(int p) { return 3 * (p + 1) * (p + 2) + 2 * (p + 1) * (p + 1); }
+constexpr int adj_offset(int p) {
+  int s = 0;
+  for (int k = 1; k < p; ++k) s += adj_size(k);
+  return s;
+}
+constexpr int kAdjTotal = adj_offset(kMaxOrder + 1);
+static __constant__ double c_adj[kAdjTotal];
+
+template <int P> struct AT {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, b = adj_offset(P);
+  __device__ static __forceinline__ const double *BTt() { return c_adj + b; }
+  __device__ static __forceinline__ const double *BWt() { return c_adj + b + L * Q; }
+  __device__ static __forceinline__ const double *QDt() { return c_adj + b + 2 * L * Q; }
+  __device__ static __forceinline__ const double *PWt() { return c_adj + b + 3 * L * Q; }
+  __device__ static __forceinline__ const double *Bgt() { return c_adj + b + 3 * L * Q + QF * L; }
+};
+
+// Adjoint traces of g: Lt[e][s][v][g] = sum_l PW[l][g] (the Bend collapse of g
+// on side s)[v][l], the transpose of the lift's face integral: the same
+// pattern as the face traces, with PW^T in place of Bg.  With a.lamU set, g is
+// first formed in place from the stage recurrence, g = lam + B lamU (lam's
+// own element only).
 template <int P>
-__global__ void __launch_bounds__(64) k_adjtraces2(const double *__restrict__ lam, double *__restrict__ Lt, int ne) {
+__global__ void __launch_bounds__(64) k_adjtraces2(AdjArgs a) {
   constexpr int L = P + 1, QF = P + 1, L2 = L * L, NT = 64, NF = 4;
-  __shared__ double A[NF * L2], K[NF * 4 * L];
-  const int e = blockIdx.x, tid = threadIdx.x;
+  __shared__ double A[NF * L2], Kb[NF * 2 * L], Ka[NF * 2 * L], O[NF * 2 * QF];
+  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   for (int i = tid; i < NF * L2; i += NT) {
     const int v = i / L2;
-    A[i] = lam[((size_t)v * ne + e) * L2 + (i - v * L2)];
-  }
-  __syncthreads();
-  const double *B0 = CT<P>::Bend(), *B1 = CT<P>::Bend() + L;
-  // K[v][s][k]: edges 0 / 2 collapse m with Bend(-1) / Bend(+1) (index l);
-  // edges 3 / 1 collapse l (index m) -- as face_add2 distributes them
-  for (int i = tid; i < NF * 4 * L; i += NT) {
-    const int v = i / (4 * L), s = (i / L) & 3, k = i % L;
-    const double *a = A + v * L2;
-    double acc = 0.0;
-    if (s == 0 || s == 2) {
-      const double *b = s == 0 ? B0 : B1;
-      for (int m = 0; m < L; ++m) acc = fma(a[k * L + m], b[m], acc);
-    } else {
-      const double *b = s == 3 ? B0 : B1;
-      for (int l = 0; l < L; ++l) acc = fma(a[l * L + k], b[l], acc);
+    const size_t o = ((size_t)v * ne + e) * L2 + (i - v * L2);
+    double g = a.lam[o];
+    if (a.lamU) {
+      g = fma(a.B, a.lamU[o], g);   // as the axpy lam = B lamU + lam
+      a.lam[o] = g;
     }
-    K[i] = acc;
+    A[i] = g;
   }
   __syncthreads();
-  const double *PW = CT<P>::PW();   // [L][QF]
-  double *Le = Lt + (size_t)e * 4 * NF * QF;
-  for (int i = tid; i < 4 * NF * QF; i += NT) {
-    const int s = i / (NF * QF), v = (i / QF) % NF, g = i % QF;
-    const double *k = K + (v * 4 + s) * L;
-    double acc = 0.0;
-#pragma unroll
-    for (int l = 0; l < L; ++l) acc = fma(PW[l * QF + g], k[l], acc);
-    Le[(s * NF + v) * QF + g] = acc;
+  double *Le = a.Lt + (size_t)e * 4 * NF * QF;
+  contract<NF, L, L, 2, 2, NT>(CT<P>::Bend(), A, Kb, tid);   // [NF][L][2]: edges 0 / 2
+  contract<NF, L, L, 1, 2, NT>(CT<P>::Bend(), A, Ka, tid);   // [NF][2][L]: edges 3 / 1
+  __syncthreads();
+  contract<NF, L, 2, 1, QF, NT>(AT<P>::PWt(), Kb, O, tid);   // [NF][QF][2]
+  __syncthreads();
+  for (int i = tid; i < NF * 2 * QF; i += NT) {
+    const int v = i / (2 * QF), r = i - v * 2 * QF, g = r >> 1, end = r & 1;
+    Le[((end ? 2 : 0) * NF + v) * QF + g] = O[i];
+  }
+  __syncthreads();
+  contract<NF, 2, L, 2, QF, NT>(AT<P>::PWt(), Ka, O, tid);   // [NF][2][QF]
+  __syncthreads();
+  for (int i = tid; i < NF * 2 * QF; i += NT) {
+    const int v = i / (2 * QF), r = i - v * 2 * QF, end = r / QF, g = r - end * QF;
+    Le[((end ? 1 : 3) * NF + v) * QF + g] = O[i];
   }
 }
 
-// out = (dR/dU)^T lam for element e (see the file comment).  T: traces of U,
-// Lt: adjoint traces of lam (k_adjtraces2).
+// out = scale (dR/dU)^T g for element e (see the file comment); T: traces of
+// U, Lt: adjoint traces of g (k_adjtraces2).  With a.stage, g's own element
+// is then replaced by A g (the stage recurrence's lam = A_s g).
 template <int P>
 __global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
   constexpr int L = P + 1, Q = P + 2, QF = P + 1, L2 = L * L, Q2 = Q * Q, NT = 64, NF = 4;
   constexpr int NPT = (Q2 + NT - 1) / NT;
-  __shared__ double M[NF * L2], X[NF * Q * L], N1[NF * Q2], N2[NF * Q2], Ub[NF * Q2], gT[NF * 4 * QF],
+  __shared__ double M[NF * L2], X[NF * Q * L], X2[NF * Q * L], N1[NF * Q2], N2[NF * Q2], gT[NF * 4 * QF],
       Kb[NF * 4 * L];
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const Geo &geo = a.geo;
@@ -159,34 +183,23 @@ __global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
 #pragma unroll
     for (int v = 0; v < NF; ++v) un[r][v] = k < Q2 ? N1[v * Q2 + k] : 1.0;
   }
-  __syncthreads();
-  // volume transpose: N1bar[i][j] = sum_l QD[l][i] sum_m BW[m][j] lam[l][m],
-  //                   N2bar[i][j] = sum_l BW[l][i] sum_m QD[m][j] lam[l][m]
+  // g; with a.stage the element's lam becomes A g
   for (int i = tid; i < NF * L2; i += NT) {
     const int v = i / L2;
-    M[i] = a.lam[((size_t)v * ne + e) * L2 + (i - v * L2)];
+    const size_t o = ((size_t)v * ne + e) * L2 + (i - v * L2);
+    const double g = a.lam[o];
+    M[i] = g;
+    if (a.stage) a.lam[o] = a.A * g;
   }
   __syncthreads();
-  const double *QD = CT<P>::QD(), *BW = CT<P>::BW();   // [L][Q]
-  for (int pass = 0; pass < 2; ++pass) {
-    const double *Mi = pass == 0 ? QD : BW, *Mj = pass == 0 ? BW : QD;
-    double *Nb = pass == 0 ? N1 : N2;
-    for (int t = tid; t < NF * L * Q; t += NT) {   // X[v][l][j] = sum_m Mj[m][j] lam[v][l][m]
-      const int v = t / (L * Q), l = (t / Q) % L, j = t % Q;
-      double acc = 0.0;
-      for (int m = 0; m < L; ++m) acc = fma(Mj[m * Q + j], M[v * L2 + l * L + m], acc);
-      X[t] = acc;
-    }
-    __syncthreads();
-    for (int t = tid; t < NF * Q2; t += NT) {   // Nb[v][i][j] = sum_l Mi[l][i] X[v][l][j]
-      const int v = t / Q2, i = (t / Q) % Q, j = t % Q;
-      double acc = 0.0;
-      for (int l = 0; l < L; ++l) acc = fma(Mi[l * Q + i], X[(v * L + l) * Q + j], acc);
-      Nb[t] = acc;
-    }
-    __syncthreads();
-  }
-  // pointwise: ubar = A(u; rx, ry)^T N1bar + A(u; sx, sy)^T N2bar
+  // volume transpose: N1bar = (QD^T (x) BW^T) g, N2bar = (BW^T (x) QD^T) g
+  contract<NF, L, L, 2, Q, NT>(AT<P>::BWt(), M, X, tid);    // [NF][L][Q]
+  contract<NF, L, L, 2, Q, NT>(AT<P>::QDt(), M, X2, tid);   // [NF][L][Q]
+  __syncthreads();
+  contract<NF, L, Q, 1, Q, NT>(AT<P>::QDt(), X, N1, tid);   // [NF][Q][Q]
+  contract<NF, L, Q, 1, Q, NT>(AT<P>::BWt(), X2, N2, tid);  // [NF][Q][Q]
+  __syncthreads();
+  // pointwise: ubar = A(u; rx, ry)^T N1bar + A(u; sx, sy)^T N2bar, into N1
   const double *mt = geo.metric + 9 * (size_t)e;
   const double rx = mt[0], ry = mt[1], sx = mt[3], sy = mt[4];
 #pragma unroll
@@ -199,7 +212,7 @@ __global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
     euler_jac_t(un[r], rx, ry, gm1, w1, o1);
     euler_jac_t(un[r], sx, sy, gm1, w2, o2);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Ub[v * Q2 + k] = o1[v] + o2[v];
+    for (int v = 0; v < 4; ++v) N1[v * Q2 + k] = o1[v] + o2[v];
   }
   // faces: adjoint of the own traces at the Gauss points
   for (int i = tid; i < 4 * QF; i += NT) {
@@ -226,41 +239,33 @@ __global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
     for (int v = 0; v < 4; ++v) gT[(v * 4 + s) * QF + g] = aL[v] - jf[v] + bR[v];
   }
   __syncthreads();
-  // transpose of the trace evaluation: Kb[v][s][k] = sum_g Bg[g][k] gT[v][s][g]
-  const double *Bg = CT<P>::Bg();   // [QF][L]
-  for (int i = tid; i < NF * 4 * L; i += NT) {
-    const int vs = i / L, k = i % L;
-    double acc = 0.0;
-#pragma unroll
-    for (int g = 0; g < QF; ++g) acc = fma(Bg[g * L + k], gT[vs * QF + g], acc);
-    Kb[i] = acc;
-  }
-  // modal: (BT (x) BT)^T ubar + the Bend distribution of Kb (face_add2's pattern)
-  for (int t = tid; t < NF * Q * L; t += NT) {   // X[v][i][m] = sum_j BT[j][m] Ub[v][i][j]
-    const int v = t / (Q * L), i = (t / L) % Q, m = t % L;
-    double acc = 0.0;
-    for (int j = 0; j < Q; ++j) acc = fma(CT<P>::BT()[j * L + m], Ub[(v * Q + i) * Q + j], acc);
-    X[t] = acc;
-  }
+  // transpose of the trace evaluation, Kb = Bg^T gT; modal (BT^T (x) BT^T) ubar
+  contract<NF, 4, QF, 2, L, NT>(AT<P>::Bgt(), gT, Kb, tid);     // [NF][4][L]
+  contract<NF, Q, Q, 2, L, NT>(AT<P>::BTt(), N1, X, tid);   // [NF][Q][L]
+  __syncthreads();
+  contract<NF, Q, L, 1, L, NT>(AT<P>::BTt(), X, M, tid);    // [NF][L][L]
   __syncthreads();
   const double cw[4] = {1.0, 1.0, 1.0, 1.0};
   for (int t = tid; t < NF * L2; t += NT) {
     const int v = t / L2, k = t - v * L2, l = k / L, m = k - l * L;
-    double acc = 0.0;
-    for (int i = 0; i < Q; ++i) acc = fma(CT<P>::BT()[i * L + l], X[(v * Q + i) * L + m], acc);
-    acc += face_add2<P>(CT<P>::Bend(), Kb + v * 4 * L, l, m, cw);
+    const double acc = M[t] + face_add2<P>(CT<P>::Bend(), Kb + v * 4 * L, l, m, cw);
     const size_t o = ((size_t)v * ne + e) * L2 + k;
     a.out[o] = a.accumulate ? fma(a.scale, acc, a.out[o]) : a.scale * acc;
   }
 }
 
-// the state at receiver r: u[f] = sum_m w[r][m] U[f][e_r][m]
+// The state at receiver r, u[f] = sum_m w[r][m] U[f][e_r][m], by one
+// wavefront: lane-strided partial sums, then a butterfly over the 64 lanes
+// (every lane ends with the same, deterministic value).
 __device__ __forceinline__ void receiver_state(const double *__restrict__ U, int e, const double *__restrict__ w,
                                                int ne, int L2, double u[4]) {
+  const int lane = threadIdx.x;
   for (int f = 0; f < 4; ++f) {
     const double *Uf = U + ((size_t)f * ne + e) * L2;
     double s = 0.0;
-    for (int m = 0; m < L2; ++m) s = fma(w[m], Uf[m], s);
+    for (int m = lane; m < L2; m += 64) s = fma(w[m], Uf[m], s);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     u[f] = s;
   }
 }
@@ -276,50 +281,59 @@ __device__ __forceinline__ double receiver_pressure(const double u[4], double gm
   return gm1 * (u[3] - 0.5 * (u[1] * vx + u[2] * vy));
 }
 
-// y[r] = the pressure of the state at receiver r; one thread per receiver
-__global__ void k_pobs2(const double *__restrict__ U, const int *__restrict__ rec_e, const double *__restrict__ rec_w,
-                        int nrec, int ne, int L2, double gamma, double *__restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrec) return;
+// y[r] = the pressure of the state at receiver r; one wavefront per receiver
+__global__ void __launch_bounds__(64) k_pobs2(ObsArgs a) {
+  const int r = blockIdx.x;
   double u[4], vx, vy;
-  receiver_state(U, rec_e[r], rec_w + (size_t)r * L2, ne, L2, u);
-  y[r] = receiver_pressure(u, gamma - 1.0, vx, vy);
+  receiver_state(a.U, a.rec_e[r], a.rec_w + (size_t)r * a.L2, a.ne, a.L2, u);
+  if (threadIdx.x == 0) a.y[r] = receiver_pressure(u, a.gamma - 1.0, vx, vy);
 }
 
-// adjoint of the objective 1/2 dt sum_r (p_r - d_r)^2 at one observation:
-// lam[f][e_r][m] += dt (p_r - d_r) dp/du_f w[r][m], J += 1/2 dt sum (p_r - d_r)^2;
-// one workgroup, receivers in order (no atomics: receivers may share elements)
-__global__ void __launch_bounds__(64) k_pobs2_adj(const double *__restrict__ U, double *__restrict__ lam,
-                                                  const int *__restrict__ rec_e, const double *__restrict__ rec_w,
-                                                  int nrec, int ne, int L2, double gamma, const double *__restrict__ d,
-                                                  double dt, double *__restrict__ J) {
-  __shared__ double c[4], res;
-  const int tid = threadIdx.x;
-  double jacc = 0.0;
-  for (int r = 0; r < nrec; ++r) {
-    const double *w = rec_w + (size_t)r * L2;
-    if (tid == 0) {
-      double u[4], vx, vy;
-      receiver_state(U, rec_e[r], w, ne, L2, u);
-      const double gm1 = gamma - 1.0, p = receiver_pressure(u, gm1, vx, vy);
-      const double dr = p - d[r];
-      jacc += 0.5 * dt * dr * dr;
-      res = dt * dr;
-      c[0] = 0.5 * gm1 * (vx * vx + vy * vy); c[1] = -gm1 * vx; c[2] = -gm1 * vy; c[3] = gm1;
-    }
-    __syncthreads();
-    for (int i = tid; i < 4 * L2; i += 64) {
-      const int f = i / L2, m = i - f * L2;
-      lam[((size_t)f * ne + rec_e[r]) * L2 + m] += res * c[f] * w[m];
-    }
-    __syncthreads();
+// Adjoint of the objective 1/2 dt sum_r (p_r - d_r)^2 at one observation,
+// first per receiver (one wavefront each): coef[r] = (dt (p_r - d_r) dp/du_f,
+// f = 0..3; 1/2 dt (p_r - d_r)^2)
+__global__ void __launch_bounds__(64) k_pobs2_res(ObsArgs a) {
+  const int r = blockIdx.x;
+  double u[4], vx, vy;
+  receiver_state(a.U, a.rec_e[r], a.rec_w + (size_t)r * a.L2, a.ne, a.L2, u);
+  if (threadIdx.x == 0) {
+    const double gm1 = a.gamma - 1.0, p = receiver_pressure(u, gm1, vx, vy);
+    const double dr = p - a.d[r], res = a.dt * dr;
+    double *c = a.coef + 5 * (size_t)r;
+    c[0] = res * (0.5 * gm1 * (vx * vx + vy * vy));
+    c[1] = res * (-gm1 * vx);
+    c[2] = res * (-gm1 * vy);
+    c[3] = res * gm1;
+    c[4] = 0.5 * a.dt * dr * dr;
   }
-  if (tid == 0) J[0] += jacc;
+}
+
+// then per element holding receivers (one workgroup each, receivers in their
+// order, so the sums are deterministic without atomics):
+// lam[f][e][m] += sum_r coef[r][f] w[r][m]; workgroup 0 adds sum_r coef[r][4]
+// to J in receiver order
+__global__ void __launch_bounds__(64) k_pobs2_scatter(ObsArgs a) {
+  const int g = blockIdx.x, e = a.grp_e[g], r0 = a.grp_ptr[g], r1 = a.grp_ptr[g + 1];
+  for (int t = threadIdx.x; t < 4 * a.L2; t += 64) {
+    const int f = t / a.L2, m = t - f * a.L2;
+    const size_t o = ((size_t)f * a.ne + e) * a.L2 + m;
+    double v = a.lam[o];
+    for (int k = r0; k < r1; ++k) {
+      const int r = a.grp_r[k];
+      v += a.coef[5 * (size_t)r + f] * a.rec_w[(size_t)r * a.L2 + m];
+    }
+    a.lam[o] = v;
+  }
+  if (g == 0 && threadIdx.x == 0) {
+    double jacc = 0.0;
+    for (int r = 0; r < a.nrec; ++r) jacc += a.coef[5 * (size_t)r + 4];
+    a.J[0] += jacc;
+  }
 }
 
 template <int P>
 static hipError_t launch_adj2_p(const AdjArgs &a) {
-  if (a.op == 0) hipLaunchKernelGGL(k_adjtraces2<P>, dim3(a.ne), dim3(64), 0, a.stream, a.lam, a.Lt, a.ne);
+  if (a.op == 0) hipLaunchKernelGGL(k_adjtraces2<P>, dim3(a.ne), dim3(64), 0, a.stream, a);
   else hipLaunchKernelGGL(k_vjp2<P>, dim3(a.ne), dim3(64), 0, a.stream, a);
   return hipGetLastError();
 }
@@ -334,20 +348,42 @@ hipError_t launch_adj2(const AdjArgs &a) {
   }
 }
 
-hipError_t launch_pobs2(const double *U, const int *rec_e, const double *rec_w, int nrec, int ne, int L2,
-                        double gamma, double *y, hipStream_t s) {
-  hipLaunchKernelGGL(k_pobs2, dim3((nrec + 63) / 64), dim3(64), 0, s, U, rec_e, rec_w, nrec, ne, L2, gamma, y);
+hipError_t launch_pobs2(const ObsArgs &a) {
+  if (a.nrec < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_pobs2, dim3(a.nrec), dim3(64), 0, a.stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_pobs2_adj(const double *U, double *lam, const int *rec_e, const double *rec_w, int nrec, int ne,
-                            int L2, double gamma, const double *d, double dt, double *J, hipStream_t s) {
-  hipLaunchKernelGGL(k_pobs2_adj, dim3(1), dim3(64), 0, s, U, lam, rec_e, rec_w, nrec, ne, L2, gamma, d, dt, J);
+hipError_t launch_pobs2_adj(const ObsArgs &a) {
+  if (a.nrec < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_pobs2_res, dim3(a.nrec), dim3(64), 0, a.stream, a);
+  hipLaunchKernelGGL(k_pobs2_scatter, dim3(a.ngrp), dim3(64), 0, a.stream, a);
   return hipGetLastError();
 }
 
 hipError_t upload_tables_a2d(const double *h) {
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  if (e != hipSuccess) return e;
+  std::vector<double> t(kAdjTotal);
+  for (int p = 1; p <= kMaxOrder; ++p) {
+    const int L = p + 1, Q = p + 2, QF = p + 1;
+    const double *BT = h + tab_offset(p), *BW = BT + Q * L, *QD = BW + L * Q, *Bg = QD + L * Q;
+    const double *PW = Bg + QF * L + 2 * L;
+    double *o = t.data() + adj_offset(p);
+    double *BTt = o, *BWt = o + L * Q, *QDt = o + 2 * L * Q, *PWt = o + 3 * L * Q, *Bgt = PWt + QF * L;
+    for (int i = 0; i < Q; ++i)
+      for (int l = 0; l < L; ++l) {
+        BTt[l * Q + i] = BT[i * L + l];
+        BWt[i * L + l] = BW[l * Q + i];
+        QDt[i * L + l] = QD[l * Q + i];
+      }
+    for (int g = 0; g < QF; ++g)
+      for (int l = 0; l < L; ++l) {
+        PWt[g * L + l] = PW[l * QF + g];
+        Bgt[l * QF + g] = Bg[g * L + l];
+      }
+  }
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_adj), t.data(), sizeof(double) * kAdjTotal);
 }
 
 }  // namespace dgx

@@ -21,6 +21,7 @@
 // advancing it).
 #include "dgx_adjoint_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -43,8 +44,9 @@ struct dgx_euler_adjoint {
   dgx_handle dom = nullptr;
   int p = 0, ne = 0, nm = 0, nrec = 0, nsteps = 0, nsnap = 0;
   double dt = 0.0;
-  int *d_rec_e = nullptr;
-  double *d_rec_w = nullptr, *d_y = nullptr, *d_data = nullptr, *d_J = nullptr;
+  int *d_rec_e = nullptr, *d_grp = nullptr;   // d_grp: receivers grouped by element (ObsArgs)
+  int ngrp = 0;
+  double *d_rec_w = nullptr, *d_y = nullptr, *d_data = nullptr, *d_J = nullptr, *d_coef = nullptr;
   double *lamU = nullptr, *lamT = nullptr, *Lt = nullptr, *SB = nullptr;
   std::vector<double *> snaps;
   int rec_step = -1;       // the step whose stage states the next advance records
@@ -58,12 +60,22 @@ struct dgx_euler_adjoint {
 namespace {
 
 void release(dgx_euler_adjoint *a) {
-  void *ptrs[] = {a->d_rec_e, a->d_rec_w, a->d_y, a->d_data, a->d_J, a->lamU, a->lamT, a->Lt, a->SB};
+  void *ptrs[] = {a->d_rec_e, a->d_grp, a->d_rec_w, a->d_y, a->d_data, a->d_J, a->d_coef,
+                  a->lamU, a->lamT, a->Lt, a->SB};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *s : a->snaps)
     if (s) (void)hipFree(s);
   if (a->dom) dgx_domain_destroy(a->dom);
+}
+
+dgx::ObsArgs obs_args(dgx_euler_adjoint *a) {
+  dgx::ObsArgs o;
+  o.U = a->dom->U; o.rec_e = a->d_rec_e; o.rec_w = a->d_rec_w; o.nrec = a->nrec; o.ne = a->ne;
+  o.L2 = (a->p + 1) * (a->p + 1); o.gamma = a->dom->gamma; o.stream = a->dom->stream;
+  o.lam = a->lamU; o.J = a->d_J; o.coef = a->d_coef; o.dt = a->dt;
+  o.grp_e = a->d_grp; o.grp_ptr = a->d_grp + a->ngrp; o.grp_r = a->d_grp + 2 * a->ngrp + 1; o.ngrp = a->ngrp;
+  return o;
 }
 
 int traces(dgx_euler_adjoint *a, const double *U) {
@@ -90,15 +102,20 @@ int stage(dgx_euler_adjoint *a, int s) {
   return DGX_OK;
 }
 
-// out = scale (dR/dU(U))^T lam (+ out when accumulate); overwrites the
-// domain's traces with those of U
-int vjp(dgx_euler_adjoint *a, const double *U, const double *lam, double *out, double scale, bool accumulate) {
+// out = (dR/dU(U))^T lam; or, with `s` >= 0, reversed LSRK stage s at the
+// stage state U: g = lam + B_s lamU, lamU -= dt (dR/dU(U))^T g, lam = A_s g
+// (lam = lamT, out = lamU).  Overwrites the domain's traces with those of U.
+int vjp(dgx_euler_adjoint *a, const double *U, double *lam, double *out, int s) {
   dgx_domain *d = a->dom;
   int rc = traces(a, U);
   if (rc) return rc;
   dgx::AdjArgs aa;
   aa.p = a->p; aa.ne = a->ne; aa.U = U; aa.T = d->T; aa.lam = lam; aa.Lt = a->Lt; aa.out = out;
-  aa.scale = scale; aa.accumulate = accumulate; aa.gamma = d->gamma; aa.geo = d->geo; aa.stream = d->stream;
+  aa.gamma = d->gamma; aa.geo = d->geo; aa.stream = d->stream;
+  if (s >= 0) {
+    aa.lamU = out; aa.B = dgx::kLsrkB[s]; aa.A = dgx::kLsrkA[s]; aa.stage = true;
+    aa.scale = -a->dt; aa.accumulate = true;
+  }
   aa.op = 0;
   HIPCHK(dgx::launch_adj2(aa));
   aa.op = 1;
@@ -126,16 +143,15 @@ int step_forward(dgx_euler_adjoint *a, int k) {
 int step_adjoint(dgx_euler_adjoint *a, int k) {
   dgx_domain *d = a->dom;
   if (a->have_step != k) return fail(DGX_ERR_ARGUMENT, "adjoint step " + std::to_string(k) + " has no stage states");
-  const int L2 = (a->p + 1) * (a->p + 1);
   const size_t n = a->n();
-  if (a->nrec > 0)
-    HIPCHK(dgx::launch_pobs2_adj(d->U, a->lamU, a->d_rec_e, a->d_rec_w, a->nrec, a->ne, L2, d->gamma,
-                                 a->d_data + (size_t)k * a->nrec, a->dt, a->d_J, d->stream));
+  if (a->nrec > 0) {
+    dgx::ObsArgs o = obs_args(a);
+    o.d = a->d_data + (size_t)k * a->nrec;
+    HIPCHK(dgx::launch_pobs2_adj(o));
+  }
   for (int s = kStages - 1; s >= 0; --s) {
-    HIPCHK(dgx::blas_axpy(dgx::kLsrkB[s], a->lamU, a->lamT, a->lamT, n, d->stream));   // g = lamT + B_s lamU
-    int rc = vjp(a, a->SB + s * n, a->lamT, a->lamU, -a->dt, true);                    // lamU -= dt J^T g
+    int rc = vjp(a, a->SB + s * n, a->lamT, a->lamU, s);
     if (rc) return rc;
-    HIPCHK(dgx::blas_scale(dgx::kLsrkA[s], a->lamT, n, d->stream));                   // lamT = A_s g
   }
   a->have_step = -1;
   a->rec_step = k - 1;
@@ -215,6 +231,25 @@ int dgx_euler_adjoint_create(const dgx_domain_desc *mesh, const dgx_euler_adjoin
     release(a); delete a;
     return fail(DGX_ERR_MEMORY, std::string("hipMalloc failed: ") + what);
   };
+  // receivers grouped by element, in increasing order within each element:
+  // [grp_e (ngrp)][grp_ptr (ngrp + 1)][grp_r (nrec)]
+  std::vector<int> order(a->nrec);
+  for (int r = 0; r < a->nrec; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return rec_e[x] < rec_e[y]; });
+  std::vector<int> ge, gp;
+  for (int k = 0; k < a->nrec; ++k)
+    if (k == 0 || rec_e[order[k]] != rec_e[order[k - 1]]) {
+      ge.push_back(rec_e[order[k]]);
+      gp.push_back(k);
+    }
+  gp.push_back(a->nrec);
+  a->ngrp = (int)ge.size();
+  std::vector<int> grp(ge);
+  grp.insert(grp.end(), gp.begin(), gp.end());
+  grp.insert(grp.end(), order.begin(), order.end());
+  if (hipMalloc((void **)&a->d_grp, std::max<size_t>(1, grp.size()) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&a->d_coef, std::max<size_t>(1, 5 * (size_t)a->nrec) * 8) != hipSuccess)
+    return fail_mem("receiver groups");
   if (hipMalloc((void **)&a->d_rec_e, std::max(1, a->nrec) * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&a->d_rec_w, std::max<size_t>(1, rec_w.size()) * 8) != hipSuccess ||
       hipMalloc((void **)&a->d_y, nobs * 8) != hipSuccess || hipMalloc((void **)&a->d_data, nobs * 8) != hipSuccess ||
@@ -229,6 +264,7 @@ int dgx_euler_adjoint_create(const dgx_domain_desc *mesh, const dgx_euler_adjoin
   if (a->nrec > 0) {
     (void)hipMemcpyAsync(a->d_rec_e, rec_e.data(), a->nrec * sizeof(int), hipMemcpyHostToDevice, st);
     (void)hipMemcpyAsync(a->d_rec_w, rec_w.data(), rec_w.size() * 8, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(a->d_grp, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice, st);
   }
   if (hipStreamSynchronize(st) != hipSuccess) { release(a); delete a; return fail(DGX_ERR_DEVICE, "upload"); }
   *out = a;
@@ -248,13 +284,14 @@ int dgx_euler_adjoint_forward(dgx_euler_adjoint_handle a, const double *U0, doub
   int rc;
   if ((rc = upload_state(a, U0))) return rc;
   dgx_domain *d = a->dom;
-  const int L2 = (a->p + 1) * (a->p + 1);
   a->rec_step = -1;
   for (int k = 0; k < a->nsteps; ++k) {
     if ((rc = step_forward(a, k))) return rc;
-    if (a->nrec > 0)
-      HIPCHK(dgx::launch_pobs2(d->U, a->d_rec_e, a->d_rec_w, a->nrec, a->ne, L2, d->gamma,
-                               a->d_y + (size_t)k * a->nrec, d->stream));
+    if (a->nrec > 0) {
+      dgx::ObsArgs o = obs_args(a);
+      o.y = a->d_y + (size_t)k * a->nrec;
+      HIPCHK(dgx::launch_pobs2(o));
+    }
   }
   if (y && a->nrec > 0)
     HIPCHK(hipMemcpyAsync(y, a->d_y, (size_t)a->nsteps * a->nrec * 8, hipMemcpyDeviceToHost, d->stream));
@@ -308,7 +345,7 @@ int dgx_euler_adjoint_vjp(dgx_euler_adjoint_handle a, const double *U, const dou
   const size_t n = a->n();
   HIPCHK(hipMemcpyAsync(d->U, U, n * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(a->lamT, lam, n * 8, hipMemcpyHostToDevice, st));
-  int rc = vjp(a, d->U, a->lamT, a->lamU, 1.0, false);
+  int rc = vjp(a, d->U, a->lamT, a->lamU, -1);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(out, a->lamU, n * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

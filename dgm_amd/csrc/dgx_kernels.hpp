@@ -170,25 +170,43 @@ hipError_t launch_lin2(const LinArgs &a);
 
 // Backward kernels of the 2D Euler residual R = -dU/dt on periodic affine
 // quads with the Lax-Friedrichs flux (dgx_adjoint2d.hip).  op 0: adjoint
-// traces Lt [ne][4][4][QF] of lam; op 1: out = scale (dR/dU(U))^T lam, added
-// to out when accumulate, with T the face traces of U and Lt those of lam.
+// traces Lt [ne][4][4][QF] of g = lam (first formed in place as
+// lam = lam + B lamU when lamU is set); op 1: out = scale (dR/dU(U))^T g, added
+// to out when accumulate, with T the face traces of U and Lt those of g, and
+// with `stage` lam = A g afterwards.  A reversed LSRK stage is op 0 and op 1
+// with lamU = out = the state's adjoint, B = B_s, A = A_s, scale = -dt.
 struct AdjArgs {
   int op = 0, p = 0, ne = 0;
-  const double *U = nullptr, *T = nullptr, *lam = nullptr;
-  double *Lt = nullptr, *out = nullptr;
-  double scale = 1.0, gamma = 1.4;
-  bool accumulate = false;
+  const double *U = nullptr, *T = nullptr, *lamU = nullptr;
+  double *lam = nullptr, *Lt = nullptr, *out = nullptr;
+  double scale = 1.0, gamma = 1.4, A = 0.0, B = 0.0;
+  bool accumulate = false, stage = false;
   Geo geo;
   hipStream_t stream = nullptr;
 };
 hipError_t launch_adj2(const AdjArgs &a);
 // Pressure (gamma-1)(E - |m|^2 / (2 rho)) of the state at receiver points,
 // y_r = p(sum_m w[r][m] U[.][e_r][m]); and the adjoint of the objective
-// 1/2 dt sum_r (y_r - d_r)^2 into lam, with J += the objective.
-hipError_t launch_pobs2(const double *U, const int *rec_e, const double *rec_w, int nrec, int ne, int L2,
-                        double gamma, double *y, hipStream_t s);
-hipError_t launch_pobs2_adj(const double *U, double *lam, const int *rec_e, const double *rec_w, int nrec, int ne,
-                            int L2, double gamma, const double *d, double dt, double *J, hipStream_t s);
+// 1/2 dt sum_r (y_r - d_r)^2 into lam, with J += the objective.  The adjoint
+// takes coef [nrec][5] as scratch and the receivers grouped by element:
+// group g holds element grp_e[g] and receivers grp_r[grp_ptr[g] ..
+// grp_ptr[g+1]) in increasing order.
+struct ObsArgs {
+  const double *U = nullptr;
+  const int *rec_e = nullptr;
+  const double *rec_w = nullptr;
+  int nrec = 0, ne = 0, L2 = 0;
+  double gamma = 1.4;
+  double *y = nullptr;                              // forward: y[nrec]
+  double *lam = nullptr, *J = nullptr, *coef = nullptr;
+  const double *d = nullptr;                        // adjoint: data d[nrec]
+  double dt = 0.0;
+  const int *grp_e = nullptr, *grp_ptr = nullptr, *grp_r = nullptr;
+  int ngrp = 0;
+  hipStream_t stream = nullptr;
+};
+hipError_t launch_pobs2(const ObsArgs &a);
+hipError_t launch_pobs2_adj(const ObsArgs &a);
 hipError_t upload_tables_a2d(const double *h);
 
 // Receiver observations y_r = sum_f c_f sum_m w[r][m] U[f][e_r][m] into y[r]
