@@ -360,6 +360,7 @@ def curved_sample(args, seed=5):
     gpu.synchronize()
     el = time.perf_counter() - t0
     finite = bool(np.all(np.isfinite(gpu.norms())))
+    cg_iters = gpu.solver_iterations()
     gpu.close()
     value = dofs * 14 * args.cv_steps / el
     # HBM bytes per DOF-stage: 32 for the 2N state streams, plus the per-node
@@ -369,6 +370,7 @@ def curved_sample(args, seed=5):
                         f"on top, periodic sides), p={p}, Mach 0.5 + 1e-3 seeded density noise, LSRK14, FP64",
             "value": value, "unit": "DOF-stage/s", "dofs": dofs, "steps": args.cv_steps, "dt": dt,
             "ms_per_step": el / args.cv_steps * 1e3, "setup_s": t_setup, "state_finite": finite,
+            "mass_solve_cg_iterations_max": cg_iters,
             "hbm_gbs_at_algorithmic_bytes": value * (32.0 + metric_b) / 1e9,
             "algorithmic_bytes_per_dof_stage": 32.0 + metric_b}
 

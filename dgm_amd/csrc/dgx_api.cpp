@@ -329,7 +329,7 @@ int launch(dgx_domain *d, int op, const double *U, double *out) {
 
 dgx::CurvedArgs curved_args(dgx_domain *d) {
   dgx::CurvedArgs ca;
-  ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.geo = d->geo;
+  ca.gm = d->d_gm; ca.fgeo = d->d_fgeo; ca.jav = d->d_jav; ca.pre = d->d_pre; ca.geo = d->geo;
   ca.chol = d->nsd == 2 ? d->d_chol : nullptr;
   ca.phys = d->phys; ca.ne = d->ne; ca.p = d->p; ca.flag = d->d_flag; ca.iters = d->d_iters;
   ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
@@ -447,6 +447,21 @@ int step2(dgx_domain *d, double A, double B, double dt) {
   hipError_t e = timed(d, dgx::OP_RHS, [&] { return dgx::launch_2d(a); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("stage launch: ") + hipGetErrorString(e));
   if (d->bal == 1) d->bal = 2;   // the first residual is stored
+  ++d->stages;
+  return DGX_OK;
+}
+
+// One LSRK stage of a curved hex domain: the traces, then k_rhs3c with the
+// register update fused into its store (bitwise the same as the residual
+// followed by the axpby and axpy).
+int step_curved3(dgx_domain *d, double A, double B, double dt) {
+  int rc;
+  if ((rc = launch(d, dgx::OP_TRACES, d->U, nullptr))) return rc;
+  dgx::CurvedArgs ca = curved_args(d);
+  ca.U = d->U; ca.T = d->T; ca.R = nullptr;
+  ca.update = true; ca.Us = d->U; ca.Ut = d->Ut; ca.ca = A; ca.cb = B; ca.dt = dt;
+  hipError_t e = timed(d, dgx::OP_RHS, [&] { return launch_curved(d, ca); });
+  if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved stage: ") + hipGetErrorString(e));
   ++d->stages;
   return DGX_OK;
 }
@@ -700,11 +715,43 @@ int curved_geometry2(dgx_domain *d, const double *xn) {
   return DGX_OK;
 }
 
+// Inverse of a small symmetric positive definite n x n matrix (row-major) by
+// Cholesky factorization; false if it is not positive definite.
+bool spd_inverse(const double *A, int n, double *out) {
+  std::vector<double> C((size_t)n * n, 0.0), y(n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c <= r; ++c) {
+      double t = A[r * n + c];
+      for (int k = 0; k < c; ++k) t -= C[r * n + k] * C[c * n + k];
+      if (r == c) {
+        if (!(t > 0.0)) return false;
+        C[r * n + r] = std::sqrt(t);
+      } else {
+        C[r * n + c] = t / C[c * n + c];
+      }
+    }
+  for (int col = 0; col < n; ++col) {
+    for (int r = 0; r < n; ++r) {   // C y = e_col
+      double t = r == col ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) t -= C[r * n + k] * y[k];
+      y[r] = t / C[r * n + r];
+    }
+    for (int r = n - 1; r >= 0; --r) {   // C^T x = y
+      double t = y[r];
+      for (int k = r + 1; k < n; ++k) t -= C[k * n + r] * out[k * n + col];
+      out[r * n + col] = t / C[r * n + r];
+    }
+  }
+  return true;
+}
+
 int curved_geometry(dgx_domain *d, const double *xn) {
   const int ne = d->ne, Q = d->Q, QF = d->QF, Q3 = d->nq, QF2 = QF * QF, Q2 = Q * Q;
   const HostBasis &b = d->hb;
   const std::vector<double> &D = b.D, &w = b.w, &Ig = b.Ig;
   std::vector<double> gm((size_t)ne * 10 * Q3), fgeo((size_t)ne * 6 * QF2 * 4), jav(ne);
+  const int L = d->L, LL = L * L;
+  std::vector<double> pre((size_t)ne * 3 * LL);
   std::atomic<int> bad_elem(-1);
   auto work = [&](int e0, int e1) {
   std::vector<double> der(9 * (size_t)Q3), mn(9 * (size_t)Q3), J(Q3), wJ(Q3);
@@ -780,6 +827,32 @@ int curved_geometry(dgx_domain *d, const double *xn) {
       gm[((size_t)e * 10 + 9) * Q3 + n] = wJ[n];
     }
     jav[e] = swJ / sw;
+    // Preconditioner of the mass solve: J ~ ja(i) jb(j) jc(k), the log-mean
+    // separable fit, whose mass matrix is the tensor product of the 1D
+    // matrices BT^T diag(w jd) BT; their inverses, [3][L][L] per element
+    {
+      std::vector<double> lg(3 * (size_t)Q, 0.0);
+      double lm = 0.0;
+      for (int n = 0; n < Q3; ++n) {
+        const double l = std::log(J[n]);
+        const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+        lg[i] += l; lg[Q + j] += l; lg[2 * Q + k] += l;
+        lm += l;
+      }
+      lm /= Q3;
+      std::vector<double> A(LL), Ai(LL);
+      for (int dd = 0; dd < 3; ++dd) {
+        for (int a = 0; a < L; ++a)
+          for (int c = 0; c < L; ++c) {
+            double t = 0.0;
+            for (int i = 0; i < Q; ++i)
+              t += b.BT[i * L + a] * w[i] * std::exp(lg[dd * Q + i] / Q2 - 2.0 * lm / 3.0) * b.BT[i * L + c];
+            A[a * L + c] = t;
+          }
+        if (!spd_inverse(A.data(), L, Ai.data())) { int none = -1; bad_elem.compare_exchange_strong(none, e); return; }
+        std::copy(Ai.begin(), Ai.end(), pre.begin() + ((size_t)e * 3 + dd) * LL);
+      }
+    }
     for (int s = 0; s < 6; ++s) {
       const int row = (s == 0 || s == 5) ? 6 : ((s == 1 || s == 3) ? 3 : 0);
       const double sg = (s == 0 || s == 1 || s == 4) ? -1.0 : 1.0;
@@ -822,12 +895,14 @@ int curved_geometry(dgx_domain *d, const double *xn) {
     return fail(DGX_ERR_ARGUMENT, "element " + std::to_string(bad_elem.load()) + " has a nonpositive Jacobian");
 
   int rc;
-  if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())))
+  if ((rc = alloc(&d->d_gm, gm.size())) || (rc = alloc(&d->d_fgeo, fgeo.size())) || (rc = alloc(&d->d_jav, jav.size())) ||
+      (rc = alloc(&d->d_pre, pre.size())))
     return rc;
   HIPCHK(hipMemcpy(d->d_gm, gm.data(), gm.size() * 8, hipMemcpyHostToDevice));
   d->gm_host = gm;   // the norms integrate with the nodal metrics
   HIPCHK(hipMemcpy(d->d_fgeo, fgeo.data(), fgeo.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d->d_jav, jav.data(), jav.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->d_pre, pre.data(), pre.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&d->d_iters, 64));
   HIPCHK(hipMemset(d->d_iters, 0, 64));
   std::copy(xn, xn + (size_t)3 * ne * Q3, d->coords.begin());   // the nodes are the element's coordinates
@@ -1269,7 +1344,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
 void release(dgx_domain *d) {
   void *ptrs[] = {d->d_tab, d->d_metric, d->d_fnorm, d->d_fjr, d->d_nbr, d->d_nside, d->d_flag,
                   d->d_permid, d->d_perm, d->U, d->Ut, d->R0, d->R1, d->T, d->G, d->V, d->TV, d->nodal,
-                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_iters, d->Bal, d->d_master,
+                  d->T2, d->Vf, d->TVf, d->d_gm, d->d_fgeo, d->d_jav, d->d_pre, d->d_iters, d->Bal, d->d_master,
                   d->d_fcode, d->d_fconst, d->d_fxy, d->d_xn, d->d_J, d->d_cgeo, d->d_part, d->d_chol,
                   d->d_velems, d->d_vqs, d->d_vflip, d->d_vtab};
   for (void *p : ptrs)
@@ -1590,6 +1665,8 @@ int run_stage(dgx_domain *h, int inttype, int st, double dt, bool traces_ready) 
       ++h->stages;
     } else if (h->nsd == 2 && !h->curved && !h->varp && !h->bulk) {
       RUN(step2(h, A, B, dt));
+    } else if (h->nsd == 3 && h->curved && !h->visc && !h->bulk) {
+      RUN(step_curved3(h, A, B, dt));
     } else {
       RUN(residual(h, h->U, h->R0));
       BL(dgx::blas_axpby(-dt, h->R0, A, h->Ut, h->Ut, n, s));
@@ -1667,6 +1744,15 @@ int dgx_synchronize(dgx_handle h) {
 }
 
 long long dgx_stage_count(dgx_handle h) { return h ? h->stages : -1; }
+
+int dgx_solver_iterations(dgx_handle h, int *max_iters) {
+  if (!h || !max_iters) return fail(DGX_ERR_ARGUMENT, "null argument");
+  *max_iters = 0;
+  if (!h->d_iters) return DGX_OK;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(max_iters, h->d_iters, sizeof(int), hipMemcpyDeviceToHost));
+  return DGX_OK;
+}
 
 int dgx_profile(dgx_handle h, int inttype, double dt, int nsteps, double *ms, int *count, int nkinds) {
   if (!h || !ms || nkinds < 7) return fail(DGX_ERR_ARGUMENT, "dgx_profile needs 7 slots");

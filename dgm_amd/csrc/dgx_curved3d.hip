@@ -10,9 +10,12 @@
 //                 G2GL((Fhat - F-.n) J_side/J_vol) / w_end (src/Hex.cpp:1786-1952)
 //   project       b = B^T (wJ f) and M x = b with M = B^T diag(wJ) B, the curved
 //                 element's mass matrix (src/Hex.cpp:289-318), solved by
-//                 conjugate gradients inside the workgroup: M is symmetric
-//                 positive definite and close to mean(J) I, so about a dozen
-//                 iterations reach rounding.
+//                 preconditioned conjugate gradients inside the workgroup.  The
+//                 preconditioner is the exact inverse for a separable fit
+//                 J ~ ja(xi) jb(eta) jc(zeta): the tensor product of three
+//                 L x L inverses per element, built on the host.  On smoothly
+//                 curved elements the fit leaves M's spectrum within ~1e-5 of
+//                 the preconditioner's, so two or three iterations reach rounding.
 #include "dgx_kernels.hpp"
 
 namespace dgx {
@@ -26,11 +29,12 @@ template <int P> struct C3 {
   // LDS (doubles): Fw [5][Q3] (nodal residual, then the CG direction); scratch
   // S1, S2 [SZ] (the largest intermediate: a transform of 5 fields or the 30
   // face blocks at the GLL points; S1 also holds the 3 flux components of the
-  // volume term); G [GZ] (the face terms gF [5][6][QF2], then b and M p
-  // [5][L3]); reductions [NW][8].  The nodal state lives in registers.
+  // volume term); G [GZ] (the face terms gF [5][6][QF2], then b, M p and the
+  // preconditioned residual [5][L3]); the element's preconditioner [3][L][L];
+  // reductions [NW][16].  The nodal state lives in registers.
   static constexpr int SZ = (5 * Q3 > 30 * Q2) ? 5 * Q3 : 30 * Q2;
   static constexpr int GZ = (30 * QF2 > 5 * L3) ? 30 * QF2 : 5 * L3;
-  static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 8 * NW) * 8;
+  static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 3 * L2 + 16 * NW) * 8;
 };
 
 // [5][N]^3 blocks (compact) through matrix M [NO][N] along all three dimensions
@@ -45,13 +49,15 @@ __device__ __forceinline__ void chain5(const double *M, const double *in, double
   __syncthreads();
 }
 
-// per-field sums of a[g] * b[g] over the 5 x L3 modal entries, reduced over the workgroup
-template <int P>
-__device__ __forceinline__ void field_dots(const double (&part)[5], double *red, double (&out)[5], int tid) {
+// N per-thread partial sums, reduced over the workgroup (every thread gets
+// the totals; red holds [NW][16])
+template <int P, int N>
+__device__ __forceinline__ void wg_sums(const double (&part)[N], double *red, double (&out)[N], int tid) {
   using S = C3<P>;
-  double v[5];
+  static_assert(N <= 16, "reduction slots");
+  double v[N];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < N; ++f) {
     double t = part[f];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
@@ -60,29 +66,44 @@ __device__ __forceinline__ void field_dots(const double (&part)[5], double *red,
   const int w = tid >> 6;
   if ((tid & 63) == 0) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) red[w * 8 + f] = v[f];
+    for (int f = 0; f < N; ++f) red[w * 16 + f] = v[f];
   }
   __syncthreads();
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < N; ++f) {
     double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < S::NW; ++k) t += red[k * 8 + f];
+    for (int k = 0; k < S::NW; ++k) t += red[k * 16 + f];
     out[f] = t;
   }
   __syncthreads();
 }
 
+// out = (Pa (x) Pb (x) Pc) in on [5][L][L][L] (in and out may alias), s1, s2 scratch
+template <int L, int NT>
+__device__ __forceinline__ void precondition(const double *Pre, const double *in, double *s1, double *s2,
+                                             double *out, int tid) {
+  contract<5, L, L * L, 1, L, NT>(Pre, in, s1, tid);                  // along a
+  __syncthreads();
+  contract<5 * L, L, L, 1, L, NT>(Pre + L * L, s1, s2, tid);          // along b
+  __syncthreads();
+  contract<5 * L * L, L, 1, 1, L, NT>(Pre + 2 * L * L, s2, out, tid); // along c
+  __syncthreads();
+}
+
 // PROJ: project the nodal fields a.U [5][ne][Q^3] instead (the forward
 // transform of a curved element: x = M^-1 B^T (wJ u)).
+// Three workgroups fit a CU's LDS at p = 5; asking the compiler for three
+// waves per SIMD (168 VGPRs, a few spills) measured faster than leaving it
+// two (7.8e9 against 7.5e9 DOF-stage/s on 96^3 bump hexes at p = 5).
 template <int P, bool PROJ>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_rhs3c(CurvedArgs a) {
   using S = C3<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
   constexpr int NT = S::NT, NPT = S::NPT, NML = S::NML;
   extern __shared__ double sm[];
-  double *Fw = sm, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *G = S2 + S::SZ, *red = G + S::GZ;
+  double *Fw = sm, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *G = S2 + S::SZ, *Pre = G + S::GZ, *red = Pre + 3 * S::L2;
   double *gF = G, *Fc = S1, *Bv = G;
   static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ && 3 * Q3 <= S::SZ && 5 * Q * L * L <= S::SZ, "scratch");
   static_assert(S::lds <= 160 * 1024, "LDS");
@@ -90,6 +111,7 @@ k_rhs3c(CurvedArgs a) {
   const Geo &geo = a.geo;
   const double *ge = a.gm + (size_t)e * 10 * Q3;   // wJ dxi_d/dx_x (9 rows), wJ
   const double gm1 = a.phys.gamma - 1.0;
+  for (int i = tid; i < 3 * S::L2; i += NT) Pre[i] = a.pre[(size_t)e * 3 * S::L2 + i];
 
   if (PROJ) {
     for (int i = tid; i < 5 * Q3; i += NT) {
@@ -146,7 +168,15 @@ k_rhs3c(CurvedArgs a) {
   }
   if (bad) atomicOr(a.flag, 1);
 
-  // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by field
+  // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by
+  // field; each thread's nodes keep their 9 metric terms in registers
+  double mg[NPT][9];
+#pragma unroll
+  for (int rr = 0; rr < NPT; ++rr) {
+    const int n = tid + rr * NT;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) mg[rr][c] = n < Q3 ? ge[c * Q3 + n] : 0.0;
+  }
   for (int v = 0; v < 5; ++v) {
     __syncthreads();
 #pragma unroll
@@ -182,9 +212,9 @@ k_rhs3c(CurvedArgs a) {
             ds = fma(Dn[j * Q + m], F[(i * Q + m) * Q + k], ds);
             dt = fma(Dn[k * Q + m], F[(i * Q + j) * Q + m], dt);
           }
-          acc = fma(ge[x * Q3 + n], dr, acc);
-          acc = fma(ge[(3 + x) * Q3 + n], ds, acc);
-          acc = fma(ge[(6 + x) * Q3 + n], dt, acc);
+          acc = fma(mg[rr][x], dr, acc);
+          acc = fma(mg[rr][3 + x], ds, acc);
+          acc = fma(mg[rr][6 + x], dt, acc);
         }
         Fw[v * Q3 + n] = acc;
       }
@@ -223,12 +253,13 @@ k_rhs3c(CurvedArgs a) {
   // b = B^T (wJ f): into Bv (compact [5][L3], the face-term region, free now)
   chain5<Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Bv, tid);
 
-  // conjugate gradients on M x = b, M = B^T diag(wJ) B, five fields at once;
-  // x0 = b / mean(J).  Pv, AP (compact [5][L3]) reuse Fw and Bv.
+  // preconditioned conjugate gradients on M x = b, M = B^T diag(wJ) B, five
+  // fields at once, with the element's separable preconditioner P:
+  // x0 = P b, r0 = b - M x0, z = P r, p = z.  Pv (compact [5][L3]) reuses
+  // Fw; AP = M p, then r and z = P r, reuse Bv.
   double *Pv = Fw, *AP = Bv;
   double x[NML], res[NML], bvec[NML];
-  double part[5], bb[5], rr[5], pap[5];
-  const double jinv = 1.0 / a.jav[e];
+  double part[5], bb[5], pap[5], part2[10], rz2[10];
 #pragma unroll
   for (int f = 0; f < 5; ++f) part[f] = 0.0;
 #pragma unroll
@@ -237,17 +268,16 @@ k_rhs3c(CurvedArgs a) {
     x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
     if (g < 5 * L3) {
       bvec[k] = Bv[g];
-      x[k] = bvec[k] * jinv;
       part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
     }
   }
-  __syncthreads();
+  wg_sums<P, 5>(part, red, bb, tid);
+  precondition<L, NT>(Pre, Bv, S1, S2, Pv, tid);   // x0 = P b, into Pv
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
     const int g = tid + k * NT;
-    if (g < 5 * L3) Pv[g] = x[k];
+    if (g < 5 * L3) x[k] = Pv[g];
   }
-  field_dots<P>(part, red, bb, tid);
   // AP = M Pv
   auto mass = [&]() {
     contract<5, L, L * L, 1, Q, NT>(CT<P>::BT(), Pv, S1, tid);        // [5][Q][L L]
@@ -265,31 +295,48 @@ k_rhs3c(CurvedArgs a) {
     contract<5 * L * L, Q, 1, 1, L, NT>(CT<P>::BL(), S1, AP, tid);    // [5][L3]
     __syncthreads();
   };
-  mass();
-  // r0 = b - M x0, p0 = r0
+  // r and z = P r from AP holding r (each thread's own entries); rr and r.z
+  auto residual_step = [&]() {
+    precondition<L, NT>(Pre, AP, S1, S2, AP, tid);
 #pragma unroll
-  for (int f = 0; f < 5; ++f) part[f] = 0.0;
+    for (int f = 0; f < 10; ++f) part2[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      if (g < 5 * L3) {
+        const int f = g / L3;
+        part2[f] = fma(res[k], res[k], part2[f]);
+        part2[5 + f] = fma(res[k], AP[g], part2[5 + f]);
+      }
+    }
+    wg_sums<P, 10>(part2, red, rz2, tid);
+  };
+  mass();
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
     const int g = tid + k * NT;
     if (g < 5 * L3) {
       res[k] = bvec[k] - AP[g];
-      part[g / L3] = fma(res[k], res[k], part[g / L3]);
+      AP[g] = res[k];
     }
   }
   __syncthreads();
+  residual_step();
+  double rz[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) rz[f] = rz2[5 + f];
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
     const int g = tid + k * NT;
-    if (g < 5 * L3) Pv[g] = res[k];
+    if (g < 5 * L3) Pv[g] = AP[g];
   }
-  field_dots<P>(part, red, rr, tid);
+  __syncthreads();
   constexpr double tol2 = 1e-30;   // (1e-15 relative)^2
   int it = 0;
   for (; it < 60; ++it) {
     bool conv = true;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) conv = conv && (rr[f] <= tol2 * bb[f]);
+    for (int f = 0; f < 5; ++f) conv = conv && (rz2[f] <= tol2 * bb[f]);
     if (conv) break;
     mass();
 #pragma unroll
@@ -299,12 +346,10 @@ k_rhs3c(CurvedArgs a) {
       const int g = tid + k * NT;
       if (g < 5 * L3) part[g / L3] = fma(Pv[g], AP[g], part[g / L3]);
     }
-    field_dots<P>(part, red, pap, tid);
+    wg_sums<P, 5>(part, red, pap, tid);
     double alpha[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) alpha[f] = (rr[f] > 0.0 && pap[f] > 0.0) ? rr[f] / pap[f] : 0.0;
-#pragma unroll
-    for (int f = 0; f < 5; ++f) part[f] = 0.0;
+    for (int f = 0; f < 5; ++f) alpha[f] = (rz[f] > 0.0 && pap[f] > 0.0) ? rz[f] / pap[f] : 0.0;
 #pragma unroll
     for (int k = 0; k < NML; ++k) {
       const int g = tid + k * NT;
@@ -312,30 +357,40 @@ k_rhs3c(CurvedArgs a) {
         const int f = g / L3;
         x[k] = fma(alpha[f], Pv[g], x[k]);
         res[k] = fma(-alpha[f], AP[g], res[k]);
-        part[f] = fma(res[k], res[k], part[f]);
+        AP[g] = res[k];
       }
     }
-    double rn[5];
-    field_dots<P>(part, red, rn, tid);
+    __syncthreads();
+    residual_step();
 #pragma unroll
     for (int k = 0; k < NML; ++k) {
       const int g = tid + k * NT;
       if (g < 5 * L3) {
         const int f = g / L3;
-        const double beta = rr[f] > 0.0 ? rn[f] / rr[f] : 0.0;
-        Pv[g] = fma(beta, Pv[g], res[k]);
+        const double beta = rz[f] > 0.0 ? rz2[5 + f] / rz[f] : 0.0;
+        Pv[g] = fma(beta, Pv[g], AP[g]);
       }
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) rr[f] = rn[f];
+    for (int f = 0; f < 5; ++f) rz[f] = rz2[5 + f];
     __syncthreads();
   }
+  // the residual, or (update) the LSRK stage in place: this element's modes
+  // were read at the start and no other workgroup reads them
+  const double mdt = -a.dt;
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
     const int g = tid + k * NT;
     if (g < 5 * L3) {
       const int v = g / L3;
-      a.R[((size_t)v * ne + e) * L3 + (g - v * L3)] = x[k];
+      const size_t o = ((size_t)v * ne + e) * L3 + (g - v * L3);
+      if (!PROJ && a.update) {
+        const double ut = fma(mdt, x[k], a.ca * a.Ut[o]);   // as the axpby Ut = ca Ut - dt R
+        a.Ut[o] = ut;
+        a.Us[o] = fma(a.cb, ut, a.Us[o]);                   // as the axpy U += cb Ut
+      } else {
+        a.R[o] = x[k];
+      }
     }
   }
   if (tid == 0 && a.iters) atomicMax(a.iters, it);

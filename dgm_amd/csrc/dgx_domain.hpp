@@ -53,6 +53,7 @@ struct dgx_domain {
   bool fused = false;   // hex fused stage kernels available for this order
   bool curved = false;  // curved hexes (dgx_curved3d.hip): nodal metrics, mass-matrix projection
   double *d_gm = nullptr, *d_fgeo = nullptr, *d_jav = nullptr;
+  double *d_pre = nullptr;  // curved hexes: [ne][3][L][L] preconditioner of the mass solve
   int *d_iters = nullptr;
   double time = 0.0;       // solution time: boundary and source functions are evaluated at it
   double t_step = 0.0;     // time at the start of the current step

@@ -114,7 +114,9 @@ struct CurvedArgs {
   double *R = nullptr;                       // modal residual
   const double *gm = nullptr;                // [ne][10][Q^3]: wJ dxi_d/dx_x (d-major), wJ
   const double *fgeo = nullptr;              // [ne][6][QF^2][4]: normal, J_side/J_vol
-  const double *jav = nullptr;               // [ne]: mean J (first guess of the mass solve)
+  const double *jav = nullptr;               // [ne]: mean J
+  const double *pre = nullptr;               // hexes: [ne][3][L][L] inverses of the 1D mass matrices of a
+                                             // separable fit of J, the mass solve's preconditioner
   const double *chol = nullptr;              // quads: lower Cholesky factors of the element mass
                                              // matrices, rows packed, [ne][L2 (L2 + 1) / 2]
   Geo geo;
@@ -136,6 +138,11 @@ struct CurvedArgs {
   double t = 0.0;
   double *Bal = nullptr;
   int bal = 0;
+  // hexes: one LSRK stage in place instead of R (update): Ut = ca Ut - dt R,
+  // Us += cb Ut, with Us the modal state U points to
+  bool update = false;
+  double *Us = nullptr, *Ut = nullptr;
+  double ca = 0.0, cb = 0.0, dt = 0.0;
   hipStream_t stream = nullptr;
 };
 // Quads use the same arguments with gm [ne][5][Q^2] (wJ dxi_d/dx_x, wJ),

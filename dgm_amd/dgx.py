@@ -71,6 +71,7 @@ def load():
         "dgx_state_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dgx_state_device_ptr": ([h, ctypes.POINTER(_dp)], ctypes.c_int),
         "dgx_rhs": ([h], ctypes.c_int),
+        "dgx_solver_iterations": ([h, _ip], ctypes.c_int),
         "dgx_set_time": ([h, ctypes.c_double], ctypes.c_int),
         "dgx_get_time": ([h, _dp], ctypes.c_int),
         "dgx_residual_download": ([h, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -223,6 +224,12 @@ class GPUDomain:
         _check(self.lib.dgx_residual_download(self.h, _ptr(F), layout, 0))
         self.synchronize()
         return F
+
+    def solver_iterations(self):
+        """Most conjugate-gradient iterations of a curved hex mass solve so far."""
+        it = ctypes.c_int()
+        _check(self.lib.dgx_solver_iterations(self.h, ctypes.byref(it)))
+        return it.value
 
     def set_time(self, t):
         _check(self.lib.dgx_set_time(self.h, float(t)))

@@ -271,6 +271,12 @@ int dgx_norms(dgx_handle h, double *out);
 /* Number of RHS evaluations (stages) performed so far. */
 long long dgx_stage_count(dgx_handle h);
 
+/* Diagnostic: the most conjugate-gradient iterations any curved element's
+ * mass-matrix solve has taken since the domain was created (0 on affine
+ * domains and on curved quads, which use Cholesky factors).  The reference
+ * factors each curved element's mass matrix instead (src/Hex.cpp:289-318). */
+int dgx_solver_iterations(dgx_handle h, int *max_iters);
+
 /* Run dgx_advance with HIP events around every kernel on the domain's
  * stream and return the summed milliseconds (`ms`) and launch counts per
  * kernel kind: 0 face traces, 1 residual, 2 lifted gradients, 3 viscous
