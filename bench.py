@@ -189,7 +189,7 @@ def serving_sample(args, seed=1234):
     if args.serve_requests > 64:
         print(f"[serve] built {len(jobs)} requests", flush=True)
     srv = serve.Server(hbm_budget=200 * 2**30, max_active=args.serve_active, batch=not args.serve_unbatched,
-                       steps_per_slice=args.serve_slice)
+                       steps_per_slice=args.serve_slice, persistent=args.serve_persistent)
     arrivals = serve.poisson_arrivals(len(jobs), rate=args.serve_rate, rng=rng)
     t0 = time.perf_counter()
     ids = serve.serve_timeline(srv, jobs, arrivals, timeout_s=1200.0,
@@ -212,7 +212,9 @@ def serving_sample(args, seed=1234):
                       f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, FP64, "
                       f"{args.serve_active} slots, slices of {args.serve_slice} steps, "
                       + ("per-request launches replayed from hipGraphs" if args.serve_unbatched
-                         else "requests of one order and physics batched into shared launches")}
+                         else "requests of one order and physics batched into shared launches"
+                         + (", each group slice one persistent cooperative kernel" if args.serve_persistent
+                            else ", two launches per stage"))}
 
 
 def adjoint_sample(args, seed=4096):
@@ -533,6 +535,8 @@ def main():
     ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--serve-slice", type=int, default=20)
     ap.add_argument("--serve-unbatched", action="store_true", help="serve every request on its own launches")
+    ap.add_argument("--serve-persistent", action="store_true",
+                    help="run each batched group slice as one persistent cooperative kernel")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving sample and print its JSON (for profiling it alone)")
     ap.add_argument("--copy-calibration", action="store_true",

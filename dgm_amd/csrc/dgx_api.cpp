@@ -1321,7 +1321,11 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
     if ((rc = alloc(&d->d_J, ne))) return rc;
     HIPCHK(hipMemcpy(d->d_J, d->J.data(), ne * sizeof(double), hipMemcpyHostToDevice));
   }
-  if (d->fused && (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs))) return rc;
+  // a second trace buffer: the fused hex stages write the next stage's traces
+  // into it; affine quads keep it for the serving loop's persistent slices
+  if ((d->fused || (d->nsd == 2 && !d->curved && !d->varp)) &&
+      (rc = alloc(&d->T2, (size_t)ne * d->ns * d->nf * qs)))
+    return rc;
   d->mixed = d->prec == DGX_MIXED;
   if (d->visc && d->mixed) {
     const size_t nv = (size_t)ne * (d->nf - 1) * d->nm, ntv = (size_t)ne * d->ns * (d->nf - 1) * qs;
@@ -1371,7 +1375,7 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   const bool fused = nsd == 3 && dgx::stage3_supported(desc->p);
   const bool mixed = desc->precision == DGX_MIXED;
   size_t b = 4 * nf * ne * nm * 8;                          // U, Ut, R0, R1
-  b += (fused ? 2 : 1) * ne * ns * nf * qs * 8;             // face traces
+  b += (fused || (nsd == 2 && !desc->porder) ? 2 : 1) * ne * ns * nf * qs * 8;   // face traces
   if (visc) {
     const size_t w = mixed ? 4 : 8;
     if (!fused && !mixed) b += ne * (nsd == 2 ? 6 : 12) * nm * 8;

@@ -40,7 +40,7 @@ struct LaunchArgs {
 // LSRK register Ut, face traces T, balance term, error flag, geometry and
 // physics; off = its first work item in the group's flat element list.
 struct Batch2 {
-  double *U = nullptr, *Ut = nullptr, *T = nullptr, *Bal = nullptr;
+  double *U = nullptr, *Ut = nullptr, *T = nullptr, *T2 = nullptr, *Bal = nullptr;
   int *flag = nullptr;
   Geo geo{};
   Phys phys{};
@@ -52,15 +52,19 @@ struct Batch2 {
 constexpr int kBatchGrid = 8192;
 
 struct BatchLaunch {
-  int op = 0;                    // 0: face traces, 1: residual + LSRK14 register update
+  int op = 0;                    // 0: face traces, 1: residual + LSRK14 register update,
+                                 // 2: a persistent slice of nsteps LSRK14 steps (cooperative launch)
   int p = 0, nreq = 0, nwork = 0;
   bool shallow = false;          // shallow water (3 fields) or Euler (4 fields)
   const Batch2 *b = nullptr;     // device array of nreq members
   double A = 0.0, B = 0.0;       // LSRK14 coefficients of the stage
+  int nsteps = 0;                // op 2: steps of the slice
+  double lsrkA[14] = {}, lsrkB[14] = {};   // op 2: the LSRK14 coefficients of every stage
+  int grid = 0;                  // op 2, out: workgroups launched
   hipStream_t stream = nullptr;
 };
 
-hipError_t launch_batch2(const BatchLaunch &a);
+hipError_t launch_batch2(BatchLaunch &a);
 
 // Fill each translation unit's constant-memory basis tables (kTabTotal doubles).
 hipError_t upload_tables_2d(const double *h);

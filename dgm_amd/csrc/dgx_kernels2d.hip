@@ -8,6 +8,8 @@
 //   shallow water    Shallow::convection, LF flux     src/Shallow.cpp:251-525
 //   linear system    LinEuler_quasi::convection       src/LinEuler_quasi.cpp:151-495
 //                    and its discrete adjoint
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 
 #include "dgx_kernels.hpp"
@@ -699,8 +701,85 @@ k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, doubl
   }
 }
 
+struct LsrkTab {
+  double A[14], B[14];
+};
+
+// A persistent slice of a batched group: nsteps LSRK14 steps of every member
+// in one cooperative launch, whose workgroups are all resident at once.  The
+// face traces of the slice's first state are computed first; then each stage
+// is one pass over the members' elements, in which an element takes its
+// residual from the current traces, updates its register and state in place
+// and writes the traces of its new state into the other trace buffer,
+// followed by one grid-wide barrier.  The buffers alternate T, T2, T, ... so
+// that no element overwrites traces a neighbour may still be reading; after
+// the even number of stages of a step the current traces are in T again.
+// Each element's arithmetic is that of k_batch_traces2 and k_batch_step2, so
+// the results are bitwise those of the per-stage launches.
+template <int P, bool SHALLOW>
+__global__ void __launch_bounds__(64)
+k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, LsrkTab c) {
+  constexpr int NF = SHALLOW ? 3 : 4;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  {
+    int r = 0;
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+      while (r + 1 < nreq && b[r + 1].off <= w) ++r;
+      traces2_elem<P, NF>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
+    }
+  }
+  grid.sync();
+  for (int step = 0; step < nsteps; ++step)
+    for (int st = 0; st < 14; ++st) {
+      const double A = c.A[st], B = c.B[st];
+      const bool odd = st & 1, last = step == nsteps - 1 && st == 13;
+      int r = 0;
+      for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        while (r + 1 < nreq && b[r + 1].off <= w) ++r;
+        const int e = w - b[r].off, ne = b[r].ne;
+        double *U = b[r].U, *Ut = b[r].Ut;
+        const double *Tc = odd ? b[r].T2 : b[r].T;
+        double *Tn = odd ? b[r].T : b[r].T2;
+        const double mdt = -b[r].dt;
+        auto update = [=](size_t o, double rr) {
+          const double ut = fma(mdt, rr, A * Ut[o]);   // as the direct path's axpby
+          Ut[o] = ut;
+          U[o] = fma(B, ut, U[o]);
+        };
+        const Geo geo = b[r].geo;
+        const Phys ph = b[r].phys;
+        if constexpr (SHALLOW)
+          shallow2_elem<P>(U, Tc, b[r].Bal, b[r].bal, geo, ph, ne, e, b[r].flag, update);
+        else
+          rhs2_elem<P, false>(U, Tc, nullptr, nullptr, geo, ph, ne, e, b[r].flag, update);
+        if (!last) traces2_elem<P, NF>(U, Tn, ne, e);   // the element's new state, read back by its own workgroup
+      }
+      grid.sync();
+    }
+}
+
+template <int P, bool SHALLOW>
+static hipError_t launch_persist2(BatchLaunch &a) {
+  const void *kern = reinterpret_cast<const void *>(&k_batch_persist2<P, SHALLOW>);
+  int dev = 0, ncu = 0, per = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64, 0)) != hipSuccess) return e;
+  if (per < 1 || ncu < 1) return hipErrorInvalidConfiguration;
+  const int grid = std::max(1, std::min(a.nwork, per * ncu));
+  LsrkTab c;
+  for (int s = 0; s < 14; ++s) { c.A[s] = a.lsrkA[s]; c.B[s] = a.lsrkB[s]; }
+  const Batch2 *b = a.b;
+  int nreq = a.nreq, nwork = a.nwork, nsteps = a.nsteps;
+  void *args[] = {&b, &nreq, &nwork, &nsteps, &c};
+  a.grid = grid;
+  return hipLaunchCooperativeKernel(kern, dim3(grid), dim3(64), args, 0, a.stream);
+}
+
 template <int P>
-static hipError_t launch_batch2_p(const BatchLaunch &a) {
+static hipError_t launch_batch2_p(BatchLaunch &a) {
+  if (a.op == 2) return a.shallow ? launch_persist2<P, true>(a) : launch_persist2<P, false>(a);
   const int grid = std::max(1, std::min(a.nwork, kBatchGrid));
   if (a.op == 0) {
     if (a.shallow)
@@ -718,8 +797,8 @@ static hipError_t launch_batch2_p(const BatchLaunch &a) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch2(const BatchLaunch &a) {
-  if (a.nreq < 1 || a.nwork < 1 || !a.b) return hipErrorInvalidValue;
+hipError_t launch_batch2(BatchLaunch &a) {
+  if (a.nreq < 1 || a.nwork < 1 || !a.b || (a.op == 2 && a.nsteps < 1)) return hipErrorInvalidValue;
   switch (a.p) {
 #define DGX_CASE(PP) case PP: return launch_batch2_p<PP>(a);
     DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5)

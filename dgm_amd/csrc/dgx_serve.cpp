@@ -14,7 +14,10 @@
 // traces; residual with the fused register update) over the elements of all
 // its members, on fixed grids of workgroups that walk the members' elements.
 // Members step in lockstep slices of min(steps_per_slice, fewest remaining
-// steps); they join and leave at slice boundaries.
+// steps); they join and leave at slice boundaries.  With `persistent`, a
+// group's whole slice is one cooperative launch of resident workgroups that
+// step every stage with grid-wide barriers between them; all groups' slices
+// then share one stream, so that two such launches never wait on each other.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,6 +75,7 @@ struct Group {
   size_t cap = 0;
   int nwork = 0, k = 0;
   bool dirty = true, inflight = false;
+  bool shared_stream = false;        // persistent slices: the server's stream
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
 };
@@ -88,6 +92,7 @@ struct dgx_server {
   std::vector<std::unique_ptr<Group>> groups;
   size_t used = 0, peak_bytes = 0;
   int peak_active = 0;
+  hipStream_t pstream = nullptr;   // persistent slices of every group, one after another
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -159,7 +164,7 @@ int enqueue_slice(dgx_server *s, Request *r) {
 // the curved path (xnodes, non-parallelogram quads) step on their own.
 bool batchable(const dgx_server *s, const Request *r) {
   const dgx_domain_desc &d = r->desc;
-  return s->cfg.batch && d.nsd == 2 && !d.xnodes && r->h && !r->h->curved &&
+  return s->cfg.batch && d.nsd == 2 && !d.xnodes && r->h && !r->h->curved && !r->h->varp &&
          (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) && d.precision == DGX_FP64 &&
          r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
 }
@@ -172,7 +177,16 @@ int join_group(dgx_server *s, Request *r) {
   if (!g) {
     auto ng = std::make_unique<Group>();
     ng->eqn = r->desc.eqntype; ng->p = r->desc.p;
-    if (hipStreamCreate(&ng->stream) != hipSuccess) return sfail(DGX_ERR_DEVICE, "hipStreamCreate failed");
+    if (s->cfg.persistent) {
+      // cooperative launches must not run side by side (each waits for all its
+      // workgroups to be resident): every group's slices share one stream
+      if (!s->pstream && hipStreamCreate(&s->pstream) != hipSuccess)
+        return sfail(DGX_ERR_DEVICE, "hipStreamCreate failed");
+      ng->stream = s->pstream;
+      ng->shared_stream = true;
+    } else if (hipStreamCreate(&ng->stream) != hipSuccess) {
+      return sfail(DGX_ERR_DEVICE, "hipStreamCreate failed");
+    }
     if (hipEventCreateWithFlags(&ng->ev, hipEventDisableTiming) != hipSuccess)
       return sfail(DGX_ERR_DEVICE, "hipEventCreate failed");
     g = ng.get();
@@ -208,7 +222,8 @@ int enqueue_group(dgx_server *s, Group *g) {
     for (size_t i = 0; i < g->members.size(); ++i) {
       const dgx_domain *d = g->members[i]->h;
       dgx::Batch2 &b = g->hd[i];
-      b.U = d->U; b.Ut = d->Ut; b.T = d->T; b.Bal = d->Bal; b.flag = d->d_flag;
+      if (s->cfg.persistent && !d->T2) return sfail(DGX_ERR_ARGUMENT, "persistent slice: a member has no second trace buffer");
+      b.U = d->U; b.Ut = d->Ut; b.T = d->T; b.T2 = d->T2; b.Bal = d->Bal; b.flag = d->d_flag;
       b.geo = d->geo; b.phys = d->phys; b.ne = d->ne; b.off = off; b.bal = d->bal;
       b.dt = g->members[i]->dt;
       off += d->ne;
@@ -230,6 +245,13 @@ int enqueue_group(dgx_server *s, Group *g) {
   dgx::BatchLaunch a;
   a.p = g->p; a.nreq = (int)g->hd.size(); a.nwork = g->nwork; a.shallow = g->eqn == DGX_SHALLOW;
   a.b = g->dd; a.stream = g->stream;
+  if (s->cfg.persistent) {   // the whole slice in one cooperative launch
+    a.op = 2; a.nsteps = k;
+    std::copy(dgx::kLsrkA, dgx::kLsrkA + 14, a.lsrkA);
+    std::copy(dgx::kLsrkB, dgx::kLsrkB + 14, a.lsrkB);
+    const hipError_t e = dgx::launch_batch2(a);
+    if (e != hipSuccess) return sfail(DGX_ERR_DEVICE, std::string("persistent slice: ") + hipGetErrorString(e));
+  } else
   for (int step = 0; step < k; ++step)
     for (int st = 0; st < 14; ++st) {
       a.op = 0;
@@ -287,8 +309,9 @@ int dgx_server_destroy(dgx_server_handle s) {
   for (auto &g : s->groups) {
     if (g->dd) (void)hipFree(g->dd);
     if (g->ev) (void)hipEventDestroy(g->ev);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream && !g->shared_stream) (void)hipStreamDestroy(g->stream);
   }
+  if (s->pstream) (void)hipStreamDestroy(s->pstream);
   delete s;
   return DGX_OK;
 }

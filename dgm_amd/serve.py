@@ -25,7 +25,8 @@ QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int),
-                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int), ("batch", ctypes.c_int)]
+                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int), ("batch", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
 
 
 def _bind(lib):
@@ -94,10 +95,10 @@ class Server:
     """Continuous-batching server of forward solves on one GPU."""
 
     def __init__(self, hbm_budget: int, max_active: int = 64, steps_per_slice: int = 10,
-                 use_graphs: bool = True, batch: bool = True):
+                 use_graphs: bool = True, batch: bool = True, persistent: bool = False):
         self.lib = _bind(dgx.load())
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(steps_per_slice), int(bool(use_graphs)),
-                           int(bool(batch)))
+                           int(bool(batch)), int(bool(persistent)))
         self.h = ctypes.c_void_p()
         _check(self.lib, self.lib.dgx_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.requests: Dict[int, Request] = {}

@@ -327,6 +327,9 @@ typedef struct {
   int use_graphs;        /* replay slices from a captured hipGraph (1) or launch directly (0) */
   int batch;             /* 1: batch 2D Euler / shallow-water FP64 LSRK14 requests of one order
                             into shared launches (one per stage and kind for all of them) */
+  int persistent;        /* with batch: 1 runs each group slice as one persistent cooperative
+                            kernel (every stage, grid-wide barriers between stages) instead of
+                            two launches per stage; results are bitwise the same */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_QUEUED = 0, DGX_RUNNING = 1, DGX_DONE = 2, DGX_FAILED = 3 };

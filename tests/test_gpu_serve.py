@@ -142,6 +142,34 @@ def test_batched_groups_match_unbatched_service():
         assert rel <= 1e-13, rel
 
 
+@pytest.mark.parametrize("slice_steps", [4, 9])
+def test_persistent_slices_match_per_stage_launches(slice_steps):
+    """A group's slice as one persistent cooperative kernel (grid-wide barriers
+    between stages, alternating trace buffers) gives bitwise the results of
+    two launches per stage: both physics, two orders, members leaving at
+    different slice boundaries, later arrivals joining running groups."""
+    rng = np.random.default_rng(47)
+    jobs = [_euler2d_job(rng, 5, 3, 9), _euler2d_job(rng, 7, 3, 23), _shallow_job(rng, 6, 3, 17),
+            _shallow_job(rng, 4, 5, 6), _euler2d_job(rng, 6, 5, 14), _shallow_job(rng, 5, 3, 11, balance=True),
+            _euler2d_job(rng, 9, 3, 12)]
+    out = []
+    for persistent in (True, False):
+        print(f"[serve] persistent={persistent} slice={slice_steps}", flush=True)
+        srv = serve.Server(hbm_budget=1 << 32, max_active=8, steps_per_slice=slice_steps, batch=True,
+                           persistent=persistent)
+        ids = [srv.submit(**j) for j in jobs[:4]]
+        srv.progress()
+        ids += [srv.submit(**j) for j in jobs[4:]]
+        srv.run_until_done(timeout_s=60.0)
+        for i in ids:
+            assert srv.status(i) == serve.DONE
+        out.append([srv.result(i)[0] for i in ids])
+        srv.close()
+    for a, b in zip(*out):
+        print(f"[serve] persistent vs per-stage: max diff {np.max(np.abs(a - b)):.3e}", flush=True)
+        assert np.array_equal(a, b)
+
+
 def test_member_joining_during_a_slice_steps_from_the_next():
     """A request admitted while its group's slice is in flight joins the group
     at the next slice and still takes exactly its own number of steps."""

@@ -153,10 +153,10 @@ def test_server_argument_checks(lib):
     from dgm_amd import serve
     serve._bind(lib)
     h = ctypes.c_void_p()
-    bad = serve.ServerConfig(0, 4, 5, 1, 1)
+    bad = serve.ServerConfig(0, 4, 5, 1, 1, 0)
     assert lib.dgx_server_create(ctypes.byref(bad), ctypes.byref(h)) == -1
     assert b"positive" in lib.dgx_last_error()
-    good = serve.ServerConfig(1 << 30, 4, 5, 1, 1)
+    good = serve.ServerConfig(1 << 30, 4, 5, 1, 1, 0)
     assert lib.dgx_server_create(ctypes.byref(good), ctypes.byref(h)) == 0
     m = meshgen.periodic_quads(2, 2)
     d, keep = _desc(m, 3, 1)
