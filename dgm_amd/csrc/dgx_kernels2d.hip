@@ -766,6 +766,12 @@ static hipError_t launch_persist2(BatchLaunch &a) {
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64, 0)) != hipSuccess) return e;
+  // every workgroup must be resident at once for the grid-wide barriers: keep
+  // well inside the occupancy the runtime reports, counting a CU's LDS as
+  // 64 KiB
+  hipFuncAttributes fa;
+  if ((e = hipFuncGetAttributes(&fa, kern)) != hipSuccess) return e;
+  per = std::min(per, (int)(65536 / std::max<size_t>(fa.sharedSizeBytes, 1)));
   if (per < 1 || ncu < 1) return hipErrorInvalidConfiguration;
   const int grid = std::max(1, std::min(a.nwork, per * ncu));
   LsrkTab c;
