@@ -42,8 +42,10 @@ Also printed in the same JSON line:
                 p=5, 16 pressure receivers, 200 LSRK14 steps, 15 checkpoints):
                 forward and gradient times, central differences in 2 directions
   serving       p50/p99 latency and throughput of a serving-loop sample: Poisson
-                arrivals of 2D Euler forward solves of random size and order,
-                continuously batched (dgm_amd.serve)
+                arrivals of 2D Euler and shallow-water forward solves of random
+                size and order, continuously batched (dgm_amd.serve), in the
+                FP32-mixed precision the ensemble configuration names;
+                serving_fp64 the same sample in FP64
   element_block the element-block seam (GpGpu::hex_kernels' inner_product and
                 backward_transform) on 32768 hexes at 8 modes / 9 points per
                 direction, device-resident, with its HBM GB/s
@@ -156,13 +158,14 @@ def cpu_baseline(p, mu, n=32, workers=None, steps=1, flux=0, viscous=0):
                       f"{wall:.1f} s"}
 
 
-def serving_sample(args, seed=1234):
+def serving_sample(args, seed=1234, precision=dgx.DGX_MIXED):
     """A serving mix shaped like the ensemble workload: Poisson arrivals of 2D
     forward solves, each shallow water about rest (phi = 1) or Euler about a
     freestream (seeded choice), plus 8 seeded Fourier modes of amplitude 0.05;
     n x n periodic quads with n log-uniform in [8, nmax], p uniform in {2..8},
-    200 LSRK14 steps each, FP64, continuous batching on one GPU (requests of
-    one order and physics share their launches unless --serve-unbatched).
+    200 LSRK14 steps each, FP32-mixed (FP32 element contractions, FP64 state)
+    or FP64, continuous batching on one GPU (requests of one order, physics
+    and precision share their launches unless --serve-unbatched).
     The shallow-water requests leave out the balance term, which would hold
     them at their initial state."""
     from dgm_amd import serve
@@ -181,7 +184,8 @@ def serving_sample(args, seed=1234):
         x = quad_nodes(m, pp)   # nodal GLL coordinates without a device domain
         U = np.stack([b + ic.fourier_modes(x, (10.0, 10.0), 8, 0.05, rng) for b in base[eqn]])
         dt = 0.2 * (10.0 / nn) / ((pp + 1) ** 2 * speed[eqn])
-        job = dict(mesh=m, p=pp, eqntype=eqn, U0=U, inttype=dgx.DGX_LSRK14, dt=dt, nsteps=args.serve_steps)
+        job = dict(mesh=m, p=pp, eqntype=eqn, U0=U, inttype=dgx.DGX_LSRK14, dt=dt, nsteps=args.serve_steps,
+                   precision=precision)
         if eqn == dgx.DGX_SHALLOW:
             job["balance"] = False
         jobs.append(job)
@@ -208,8 +212,10 @@ def serving_sample(args, seed=1234):
             "mean_queue_wait_ms": float(np.mean(wait) * 1e3),
             "throughput_dof_stage_per_s": float(sum(dofs) * 14 * args.serve_steps / wall),
             "makespan_s": wall, "peak_active": peak_active, "peak_device_bytes": peak_bytes,
+            "precision": "mixed" if precision == dgx.DGX_MIXED else "fp64",
             "sample": f"{len(jobs)} 2D solves ({nshallow} shallow water, {len(jobs) - nshallow} Euler), n log-uniform in [8,{args.serve_nmax}], p in 2..8, "
-                      f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, FP64, "
+                      f"{args.serve_steps} LSRK14 steps, Poisson arrivals at {args.serve_rate}/s, "
+                      f"{'FP32-mixed' if precision == dgx.DGX_MIXED else 'FP64'}, "
                       f"{args.serve_active} slots, slices of {args.serve_slice} steps, "
                       + ("per-request launches replayed from hipGraphs" if args.serve_unbatched
                          else "requests of one order and physics batched into shared launches"
@@ -535,6 +541,8 @@ def main():
     ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--serve-slice", type=int, default=20)
     ap.add_argument("--serve-unbatched", action="store_true", help="serve every request on its own launches")
+    ap.add_argument("--serve-precision", choices=("mixed", "fp64"), default="mixed",
+                    help="precision of the requests with --serve-only")
     ap.add_argument("--serve-persistent", action="store_true",
                     help="run each batched group slice as one persistent cooperative kernel")
     ap.add_argument("--serve-only", action="store_true",
@@ -548,7 +556,8 @@ def main():
               "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.serve_only:
-        print(json.dumps(serving_sample(args)), flush=True)
+        print(json.dumps(serving_sample(args, precision=dgx.DGX_FP64 if args.serve_precision == "fp64"
+                                        else dgx.DGX_MIXED)), flush=True)
         return
 
     n, p, mu = args.n, args.p, 1.0 / 1600.0
@@ -616,7 +625,8 @@ def main():
             "vs_cpu_baseline": (ref["value"] / cpu["value"]) if cpu else None,
             "roofline": ref["roofline"], "kernels": ref["kernels"]}
     if not args.no_serve:
-        line["serving"] = serving_sample(args)
+        line["serving"] = serving_sample(args, precision=dgx.DGX_MIXED)      # the ensemble mix is FP32-mixed
+        line["serving_fp64"] = serving_sample(args, precision=dgx.DGX_FP64)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_sample(args)
         line["euler_adjoint"] = euler_adjoint_sample(args)

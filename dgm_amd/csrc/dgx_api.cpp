@@ -313,6 +313,7 @@ dgx::LaunchArgs launch_args(dgx_domain *d, int op, const double *U, double *out)
   a.U = U; a.T = d->T; a.R = out; a.G = d->G; a.V = d->V; a.TV = d->TV;
   a.flag = d->d_flag; a.tab = d->tab; a.geo = d->geo; a.phys = d->phys; a.stream = d->stream;
   a.eqn = d->eqn; a.Bal = d->Bal; a.bal = (op == dgx::OP_RHS || op == dgx::OP_STEP) ? d->bal : 0;
+  a.mixed = d->mixed && d->nsd == 2;
   return a;
 }
 
@@ -1425,8 +1426,13 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "DGX_VISCOUS_BR2 is available for Navier_Stokes3d");
   if (!(desc->br2_eta >= 0.0) || !std::isfinite(desc->br2_eta))
     return fail(DGX_ERR_ARGUMENT, "br2_eta must be finite and >= 0");
-  if (desc->precision == DGX_MIXED && !(desc->nsd == 3 && dgx::mixed3_supported(desc->p)))
-    return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8");
+  if (desc->precision == DGX_MIXED &&
+      !((desc->nsd == 3 && dgx::mixed3_supported(desc->p)) ||
+        (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_SHALLOW))))
+    return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8 and for "
+                                  "affine quads (Euler, Shallow)");
+  if (desc->precision == DGX_MIXED && (desc->bc_function || desc->source))
+    return fail(DGX_ERR_ARGUMENT, "DGX_MIXED: boundary and source functions take the curved path, which is DGX_FP64");
   if (!desc->xnodes && !all_quads_affine(desc) && desc->precision != DGX_FP64)
     return fail(DGX_ERR_ARGUMENT, "non-parallelogram quads take the curved path, which is DGX_FP64");
   if (desc->xnodes && !(desc->precision == DGX_FP64 &&

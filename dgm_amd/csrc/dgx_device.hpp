@@ -233,10 +233,10 @@ __device__ __forceinline__ void ghost_prim3(int bc, double rho, const Phys &ph, 
 // pencil, its inputs are read once and all outputs computed from registers;
 // otherwise passes over output chunks are outermost so that only OC x NI
 // coefficients are live in scalar registers at a time.  Both orders give
-// bitwise the same sums.  ACC adds into `out`.
-template <int N0, int N1, int N2, int DIMC, int NO, int NT, bool ACC = false>
-__device__ __forceinline__ void contract(const double *__restrict__ M, const double *in,
-                                         double *out, int tid) {
+// bitwise the same sums.  ACC adds into `out`.  T is the arithmetic type of
+// the matrix, the tensors and the sums (double, or float on the mixed paths).
+template <int N0, int N1, int N2, int DIMC, int NO, int NT, bool ACC = false, typename T = double>
+__device__ __forceinline__ void contract(const T *__restrict__ M, const T *in, T *out, int tid) {
   constexpr int NI = DIMC == 0 ? N0 : (DIMC == 1 ? N1 : N2);
   constexpr int NP = (N0 * N1 * N2) / NI;
   constexpr int O1 = DIMC == 1 ? NO : N1;
@@ -251,12 +251,12 @@ __device__ __forceinline__ void contract(const double *__restrict__ M, const dou
     else { i0 = pc / N1; i1 = pc - i0 * N1; i2 = 0; }
     const int bi = (i0 * N1 + i1) * N2 + i2;
     const int bo = (i0 * O1 + i1) * O2 + i2;
-    double x[NI];
+    T x[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) x[i] = in[bi + i * SI];
 #pragma unroll
     for (int o = o0; o < o1; ++o) {
-      double s = 0.0;
+      T s = 0;
 #pragma unroll
       for (int i = 0; i < NI; ++i) s = fma(M[o * NI + i], x[i], s);
       if (ACC) out[bo + o * SO] += s;
@@ -272,17 +272,17 @@ __device__ __forceinline__ void contract(const double *__restrict__ M, const dou
   }
 }
 
-// Edge contributions at modal (l,m): edges 0,2 normal to b, edges 1,3 normal to a.
-template <int P>
-__device__ __forceinline__ double face_add2(const double *__restrict__ Bend, const double *H, int l,
-                                            int m, const double c[4]) {
+// Edge contributions at modal (l,m): edges 0,2 normal to b, edges 1,3 normal to a
+// (arithmetic in T: double, or float on the mixed paths).
+template <int P, typename T>
+__device__ __forceinline__ T face_add2(const T *__restrict__ Bend, const T *H, int l, int m, const double c[4]) {
   constexpr int L = P + 1;
-  const double *B0 = Bend, *B1 = Bend + L;
-  double s = 0.0;
-  s = fma(c[0] * B0[m], H[0 * L + l], s);
-  s = fma(c[2] * B1[m], H[2 * L + l], s);
-  s = fma(c[1] * B1[l], H[1 * L + m], s);
-  s = fma(c[3] * B0[l], H[3 * L + m], s);
+  const T *B0 = Bend, *B1 = Bend + L;
+  T s = 0;
+  s = fma((T)c[0] * B0[m], H[0 * L + l], s);
+  s = fma((T)c[2] * B1[m], H[2 * L + l], s);
+  s = fma((T)c[1] * B1[l], H[1 * L + m], s);
+  s = fma((T)c[3] * B0[l], H[3 * L + m], s);
   return s;
 }
 

@@ -30,6 +30,7 @@ struct LaunchArgs {
   int bal = 0;                   // 0 no balance term, 1 store the residual in Bal, 2 subtract Bal
   double *Us = nullptr, *Ut = nullptr;   // OP_STEP: the state (updated in place) and the LSRK register
   double ca = 0.0, cb = 0.0, dt = 0.0;   // OP_STEP: LSRK coefficients and time step
+  bool mixed = false;            // quads, inviscid: FP32 LDS tensors and contractions (traces, residual, step)
   Tab tab;
   Geo geo;
   Phys phys;
@@ -61,6 +62,7 @@ struct BatchLaunch {
   int nsteps = 0;                // op 2: steps of the slice
   double lsrkA[14] = {}, lsrkB[14] = {};   // op 2: the LSRK14 coefficients of every stage
   int grid = 0;                  // op 2, out: workgroups launched
+  bool mixed = false;            // FP32 LDS tensors and contractions
   hipStream_t stream = nullptr;
 };
 

@@ -11,10 +11,28 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "dgx_kernels.hpp"
 
 namespace dgx {
+
+// Float copies of the basis tables for the mixed quad kernels, which hold
+// their LDS tensors and run their contractions in FP32 while the state, face
+// traces, LSRK registers and pointwise physics stay FP64; filled beside c_tab
+// at upload.  TB<R, P> gives the tables of arithmetic type R.
+static __constant__ float c_tabf[kTabTotal];
+template <typename R, int P> struct TB;
+template <int P> struct TB<double, P> : CT<P> {};
+template <int P> struct TB<float, P> {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, b = tab_offset(P);
+  __device__ static __forceinline__ const float *BT() { return c_tabf + b; }
+  __device__ static __forceinline__ const float *BW() { return c_tabf + b + Q * L; }
+  __device__ static __forceinline__ const float *QD() { return c_tabf + b + 2 * Q * L; }
+  __device__ static __forceinline__ const float *Bg() { return c_tabf + b + 3 * Q * L; }
+  __device__ static __forceinline__ const float *Bend() { return c_tabf + b + 3 * Q * L + QF * L; }
+  __device__ static __forceinline__ const float *PW() { return c_tabf + b + 3 * Q * L + QF * L + 2 * L; }
+};
 
 template <int P> struct H2 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
@@ -34,29 +52,29 @@ __device__ __forceinline__ void backward2(const Tab &tb, const double *m, double
 
 // Gauss-point traces of the NF fields of element e on its four edges, all
 // fields per contraction (one load phase, four contraction phases).
-template <int P, int NF>
+template <int P, int NF, typename R = double>
 __device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne, int e) {
   using S = H2<P>;
   constexpr int L = S::L, QF = S::QF, L2 = S::L2, NT = S::NT;
-  __shared__ double A[NF * L2], Kb[NF * 2 * L], Ka[NF * 2 * L], O[NF * 2 * QF];
+  __shared__ R A[NF * L2], Kb[NF * 2 * L], Ka[NF * 2 * L], O[NF * 2 * QF];
   const int tid = threadIdx.x;
   double *Te = T + (size_t)e * 4 * NF * QF;
   for (int i = tid; i < NF * L2; i += NT) {
     const int v = i / L2;
-    A[i] = U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+    A[i] = (R)U[((size_t)v * ne + e) * L2 + (i - v * L2)];
   }
   __syncthreads();
-  contract<NF, L, L, 2, 2, NT>(CT<P>::Bend(), A, Kb, tid);   // [NF][L][2]
-  contract<NF, L, L, 1, 2, NT>(CT<P>::Bend(), A, Ka, tid);   // [NF][2][L]
+  contract<NF, L, L, 2, 2, NT>(TB<R, P>::Bend(), A, Kb, tid);   // [NF][L][2]
+  contract<NF, L, L, 1, 2, NT>(TB<R, P>::Bend(), A, Ka, tid);   // [NF][2][L]
   __syncthreads();
-  contract<NF, L, 2, 1, QF, NT>(CT<P>::Bg(), Kb, O, tid);    // [NF][QF][2]: edges 0 / 2
+  contract<NF, L, 2, 1, QF, NT>(TB<R, P>::Bg(), Kb, O, tid);    // [NF][QF][2]: edges 0 / 2
   __syncthreads();
   for (int i = tid; i < NF * 2 * QF; i += NT) {
     const int v = i / (2 * QF), r = i - v * 2 * QF, a = r >> 1, end = r & 1;
     Te[((end ? 2 : 0) * NF + v) * QF + a] = O[i];
   }
   __syncthreads();
-  contract<NF, 2, L, 2, QF, NT>(CT<P>::Bg(), Ka, O, tid);    // [NF][2][QF]: edges 3 / 1
+  contract<NF, 2, L, 2, QF, NT>(TB<R, P>::Bg(), Ka, O, tid);    // [NF][2][QF]: edges 3 / 1
   __syncthreads();
   for (int i = tid; i < NF * 2 * QF; i += NT) {
     const int v = i / (2 * QF), r = i - v * 2 * QF, end = r / QF, b = r - end * QF;
@@ -65,11 +83,11 @@ __device__ __forceinline__ void traces2_elem(const double *__restrict__ U, doubl
   __syncthreads();
 }
 
-template <int P>
+template <int P, typename R = double>
 __global__ void __launch_bounds__(64)
 k_traces2(const double *__restrict__ U, double *__restrict__ T, Tab tb, int ne, int nf) {
-  if (nf == 3) traces2_elem<P, 3>(U, T, ne, blockIdx.x);
-  else traces2_elem<P, 4>(U, T, ne, blockIdx.x);
+  if (nf == 3) traces2_elem<P, 3, R>(U, T, ne, blockIdx.x);
+  else traces2_elem<P, 4, R>(U, T, ne, blockIdx.x);
 }
 
 // Residual of element e (Euler, or with NS the BR1 viscous terms from V, TV);
@@ -84,28 +102,28 @@ struct H2F {
   static constexpr int NM = NF * L2, NX = NF * Q * L, NN = NF * Q2;
 };
 
-template <int P, int NF>
-__device__ __forceinline__ double *rhs2_smem() {
+template <int P, int NF, typename R = double>
+__device__ __forceinline__ R *rhs2_smem() {
   using F = H2F<P, NF>;
-  __shared__ double sm[F::NM + 2 * F::NX + 2 * F::NN + NF * 4 * F::QF + NF * 4 * F::L];
+  __shared__ R sm[F::NM + 2 * F::NX + 2 * F::NN + NF * 4 * F::QF + NF * 4 * F::L];
   return sm;
 }
 
 // Element state modal -> nodal for all NF fields; each thread keeps its nodes
 // k = tid + r*NT in registers (un[r][v]).
-template <int P, int NF, int NPT>
-__device__ __forceinline__ void load_nodal2(const double *__restrict__ U, int ne, int e, double *M, double *X,
-                                            double *N, double (&un)[NPT][NF], int tid) {
+template <int P, int NF, int NPT, typename R = double>
+__device__ __forceinline__ void load_nodal2(const double *__restrict__ U, int ne, int e, R *M, R *X,
+                                            R *N, double (&un)[NPT][NF], int tid) {
   using F = H2F<P, NF>;
   constexpr int L = F::L, Q = F::Q, L2 = F::L2, Q2 = F::Q2, NT = 64;
   for (int i = tid; i < NF * L2; i += NT) {
     const int v = i / L2;
-    M[i] = U[((size_t)v * ne + e) * L2 + (i - v * L2)];
+    M[i] = (R)U[((size_t)v * ne + e) * L2 + (i - v * L2)];
   }
   __syncthreads();
-  contract<NF, L, L, 1, Q, NT>(CT<P>::BT(), M, X, tid);   // [NF][Q][L]
+  contract<NF, L, L, 1, Q, NT>(TB<R, P>::BT(), M, X, tid);   // [NF][Q][L]
   __syncthreads();
-  contract<NF, Q, L, 2, Q, NT>(CT<P>::BT(), X, N, tid);   // [NF][Q][Q]
+  contract<NF, Q, L, 2, Q, NT>(TB<R, P>::BT(), X, N, tid);   // [NF][Q][Q]
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
@@ -117,22 +135,21 @@ __device__ __forceinline__ void load_nodal2(const double *__restrict__ U, int ne
 
 // Projected strong-form residual of all NF fields from the contravariant
 // fluxes in N1 (r) and N2 (s) and the face terms gF; result in M [NF][L2].
-template <int P, int NF>
-__device__ __forceinline__ void divergence2(double *M, double *X1, double *X2, double *N1, double *N2,
-                                            const double *gF, double *H, int tid) {
+template <int P, int NF, typename R = double>
+__device__ __forceinline__ void divergence2(R *M, R *X1, R *X2, R *N1, R *N2, const R *gF, R *H, int tid) {
   using F = H2F<P, NF>;
   constexpr int L = F::L, Q = F::Q, QF = F::QF, NT = 64;
-  contract<NF, Q, Q, 2, L, NT>(CT<P>::BW(), N1, X1, tid);
-  contract<NF, Q, Q, 2, L, NT>(CT<P>::QD(), N2, X2, tid);
-  contract<NF, 4, QF, 2, L, NT>(CT<P>::PW(), gF, H, tid);
+  contract<NF, Q, Q, 2, L, NT>(TB<R, P>::BW(), N1, X1, tid);
+  contract<NF, Q, Q, 2, L, NT>(TB<R, P>::QD(), N2, X2, tid);
+  contract<NF, 4, QF, 2, L, NT>(TB<R, P>::PW(), gF, H, tid);
   __syncthreads();
-  contract<NF, Q, L, 1, L, NT>(CT<P>::QD(), X1, M, tid);
+  contract<NF, Q, L, 1, L, NT>(TB<R, P>::QD(), X1, M, tid);
   __syncthreads();
-  contract<NF, Q, L, 1, L, NT, true>(CT<P>::BW(), X2, M, tid);
+  contract<NF, Q, L, 1, L, NT, true>(TB<R, P>::BW(), X2, M, tid);
   __syncthreads();
 }
 
-template <int P, bool NS, class Out>
+template <int P, bool NS, typename R = double, class Out>
 __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const double *__restrict__ T,
                                           const double *__restrict__ V, const double *__restrict__ TV,
                                           const Geo &geo, const Phys &ph, int ne, int e, int *__restrict__ flag,
@@ -140,12 +157,12 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
   using F = H2F<P, 4>;
   constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
   constexpr int NPT = (Q2 + NT - 1) / NT;
-  double *M = rhs2_smem<P, 4>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
-  double *gF = N2 + F::NN, *H = gF + 16 * QF;
+  R *M = rhs2_smem<P, 4, R>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  R *gF = N2 + F::NN, *H = gF + 16 * QF;
   const int tid = threadIdx.x;
   const double gm1 = ph.gamma - 1.0;
   double un[NPT][4];
-  load_nodal2<P, 4, NPT>(U, ne, e, M, X1, N1, un, tid);
+  load_nodal2<P, 4, NPT, R>(U, ne, e, M, X1, N1, un, tid);
   bool bad = false;
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, a = i - s * QF;
@@ -184,7 +201,7 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
         const double *vr = TV + ((size_t)nb * 4 + ns) * 3 * QF;
         g += 0.5 * jr * (vl[(v - 1) * QF + a] + vr[(v - 1) * QF + an]);
       }
-      gF[(v * 4 + s) * QF + a] = g;
+      gF[(v * 4 + s) * QF + a] = (R)g;
     }
   }
   if (bad) atomicOr(flag, 1);
@@ -201,28 +218,28 @@ __device__ __forceinline__ void rhs2_elem(const double *__restrict__ U, const do
     const double fx[4] = {ru, ru * u + p, rv * u, u * h}, fy[4] = {rv, ru * vv, rv * vv + p, vv * h};
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      N1[v * Q2 + k] = rx * fx[v] + ry * fy[v];
-      N2[v * Q2 + k] = sx * fx[v] + sy * fy[v];
+      N1[v * Q2 + k] = (R)(rx * fx[v] + ry * fy[v]);
+      N2[v * Q2 + k] = (R)(sx * fx[v] + sy * fy[v]);
     }
   }
   __syncthreads();
-  divergence2<P, 4>(M, X1, X2, N1, N2, gF, H, tid);
+  divergence2<P, 4, R>(M, X1, X2, N1, N2, gF, H, tid);
   const double cw[4] = {1.0, 1.0, 1.0, 1.0};
   for (int i = tid; i < 4 * L2; i += NT) {
     const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
-    double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    double r = M[i] + face_add2<P>(TB<R, P>::Bend(), H + v * 4 * L, l, m, cw);
     if (NS && v > 0) r -= V[((size_t)e * 3 + (v - 1)) * L2 + k];
     out(((size_t)v * ne + e) * L2 + k, r);
   }
   __syncthreads();
 }
 
-template <int P, bool NS>
+template <int P, bool NS, typename RT = double>
 __global__ void __launch_bounds__(64)
 k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double *__restrict__ V,
        const double *__restrict__ TV, double *__restrict__ R, Tab tb, Geo geo, Phys ph, int ne,
        int *__restrict__ flag) {
-  rhs2_elem<P, NS>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
+  rhs2_elem<P, NS, RT>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
 }
 
 // One LSRK14 stage of an affine 2D domain in one kernel after the traces (and
@@ -230,11 +247,11 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
 // update Ut = A Ut - dt R, U += B Ut in its store, bitwise the same as the
 // residual followed by the axpby and axpy.  An element reads its own modes
 // only (its neighbours' through the traces), before its own store.
-template <int P, bool NS>
+template <int P, bool NS, typename R = double>
 __global__ void __launch_bounds__(64)
 k_step2(double *U, const double *__restrict__ T, const double *__restrict__ V, const double *__restrict__ TV,
         double *__restrict__ Ut, Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
-  rhs2_elem<P, NS>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
+  rhs2_elem<P, NS, R>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
     const double ut = fma(mdt, r, A * Ut[o]);
     Ut[o] = ut;
     U[o] = fma(B, ut, U[o]);
@@ -245,18 +262,18 @@ k_step2(double *U, const double *__restrict__ T, const double *__restrict__ V, c
 // divergence at the GLL points, Lax-Friedrichs face flux, lift, projection.
 // The balance term (src/Shallow.cpp:251-266): bal 1 stores this residual in
 // Bal and returns R - Bal = 0, bal 2 returns R - Bal.
-template <int P, class Out>
+template <int P, typename R = double, class Out>
 __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, const double *__restrict__ T,
                                               double *__restrict__ Bal, int bal, const Geo &geo, const Phys &ph,
                                               int ne, int e, int *__restrict__ flag, Out out) {
   using F = H2F<P, 3>;
   constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
   constexpr int NPT = (Q2 + NT - 1) / NT;
-  double *M = rhs2_smem<P, 3>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
-  double *gF = N2 + F::NN, *H = gF + 12 * QF;
+  R *M = rhs2_smem<P, 3, R>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  R *gF = N2 + F::NN, *H = gF + 12 * QF;
   const int tid = threadIdx.x;
   double un[NPT][3];
-  load_nodal2<P, 3, NPT>(U, ne, e, M, X1, N1, un, tid);
+  load_nodal2<P, 3, NPT, R>(U, ne, e, M, X1, N1, un, tid);
   bool bad = false;
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, a = i - s * QF;
@@ -282,7 +299,7 @@ __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, cons
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       const double Fh = 0.5 * (FL[v] + FR[v] + lmax * (UL[v] - UR[v]));
-      gF[(v * 4 + s) * QF + a] = jr * (Fh - FL[v]);
+      gF[(v * 4 + s) * QF + a] = (R)(jr * (Fh - FL[v]));
     }
   }
   if (bad) atomicOr(flag, 1);
@@ -297,16 +314,16 @@ __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, cons
     const double fx[3] = {mx, mx * u + p, my * u}, fy[3] = {my, mx * vv, my * vv + p};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      N1[v * Q2 + k] = rx * fx[v] + ry * fy[v];
-      N2[v * Q2 + k] = sx * fx[v] + sy * fy[v];
+      N1[v * Q2 + k] = (R)(rx * fx[v] + ry * fy[v]);
+      N2[v * Q2 + k] = (R)(sx * fx[v] + sy * fy[v]);
     }
   }
   __syncthreads();
-  divergence2<P, 3>(M, X1, X2, N1, N2, gF, H, tid);
+  divergence2<P, 3, R>(M, X1, X2, N1, N2, gF, H, tid);
   const double cw[4] = {1.0, 1.0, 1.0, 1.0};
   for (int i = tid; i < 3 * L2; i += NT) {
     const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
-    const double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    const double r = M[i] + face_add2<P>(TB<R, P>::Bend(), H + v * 4 * L, l, m, cw);
     const size_t o = ((size_t)v * ne + e) * L2 + k;
     if (bal == 0) {
       out(o, r);
@@ -320,18 +337,18 @@ __device__ __forceinline__ void shallow2_elem(const double *__restrict__ U, cons
   __syncthreads();
 }
 
-template <int P>
+template <int P, typename RT = double>
 __global__ void __launch_bounds__(64)
 k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *__restrict__ R,
            double *__restrict__ Bal, int bal, Tab tb, Geo geo, Phys ph, int ne, int *__restrict__ flag) {
-  shallow2_elem<P>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
+  shallow2_elem<P, RT>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) { R[o] = r; });
 }
 
-template <int P>
+template <int P, typename R = double>
 __global__ void __launch_bounds__(64)
 k_shallow_step2(double *U, const double *__restrict__ T, double *__restrict__ Ut, double *__restrict__ Bal, int bal,
                 Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
-  shallow2_elem<P>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
+  shallow2_elem<P, R>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
     const double ut = fma(mdt, r, A * Ut[o]);
     Ut[o] = ut;
     U[o] = fma(B, ut, U[o]);
@@ -669,17 +686,17 @@ k_observe_adjoint(const double *__restrict__ U, double *__restrict__ Ubar, const
 // are in work-item order, b[r].off their first item.  The step kernel fuses
 // the LSRK14 register update of the direct path (vField axpby, axpy:
 // Ut = A Ut - dt R, U += B Ut) into the residual's store.
-template <int P, int NF>
+template <int P, int NF, typename R = double>
 __global__ void __launch_bounds__(64)
 k_batch_traces2(const Batch2 *__restrict__ b, int nreq, int nwork) {
   int r = 0;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     while (r + 1 < nreq && b[r + 1].off <= w) ++r;
-    traces2_elem<P, NF>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
+    traces2_elem<P, NF, R>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
   }
 }
 
-template <int P, bool SHALLOW>
+template <int P, bool SHALLOW, typename R = double>
 __global__ void __launch_bounds__(64)
 k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, double B) {
   int r = 0;
@@ -695,9 +712,9 @@ k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, doubl
     const Geo geo = b[r].geo;
     const Phys ph = b[r].phys;
     if constexpr (SHALLOW)
-      shallow2_elem<P>(U, b[r].T, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+      shallow2_elem<P, R>(U, b[r].T, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
     else
-      rhs2_elem<P, false>(U, b[r].T, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+      rhs2_elem<P, false, R>(U, b[r].T, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
   }
 }
 
@@ -716,7 +733,7 @@ struct LsrkTab {
 // the even number of stages of a step the current traces are in T again.
 // Each element's arithmetic is that of k_batch_traces2 and k_batch_step2, so
 // the results are bitwise those of the per-stage launches.
-template <int P, bool SHALLOW>
+template <int P, bool SHALLOW, typename R = double>
 __global__ void __launch_bounds__(64)
 k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, LsrkTab c) {
   constexpr int NF = SHALLOW ? 3 : 4;
@@ -725,7 +742,7 @@ k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, 
     int r = 0;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
       while (r + 1 < nreq && b[r + 1].off <= w) ++r;
-      traces2_elem<P, NF>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
+      traces2_elem<P, NF, R>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
     }
   }
   grid.sync();
@@ -749,18 +766,18 @@ k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, 
         const Geo geo = b[r].geo;
         const Phys ph = b[r].phys;
         if constexpr (SHALLOW)
-          shallow2_elem<P>(U, Tc, b[r].Bal, b[r].bal, geo, ph, ne, e, b[r].flag, update);
+          shallow2_elem<P, R>(U, Tc, b[r].Bal, b[r].bal, geo, ph, ne, e, b[r].flag, update);
         else
-          rhs2_elem<P, false>(U, Tc, nullptr, nullptr, geo, ph, ne, e, b[r].flag, update);
-        if (!last) traces2_elem<P, NF>(U, Tn, ne, e);   // the element's new state, read back by its own workgroup
+          rhs2_elem<P, false, R>(U, Tc, nullptr, nullptr, geo, ph, ne, e, b[r].flag, update);
+        if (!last) traces2_elem<P, NF, R>(U, Tn, ne, e);   // the element's new state, read back by its own workgroup
       }
       grid.sync();
     }
 }
 
-template <int P, bool SHALLOW>
+template <int P, bool SHALLOW, typename R>
 static hipError_t launch_persist2(BatchLaunch &a) {
-  const void *kern = reinterpret_cast<const void *>(&k_batch_persist2<P, SHALLOW>);
+  const void *kern = reinterpret_cast<const void *>(&k_batch_persist2<P, SHALLOW, R>);
   int dev = 0, ncu = 0, per = 0;
   hipError_t e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -783,24 +800,29 @@ static hipError_t launch_persist2(BatchLaunch &a) {
   return hipLaunchCooperativeKernel(kern, dim3(grid), dim3(64), args, 0, a.stream);
 }
 
-template <int P>
-static hipError_t launch_batch2_p(BatchLaunch &a) {
-  if (a.op == 2) return a.shallow ? launch_persist2<P, true>(a) : launch_persist2<P, false>(a);
+template <int P, typename R>
+static hipError_t launch_batch2_pr(BatchLaunch &a) {
+  if (a.op == 2) return a.shallow ? launch_persist2<P, true, R>(a) : launch_persist2<P, false, R>(a);
   const int grid = std::max(1, std::min(a.nwork, kBatchGrid));
   if (a.op == 0) {
     if (a.shallow)
-      hipLaunchKernelGGL((k_batch_traces2<P, 3>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
+      hipLaunchKernelGGL((k_batch_traces2<P, 3, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
     else
-      hipLaunchKernelGGL((k_batch_traces2<P, 4>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
+      hipLaunchKernelGGL((k_batch_traces2<P, 4, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork);
   } else {
     if (a.shallow)
-      hipLaunchKernelGGL((k_batch_step2<P, true>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
+      hipLaunchKernelGGL((k_batch_step2<P, true, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
                          a.A, a.B);
     else
-      hipLaunchKernelGGL((k_batch_step2<P, false>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
+      hipLaunchKernelGGL((k_batch_step2<P, false, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
                          a.A, a.B);
   }
   return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_batch2_p(BatchLaunch &a) {
+  return a.mixed ? launch_batch2_pr<P, float>(a) : launch_batch2_pr<P, double>(a);
 }
 
 hipError_t launch_batch2(BatchLaunch &a) {
@@ -852,10 +874,19 @@ static hipError_t launch2(const LaunchArgs &a) {
   const int ne = a.ne;
   switch (a.op) {
     case OP_TRACES:
-      hipLaunchKernelGGL(k_traces2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, a.nfield);
+      if (a.mixed)
+        hipLaunchKernelGGL((k_traces2<P, float>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, a.nfield);
+      else
+        hipLaunchKernelGGL(k_traces2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.tab, ne, a.nfield);
       break;
     case OP_RHS:
-      if (a.eqn == 30)
+      if (a.mixed && a.eqn == 30)
+        hipLaunchKernelGGL((k_shallow2<P, float>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.R, a.Bal, a.bal,
+                           a.tab, a.geo, a.phys, ne, a.flag);
+      else if (a.mixed)
+        hipLaunchKernelGGL((k_rhs2<P, false, float>), dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.V, a.TV,
+                           a.R, a.tab, a.geo, a.phys, ne, a.flag);
+      else if (a.eqn == 30)
         hipLaunchKernelGGL(k_shallow2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.R, a.Bal, a.bal,
                            a.tab, a.geo, a.phys, ne, a.flag);
       else if (a.ns)
@@ -866,7 +897,13 @@ static hipError_t launch2(const LaunchArgs &a) {
                            a.R, a.tab, a.geo, a.phys, ne, a.flag);
       break;
     case OP_STEP:
-      if (a.eqn == 30)
+      if (a.mixed && a.eqn == 30)
+        hipLaunchKernelGGL((k_shallow_step2<P, float>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.Ut, a.Bal,
+                           a.bal, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+      else if (a.mixed)
+        hipLaunchKernelGGL((k_step2<P, false, float>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV,
+                           a.Ut, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+      else if (a.eqn == 30)
         hipLaunchKernelGGL(k_shallow_step2<P>, dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.Ut, a.Bal, a.bal,
                            a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
       else if (a.ns)
@@ -907,7 +944,10 @@ hipError_t launch_2d(const LaunchArgs &a) {
 }
 
 hipError_t upload_tables_2d(const double *h) {
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  if (e != hipSuccess) return e;
+  std::vector<float> f(h, h + kTabTotal);
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tabf), f.data(), sizeof(float) * kTabTotal);
 }
 
 }  // namespace dgx

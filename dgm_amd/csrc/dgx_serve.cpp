@@ -9,8 +9,8 @@
 // second is captured into a hipGraph (no memset nodes: the stepping kernels
 // clear nothing), which later full slices replay.
 //
-// With batching, 2D Euler and shallow-water FP64 LSRK14 requests of the same
-// order join a group instead: each stage of a group is two launches (face
+// With batching, 2D Euler and shallow-water LSRK14 requests of the same order
+// and precision (FP64, or DGX_MIXED: FP32 contractions) join a group instead: each stage of a group is two launches (face
 // traces; residual with the fused register update) over the elements of all
 // its members, on fixed grids of workgroups that walk the members' elements.
 // Members step in lockstep slices of min(steps_per_slice, fewest remaining
@@ -68,6 +68,7 @@ struct Request {
 // descriptors, and the stream and event of the group's slices.
 struct Group {
   int eqn = 0, p = 0;
+  bool mixed = false;                // FP32 LDS tensors and contractions (DGX_MIXED members)
   std::vector<Request *> members;
   std::vector<Request *> stepping;   // the members of the slice in flight
   std::vector<dgx::Batch2> hd;
@@ -165,18 +166,20 @@ int enqueue_slice(dgx_server *s, Request *r) {
 bool batchable(const dgx_server *s, const Request *r) {
   const dgx_domain_desc &d = r->desc;
   return s->cfg.batch && d.nsd == 2 && !d.xnodes && r->h && !r->h->curved && !r->h->varp &&
-         (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) && d.precision == DGX_FP64 &&
+         (d.eqntype == DGX_EULER || d.eqntype == DGX_SHALLOW) &&
+         (d.precision == DGX_FP64 || d.precision == DGX_MIXED) &&
          r->inttype == DGX_LSRK14 && d.p >= 1 && d.p <= 10;
 }
 
 // Join r (admitted, state uploaded) to the group of its physics and order.
 int join_group(dgx_server *s, Request *r) {
   Group *g = nullptr;
+  const bool mixed = r->desc.precision == DGX_MIXED;
   for (auto &x : s->groups)
-    if (x->eqn == r->desc.eqntype && x->p == r->desc.p) g = x.get();
+    if (x->eqn == r->desc.eqntype && x->p == r->desc.p && x->mixed == mixed) g = x.get();
   if (!g) {
     auto ng = std::make_unique<Group>();
-    ng->eqn = r->desc.eqntype; ng->p = r->desc.p;
+    ng->eqn = r->desc.eqntype; ng->p = r->desc.p; ng->mixed = mixed;
     if (s->cfg.persistent) {
       // cooperative launches must not run side by side (each waits for all its
       // workgroups to be resident): every group's slices share one stream
@@ -244,7 +247,7 @@ int enqueue_group(dgx_server *s, Group *g) {
   }
   dgx::BatchLaunch a;
   a.p = g->p; a.nreq = (int)g->hd.size(); a.nwork = g->nwork; a.shallow = g->eqn == DGX_SHALLOW;
-  a.b = g->dd; a.stream = g->stream;
+  a.mixed = g->mixed; a.b = g->dd; a.stream = g->stream;
   if (s->cfg.persistent) {   // the whole slice in one cooperative launch
     a.op = 2; a.nsteps = k;
     std::copy(dgx::kLsrkA, dgx::kLsrkA + 14, a.lsrkA);

@@ -62,7 +62,11 @@ enum dgx_flux { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE = 1 };
  * lifted over all faces. */
 enum dgx_viscous { DGX_VISCOUS_BR1 = 0, DGX_VISCOUS_BR2 = 1 };
 
-/* Arithmetic of the residual. */
+/* Arithmetic of the residual.  DGX_MIXED: hexes (Euler3d, Navier_Stokes3d,
+ * p <= 8) run their sum-factorized contractions on FP32 MFMA; affine quads
+ * (Euler, Shallow) hold their element tensors in LDS and contract them in
+ * FP32.  The state, RK registers, face traces, metric terms, Riemann fluxes
+ * and pointwise physics stay FP64. */
 enum dgx_precision { DGX_FP64 = 0, DGX_MIXED = 1 };
 
 /* Time integrators: the reference's inttype numbers
@@ -309,8 +313,8 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes);
  *  - The first slice of a request runs directly.
  *  - The second slice is captured into a hipGraph, which later full slices
  *    replay, so small meshes are not bound by launch overhead.
- *  - With `batch`, 2D Euler and shallow-water FP64 LSRK14 requests of one
- *    order form a group: each stage is one face-trace launch and one
+ *  - With `batch`, 2D Euler and shallow-water LSRK14 requests of one order
+ *    and precision form a group: each stage is one face-trace launch and one
  *    residual-and-update launch over the elements of every member, on a fixed
  *    grid of workgroups that walk the members' elements.  Members step in
  *    lockstep slices and join or leave at slice boundaries.
@@ -325,8 +329,8 @@ typedef struct {
   int max_active;        /* admitted requests at once */
   int steps_per_slice;   /* steps enqueued per request and pass */
   int use_graphs;        /* replay slices from a captured hipGraph (1) or launch directly (0) */
-  int batch;             /* 1: batch 2D Euler / shallow-water FP64 LSRK14 requests of one order
-                            into shared launches (one per stage and kind for all of them) */
+  int batch;             /* 1: batch 2D Euler / shallow-water LSRK14 requests of one order and
+                            precision into shared launches (one per stage and kind for all of them) */
   int persistent;        /* with batch: 1 runs each group slice as one persistent cooperative
                             kernel (every stage, grid-wide barriers between stages) instead of
                             two launches per stage; results are bitwise the same */

@@ -103,7 +103,7 @@ def test_create_rejects_bad_arguments(lib, case, msg):
 
 @pytest.mark.parametrize("case,msg", [
     (dict(flux=1), b"Lax-Friedrichs flux only"),
-    (dict(precision=1), b"DGX_MIXED"),
+    (dict(precision=7), b"precision must be"),
     (dict(nsd3=True), b"eqntype does not match nsd"),
     (dict(boundary=4), b"unknown boundary type"),
     (dict(boundary=0, eqn=2), b"boundary sides are available"),
@@ -133,12 +133,17 @@ def test_curved_navier_stokes_quads_rejected_with_boundaries(lib):
     assert b"boundary sides are available" in lib.dgx_last_error()
 
 
-def test_mixed_rejected_in_2d(lib):
+def test_mixed_rejected_for_2d_navier_stokes_and_curved_quads(lib):
     m = meshgen.periodic_quads(2, 2)
-    d, keep = _desc(m, 3, 1, precision=1)
+    d, keep = _desc(m, 3, 2, precision=1, vis=0.01)
     h = ctypes.c_void_p()
     assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
     assert b"DGX_MIXED" in lib.dgx_last_error()
+    d, keep = _desc(m, 3, 1, precision=1)
+    xn = np.ascontiguousarray(meshgen.quad_nodes(m, 3))
+    d.xnodes = xn.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.dgx_domain_create(ctypes.byref(d), None, ctypes.byref(h)) == -1
+    assert b"DGX_FP64" in lib.dgx_last_error()
 
 
 def test_null_arguments_rejected(lib):
