@@ -257,6 +257,16 @@ def adjoint_sample(args, seed=4096):
         fd = (prob.objective(theta + h * d, data) - prob.objective(theta - h * d, data)) / (2 * h)
         errs.append(abs(fd - g @ d) / max(abs(fd), 1e-300))
     prob.close()
+    # the mixed variant (FP32 element contractions) on the same inputs
+    pm = acoustic.AcousticProblem(m, p, (1.0, 0.0, 0.0, 1.0 / 1.4), rec, f0, t0, dt, nsteps, nsnap=nsnap,
+                                  precision=dgx.DGX_MIXED)
+    t0w = time.perf_counter()
+    pm.forward(theta)
+    tm_fwd = time.perf_counter() - t0w
+    t0w = time.perf_counter()
+    Jm, gm = pm.gradient(theta, data)
+    tm_grad = time.perf_counter() - t0w
+    pm.close()
     dofs = 4 * m.ne * (p + 1) ** 2
     adv, _, _ = acoustic.revolve_count(nsteps, nsnap)
     return {"workload": f"linearized 2D Euler acoustics, {n}x{n} quads, p={p}, {m.ne} source coefficients, "
@@ -265,7 +275,10 @@ def adjoint_sample(args, seed=4096):
             "forward_steps_per_gradient": adv, "adjoint_steps_per_gradient": nsteps,
             "forward_dof_stage_per_s": dofs * 14 * nsteps / t_fwd,
             "fd_directions": len(errs), "fd_rel_err_max": float(max(errs)) if errs else None,
-            "fd_rel_err": [float(e) for e in errs]}
+            "fd_rel_err": [float(e) for e in errs],
+            "mixed": {"forward_s": tm_fwd, "gradient_s": tm_grad, "J_rel_vs_fp64": abs(Jm - J) / J,
+                      "gradient_rel_l2_vs_fp64": float(np.linalg.norm(gm - g) / np.linalg.norm(g)),
+                      "speedup_gradient_vs_fp64": t_grad / tm_grad}}
 
 
 def euler_adjoint_sample(args, seed=77):

@@ -75,14 +75,15 @@ def revolve_count(nsteps, nsnap):
 class AcousticProblem:
     """One inverse-source problem on the GPU (FP64)."""
 
-    def __init__(self, mesh, p, mean, receivers, f0, t0, dt, nsteps, nsnap=0, gamma=1.4, stream=None):
+    def __init__(self, mesh, p, mean, receivers, f0, t0, dt, nsteps, nsnap=0, gamma=1.4, stream=None,
+                 precision=dgx.DGX_FP64):
         self.lib = _bind(dgx.load())
         self.ne, self.nsteps = int(mesh.ne), int(nsteps)
         self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
         self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
         self._rec = np.ascontiguousarray(np.asarray(receivers, dtype=np.float64).reshape(-1, 2))
         self.nrec = len(self._rec)
-        md = dgx.DomainDesc(2, self.ne, int(p), DGX_LINEULER, 0, dgx.DGX_FP64, float(gamma), 0.0, 0.72,
+        md = dgx.DomainDesc(2, self.ne, int(p), DGX_LINEULER, 0, int(precision), float(gamma), 0.0, 0.72,
                             _dp(self._vert), len(self._con), self._con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                             0, 0.0)
         rho0, u0, v0, p0 = (float(v) for v in mean)

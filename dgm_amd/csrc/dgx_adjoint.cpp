@@ -41,6 +41,7 @@ double ricker(double t, double f0, double t0) {
 struct dgx_acoustic {
   dgx_handle dom = nullptr;
   int p = 0, ne = 0, nm = 0, nrec = 0, nsteps = 0, nsnap = 0;
+  bool mixed = false;   // DGX_MIXED: FP32 contractions in k_lin2 and the traces
   double dt = 0, f0 = 0, t0 = 0, u0 = 0, v0 = 0, c0 = 0;
   double Ax[16], Ay[16], cobs[4];
   int *d_rec_e = nullptr;
@@ -68,7 +69,8 @@ void release(dgx_acoustic *a) {
 dgx::LinArgs lin_args(dgx_acoustic *a, bool adjoint) {
   dgx_domain *d = a->dom;
   dgx::LinArgs la;
-  la.ne = a->ne; la.p = a->p; la.adjoint = adjoint; la.tab = d->tab; la.geo = d->geo; la.stream = d->stream;
+  la.ne = a->ne; la.p = a->p; la.adjoint = adjoint; la.mixed = a->mixed; la.tab = d->tab; la.geo = d->geo;
+  la.stream = d->stream;
   la.u0 = a->u0; la.v0 = a->v0; la.c0 = a->c0; la.dt = a->dt;
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
@@ -82,7 +84,7 @@ int traces(dgx_acoustic *a, const double *X) {
   dgx_domain *d = a->dom;
   dgx::LaunchArgs la;
   la.op = dgx::OP_TRACES; la.p = a->p; la.ne = a->ne; la.nfield = 4; la.U = X; la.T = d->T;
-  la.tab = d->tab; la.geo = d->geo; la.stream = d->stream;
+  la.tab = d->tab; la.geo = d->geo; la.stream = d->stream; la.mixed = a->mixed;
   HIPCHK(dgx::launch_2d(la));
   return DGX_OK;
 }
@@ -152,7 +154,8 @@ int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad
   *out = nullptr;
   if (mesh->nsd != 2 || mesh->eqntype != DGX_LINEULER)
     return fail(DGX_ERR_ARGUMENT, "the acoustic problem needs a 2D mesh with eqntype DGX_LINEULER");
-  if (mesh->precision != DGX_FP64) return fail(DGX_ERR_ARGUMENT, "the acoustic problem runs in DGX_FP64");
+  if (mesh->precision != DGX_FP64 && mesh->precision != DGX_MIXED)
+    return fail(DGX_ERR_ARGUMENT, "the acoustic problem runs in DGX_FP64 or DGX_MIXED");
   if (!(ad->rho0 > 0) || !(ad->p0 > 0) || !std::isfinite(ad->u0) || !std::isfinite(ad->v0))
     return fail(DGX_ERR_ARGUMENT, "mean flow needs rho0 > 0, p0 > 0 and finite velocity");
   if (!(ad->dt > 0) || ad->nsteps < 1) return fail(DGX_ERR_ARGUMENT, "dt > 0 and nsteps >= 1 required");
@@ -162,6 +165,7 @@ int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad
   dd.eqntype = DGX_EULER;   // geometry, links, tables and work buffers of a 4-field quad domain
   dd.flux = DGX_FLUX_LAX_FRIEDRICHS;
   dd.vis = 0.0;
+  dd.precision = DGX_FP64;   // the acoustic kernels take the precision from the problem
   if (mesh->xnodes || mesh->bc_function || mesh->source)
     return fail(DGX_ERR_ARGUMENT, "the acoustic problem runs on affine quads (no xnodes or functions)");
   for (int r = 0; r < mesh->ncon; ++r)
@@ -177,6 +181,7 @@ int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad
     return fail(DGX_ERR_ARGUMENT, "the acoustic problem needs parallelogram quads (constant Jacobians)");
   }
   a->p = d->p; a->ne = d->ne; a->nm = d->nm; a->nrec = ad->nrec; a->nsteps = ad->nsteps;
+  a->mixed = mesh->precision == DGX_MIXED;
   a->dt = ad->dt; a->f0 = ad->f0; a->t0 = ad->t0; a->u0 = ad->u0; a->v0 = ad->v0;
   const double g = d->gamma, g1 = g - 1.0, rho0 = ad->rho0, u0 = ad->u0, v0 = ad->v0, p0 = ad->p0;
   a->c0 = std::sqrt(g * p0 / rho0);

@@ -175,6 +175,7 @@ struct LinArgs {
   double ca = 0, cb = 0, dt = 0, g = 0;   // LSRK coefficients, source amplitude g(t_stage)
   int ne = 0, p = 0;
   bool adjoint = false;
+  bool mixed = false;            // FP32 LDS tensors and contractions, FP64 state and registers
   Tab tab;
   Geo geo;
   hipStream_t stream = nullptr;

@@ -540,14 +540,14 @@ k_backward2(const double *__restrict__ Fm, double *__restrict__ Fn, Tab tb, int 
 // modal coordinates (exact quadrature throughout), and the kernel performs one
 // reversed stage: Ubar += -dt L^T X, dJ/dtheta_e += 2 g dt X[E][e][0],
 // Ubar_t = ca X.
-template <int P, bool ADJ>
+template <int P, bool ADJ, typename R = double>
 __global__ void __launch_bounds__(64)
 k_lin2(LinArgs a) {
   using F = H2F<P, 4>;
   constexpr int L = F::L, QF = F::QF, L2 = F::L2, Q2 = F::Q2, NT = 64;
   constexpr int NPT = (Q2 + NT - 1) / NT;
-  double *M = rhs2_smem<P, 4>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
-  double *gF = N2 + F::NN, *H = gF + 16 * QF;
+  R *M = rhs2_smem<P, 4, R>(), *X1 = M + F::NM, *X2 = X1 + F::NX, *N1 = X2 + F::NX, *N2 = N1 + F::NN;
+  R *gF = N2 + F::NN, *H = gF + 16 * QF;
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const Geo &geo = a.geo;
   const double *mt = geo.metric + 9 * (size_t)e;
@@ -555,7 +555,7 @@ k_lin2(LinArgs a) {
   const double dm = rx * sy - ry * sx;   // 1 / J_e
   const double sc = ADJ ? dm : 1.0;
   double un[NPT][4];
-  load_nodal2<P, 4, NPT>(a.X, ne, e, M, X1, N1, un, tid);
+  load_nodal2<P, 4, NPT, R>(a.X, ne, e, M, X1, N1, un, tid);
   for (int i = tid; i < 4 * QF; i += NT) {
     const int s = i / QF, q = i - s * QF;
     const int es = e * 4 + s;
@@ -584,7 +584,7 @@ k_lin2(LinArgs a) {
         fr = fma(An, UR[j], fr);
       }
       const double fh = 0.5 * (fl + fr + lam * (UL[v] - UR[v]));
-      gF[(v * 4 + s) * QF + q] = jr * (fh - fl);
+      gF[(v * 4 + s) * QF + q] = (R)(jr * (fh - fl));
     }
   }
 #pragma unroll
@@ -600,16 +600,16 @@ k_lin2(LinArgs a) {
         fx = fma(a.Ax[4 * v + j], u, fx);
         fy = fma(a.Ay[4 * v + j], u, fy);
       }
-      N1[v * Q2 + k] = rx * fx + ry * fy;
-      N2[v * Q2 + k] = sx * fx + sy * fy;
+      N1[v * Q2 + k] = (R)(rx * fx + ry * fy);
+      N2[v * Q2 + k] = (R)(sx * fx + sy * fy);
     }
   }
   __syncthreads();
-  divergence2<P, 4>(M, X1, X2, N1, N2, gF, H, tid);
+  divergence2<P, 4, R>(M, X1, X2, N1, N2, gF, H, tid);
   const double cw[4] = {1.0, 1.0, 1.0, 1.0};
   for (int i = tid; i < 4 * L2; i += NT) {
     const int v = i / L2, k = i - v * L2, l = k / L, m = k - l * L;
-    double r = M[i] + face_add2<P>(CT<P>::Bend(), H + v * 4 * L, l, m, cw);
+    double r = M[i] + face_add2<P>(TB<R, P>::Bend(), H + v * 4 * L, l, m, cw);
     const size_t gi = ((size_t)v * ne + e) * L2 + k;
     if (!ADJ) {
       if (v == 3 && k == 0) r -= 2.0 * a.theta[e] * a.g;   // source: mode 0 of the constant theta_e
@@ -838,8 +838,12 @@ hipError_t launch_batch2(BatchLaunch &a) {
 
 template <int P>
 static hipError_t launch_lin2_p(const LinArgs &a) {
-  if (a.adjoint)
+  if (a.adjoint && a.mixed)
+    hipLaunchKernelGGL((k_lin2<P, true, float>), dim3(a.ne), dim3(64), 0, a.stream, a);
+  else if (a.adjoint)
     hipLaunchKernelGGL((k_lin2<P, true>), dim3(a.ne), dim3(64), 0, a.stream, a);
+  else if (a.mixed)
+    hipLaunchKernelGGL((k_lin2<P, false, float>), dim3(a.ne), dim3(64), 0, a.stream, a);
   else
     hipLaunchKernelGGL((k_lin2<P, false>), dim3(a.ne), dim3(64), 0, a.stream, a);
   return hipGetLastError();

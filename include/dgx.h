@@ -394,7 +394,9 @@ typedef struct {
   int nsnap;                 /* checkpoint slots, state 0 included (0: ceil(sqrt(nsteps)) + 1) */
 } dgx_acoustic_desc;
 
-/* `mesh`: nsd 2, eqntype DGX_LINEULER, precision DGX_FP64 (flux and vis are
+/* `mesh`: nsd 2, eqntype DGX_LINEULER, precision DGX_FP64 or DGX_MIXED (FP32
+ * element contractions in the forward and adjoint kernels, FP64 state,
+ * registers, observations and gradient accumulation; flux and vis are
  * ignored).  Allocates nsnap + 4 states of 4 x ne x (p+1)^2 doubles. */
 int dgx_acoustic_create(const dgx_domain_desc *mesh, const dgx_acoustic_desc *ad, void *stream,
                         dgx_acoustic_handle *out);

@@ -14,7 +14,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from dgm_amd import acoustic, mesh as meshgen
+from dgm_amd import acoustic, dgx, mesh as meshgen
 from acoustic_ref import AcousticRef
 from test_oracle_acoustic import stretched_quads
 
@@ -100,3 +100,40 @@ def test_checkpoint_count_does_not_change_the_gradient():
         gpu.close()
     for J, g in out[1:]:
         assert J == out[0][0] and np.array_equal(g, out[0][1])
+
+
+def test_mixed_against_fp64():
+    """DGX_MIXED (FP32 element contractions in the forward and adjoint
+    kernels; FP64 state, registers, observations and gradient sums) against
+    the FP64 problem on the same inputs, and the mixed gradient against central
+    differences of the mixed objective.  Stated bars (measured in brackets):
+    observations 1e-5 [1.9e-6], J 1e-6 [3.9e-8], gradient 1e-5 [8.2e-7]; the
+    mixed gradient against its own objective's differences, |fd - g.d| /
+    (|g| |d|), 1e-6 (relative to |fd| alone it is large in directions nearly
+    orthogonal to g, where fd is small)."""
+    m, p, rec, dt = _medium()
+    nsteps = 200
+    print(f"[acoustic] mixed vs FP64 16x16 p={p} {nsteps} steps", flush=True)
+    g64 = acoustic.AcousticProblem(m, p, MEAN, rec, 20.0, 0.05, dt, nsteps, gamma=GAMMA)
+    g32 = acoustic.AcousticProblem(m, p, MEAN, rec, 20.0, 0.05, dt, nsteps, gamma=GAMMA,
+                                   precision=dgx.DGX_MIXED)
+    rng = np.random.default_rng(4097)
+    theta_true = rng.standard_normal(m.ne)
+    y64, y32 = g64.forward(theta_true), g32.forward(theta_true)
+    ey = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
+    theta = rng.standard_normal(m.ne)
+    J64, gr64 = g64.gradient(theta, y64)
+    J32, gr32 = g32.gradient(theta, y64)
+    eJ, eg = abs(J32 - J64) / J64, np.linalg.norm(gr32 - gr64) / np.linalg.norm(gr64)
+    print(f"[acoustic] mixed: observations rel {ey:.2e}, J rel {eJ:.2e}, gradient rel {eg:.2e}", flush=True)
+    errs = []
+    for k in range(4):
+        d = rng.standard_normal(m.ne)
+        h = 1e-1
+        fd = (g32.objective(theta + h * d, y64) - g32.objective(theta - h * d, y64)) / (2 * h)
+        errs.append(abs(fd - gr32 @ d) / (np.linalg.norm(gr32) * np.linalg.norm(d)))
+    print(f"[acoustic] mixed gradient vs mixed differences: {['%.2e' % e for e in errs]}", flush=True)
+    assert ey <= 1e-5 and eJ <= 1e-6 and eg <= 1e-5, (ey, eJ, eg)
+    assert max(errs) <= 1e-6, errs
+    g64.close()
+    g32.close()

@@ -190,7 +190,7 @@ def test_domain_bytes_estimate(lib):
 @pytest.mark.parametrize("case,msg", [
     (dict(nsd=3), b"2D mesh"),
     (dict(eqn=1), b"DGX_LINEULER"),
-    (dict(precision=1), b"DGX_FP64"),
+    (dict(precision=7), b"DGX_FP64 or DGX_MIXED"),
     (dict(rho0=0.0), b"mean flow"),
     (dict(dt=0.0), b"dt > 0"),
     (dict(nsteps=0), b"nsteps"),
