@@ -349,8 +349,10 @@ def curved_sample(args, seed=5):
     carries a smooth bump of amplitude 0.05 (curved elements throughout,
     metrics at the nodes), p = 5, slip wall on the bump, the free stream imposed
     on the top boundary, periodic sides; free stream at Mach 0.5 plus seeded
-    density noise of 1e-3.  FP64, LSRK14; one warmup step, then `cv_steps`
-    timed steps."""
+    density noise of 1e-3.  LSRK14 in FP64 and in the mixed precision (FP32
+    residual contractions, FP64 mass solve and state) from the same state;
+    one warmup step, then `cv_steps` timed steps each, and the mixed state's
+    deviation from the FP64 one."""
     n, p = args.cv_n, 5
     lo, hi = (0.0, 0.0, 0.0), (3.0, 3.0, 1.0)
     m = meshgen.channel_hexes(n, n, n, lo, hi, bc_bottom=dgx.DGX_BC_WALL, bc_top=dgx.DGX_BC_STATE)
@@ -365,24 +367,33 @@ def curved_sample(args, seed=5):
     U[2] = 0.0
     U[3] = 0.0
     U[4] = E0
-    gpu = dgx.GPUDomain(m, p, dgx.DGX_EULER3D, gamma=1.4, bc_state=[rho0, rho0 * u0, 0.0, 0.0, E0], xnodes=xn)
-    del xn
-    gpu.upload(U)
-    del U
-    gpu.synchronize()
-    t_setup = time.perf_counter() - t0
     dt = 0.3 * (hi[2] / n) / ((p + 1) ** 2 * (u0 + 1.0))
     dofs = 5 * m.ne * (p + 1) ** 3
-    print(f"[bench] curved: {n}^3 p={p}: {dofs / 1e6:.1f} M DOF, setup {t_setup:.1f} s", file=sys.stderr, flush=True)
-    gpu.advance(dgx.DGX_LSRK14, dt, 1)
-    gpu.synchronize()
-    t0 = time.perf_counter()
-    gpu.advance(dgx.DGX_LSRK14, dt, args.cv_steps)
-    gpu.synchronize()
-    el = time.perf_counter() - t0
-    finite = bool(np.all(np.isfinite(gpu.norms())))
-    cg_iters = gpu.solver_iterations()
-    gpu.close()
+    res = {}
+    for prec in (dgx.DGX_FP64, dgx.DGX_MIXED):
+        t0 = time.perf_counter()
+        gpu = dgx.GPUDomain(m, p, dgx.DGX_EULER3D, gamma=1.4, bc_state=[rho0, rho0 * u0, 0.0, 0.0, E0], xnodes=xn,
+                            precision=prec)
+        gpu.upload(U)
+        gpu.synchronize()
+        t_setup = time.perf_counter() - t0
+        print(f"[bench] curved: {n}^3 p={p} precision {prec}: {dofs / 1e6:.1f} M DOF, setup {t_setup:.1f} s",
+              file=sys.stderr, flush=True)
+        gpu.advance(dgx.DGX_LSRK14, dt, 1)
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        gpu.advance(dgx.DGX_LSRK14, dt, args.cv_steps)
+        gpu.synchronize()
+        el = time.perf_counter() - t0
+        finite = bool(np.all(np.isfinite(gpu.norms())))
+        res[prec] = (el, t_setup, finite, gpu.solver_iterations(), gpu.download(dgx.DGX_MODAL))
+        gpu.close()
+    del xn, U
+    el, t_setup, finite, cg_iters, U64 = res[dgx.DGX_FP64]
+    elm, _, finite_m, _, U32 = res[dgx.DGX_MIXED]
+    err_f = [float(np.linalg.norm(U32[v] - U64[v]) / np.linalg.norm(U64[v])) for v in range(5)]
+    err_w = [float(np.linalg.norm(U32[v] - U64[v]) / np.linalg.norm(U64)) for v in range(5)]
+    del res, U64, U32
     value = dofs * 14 * args.cv_steps / el
     # HBM bytes per DOF-stage: 32 for the 2N state streams, plus the per-node
     # metrics read once per stage (10 doubles per node over the 5 fields' modes)
@@ -392,6 +403,9 @@ def curved_sample(args, seed=5):
             "value": value, "unit": "DOF-stage/s", "dofs": dofs, "steps": args.cv_steps, "dt": dt,
             "ms_per_step": el / args.cv_steps * 1e3, "setup_s": t_setup, "state_finite": finite,
             "mass_solve_cg_iterations_max": cg_iters,
+            "mixed": {"value": dofs * 14 * args.cv_steps / elm, "speedup_vs_fp64": el / elm, "state_finite": finite_m,
+                      "rel_l2_per_field_vs_fp64": err_f, "rel_l2_per_field_of_state_vs_fp64": err_w,
+                      "after_steps": args.cv_steps + 1},
             "hbm_gbs_at_algorithmic_bytes": value * (32.0 + metric_b) / 1e9,
             "algorithmic_bytes_per_dof_stage": 32.0 + metric_b}
 

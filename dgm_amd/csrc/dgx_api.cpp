@@ -336,6 +336,7 @@ dgx::CurvedArgs curved_args(dgx_domain *d) {
   ca.shallow = d->eqn == DGX_SHALLOW; ca.stream = d->stream;
   ca.ns = d->visc; ca.G = d->G; ca.V = d->V; ca.TV = d->TV;
   ca.fn = d->fn; ca.t = d->time;
+  ca.mixed = d->mixed && d->nsd == 3;
   return ca;
 }
 
@@ -1431,16 +1432,19 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
         (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_SHALLOW))))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED is available for hexes (Euler3d, Navier_Stokes3d) with p <= 8 and for "
                                   "affine quads (Euler, Shallow)");
+  const bool curved_hex_mixed = desc->precision == DGX_MIXED && desc->nsd == 3 && desc->xnodes &&
+                                desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p);
   if (desc->precision == DGX_MIXED && (desc->bc_function || desc->source))
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED: boundary and source functions take the curved path, which is DGX_FP64");
   if (!desc->xnodes && !all_quads_affine(desc) && desc->precision != DGX_FP64)
     return fail(DGX_ERR_ARGUMENT, "non-parallelogram quads take the curved path, which is DGX_FP64");
-  if (desc->xnodes && !(desc->precision == DGX_FP64 &&
+  if (desc->xnodes && !((desc->precision == DGX_FP64 || curved_hex_mixed) &&
                         ((desc->nsd == 3 && desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p)) ||
                          (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
                                              desc->eqntype == DGX_SHALLOW)))))
     return fail(DGX_ERR_ARGUMENT,
-                "curved elements (xnodes) are available for Euler3d (p <= 6), Euler, Navier_Stokes and Shallow, DGX_FP64");
+                "curved elements (xnodes) are available for Euler3d (p <= 6, DGX_FP64 or DGX_MIXED), Euler, "
+                "Navier_Stokes and Shallow (DGX_FP64)");
   const bool ns3 = desc->eqntype == DGX_NAVIER_STOKES3D;
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {

@@ -16,9 +16,26 @@
 //                 L x L inverses per element, built on the host.  On smoothly
 //                 curved elements the fit leaves M's spectrum within ~1e-5 of
 //                 the preconditioner's, so two or three iterations reach rounding.
+#include <type_traits>
+#include <vector>
+
 #include "dgx_kernels.hpp"
 
 namespace dgx {
+
+// Float copies of the basis tables for the mixed residual (FP32 contractions
+// and derivatives; FP64 metrics, fluxes, state and the mass solve), filled
+// beside c_tab at upload.  TB3<R, P> gives the tables of arithmetic type R.
+static __constant__ float c_tabf3[kTabTotal];
+template <typename R, int P> struct TB3;
+template <int P> struct TB3<double, P> : CT<P> {};
+template <int P> struct TB3<float, P> {
+  static constexpr int L = P + 1, Q = P + 2, QF = P + 1, b = tab_offset(P);
+  __device__ static __forceinline__ const float *BT() { return c_tabf3 + b; }
+  __device__ static __forceinline__ const float *BL() { return c_tabf3 + (CT<P>::BL() - c_tab); }
+  __device__ static __forceinline__ const float *GL() { return c_tabf3 + (CT<P>::GL() - c_tab); }
+  __device__ static __forceinline__ const float *DN() { return c_tabf3 + (CT<P>::DN() - c_tab); }
+};
 
 template <int P> struct C3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
@@ -38,9 +55,8 @@ template <int P> struct C3 {
 };
 
 // [5][N]^3 blocks (compact) through matrix M [NO][N] along all three dimensions
-template <int N, int NO, int NT>
-__device__ __forceinline__ void chain5(const double *M, const double *in, double *s1, double *s2, double *out,
-                                       int tid) {
+template <int N, int NO, int NT, typename T>
+__device__ __forceinline__ void chain5(const T *M, const T *in, T *s1, T *s2, T *out, int tid) {
   contract<5, N, N * N, 1, NO, NT>(M, in, s1, tid);            // [5][NO][N N]
   __syncthreads();
   contract<5 * NO, N, N, 1, NO, NT>(M, s1, s2, tid);           // [5 NO][NO][N]
@@ -96,7 +112,10 @@ __device__ __forceinline__ void precondition(const double *Pre, const double *in
 // Three workgroups fit a CU's LDS at p = 5; asking the compiler for three
 // waves per SIMD (168 VGPRs, a few spills) measured faster than leaving it
 // two (7.8e9 against 7.5e9 DOF-stage/s on 96^3 bump hexes at p = 5).
-template <int P, bool PROJ>
+// R: arithmetic of the residual before the mass solve (its element tensors,
+// contractions and derivatives): double, or float on the mixed path, whose
+// LDS views alias the same memory as the FP64 buffers of the mass solve.
+template <int P, bool PROJ, typename R = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_rhs3c(CurvedArgs a) {
   using S = C3<P>;
@@ -104,7 +123,10 @@ k_rhs3c(CurvedArgs a) {
   constexpr int NT = S::NT, NPT = S::NPT, NML = S::NML;
   extern __shared__ double sm[];
   double *Fw = sm, *S1 = Fw + 5 * Q3, *S2 = S1 + S::SZ, *G = S2 + S::SZ, *Pre = G + S::GZ, *red = Pre + 3 * S::L2;
-  double *gF = G, *Fc = S1, *Bv = G;
+  double *Bv = G;
+  // the residual's views (type R) of the same buffers
+  R *Fr = reinterpret_cast<R *>(Fw), *S1r = reinterpret_cast<R *>(S1), *S2r = reinterpret_cast<R *>(S2);
+  R *gF = reinterpret_cast<R *>(G), *Fc = S1r, *Bvr = reinterpret_cast<R *>(G);
   static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ && 3 * Q3 <= S::SZ && 5 * Q * L * L <= S::SZ, "scratch");
   static_assert(S::lds <= 160 * 1024, "LDS");
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
@@ -116,23 +138,23 @@ k_rhs3c(CurvedArgs a) {
   if (PROJ) {
     for (int i = tid; i < 5 * Q3; i += NT) {
       const int v = i / Q3, n = i - v * Q3;
-      Fw[i] = ge[9 * Q3 + n] * a.U[((size_t)v * ne + e) * Q3 + n];
+      Fr[i] = (R)(ge[9 * Q3 + n] * a.U[((size_t)v * ne + e) * Q3 + n]);
     }
     __syncthreads();
   } else {
   // modal state -> nodal (5 fields), kept in registers: node n = tid + r NT
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3;
-    S1[i] = a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
+    S1r[i] = (R)a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
   }
   __syncthreads();
-  chain5<L, Q, NT>(CT<P>::BT(), S1, S2, S1, Fw, tid);
+  chain5<L, Q, NT>(TB3<R, P>::BT(), S1r, S2r, S1r, Fr, tid);
   double un[NPT][5];
 #pragma unroll
   for (int rr = 0; rr < NPT; ++rr) {
     const int n = tid + rr * NT;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) un[rr][v] = n < Q3 ? Fw[v * Q3 + n] : 1.0;
+    for (int v = 0; v < 5; ++v) un[rr][v] = n < Q3 ? (double)Fr[v * Q3 + n] : 1.0;
   }
   __syncthreads();
 
@@ -164,7 +186,7 @@ k_rhs3c(CurvedArgs a) {
     }
     riemann3(UL, UR, nx, ny, nz, a.phys.gamma, a.phys.flux, FL, Fh, bad);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) gF[(v * 6 + s) * QF2 + q] = jr * (Fh[v] - FL[v]);
+    for (int v = 0; v < 5; ++v) gF[(v * 6 + s) * QF2 + q] = (R)(jr * (Fh[v] - FL[v]));
   }
   if (bad) atomicOr(a.flag, 1);
 
@@ -192,10 +214,10 @@ k_rhs3c(CurvedArgs a) {
       else if (v == 2) { fx = my * u; fy = my * vv + p; fz = my * w; }
       else if (v == 3) { fx = mz * u; fy = mz * vv; fz = mz * w + p; }
       else { const double h = E + p; fx = h * u; fy = h * vv; fz = h * w; }
-      Fc[n] = fx; Fc[Q3 + n] = fy; Fc[2 * Q3 + n] = fz;
+      Fc[n] = (R)fx; Fc[Q3 + n] = (R)fy; Fc[2 * Q3 + n] = (R)fz;
     }
     __syncthreads();
-    const double *Dn = CT<P>::DN();
+    const R *Dn = TB3<R, P>::DN();
 #pragma unroll
     for (int rr = 0; rr < NPT; ++rr) {
       const int n = tid + rr * NT;
@@ -204,27 +226,27 @@ k_rhs3c(CurvedArgs a) {
         double acc = 0.0;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-          const double *F = Fc + x * Q3;
-          double dr = 0.0, ds = 0.0, dt = 0.0;
+          const R *F = Fc + x * Q3;
+          R dr = 0, ds = 0, dt = 0;
 #pragma unroll
           for (int m = 0; m < Q; ++m) {
             dr = fma(Dn[i * Q + m], F[(m * Q + j) * Q + k], dr);
             ds = fma(Dn[j * Q + m], F[(i * Q + m) * Q + k], ds);
             dt = fma(Dn[k * Q + m], F[(i * Q + j) * Q + m], dt);
           }
-          acc = fma(mg[rr][x], dr, acc);
-          acc = fma(mg[rr][3 + x], ds, acc);
-          acc = fma(mg[rr][6 + x], dt, acc);
+          acc = fma(mg[rr][x], (double)dr, acc);
+          acc = fma(mg[rr][3 + x], (double)ds, acc);
+          acc = fma(mg[rr][6 + x], (double)dt, acc);
         }
-        Fw[v * Q3 + n] = acc;
+        Fr[v * Q3 + n] = (R)acc;
       }
     }
   }
   __syncthreads();
   // faces: Gauss -> GLL face nodes for 30 blocks, then wJ / w_end at the face nodes
-  contract<30, QF, QF, 1, Q, NT>(CT<P>::GL(), gF, S1, tid);   // [30][Q][QF]
+  contract<30, QF, QF, 1, Q, NT>(TB3<R, P>::GL(), gF, S1r, tid);   // [30][Q][QF]
   __syncthreads();
-  contract<30, Q, QF, 2, Q, NT>(CT<P>::GL(), S1, S2, tid);    // [30][Q][Q]
+  contract<30, Q, QF, 2, Q, NT>(TB3<R, P>::GL(), S1r, S2r, tid);    // [30][Q][Q]
   __syncthreads();
   {
     const double wend = CT<P>::BW()[0] / CT<P>::BT()[0];   // GLL end weight: phi_0 w_0 / phi_0
@@ -244,14 +266,14 @@ k_rhs3c(CurvedArgs a) {
         for (int f = 0; f < cnt; ++f)
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            Fw[v * Q3 + n] = fma(c, S2[((v * 6 + fs[f]) * Q + t1[f]) * Q + t2[f]], Fw[v * Q3 + n]);
+            Fr[v * Q3 + n] = fma((R)c, S2r[((v * 6 + fs[f]) * Q + t1[f]) * Q + t2[f]], Fr[v * Q3 + n]);
       }
     }
   }
   __syncthreads();
   }   // !PROJ
   // b = B^T (wJ f): into Bv (compact [5][L3], the face-term region, free now)
-  chain5<Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Bv, tid);
+  chain5<Q, L, NT>(TB3<R, P>::BL(), Fr, S1r, S2r, Bvr, tid);
 
   // preconditioned conjugate gradients on M x = b, M = B^T diag(wJ) B, five
   // fields at once, with the element's separable preconditioner P:
@@ -267,11 +289,19 @@ k_rhs3c(CurvedArgs a) {
     const int g = tid + k * NT;
     x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
     if (g < 5 * L3) {
-      bvec[k] = Bv[g];
+      bvec[k] = (double)Bvr[g];
       part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
     }
   }
   wg_sums<P, 5>(part, red, bb, tid);
+  if (!std::is_same<R, double>::value) {   // b to the FP64 view of the same buffer
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      if (g < 5 * L3) Bv[g] = bvec[k];
+    }
+    __syncthreads();
+  }
   precondition<L, NT>(Pre, Bv, S1, S2, Pv, tid);   // x0 = P b, into Pv
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
@@ -399,9 +429,13 @@ k_rhs3c(CurvedArgs a) {
 template <int P>
 static hipError_t launch_c3(const CurvedArgs &a) {
   const int sh = C3<P>::lds;
-  if (a.project) {
+  if (a.project) {   // the projection of given nodal data stays FP64 on both paths
     (void)hipFuncSetAttribute((const void *)k_rhs3c<P, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
     hipLaunchKernelGGL((k_rhs3c<P, true>), dim3(a.ne), dim3(C3<P>::NT), sh, a.stream, a);
+  } else if (a.mixed) {
+    (void)hipFuncSetAttribute((const void *)k_rhs3c<P, false, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              sh);
+    hipLaunchKernelGGL((k_rhs3c<P, false, float>), dim3(a.ne), dim3(C3<P>::NT), sh, a.stream, a);
   } else {
     (void)hipFuncSetAttribute((const void *)k_rhs3c<P, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sh);
     hipLaunchKernelGGL((k_rhs3c<P, false>), dim3(a.ne), dim3(C3<P>::NT), sh, a.stream, a);
@@ -421,7 +455,10 @@ hipError_t launch_curved3(const CurvedArgs &a) {
 }
 
 hipError_t upload_tables_c3d(const double *h) {
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), h, sizeof(double) * kTabTotal);
+  if (e != hipSuccess) return e;
+  std::vector<float> f(h, h + kTabTotal);
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tabf3), f.data(), sizeof(float) * kTabTotal);
 }
 
 }  // namespace dgx

@@ -147,6 +147,7 @@ struct CurvedArgs {
   // hexes: one LSRK stage in place instead of R (update): Ut = ca Ut - dt R,
   // Us += cb Ut, with Us the modal state U points to
   bool update = false;
+  bool mixed = false;                        // hexes: FP32 residual arithmetic before the FP64 mass solve
   double *Us = nullptr, *Ut = nullptr;
   double ca = 0.0, cb = 0.0, dt = 0.0;
   hipStream_t stream = nullptr;

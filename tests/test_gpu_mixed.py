@@ -153,3 +153,40 @@ def test_mixed_residual_with_walls():
     rel = _rel(out[dgx.DGX_MIXED], out[dgx.DGX_FP64])
     print(f"[mixed] walls: residual rel L2 per field: {rel}", flush=True)
     assert np.all(rel < 1e-5), rel
+
+
+@pytest.mark.parametrize("p", [3, 5])
+def test_mixed_curved_hexes_against_fp64(p):
+    """DGX_MIXED on curved hexes (the bump channel: slip wall under the bump,
+    imposed state on top): the residual's element tensors, contractions and
+    derivatives in FP32, the metric terms, Riemann fluxes, mass solve, state
+    and registers FP64.  Stated errors against the FP64 path (measured
+    maxima in brackets): one residual, relative L2 per field, <= 1e-5
+    [2.9e-6]; the state after 50 LSRK14 steps against the norm of the whole
+    state <= 5e-6 [1.2e-6], per field <= 1e-4 [2.5e-5] (the momenta that are
+    only the flow's deflection by the bump have small norms)."""
+    from test_gpu_parity import _channel_state
+    lo, hi = (0.0, 0.0, 0.0), (2.0, 2.0, 1.0)
+    m = meshgen.channel_hexes(4, 3, 3, lo, hi, bc_bottom=0, bc_top=1)
+    xn = meshgen.mapped_nodes(m, p, meshgen.bump_map(lo, hi, 0.1))
+    bc_state = [1.0, 0.3, 0.0, 0.0, 1.0 / 0.4 + 0.045]
+    U = _channel_state(xn, 1.0)
+    g64 = dgx.GPUDomain(m, p, dgx.DGX_EULER3D, bc_state=bc_state, xnodes=xn)
+    g32 = dgx.GPUDomain(m, p, dgx.DGX_EULER3D, bc_state=bc_state, xnodes=xn, precision=dgx.DGX_MIXED)
+    g64.upload(U)
+    g32.upload(U)
+    assert np.array_equal(g64.download(dgx.DGX_MODAL), g32.download(dgx.DGX_MODAL))
+    F64, F32 = g64.rhs(dgx.DGX_MODAL), g32.rhs(dgx.DGX_MODAL)
+    e_rhs = np.array([np.linalg.norm(F32[v] - F64[v]) / np.linalg.norm(F64[v]) for v in range(5)])
+    print(f"[mixed] curved hexes p={p}: residual rel L2 per field {e_rhs}", flush=True)
+    dt = 0.3 * (1.0 / 3) / ((p + 1) ** 2 * 1.3)
+    g64.advance(dgx.DGX_LSRK14, dt, 50)
+    g32.advance(dgx.DGX_LSRK14, dt, 50)
+    U64, U32 = g64.download(dgx.DGX_MODAL), g32.download(dgx.DGX_MODAL)
+    e_f = np.array([np.linalg.norm(U32[v] - U64[v]) / np.linalg.norm(U64[v]) for v in range(5)])
+    e_w = np.array([np.linalg.norm(U32[v] - U64[v]) for v in range(5)]) / np.linalg.norm(U64)
+    print(f"[mixed] curved hexes p={p}: 50 steps rel L2 per field {e_f}, against the whole state {e_w}", flush=True)
+    assert np.all(e_rhs <= 1e-5), e_rhs
+    assert np.all(e_w <= 5e-6) and np.all(e_f <= 1e-4), (e_f, e_w)
+    g64.close()
+    g32.close()
