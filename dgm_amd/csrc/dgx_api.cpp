@@ -1443,7 +1443,7 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
                          (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
                                              desc->eqntype == DGX_SHALLOW)))))
     return fail(DGX_ERR_ARGUMENT,
-                "curved elements (xnodes) are available for Euler3d (p <= 6, DGX_FP64 or DGX_MIXED), Euler, "
+                "curved elements (xnodes) are available for Euler3d (p <= 8, DGX_FP64 or DGX_MIXED), Euler, "
                 "Navier_Stokes and Shallow (DGX_FP64)");
   const bool ns3 = desc->eqntype == DGX_NAVIER_STOKES3D;
   for (int r = 0; r < desc->ncon; ++r)

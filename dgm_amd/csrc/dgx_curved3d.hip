@@ -1,4 +1,4 @@
-// Curved hexes (Euler3d, p <= 6): the residual with metrics at the nodes and
+// Curved hexes (Euler3d, p <= 8): the residual with metrics at the nodes and
 // the projection through the element mass matrix.
 //
 //   volume term   Euler3d::convection with Hex::gradient on curved geometry:
@@ -40,7 +40,9 @@ template <int P> struct TB3<float, P> {
 template <int P> struct C3 {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L2 * L, Q2 = Q * Q, Q3 = Q2 * Q, QF2 = QF * QF;
-  static constexpr int NT = 256, NW = NT / 64;
+  // 256 threads up to p = 6 (three workgroups fit a CU's LDS at p = 5); one
+  // 512-thread workgroup per CU at p = 7, 8, where the LDS holds only one
+  static constexpr int NT = P >= 7 ? 512 : 256, NW = NT / 64;
   static constexpr int NPT = (Q3 + NT - 1) / NT;          // nodes per thread
   static constexpr int NML = (5 * L3 + NT - 1) / NT;      // modal entries per thread (5 fields)
   // LDS (doubles): Fw [5][Q3] (nodal residual, then the CG direction); scratch
@@ -111,12 +113,13 @@ __device__ __forceinline__ void precondition(const double *Pre, const double *in
 // transform of a curved element: x = M^-1 B^T (wJ u)).
 // Three workgroups fit a CU's LDS at p = 5; asking the compiler for three
 // waves per SIMD (168 VGPRs, a few spills) measured faster than leaving it
-// two (7.8e9 against 7.5e9 DOF-stage/s on 96^3 bump hexes at p = 5).
+// two (7.8e9 against 7.5e9 DOF-stage/s on 96^3 bump hexes at p = 5).  At
+// p = 7, 8 one 512-thread workgroup per CU gives two waves per SIMD.
 // R: arithmetic of the residual before the mass solve (its element tensors,
 // contractions and derivatives): double, or float on the mixed path, whose
 // LDS views alias the same memory as the FP64 buffers of the mass solve.
 template <int P, bool PROJ, typename R = double>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(C3<P>::NT) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 2 : 3)))
 k_rhs3c(CurvedArgs a) {
   using S = C3<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
@@ -443,12 +446,12 @@ static hipError_t launch_c3(const CurvedArgs &a) {
   return hipGetLastError();
 }
 
-bool curved3_supported(int p) { return p >= 1 && p <= 6; }
+bool curved3_supported(int p) { return p >= 1 && p <= 8; }
 
 hipError_t launch_curved3(const CurvedArgs &a) {
   switch (a.p) {
 #define DGX_CASE(PP) case PP: return launch_c3<PP>(a);
-    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5) DGX_CASE(6)
+    DGX_CASE(1) DGX_CASE(2) DGX_CASE(3) DGX_CASE(4) DGX_CASE(5) DGX_CASE(6) DGX_CASE(7) DGX_CASE(8)
 #undef DGX_CASE
     default: return hipErrorInvalidValue;
   }

@@ -102,7 +102,7 @@ hipError_t launch_stage3(const StageLaunch &s);
 bool mixed3_supported(int p);
 hipError_t launch_mixed3(const StageLaunch &s);
 
-// Curved hexes, Euler3d, p <= 6 (dgx_curved3d.hip): residual R = -dU/dt with
+// Curved hexes, Euler3d, p <= 8 (dgx_curved3d.hip): residual R = -dU/dt with
 // metrics at the nodes and the element mass matrix.
 // Boundary and source functions of (x, y, t) as postfix programs (dgx_function):
 // bc[c] .. bc[c+1] and src[c] .. src[c+1] bound component c's instructions in code.

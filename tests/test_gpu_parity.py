@@ -202,7 +202,7 @@ def test_advance_euler3d_channel_lsrk14():
     assert np.all(err <= TOL_STEP), err
 
 
-@pytest.mark.parametrize("p,flux", [(3, 0), (5, 1), (6, 0)])
+@pytest.mark.parametrize("p,flux", [(3, 0), (5, 1), (6, 0), (7, 1), (8, 0)])
 def test_rhs_euler3d_curved_bump_channel(p, flux):
     """Curved hexes: a channel with a bump on its bottom wall (slip wall) and
     an imposed state on top; metrics at the nodes, mass-matrix projection."""

@@ -79,8 +79,8 @@ def _desc(m, p, eqn, **kw):
     (dict(p=3, eqn=9, bulk=1, precision=1), b"bulk_forcing is available"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
     (dict(p=3, eqn=9, curved=True), b"curved elements"),
-    (dict(p=7, eqn=8, curved=True), b"curved elements"),
-    (dict(p=7, eqn=8, precision=1, curved=True), b"curved elements"),
+    (dict(p=9, eqn=8, curved=True), b"curved elements"),
+    (dict(p=9, eqn=8, precision=1, curved=True), b"DGX_MIXED"),
 ])
 def test_create_rejects_bad_arguments(lib, case, msg):
     m = meshgen.periodic_hexes(2, 2, 2)
