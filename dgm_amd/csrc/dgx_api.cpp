@@ -434,20 +434,22 @@ int residual_core(dgx_domain *d, const double *U, double *R) {
 }
 
 // One LSRK stage of an affine 2D domain (not curved, one order): the traces
-// (and the viscous terms), then the residual fused with the register update
-// Ut = A Ut - dt R, U += B Ut (k_step2), bitwise the same as the residual
-// followed by the axpby and axpy.
-int step2(dgx_domain *d, double A, double B, double dt) {
+// (unless the previous stage of this call wrote them) and the viscous terms,
+// then the residual fused with the register update Ut = A Ut - dt R,
+// U += B Ut and with the traces of the new state (k_step2), bitwise the same
+// as the residual followed by the axpby and axpy.
+int step2(dgx_domain *d, double A, double B, double dt, bool traces_ready) {
   int rc;
-  if ((rc = launch(d, dgx::OP_TRACES, d->U, nullptr))) return rc;
+  if (!traces_ready && (rc = launch(d, dgx::OP_TRACES, d->U, nullptr))) return rc;
   if (d->visc) {
     if ((rc = launch(d, dgx::OP_GRAD, d->U, nullptr))) return rc;
     if ((rc = launch(d, dgx::OP_VISC, d->U, nullptr))) return rc;
   }
   dgx::LaunchArgs a = launch_args(d, dgx::OP_STEP, d->U, nullptr);
-  a.Us = d->U; a.Ut = d->Ut; a.ca = A; a.cb = B; a.dt = dt;
+  a.Us = d->U; a.Ut = d->Ut; a.ca = A; a.cb = B; a.dt = dt; a.Tn = d->T2;
   hipError_t e = timed(d, dgx::OP_RHS, [&] { return dgx::launch_2d(a); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("stage launch: ") + hipGetErrorString(e));
+  std::swap(d->T, d->T2);        // the next stage's traces
   if (d->bal == 1) d->bal = 2;   // the first residual is stored
   ++d->stages;
   return DGX_OK;
@@ -1678,7 +1680,7 @@ int run_stage(dgx_domain *h, int inttype, int st, double dt, bool traces_ready) 
       std::swap(h->T, h->T2);
       ++h->stages;
     } else if (h->nsd == 2 && !h->curved && !h->varp && !h->bulk) {
-      RUN(step2(h, A, B, dt));
+      RUN(step2(h, A, B, dt, traces_ready));
     } else if (h->nsd == 3 && h->curved && !h->visc && !h->bulk) {
       RUN(step_curved3(h, A, B, dt));
     } else {

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -87,18 +88,20 @@ int traces(dgx_euler_adjoint *a, const double *U) {
   return DGX_OK;
 }
 
-// one LSRK14 stage of the domain's state: the traces, then the residual fused
-// with the register update Ut = A Ut - dt R, U += B Ut (k_step2)
+// one LSRK14 stage of the domain's state: the residual fused with the
+// register update Ut = A Ut - dt R, U += B Ut and with the traces of the new
+// state for the next stage (k_step2); stage 0 evaluates its own traces
 int stage(dgx_euler_adjoint *a, int s) {
   dgx_domain *d = a->dom;
-  int rc = traces(a, d->U);
-  if (rc) return rc;
+  int rc;
+  if (s == 0 && (rc = traces(a, d->U))) return rc;
   dgx::LaunchArgs la;
   la.op = dgx::OP_STEP; la.p = a->p; la.ne = a->ne; la.nfield = 4; la.eqn = DGX_EULER;
-  la.U = d->U; la.Us = d->U; la.Ut = d->Ut; la.T = d->T; la.flag = d->d_flag;
+  la.U = d->U; la.Us = d->U; la.Ut = d->Ut; la.T = d->T; la.Tn = d->T2; la.flag = d->d_flag;
   la.ca = dgx::kLsrkA[s]; la.cb = dgx::kLsrkB[s]; la.dt = a->dt;
   la.tab = d->tab; la.geo = d->geo; la.phys = d->phys; la.stream = d->stream;
   HIPCHK(dgx::launch_2d(la));
+  std::swap(d->T, d->T2);
   return DGX_OK;
 }
 

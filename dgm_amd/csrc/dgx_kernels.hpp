@@ -30,6 +30,7 @@ struct LaunchArgs {
   int bal = 0;                   // 0 no balance term, 1 store the residual in Bal, 2 subtract Bal
   double *Us = nullptr, *Ut = nullptr;   // OP_STEP: the state (updated in place) and the LSRK register
   double ca = 0.0, cb = 0.0, dt = 0.0;   // OP_STEP: LSRK coefficients and time step
+  double *Tn = nullptr;                  // OP_STEP: if set, the face traces of the new state go here
   bool mixed = false;            // quads, inviscid: FP32 LDS tensors and contractions (traces, residual, step)
   Tab tab;
   Geo geo;
@@ -53,12 +54,14 @@ struct Batch2 {
 constexpr int kBatchGrid = 8192;
 
 struct BatchLaunch {
-  int op = 0;                    // 0: face traces, 1: residual + LSRK14 register update,
+  int op = 0;                    // 0: face traces, 1: residual + LSRK14 register update (+ next traces),
                                  // 2: a persistent slice of nsteps LSRK14 steps (cooperative launch)
   int p = 0, nreq = 0, nwork = 0;
   bool shallow = false;          // shallow water (3 fields) or Euler (4 fields)
   const Batch2 *b = nullptr;     // device array of nreq members
   double A = 0.0, B = 0.0;       // LSRK14 coefficients of the stage
+  int odd = 0, next = 0;         // op 1: read the traces of T2 and write into T (odd) or the reverse;
+                                 // next: write the next stage's traces at all
   int nsteps = 0;                // op 2: steps of the slice
   double lsrkA[14] = {}, lsrkB[14] = {};   // op 2: the LSRK14 coefficients of every stage
   int grid = 0;                  // op 2, out: workgroups launched

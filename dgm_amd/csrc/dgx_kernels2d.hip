@@ -52,11 +52,16 @@ __device__ __forceinline__ void backward2(const Tab &tb, const double *m, double
 
 // Gauss-point traces of the NF fields of element e on its four edges, all
 // fields per contraction (one load phase, four contraction phases).
-template <int P, int NF, typename R = double>
-__device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne, int e) {
+// LDS of the traces: A [NF][L2], Kb, Ka [NF][2][L], O [NF][2][QF], carved from `sm`.
+template <int P, int NF>
+constexpr int traces2_lds() { return NF * (P + 1) * (P + 1) + 4 * NF * (P + 1) + 2 * NF * (P + 1); }
+
+template <int P, int NF, typename R>
+__device__ __forceinline__ void traces2_core(const double *__restrict__ U, double *__restrict__ T, int ne, int e,
+                                             R *sm) {
   using S = H2<P>;
   constexpr int L = S::L, QF = S::QF, L2 = S::L2, NT = S::NT;
-  __shared__ R A[NF * L2], Kb[NF * 2 * L], Ka[NF * 2 * L], O[NF * 2 * QF];
+  R *A = sm, *Kb = A + NF * L2, *Ka = Kb + NF * 2 * L, *O = Ka + NF * 2 * L;
   const int tid = threadIdx.x;
   double *Te = T + (size_t)e * 4 * NF * QF;
   for (int i = tid; i < NF * L2; i += NT) {
@@ -81,6 +86,12 @@ __device__ __forceinline__ void traces2_elem(const double *__restrict__ U, doubl
     Te[((end ? 1 : 3) * NF + v) * QF + b] = O[i];
   }
   __syncthreads();
+}
+
+template <int P, int NF, typename R = double>
+__device__ __forceinline__ void traces2_elem(const double *__restrict__ U, double *__restrict__ T, int ne, int e) {
+  __shared__ R sm[traces2_lds<P, NF>()];
+  traces2_core<P, NF, R>(U, T, ne, e, sm);
 }
 
 template <int P, typename R = double>
@@ -247,15 +258,20 @@ k_rhs2(const double *__restrict__ U, const double *__restrict__ T, const double 
 // update Ut = A Ut - dt R, U += B Ut in its store, bitwise the same as the
 // residual followed by the axpby and axpy.  An element reads its own modes
 // only (its neighbours' through the traces), before its own store.
+// With Tn, the element then writes the face traces of its new state into Tn
+// (the next stage's traces, in the residual's LDS): it reads back its own
+// modes, which its workgroup has just stored.
 template <int P, bool NS, typename R = double>
 __global__ void __launch_bounds__(64)
 k_step2(double *U, const double *__restrict__ T, const double *__restrict__ V, const double *__restrict__ TV,
-        double *__restrict__ Ut, Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
+        double *__restrict__ Ut, Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt,
+        double *__restrict__ Tn) {
   rhs2_elem<P, NS, R>(U, T, V, TV, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
     const double ut = fma(mdt, r, A * Ut[o]);
     Ut[o] = ut;
     U[o] = fma(B, ut, U[o]);
   });
+  if (Tn) traces2_core<P, 4, R>(U, Tn, ne, blockIdx.x, rhs2_smem<P, 4, R>());
 }
 
 // Shallow water (Shallow::time_derivative, src/Shallow.cpp:255-336): flux
@@ -347,12 +363,14 @@ k_shallow2(const double *__restrict__ U, const double *__restrict__ T, double *_
 template <int P, typename R = double>
 __global__ void __launch_bounds__(64)
 k_shallow_step2(double *U, const double *__restrict__ T, double *__restrict__ Ut, double *__restrict__ Bal, int bal,
-                Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt) {
+                Geo geo, Phys ph, int ne, int *__restrict__ flag, double A, double B, double mdt,
+                double *__restrict__ Tn) {
   shallow2_elem<P, R>(U, T, Bal, bal, geo, ph, ne, blockIdx.x, flag, [=](size_t o, double r) {
     const double ut = fma(mdt, r, A * Ut[o]);
     Ut[o] = ut;
     U[o] = fma(B, ut, U[o]);
   });
+  if (Tn) traces2_core<P, 3, R>(U, Tn, ne, blockIdx.x, rhs2_smem<P, 3, R>());
 }
 
 // lifted, projected gradients of (u, v, e): G[e][2*prim + dir][L2]
@@ -696,12 +714,17 @@ k_batch_traces2(const Batch2 *__restrict__ b, int nreq, int nwork) {
   }
 }
 
+// odd: the stage reads the traces of T2 and writes the next ones into T
+// (else T and T2); next: write the next stage's traces at all.
 template <int P, bool SHALLOW, typename R = double>
 __global__ void __launch_bounds__(64)
-k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, double B) {
+k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, double B, int odd, int next) {
+  constexpr int NF = SHALLOW ? 3 : 4;
   int r = 0;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     while (r + 1 < nreq && b[r + 1].off <= w) ++r;
+    const double *Tc = odd ? b[r].T2 : b[r].T;
+    double *Tn = odd ? b[r].T : b[r].T2;
     double *U = b[r].U, *Ut = b[r].Ut;
     const double mdt = -b[r].dt;
     auto update = [=](size_t o, double rr) {
@@ -712,9 +735,10 @@ k_batch_step2(const Batch2 *__restrict__ b, int nreq, int nwork, double A, doubl
     const Geo geo = b[r].geo;
     const Phys ph = b[r].phys;
     if constexpr (SHALLOW)
-      shallow2_elem<P, R>(U, b[r].T, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+      shallow2_elem<P, R>(U, Tc, b[r].Bal, b[r].bal, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
     else
-      rhs2_elem<P, false, R>(U, b[r].T, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+      rhs2_elem<P, false, R>(U, Tc, nullptr, nullptr, geo, ph, b[r].ne, w - b[r].off, b[r].flag, update);
+    if (next) traces2_core<P, NF, R>(U, Tn, b[r].ne, w - b[r].off, rhs2_smem<P, NF, R>());
   }
 }
 
@@ -742,7 +766,7 @@ k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, 
     int r = 0;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
       while (r + 1 < nreq && b[r + 1].off <= w) ++r;
-      traces2_elem<P, NF, R>(b[r].U, b[r].T, b[r].ne, w - b[r].off);
+      traces2_core<P, NF, R>(b[r].U, b[r].T, b[r].ne, w - b[r].off, rhs2_smem<P, NF, R>());
     }
   }
   grid.sync();
@@ -769,7 +793,7 @@ k_batch_persist2(const Batch2 *__restrict__ b, int nreq, int nwork, int nsteps, 
           shallow2_elem<P, R>(U, Tc, b[r].Bal, b[r].bal, geo, ph, ne, e, b[r].flag, update);
         else
           rhs2_elem<P, false, R>(U, Tc, nullptr, nullptr, geo, ph, ne, e, b[r].flag, update);
-        if (!last) traces2_elem<P, NF, R>(U, Tn, ne, e);   // the element's new state, read back by its own workgroup
+        if (!last) traces2_core<P, NF, R>(U, Tn, ne, e, rhs2_smem<P, NF, R>());   // its new state, read back
       }
       grid.sync();
     }
@@ -812,10 +836,10 @@ static hipError_t launch_batch2_pr(BatchLaunch &a) {
   } else {
     if (a.shallow)
       hipLaunchKernelGGL((k_batch_step2<P, true, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
-                         a.A, a.B);
+                         a.A, a.B, a.odd, a.next);
     else
       hipLaunchKernelGGL((k_batch_step2<P, false, R>), dim3(grid), dim3(64), 0, a.stream, a.b, a.nreq, a.nwork,
-                         a.A, a.B);
+                         a.A, a.B, a.odd, a.next);
   }
   return hipGetLastError();
 }
@@ -903,19 +927,19 @@ static hipError_t launch2(const LaunchArgs &a) {
     case OP_STEP:
       if (a.mixed && a.eqn == 30)
         hipLaunchKernelGGL((k_shallow_step2<P, float>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.Ut, a.Bal,
-                           a.bal, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+                           a.bal, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt, a.Tn);
       else if (a.mixed)
         hipLaunchKernelGGL((k_step2<P, false, float>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV,
-                           a.Ut, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+                           a.Ut, a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt, a.Tn);
       else if (a.eqn == 30)
         hipLaunchKernelGGL(k_shallow_step2<P>, dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.Ut, a.Bal, a.bal,
-                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt, a.Tn);
       else if (a.ns)
         hipLaunchKernelGGL((k_step2<P, true>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV, a.Ut,
-                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt, a.Tn);
       else
         hipLaunchKernelGGL((k_step2<P, false>), dim3(ne), dim3(64), 0, a.stream, a.Us, a.T, a.V, a.TV, a.Ut,
-                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt);
+                           a.geo, a.phys, ne, a.flag, a.ca, a.cb, -a.dt, a.Tn);
       break;
     case OP_GRAD:
       hipLaunchKernelGGL(k_grad2<P>, dim3(ne), dim3(64), 0, a.stream, a.U, a.T, a.G, a.tab, a.geo, ne);

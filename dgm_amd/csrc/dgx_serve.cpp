@@ -225,7 +225,7 @@ int enqueue_group(dgx_server *s, Group *g) {
     for (size_t i = 0; i < g->members.size(); ++i) {
       const dgx_domain *d = g->members[i]->h;
       dgx::Batch2 &b = g->hd[i];
-      if (s->cfg.persistent && !d->T2) return sfail(DGX_ERR_ARGUMENT, "persistent slice: a member has no second trace buffer");
+      if (!d->T2) return sfail(DGX_ERR_ARGUMENT, "batched slice: a member has no second trace buffer");
       b.U = d->U; b.Ut = d->Ut; b.T = d->T; b.T2 = d->T2; b.Bal = d->Bal; b.flag = d->d_flag;
       b.geo = d->geo; b.phys = d->phys; b.ne = d->ne; b.off = off; b.bal = d->bal;
       b.dt = g->members[i]->dt;
@@ -257,10 +257,17 @@ int enqueue_group(dgx_server *s, Group *g) {
   } else
   for (int step = 0; step < k; ++step)
     for (int st = 0; st < 14; ++st) {
-      a.op = 0;
-      hipError_t e = dgx::launch_batch2(a);
+      // the slice's first stage evaluates the traces; every stage then writes
+      // the next stage's into the other buffer (T, T2 alternate; after the
+      // even number of stages of a step the current ones are in T again)
+      hipError_t e = hipSuccess;
+      if (step == 0 && st == 0) {
+        a.op = 0;
+        e = dgx::launch_batch2(a);
+      }
       if (e == hipSuccess) {
         a.op = 1; a.A = dgx::kLsrkA[st]; a.B = dgx::kLsrkB[st];
+        a.odd = st & 1; a.next = !(step == k - 1 && st == 13);
         e = dgx::launch_batch2(a);
       }
       if (e != hipSuccess) return sfail(DGX_ERR_DEVICE, std::string("batched launch: ") + hipGetErrorString(e));
