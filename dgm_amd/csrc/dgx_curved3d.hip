@@ -50,10 +50,12 @@ template <int P> struct C3 {
   // face blocks at the GLL points; S1 also holds the 3 flux components of the
   // volume term); G [GZ] (the face terms gF [5][6][QF2], then b, M p and the
   // preconditioned residual [5][L3]); the element's preconditioner [3][L][L];
-  // reductions [NW][16].  The nodal state lives in registers.
+  // reductions [NW][16] and the mass solve's per-field totals [32] (the same
+  // in every thread, so read from LDS instead of held in registers).  The
+  // nodal state lives in registers.
   static constexpr int SZ = (5 * Q3 > 30 * Q2) ? 5 * Q3 : 30 * Q2;
   static constexpr int GZ = (30 * QF2 > 5 * L3) ? 30 * QF2 : 5 * L3;
-  static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 3 * L2 + 16 * NW) * 8;
+  static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 3 * L2 + 16 * NW + 32) * 8;
 };
 
 // [5][N]^3 blocks (compact) through matrix M [NO][N] along all three dimensions
@@ -67,33 +69,47 @@ __device__ __forceinline__ void chain5(const T *M, const T *in, T *s1, T *s2, T 
   __syncthreads();
 }
 
-// N per-thread partial sums, reduced over the workgroup (every thread gets
-// the totals; red holds [NW][16])
+// N per-thread partial sums, reduced over the workgroup into dst[0 .. N) in
+// LDS: each wave's shuffle sum, then the waves' sums in ascending order
 template <int P, int N>
-__device__ __forceinline__ void wg_sums(const double (&part)[N], double *red, double (&out)[N], int tid) {
+__device__ __forceinline__ void wg_sums_lds(const double (&part)[N], double *red, double *dst, int tid) {
   using S = C3<P>;
   static_assert(N <= 16, "reduction slots");
-  double v[N];
 #pragma unroll
   for (int f = 0; f < N; ++f) {
     double t = part[f];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    v[f] = t;
-  }
-  const int w = tid >> 6;
-  if ((tid & 63) == 0) {
-#pragma unroll
-    for (int f = 0; f < N; ++f) red[w * 16 + f] = v[f];
+    if ((tid & 63) == 0) red[(tid >> 6) * 16 + f] = t;
   }
   __syncthreads();
-#pragma unroll
-  for (int f = 0; f < N; ++f) {
+  if (tid < N) {
     double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < S::NW; ++k) t += red[k * 16 + f];
-    out[f] = t;
+    for (int k = 0; k < S::NW; ++k) t += red[k * 16 + tid];
+    dst[tid] = t;
   }
+  __syncthreads();
+}
+
+// AP = M Pv with M = B^T diag(wJ) B, on [5][L3] (S1, S2 scratch)
+template <int P, int NT>
+__device__ __forceinline__ void mass_apply(const double *Pv, double *AP, double *S1, double *S2, const double *ge,
+                                           int tid) {
+  constexpr int L = P + 1, Q = P + 2, Q3 = Q * Q * Q;
+  contract<5, L, L * L, 1, Q, NT>(CT<P>::BT(), Pv, S1, tid);        // [5][Q][L L]
+  __syncthreads();
+  contract<5 * Q, L, L, 1, Q, NT>(CT<P>::BT(), S1, S2, tid);        // [5 Q][Q][L]
+  __syncthreads();
+  contract<5 * Q * Q, L, 1, 1, Q, NT>(CT<P>::BT(), S2, S1, tid);    // nodal [5][Q3]
+  __syncthreads();
+  for (int i = tid; i < 5 * Q3; i += NT) S1[i] *= ge[9 * Q3 + (i % Q3)];
+  __syncthreads();
+  contract<5, Q, Q * Q, 1, L, NT>(CT<P>::BL(), S1, S2, tid);        // [5][L][Q Q]
+  __syncthreads();
+  contract<5 * L, Q, Q, 1, L, NT>(CT<P>::BL(), S2, S1, tid);        // [5 L][L][Q]
+  __syncthreads();
+  contract<5 * L * L, Q, 1, 1, L, NT>(CT<P>::BL(), S1, AP, tid);    // [5][L3]
   __syncthreads();
 }
 
@@ -145,23 +161,9 @@ k_rhs3c(CurvedArgs a) {
     }
     __syncthreads();
   } else {
-  // modal state -> nodal (5 fields), kept in registers: node n = tid + r NT
-  for (int i = tid; i < 5 * L3; i += NT) {
-    const int v = i / L3;
-    S1r[i] = (R)a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
-  }
-  __syncthreads();
-  chain5<L, Q, NT>(TB3<R, P>::BT(), S1r, S2r, S1r, Fr, tid);
-  double un[NPT][5];
-#pragma unroll
-  for (int rr = 0; rr < NPT; ++rr) {
-    const int n = tid + rr * NT;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) un[rr][v] = n < Q3 ? (double)Fr[v * Q3 + n] : 1.0;
-  }
-  __syncthreads();
-
-  // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points
+  // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points,
+  // before the nodal state is held in registers (the Riemann solver's
+  // temporaries and the state would not fit beside each other)
   bool bad = false;
   for (int i = tid; i < 6 * QF2; i += NT) {
     const int s = i / QF2, q = i - s * QF2;
@@ -193,6 +195,17 @@ k_rhs3c(CurvedArgs a) {
   }
   if (bad) atomicOr(a.flag, 1);
 
+
+  // modal state -> nodal (5 fields), held in S2 through the volume term
+  // (S2 is free until the face lift; registers are not)
+  for (int i = tid; i < 5 * L3; i += NT) {
+    const int v = i / L3;
+    S1r[i] = (R)a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
+  }
+  __syncthreads();
+  chain5<L, Q, NT>(TB3<R, P>::BT(), S1r, Fr, S1r, S2r, tid);
+  const R *Un = S2r;
+
   // volume: wJ * sum_x sum_d (dxi_d/dx_x) D_d F_x at every node, field by
   // field; each thread's nodes keep their 9 metric terms in registers
   double mg[NPT][9];
@@ -208,7 +221,7 @@ k_rhs3c(CurvedArgs a) {
     for (int rr = 0; rr < NPT; ++rr) {
       const int n = tid + rr * NT;
       if (n >= Q3) continue;
-      const double r = un[rr][0], mx = un[rr][1], my = un[rr][2], mz = un[rr][3], E = un[rr][4];
+      const double r = Un[n], mx = Un[Q3 + n], my = Un[2 * Q3 + n], mz = Un[3 * Q3 + n], E = Un[4 * Q3 + n];
       const double ir = 1.0 / r, u = mx * ir, vv = my * ir, w = mz * ir;
       const double p = gm1 * (E - 0.5 * (mx * u + my * vv + mz * w));
       double fx, fy, fz;
@@ -281,56 +294,56 @@ k_rhs3c(CurvedArgs a) {
   // preconditioned conjugate gradients on M x = b, M = B^T diag(wJ) B, five
   // fields at once, with the element's separable preconditioner P:
   // x0 = P b, r0 = b - M x0, z = P r, p = z.  Pv (compact [5][L3]) reuses
-  // Fw; AP = M p, then r and z = P r, reuse Bv.
-  double *Pv = Fw, *AP = Bv;
-  double x[NML], res[NML], bvec[NML];
-  double part[5], bb[5], pap[5], part2[10], rz2[10];
+  // Fw; AP = M p, then r and z = P r, reuse Bv.  Per-field totals in LDS,
+  // Tot: |b|^2 [0, 5), r.r [5, 10), r.z [10, 15), p.Ap [15, 20), the previous
+  // r.z [20, 25).
+  double *Pv = Fw, *AP = Bv, *Tot = red + 16 * S::NW;
+  double x[NML], res[NML];
+  {
+    double bvec[NML], part[5];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) part[f] = 0.0;
-#pragma unroll
-  for (int k = 0; k < NML; ++k) {
-    const int g = tid + k * NT;
-    x[k] = 0.0; res[k] = 0.0; bvec[k] = 0.0;
-    if (g < 5 * L3) {
-      bvec[k] = (double)Bvr[g];
-      part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
-    }
-  }
-  wg_sums<P, 5>(part, red, bb, tid);
-  if (!std::is_same<R, double>::value) {   // b to the FP64 view of the same buffer
+    for (int f = 0; f < 5; ++f) part[f] = 0.0;
 #pragma unroll
     for (int k = 0; k < NML; ++k) {
       const int g = tid + k * NT;
-      if (g < 5 * L3) Bv[g] = bvec[k];
+      bvec[k] = 0.0;
+      if (g < 5 * L3) {
+        bvec[k] = (double)Bvr[g];
+        part[g / L3] = fma(bvec[k], bvec[k], part[g / L3]);
+      }
+    }
+    wg_sums_lds<P, 5>(part, red, Tot, tid);
+    if (!std::is_same<R, double>::value) {   // b to the FP64 view of the same buffer
+#pragma unroll
+      for (int k = 0; k < NML; ++k) {
+        const int g = tid + k * NT;
+        if (g < 5 * L3) Bv[g] = bvec[k];
+      }
+      __syncthreads();
+    }
+    precondition<L, NT>(Pre, Bv, S1, S2, Pv, tid);   // x0 = P b, into Pv
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      x[k] = g < 5 * L3 ? Pv[g] : 0.0;
+    }
+    // AP = M x0, r0 = b - AP
+    mass_apply<P, NT>(Pv, AP, S1, S2, ge, tid);
+#pragma unroll
+    for (int k = 0; k < NML; ++k) {
+      const int g = tid + k * NT;
+      res[k] = 0.0;
+      if (g < 5 * L3) {
+        res[k] = bvec[k] - AP[g];
+        AP[g] = res[k];
+      }
     }
     __syncthreads();
   }
-  precondition<L, NT>(Pre, Bv, S1, S2, Pv, tid);   // x0 = P b, into Pv
-#pragma unroll
-  for (int k = 0; k < NML; ++k) {
-    const int g = tid + k * NT;
-    if (g < 5 * L3) x[k] = Pv[g];
-  }
-  // AP = M Pv
-  auto mass = [&]() {
-    contract<5, L, L * L, 1, Q, NT>(CT<P>::BT(), Pv, S1, tid);        // [5][Q][L L]
-    __syncthreads();
-    contract<5 * Q, L, L, 1, Q, NT>(CT<P>::BT(), S1, S2, tid);        // [5 Q][Q][L]
-    __syncthreads();
-    contract<5 * Q * Q, L, 1, 1, Q, NT>(CT<P>::BT(), S2, S1, tid);    // nodal [5][Q3]
-    __syncthreads();
-    for (int i = tid; i < 5 * Q3; i += NT) S1[i] *= ge[9 * Q3 + (i % Q3)];
-    __syncthreads();
-    contract<5, Q, Q * Q, 1, L, NT>(CT<P>::BL(), S1, S2, tid);        // [5][L][Q Q]
-    __syncthreads();
-    contract<5 * L, Q, Q, 1, L, NT>(CT<P>::BL(), S2, S1, tid);        // [5 L][L][Q]
-    __syncthreads();
-    contract<5 * L * L, Q, 1, 1, L, NT>(CT<P>::BL(), S1, AP, tid);    // [5][L3]
-    __syncthreads();
-  };
-  // r and z = P r from AP holding r (each thread's own entries); rr and r.z
+  // z = P r from AP holding r (each thread's own entries); r.r and r.z
   auto residual_step = [&]() {
     precondition<L, NT>(Pre, AP, S1, S2, AP, tid);
+    double part2[10];
 #pragma unroll
     for (int f = 0; f < 10; ++f) part2[f] = 0.0;
 #pragma unroll
@@ -342,22 +355,10 @@ k_rhs3c(CurvedArgs a) {
         part2[5 + f] = fma(res[k], AP[g], part2[5 + f]);
       }
     }
-    wg_sums<P, 10>(part2, red, rz2, tid);
+    wg_sums_lds<P, 10>(part2, red, Tot + 5, tid);
   };
-  mass();
-#pragma unroll
-  for (int k = 0; k < NML; ++k) {
-    const int g = tid + k * NT;
-    if (g < 5 * L3) {
-      res[k] = bvec[k] - AP[g];
-      AP[g] = res[k];
-    }
-  }
-  __syncthreads();
   residual_step();
-  double rz[5];
-#pragma unroll
-  for (int f = 0; f < 5; ++f) rz[f] = rz2[5 + f];
+  if (tid < 5) Tot[20 + tid] = Tot[10 + tid];
 #pragma unroll
   for (int k = 0; k < NML; ++k) {
     const int g = tid + k * NT;
@@ -369,27 +370,29 @@ k_rhs3c(CurvedArgs a) {
   for (; it < 60; ++it) {
     bool conv = true;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) conv = conv && (rz2[f] <= tol2 * bb[f]);
+    for (int f = 0; f < 5; ++f) conv = conv && (Tot[5 + f] <= tol2 * Tot[f]);
     if (conv) break;
-    mass();
+    mass_apply<P, NT>(Pv, AP, S1, S2, ge, tid);
+    {
+      double part[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) part[f] = 0.0;
+      for (int f = 0; f < 5; ++f) part[f] = 0.0;
 #pragma unroll
-    for (int k = 0; k < NML; ++k) {
-      const int g = tid + k * NT;
-      if (g < 5 * L3) part[g / L3] = fma(Pv[g], AP[g], part[g / L3]);
+      for (int k = 0; k < NML; ++k) {
+        const int g = tid + k * NT;
+        if (g < 5 * L3) part[g / L3] = fma(Pv[g], AP[g], part[g / L3]);
+      }
+      wg_sums_lds<P, 5>(part, red, Tot + 15, tid);
     }
-    wg_sums<P, 5>(part, red, pap, tid);
-    double alpha[5];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) alpha[f] = (rz[f] > 0.0 && pap[f] > 0.0) ? rz[f] / pap[f] : 0.0;
 #pragma unroll
     for (int k = 0; k < NML; ++k) {
       const int g = tid + k * NT;
       if (g < 5 * L3) {
         const int f = g / L3;
-        x[k] = fma(alpha[f], Pv[g], x[k]);
-        res[k] = fma(-alpha[f], AP[g], res[k]);
+        const double rz = Tot[20 + f], pap = Tot[15 + f];
+        const double alpha = (rz > 0.0 && pap > 0.0) ? rz / pap : 0.0;
+        x[k] = fma(alpha, Pv[g], x[k]);
+        res[k] = fma(-alpha, AP[g], res[k]);
         AP[g] = res[k];
       }
     }
@@ -400,12 +403,13 @@ k_rhs3c(CurvedArgs a) {
       const int g = tid + k * NT;
       if (g < 5 * L3) {
         const int f = g / L3;
-        const double beta = rz[f] > 0.0 ? rz2[5 + f] / rz[f] : 0.0;
+        const double rz = Tot[20 + f];
+        const double beta = rz > 0.0 ? Tot[10 + f] / rz : 0.0;
         Pv[g] = fma(beta, Pv[g], AP[g]);
       }
     }
-#pragma unroll
-    for (int f = 0; f < 5; ++f) rz[f] = rz2[5 + f];
+    __syncthreads();
+    if (tid < 5) Tot[20 + tid] = Tot[10 + tid];
     __syncthreads();
   }
   // the residual, or (update) the LSRK stage in place: this element's modes
