@@ -86,10 +86,9 @@ BR2_VISC_FLOPS = {7: 85.0}
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
 # (profiles/tgv_ns3d_32cubed_p7_roe_br2.md, profiles/tgv_ns3d_32cubed_p7_lf_br1.md)
 # keys: path, n, p, kernel, flux, viscous scheme
-MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "lf", "br1"): 2.301e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.755e9,
-                    ("mixed", 32, 7, "visc", "lf", "br1"): 1.831e9, ("mixed", 32, 7, "residual", "lf", "br1"): 5.175e9,
-                    ("fp64", 32, 7, "visc", "roe", "br2"): 2.301e9, ("fp64", 32, 7, "residual", "roe", "br2"): 5.755e9,
-                    ("mixed", 32, 7, "visc", "roe", "br2"): 2.079e9, ("mixed", 32, 7, "residual", "roe", "br2"): 5.175e9}
+MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "roe", "br2"): 3.008e9, ("fp64", 32, 7, "residual", "roe", "br2"): 5.759e9,
+                    ("mixed", 32, 7, "visc", "roe", "br2"): 2.096e9, ("mixed", 32, 7, "residual", "roe", "br2"): 5.177e9,
+                    ("fp64", 32, 7, "visc", "lf", "br1"): 3.032e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.759e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 FLUXES = {"lf": dgx.DGX_FLUX_LF, "roe": dgx.DGX_FLUX_ROE}
 VISCOUS = {"br1": dgx.DGX_VISCOUS_BR1, "br2": dgx.DGX_VISCOUS_BR2}
@@ -220,7 +219,7 @@ def serving_sample(args, seed=1234, precision=dgx.DGX_MIXED):
                       + ("per-request launches replayed from hipGraphs" if args.serve_unbatched
                          else "requests of one order and physics batched into shared launches"
                          + (", each group slice one persistent cooperative kernel" if args.serve_persistent
-                            else ", two launches per stage"))}
+                            else ", one launch per stage (traces of the next stage written by the residual)"))}
 
 
 def adjoint_sample(args, seed=4096):
@@ -574,6 +573,8 @@ def main():
                     help="run each batched group slice as one persistent cooperative kernel")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving sample and print its JSON (for profiling it alone)")
+    ap.add_argument("--only", choices=("serving", "adjoint", "euler_adjoint", "curved", "element_block"),
+                    help="run only this sample and print its JSON (for profiling it alone)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -582,9 +583,14 @@ def main():
         print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
               "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.serve_only:
+    if args.serve_only or args.only == "serving":
         print(json.dumps(serving_sample(args, precision=dgx.DGX_FP64 if args.serve_precision == "fp64"
                                         else dgx.DGX_MIXED)), flush=True)
+        return
+    if args.only:
+        fn = {"adjoint": adjoint_sample, "euler_adjoint": euler_adjoint_sample, "curved": curved_sample,
+              "element_block": element_block_sample}[args.only]
+        print(json.dumps(fn(args)), flush=True)
         return
 
     n, p, mu = args.n, args.p, 1.0 / 1600.0
