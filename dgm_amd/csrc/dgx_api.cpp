@@ -455,14 +455,22 @@ int step2(dgx_domain *d, double A, double B, double dt, bool traces_ready) {
   return DGX_OK;
 }
 
-// One LSRK stage of a curved hex domain: the traces, then k_rhs3c with the
-// register update fused into its store (bitwise the same as the residual
-// followed by the axpby and axpy).
+// One LSRK stage of a curved hex domain: the traces, on Navier-Stokes the
+// lifted gradients and viscous terms, then k_rhs3c with the register update
+// fused into its store (bitwise the same as the residual followed by the
+// axpby and axpy).
 int step_curved3(dgx_domain *d, double A, double B, double dt) {
   int rc;
   if ((rc = launch(d, dgx::OP_TRACES, d->U, nullptr))) return rc;
   dgx::CurvedArgs ca = curved_args(d);
   ca.U = d->U; ca.T = d->T; ca.R = nullptr;
+  for (int op : {1, 2}) {   // Navier-Stokes: lifted gradients, then the viscous terms
+    if (!d->visc) break;
+    ca.op = op;
+    hipError_t e = timed(d, op == 1 ? dgx::OP_GRAD : dgx::OP_VISC, [&] { return launch_curved(d, ca); });
+    if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved viscous terms: ") + hipGetErrorString(e));
+  }
+  ca.op = 0;
   ca.update = true; ca.Us = d->U; ca.Ut = d->Ut; ca.ca = A; ca.cb = B; ca.dt = dt;
   hipError_t e = timed(d, dgx::OP_RHS, [&] { return launch_curved(d, ca); });
   if (e != hipSuccess) return fail(DGX_ERR_DEVICE, std::string("curved stage: ") + hipGetErrorString(e));
@@ -1341,7 +1349,7 @@ int build(dgx_domain *d, const dgx_domain_desc *desc) {
   } else if (d->visc) {
     const int ng = nsd == 2 ? 6 : 12;
     if (!d->fused && (rc = alloc(&d->G, (size_t)ne * ng * d->nm))) return rc;
-    // curved quads keep the weighted nodal viscous divergence (Q^2 per field)
+    // curved elements keep the weighted nodal viscous divergence (Q^nsd per field)
     if ((rc = alloc(&d->V, (size_t)ne * (d->nf - 1) * (d->curved ? d->nq : d->nm)))) return rc;
     if ((rc = alloc(&d->TV, (size_t)ne * d->ns * (d->nf - 1) * qs))) return rc;
   }
@@ -1376,14 +1384,14 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes) {
   const size_t nf = shallow ? 3 : nsd + 2, nm = nsd == 2 ? L * L : L * L * L, nq = nsd == 2 ? Q * Q : Q * Q * Q;
   const size_t ns = nsd == 2 ? 4 : 6, qs = nsd == 2 ? QF : QF * QF;
   const bool visc = desc->eqntype == DGX_NAVIER_STOKES || desc->eqntype == DGX_NAVIER_STOKES3D;
-  const bool fused = nsd == 3 && dgx::stage3_supported(desc->p);
+  const bool fused = nsd == 3 && dgx::stage3_supported(desc->p) && !desc->xnodes;
   const bool mixed = desc->precision == DGX_MIXED;
   size_t b = 4 * nf * ne * nm * 8;                          // U, Ut, R0, R1
   b += (fused || (nsd == 2 && !desc->porder) ? 2 : 1) * ne * ns * nf * qs * 8;   // face traces
   if (visc) {
     const size_t w = mixed ? 4 : 8;
     if (!fused && !mixed) b += ne * (nsd == 2 ? 6 : 12) * nm * 8;
-    b += ne * (nf - 1) * (desc->xnodes && nsd == 2 ? nq : nm) * w + ne * ns * (nf - 1) * qs * w;
+    b += ne * (nf - 1) * (desc->xnodes ? nq : nm) * w + ne * ns * (nf - 1) * qs * w;
   }
   b += nf * ne * nq * 8;                                    // nodal transfer buffer
   b += ne * (9 + 4 * ns) * 8 + ne * ns * (4 + 4 + 1);       // geometry and links
@@ -1440,13 +1448,16 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
     return fail(DGX_ERR_ARGUMENT, "DGX_MIXED: boundary and source functions take the curved path, which is DGX_FP64");
   if (!desc->xnodes && !all_quads_affine(desc) && desc->precision != DGX_FP64)
     return fail(DGX_ERR_ARGUMENT, "non-parallelogram quads take the curved path, which is DGX_FP64");
+  const bool curved_hex_ns = desc->nsd == 3 && desc->eqntype == DGX_NAVIER_STOKES3D &&
+                             desc->viscous == DGX_VISCOUS_BR1 && dgx::curved3_ns_supported(desc->p);
   if (desc->xnodes && !((desc->precision == DGX_FP64 || curved_hex_mixed) &&
                         ((desc->nsd == 3 && desc->eqntype == DGX_EULER3D && dgx::curved3_supported(desc->p)) ||
+                         curved_hex_ns ||
                          (desc->nsd == 2 && (desc->eqntype == DGX_EULER || desc->eqntype == DGX_NAVIER_STOKES ||
                                              desc->eqntype == DGX_SHALLOW)))))
     return fail(DGX_ERR_ARGUMENT,
-                "curved elements (xnodes) are available for Euler3d (p <= 8, DGX_FP64 or DGX_MIXED), Euler, "
-                "Navier_Stokes and Shallow (DGX_FP64)");
+                "curved elements (xnodes) are available for Euler3d (p <= 8, DGX_FP64 or DGX_MIXED), "
+                "Navier_Stokes3d (p <= 7, DGX_VISCOUS_BR1, DGX_FP64), Euler, Navier_Stokes and Shallow (DGX_FP64)");
   const bool ns3 = desc->eqntype == DGX_NAVIER_STOKES3D;
   for (int r = 0; r < desc->ncon; ++r)
     if (desc->con[4 * r + 2] == -1) {
@@ -1463,9 +1474,9 @@ int dgx_domain_create(const dgx_domain_desc *desc, void *stream, dgx_handle *out
                                           ": Navier_Stokes3d has no slip-wall viscous flux "
                                           "(Euler3d::WallBC::apply_flux); use DGX_BC_ISOTHERMAL");
       if (ns3 && (desc->precision != DGX_FP64 || desc->viscous != DGX_VISCOUS_BR1 ||
-                  !dgx::stage3_supported(desc->p) || desc->xnodes))
-        return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d boundaries are available for affine hexes, p <= 9, "
-                                      "DGX_FP64 and DGX_VISCOUS_BR1");
+                  !(desc->xnodes ? dgx::curved3_ns_supported(desc->p) : dgx::stage3_supported(desc->p))))
+        return fail(DGX_ERR_ARGUMENT, "Navier_Stokes3d boundaries are available for affine hexes (p <= 9) and "
+                                      "curved hexes (p <= 7), DGX_FP64 and DGX_VISCOUS_BR1");
       if (t == DGX_BC_ISOTHERMAL && !(std::isfinite(desc->wall_T) && std::isfinite(desc->wall_Eref)))
         return fail(DGX_ERR_ARGUMENT, "wall_T and wall_Eref must be finite");
       if (t == DGX_BC_FUNCTION && !desc->bc_function)
@@ -1681,7 +1692,7 @@ int run_stage(dgx_domain *h, int inttype, int st, double dt, bool traces_ready) 
       ++h->stages;
     } else if (h->nsd == 2 && !h->curved && !h->varp && !h->bulk) {
       RUN(step2(h, A, B, dt, traces_ready));
-    } else if (h->nsd == 3 && h->curved && !h->visc && !h->bulk) {
+    } else if (h->nsd == 3 && h->curved && !h->bulk) {
       RUN(step_curved3(h, A, B, dt));
     } else {
       RUN(residual(h, h->U, h->R0));

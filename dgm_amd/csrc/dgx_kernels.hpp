@@ -160,6 +160,7 @@ struct CurvedArgs {
 hipError_t launch_curved2(const CurvedArgs &a);
 hipError_t upload_tables_c2d(const double *h);
 bool curved3_supported(int p);
+bool curved3_ns_supported(int p);   // Navier_Stokes3d (BR1) on curved hexes
 hipError_t launch_curved3(const CurvedArgs &a);
 hipError_t upload_tables_c3d(const double *h);
 

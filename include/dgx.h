@@ -141,7 +141,8 @@ typedef struct {
   int viscous;      /* enum dgx_viscous (Navier-Stokes; 0 = BR1) */
   double br2_eta;   /* BR2 penalty; 0 selects 6, the number of hex faces */
   double bc_state[5];  /* conservative state of DGX_BC_STATE boundaries (Shallow: the first 3) */
-  const double *xnodes;  /* curved elements, FP64: hexes (Euler3d, p <= 8) coordinates [3][ne][q^3]
+  const double *xnodes;  /* curved elements, FP64: hexes (Euler3d, p <= 8; Navier_Stokes3d with
+                            DGX_VISCOUS_BR1, p <= 7) coordinates [3][ne][q^3]
                             of every element's GLL nodes, node (i*q + j)*q + k; quads (Euler,
                             Navier_Stokes, Shallow) [2][ne][q^2], node i*q + j; NULL: affine */
   int no_balance;   /* Shallow: 1 leaves out the balance term.  By default, as

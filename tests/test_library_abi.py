@@ -78,7 +78,10 @@ def _desc(m, p, eqn, **kw):
     (dict(p=10, eqn=9, boundary=3, top=3), b"Navier_Stokes3d boundaries are available"),
     (dict(p=3, eqn=9, bulk=1, precision=1), b"bulk_forcing is available"),
     (dict(p=3, eqn=8, boundary=5), b"unknown boundary type"),
-    (dict(p=3, eqn=9, curved=True), b"curved elements"),
+    (dict(p=8, eqn=9, curved=True), b"curved elements"),
+    (dict(p=3, eqn=9, viscous=1, curved=True), b"curved elements"),
+    (dict(p=3, eqn=9, precision=1, curved=True), b"DGX_MIXED"),
+    (dict(p=8, eqn=9, boundary=3, top=3, curved=True), b"curved elements"),
     (dict(p=9, eqn=8, curved=True), b"curved elements"),
     (dict(p=9, eqn=8, precision=1, curved=True), b"DGX_MIXED"),
 ])

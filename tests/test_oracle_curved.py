@@ -9,7 +9,9 @@ curved-side files, so the 3D curved path is pinned by:
 * the projection of any modal field is exact: forward(backward(u)) = u, on
   strongly curved elements (a wrong mass matrix or solve breaks it);
 * consistency on a curved mesh: the residual of a smooth state converges
-  spectrally to the exact flux divergence, under p-refinement.
+  spectrally to the exact flux divergence, under p-refinement; for
+  Navier-Stokes (BR1) to the exact divergence of the inviscid minus the
+  viscous flux, derived symbolically.
 """
 from __future__ import annotations
 
@@ -103,6 +105,84 @@ def test_curved_residual_converges_spectrally():
     # a wrong normal, metric or mass matrix would leave an O(1) inconsistency
     flat = _consistency_error(5, n=4, amp=0.0)
     assert errs[1] < 1.5 * flat, (errs[1], flat)
+
+
+# ---- Navier-Stokes 3D (BR1) on curved hexes (src/Navier_Stokes3d.cpp:113-258) ----
+
+PR = 0.72
+
+
+def test_ns3d_affine_nodes_reproduce_affine_path():
+    """Isothermal bottom wall and an imposed state on top (the viscous ghosts
+    of Navier_Stokes3d's own BC database), affine nodes through the curved
+    path: the affine Navier-Stokes residual is pinned by runs/chan."""
+    m = meshgen.channel_hexes(2, 2, 2, (0, 0, 0), (2 * np.pi, 2 * np.pi, 2.0), bc_bottom=3, bc_top=1)
+    bc = [1.0, 0.1, 0.0, 0.0, 2.6]
+    for p in (3, 4):
+        a, b = Oracle(m, p, 9, vis=0.05), Oracle(m, p, 9, vis=0.05)
+        for o in (a, b):
+            o.set_bc_state(bc)
+            o.set_isothermal(1.0, 2.5)
+        b.set_nodes(meshgen.hex_nodes(m, p))
+        x = a.coords()
+        U = _state(x[0], x[1], x[2])
+        Fa, Fb = a.rhs(U), b.rhs(U)
+        assert np.max(np.abs(Fa - Fb)) <= 1e-12 * np.max(np.abs(Fa)), (p, np.max(np.abs(Fa - Fb)))
+
+
+def _ns_flux_div(mu):
+    """Exact div (F_c - F_v) of the state _state, symbolically: the stresses
+    mu (grad u + grad u^T - 2/3 div u I) and the energy flux tau.u +
+    (gamma / Pr) mu grad e with e = E / rho - |u|^2 / 2."""
+    import sympy as sp
+    x, y, z = sp.symbols("x y z")
+    X = (x, y, z)
+    rho = 1 + sp.Rational(1, 10) * sp.sin(x) * sp.cos(y) * sp.cos(z)
+    vel = (sp.Rational(3, 10) * sp.sin(x) * sp.cos(y), -sp.Rational(3, 10) * sp.cos(x) * sp.sin(y) * sp.cos(z),
+           sp.Rational(2, 10) * sp.sin(z) * sp.cos(x))
+    p = 1 + sp.Rational(1, 10) * sp.cos(2 * x) * sp.cos(y + z)
+    k2 = sum(c * c for c in vel)
+    E = p / (GAMMA - 1) + rho * k2 / 2
+    e = E / rho - k2 / 2
+    div_u = sum(sp.diff(vel[i], X[i]) for i in range(3))
+    tau = [[mu * (sp.diff(vel[i], X[j]) + sp.diff(vel[j], X[i])) - (sp.Rational(2, 3) * mu * div_u if i == j else 0)
+            for j in range(3)] for i in range(3)]
+    Fc = ([[rho * vel[j] for j in range(3)]] +
+          [[rho * vel[i] * vel[j] + (p if i == j else 0) for j in range(3)] for i in range(3)] +
+          [[(E + p) * vel[j] for j in range(3)]])
+    Fv = ([[0, 0, 0]] + [list(tau[i]) for i in range(3)] +
+          [[sum(tau[j][k] * vel[k] for k in range(3)) + GAMMA / PR * mu * sp.diff(e, X[j]) for j in range(3)]])
+    div = [sum(sp.diff(Fc[f][j] - Fv[f][j], X[j]) for j in range(3)) for f in range(5)]
+    return sp.lambdify(X, div, "numpy")
+
+
+def _ns_consistency_error(p, fn, mu, n=4, amp=0.15):
+    L = 2 * np.pi
+    m = meshgen.periodic_hexes(n, n, n, (0, 0, 0), (L, L, L))
+    o = Oracle(m, p, 9, vis=mu, Pr=PR)
+    xyz = meshgen.mapped_nodes(m, p, wavy_map(L, amp))
+    o.set_nodes(xyz)
+    F = o.rhs(_state(*xyz))
+    o.set_state(np.stack([np.broadcast_to(c, xyz[0].shape) for c in fn(*xyz)]))
+    o.project_state()
+    ex = o.state()
+    return np.sqrt(np.sum((F - ex) ** 2)) / np.sqrt(np.sum(ex ** 2))
+
+
+def test_ns3d_curved_residual_converges_spectrally():
+    """The Navier-Stokes residual on curved hexes converges spectrally to
+    the exact div (F_c - F_v) (5.9e-3, 4.5e-4, 2.5e-5 at p = 3, 5, 7 with
+    mu = 0.1, where dropping the viscous flux leaves 0.24).  A wrong metric,
+    normal, lift or mass matrix in the viscous path would leave an O(1)
+    inconsistency that does not converge."""
+    mu = 0.1
+    fn = _ns_flux_div(mu)
+    errs = [_ns_consistency_error(p, fn, mu) for p in (3, 5, 7)]
+    assert errs[2] < 5e-5, errs
+    assert errs[0] / errs[1] > 10 and errs[1] / errs[2] > 10, errs
+    flat = _ns_consistency_error(5, fn, mu, amp=0.0)
+    assert errs[1] < 2.5 * flat, (errs[1], flat)
+    assert _ns_consistency_error(5, _ns_flux_div(0.0), mu) > 0.1   # the viscous terms matter here
 
 
 # ---- curved quads (Quad::compute_curved_geometry, src/Quad.cpp:1309-1452) ----
