@@ -442,6 +442,44 @@ def element_block_sample(args, seed=11):
     return out
 
 
+def vortex_sample(args, seed=2024):
+    """The 2D isentropic-vortex workload: periodic [0,10]^2 with 32x32 affine
+    quads, p = 4, free stream rho = 1, u = 0.5, v = 0, p = 1/1.4 plus a
+    Gaussian vortex of strength 5 centred at a seeded point, Lax-Friedrichs,
+    RK4, dt from CFL 0.2 (dgx_set_time_step on the initial state), 100 steps;
+    FP64, then mixed from the same state, and the mixed state's per-field
+    relative L2 against the FP64 one."""
+    n, p, steps = 32, 4, args.vortex_steps
+    m = meshgen.periodic_quads(n, n, 0.0, 10.0, 0.0, 10.0)
+    rng = np.random.default_rng(seed)
+    x0, y0 = rng.uniform(0.0, 10.0, 2)
+    res, states = {}, {}
+    for prec in (dgx.DGX_FP64, dgx.DGX_MIXED):
+        gpu = dgx.GPUDomain(m, p, dgx.DGX_EULER, gamma=1.4, precision=prec)
+        x = gpu.coords()
+        gpu.upload(ic.isentropic_vortex(x[0], x[1], x0, y0, period=(10.0, 10.0)))
+        _, dt = gpu.set_time_step(1.0, CFLmax=0.2)
+        gpu.advance(dgx.DGX_RK4, dt, 2)            # warmup, then the timed run from the same state
+        gpu.upload(ic.isentropic_vortex(x[0], x[1], x0, y0, period=(10.0, 10.0)))
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        gpu.advance(dgx.DGX_RK4, dt, steps)
+        gpu.synchronize()
+        el = time.perf_counter() - t0
+        dofs = 4 * m.ne * (p + 1) ** 2
+        states[prec] = gpu.download(dgx.DGX_MODAL)
+        res[prec] = {"value": dofs * 4 * steps / el, "ms_per_step": el / steps * 1e3, "dt": dt}
+        gpu.close()
+    a, b = states[dgx.DGX_MIXED], states[dgx.DGX_FP64]
+    rel = [float(np.linalg.norm(a[v] - b[v]) / np.linalg.norm(b[v])) for v in range(4)]
+    out = {"workload": f"2D Euler isentropic vortex, periodic [0,10]^2, {n}x{n} quads, p={p}, LF, RK4, dt from CFL 0.2, "
+                       f"{steps} steps, {4 * m.ne * (p + 1) ** 2} DOF, FP64",
+           "unit": "DOF-stage/s", **res[dgx.DGX_FP64]}
+    out["mixed"] = {"value": res[dgx.DGX_MIXED]["value"], "speedup_vs_fp64": res[dgx.DGX_MIXED]["value"] /
+                    res[dgx.DGX_FP64]["value"], "rel_l2_per_field_vs_fp64": rel}
+    return out
+
+
 def run_path(m, n, p, mu, precision, args, flux=None, viscous=None, compare=True):
     """Time one precision on the TGV workload; returns (result dict, final modal state)."""
     flux = flux or args.flux
@@ -573,7 +611,9 @@ def main():
                     help="run each batched group slice as one persistent cooperative kernel")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving sample and print its JSON (for profiling it alone)")
-    ap.add_argument("--only", choices=("serving", "adjoint", "euler_adjoint", "curved", "element_block"),
+    ap.add_argument("--vortex-steps", type=int, default=100, help="RK4 steps of the 2D vortex sample")
+    ap.add_argument("--no-vortex", action="store_true", help="skip the 2D vortex sample")
+    ap.add_argument("--only", choices=("serving", "adjoint", "euler_adjoint", "curved", "element_block", "vortex"),
                     help="run only this sample and print its JSON (for profiling it alone)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
@@ -589,7 +629,7 @@ def main():
         return
     if args.only:
         fn = {"adjoint": adjoint_sample, "euler_adjoint": euler_adjoint_sample, "curved": curved_sample,
-              "element_block": element_block_sample}[args.only]
+              "element_block": element_block_sample, "vortex": vortex_sample}[args.only]
         print(json.dumps(fn(args)), flush=True)
         return
 
@@ -667,6 +707,8 @@ def main():
         line["curved"] = curved_sample(args)
     if not args.no_hexblock:
         line["element_block"] = element_block_sample(args)
+    if not args.no_vortex:
+        line["vortex"] = vortex_sample(args)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
