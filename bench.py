@@ -381,7 +381,15 @@ def curved_sample(args, seed=5):
         gpu.advance(dgx.DGX_LSRK14, dt, 1)
         gpu.synchronize()
         t0 = time.perf_counter()
-        gpu.advance(dgx.DGX_LSRK14, dt, args.cv_steps)
+        done = 0
+        while done < args.cv_steps:   # chunks of 20 steps, a progress line after each
+            k = min(20, args.cv_steps - done)
+            gpu.advance(dgx.DGX_LSRK14, dt, k)
+            done += k
+            if args.cv_steps > 20:
+                gpu.synchronize()
+                print(f"[bench] curved: {done} of {args.cv_steps} steps at {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
         gpu.synchronize()
         el = time.perf_counter() - t0
         finite = bool(np.all(np.isfinite(gpu.norms())))

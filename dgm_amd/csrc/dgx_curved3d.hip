@@ -407,17 +407,19 @@ k_rhs3c(CurvedArgs a) {
       if (n < Q3) {
         const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
         const double c = ge[9 * Q3 + n] / wend;
-        int fs[3], t1[3], t2[3], cnt = 0;
-        if (k == 0) { fs[cnt] = 0; t1[cnt] = i; t2[cnt++] = j; }
-        if (k == Q - 1) { fs[cnt] = 5; t1[cnt] = i; t2[cnt++] = j; }
-        if (j == 0) { fs[cnt] = 1; t1[cnt] = i; t2[cnt++] = k; }
-        if (j == Q - 1) { fs[cnt] = 3; t1[cnt] = i; t2[cnt++] = k; }
-        if (i == 0) { fs[cnt] = 4; t1[cnt] = j; t2[cnt++] = k; }
-        if (i == Q - 1) { fs[cnt] = 2; t1[cnt] = j; t2[cnt++] = k; }
-        for (int f = 0; f < cnt; ++f)
+        // the faces the node lies on, in a fixed order, each with a constant
+        // face (no indexed arrays, which would live in scratch)
+        auto add = [&](int f, int a1, int a2) {
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            Fr[v * Q3 + n] = fma((R)c, S2r[((v * 6 + fs[f]) * Q + t1[f]) * Q + t2[f]], Fr[v * Q3 + n]);
+            Fr[v * Q3 + n] = fma((R)c, S2r[((v * 6 + f) * Q + a1) * Q + a2], Fr[v * Q3 + n]);
+        };
+        if (k == 0) add(0, i, j);
+        if (k == Q - 1) add(5, i, j);
+        if (j == 0) add(1, i, k);
+        if (j == Q - 1) add(3, i, k);
+        if (i == 0) add(4, j, k);
+        if (i == Q - 1) add(2, j, k);
       }
     }
   }
@@ -600,17 +602,17 @@ k_grad3c(CurvedArgs a) {
       if (n < Q3) {
         const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
         const double cw = ge[9 * Q3 + n] / wend;
-        int fs[3], t1[3], t2[3], cnt = 0;
-        if (k == 0) { fs[cnt] = 0; t1[cnt] = i; t2[cnt++] = j; }
-        if (k == Q - 1) { fs[cnt] = 5; t1[cnt] = i; t2[cnt++] = j; }
-        if (j == 0) { fs[cnt] = 1; t1[cnt] = i; t2[cnt++] = k; }
-        if (j == Q - 1) { fs[cnt] = 3; t1[cnt] = i; t2[cnt++] = k; }
-        if (i == 0) { fs[cnt] = 4; t1[cnt] = j; t2[cnt++] = k; }
-        if (i == Q - 1) { fs[cnt] = 2; t1[cnt] = j; t2[cnt++] = k; }
-        for (int f = 0; f < cnt; ++f)
+        auto add = [&](int f, int a1, int a2) {
 #pragma unroll
           for (int x = 0; x < 3; ++x)
-            Fw[x * Q3 + n] = fma(cw, S2[((x * 6 + fs[f]) * Q + t1[f]) * Q + t2[f]], Fw[x * Q3 + n]);
+            Fw[x * Q3 + n] = fma(cw, S2[((x * 6 + f) * Q + a1) * Q + a2], Fw[x * Q3 + n]);
+        };
+        if (k == 0) add(0, i, j);
+        if (k == Q - 1) add(5, i, j);
+        if (j == 0) add(1, i, k);
+        if (j == Q - 1) add(3, i, k);
+        if (i == 0) add(4, j, k);
+        if (i == Q - 1) add(2, j, k);
       }
     }
     __syncthreads();

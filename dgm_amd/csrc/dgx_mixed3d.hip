@@ -806,25 +806,26 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
     const int n = tid + r * NT;
     if (n < Q3) {
       const int i = n / (Q * Q), j = (n / Q) % Q, k = n % Q;
-      int fs[6], l1[6], l2[6], cnt = 0;
-      if (k == 0) { fs[cnt] = 0; l1[cnt] = i; l2[cnt++] = j; }
-      if (k == Q - 1) { fs[cnt] = 5; l1[cnt] = i; l2[cnt++] = j; }
-      if (j == 0) { fs[cnt] = 1; l1[cnt] = i; l2[cnt++] = k; }
-      if (j == Q - 1) { fs[cnt] = 3; l1[cnt] = i; l2[cnt++] = k; }
-      if (i == 0) { fs[cnt] = 4; l1[cnt] = j; l2[cnt++] = k; }
-      if (i == Q - 1) { fs[cnt] = 2; l1[cnt] = j; l2[cnt++] = k; }
-      for (int c = 0; c < cnt; ++c) {
-        const double *nn = geo.fnorm + 3 * (e * 6 + fs[c]);
+      // face f at layer position (a1, a2); one call per face the node lies on,
+      // each with a constant face (no indexed arrays, which would live in scratch)
+      auto put = [&](int f, int a1, int a2) {
+        const double *nn = geo.fnorm + 3 * (e * 6 + f);
         const float n0 = (float)nn[0], n1 = (float)nn[1], n2 = (float)nn[2];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
+          Lay[((f * 4 + q) * Q + a1) * QP + a2] =
               mvisc_flux(tq[r], tq[r] + 6, uw[r][0], uw[r][1], uw[r][2], q, n0, n1, n2);
         if constexpr (BR2) {
 #pragma unroll
-          for (int q = 0; q < 3; ++q) Vel[((fs[c] * 3 + q) * Q + l1[c]) * QP + l2[c]] = uw[r][q];
+          for (int q = 0; q < 3; ++q) Vel[((f * 3 + q) * Q + a1) * QP + a2] = uw[r][q];
         }
-      }
+      };
+      if (k == 0) put(0, i, j);
+      if (k == Q - 1) put(5, i, j);
+      if (j == 0) put(1, i, k);
+      if (j == Q - 1) put(3, i, k);
+      if (i == 0) put(4, j, k);
+      if (i == Q - 1) put(2, j, k);
     }
   }
   __syncthreads();

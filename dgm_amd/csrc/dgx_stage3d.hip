@@ -180,21 +180,15 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const double gm1 = ph.gamma - 1.0;
 
-  // modal state -> nodal registers
-  for (int i = tid; i < 5 * L3; i += NT) {
+  // the modal state's loads are issued first and land in LDS after the face
+  // loop, whose dependent loads (links, orientation, neighbour traces) they overlap
+  constexpr int NML = (5 * L3 + NT - 1) / NT;
+  double um[NML];
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT;
     const int v = i / L3, m = i - v * L3;
-    A[(m / L + v * L2) * LP + (m % L)] = a.U[((size_t)v * ne + e) * L3 + m];
-  }
-  __syncthreads();
-  backward_batch<P, 5, kStage3Threads>(A, B, tid);
-  constexpr int FS = Q * Q * S::QP;   // padded nodal field size
-  double un[NPT][5];
-#pragma unroll
-  for (int r = 0; r < NPT; ++r) {
-    const int n = tid + r * NT;
-    const int pn = S::nidx(n < Q3 ? n : 0);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) un[r][v] = n < Q3 ? B[v * FS + pn] : 1.0;
+    um[k] = i < 5 * L3 ? a.U[((size_t)v * ne + e) * L3 + m] : 0.0;
   }
 
   // face corrections g = (J_side/J_vol) (Fhat - F(U-).n) [- BR1 viscous part]
@@ -232,6 +226,25 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
     }
   }
   if (bad) atomicOr(a.flag, 1);
+
+  // modal state -> nodal registers
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT;
+    const int v = i / L3, m = i - v * L3;
+    if (i < 5 * L3) A[(m / L + v * L2) * LP + (m % L)] = um[k];
+  }
+  __syncthreads();
+  backward_batch<P, 5, kStage3Threads>(A, B, tid);
+  constexpr int FS = Q * Q * S::QP;   // padded nodal field size
+  double un[NPT][5];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int n = tid + r * NT;
+    const int pn = S::nidx(n < Q3 ? n : 0);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) un[r][v] = n < Q3 ? B[v * FS + pn] : 1.0;
+  }
 
   // volume: projected divergence of the contravariant Euler fluxes
   double acc[NSL][L];
@@ -274,6 +287,23 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   vol_dir(IC<0>());
   proj_to_regs<P, 5, NSL, kStage3Threads>(A, B, acc, tid);
 
+  // the update's inputs (viscous volume term, RK register, state) are loaded
+  // before the face projection, whose barriers then cover their latency
+  double vin[NSL][L], utin[NSL][L], uin[NSL][L];
+#pragma unroll
+  for (int k = 0; k < NSL; ++k) {
+    const int G = tid + k * NT;
+    const int v = G / L2, pp = G - v * L2;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const bool in = G < 5 * L2;
+      const size_t gi = ((size_t)v * ne + e) * L3 + l * L2 + pp;
+      vin[k][l] = (NS && in && v > 0) ? a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp] : 0.0;
+      utin[k][l] = (UPDATE && in) ? a.Ut[gi] : 0.0;
+      uin[k][l] = (UPDATE && in) ? a.U[gi] : 0.0;
+    }
+  }
+
   // faces: H = (PW (x) PW) gF for 5 fields x 6 faces, in B
   face_project<P, 30, kStage3Threads>(gF, A, B, tid);
   const double one6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
@@ -285,13 +315,13 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         double r = acc[k][l] + face_sum<P>(B + v * 6 * L * LP, l, m, nn, one6);
-        if (NS && v > 0) r -= a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp];
+        if (NS && v > 0) r -= vin[k][l];
         const size_t gi = ((size_t)v * ne + e) * L3 + l * L2 + pp;
         if (!UPDATE) {
           a.R[gi] = r;
         } else {
-          const double ut = a.ca * a.Ut[gi] - a.dt * r;
-          const double u = fma(a.cb, ut, a.U[gi]);
+          const double ut = a.ca * utin[k][l] - a.dt * r;
+          const double u = fma(a.cb, ut, uin[k][l]);
           a.Ut[gi] = ut;
           a.U[gi] = u;
           acc[k][l] = u;
@@ -429,9 +459,15 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
   const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   constexpr int FS = Q * Q * QP;
 
-  for (int i = tid; i < 5 * L3; i += NT) {
+  // the modal state's loads are issued first and land in LDS after the jump
+  // loop, whose dependent loads they overlap
+  constexpr int NML = (5 * L3 + NT - 1) / NT;
+  double um[NML];
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT;
     const int v = i / L3, m = i - v * L3;
-    A[(m / L + v * L2) * LP + (m % L)] = U[((size_t)v * ne + e) * L3 + m];
+    um[k] = i < 5 * L3 ? U[((size_t)v * ne + e) * L3 + m] : 0.0;
   }
   // primitive jumps J_side/J_vol ({q} - q-) at the Gauss points (average_flux '+')
   for (int i = tid; i < 6 * QF2; i += NT) {
@@ -460,6 +496,12 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     const int qa = q / QF, qb = q - qa * QF;
 #pragma unroll
     for (int c = 0; c < 4; ++c) Jq[((c * 6 + s) * QF + qa) * FP + qb] = jr * (0.5 * (pl[c] + pq[c]) - pl[c]);
+  }
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT;
+    const int v = i / L3, m = i - v * L3;
+    if (i < 5 * L3) A[(m / L + v * L2) * LP + (m % L)] = um[k];
   }
   __syncthreads();
   backward_batch<P, 5, NT>(A, B, tid);
@@ -575,24 +617,25 @@ k_vterm3(const double *__restrict__ U, const double *__restrict__ T, double *__r
     const int n = tid + r * NT;
     if (n < Q3) {
       const int i = n / (Q * Q), j = (n / Q) % Q, k = n % Q;
-      int fs[6], l1[6], l2[6], cnt = 0;
-      if (k == 0) { fs[cnt] = 0; l1[cnt] = i; l2[cnt++] = j; }
-      if (k == Q - 1) { fs[cnt] = 5; l1[cnt] = i; l2[cnt++] = j; }
-      if (j == 0) { fs[cnt] = 1; l1[cnt] = i; l2[cnt++] = k; }
-      if (j == Q - 1) { fs[cnt] = 3; l1[cnt] = i; l2[cnt++] = k; }
-      if (i == 0) { fs[cnt] = 4; l1[cnt] = j; l2[cnt++] = k; }
-      if (i == Q - 1) { fs[cnt] = 2; l1[cnt] = j; l2[cnt++] = k; }
-      for (int c = 0; c < cnt; ++c) {
-        const double *nn = geo.fnorm + 3 * (e * 6 + fs[c]);
+      // face f at layer position (a1, a2); one call per face the node lies on,
+      // each with a constant face (no indexed arrays, which would live in scratch)
+      auto put = [&](int f, int a1, int a2) {
+        const double *nn = geo.fnorm + 3 * (e * 6 + f);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          Lay[((fs[c] * 4 + q) * Q + l1[c]) * QP + l2[c]] =
+          Lay[((f * 4 + q) * Q + a1) * QP + a2] =
               visc_flux(tq[r], tq[r] + 6, pv[r][0], pv[r][1], pv[r][2], q, nn[0], nn[1], nn[2]);
         if constexpr (BR2) {
 #pragma unroll
-          for (int q = 0; q < 3; ++q) Vel[((fs[c] * 3 + q) * Q + l1[c]) * QP + l2[c]] = pv[r][q];
+          for (int q = 0; q < 3; ++q) Vel[((f * 3 + q) * Q + a1) * QP + a2] = pv[r][q];
         }
-      }
+      };
+      if (k == 0) put(0, i, j);
+      if (k == Q - 1) put(5, i, j);
+      if (j == 0) put(1, i, k);
+      if (j == Q - 1) put(3, i, k);
+      if (i == 0) put(4, j, k);
+      if (i == Q - 1) put(2, j, k);
     }
   }
   __syncthreads();

@@ -315,10 +315,11 @@ int dgx_domain_bytes(const dgx_domain_desc *desc, size_t *bytes);
  *  - The second slice is captured into a hipGraph, which later full slices
  *    replay, so small meshes are not bound by launch overhead.
  *  - With `batch`, 2D Euler and shallow-water LSRK14 requests of one order
- *    and precision form a group: each stage is one face-trace launch and one
- *    residual-and-update launch over the elements of every member, on a fixed
- *    grid of workgroups that walk the members' elements.  Members step in
- *    lockstep slices and join or leave at slice boundaries.
+ *    and precision form a group: each stage is one residual-and-update
+ *    launch over the elements of every member, which also writes the next
+ *    stage's face traces (a face-trace launch runs only at a slice's first
+ *    stage), on a fixed grid of workgroups that walk the members' elements.
+ *    Members step in lockstep slices and join or leave at slice boundaries.
  *  - A request's result does not depend on what else runs: each element's
  *    arithmetic sees only its own request's buffers.
  *
@@ -331,7 +332,7 @@ typedef struct {
   int steps_per_slice;   /* steps enqueued per request and pass */
   int use_graphs;        /* replay slices from a captured hipGraph (1) or launch directly (0) */
   int batch;             /* 1: batch 2D Euler / shallow-water LSRK14 requests of one order and
-                            precision into shared launches (one per stage and kind for all of them) */
+                            precision into shared launches (one per stage for all of them) */
   int persistent;        /* with batch: 1 runs each group slice as one persistent cooperative
                             kernel (every stage, grid-wide barriers between stages) instead of
                             two launches per stage; results are bitwise the same */
