@@ -427,8 +427,14 @@ k_rhs3c(CurvedArgs a) {
   }   // !PROJ
   // b = B^T (wJ f): into Bv (compact [5][L3], the face-term region, free now)
   chain3<5, Q, L, NT>(TB3<R, P>::BL(), Fr, S1r, S2r, Bvr, tid);
-  // the mass solve: Pv (compact [5][L3]) reuses Fw; totals after the reductions
-  double x[NML];
+  // the mass solve: Pv (compact [5][L3]) reuses Fw; totals after the reductions.
+  // The update's RK register is loaded first: the solve's barriers cover it.
+  double x[NML], utin[NML];
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int g = tid + k * NT, v = g / L3;
+    utin[k] = (!PROJ && a.update && g < 5 * L3) ? a.Ut[((size_t)v * ne + e) * L3 + (g - v * L3)] : 0.0;
+  }
   int it;
   {
     double bvec[NML];
@@ -457,7 +463,7 @@ k_rhs3c(CurvedArgs a) {
       const int v = g / L3;
       const size_t o = ((size_t)v * ne + e) * L3 + (g - v * L3);
       if (!PROJ && a.update) {
-        const double ut = fma(mdt, x[k], a.ca * a.Ut[o]);   // as the axpby Ut = ca Ut - dt R
+        const double ut = fma(mdt, x[k], a.ca * utin[k]);   // as the axpby Ut = ca Ut - dt R
         a.Ut[o] = ut;
         a.Us[o] = fma(a.cb, ut, a.Us[o]);                   // as the axpy U += cb Ut
       } else {

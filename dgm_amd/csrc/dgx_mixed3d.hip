@@ -284,44 +284,58 @@ __device__ __forceinline__ float mface_sum(const float *H, int l, int m, int n, 
   return s;
 }
 
-// Load the modal state of element e: constant modes into cn[] (nodal value of
-// the constant mode, FP64), the other modes as FP32 into A ([5][L][L][L], pitch LP).
-template <int P, int NTH = kStageThreads>
-__device__ __forceinline__ void load_split(const double *__restrict__ U, int e, int ne, float *A,
-                                           double (&cn)[5], int tid) {
-  using S = M3<P, NTH>;
-  constexpr int L = S::L, L2 = S::L2, L3 = S::L3, LP = S::LP, NT = S::NT;
-  for (int i = tid; i < 5 * L3; i += NT) {
-    const int v = i / L3, m = i - v * L3;
-    const double u = U[((size_t)v * ne + e) * L3 + m];
-    A[(m / L + v * L2) * LP + (m % L)] = m == 0 ? 0.f : (float)u;
-  }
+// This thread's entries (index tid + k NTH of [5][L3]) of element e's modal
+// state, as registers.  The loads are issued here and consumed by store_split
+// after independent work (the face loop), whose own loads they overlap.
+template <int P, int NTH> struct ModalRegs {
+  static constexpr int N = (5 * (P + 1) * (P + 1) * (P + 1) + NTH - 1) / NTH;
+};
+template <int P, int NTH, int N>
+__device__ __forceinline__ void load_modal(const double *__restrict__ U, int e, int ne, double (&um)[N], int tid) {
+  constexpr int L3 = (P + 1) * (P + 1) * (P + 1);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) cn[v] = U[((size_t)v * ne + e) * L3] * kPhi0;
+  for (int k = 0; k < N; ++k) {
+    const int i = tid + k * NTH;
+    const int v = i / L3, m = i - v * L3;
+    um[k] = i < 5 * L3 ? U[((size_t)v * ne + e) * L3 + m] : 0.0;
+  }
 }
 
-// As load_split, also keeping the three linear modes out of FP32:
-// lin[v][x] = (1/2) u_x with u_x the coefficient of the mode linear along
-// reference dimension x (a: index L^2, b: L, c: 1), so that the nodal value of
-// the linear part is sum_x lin[v][x] B_1(xi_x) (B_0^2 = 1/2).
-template <int P>
-__device__ __forceinline__ void load_split_lin(const double *__restrict__ U, int e, int ne, float *A,
-                                               double (&cn)[5], double (&lin)[5][3], int tid) {
-  using S = M3<P>;
-  constexpr int L = S::L, L2 = S::L2, L3 = S::L3, LP = S::LP, NT = S::NT;
-  for (int i = tid; i < 5 * L3; i += NT) {
-    const int v = i / L3, m = i - v * L3;
-    const double u = U[((size_t)v * ne + e) * L3 + m];
-    const bool low = m == 0 || m == 1 || m == L || m == L2;
-    A[(m / L + v * L2) * LP + (m % L)] = low ? 0.f : (float)u;
+// The FP32 part of the modal state into A ([5][L][L][L], pitch LP): every
+// mode but the constant one, which stays FP64 (cn), and with LIN also but the
+// three linear ones (lin).
+template <int P, int NTH, bool LIN, int N>
+__device__ __forceinline__ void store_split(const double (&um)[N], float *A, int tid) {
+  using S = M3<P, NTH>;
+  constexpr int L = S::L, L2 = S::L2, L3 = S::L3, LP = S::LP;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int i = tid + k * NTH;
+    if (i < 5 * L3) {
+      const int v = i / L3, m = i - v * L3;
+      const bool low = m == 0 || (LIN && (m == 1 || m == L || m == L2));
+      A[(m / L + v * L2) * LP + (m % L)] = low ? 0.f : (float)um[k];
+    }
   }
+}
+
+// The constant modes' nodal values cn[v] (FP64) and, with lin, the linear
+// parts: lin[v][x] = (1/2) u_x with u_x the coefficient of the mode linear
+// along reference dimension x (a: index L^2, b: L, c: 1), so that the nodal
+// value of the linear part is sum_x lin[v][x] B_1(xi_x) (B_0^2 = 1/2).
+template <int P>
+__device__ __forceinline__ void low_modes(const double *__restrict__ U, int e, int ne, double (&cn)[5],
+                                          double (*lin)[3]) {
+  constexpr int L = P + 1, L2 = L * L, L3 = L2 * L;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double *u = U + ((size_t)v * ne + e) * L3;
     cn[v] = u[0] * kPhi0;
-    lin[v][0] = 0.5 * u[L2];
-    lin[v][1] = 0.5 * u[L];
-    lin[v][2] = 0.5 * u[1];
+    if (lin) {
+      lin[v][0] = 0.5 * u[L2];
+      lin[v][1] = 0.5 * u[L];
+      lin[v][2] = 0.5 * u[1];
+    }
   }
 }
 
@@ -377,9 +391,11 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   const double gm1 = ph.gamma - 1.0;
 
   double *ec = reinterpret_cast<double *>(gF + S::NG);   // 8-byte aligned: 2 NA + NG is even
+  double um[ModalRegs<P, NT>::N];
+  load_modal<P, NT>(a.U, e, ne, um, tid);
   {
     double cn[5], lin[5][3];
-    load_split_lin<P>(a.U, e, ne, A, cn, lin, tid);
+    low_modes<P>(a.U, e, ne, cn, lin);
     const double *mt = geo.metric + 9 * (size_t)e;
     if (tid < 5) {
       ec[S::EC_CN + tid] = cn[tid];
@@ -446,6 +462,7 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
     }
   }
   if (bad) atomicOr(a.flag, 1);
+  store_split<P, NT, true>(um, A, tid);
 
   __syncthreads();
   mbackward<P, 5>(A, B, tid);
@@ -512,6 +529,21 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   vol_dir(IC<1>());
   vol_dir(IC<2>());
 
+  // the update's HBM inputs (viscous volume term, RK register) are loaded
+  // before the face projection, whose barriers then cover their latency
+  double vin[NSL][L], utin[NSL][L];
+#pragma unroll
+  for (int k = 0; k < NSL; ++k) {
+    const int G = tid + k * NT;
+    const int v = G / L2, pp = G - v * L2;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const bool in = G < 5 * L2;
+      vin[k][l] = (NS && in && v > 0) ? (double)a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp] : 0.0;
+      utin[k][l] = (UPDATE && in) ? a.Ut[((size_t)v * ne + e) * L3 + l * L2 + pp] : 0.0;
+    }
+  }
+
   // faces: H = (PW (x) PW) gF for 5 fields x 6 faces, in B
   mface_project<P, 30>(gF, A, B, tid);
   const float one6[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
@@ -527,12 +559,12 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
         if (l == 0 && pp == 0)   // 2 sqrt(3) sum_d H[d][d][v]
           r += 3.4641016151377545870 * (ec[S::EC_H + 0 * 15 + 0 * 5 + v] + ec[S::EC_H + 1 * 15 + 1 * 5 + v] +
                                         ec[S::EC_H + 2 * 15 + 2 * 5 + v]);
-        if (NS && v > 0) r -= (double)a.V[((size_t)e * 4 + (v - 1)) * L3 + l * L2 + pp];
+        if (NS && v > 0) r -= vin[k][l];
         const size_t gi = ((size_t)v * ne + e) * L3 + l * L2 + pp;
         if (!UPDATE) {
           a.R[gi] = r;
         } else {
-          const double ut = a.ca * a.Ut[gi] - a.dt * r;
+          const double ut = a.ca * utin[k][l] - a.dt * r;
           const double u = fma(a.cb, ut, a.U[gi]);
           a.Ut[gi] = ut;
           a.U[gi] = u;
@@ -653,7 +685,9 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
   const int e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
 
   double cn[5];
-  load_split<P, NT>(U, e, ne, A, cn, tid);
+  double um[ModalRegs<P, NT>::N];
+  load_modal<P, NT>(U, e, ne, um, tid);
+  low_modes<P>(U, e, ne, cn, nullptr);
   // primitive jumps J_side/J_vol ({q} - q-) at the Gauss points (average_flux '+'), FP64
   for (int i = tid; i < 6 * QF2; i += NT) {
     const int s = i / QF2, q = i - s * QF2;
@@ -681,6 +715,7 @@ k_vterm3m(const double *__restrict__ U, const double *__restrict__ Tr, float *__
     for (int c = 0; c < 4; ++c)
       Jq[((c * 6 + s) * QF + qa) * FP + qb] = (float)(jr * (0.5 * (pl[c] + pq[c]) - pl[c]));
   }
+  store_split<P, NT, false>(um, A, tid);
   __syncthreads();
   mbackward<P, 5, NT>(A, B, tid);
   // primitives: deviations from the constant state (FP32) for the gradient
