@@ -288,6 +288,14 @@ k_rhs3c(CurvedArgs a) {
     }
     __syncthreads();
   } else {
+  // the modal state's loads are issued first; they land in LDS after the face
+  // loop, whose dependent loads (links, orientation, neighbour traces) they overlap
+  double um[NML];
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT, v = i / L3;
+    um[k] = i < 5 * L3 ? a.U[((size_t)v * ne + e) * L3 + (i - v * L3)] : 0.0;
+  }
   // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points,
   // before the nodal state is held in registers (the Riemann solver's
   // temporaries and the state would not fit beside each other)
@@ -332,9 +340,10 @@ k_rhs3c(CurvedArgs a) {
 
   // modal state -> nodal (5 fields), held in S2 through the volume term
   // (S2 is free until the face lift; registers are not)
-  for (int i = tid; i < 5 * L3; i += NT) {
-    const int v = i / L3;
-    S1r[i] = (R)a.U[((size_t)v * ne + e) * L3 + (i - v * L3)];
+#pragma unroll
+  for (int k = 0; k < NML; ++k) {
+    const int i = tid + k * NT;
+    if (i < 5 * L3) S1r[i] = (R)um[k];
   }
   __syncthreads();
   chain3<5, L, Q, NT>(TB3<R, P>::BT(), S1r, Fr, S1r, S2r, tid);
