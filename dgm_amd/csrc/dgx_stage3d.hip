@@ -180,8 +180,13 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const double gm1 = ph.gamma - 1.0;
 
-  // the modal state's loads are issued first and land in LDS after the face
-  // loop, whose dependent loads (links, orientation, neighbour traces) they overlap
+  // the element's metric terms and modal state are loaded first: the metric
+  // into registers before any barrier, the modal state landing in LDS after
+  // the face loop, whose dependent loads (links, orientation, neighbour traces)
+  // it overlaps
+  double mt[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) mt[c] = geo.metric[9 * (size_t)e + c];
   constexpr int NML = (5 * L3 + NT - 1) / NT;
   double um[NML];
 #pragma unroll
@@ -252,7 +257,6 @@ k_stage3(StageArgs a, Tab tb, Geo geo, Phys ph) {
   for (int k = 0; k < NSL; ++k)
 #pragma unroll
     for (int l = 0; l < L; ++l) acc[k][l] = 0.0;
-  const double *mt = geo.metric + 9 * (size_t)e;
   double pr[NPT], ux[NPT], uy[NPT], uz[NPT];
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
