@@ -86,9 +86,9 @@ BR2_VISC_FLOPS = {7: 85.0}
 # the 32^3 p=7 workload, scaled by the counters of a plain copy of known size
 # (profiles/tgv_ns3d_32cubed_p7_roe_br2.md, profiles/tgv_ns3d_32cubed_p7_lf_br1.md)
 # keys: path, n, p, kernel, flux, viscous scheme
-MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "roe", "br2"): 3.008e9, ("fp64", 32, 7, "residual", "roe", "br2"): 5.759e9,
-                    ("mixed", 32, 7, "visc", "roe", "br2"): 2.096e9, ("mixed", 32, 7, "residual", "roe", "br2"): 5.177e9,
-                    ("fp64", 32, 7, "visc", "lf", "br1"): 3.032e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.759e9}
+MEASURED_TRAFFIC = {("fp64", 32, 7, "visc", "roe", "br2"): 2.308e9, ("fp64", 32, 7, "residual", "roe", "br2"): 5.789e9,
+                    ("mixed", 32, 7, "visc", "roe", "br2"): 1.836e9, ("mixed", 32, 7, "residual", "roe", "br2"): 5.197e9,
+                    ("fp64", 32, 7, "visc", "lf", "br1"): 2.308e9, ("fp64", 32, 7, "residual", "lf", "br1"): 5.789e9}
 KINDS = ["traces", "residual", "grad", "visc", "forward", "backward", "update"]
 FLUXES = {"lf": dgx.DGX_FLUX_LF, "roe": dgx.DGX_FLUX_ROE}
 VISCOUS = {"br1": dgx.DGX_VISCOUS_BR1, "br2": dgx.DGX_VISCOUS_BR2}
