@@ -391,12 +391,15 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   const double gm1 = ph.gamma - 1.0;
 
   double *ec = reinterpret_cast<double *>(gF + S::NG);   // 8-byte aligned: 2 NA + NG is even
+  // the element's metric terms into registers before any barrier
+  double mt[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) mt[c] = geo.metric[9 * (size_t)e + c];
   double um[ModalRegs<P, NT>::N];
   load_modal<P, NT>(a.U, e, ne, um, tid);
   {
     double cn[5], lin[5][3];
     low_modes<P>(a.U, e, ne, cn, lin);
-    const double *mt = geo.metric + 9 * (size_t)e;
     if (tid < 5) {
       ec[S::EC_CN + tid] = cn[tid];
 #pragma unroll
@@ -407,13 +410,15 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
       double dU[5], dF[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) dU[v] = lin[v][x];
-      euler_flux_jvp(cn, mt[3 * d], mt[3 * d + 1], mt[3 * d + 2], gm1, dU, dF);
+      const double *mg = geo.metric + 9 * (size_t)e + 3 * d;   // d varies by thread: not the register copy
+      euler_flux_jvp(cn, mg[0], mg[1], mg[2], gm1, dU, dF);
 #pragma unroll
       for (int v = 0; v < 5; ++v) ec[S::EC_H + (d * 3 + x) * 5 + v] = dF[v];
     } else if (tid >= 32 && tid < 35) {
       // cF[d] = F_d(c)
       const int d = tid - 32;
-      const double m0 = mt[3 * d], m1 = mt[3 * d + 1], m2 = mt[3 * d + 2];
+      const double *mg = geo.metric + 9 * (size_t)e + 3 * d;
+      const double m0 = mg[0], m1 = mg[1], m2 = mg[2];
       const double ir = 1.0 / cn[0];
       const double u = cn[1] * ir, v = cn[2] * ir, w = cn[3] * ir;
       const double pc = gm1 * (cn[4] - 0.5 * (cn[1] * u + cn[2] * v + cn[3] * w));
@@ -487,7 +492,6 @@ k_stage3m(MixedArgs a, Geo geo, Phys ph) {
   for (int k = 0; k < NSL; ++k)
 #pragma unroll
     for (int ob = 0; ob < NOB; ++ob) acc[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
-  const double *mt = geo.metric + 9 * (size_t)e;
   // The flux is linearized about the constant state: F(c) + F'(c)[U_lin - c]
   // is linear in the reference coordinates, so it is subtracted at the nodes
   // (the FP32 chains see only the second-order remainder) and its projected
