@@ -417,6 +417,52 @@ def curved_sample(args, seed=5):
             "algorithmic_bytes_per_dof_stage": 32.0 + metric_b}
 
 
+def curved_ns_sample(args, seed=6):
+    """Navier-Stokes (BR1) on curved hexes: an n^3 hex channel whose bottom
+    wall carries a bump of amplitude 0.05, isothermal walls at the bottom and
+    top, periodic sides, p = 4, Re = 1000 on the channel height, Mach 0.5
+    free stream plus seeded density noise of 1e-3, LSRK14, FP64: one warmup
+    step, then `cvns_steps` timed steps."""
+    n, p = args.cvns_n, 4
+    lo, hi = (0.0, 0.0, 0.0), (3.0, 3.0, 1.0)
+    m = meshgen.channel_hexes(n, n, n, lo, hi, bc_bottom=dgx.DGX_BC_ISOTHERMAL, bc_top=dgx.DGX_BC_ISOTHERMAL)
+    xn = meshgen.mapped_nodes(m, p, meshgen.bump_map(lo, hi, 0.05))
+    rho0, u0, p0 = 1.0, 0.5, 1.0 / 1.4
+    E0 = p0 / 0.4 + 0.5 * rho0 * u0 * u0
+    rng = np.random.default_rng(seed)
+    U = np.empty((5,) + xn.shape[1:])
+    U[0] = rho0 + 1e-3 * rng.standard_normal(xn.shape[1:])
+    U[1], U[2], U[3], U[4] = rho0 * u0, 0.0, 0.0, E0
+    mu = rho0 * u0 * (hi[2] - lo[2]) / 1000.0
+    e0 = p0 / (rho0 * 0.4)
+    wall = (e0, 1.5)   # Tw, Eref: the wall ghost's internal energy (Eref - 1/2) Tw is the free stream's e0
+    dofs = 5 * m.ne * (p + 1) ** 3
+    h = (hi[2] - lo[2]) / n
+    dt = min(0.3 * h / ((p + 1) ** 2 * (u0 + 1.0)), 0.2 * rho0 * h * h / ((p + 1) ** 4 * mu))
+    t0 = time.perf_counter()
+    gpu = dgx.GPUDomain(m, p, dgx.DGX_NAVIER_STOKES3D, gamma=1.4, vis=mu, Pr=0.72, xnodes=xn, wall=wall,
+                        bc_state=[rho0, rho0 * u0, 0.0, 0.0, E0])
+    gpu.upload(U)
+    gpu.synchronize()
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] curved NS: {n}^3 p={p}: {dofs / 1e6:.1f} M DOF, setup {t_setup:.1f} s", file=sys.stderr, flush=True)
+    gpu.advance(dgx.DGX_LSRK14, dt, 1)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.advance(dgx.DGX_LSRK14, dt, args.cvns_steps)
+    gpu.synchronize()
+    el = time.perf_counter() - t0
+    finite = bool(np.all(np.isfinite(gpu.norms())))
+    cg = gpu.solver_iterations()
+    gpu.close()
+    return {"workload": f"3D Navier-Stokes (BR1, LF), {n}^3 curved hexes (bump of amplitude 0.05), isothermal "
+                        f"walls at bottom and top, periodic sides, p={p}, Re=1000, Mach 0.5 + 1e-3 seeded density "
+                        f"noise, LSRK14, FP64",
+            "value": dofs * 14 * args.cvns_steps / el, "unit": "DOF-stage/s", "dofs": dofs,
+            "steps": args.cvns_steps, "dt": dt, "ms_per_step": el / args.cvns_steps * 1e3, "setup_s": t_setup,
+            "state_finite": finite, "mass_solve_cg_iterations_max": cg}
+
+
 def element_block_sample(args, seed=11):
     """The element-block seam (GpGpu::hex_kernels: inner_product and
     backward_transform of a block of hexes sharing one basis) on the TGV mesh's
@@ -619,9 +665,13 @@ def main():
                     help="run each batched group slice as one persistent cooperative kernel")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving sample and print its JSON (for profiling it alone)")
+    ap.add_argument("--cvns-n", type=int, default=48, help="hexes per direction of the curved Navier-Stokes sample")
+    ap.add_argument("--cvns-steps", type=int, default=5)
+    ap.add_argument("--no-curved-ns", action="store_true", help="skip the curved Navier-Stokes sample")
     ap.add_argument("--vortex-steps", type=int, default=100, help="RK4 steps of the 2D vortex sample")
     ap.add_argument("--no-vortex", action="store_true", help="skip the 2D vortex sample")
-    ap.add_argument("--only", choices=("serving", "adjoint", "euler_adjoint", "curved", "element_block", "vortex"),
+    ap.add_argument("--only", choices=("serving", "adjoint", "euler_adjoint", "curved", "curved_ns", "element_block",
+                                       "vortex"),
                     help="run only this sample and print its JSON (for profiling it alone)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the modal state (a plain copy "
@@ -637,7 +687,7 @@ def main():
         return
     if args.only:
         fn = {"adjoint": adjoint_sample, "euler_adjoint": euler_adjoint_sample, "curved": curved_sample,
-              "element_block": element_block_sample, "vortex": vortex_sample}[args.only]
+              "curved_ns": curved_ns_sample, "element_block": element_block_sample, "vortex": vortex_sample}[args.only]
         print(json.dumps(fn(args)), flush=True)
         return
 
@@ -713,6 +763,8 @@ def main():
         line["euler_adjoint"] = euler_adjoint_sample(args)
     if not args.no_curved:
         line["curved"] = curved_sample(args)
+    if not args.no_curved_ns:
+        line["curved_ns"] = curved_ns_sample(args)
     if not args.no_hexblock:
         line["element_block"] = element_block_sample(args)
     if not args.no_vortex:

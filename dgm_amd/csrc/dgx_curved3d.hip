@@ -494,18 +494,27 @@ template <int P> struct C3V {
   static constexpr int L = P + 1, Q = P + 2, QF = P + 1;
   static constexpr int L2 = L * L, L3 = L2 * L, Q2 = Q * Q, Q3 = Q2 * Q, QF2 = QF * QF;
   static constexpr int NT = C3<P>::NT, NW = NT / 64, NPT = C3<P>::NPT;
-  static constexpr int ML = (3 * L3 + NT - 1) / NT;       // mass-solve entries per thread (3 fields)
-  // scratch: the 5-field transform [5][Q][L L], [5][Q Q][L]; the 18 face
-  // blocks [18][Q][QF], [18][Q][Q]; the 3-field mass operator
-  static constexpr int SZ = (5 * Q2 * L > 18 * Q2) ? 5 * Q2 * L : 18 * Q2;
-  // k_grad3c (doubles): Pq [4][Q3] primitives at the nodes, the first 5 Q3
-  // of Pq, Fw holding the nodal state before; Fw [3][Q3] one primitive's
-  // weighted gradient, then the CG direction; S1, S2 [SZ]; Jq [4][6][QF2]
-  // the jumps; Bv [3][L3]; the preconditioner [3][L][L]; reductions; totals
-  static constexpr int grad = (7 * Q3 + 2 * SZ + 24 * QF2 + 3 * L3 + 3 * L2 + 16 * NW + 32) * 8;
+  static constexpr int mx(int x, int y) { return x > y ? x : y; }
+  // k_grad3c solves NG = 3 NPR gradient components at once: two primitives
+  // per pass (6 components, 2 passes) where the LDS holds them, else one
+  // (3 components, 4 passes)
+  static constexpr int sz(int ng) {
+    return mx(mx(5 * Q2 * L, 6 * ng * Q2), ng * Q3);   // state transform, face blocks, mass operator
+  }
+  // k_grad3c (doubles): Pq [4][Q3] primitives at the nodes (the first 5 Q3
+  // of Pq and Fw hold the nodal state before); Fw [NG][Q3] the pass's
+  // weighted gradients, then the CG direction; S1, S2 [SZ]; Jq [4][6][QF2]
+  // the jumps; Bv [NG][L3]; the preconditioner [3][L][L]; reductions; totals
+  static constexpr int grad_lds(int ng) {
+    return ((4 + ng) * Q3 + 2 * sz(ng) + 24 * QF2 + ng * L3 + 3 * L2 + 16 * NW + 32) * 8;
+  }
+  static constexpr int NPR = grad_lds(6) <= 96 * 1024 ? 2 : 1, NG = 3 * NPR;
+  static constexpr int SZ = sz(NG), grad = grad_lds(NG);
+  static constexpr int ML = (NG * L3 + NT - 1) / NT;      // mass-solve entries per thread
   // k_visc3c: Un [5][Q3] the nodal state; Fc [3][Q3] one field's viscous
-  // flux (first the gradients at the nodes); S1, S2 [SZ]
-  static constexpr int visc = (8 * Q3 + 2 * SZ) * 8;
+  // flux (first the gradients at the nodes); S1, S2 [SV]
+  static constexpr int SV = mx(5 * Q2 * L, 18 * Q2);
+  static constexpr int visc = (8 * Q3 + 2 * SV) * 8;
 };
 
 bool curved3_ns_supported(int p) { return p >= 1 && p <= 7; }
@@ -522,11 +531,11 @@ __global__ void __launch_bounds__(C3<P>::NT)
 k_grad3c(CurvedArgs a) {
   using S = C3V<P>;
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
-  constexpr int NT = S::NT, NPT = S::NPT, ML = S::ML;
+  constexpr int NT = S::NT, NPT = S::NPT, ML = S::ML, NPR = S::NPR, NG = S::NG;
   extern __shared__ double sm[];
-  double *Pq = sm, *Fw = Pq + 4 * Q3, *S1 = Fw + 3 * Q3, *S2 = S1 + S::SZ, *Jq = S2 + S::SZ, *Bv = Jq + 24 * QF2;
-  double *Pre = Bv + 3 * L3, *red = Pre + 3 * S::L2, *Tot = red + 16 * S::NW;
-  static_assert(5 * Q * S::L2 <= S::SZ && 3 * Q3 <= S::SZ && 18 * Q * QF <= S::SZ, "scratch");
+  double *Pq = sm, *Fw = Pq + 4 * Q3, *S1 = Fw + NG * Q3, *S2 = S1 + S::SZ, *Jq = S2 + S::SZ, *Bv = Jq + 24 * QF2;
+  double *Pre = Bv + NG * L3, *red = Pre + 3 * S::L2, *Tot = red + 16 * S::NW;
+  static_assert(5 * Q * S::L2 <= S::SZ && NG * Q3 <= S::SZ && 6 * NG * Q * QF <= S::SZ && 5 * NG <= 32, "scratch");
   static_assert(S::grad <= 160 * 1024, "LDS");
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const Geo &geo = a.geo;
@@ -581,35 +590,39 @@ k_grad3c(CurvedArgs a) {
   }
   const double *Dn = CT<P>::DN();
   const double wend = CT<P>::BW()[0] / CT<P>::BT()[0];   // GLL end weight
-  for (int c = 0; c < 4; ++c) {
+  for (int c0 = 0; c0 < 4; c0 += NPR) {
     __syncthreads();
-    // weighted nodal gradient of primitive c into Fw [3][Q3]
-    const double *F = Pq + c * Q3;
+    // weighted nodal gradients of primitives c0 .. c0 + NPR - 1 into Fw [NG][Q3]
+#pragma unroll 1
+    for (int pc = 0; pc < NPR; ++pc) {
+      const double *F = Pq + (c0 + pc) * Q3;
 #pragma unroll
-    for (int rr = 0; rr < NPT; ++rr) {
-      const int n = tid + rr * NT;
-      if (n < Q3) {
-        const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
-        double dr = 0.0, ds = 0.0, dt = 0.0;
+      for (int rr = 0; rr < NPT; ++rr) {
+        const int n = tid + rr * NT;
+        if (n < Q3) {
+          const int i = n / Q2, j = (n / Q) % Q, k = n % Q;
+          double dr = 0.0, ds = 0.0, dt = 0.0;
 #pragma unroll
-        for (int m = 0; m < Q; ++m) {
-          dr = fma(Dn[i * Q + m], F[(m * Q + j) * Q + k], dr);
-          ds = fma(Dn[j * Q + m], F[(i * Q + m) * Q + k], ds);
-          dt = fma(Dn[k * Q + m], F[(i * Q + j) * Q + m], dt);
+          for (int m = 0; m < Q; ++m) {
+            dr = fma(Dn[i * Q + m], F[(m * Q + j) * Q + k], dr);
+            ds = fma(Dn[j * Q + m], F[(i * Q + m) * Q + k], ds);
+            dt = fma(Dn[k * Q + m], F[(i * Q + j) * Q + m], dt);
+          }
+#pragma unroll
+          for (int x = 0; x < 3; ++x)
+            Fw[(3 * pc + x) * Q3 + n] = fma(mg[rr][6 + x], dt, fma(mg[rr][3 + x], ds, mg[rr][x] * dr));
         }
-#pragma unroll
-        for (int x = 0; x < 3; ++x) Fw[x * Q3 + n] = fma(mg[rr][6 + x], dt, fma(mg[rr][3 + x], ds, mg[rr][x] * dr));
       }
     }
-    // face blocks n_x J_side/J_vol ({q} - q-) [3][6][QF2] -> GLL face nodes
-    for (int i = tid; i < 18 * QF2; i += NT) {
-      const int b = i / QF2, q = i - b * QF2, x = b / 6, s = b - x * 6;
+    // face blocks n_x J_side/J_vol ({q} - q-) [NG][6][QF2] -> GLL face nodes
+    for (int i = tid; i < 6 * NG * QF2; i += NT) {
+      const int b = i / QF2, q = i - b * QF2, g = b / 6, s = b - g * 6, c = c0 + g / 3, x = g - (g / 3) * 3;
       S2[i] = a.fgeo[(((size_t)e * 6 + s) * QF2 + q) * 4 + x] * Jq[(c * 6 + s) * QF2 + q];
     }
     __syncthreads();
-    contract<18, QF, QF, 1, Q, NT>(CT<P>::GL(), S2, S1, tid);   // [18][Q][QF]
+    contract<6 * NG, QF, QF, 1, Q, NT>(CT<P>::GL(), S2, S1, tid);   // [6 NG][Q][QF]
     __syncthreads();
-    contract<18, Q, QF, 2, Q, NT>(CT<P>::GL(), S1, S2, tid);    // [18][Q][Q]
+    contract<6 * NG, Q, QF, 2, Q, NT>(CT<P>::GL(), S1, S2, tid);    // [6 NG][Q][Q]
     __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < NPT; ++rr) {
@@ -619,8 +632,8 @@ k_grad3c(CurvedArgs a) {
         const double cw = ge[9 * Q3 + n] / wend;
         auto add = [&](int f, int a1, int a2) {
 #pragma unroll
-          for (int x = 0; x < 3; ++x)
-            Fw[x * Q3 + n] = fma(cw, S2[((x * 6 + f) * Q + a1) * Q + a2], Fw[x * Q3 + n]);
+          for (int g = 0; g < NG; ++g)
+            Fw[g * Q3 + n] = fma(cw, S2[((g * 6 + f) * Q + a1) * Q + a2], Fw[g * Q3 + n]);
         };
         if (k == 0) add(0, i, j);
         if (k == Q - 1) add(5, i, j);
@@ -632,18 +645,18 @@ k_grad3c(CurvedArgs a) {
     }
     __syncthreads();
     // b = B^T (wJ g), then M x = b
-    chain3<3, Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Bv, tid);
+    chain3<NG, Q, L, NT>(CT<P>::BL(), Fw, S1, S2, Bv, tid);
     double bvec[ML], x[ML];
 #pragma unroll
     for (int k = 0; k < ML; ++k) {
       const int g = tid + k * NT;
-      bvec[k] = g < 3 * L3 ? Bv[g] : 0.0;
+      bvec[k] = g < NG * L3 ? Bv[g] : 0.0;
     }
-    const int it = pcg_solve<P, 3>(bvec, Bv, Fw, S1, S2, Pre, ge, red, Tot, x, tid);
+    const int it = pcg_solve<P, NG>(bvec, Bv, Fw, S1, S2, Pre, ge, red, Tot, x, tid);
 #pragma unroll
     for (int k = 0; k < ML; ++k) {
       const int g = tid + k * NT;
-      if (g < 3 * L3) a.G[((size_t)e * 12 + 3 * c) * L3 + g] = x[k];
+      if (g < NG * L3) a.G[((size_t)e * 12 + 3 * c0) * L3 + g] = x[k];
     }
     if (tid == 0 && a.iters) atomicMax(a.iters, it);
   }
@@ -662,7 +675,7 @@ k_visc3c(CurvedArgs a) {
   constexpr int L = S::L, Q = S::Q, QF = S::QF, L3 = S::L3, Q2 = S::Q2, Q3 = S::Q3, QF2 = S::QF2;
   constexpr int NT = S::NT, NPT = S::NPT;
   extern __shared__ double sm[];
-  double *Un = sm, *Fc = Un + 5 * Q3, *S1 = Fc + 3 * Q3, *S2 = S1 + S::SZ;
+  double *Un = sm, *Fc = Un + 5 * Q3, *S1 = Fc + 3 * Q3, *S2 = S1 + S::SV;
   static_assert(S::visc <= 160 * 1024, "LDS");
   const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
   const double *ge = a.gm + (size_t)e * 10 * Q3;
