@@ -52,7 +52,7 @@ template <int P> struct C3 {
   // preconditioned residual [5][L3]); the element's preconditioner [3][L][L];
   // reductions [NW][16] and the mass solve's per-field totals [32] (the same
   // in every thread, so read from LDS instead of held in registers).  The
-  // nodal state lives in registers.
+  // nodal state sits in S2 through the volume term.
   static constexpr int SZ = (5 * Q3 > 30 * Q2) ? 5 * Q3 : 30 * Q2;
   static constexpr int GZ = (30 * QF2 > 5 * L3) ? 30 * QF2 : 5 * L3;
   static constexpr int lds = (5 * Q3 + 2 * SZ + GZ + 3 * L2 + 16 * NW + 32) * 8;
@@ -249,10 +249,10 @@ __device__ __forceinline__ int pcg_solve(const double (&bvec)[ML], double *Bv, d
 
 // PROJ: project the nodal fields a.U [5][ne][Q^3] instead (the forward
 // transform of a curved element: x = M^-1 B^T (wJ u)).
-// Three workgroups fit a CU's LDS at p = 5; asking the compiler for three
-// waves per SIMD (168 VGPRs, a few spills) measured faster than leaving it
-// two (7.8e9 against 7.5e9 DOF-stage/s on 96^3 bump hexes at p = 5).  At
-// p = 7, 8 one 512-thread workgroup per CU gives two waves per SIMD.
+// Three workgroups fit a CU's LDS at p = 5, and the kernel asks the compiler
+// for three waves per SIMD (at most 168 VGPRs; it needs about 150 at p = 5,
+// with no spills).  At p = 7, 8 one 512-thread workgroup per CU gives two
+// waves per SIMD.
 // R: arithmetic of the residual before the mass solve (its element tensors,
 // contractions and derivatives): double, or float on the mixed path, whose
 // LDS views alias the same memory as the FP64 buffers of the mass solve.
@@ -297,8 +297,8 @@ k_rhs3c(CurvedArgs a) {
     um[k] = i < 5 * L3 ? a.U[((size_t)v * ne + e) * L3 + (i - v * L3)] : 0.0;
   }
   // face corrections g = (J_side/J_vol)(Fhat - F(U-).n) at the Gauss points,
-  // before the nodal state is held in registers (the Riemann solver's
-  // temporaries and the state would not fit beside each other)
+  // before the nodal state and the metric terms are live (the Riemann
+  // solver's temporaries and those would not fit in registers beside each other)
   bool bad = false;
   for (int i = tid; i < 6 * QF2; i += NT) {
     const int s = i / QF2, q = i - s * QF2;
