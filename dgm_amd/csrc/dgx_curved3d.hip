@@ -275,7 +275,7 @@ k_rhs3c(CurvedArgs a) {
   R *gF = reinterpret_cast<R *>(G), *Fc = S1r, *Bvr = reinterpret_cast<R *>(G);
   static_assert(30 * QF * Q <= S::SZ && 5 * L3 <= S::SZ && 3 * Q3 <= S::SZ && 5 * Q * L * L <= S::SZ, "scratch");
   static_assert(S::lds <= 160 * 1024, "LDS");
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const Geo &geo = a.geo;
   const double *ge = a.gm + (size_t)e * 10 * Q3;   // wJ dxi_d/dx_x (9 rows), wJ
   const double gm1 = a.phys.gamma - 1.0;
@@ -537,7 +537,7 @@ k_grad3c(CurvedArgs a) {
   double *Pre = Bv + NG * L3, *red = Pre + 3 * S::L2, *Tot = red + 16 * S::NW;
   static_assert(5 * Q * S::L2 <= S::SZ && NG * Q3 <= S::SZ && 6 * NG * Q * QF <= S::SZ && 5 * NG <= 32, "scratch");
   static_assert(S::grad <= 160 * 1024, "LDS");
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const Geo &geo = a.geo;
   const double *ge = a.gm + (size_t)e * 10 * Q3;
   for (int i = tid; i < 3 * S::L2; i += NT) Pre[i] = a.pre[(size_t)e * 3 * S::L2 + i];
@@ -677,7 +677,7 @@ k_visc3c(CurvedArgs a) {
   extern __shared__ double sm[];
   double *Un = sm, *Fc = Un + 5 * Q3, *S1 = Fc + 3 * Q3, *S2 = S1 + S::SV;
   static_assert(S::visc <= 160 * 1024, "LDS");
-  const int e = blockIdx.x, tid = threadIdx.x, ne = a.ne;
+  const int ne = a.ne, e = xcd_element(blockIdx.x, ne), tid = threadIdx.x;
   const double *ge = a.gm + (size_t)e * 10 * Q3;
   for (int i = tid; i < 5 * L3; i += NT) {
     const int v = i / L3;

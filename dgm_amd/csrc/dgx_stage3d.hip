@@ -1,7 +1,7 @@
 // Fused hex stage kernels (p <= 9).
 //
-// One workgroup per element (512 threads for the stage kernel, 256 for the
-// viscous kernel, whose 198 VGPRs allow only 2 waves per SIMD).  Every sum-factorized
+// One workgroup of 512 threads per element for both kernels, two workgroups
+// per CU (4 waves per SIMD, at most 128 VGPRs).  Every sum-factorized
 // contraction runs on all conserved fields of the element at once (pencils of
 // 5 fields x q^2 fill the workgroup instead of q^2 = 81 pencils at p = 7),
 // nodal values live in registers (one thread per node), and the last
