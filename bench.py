@@ -678,8 +678,9 @@ def main():
                          "kernel of known size for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
-              "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path (a domain is one GPU's share; "
+              "sharding it by element slabs with a trace halo exchange is not built); rerun with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.serve_only or args.only == "serving":
         print(json.dumps(serving_sample(args, precision=dgx.DGX_FP64 if args.serve_precision == "fp64"
