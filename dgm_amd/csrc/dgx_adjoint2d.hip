@@ -92,6 +92,87 @@ __device__ __forceinline__ void lf_jac_t(const double UL[4], const double UR[4],
   }
 }
 
+// Forward-mode dual numbers: value and one directional derivative.  The
+// Roe flux's transposed Jacobian is assembled from its 8 directional
+// derivatives (one per input of UL and UR), evaluated by the same formula as
+// the forward flux; min(x, 0) takes the side of its argument's value.
+struct Dual {
+  double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b) { const double q = a / b.v; return {q, -q * b.d / b.v}; }
+__device__ __forceinline__ Dual dsqrt(Dual a) { const double r = sqrt(a.v); return {r, 0.5 * a.d / r}; }
+__device__ __forceinline__ Dual dmin0(Dual a) { return a.v < 0.0 ? a : Dual{0.0, 0.0}; }
+
+// The Roe face flux F^(UL, UR, n) = F(UL).n + G(UL, UR) of roe_flux2 (the same
+// rotated-frame Roe average, wave strengths and upwinded waves), in duals.
+__device__ __forceinline__ void roe_flux_dual(const Dual UL[4], const Dual UR[4], double nx, double ny,
+                                              double gamma, Dual Fh[4]) {
+  const double gm1 = gamma - 1.0;
+  const Dual rl = UL[0], rul = UL[1] * nx + UL[2] * ny, rvl = UL[2] * nx - UL[1] * ny, rEl = UL[3];
+  const Dual rr = UR[0], rur = UR[1] * nx + UR[2] * ny, rvr = UR[2] * nx - UR[1] * ny, rEr = UR[3];
+  const Dual ul = rul / rl, vl = rvl / rl, ur = rur / rr, vr = rvr / rr;
+  const Dual pl = gm1 * (rEl - 0.5 * (rul * ul + rvl * vl));
+  const Dual pr = gm1 * (rEr - 0.5 * (rur * ur + rvr * vr));
+  const Dual Hl = (rEl + pl) / rl, Hr = (rEr + pr) / rr;
+  const Dual R = dsqrt(rr / rl), iRp1 = 1.0 / (1.0 + R);
+  const Dual u = (ul + R * ur) * iRp1, v = (vl + R * vr) * iRp1, H = (Hl + R * Hr) * iRp1;
+  const Dual a = dsqrt(gm1 * (H - 0.5 * (u * u + v * v))), ainv = 1.0 / a;
+  const Dual c1inv = a * a * (1.0 / gm1);
+  Dual al[4], lm[4], G[4];
+  al[2] = (rvr - rvl) - v * (rr - rl);
+  al[1] = gm1 * ainv * ainv * ((rr - rl) * (H - u * u) + u * (rur - rul) -
+                               ((rEr - rEl) - ((rvr - rvl) - v * (rr - rl)) * v));
+  al[0] = 0.5 * ainv * ((rr - rl) * (u + a) - (rur - rul) - a * al[1]);
+  al[3] = (rr - rl) - (al[0] + al[1]);
+  lm[0] = dmin0(u - a); lm[1] = dmin0(u); lm[2] = lm[1]; lm[3] = dmin0(u + a);
+  G[0] = lm[0] * al[0] + lm[1] * al[1] + lm[3] * al[3];
+  G[1] = lm[0] * (u - a) * al[0] + lm[1] * u * al[1] + lm[3] * (u + a) * al[3];
+  G[2] = lm[0] * v * al[0] + lm[1] * v * al[1] + lm[2] * al[2] + lm[3] * v * al[3];
+  G[3] = lm[0] * (H - u * a) * al[0] + lm[1] * (H - c1inv) * al[1] + lm[2] * v * al[2] +
+         lm[3] * (H + u * a) * al[3];
+  // F(UL).n
+  const Dual ir = 1.0 / UL[0], uu = UL[1] * ir, vv = UL[2] * ir;
+  const Dual p = gm1 * (UL[3] - 0.5 * (UL[1] * uu + UL[2] * vv));
+  const Dual un = uu * nx + vv * ny;
+  Fh[0] = UL[0] * un + G[0];
+  Fh[1] = UL[1] * un + p * nx + (G[1] * nx - G[2] * ny);
+  Fh[2] = UL[2] * un + p * ny + (G[1] * ny + G[2] * nx);
+  Fh[3] = (UL[3] + p) * un + G[3];
+}
+
+// Transposed Jacobians of the Roe flux applied to w: aL = (dF^/dUL)^T w,
+// aR = (dF^/dUR)^T w, one directional derivative per input.
+__device__ __forceinline__ void roe_jac_t(const double UL[4], const double UR[4], double nx, double ny,
+                                          double gamma, const double w[4], double aL[4], double aR[4]) {
+#pragma unroll 1
+  for (int j = 0; j < 8; ++j) {
+    Dual L[4], Rr[4], F[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      L[v] = {UL[v], j == v ? 1.0 : 0.0};
+      Rr[v] = {UR[v], j == 4 + v ? 1.0 : 0.0};
+    }
+    roe_flux_dual(L, Rr, nx, ny, gamma, F);
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) t = fma(w[v], F[v].d, t);
+    if (j < 4) aL[j] = t;
+    else aR[j - 4] = t;
+  }
+}
+
 // Transposed basis tables of every order (this file's constant memory,
 // derived from the shared tables at upload): BTt [L][Q] = BT^T, BWt [Q][L] =
 // BW^T, QDt [Q][L] = QD^T, PWt [QF][L] = PW^T and Bgt [L][QF] = Bg^T, so that
@@ -232,9 +313,14 @@ __global__ void __launch_bounds__(64) k_vjp2(AdjArgs a) {
       wo[v] = jr * ll[v * QF + g]; wn[v] = jrn * lr[v * QF + gn];
     }
     double aL[4], aR[4], jf[4], bL[4], bR[4];
-    lf_jac_t(UL, UR, nx, ny, gamma, wo, aL, aR);     // own face flux F^(UL, UR, n)
+    if (a.flux == 1) {
+      roe_jac_t(UL, UR, nx, ny, gamma, wo, aL, aR);  // own face flux F^(UL, UR, n)
+      roe_jac_t(UR, UL, mx, my, gamma, wn, bL, bR);  // the neighbour's F^(UR, UL, n_nb)
+    } else {
+      lf_jac_t(UL, UR, nx, ny, gamma, wo, aL, aR);
+      lf_jac_t(UR, UL, mx, my, gamma, wn, bL, bR);
+    }
     euler_jac_t(UL, nx, ny, gm1, wo, jf);             // minus F(UL).n
-    lf_jac_t(UR, UL, mx, my, gamma, wn, bL, bR);     // the neighbour's F^(UR, UL, n_nb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) gT[(v * 4 + s) * QF + g] = aL[v] - jf[v] + bR[v];
   }

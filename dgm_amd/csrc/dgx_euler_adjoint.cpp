@@ -114,7 +114,7 @@ int vjp(dgx_euler_adjoint *a, const double *U, double *lam, double *out, int s) 
   if (rc) return rc;
   dgx::AdjArgs aa;
   aa.p = a->p; aa.ne = a->ne; aa.U = U; aa.T = d->T; aa.lam = lam; aa.Lt = a->Lt; aa.out = out;
-  aa.gamma = d->gamma; aa.geo = d->geo; aa.stream = d->stream;
+  aa.gamma = d->gamma; aa.flux = d->phys.flux; aa.geo = d->geo; aa.stream = d->stream;
   if (s >= 0) {
     aa.lamU = out; aa.B = dgx::kLsrkB[s]; aa.A = dgx::kLsrkA[s]; aa.stage = true;
     aa.scale = -a->dt; aa.accumulate = true;
@@ -198,8 +198,8 @@ int dgx_euler_adjoint_create(const dgx_domain_desc *mesh, const dgx_euler_adjoin
   if (mesh->nsd != 2 || mesh->eqntype != DGX_EULER || mesh->vis != 0.0)
     return fail(DGX_ERR_ARGUMENT, "the Euler adjoint needs a 2D mesh with eqntype DGX_EULER and vis = 0");
   if (mesh->precision != DGX_FP64) return fail(DGX_ERR_ARGUMENT, "the Euler adjoint runs in DGX_FP64");
-  if (mesh->flux != DGX_FLUX_LAX_FRIEDRICHS)
-    return fail(DGX_ERR_ARGUMENT, "the Euler adjoint differentiates the Lax-Friedrichs flux only");
+  if (mesh->flux != DGX_FLUX_LAX_FRIEDRICHS && mesh->flux != DGX_FLUX_ROE)
+    return fail(DGX_ERR_ARGUMENT, "the Euler adjoint differentiates the Lax-Friedrichs and Roe fluxes");
   if (mesh->xnodes || mesh->bc_function || mesh->source || mesh->porder)
     return fail(DGX_ERR_ARGUMENT, "the Euler adjoint runs on affine quads of one order (no xnodes, functions "
                                   "or porder)");

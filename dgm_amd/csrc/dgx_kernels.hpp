@@ -199,6 +199,7 @@ struct AdjArgs {
   const double *U = nullptr, *T = nullptr, *lamU = nullptr;
   double *lam = nullptr, *Lt = nullptr, *out = nullptr;
   double scale = 1.0, gamma = 1.4, A = 0.0, B = 0.0;
+  int flux = 0;                              // face flux: 0 Lax-Friedrichs, 1 Roe
   bool accumulate = false, stage = false;
   Geo geo;
   hipStream_t stream = nullptr;

@@ -63,14 +63,14 @@ class EulerAdjointProblem:
     """Initial-state gradients of a pressure misfit through nonlinear Euler
     time stepping on the GPU (FP64)."""
 
-    def __init__(self, mesh, p, receivers, dt, nsteps, nsnap=0, gamma=1.4, stream=None):
+    def __init__(self, mesh, p, receivers, dt, nsteps, nsnap=0, gamma=1.4, stream=None, flux=dgx.DGX_FLUX_LF):
         self.lib = _bind(dgx.load())
         self.ne, self.nsteps, self.p = int(mesh.ne), int(nsteps), int(p)
         self._vert = np.ascontiguousarray(mesh.vert, dtype=np.float64)
         self._con = np.ascontiguousarray(mesh.con, dtype=np.int32)
         self._rec = np.ascontiguousarray(np.asarray(receivers, dtype=np.float64).reshape(-1, 2))
         self.nrec = len(self._rec)
-        md = dgx.DomainDesc(2, self.ne, self.p, dgx.DGX_EULER, dgx.DGX_FLUX_LF, dgx.DGX_FP64, float(gamma), 0.0,
+        md = dgx.DomainDesc(2, self.ne, self.p, dgx.DGX_EULER, int(flux), dgx.DGX_FP64, float(gamma), 0.0,
                             0.72, _dp(self._vert), len(self._con),
                             self._con.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 0, 0.0)
         ad = EulerAdjointDesc(self.nrec, _dp(self._rec), float(dt), self.nsteps, int(nsnap))

@@ -446,9 +446,12 @@ typedef struct {
   int nsnap;                 /* checkpoint slots, state 0 included (0: ceil(sqrt(nsteps)) + 1) */
 } dgx_euler_adjoint_desc;
 
-/* `mesh`: nsd 2, eqntype DGX_EULER, vis 0, DGX_FLUX_LAX_FRIEDRICHS,
- * DGX_FP64, periodic parallelogram quads (no boundary rows, xnodes, functions
- * or porder).  Allocates nsnap + 20 states of 4 x ne x (p+1)^2 doubles. */
+/* `mesh`: nsd 2, eqntype DGX_EULER, vis 0, DGX_FLUX_LAX_FRIEDRICHS or
+ * DGX_FLUX_ROE (the backward kernels apply the chosen flux's transposed
+ * Jacobian; the Roe flux's is assembled from forward-mode derivatives of the
+ * same formula), DGX_FP64, periodic parallelogram quads (no boundary rows,
+ * xnodes, functions or porder).  Allocates nsnap + 20 states of
+ * 4 x ne x (p+1)^2 doubles. */
 int dgx_euler_adjoint_create(const dgx_domain_desc *mesh, const dgx_euler_adjoint_desc *ad, void *stream,
                              dgx_euler_adjoint_handle *out);
 int dgx_euler_adjoint_destroy(dgx_euler_adjoint_handle a);

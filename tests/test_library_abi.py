@@ -219,7 +219,7 @@ def test_acoustic_create_rejects_bad_arguments(lib, case, msg):
     (dict(eqn=30), b"DGX_EULER"),
     (dict(vis=1e-3), b"vis = 0"),
     (dict(precision=1), b"DGX_FP64"),
-    (dict(flux=1), b"Lax-Friedrichs"),
+    (dict(flux=7), b"Lax-Friedrichs and Roe"),
     (dict(walls=True), b"periodic"),
     (dict(dt=0.0), b"dt > 0"),
     (dict(nsteps=0), b"nsteps"),
