@@ -330,17 +330,34 @@ def euler_adjoint_sample(args, seed=77):
         fd = (prob.objective(U0 + h * d, data) - prob.objective(U0 - h * d, data)) / (2 * h)
         errs.append(abs(fd - float(np.sum(g * d))) / max(abs(fd), 1e-300))
     prob.close()
+    # the same gradient with the Roe flux (its transposed Jacobian from dual-number derivatives)
+    dirs = [modal(smooth((1.0, 0.4, 0.25, 2.5))) for _ in range(args.adj_dirs // 4)]
+    roe = euler_adjoint.EulerAdjointProblem(m, p, rec, dt, nsteps, nsnap=nsnap, flux=dgx.DGX_FLUX_ROE)
+    data_r, _ = roe.forward(U0 + 0.02 * modal(smooth((1.0, 0.4, 0.25, 2.5))))
+    t0w = time.perf_counter()
+    roe.forward(U0)
+    t_fwd_r = time.perf_counter() - t0w
+    t0w = time.perf_counter()
+    _, g_r = roe.gradient(U0, data_r)
+    t_grad_r = time.perf_counter() - t0w
+    errs_r = []
+    for d in dirs:
+        fd = (roe.objective(U0 + h * d, data_r) - roe.objective(U0 - h * d, data_r)) / (2 * h)
+        errs_r.append(abs(fd - float(np.sum(g_r * d))) / max(abs(fd), 1e-300))
+    roe.close()
     dom.close()
     dofs = 4 * m.ne * (p + 1) ** 2
     fwd_per_grad = st["forward_steps"] - nsteps * (1 + 2 * len(errs))
     return {"workload": f"nonlinear 2D Euler, {n}x{n} periodic quads, p={p}, initial-state gradient "
                         f"({dofs} unknowns), 16 pressure receivers, {nsteps} LSRK14 steps at CFL 0.5, "
-                        f"{nsnap - 1} checkpoints + the initial state, FP64",
+                        f"{nsnap - 1} checkpoints + the initial state, FP64, Lax-Friedrichs flux",
             "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd, "J": J,
             "forward_steps_per_gradient": fwd_per_grad, "adjoint_steps_per_gradient": nsteps,
             "forward_dof_stage_per_s": dofs * 14 * nsteps / t_fwd,
             "fd_directions": len(errs), "fd_rel_err_max": float(max(errs)) if errs else None,
-            "fd_rel_err": [float(e) for e in errs]}
+            "fd_rel_err": [float(e) for e in errs],
+            "roe": {"forward_s": t_fwd_r, "gradient_s": t_grad_r, "gradient_over_forward": t_grad_r / t_fwd_r,
+                    "fd_rel_err_max": float(max(errs_r)) if errs_r else None}}
 
 
 def curved_sample(args, seed=5):
